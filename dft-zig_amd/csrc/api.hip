@@ -1,0 +1,314 @@
+// C ABI of libdzhip (declared in include/dzhip.h).
+#include <cstring>
+#include <exception>
+#include <new>
+
+#include "ham.h"
+
+namespace dzhip {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+template <typename F>
+static int guarded(F&& f) {
+    try {
+        f();
+        return DZHIP_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return DZHIP_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return DZHIP_ERR_HIP;
+    }
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) DZ_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace dzhip
+
+using namespace dzhip;
+
+struct dzhip_ctx : Context {};
+struct dzhip_fft_plan : FftPlan {};
+struct dzhip_ham : Ham {};
+
+extern "C" {
+
+int dzhip_version(void) { return 100; }
+
+const char* dzhip_last_error(void) { return g_last_error.c_str(); }
+
+dzhip_ctx* dzhip_ctx_create(int device) {
+    dzhip_ctx* out = nullptr;
+    int rc = guarded([&] {
+        int n = 0;
+        DZ_HIP(hipGetDeviceCount(&n));
+        DZ_REQUIRE(device >= 0 && device < n, DZHIP_ERR_ARG,
+                   "device " + std::to_string(device) + " not present (" + std::to_string(n) + " visible)");
+        DeviceGuard g(device);
+        auto* c = new dzhip_ctx();
+        c->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            throw Error{DZHIP_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
+        }
+        out = c;
+    });
+    return rc == DZHIP_OK ? out : nullptr;
+}
+
+void dzhip_ctx_destroy(dzhip_ctx* ctx) {
+    if (!ctx) return;
+    (void)guarded([&] {
+        DeviceGuard g(ctx->device);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    });
+    delete ctx;
+}
+
+// ----------------------------------------------------------------- FFT
+dzhip_fft_plan* dzhip_fft_plan_create(dzhip_ctx* ctx, uint64_t nx, uint64_t ny, uint64_t nz,
+                                      int precision) {
+    dzhip_fft_plan* out = nullptr;
+    int rc = guarded([&] {
+        DZ_REQUIRE(ctx, DZHIP_ERR_STATE, "NULL context");
+        DZ_REQUIRE(precision == DZHIP_FP64 || precision == DZHIP_FP32, DZHIP_ERR_ARG, "bad precision");
+        DZ_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nx <= 256 && ny <= 256 && nz <= 256, DZHIP_ERR_ARG,
+                   "FFT dimensions must be in 1..256");
+        const size_t esz = precision == DZHIP_FP64 ? sizeof(double2) : sizeof(float2);
+        DZ_REQUIRE(nx * ny * esz <= 160 * 1024 && nx * nz * esz <= 160 * 1024, DZHIP_ERR_ARG,
+                   "FFT plane exceeds one LDS tile");
+        DeviceGuard g(ctx->device);
+        auto* p = new dzhip_fft_plan();
+        p->ctx = ctx;
+        p->nx = (int)nx;
+        p->ny = (int)ny;
+        p->nz = (int)nz;
+        p->precision = precision;
+        try {
+            p->tab.build(p->nx, p->ny, p->nz);
+        } catch (...) {
+            p->tab.release();
+            delete p;
+            throw;
+        }
+        out = p;
+    });
+    return rc == DZHIP_OK ? out : nullptr;
+}
+
+void dzhip_fft_plan_destroy(dzhip_fft_plan* plan) {
+    if (!plan) return;
+    (void)guarded([&] {
+        DeviceGuard g(plan->ctx->device);
+        plan->tab.release();
+        plan->buf64.release();
+        plan->buf32.release();
+    });
+    delete plan;
+}
+
+static int fft_exec(dzhip_fft_plan* plan, double* data, uint64_t count, bool inverse) {
+    return guarded([&] {
+        DZ_REQUIRE(plan, DZHIP_ERR_STATE, "NULL plan");
+        const size_t n = (size_t)plan->nx * plan->ny * plan->nz;
+        DZ_REQUIRE(count == n, DZHIP_ERR_SIZE, "count != nx*ny*nz");
+        DZ_REQUIRE(data, DZHIP_ERR_ARG, "NULL data");
+        std::lock_guard<std::mutex> lk(plan->mu);
+        DeviceGuard g(plan->ctx->device);
+        hipStream_t s = plan->ctx->stream;
+        plan->buf64.ensure(n * sizeof(double2));
+        DZ_HIP(hipMemcpyAsync(plan->buf64.ptr, data, n * sizeof(double2), hipMemcpyHostToDevice, s));
+        if (plan->precision == DZHIP_FP64) {
+            fft3d_device<double>(plan->buf64.as<double2>(), plan->tab, inverse, s);
+        } else {
+            plan->buf32.ensure(n * sizeof(float2));
+            convert_to_float(plan->buf64.as<double2>(), plan->buf32.as<float2>(), n, s);
+            fft3d_device<float>(plan->buf32.as<float2>(), plan->tab, inverse, s);
+            convert_to_double(plan->buf32.as<float2>(), plan->buf64.as<double2>(), n, s);
+        }
+        DZ_HIP(hipMemcpyAsync(data, plan->buf64.ptr, n * sizeof(double2), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int dzhip_fft_forward(dzhip_fft_plan* plan, double* data, uint64_t count) {
+    return fft_exec(plan, data, count, false);
+}
+int dzhip_fft_inverse(dzhip_fft_plan* plan, double* data, uint64_t count) {
+    return fft_exec(plan, data, count, true);
+}
+
+// ----------------------------------------------------------------- Hamiltonian
+dzhip_ham* dzhip_ham_create(dzhip_ctx* ctx, const dzhip_ham_desc* desc) {
+    dzhip_ham* out = nullptr;
+    int rc = guarded([&] {
+        DZ_REQUIRE(ctx, DZHIP_ERR_STATE, "NULL context");
+        DZ_REQUIRE(desc, DZHIP_ERR_ARG, "NULL descriptor");
+        DeviceGuard g(ctx->device);
+        auto* h = new dzhip_ham();
+        h->ctx = ctx;
+        try {
+            ham_build(*h, *desc);
+        } catch (...) {
+            ham_release(*h);
+            delete h;
+            throw;
+        }
+        out = h;
+    });
+    return rc == DZHIP_OK ? out : nullptr;
+}
+
+void dzhip_ham_destroy(dzhip_ham* ham) {
+    if (!ham) return;
+    (void)guarded([&] {
+        DeviceGuard g(ham->ctx->device);
+        (void)hipStreamSynchronize(ham->stream());
+        ham_release(*ham);
+    });
+    delete ham;
+}
+
+int dzhip_ham_set_local(dzhip_ham* ham, const double* local_r, uint64_t count) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        DZ_REQUIRE(local_r, DZHIP_ERR_ARG, "NULL local_r");
+        DZ_REQUIRE(count == ham->ngrid, DZHIP_ERR_SIZE, "count != nx*ny*nz");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        DZ_HIP(hipStreamSynchronize(ham->stream()));
+        ham_set_local(*ham, local_r);
+    });
+}
+
+static int ham_apply_host(dzhip_ham* ham, const double* x, double* y, uint64_t n, uint64_t ncols,
+                          int precision, bool local) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        DZ_REQUIRE(n == ham->npw, DZHIP_ERR_SIZE, "n != n_pw (InvalidMatrixSize)");
+        DZ_REQUIRE(precision == DZHIP_FP64 || precision == DZHIP_FP32, DZHIP_ERR_ARG, "bad precision");
+        if (ncols == 0) return;
+        DZ_REQUIRE(x && y, DZHIP_ERR_ARG, "NULL x or y");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        hipStream_t s = ham->stream();
+        const size_t cnt = n * ncols;
+        ham->xstage.ensure(cnt * sizeof(double2));
+        ham->ystage.ensure(cnt * sizeof(double2));
+        DZ_HIP(hipMemcpyAsync(ham->xstage.ptr, x, cnt * sizeof(double2), hipMemcpyHostToDevice, s));
+        if (precision == DZHIP_FP64) {
+            ham_apply_dev<double>(*ham, ham->xstage.ptr, ham->ystage.ptr, ncols, local, true);
+        } else {
+            ham->x32.ensure(cnt * sizeof(float2));
+            ham->y32.ensure(cnt * sizeof(float2));
+            convert_to_float(ham->xstage.as<double2>(), ham->x32.as<float2>(), cnt, s);
+            ham_apply_dev<float>(*ham, ham->x32.ptr, ham->y32.ptr, ncols, local, true);
+            convert_to_double(ham->y32.as<float2>(), ham->ystage.as<double2>(), cnt, s);
+        }
+        DZ_HIP(hipMemcpyAsync(y, ham->ystage.ptr, cnt * sizeof(double2), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int dzhip_ham_apply(dzhip_ham* ham, const double* x, double* y, uint64_t n, uint64_t ncols,
+                    int precision) {
+    return ham_apply_host(ham, x, y, n, ncols, precision, true);
+}
+
+int dzhip_ham_apply_nonlocal(dzhip_ham* ham, const double* x, double* y, uint64_t n, uint64_t ncols,
+                             int precision) {
+    return ham_apply_host(ham, x, y, n, ncols, precision, false);
+}
+
+int dzhip_ham_apply_dev(dzhip_ham* ham, const void* x_dev, void* y_dev, uint64_t n, uint64_t ncols,
+                        int precision) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        DZ_REQUIRE(n == ham->npw, DZHIP_ERR_SIZE, "n != n_pw (InvalidMatrixSize)");
+        if (ncols == 0) return;
+        DZ_REQUIRE(x_dev && y_dev, DZHIP_ERR_ARG, "NULL x or y");
+        DZ_REQUIRE(x_dev != y_dev, DZHIP_ERR_ARG, "x and y must not alias");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        if (precision == DZHIP_FP64)
+            ham_apply_dev<double>(*ham, x_dev, y_dev, ncols, true, true);
+        else if (precision == DZHIP_FP32)
+            ham_apply_dev<float>(*ham, x_dev, y_dev, ncols, true, true);
+        else
+            throw Error{DZHIP_ERR_ARG, "bad precision"};
+    });
+}
+
+int dzhip_ham_apply_overlap(dzhip_ham* ham, const double* x, double* y, uint64_t n, uint64_t ncols) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        DZ_REQUIRE(n == ham->npw, DZHIP_ERR_SIZE, "n != n_pw (InvalidMatrixSize)");
+        if (ncols == 0) return;
+        DZ_REQUIRE(x && y, DZHIP_ERR_ARG, "NULL x or y");
+        if (x != y) std::memmove(y, x, n * ncols * 2 * sizeof(double));
+    });
+}
+
+int dzhip_ham_set_stream(dzhip_ham* ham, void* stream) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        ham->user_stream = reinterpret_cast<hipStream_t>(stream);
+    });
+}
+
+int dzhip_ham_sync(dzhip_ham* ham) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        DeviceGuard g(ham->ctx->device);
+        DZ_HIP(hipStreamSynchronize(ham->stream()));
+    });
+}
+
+int dzhip_ham_profile(dzhip_ham* ham, int enable) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        ham->profile = enable != 0;
+        for (double& v : ham->phase_ms) v = 0.0;
+        ham->calls = 0;
+    });
+}
+
+int dzhip_ham_phase_times(dzhip_ham* ham, double* out_ms, uint64_t n_out, uint64_t* calls) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        for (uint64_t i = 0; i < n_out && i < DZHIP_PHASE_COUNT; ++i) out_ms[i] = ham->phase_ms[i];
+        if (calls) *calls = ham->calls;
+    });
+}
+
+int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        const uint64_t v[5] = {ham->npw, ham->ngrid, (uint64_t)ham->nplanes, (uint64_t)ham->nproj,
+                               ham->device_bytes()};
+        for (uint64_t i = 0; i < n_out && i < 5; ++i) out[i] = v[i];
+    });
+}
+
+}  // extern "C"

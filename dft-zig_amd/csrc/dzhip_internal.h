@@ -1,0 +1,128 @@
+// Internal definitions shared by the HIP translation units of libdzhip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dzhip.h"
+
+namespace dzhip {
+
+// ----------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define DZ_HIP(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            throw ::dzhip::Error{DZHIP_ERR_HIP, std::string(#call) + ": " +          \
+                                                    hipGetErrorString(e_)};          \
+    } while (0)
+
+#define DZ_REQUIRE(cond, code, msg)                                                  \
+    do {                                                                             \
+        if (!(cond)) throw ::dzhip::Error{(code), (msg)};                            \
+    } while (0)
+
+// ----------------------------------------------------------------- complex
+template <typename T> struct C2T;
+template <> struct C2T<float> { using type = float2; };
+template <> struct C2T<double> { using type = double2; };
+
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename C>
+__device__ __forceinline__ C cmulc(C a, C b) {
+    return C{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename C, typename T>
+__device__ __forceinline__ C cscale(C a, T s) { return C{a.x * s, a.y * s}; }
+// i * s * a  (s = +-1 folded by caller)
+template <typename C>
+__device__ __forceinline__ C cmul_i(C a) { return C{-a.y, a.x}; }
+
+// ----------------------------------------------------------------- FFT axis
+// One transform length n = prod radix[s] (radices 2/3/4/5), with device tables:
+//   twiddle[t] = exp(-2 pi i t / n), t in [0, n)   (inverse uses the conjugate)
+//   perm[i]    = LDS position of input index i for the in-place DIT, which is also
+//                the position where the in-place DIF leaves output frequency i.
+constexpr int kMaxStages = 12;
+constexpr int kMaxFftLen = 256;
+
+struct FftAxis {
+    int n;
+    int nstages;
+    int radix[kMaxStages];
+    int tw_off;   // element offset of this axis' twiddles in the table
+    int perm_off; // element offset of this axis' permutation in the table
+};
+
+// Host-side factorisation and tables (fft_core.cpp-like helpers in fft.hip).
+bool factor_fft_len(int n, std::vector<int>& radices);
+std::vector<int> digit_reverse_perm(int n, const std::vector<int>& radices);
+
+// Device-resident set of axis tables for a grid (up to 3 distinct lengths).
+struct AxisTables {
+    FftAxis ax[3];
+    double2* tw64 = nullptr;
+    float2* tw32 = nullptr;
+    int* perm = nullptr;
+    size_t bytes = 0;
+    void build(int nx, int ny, int nz);
+    void release();
+};
+
+// ----------------------------------------------------------------- context
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+};
+
+inline size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Device buffer that only grows.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t need) {
+        if (need <= bytes) return;
+        if (ptr) DZ_HIP(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&ptr, need);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            throw Error{DZHIP_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(need) +
+                                              "): " + hipGetErrorString(e)};
+        }
+        bytes = need;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T* as() const { return reinterpret_cast<T*>(ptr); }
+};
+
+void set_max_dynamic_lds(const void* func, size_t bytes);
+
+}  // namespace dzhip

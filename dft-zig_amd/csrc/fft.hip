@@ -1,0 +1,197 @@
+// Standalone batched 3D FFT (the `hip` FFT backend) and the shared axis tables.
+//
+// Replaces Fft3dPlan.forward/inverse for the fftw backend
+// (src/features/fft/fft.zig:224-262, src/lib/fft/fftw_fft.zig:141-166):
+// in place, forward unnormalised, inverse scaled by 1/N.
+//
+// Two passes over a grid stored x + nx*(y + ny*z):
+//   pass 1: one workgroup per z-plane (contiguous nx*ny block): x- and y-lines in LDS;
+//   pass 2: one workgroup per y (nz rows of nx contiguous values): z-lines in LDS.
+#include <cmath>
+#include <cstring>
+
+#include "fft_lds.h"
+
+namespace dzhip {
+
+bool factor_fft_len(int n, std::vector<int>& radices) {
+    radices.clear();
+    if (n < 1 || n > kMaxFftLen) return false;
+    if (n == 1) return true;
+    int m = n;
+    // Prefer radix 4, then 2, 3, 5 (fewer LDS round trips than radix 2).
+    while (m % 4 == 0 && m > 4) { radices.push_back(4); m /= 4; }
+    for (int r : {4, 2, 3, 5}) {
+        while (m % r == 0) { radices.push_back(r); m /= r; }
+    }
+    if (m != 1) return false;
+    return (int)radices.size() <= kMaxStages;
+}
+
+// Position of input index i for the in-place DIT whose stage s uses radices[s]:
+// the last stage splits the input by i mod R_S into blocks of n/R_S, recursively.
+static int dr_pos(int i, int n, const int* rad, int nrad) {
+    if (nrad == 0) return 0;
+    const int R = rad[nrad - 1];
+    const int m = n / R;
+    return (i % R) * m + dr_pos(i / R, m, rad, nrad - 1);
+}
+
+std::vector<int> digit_reverse_perm(int n, const std::vector<int>& radices) {
+    std::vector<int> p(n);
+    for (int i = 0; i < n; ++i) p[i] = dr_pos(i, n, radices.data(), (int)radices.size());
+    return p;
+}
+
+void AxisTables::build(int nx, int ny, int nz) {
+    const int lens[3] = {nx, ny, nz};
+    size_t tw_total = 0, perm_total = 0;
+    std::vector<std::vector<int>> rads(3);
+    for (int d = 0; d < 3; ++d) {
+        DZ_REQUIRE(factor_fft_len(lens[d], rads[d]), DZHIP_ERR_ARG,
+                   "FFT length " + std::to_string(lens[d]) + " is not 2/3/5-smooth or exceeds 256");
+        ax[d].n = lens[d];
+        ax[d].nstages = (int)rads[d].size();
+        for (int s = 0; s < kMaxStages; ++s) ax[d].radix[s] = s < ax[d].nstages ? rads[d][s] : 1;
+        ax[d].tw_off = (int)tw_total;
+        ax[d].perm_off = (int)perm_total;
+        tw_total += lens[d];
+        perm_total += lens[d];
+    }
+    std::vector<double2> h64(tw_total);
+    std::vector<float2> h32(tw_total);
+    std::vector<int> hp(perm_total);
+    for (int d = 0; d < 3; ++d) {
+        const int n = lens[d];
+        for (int t = 0; t < n; ++t) {
+            // exp(-2 pi i t/n), evaluated with the angle reduced to [0, pi/4] symmetry-free
+            // direct form in long double for an accurately rounded table.
+            long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / n;
+            double c = (double)cosl(ang), s = (double)sinl(ang);
+            h64[ax[d].tw_off + t] = make_double2(c, s);
+            h32[ax[d].tw_off + t] = make_float2((float)c, (float)s);
+        }
+        std::vector<int> p = digit_reverse_perm(n, rads[d]);
+        std::memcpy(&hp[ax[d].perm_off], p.data(), n * sizeof(int));
+    }
+    DZ_HIP(hipMalloc(&tw64, tw_total * sizeof(double2)));
+    DZ_HIP(hipMalloc(&tw32, tw_total * sizeof(float2)));
+    DZ_HIP(hipMalloc(&perm, perm_total * sizeof(int)));
+    DZ_HIP(hipMemcpy(tw64, h64.data(), tw_total * sizeof(double2), hipMemcpyHostToDevice));
+    DZ_HIP(hipMemcpy(tw32, h32.data(), tw_total * sizeof(float2), hipMemcpyHostToDevice));
+    DZ_HIP(hipMemcpy(perm, hp.data(), perm_total * sizeof(int), hipMemcpyHostToDevice));
+    bytes = tw_total * (sizeof(double2) + sizeof(float2)) + perm_total * sizeof(int);
+}
+
+void AxisTables::release() {
+    if (tw64) (void)hipFree(tw64);
+    if (tw32) (void)hipFree(tw32);
+    if (perm) (void)hipFree(perm);
+    tw64 = nullptr;
+    tw32 = nullptr;
+    perm = nullptr;
+}
+
+void set_max_dynamic_lds(const void* func, size_t bytes) {
+    if (bytes <= 65536) return;
+    DZ_HIP(hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+// ----------------------------------------------------------------- kernels
+constexpr int kFftThreads = 256;
+
+// Pass 1: z-plane = contiguous [y][x] tile; transform along x, then y.
+template <typename T>
+__global__ __launch_bounds__(kFftThreads) void k_fft_xy(typename C2T<T>::type* __restrict__ data,
+                                                        FftAxis axx, FftAxis axy,
+                                                        const typename C2T<T>::type* __restrict__ tw,
+                                                        const int* __restrict__ perm, T sign) {
+    using C = typename C2T<T>::type;
+    extern __shared__ unsigned char smem_raw[];
+    C* buf = reinterpret_cast<C*>(smem_raw);
+    const int nx = axx.n, ny = axy.n;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    C* plane = data + (size_t)blockIdx.x * nx * ny;
+    const int* px = perm + axx.perm_off;
+    const int* py = perm + axy.perm_off;
+    for (int i = tid; i < nx * ny; i += nthr) {
+        const int y = i / nx, x = i - y * nx;
+        buf[px[x] + nx * py[y]] = plane[i];
+    }
+    __syncthreads();
+    lds_fft_lines<T, C, true>(buf, axx, tw + axx.tw_off, ny, 1, nx, sign, tid, nthr);
+    lds_fft_lines<T, C, true>(buf, axy, tw + axy.tw_off, nx, nx, 1, sign, tid, nthr);
+    for (int i = tid; i < nx * ny; i += nthr) plane[i] = buf[i];
+}
+
+// Pass 2: fixed y, tile [z][x]; transform along z; optional scale.
+template <typename T>
+__global__ __launch_bounds__(kFftThreads) void k_fft_z(typename C2T<T>::type* __restrict__ data,
+                                                       int nx, int ny, FftAxis axz,
+                                                       const typename C2T<T>::type* __restrict__ tw,
+                                                       const int* __restrict__ perm, T sign,
+                                                       T scale) {
+    using C = typename C2T<T>::type;
+    extern __shared__ unsigned char smem_raw[];
+    C* buf = reinterpret_cast<C*>(smem_raw);
+    const int nz = axz.n;
+    const int y = blockIdx.x;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int* pz = perm + axz.perm_off;
+    for (int i = tid; i < nx * nz; i += nthr) {
+        const int z = i / nx, x = i - z * nx;
+        buf[x + nx * pz[z]] = data[x + (size_t)nx * (y + (size_t)ny * z)];
+    }
+    __syncthreads();
+    lds_fft_lines<T, C, true>(buf, axz, tw + axz.tw_off, nx, nx, 1, sign, tid, nthr);
+    for (int i = tid; i < nx * nz; i += nthr) {
+        const int z = i / nx, x = i - z * nx;
+        data[x + (size_t)nx * (y + (size_t)ny * z)] = cscale(buf[i], scale);
+    }
+}
+
+template <typename T>
+__global__ void k_to_float(const double2* __restrict__ in, float2* __restrict__ out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+__global__ void k_to_double(const float2* __restrict__ in, double2* __restrict__ out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_double2((double)in[i].x, (double)in[i].y);
+}
+
+void convert_to_float(const double2* in, float2* out, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_to_float<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n);
+    DZ_HIP(hipGetLastError());
+}
+void convert_to_double(const float2* in, double2* out, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_to_double, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n);
+    DZ_HIP(hipGetLastError());
+}
+
+template <typename T>
+void fft3d_device(typename C2T<T>::type* data, const AxisTables& tab, bool inverse,
+                  hipStream_t s) {
+    using C = typename C2T<T>::type;
+    const int nx = tab.ax[0].n, ny = tab.ax[1].n, nz = tab.ax[2].n;
+    const C* tw = TwSel<T>::get(tab.tw64, tab.tw32);
+    const T sign = inverse ? T(1) : T(-1);
+    const T scale = inverse ? T(1.0 / ((double)nx * ny * nz)) : T(1);
+    size_t lds1 = (size_t)nx * ny * sizeof(C);
+    size_t lds2 = (size_t)nx * nz * sizeof(C);
+    set_max_dynamic_lds((const void*)k_fft_xy<T>, lds1);
+    set_max_dynamic_lds((const void*)k_fft_z<T>, lds2);
+    hipLaunchKernelGGL(k_fft_xy<T>, dim3(nz), dim3(kFftThreads), lds1, s, data, tab.ax[0],
+                       tab.ax[1], tw, tab.perm, sign);
+    DZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fft_z<T>, dim3(ny), dim3(kFftThreads), lds2, s, data, nx, ny, tab.ax[2],
+                       tw, tab.perm, sign, scale);
+    DZ_HIP(hipGetLastError());
+}
+
+template void fft3d_device<double>(double2*, const AxisTables&, bool, hipStream_t);
+template void fft3d_device<float>(float2*, const AxisTables&, bool, hipStream_t);
+
+}  // namespace dzhip

@@ -1,0 +1,75 @@
+// Per-k Hamiltonian context held on the device (host-side struct).
+#pragma once
+
+#include "dzhip_internal.h"
+
+namespace dzhip {
+
+template <typename T>
+void fft3d_device(typename C2T<T>::type* data, const AxisTables& tab, bool inverse, hipStream_t s);
+void convert_to_float(const double2* in, float2* out, size_t n, hipStream_t s);
+void convert_to_double(const float2* in, double2* out, size_t n, hipStream_t s);
+
+struct FftPlan {
+    Context* ctx = nullptr;
+    int nx = 0, ny = 0, nz = 0;
+    int precision = DZHIP_FP64;
+    AxisTables tab;
+    DevBuf buf64, buf32;
+    std::mutex mu;
+};
+
+// Projector metadata: row p of the complex projector matrix belongs to the atom
+// block starting at row `base`, is its m-row `m`, the block has `mt` rows and
+// the species' dense m-resolved D starts at `dm_off` in the D table.
+struct ProjMeta {
+    int base, m, mt, dm_off;
+};
+
+struct Ham {
+    Context* ctx = nullptr;
+    hipStream_t user_stream = nullptr;
+    hipStream_t stream() const { return user_stream ? user_stream : ctx->stream; }
+
+    int nx = 0, ny = 0, nz = 0;
+    size_t npw = 0, ngrid = 0;
+    double inv_volume = 1.0;
+    AxisTables tab;
+
+    // G-sphere split into x-planes (one plane per distinct FFT x of a plane wave):
+    //   plane_start[p]..plane_start[p+1]: entries of plane p
+    //   plane_pw[j]  : plane-wave index of entry j
+    //   plane_pos[j] : LDS position py[y] + ny*pz[z] of that plane wave in the yz tile
+    //   plane_xpos[p]: LDS row px[x] of plane p in the x-line tile
+    int nplanes = 0;
+    DevBuf plane_start, plane_pw, plane_pos, plane_xpos;
+
+    DevBuf kinetic64, kinetic32;
+    DevBuf vloc64, vloc32;  // V_loc(r)/N in Ry
+    bool has_local = false;
+
+    // Nonlocal: complex projector matrix A[p][G] = phi(G) e^{iG.tau_a} (P x n_pw), D tables.
+    int nproj = 0;
+    DevBuf a64, a32, meta, dm64, dm32;
+
+    // Work buffers.
+    DevBuf xstage, ystage, x32, y32, tbuf, partial, qbuf;
+
+    // Profiling.
+    bool profile = false;
+    hipEvent_t ev[DZHIP_PHASE_COUNT + 1] = {};
+    double phase_ms[DZHIP_PHASE_COUNT] = {};
+    uint64_t calls = 0;
+
+    size_t device_bytes() const;
+    std::mutex mu;
+};
+
+void ham_build(Ham& h, const dzhip_ham_desc& d);
+void ham_set_local(Ham& h, const double* local_r);
+void ham_release(Ham& h);
+// Device apply; x/y hold C2T<T> elements, column-major with leading dimension npw.
+template <typename T>
+void ham_apply_dev(Ham& h, const void* x, void* y, size_t ncols, bool local, bool nonlocal);
+
+}  // namespace dzhip

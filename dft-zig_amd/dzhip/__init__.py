@@ -1,0 +1,341 @@
+"""Python binding of libdzhip (ctypes over the C ABI in include/dzhip.h).
+
+Mirrors the reference's operator interface for the H.psi path:
+  FftPlan3d.forward/inverse      <- Fft3dPlan (src/features/fft/fft.zig:44-263)
+  Hamiltonian.apply / apply_batch <- Operator.apply / apply_batch (src/features/linalg/lobpcg/common.zig:12-41)
+                                    backed by apply_hamiltonian(_batched) (src/features/scf/apply.zig:643-658, 1584-1618)
+  Hamiltonian.apply_s            <- apply_overlap (apply.zig:1041-1072), S = I for norm-conserving species
+Errors: InvalidMatrixSize (reference error.InvalidMatrixSize) and HipFailure.
+
+There is no CPU fallback: the library is loaded on first use and a missing
+build raises ImportError; every compute call goes to the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libdzhip.so"
+
+FP64 = 0
+FP32 = 1
+
+PHASES = ("nl_project", "plane_inv", "xline", "plane_fwd", "nl_back")
+
+EXPORTS = (
+    "dzhip_version", "dzhip_last_error", "dzhip_ctx_create", "dzhip_ctx_destroy",
+    "dzhip_fft_plan_create", "dzhip_fft_plan_destroy", "dzhip_fft_forward", "dzhip_fft_inverse",
+    "dzhip_ham_create", "dzhip_ham_destroy", "dzhip_ham_set_local", "dzhip_ham_apply",
+    "dzhip_ham_apply_dev", "dzhip_ham_apply_nonlocal", "dzhip_ham_apply_overlap",
+    "dzhip_ham_set_stream", "dzhip_ham_sync", "dzhip_ham_profile", "dzhip_ham_phase_times",
+    "dzhip_ham_info",
+)
+
+
+class DzhipError(RuntimeError):
+    code = -2
+
+
+class InvalidMatrixSize(DzhipError):
+    code = -1
+
+
+class HipFailure(DzhipError):
+    code = -2
+
+
+class OutOfMemory(DzhipError):
+    code = -3
+
+
+class BadState(DzhipError):
+    code = -4
+
+
+class BadArgument(DzhipError):
+    code = -5
+
+
+_ERRS = {-1: InvalidMatrixSize, -2: HipFailure, -3: OutOfMemory, -4: BadState, -5: BadArgument}
+
+
+class SpeciesDesc(ct.Structure):
+    _fields_ = [
+        ("n_beta", ct.c_uint64), ("l_list", ct.POINTER(ct.c_int32)), ("dij", ct.POINTER(ct.c_double)),
+        ("dij_m", ct.POINTER(ct.c_double)), ("m_total", ct.c_uint64), ("phi", ct.POINTER(ct.c_double)),
+    ]
+
+
+class HamDesc(ct.Structure):
+    _fields_ = [
+        ("nx", ct.c_uint64), ("ny", ct.c_uint64), ("nz", ct.c_uint64), ("n_pw", ct.c_uint64),
+        ("fft_index", ct.POINTER(ct.c_int32)), ("kinetic", ct.POINTER(ct.c_double)),
+        ("gcart", ct.POINTER(ct.c_double)), ("local_r", ct.POINTER(ct.c_double)),
+        ("inv_volume", ct.c_double), ("n_species", ct.c_uint64), ("species", ct.POINTER(SpeciesDesc)),
+        ("n_atoms", ct.c_uint64), ("atom_pos", ct.POINTER(ct.c_double)),
+        ("atom_species", ct.POINTER(ct.c_int32)),
+    ]
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(f"libdzhip.so not built at {LIB_PATH}; run dft-zig_amd/dzhip/build.py "
+                          "(there is no CPU fallback)")
+    # torch ships its own libamdhip64.so.7 / libhsa-runtime64.so.1.  Loading it first lets the
+    # dynamic loader bind libdzhip to that same runtime (same sonames); loading libdzhip first
+    # would put a second HIP runtime into the process and the later one finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = ct.CDLL(str(LIB_PATH))
+    vp, u64, i32, dp = ct.c_void_p, ct.c_uint64, ct.c_int, ct.POINTER(ct.c_double)
+    sig = {
+        "dzhip_version": ([], i32), "dzhip_last_error": ([], ct.c_char_p),
+        "dzhip_ctx_create": ([i32], vp), "dzhip_ctx_destroy": ([vp], None),
+        "dzhip_fft_plan_create": ([vp, u64, u64, u64, i32], vp), "dzhip_fft_plan_destroy": ([vp], None),
+        "dzhip_fft_forward": ([vp, dp, u64], i32), "dzhip_fft_inverse": ([vp, dp, u64], i32),
+        "dzhip_ham_create": ([vp, ct.POINTER(HamDesc)], vp), "dzhip_ham_destroy": ([vp], None),
+        "dzhip_ham_set_local": ([vp, dp, u64], i32),
+        "dzhip_ham_apply": ([vp, dp, dp, u64, u64, i32], i32),
+        "dzhip_ham_apply_dev": ([vp, vp, vp, u64, u64, i32], i32),
+        "dzhip_ham_apply_nonlocal": ([vp, dp, dp, u64, u64, i32], i32),
+        "dzhip_ham_apply_overlap": ([vp, dp, dp, u64, u64], i32),
+        "dzhip_ham_set_stream": ([vp, vp], i32), "dzhip_ham_sync": ([vp], i32),
+        "dzhip_ham_profile": ([vp, i32], i32),
+        "dzhip_ham_phase_times": ([vp, dp, u64, ct.POINTER(ct.c_uint64)], i32),
+        "dzhip_ham_info": ([vp, ct.POINTER(ct.c_uint64), u64], i32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+_LIB = None
+
+
+def L():
+    """The loaded library (loaded on first use; raises if it was never built)."""
+    global _LIB
+    if _LIB is None:
+        _LIB = _load()
+    return _LIB
+
+
+def last_error() -> str:
+    return L().dzhip_last_error().decode(errors="replace")
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise _ERRS.get(rc, DzhipError)(f"dzhip error {rc}: {last_error()}")
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ct.POINTER(ct.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(ct.POINTER(ct.c_int32))
+
+
+def _as_c128(x) -> np.ndarray:
+    return np.ascontiguousarray(x, dtype=np.complex128)
+
+
+class Context:
+    """One device context (metal_create_context counterpart)."""
+
+    def __init__(self, device: int = 0):
+        self.handle = L().dzhip_ctx_create(int(device))
+        if not self.handle:
+            raise HipFailure(f"dzhip_ctx_create({device}) failed: {last_error()}")
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            L().dzhip_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FftPlan3d:
+    """In-place 3D FFT on host arrays with the reference conventions (forward unnormalised,
+    inverse 1/N), index x + nx*(y + ny*z)."""
+
+    def __init__(self, ctx: Context, nx: int, ny: int, nz: int, precision: int = FP64):
+        self.ctx = ctx
+        self.shape = (nx, ny, nz)
+        self.handle = L().dzhip_fft_plan_create(ctx.handle, nx, ny, nz, precision)
+        if not self.handle:
+            raise BadArgument(f"dzhip_fft_plan_create failed: {last_error()}")
+
+    def forward(self, data: np.ndarray) -> None:
+        check(L().dzhip_fft_forward(self.handle, _dp(data.view(np.float64)), data.size))
+
+    def inverse(self, data: np.ndarray) -> None:
+        check(L().dzhip_fft_inverse(self.handle, _dp(data.view(np.float64)), data.size))
+
+    def close(self):
+        if self.handle:
+            L().dzhip_fft_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Hamiltonian:
+    """Per-k H.psi context on the device (ApplyContext counterpart).
+
+    Arguments mirror what the reference's SCF driver holds for one k-point:
+    grid (nx, ny, nz), fft_index (PwGridMap.fft_indices), kinetic (|k+G|^2, Ry),
+    gcart (G, bohr^-1), local_r (Ry, FFT order), inv_volume, species list of
+    dicts {l_list, dij, phi, [dij_m]}, atom positions (bohr) and species indices.
+    """
+
+    def __init__(self, ctx: Context, grid, fft_index, kinetic, gcart, local_r, inv_volume,
+                 species=(), atom_pos=None, atom_species=None):
+        self.ctx = ctx
+        self.grid = tuple(int(g) for g in grid)
+        self.n = int(np.asarray(kinetic).shape[0])
+        keep = []
+
+        def arr(a, dt):
+            v = np.ascontiguousarray(a, dtype=dt)
+            keep.append(v)
+            return v
+
+        fidx = arr(fft_index, np.int32)
+        kin = arr(kinetic, np.float64)
+        gc = arr(gcart, np.float64).reshape(-1)
+        sp_arr = (SpeciesDesc * max(1, len(species)))()
+        for i, sp in enumerate(species):
+            ls = arr(sp["l_list"], np.int32)
+            dij = arr(sp["dij"], np.float64).reshape(-1)
+            phi = arr(sp["phi"], np.float64).reshape(-1)
+            mt = int(sum(2 * int(l) + 1 for l in ls))
+            sp_arr[i].n_beta = ls.shape[0]
+            sp_arr[i].l_list = _ip(ls)
+            sp_arr[i].dij = _dp(dij)
+            if sp.get("dij_m") is not None:
+                dm = arr(sp["dij_m"], np.float64).reshape(-1)
+                sp_arr[i].dij_m = _dp(dm)
+            else:
+                sp_arr[i].dij_m = ct.POINTER(ct.c_double)()
+            sp_arr[i].m_total = mt
+            sp_arr[i].phi = _dp(phi)
+        d = HamDesc()
+        d.nx, d.ny, d.nz = self.grid
+        d.n_pw = self.n
+        d.fft_index = _ip(fidx)
+        d.kinetic = _dp(kin)
+        d.gcart = _dp(gc)
+        if local_r is not None:
+            lr = arr(local_r, np.float64)
+            d.local_r = _dp(lr)
+        else:
+            d.local_r = ct.POINTER(ct.c_double)()
+        d.inv_volume = float(inv_volume)
+        d.n_species = len(species)
+        d.species = sp_arr if species else ct.POINTER(SpeciesDesc)()
+        if species and atom_pos is not None and len(atom_pos) > 0:
+            ap = arr(atom_pos, np.float64).reshape(-1)
+            asp = arr(atom_species, np.int32)
+            d.n_atoms = asp.shape[0]
+            d.atom_pos = _dp(ap)
+            d.atom_species = _ip(asp)
+        else:
+            d.n_atoms = 0
+            d.atom_pos = ct.POINTER(ct.c_double)()
+            d.atom_species = ct.POINTER(ct.c_int32)()
+        self.handle = L().dzhip_ham_create(ctx.handle, ct.byref(d))
+        if not self.handle:
+            raise BadArgument(f"dzhip_ham_create failed: {last_error()}")
+
+    @classmethod
+    def from_workload(cls, ctx: Context, w, local_r=None):
+        from . import tables
+
+        species = [{"l_list": sp.l_list, "dij": sp.dij, "phi": w.phi(s)} for s, sp in enumerate(w.species)]
+        return cls(ctx, w.grid, tables.fft_indices(w.basis, w.grid), w.basis.kinetic, w.basis.cart,
+                   w.local_r if local_r is None else local_r, w.inv_volume, species, w.atom_pos,
+                   w.atom_species)
+
+    # -- reference callback shapes ------------------------------------------------
+    def set_local(self, local_r: np.ndarray) -> None:
+        lr = np.ascontiguousarray(local_r, dtype=np.float64)
+        check(L().dzhip_ham_set_local(self.handle, _dp(lr), lr.size))
+
+    def _host_call(self, fn, x, precision, *extra):
+        x = _as_c128(x)
+        if x.ndim == 1:
+            ncols, n = 1, x.shape[0]
+        else:
+            ncols, n = x.shape
+        y = np.empty_like(x)
+        check(fn(self.handle, _dp(x.view(np.float64)), _dp(y.view(np.float64)), n, ncols, *extra))
+        return y
+
+    def apply(self, x, precision: int = FP64) -> np.ndarray:
+        """y = H x for one vector (apply_hamiltonian) or (ncols, n) block (apply_hamiltonian_batched)."""
+        return self._host_call(L().dzhip_ham_apply, x, precision, precision)
+
+    apply_batch = apply
+
+    def apply_nonlocal(self, x, precision: int = FP64) -> np.ndarray:
+        return self._host_call(L().dzhip_ham_apply_nonlocal, x, precision, precision)
+
+    def apply_s(self, x) -> np.ndarray:
+        return self._host_call(L().dzhip_ham_apply_overlap, x, FP64)
+
+    def apply_dev(self, x_ptr: int, y_ptr: int, ncols: int, precision: int = FP64) -> None:
+        """Device pointers (e.g. torch tensors' data_ptr()), enqueued without waiting."""
+        check(L().dzhip_ham_apply_dev(self.handle, ct.c_void_p(x_ptr), ct.c_void_p(y_ptr), self.n,
+                                      ncols, precision))
+
+    def set_stream(self, stream_handle: int | None) -> None:
+        check(L().dzhip_ham_set_stream(self.handle, ct.c_void_p(stream_handle or 0)))
+
+    def sync(self) -> None:
+        check(L().dzhip_ham_sync(self.handle))
+
+    def profile(self, enable: bool) -> None:
+        check(L().dzhip_ham_profile(self.handle, 1 if enable else 0))
+
+    def phase_times(self):
+        out = (ct.c_double * len(PHASES))()
+        calls = ct.c_uint64()
+        check(L().dzhip_ham_phase_times(self.handle, out, len(PHASES), ct.byref(calls)))
+        return dict(zip(PHASES, list(out))), int(calls.value)
+
+    def info(self):
+        out = (ct.c_uint64 * 5)()
+        check(L().dzhip_ham_info(self.handle, out, 5))
+        keys = ("n_pw", "grid_points", "planes", "projectors", "device_bytes")
+        return dict(zip(keys, [int(v) for v in out]))
+
+    def close(self):
+        if self.handle:
+            L().dzhip_ham_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,163 @@
+/*
+ * dzhip.h — C ABI of the MI355X (gfx950) plane-wave H·psi / eigensolver library.
+ *
+ * This is the `hip` backend that sits beside dft-zig's fftw/zig/vdsp/metal FFT
+ * backends and behind the H·psi callback its SCF, band and DFPT solvers call.
+ * Every entry point takes plain pointers and sizes; complex arrays are
+ * interleaved (re, im) doubles, blocks of vectors are column-major
+ * [n x ncols] with column b at [b*n, (b+1)*n)
+ * (reference: src/features/linalg/lobpcg/common.zig:54-62; math/complex.zig:3-6).
+ *
+ * Status codes: 0 ok, <0 error; dzhip_last_error() returns a message for the
+ * calling thread.  Reference errors map as error.InvalidMatrixSize ->
+ * DZHIP_ERR_SIZE (src/features/scf/apply.zig:584, 1592).
+ *
+ * Units follow the reference: Rydberg energies, bohr lengths
+ * (docs/adr/0001-rydberg-unit-system.md).
+ */
+#ifndef DZHIP_H
+#define DZHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DZHIP_OK 0
+#define DZHIP_ERR_SIZE (-1)   /* length mismatch (reference error.InvalidMatrixSize) */
+#define DZHIP_ERR_HIP (-2)    /* HIP runtime / library failure */
+#define DZHIP_ERR_NOMEM (-3)  /* device or host allocation failed */
+#define DZHIP_ERR_STATE (-4)  /* NULL / uninitialised handle, local potential not set */
+#define DZHIP_ERR_ARG (-5)    /* unsupported argument (grid not 2/3/5-smooth, l > 4, ...) */
+
+#define DZHIP_FP64 0 /* FFTs, V_loc multiply and projector GEMMs in double */
+#define DZHIP_FP32 1 /* same in float, FP32 accumulation; inputs/outputs stay double at the host ABI */
+
+typedef struct dzhip_ctx dzhip_ctx;
+typedef struct dzhip_fft_plan dzhip_fft_plan;
+typedef struct dzhip_ham dzhip_ham;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int dzhip_version(void);
+
+/* Message describing the last failed call on this thread ("" if none). */
+const char* dzhip_last_error(void);
+
+/* ------------------------------------------------------------------ context
+ * One context per (process, device).  Replaces metal_create_context
+ * (src/lib/gpu/metal_bridge.h:16-30): returns NULL on failure, with the
+ * reason in dzhip_last_error(). */
+dzhip_ctx* dzhip_ctx_create(int device);
+void dzhip_ctx_destroy(dzhip_ctx* ctx);
+
+/* ------------------------------------------------------------------ 3D FFT
+ * Replaces Fft3dPlan / FftwPlan3d (src/features/fft/fft.zig:44-263,
+ * src/lib/fft/fftw_fft.zig:63-172) and the Metal bridge's plan calls
+ * (src/lib/gpu/metal_bridge.h:32-72).  In place on host memory, complex
+ * interleaved, index x + nx*(y + ny*z); forward = sum x e^{-2 pi i jk/n}
+ * unnormalised, inverse = (1/N) sum x e^{+2 pi i jk/n}
+ * (src/lib/fft/mixed_radix.zig:47-61, fftw_fft.zig:141-166).
+ * Each of nx, ny, nz must be 2/3/5-smooth and <= 256.  count must equal
+ * nx*ny*nz (the Metal bridge silently ignored a mismatch; here it is
+ * DZHIP_ERR_SIZE). */
+dzhip_fft_plan* dzhip_fft_plan_create(dzhip_ctx* ctx, uint64_t nx, uint64_t ny, uint64_t nz,
+                                      int precision);
+void dzhip_fft_plan_destroy(dzhip_fft_plan* plan);
+int dzhip_fft_forward(dzhip_fft_plan* plan, double* data, uint64_t count);
+int dzhip_fft_inverse(dzhip_fft_plan* plan, double* data, uint64_t count);
+
+/* ------------------------------------------------------------------ H·psi
+ * Per-k Hamiltonian context: the data ApplyContext + KpointApplyCache carry
+ * (src/features/scf/apply.zig:45-574, nonlocal_context.zig:9-246).
+ * All arrays are copied at create time; the caller keeps ownership. */
+typedef struct dzhip_species_desc {
+    uint64_t n_beta;       /* projectors (beta functions) of this species */
+    const int32_t* l_list; /* [n_beta] angular momentum, 0..4 */
+    const double* dij;     /* [n_beta*n_beta] row-major D_ij (Ry), UPF PP_DIJ; only same-l pairs act
+                              (apply.zig:940-966) */
+    const double* dij_m;   /* optional [m_total*m_total] dense m-resolved D (apply.zig:926-939); NULL = use dij */
+    uint64_t m_total;      /* sum_beta (2 l + 1) */
+    const double* phi;     /* [m_total*n_pw] phi[(off_beta+m)*n_pw + G] = 4 pi R_beta(|k+G|) Y_lm(k+G)
+                              (nonlocal_context.zig:524-549), m = -l..l */
+} dzhip_species_desc;
+
+typedef struct dzhip_ham_desc {
+    uint64_t nx, ny, nz;       /* FFT grid (pw_grid.zig:9-24) */
+    uint64_t n_pw;             /* plane waves at this k */
+    const int32_t* fft_index;  /* [n_pw] FFT-order position of each PW (PwGridMap.fft_indices,
+                                  pw_grid_map.zig:58-79) */
+    const double* kinetic;     /* [n_pw] |k+G|^2 in Ry (basis.zig:57) */
+    const double* gcart;       /* [n_pw*3] Cartesian G (not k+G) for the e^{iG.tau} phase (apply.zig:893) */
+    const double* local_r;     /* [nx*ny*nz] V_loc(r) in Ry, FFT order; may be NULL (set later) */
+    double inv_volume;         /* 1/Omega (bohr^-3) */
+    uint64_t n_species;
+    const dzhip_species_desc* species; /* [n_species]; may be NULL when n_species == 0 (no nonlocal) */
+    uint64_t n_atoms;
+    const double* atom_pos;      /* [n_atoms*3] Cartesian, bohr */
+    const int32_t* atom_species; /* [n_atoms] index into species */
+} dzhip_ham_desc;
+
+/* Build the per-k device context (index tables, projector matrix, FFT plans).
+ * Replaces ApplyContext.init_* (apply.zig:336-532).  NULL on failure. */
+dzhip_ham* dzhip_ham_create(dzhip_ctx* ctx, const dzhip_ham_desc* desc);
+void dzhip_ham_destroy(dzhip_ham* ham);
+
+/* Replace V_loc(r) (once per SCF iteration; build_local_potential_real,
+ * src/features/scf/potential.zig:320-334).  count must be nx*ny*nz. */
+int dzhip_ham_set_local(dzhip_ham* ham, const double* local_r, uint64_t count);
+
+/* y = H x for ncols vectors, host memory, blocking.  Replaces
+ * apply_hamiltonian (ncols == 1, apply.zig:643-658) and
+ * apply_hamiltonian_batched (apply.zig:1584-1618).  y is overwritten, never
+ * accumulated.  n must equal n_pw (else DZHIP_ERR_SIZE). */
+int dzhip_ham_apply(dzhip_ham* ham, const double* x, double* y, uint64_t n, uint64_t ncols,
+                    int precision);
+
+/* Device-resident variant used by the eigensolver and by callers that keep
+ * psi in HBM.  x_dev/y_dev hold double2 (DZHIP_FP64) or float2 (DZHIP_FP32)
+ * elements, column-major with leading dimension n.  Enqueued on the
+ * context's stream (see dzhip_ham_set_stream); returns without waiting. */
+int dzhip_ham_apply_dev(dzhip_ham* ham, const void* x_dev, void* y_dev, uint64_t n, uint64_t ncols,
+                        int precision);
+
+/* Nonlocal part only, y = V_nl x (apply_nonlocal_potential, apply.zig:747-760 and
+ * apply_nonlocal_batched, apply.zig:1449-1482); host memory, blocking. */
+int dzhip_ham_apply_nonlocal(dzhip_ham* ham, const double* x, double* y, uint64_t n, uint64_t ncols,
+                             int precision);
+
+/* y = S x.  Norm-conserving species carry no q_ij, so S = I and this is a
+ * copy (apply_overlap, apply.zig:1041-1111; the callback's apply_s = null). */
+int dzhip_ham_apply_overlap(dzhip_ham* ham, const double* x, double* y, uint64_t n, uint64_t ncols);
+
+/* Launch subsequent work of this context on `stream` (a hipStream_t, or NULL
+ * for the context's own stream).  Lets callers interleave with their own
+ * streams and time with their own events. */
+int dzhip_ham_set_stream(dzhip_ham* ham, void* stream);
+
+/* Wait for all work enqueued by this context. */
+int dzhip_ham_sync(dzhip_ham* ham);
+
+/* Phase timers (the names of ScfProfile, src/features/scf/logging.zig:114-131):
+ * when enabled, each apply records HIP events around its phases and
+ * dzhip_ham_phase_times() returns the accumulated milliseconds in
+ * out[DZHIP_PHASE_COUNT] and the number of applies in *calls. */
+#define DZHIP_PHASE_NL_PROJECT 0 /* <beta|psi> projection + D_ij */
+#define DZHIP_PHASE_PLANE_INV 1  /* scatter + inverse FFT along z, y (per x-plane) */
+#define DZHIP_PHASE_XLINE 2      /* inverse FFT along x, V_loc multiply, forward FFT along x */
+#define DZHIP_PHASE_PLANE_FWD 3  /* forward FFT along y, z + gather + kinetic */
+#define DZHIP_PHASE_NL_BACK 4    /* back-projection accumulate */
+#define DZHIP_PHASE_COUNT 5
+int dzhip_ham_profile(dzhip_ham* ham, int enable);
+int dzhip_ham_phase_times(dzhip_ham* ham, double* out_ms, uint64_t n_out, uint64_t* calls);
+
+/* Sizes the library uses for this context (for roofline accounting):
+ * out[0] = n_pw, [1] = grid points, [2] = x-planes holding the G-sphere,
+ * [3] = projectors P, [4] = device bytes held. */
+int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DZHIP_H */

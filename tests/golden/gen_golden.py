@@ -1,0 +1,61 @@
+"""Writes the golden H.psi / eigen / FFT vectors used by the GPU parity tests.
+
+Inputs are the seeded synthetic workloads (dzhip.workloads); outputs come from
+the FP64 oracle (oracle/), itself pinned by tests/test_oracle_*.py.  Each file
+stores a SHA-256 of its inputs so a drift in the generator is detected.
+
+Run:  python tests/golden/gen_golden.py
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "dft-zig_amd"))
+
+from dzhip import workloads  # noqa: E402
+from oracle import fftconv, hamiltonian  # noqa: E402
+
+sys.path.insert(0, str(HERE.parent))
+from test_oracle_physics import oracle_ctx  # noqa: E402
+from golden_util import input_hash  # noqa: E402
+
+
+def hpsi_file(name: str, nb: int | None = None):
+    w = workloads.BUILDERS[name]()
+    ctx = oracle_ctx(w)
+    psi = w.psi if nb is None else w.psi[:nb]
+    y = hamiltonian.apply_hamiltonian_batched(ctx, psi)
+    ynl = hamiltonian.apply_nonlocal_batched(ctx, psi)
+    out = {"y": y, "y_nonlocal": ynl, "nbands": np.array(psi.shape[0]), "sha256": np.array(input_hash(w))}
+    if ctx.npw <= 400:
+        h = hamiltonian.dense_hamiltonian(ctx)
+        ev, _ = np.linalg.eigh(h)
+        out["eigvals"] = ev[:12]
+    np.savez_compressed(HERE / f"hpsi_{name}.npz", **out)
+    print(name, ctx.npw, psi.shape[0], input_hash(w)[:16])
+
+
+def fft_file():
+    rng = np.random.default_rng(20261202)
+    out = {}
+    for grid in ((12, 10, 8), (6, 5, 4)):
+        n = int(np.prod(grid))
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        key = "x".join(map(str, grid))
+        out[f"in_{key}"] = x
+        out[f"fwd_{key}"] = fftconv.forward3d(x, grid)
+        out[f"inv_{key}"] = fftconv.inverse3d(x, grid)
+    np.savez_compressed(HERE / "fft_grids.npz", **out)
+    print("fft", list(out)[:3])
+
+
+if __name__ == "__main__":
+    hpsi_file("tiny")
+    hpsi_file("si8")
+    fft_file()

@@ -1,0 +1,168 @@
+"""GPU parity of the HIP H.psi path and FFT backend against the FP64 oracle and golden vectors.
+
+Tolerances (relative to the largest |H psi| entry of the block):
+  FP64 path: 1e-12  (north star: H.psi checked against an FP64 oracle; FP64 mode must be exact
+             to rounding so that final eigenvalues match to 1e-8 Ha)
+  FP32 path: 2e-5   (FP32 FFT / V_loc / projector arithmetic with FP32 accumulation)
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from dzhip import workloads
+import dzhip.tables as tables
+from oracle import fftconv, hamiltonian
+
+sys.path.insert(0, str(Path(__file__).resolve().parent))
+from test_oracle_physics import oracle_ctx  # noqa: E402
+from golden_util import input_hash  # noqa: E402
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+TOL64 = 1e-12
+TOL32 = 2e-5
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def make_ham(dz, ctx, w):
+    return dz.Hamiltonian.from_workload(ctx, w)
+
+
+@pytest.mark.parametrize("grid", [(12, 10, 8), (6, 5, 4)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fft_golden(dz, ctx, grid, prec):
+    g = np.load(GOLDEN / "fft_grids.npz")
+    key = "x".join(map(str, grid))
+    tol = TOL64 if prec == 0 else TOL32
+    plan = dz.FftPlan3d(ctx, *grid, precision=prec)
+    x = g[f"in_{key}"].copy()
+    plan.forward(x)
+    assert relerr(x, g[f"fwd_{key}"]) < tol
+    x = g[f"in_{key}"].copy()
+    plan.inverse(x)
+    assert relerr(x, g[f"inv_{key}"]) < tol
+
+
+@pytest.mark.parametrize("grid", [(32, 32, 32), (30, 30, 30), (72, 72, 72), (75, 75, 75), (90, 90, 90),
+                                  (24, 36, 40)])
+def test_fft_sizes_vs_numpy(dz, ctx, grid):
+    rng = np.random.default_rng(sum(grid))
+    n = int(np.prod(grid))
+    x0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for prec, tol in ((0, TOL64), (1, TOL32)):
+        plan = dz.FftPlan3d(ctx, *grid, precision=prec)
+        x = x0.copy()
+        plan.forward(x)
+        assert relerr(x, fftconv.forward3d(x0, grid)) < tol, (grid, prec)
+        plan.inverse(x)
+        assert relerr(x, x0) < tol * 4, (grid, prec)
+
+
+def test_fft_size_mismatch_and_bad_grid(dz, ctx):
+    plan = dz.FftPlan3d(ctx, 8, 8, 8)
+    with pytest.raises(dz.InvalidMatrixSize):
+        plan.forward(np.zeros(8 * 8 * 7, complex))
+    with pytest.raises(dz.BadArgument):
+        dz.FftPlan3d(ctx, 7, 8, 8)  # 7 is not 2/3/5-smooth
+
+
+@pytest.mark.parametrize("name", ["tiny", "si8"])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hpsi_golden(dz, ctx, name, prec):
+    w = workloads.BUILDERS[name]()
+    g = np.load(GOLDEN / f"hpsi_{name}.npz")
+    assert str(g["sha256"]) == input_hash(w), "workload generator drifted from the golden inputs"
+    ham = make_ham(dz, ctx, w)
+    tol = TOL64 if prec == 0 else TOL32
+    y = ham.apply(w.psi, precision=prec)
+    assert relerr(y, g["y"]) < tol
+    ynl = ham.apply_nonlocal(w.psi, precision=prec)
+    assert relerr(ynl, g["y_nonlocal"]) < tol
+    # Single-vector callback (apply_hamiltonian) equals the batched one.
+    y1 = ham.apply(w.psi[1], precision=prec)
+    assert relerr(y1, g["y"][1]) < tol
+
+
+def test_hpsi_bench_shape_vs_oracle(dz, ctx):
+    """Full bench shape (72^3, 23,847 PWs, 64 atoms x 4 projectors), 8 bands against the oracle."""
+    w = workloads.bench(nbands=8)
+    ctx_o = oracle_ctx(w)
+    ref = hamiltonian.apply_hamiltonian_batched(ctx_o, w.psi)
+    ham = make_ham(dz, ctx, w)
+    assert relerr(ham.apply(w.psi, 0), ref) < TOL64
+    e32 = relerr(ham.apply(w.psi, 1), ref)
+    assert e32 < TOL32
+    print("bench fp32 max rel err", e32)
+
+
+def test_hpsi_hermitian_and_linear_full_block(dz, ctx):
+    """Size-independent properties at 160 bands: <a|H b> = <H a|b>, linearity."""
+    w = workloads.bench(nbands=160)
+    ham = make_ham(dz, ctx, w)
+    y = ham.apply(w.psi, 0)
+    a, b = w.psi[3], w.psi[77]
+    lhs = np.vdot(a, y[77])
+    rhs = np.vdot(y[3], b)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    comb = 0.3 * w.psi[5] - 1.7j * w.psi[9]
+    yc = ham.apply(comb, 0)
+    assert relerr(yc, 0.3 * y[5] - 1.7j * y[9]) < 1e-12
+
+
+def test_hpsi_column_chunking_and_odd_counts(dz, ctx):
+    """ncols above one pass (256) and odd counts give the same columns as small calls."""
+    w = workloads.si8(nbands=300)
+    ham = make_ham(dz, ctx, w)
+    y = ham.apply(w.psi, 0)
+    y7 = ham.apply(w.psi[290:297], 0)
+    assert relerr(y7, y[290:297]) < 1e-14
+
+
+def test_hpsi_errors(dz, ctx):
+    w = workloads.tiny()
+    ham = make_ham(dz, ctx, w)
+    with pytest.raises(dz.InvalidMatrixSize):
+        ham.apply(w.psi[0][:-1])
+    with pytest.raises(dz.InvalidMatrixSize):
+        ham.set_local(w.local_r[:-1])
+    s = ham.apply_s(w.psi)
+    np.testing.assert_array_equal(s, w.psi)
+    # No local potential yet -> BadState.
+    ham2 = dz.Hamiltonian(ctx, w.grid, tables.fft_indices(w.basis, w.grid), w.basis.kinetic,
+                          w.basis.cart, None, w.inv_volume)
+    with pytest.raises(dz.BadState):
+        ham2.apply(w.psi[0])
+    ham2.set_local(w.local_r)
+    ctx_o = oracle_ctx(w)
+    ctx_o.species = []
+    ref = np.array([hamiltonian.apply_hamiltonian(ctx_o, p) for p in w.psi])
+    assert relerr(ham2.apply(w.psi, 0), ref) < TOL64
+
+
+def test_apply_dev_with_torch_buffers(dz, ctx):
+    import torch
+
+    w = workloads.si8(nbands=12)
+    ham = make_ham(dz, ctx, w)
+    ref = ham.apply(w.psi, 0)
+    x = torch.from_numpy(w.psi.view(np.float64).copy()).cuda()
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    ham.set_stream(stream.cuda_stream)
+    ham.apply_dev(x.data_ptr(), y.data_ptr(), 12, 0)
+    torch.cuda.synchronize()
+    got = y.cpu().numpy().view(np.complex128).reshape(12, -1)
+    assert relerr(got, ref) < 1e-15
+    x32 = torch.from_numpy(w.psi.astype(np.complex64).view(np.float32).copy()).cuda()
+    y32 = torch.empty_like(x32)
+    ham.apply_dev(x32.data_ptr(), y32.data_ptr(), 12, 1)
+    torch.cuda.synchronize()
+    got32 = y32.cpu().numpy().view(np.complex64).reshape(12, -1)
+    assert relerr(got32.astype(np.complex128), ref) < TOL32
+    ham.set_stream(None)
