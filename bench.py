@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""H.psi block-apply benchmark on one MI355X (the isolated H.psi kernel configuration).
+
+Workload (synthetic, seeded): cubic box L = 20.5 bohr, ecut 15 Ha, 72^3 grid, N_pw = 23,847,
+160 bands, V_loc = 64 Gaussians, 64 atoms x (s + 3p) Gaussian KB projectors.  One step = one
+apply of H to the whole 160-band block with psi already resident in HBM.
+
+value = algorithmic H.psi bytes per block / time per block (GB/s), for the FP32 apply (the
+mixed-precision SCF path); the FP64 apply is timed the same way and reported beside it.
+Algorithmic bytes per block (lower bound for any design that materialises psi(r) once per band):
+    B = nb*s_c*(2*N_pw + 2*N) + s_r*N + 4*N_pw + s_r*4*N_pw
+with s_c/s_r = 8/4 (FP32) or 16/8 (FP64).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "dft-zig_amd"))
+
+METRIC = "SCF time-to-1e-6 Ha & H·ψ GB/s, Si-64: mixed+batched+learned vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_bench.txt"
+
+
+def hpsi_bytes(nb, npw, ngrid, prec):
+    sc, sr = (8, 4) if prec == "fp32" else (16, 8)
+    return nb * sc * (2 * npw + 2 * ngrid) + sr * ngrid + 4 * npw + sr * 4 * npw
+
+
+def kernel_model(phase, nb, npw, ngrid, planes, plane_elems, nproj, prec):
+    """(bound, algorithmic amount per launch, unit) for each phase of the apply."""
+    sc, sr = (8, 4) if prec == "fp32" else (16, 8)
+    half = planes * plane_elems
+    if phase == "plane_inv":
+        return "hbm", nb * (npw + half) * sc, "B"
+    if phase == "xline":
+        return "hbm", nb * 2 * half * sc + ngrid * sr, "B"
+    if phase == "plane_fwd":
+        return "hbm", nb * (half + 2 * npw) * sc + npw * sr, "B"
+    # complex GEMM, structure factor folded into the projector matrix: 8 flops per complex MAC
+    return "mfma", 8.0 * nproj * npw * nb, "flop"
+
+
+def read_traffic():
+    """Bytes per launch of the dominant kernel from the committed PMC summary, if present."""
+    if not TRAFFIC_FILE.exists():
+        return {}
+    out = {}
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch"):
+            _, kern, val = line.split()[:3]
+            out[kern] = float(val)
+    return out
+
+
+def time_gpu(ham, x, y, ncols, prec_code, steps, warmup, torch):
+    # An explicit (non-default) stream: the library launches on it and torch's events bracket
+    # exactly those launches.  (The default stream's handle is 0, which the library reads as
+    # "use the context's own stream".)
+    stream = torch.cuda.Stream()
+    ham.set_stream(stream.cuda_stream)
+    for _ in range(warmup):
+        ham.apply_dev(x.data_ptr(), y.data_ptr(), ncols, prec_code)
+    torch.cuda.synchronize()
+    ham.profile(True)
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for _ in range(steps):
+        ham.apply_dev(x.data_ptr(), y.data_ptr(), ncols, prec_code)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / steps
+    phases, calls = ham.phase_times()
+    ham.profile(False)
+    ham.set_stream(None)
+    return ms, {k: v / max(calls, 1) for k, v in phases.items()}
+
+
+def cpu_baseline(w, budget_s=12.0):
+    """Oracle (numpy FP64 restatement of the reference apply) on one host core."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import adapters, hamiltonian
+
+    ctx_o = adapters.context_from_workload(w)
+    done, t_used = 0, 0.0
+    with threadpool_limits(limits=1):
+        while t_used < budget_s and done < w.nbands:
+            nb = min(4, w.nbands - done)
+            t = time.perf_counter()
+            hamiltonian.apply_hamiltonian_batched(ctx_o, w.psi[done:done + nb])
+            t_used += time.perf_counter() - t
+            done += nb
+    gbs = hpsi_bytes(done, w.npw, int(np.prod(w.grid)), "fp64") / t_used / 1e9
+    return {"value": round(gbs, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"oracle FP64 H.psi (numpy pocketfft + 1-thread BLAS) on {done} of {w.nbands} bands "
+                      f"of the same workload, {t_used:.1f} s; "
+                      f"{t_used / done * 1e3:.1f} ms/band -> {t_used / done * w.nbands:.1f} s per 160-band block"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (k-point sharding is "
+              "design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import dzhip
+    from dzhip import workloads
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    w = workloads.bench(nbands=160)
+    ngrid = int(np.prod(w.grid))
+    ctx = dzhip.Context(0)
+    ham = dzhip.Hamiltonian.from_workload(ctx, w)
+    info = ham.info()
+    nb, npw = w.nbands, w.npw
+
+    x64 = torch.from_numpy(w.psi.view(np.float64).copy()).cuda()
+    y64 = torch.empty_like(x64)
+    x32 = torch.from_numpy(w.psi.astype(np.complex64).view(np.float32).copy()).cuda()
+    y32 = torch.empty_like(x32)
+
+    ms32, ph32 = time_gpu(ham, x32, y32, nb, dzhip.FP32, args.steps, args.warmup, torch)
+    ms64, ph64 = time_gpu(ham, x64, y64, nb, dzhip.FP64, args.steps, args.warmup, torch)
+
+    h32 = y32.cpu().numpy().view(np.complex64).reshape(nb, npw).astype(np.complex128)
+    h64 = y64.cpu().numpy().view(np.complex128).reshape(nb, npw)
+    err = float(np.max(np.linalg.norm(h32 - h64, axis=1) / np.linalg.norm(h64, axis=1)))
+    abserr = float(np.max(np.abs(h32 - h64)))
+
+    gbs32 = hpsi_bytes(nb, npw, ngrid, "fp32") / (ms32 * 1e-3) / 1e9
+    gbs64 = hpsi_bytes(nb, npw, ngrid, "fp64") / (ms64 * 1e-3) / 1e9
+
+    # Roofline of the dominant kernel of the FP32 apply (per-phase HIP events, same stream).
+    dom = max(ph32, key=ph32.get)
+    plane_elems = w.grid[1] * w.grid[2]
+    bound, amount, _ = kernel_model(dom, nb, npw, ngrid, info["planes"], plane_elems, info["projectors"], "fp32")
+    t_dom = ph32[dom] * 1e-3
+    traffic = read_traffic().get(dom)
+    if bound == "hbm":
+        achieved = amount / t_dom / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes": amount, "avg_ms": round(ph32[dom], 5)}
+    else:
+        achieved = amount / t_dom / 1e12
+        peak = MFMA_PEAK_TFLOPS["fp32"]
+        roof = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 2), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
+                "algorithmic_flops": amount, "avg_ms": round(ph32[dom], 5)}
+
+    cpu = None if args.no_cpu else cpu_baseline(w)
+
+    out = {
+        "metric": METRIC,
+        "value": round(gbs32, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded PCG64 20261202: Gaussian V_loc, Gaussian s/p KB projectors, random psi)",
+        "config": {"workload": "H.psi block apply: L=20.5 bohr cube, ecut 15 Ha, 72^3 grid, N_pw=23847, "
+                               "160 bands, 64 atoms x (s+3p) projectors", "n_pw": npw, "grid": list(w.grid),
+                   "bands": nb, "projectors": info["projectors"], "x_planes": info["planes"],
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp64": {"value": round(gbs64, 2), "unit": "GB/s", "ms_per_step": round(ms64, 4),
+                 "frac_hbm": round(gbs64 / HBM_PEAK_GBS, 4)},
+        "fp32_frac_hbm": round(gbs32 / HBM_PEAK_GBS, 4),
+        "fp32_vs_fp64_err": {"max_band_rel_l2": err, "max_abs": abserr},
+        "phases_ms_fp32": {k: round(v, 5) for k, v in ph32.items()},
+        "phases_ms_fp64": {k: round(v, 5) for k, v in ph64.items()},
+        "scf_time_to_1e-6_Ha": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

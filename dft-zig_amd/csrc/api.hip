@@ -93,8 +93,9 @@ dzhip_fft_plan* dzhip_fft_plan_create(dzhip_ctx* ctx, uint64_t nx, uint64_t ny, 
         DZ_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nx <= 256 && ny <= 256 && nz <= 256, DZHIP_ERR_ARG,
                    "FFT dimensions must be in 1..256");
         const size_t esz = precision == DZHIP_FP64 ? sizeof(double2) : sizeof(float2);
-        DZ_REQUIRE(nx * ny * esz <= 160 * 1024 && nx * nz * esz <= 160 * 1024, DZHIP_ERR_ARG,
-                   "FFT plane exceeds one LDS tile");
+        DZ_REQUIRE(fft_xy_lds_bytes((int)nx, (int)ny, esz) <= kMaxLdsBytes &&
+                       fft_z_lds_bytes((int)nx, (int)nz, esz) <= kMaxLdsBytes,
+                   DZHIP_ERR_ARG, "FFT plane exceeds one LDS tile");
         DeviceGuard g(ctx->device);
         auto* p = new dzhip_fft_plan();
         p->ctx = ctx;
@@ -287,6 +288,8 @@ int dzhip_ham_profile(dzhip_ham* ham, int enable) {
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
         std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        ham->collect_profile();
         ham->profile = enable != 0;
         for (double& v : ham->phase_ms) v = 0.0;
         ham->calls = 0;
@@ -297,6 +300,8 @@ int dzhip_ham_phase_times(dzhip_ham* ham, double* out_ms, uint64_t n_out, uint64
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
         std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        ham->collect_profile();
         for (uint64_t i = 0; i < n_out && i < DZHIP_PHASE_COUNT; ++i) out_ms[i] = ham->phase_ms[i];
         if (calls) *calls = ham->calls;
     });

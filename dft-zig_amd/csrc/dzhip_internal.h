@@ -59,7 +59,7 @@ template <typename C>
 __device__ __forceinline__ C cmul_i(C a) { return C{-a.y, a.x}; }
 
 // ----------------------------------------------------------------- FFT axis
-// One transform length n = prod radix[s] (radices 2/3/4/5), with device tables:
+// One transform length n = prod radix[s] (radices 2/3/4/5/8/9), with device tables:
 //   twiddle[t] = exp(-2 pi i t / n), t in [0, n)   (inverse uses the conjugate)
 //   perm[i]    = LDS position of input index i for the in-place DIT, which is also
 //                the position where the in-place DIF leaves output frequency i.
@@ -96,7 +96,7 @@ struct Context {
     std::mutex mu;
 };
 
-inline size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+__host__ __device__ inline size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Device buffer that only grows.
 struct DevBuf {
@@ -124,5 +124,8 @@ struct DevBuf {
 };
 
 void set_max_dynamic_lds(const void* func, size_t bytes);
+size_t fft_xy_lds_bytes(int nx, int ny, size_t esz);
+size_t fft_z_lds_bytes(int nx, int nz, size_t esz);
+constexpr size_t kMaxLdsBytes = 160 * 1024;
 
 }  // namespace dzhip

@@ -17,11 +17,9 @@ namespace dzhip {
 bool factor_fft_len(int n, std::vector<int>& radices) {
     radices.clear();
     if (n < 1 || n > kMaxFftLen) return false;
-    if (n == 1) return true;
     int m = n;
-    // Prefer radix 4, then 2, 3, 5 (fewer LDS round trips than radix 2).
-    while (m % 4 == 0 && m > 4) { radices.push_back(4); m /= 4; }
-    for (int r : {4, 2, 3, 5}) {
+    // Largest register radices first: fewer LDS round trips per line.
+    for (int r : {9, 8, 4, 2, 3, 5}) {
         while (m % r == 0) { radices.push_back(r); m /= r; }
     }
     if (m != 1) return false;
@@ -64,8 +62,7 @@ void AxisTables::build(int nx, int ny, int nz) {
     for (int d = 0; d < 3; ++d) {
         const int n = lens[d];
         for (int t = 0; t < n; ++t) {
-            // exp(-2 pi i t/n), evaluated with the angle reduced to [0, pi/4] symmetry-free
-            // direct form in long double for an accurately rounded table.
+            // exp(-2 pi i t/n), evaluated in long double and rounded once.
             long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / n;
             double c = (double)cosl(ang), s = (double)sinl(ang);
             h64[ax[d].tw_off + t] = make_double2(c, s);
@@ -92,61 +89,72 @@ void AxisTables::release() {
     perm = nullptr;
 }
 
+size_t fft_xy_lds_bytes(int nx, int ny, size_t esz) { return ((size_t)(nx + 1) * ny + nx + ny) * esz; }
+size_t fft_z_lds_bytes(int nx, int nz, size_t esz) { return ((size_t)(nx + 1) * nz + nz) * esz; }
+
 void set_max_dynamic_lds(const void* func, size_t bytes) {
     if (bytes <= 65536) return;
     DZ_HIP(hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
 // ----------------------------------------------------------------- kernels
-constexpr int kFftThreads = 256;
-
-// Pass 1: z-plane = contiguous [y][x] tile; transform along x, then y.
+// Pass 1: z-plane = contiguous [y][x] block; LDS tile rows of pitch nx+1; transform along
+// x, then y.
 template <typename T>
-__global__ __launch_bounds__(kFftThreads) void k_fft_xy(typename C2T<T>::type* __restrict__ data,
-                                                        FftAxis axx, FftAxis axy,
-                                                        const typename C2T<T>::type* __restrict__ tw,
-                                                        const int* __restrict__ perm, T sign) {
+__global__ __launch_bounds__(TileThreads<T>::value) void k_fft_xy(
+    typename C2T<T>::type* __restrict__ data, FftAxis axx, FftAxis axy,
+    const typename C2T<T>::type* __restrict__ tw, const int* __restrict__ perm, T sign) {
     using C = typename C2T<T>::type;
     extern __shared__ unsigned char smem_raw[];
+    const int nx = axx.n, ny = axy.n, pitch = nx + 1;
     C* buf = reinterpret_cast<C*>(smem_raw);
-    const int nx = axx.n, ny = axy.n;
+    C* twx = buf + pitch * ny;
+    C* twy = twx + nx;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    load_twiddles(twx, tw + axx.tw_off, nx, tid, nthr);
+    load_twiddles(twy, tw + axy.tw_off, ny, tid, nthr);
     C* plane = data + (size_t)blockIdx.x * nx * ny;
     const int* px = perm + axx.perm_off;
     const int* py = perm + axy.perm_off;
+    const unsigned mx = div_magic(nx);
     for (int i = tid; i < nx * ny; i += nthr) {
-        const int y = i / nx, x = i - y * nx;
-        buf[px[x] + nx * py[y]] = plane[i];
+        const int y = fdiv(i, nx, mx), x = i - y * nx;
+        buf[px[x] + pitch * py[y]] = plane[i];
     }
     __syncthreads();
-    lds_fft_lines<T, C, true>(buf, axx, tw + axx.tw_off, ny, 1, nx, sign, tid, nthr);
-    lds_fft_lines<T, C, true>(buf, axy, tw + axy.tw_off, nx, nx, 1, sign, tid, nthr);
-    for (int i = tid; i < nx * ny; i += nthr) plane[i] = buf[i];
+    lds_fft_lines<T, C, true>(buf, axx, twx, LineSet{ny, 1, pitch, nullptr}, sign, tid, nthr);
+    lds_fft_lines<T, C, true>(buf, axy, twy, LineSet{nx, pitch, 1, nullptr}, sign, tid, nthr);
+    for (int i = tid; i < nx * ny; i += nthr) {
+        const int y = fdiv(i, nx, mx), x = i - y * nx;
+        plane[i] = buf[x + pitch * y];
+    }
 }
 
-// Pass 2: fixed y, tile [z][x]; transform along z; optional scale.
+// Pass 2: fixed y, LDS tile [z][x] of pitch nx+1; transform along z; optional scale.
 template <typename T>
-__global__ __launch_bounds__(kFftThreads) void k_fft_z(typename C2T<T>::type* __restrict__ data,
-                                                       int nx, int ny, FftAxis axz,
-                                                       const typename C2T<T>::type* __restrict__ tw,
-                                                       const int* __restrict__ perm, T sign,
-                                                       T scale) {
+__global__ __launch_bounds__(TileThreads<T>::value) void k_fft_z(
+    typename C2T<T>::type* __restrict__ data, int nx, int ny, FftAxis axz,
+    const typename C2T<T>::type* __restrict__ tw, const int* __restrict__ perm, T sign, T scale) {
     using C = typename C2T<T>::type;
     extern __shared__ unsigned char smem_raw[];
+    const int nz = axz.n, pitch = nx + 1;
     C* buf = reinterpret_cast<C*>(smem_raw);
-    const int nz = axz.n;
+    C* twz = buf + pitch * nz;
     const int y = blockIdx.x;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    load_twiddles(twz, tw + axz.tw_off, nz, tid, nthr);
     const int* pz = perm + axz.perm_off;
+    const unsigned mx = div_magic(nx);
     for (int i = tid; i < nx * nz; i += nthr) {
-        const int z = i / nx, x = i - z * nx;
-        buf[x + nx * pz[z]] = data[x + (size_t)nx * (y + (size_t)ny * z)];
+        const int z = fdiv(i, nx, mx), x = i - z * nx;
+        buf[x + pitch * pz[z]] = data[x + (size_t)nx * (y + (size_t)ny * z)];
     }
     __syncthreads();
-    lds_fft_lines<T, C, true>(buf, axz, tw + axz.tw_off, nx, nx, 1, sign, tid, nthr);
+    lds_fft_lines<T, C, true>(buf, axz, twz, LineSet{nx, pitch, 1, nullptr}, sign, tid, nthr);
     for (int i = tid; i < nx * nz; i += nthr) {
-        const int z = i / nx, x = i - z * nx;
-        data[x + (size_t)nx * (y + (size_t)ny * z)] = cscale(buf[i], scale);
+        const int z = fdiv(i, nx, mx), x = i - z * nx;
+        const C v = buf[x + pitch * z];
+        data[x + (size_t)nx * (y + (size_t)ny * z)] = C{v.x * scale, v.y * scale};
     }
 }
 
@@ -179,15 +187,16 @@ void fft3d_device(typename C2T<T>::type* data, const AxisTables& tab, bool inver
     const C* tw = TwSel<T>::get(tab.tw64, tab.tw32);
     const T sign = inverse ? T(1) : T(-1);
     const T scale = inverse ? T(1.0 / ((double)nx * ny * nz)) : T(1);
-    size_t lds1 = (size_t)nx * ny * sizeof(C);
-    size_t lds2 = (size_t)nx * nz * sizeof(C);
+    const size_t lds1 = fft_xy_lds_bytes(nx, ny, sizeof(C));
+    const size_t lds2 = fft_z_lds_bytes(nx, nz, sizeof(C));
     set_max_dynamic_lds((const void*)k_fft_xy<T>, lds1);
     set_max_dynamic_lds((const void*)k_fft_z<T>, lds2);
-    hipLaunchKernelGGL(k_fft_xy<T>, dim3(nz), dim3(kFftThreads), lds1, s, data, tab.ax[0],
-                       tab.ax[1], tw, tab.perm, sign);
+    constexpr int nt = TileThreads<T>::value;
+    hipLaunchKernelGGL(k_fft_xy<T>, dim3(nz), dim3(nt), lds1, s, data, tab.ax[0], tab.ax[1], tw,
+                       tab.perm, sign);
     DZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_fft_z<T>, dim3(ny), dim3(kFftThreads), lds2, s, data, nx, ny, tab.ax[2],
-                       tw, tab.perm, sign, scale);
+    hipLaunchKernelGGL(k_fft_z<T>, dim3(ny), dim3(nt), lds2, s, data, nx, ny, tab.ax[2], tw,
+                       tab.perm, sign, scale);
     DZ_HIP(hipGetLastError());
 }
 

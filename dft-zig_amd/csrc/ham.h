@@ -39,10 +39,12 @@ struct Ham {
     // G-sphere split into x-planes (one plane per distinct FFT x of a plane wave):
     //   plane_start[p]..plane_start[p+1]: entries of plane p
     //   plane_pw[j]  : plane-wave index of entry j
-    //   plane_pos[j] : LDS position py[y] + ny*pz[z] of that plane wave in the yz tile
+    //   plane_pos[j] : LDS position py[y] + (ny+1)*pz[z] of that plane wave in the yz tile
     //   plane_xpos[p]: LDS row px[x] of plane p in the x-line tile
+    //   plane_ystart[p]..plane_ystart[p+1]: plane_ycols entries = LDS columns py[y] of the
+    //                  y values present in plane p (the z-transforms that are not all zero)
     int nplanes = 0;
-    DevBuf plane_start, plane_pw, plane_pos, plane_xpos;
+    DevBuf plane_start, plane_pw, plane_pos, plane_xpos, plane_ystart, plane_ycols;
 
     DevBuf kinetic64, kinetic32;
     DevBuf vloc64, vloc32;  // V_loc(r)/N in Ry
@@ -55,11 +57,14 @@ struct Ham {
     // Work buffers.
     DevBuf xstage, ystage, x32, y32, tbuf, partial, qbuf;
 
-    // Profiling.
+    // Profiling: one set of phase-boundary events per column pass, recorded on the stream
+    // without waiting; read (and recycled) by collect_profile().
     bool profile = false;
-    hipEvent_t ev[DZHIP_PHASE_COUNT + 1] = {};
+    std::vector<std::vector<hipEvent_t>> ev_sets;  // each: DZHIP_PHASE_COUNT + 1 events
+    size_t ev_used = 0;
     double phase_ms[DZHIP_PHASE_COUNT] = {};
     uint64_t calls = 0;
+    void collect_profile();
 
     size_t device_bytes() const;
     std::mutex mu;

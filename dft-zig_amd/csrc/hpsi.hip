@@ -27,89 +27,128 @@
 
 namespace dzhip {
 
-constexpr int kPlaneThreads = 256;
+// LDS layout of the tile kernels: tile rows of pitch ny+1 (odd in elements), then the
+// twiddle tables of the axes used, then (plane kernels) the list of active y columns.
+template <typename C>
+__host__ __device__ inline size_t plane_lds_bytes(int ny, int nz) {
+    return ((size_t)nz * (ny + 1) + ny + nz) * sizeof(C) + round_up((size_t)ny * sizeof(short), 16);
+}
+template <typename C>
+__host__ __device__ inline size_t xline_lds_bytes(int nx, int ny) {
+    return ((size_t)nx * (ny + 1) + nx) * sizeof(C);
+}
 
 template <typename T>
-__global__ __launch_bounds__(kPlaneThreads) void k_plane_inv(
+__global__ __launch_bounds__(TileThreads<T>::value) void k_plane_inv(
     const typename C2T<T>::type* __restrict__ x, size_t ldx, typename C2T<T>::type* __restrict__ tbuf,
     const int* __restrict__ pstart, const int* __restrict__ ppw, const int* __restrict__ ppos,
-    int ny, int nz, int nplanes, FftAxis ay, FftAxis az, const typename C2T<T>::type* __restrict__ tw) {
+    const int* __restrict__ ystart, const short* __restrict__ ycols, int ny, int nz, int nplanes,
+    FftAxis ay, FftAxis az, const typename C2T<T>::type* __restrict__ tw) {
     using C = typename C2T<T>::type;
     extern __shared__ unsigned char smem_raw[];
+    const int pitch = ny + 1;
     C* buf = reinterpret_cast<C*>(smem_raw);
+    C* twy = buf + nz * pitch;
+    C* twz = twy + ny;
+    short* cols = reinterpret_cast<short*>(twz + nz);
     const int p = blockIdx.x;
     const size_t col = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int ne = ny * nz;
-    for (int i = tid; i < ne; i += nthr) buf[i] = C{T(0), T(0)};
+    for (int i = tid; i < nz * pitch; i += nthr) buf[i] = C{T(0), T(0)};
+    load_twiddles(twy, tw + ay.tw_off, ny, tid, nthr);
+    load_twiddles(twz, tw + az.tw_off, nz, tid, nthr);
+    const int c0 = ystart[p], nact = ystart[p + 1] - c0;
+    for (int i = tid; i < nact; i += nthr) cols[i] = ycols[c0 + i];
     __syncthreads();
     const C* xc = x + col * ldx;
     const int j1 = pstart[p + 1];
     for (int j = pstart[p] + tid; j < j1; j += nthr) buf[ppos[j]] = xc[ppw[j]];
     __syncthreads();
-    // Inverse (e^{+i}) along z: lines are y-columns, elements ny apart.
-    lds_fft_lines<T, C, true>(buf, az, tw + az.tw_off, ny, ny, 1, T(1), tid, nthr);
-    // Inverse along y: lines are z-rows, elements contiguous.
-    lds_fft_lines<T, C, true>(buf, ay, tw + ay.tw_off, nz, 1, ny, T(1), tid, nthr);
-    C* out = tbuf + (col * nplanes + p) * (size_t)ne;
-    for (int i = tid; i < ne; i += nthr) out[i] = buf[i];
+    // Inverse (e^{+i}) along z, only on the y-columns that hold plane waves.
+    lds_fft_lines<T, C, true>(buf, az, twz, LineSet{nact, pitch, 0, cols}, T(1), tid, nthr);
+    // Inverse along y on every z-row.
+    lds_fft_lines<T, C, true>(buf, ay, twy, LineSet{nz, 1, pitch, nullptr}, T(1), tid, nthr);
+    C* out = tbuf + (col * nplanes + p) * (size_t)ny * nz;
+    const unsigned my = div_magic(ny);
+    for (int i = tid; i < ny * nz; i += nthr) {
+        const int z = fdiv(i, ny, my), y = i - z * ny;
+        out[i] = buf[y + pitch * z];
+    }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kPlaneThreads) void k_xline(
+__global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
     typename C2T<T>::type* __restrict__ tbuf, const int* __restrict__ pxpos, int nplanes,
     const T* __restrict__ vloc, int nx, int ny, int nz, FftAxis ax,
     const typename C2T<T>::type* __restrict__ tw) {
     using C = typename C2T<T>::type;
     extern __shared__ unsigned char smem_raw[];
-    C* buf = reinterpret_cast<C*>(smem_raw);  // [x][y]: element (x, y) at y + ny*x
+    const int pitch = ny + 1;
+    C* buf = reinterpret_cast<C*>(smem_raw);  // [x][y]: element (x, y) at y + pitch*x
+    C* twx = buf + nx * pitch;
     const int z = blockIdx.x;
     const size_t col = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int ne = nx * ny;
-    for (int i = tid; i < ne; i += nthr) buf[i] = C{T(0), T(0)};
+    for (int i = tid; i < nx * pitch; i += nthr) buf[i] = C{T(0), T(0)};
+    load_twiddles(twx, tw + ax.tw_off, nx, tid, nthr);
     __syncthreads();
     C* tcol = tbuf + col * (size_t)nplanes * ny * nz;
+    const unsigned my = div_magic(ny);
     for (int i = tid; i < nplanes * ny; i += nthr) {
-        const int p = i / ny, y = i - p * ny;
-        buf[y + ny * pxpos[p]] = tcol[((size_t)p * nz + z) * ny + y];
+        const int p = fdiv(i, ny, my), y = i - p * ny;
+        buf[y + pitch * pxpos[p]] = tcol[((size_t)p * nz + z) * ny + y];
     }
     __syncthreads();
-    lds_fft_lines<T, C, true>(buf, ax, tw + ax.tw_off, ny, ny, 1, T(1), tid, nthr);
+    const LineSet xl{ny, pitch, 1, nullptr};
+    lds_fft_lines<T, C, true>(buf, ax, twx, xl, T(1), tid, nthr);
     const T* vz = vloc + (size_t)nx * ny * z;
-    for (int i = tid; i < ne; i += nthr) {
-        const int y = i / nx, xx = i - y * nx;
-        C& e = buf[y + ny * xx];
+    const unsigned mx = div_magic(nx);
+    for (int i = tid; i < nx * ny; i += nthr) {
+        const int y = fdiv(i, nx, mx), xx = i - y * nx;
+        C& e = buf[y + pitch * xx];
         const T v = vz[i];
         e = C{e.x * v, e.y * v};
     }
     __syncthreads();
-    lds_fft_lines<T, C, false>(buf, ax, tw + ax.tw_off, ny, ny, 1, T(-1), tid, nthr);
+    lds_fft_lines<T, C, false>(buf, ax, twx, xl, T(-1), tid, nthr);
     for (int i = tid; i < nplanes * ny; i += nthr) {
-        const int p = i / ny, y = i - p * ny;
-        tcol[((size_t)p * nz + z) * ny + y] = buf[y + ny * pxpos[p]];
+        const int p = fdiv(i, ny, my), y = i - p * ny;
+        tcol[((size_t)p * nz + z) * ny + y] = buf[y + pitch * pxpos[p]];
     }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kPlaneThreads) void k_plane_fwd(
+__global__ __launch_bounds__(TileThreads<T>::value) void k_plane_fwd(
     const typename C2T<T>::type* __restrict__ tbuf, const typename C2T<T>::type* __restrict__ x,
     size_t ldx, typename C2T<T>::type* __restrict__ y, size_t ldy, const T* __restrict__ kinetic,
     const int* __restrict__ pstart, const int* __restrict__ ppw, const int* __restrict__ ppos,
-    int ny, int nz, int nplanes, FftAxis ay, FftAxis az, const typename C2T<T>::type* __restrict__ tw,
-    int add_kinetic) {
+    const int* __restrict__ ystart, const short* __restrict__ ycols, int ny, int nz, int nplanes,
+    FftAxis ay, FftAxis az, const typename C2T<T>::type* __restrict__ tw, int add_kinetic) {
     using C = typename C2T<T>::type;
     extern __shared__ unsigned char smem_raw[];
+    const int pitch = ny + 1;
     C* buf = reinterpret_cast<C*>(smem_raw);
+    C* twy = buf + nz * pitch;
+    C* twz = twy + ny;
+    short* cols = reinterpret_cast<short*>(twz + nz);
     const int p = blockIdx.x;
     const size_t col = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int ne = ny * nz;
-    const C* in = tbuf + (col * nplanes + p) * (size_t)ne;
-    for (int i = tid; i < ne; i += nthr) buf[i] = in[i];
+    load_twiddles(twy, tw + ay.tw_off, ny, tid, nthr);
+    load_twiddles(twz, tw + az.tw_off, nz, tid, nthr);
+    const int c0 = ystart[p], nact = ystart[p + 1] - c0;
+    for (int i = tid; i < nact; i += nthr) cols[i] = ycols[c0 + i];
+    const C* in = tbuf + (col * nplanes + p) * (size_t)ny * nz;
+    const unsigned my = div_magic(ny);
+    for (int i = tid; i < ny * nz; i += nthr) {
+        const int z = fdiv(i, ny, my), yy = i - z * ny;
+        buf[yy + pitch * z] = in[i];
+    }
     __syncthreads();
-    lds_fft_lines<T, C, false>(buf, ay, tw + ay.tw_off, nz, 1, ny, T(-1), tid, nthr);
-    lds_fft_lines<T, C, false>(buf, az, tw + az.tw_off, ny, ny, 1, T(-1), tid, nthr);
+    // Forward along y on every z-row (frequency y lands at py[y]), then along z only on the
+    // columns whose frequencies are gathered.
+    lds_fft_lines<T, C, false>(buf, ay, twy, LineSet{nz, 1, pitch, nullptr}, T(-1), tid, nthr);
+    lds_fft_lines<T, C, false>(buf, az, twz, LineSet{nact, pitch, 0, cols}, T(-1), tid, nthr);
     const C* xc = x + col * ldx;
     C* yc = y + col * ldy;
     const int j1 = pstart[p + 1];
@@ -265,7 +304,8 @@ __global__ void k_d2f(const double* __restrict__ in, float* __restrict__ out, si
 
 // ----------------------------------------------------------------- host side
 size_t Ham::device_bytes() const {
-    const DevBuf* bufs[] = {&plane_start, &plane_pw, &plane_pos, &plane_xpos, &kinetic64,
+    const DevBuf* bufs[] = {&plane_start, &plane_pw, &plane_pos, &plane_xpos, &plane_ystart,
+                            &plane_ycols, &kinetic64,
                             &kinetic32, &vloc64, &vloc32, &a64, &a32, &meta, &dm64, &dm32,
                             &xstage, &ystage, &x32, &y32, &tbuf, &partial, &qbuf};
     size_t s = tab.bytes;
@@ -279,11 +319,6 @@ static void upload(DevBuf& buf, const std::vector<V>& v) {
     if (!v.empty()) DZ_HIP(hipMemcpy(buf.ptr, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
 }
 
-static void lds_check(size_t elems, size_t elem_bytes, const char* what) {
-    DZ_REQUIRE(elems * elem_bytes <= 160 * 1024, DZHIP_ERR_ARG,
-               std::string("grid too large for one LDS tile (") + what + ")");
-}
-
 void ham_build(Ham& h, const dzhip_ham_desc& d) {
     DZ_REQUIRE(d.nx > 0 && d.ny > 0 && d.nz > 0, DZHIP_ERR_ARG, "grid dimensions must be positive");
     DZ_REQUIRE(d.n_pw > 0 && d.fft_index && d.kinetic, DZHIP_ERR_ARG,
@@ -295,8 +330,9 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
     h.ngrid = (size_t)d.nx * d.ny * d.nz;
     h.inv_volume = d.inv_volume;
     DZ_REQUIRE(h.ngrid < (size_t)1 << 31, DZHIP_ERR_ARG, "grid too large");
-    lds_check((size_t)h.ny * h.nz, sizeof(double2), "y*z plane");
-    lds_check((size_t)h.nx * h.ny, sizeof(double2), "x*y plane");
+    DZ_REQUIRE(plane_lds_bytes<double2>(h.ny, h.nz) <= kMaxLdsBytes &&
+                   xline_lds_bytes<double2>(h.nx, h.ny) <= kMaxLdsBytes,
+               DZHIP_ERR_ARG, "grid too large for one LDS tile");
     h.tab.build(h.nx, h.ny, h.nz);
 
     // Host copies of the permutations to place plane-wave entries.
@@ -331,11 +367,23 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
     for (int p = 0; p < h.nplanes; ++p) start[p + 1] = start[p] + count[xval[p]];
     std::vector<int> fill(start.begin(), start.end() - 1);
     std::vector<int> pw(h.npw), pos(h.npw);
+    const int pitch = h.ny + 1;  // LDS row pitch of the [z][y] plane tile
+    std::vector<std::vector<char>> yused(h.nplanes, std::vector<char>(h.ny, 0));
     for (size_t i = 0; i < h.npw; ++i) {
         const int p = plane_of_x[xs[i]];
         const int j = fill[p]++;
         pw[j] = (int)i;
-        pos[j] = py[ys[i]] + h.ny * pz[zs[i]];
+        pos[j] = py[ys[i]] + pitch * pz[zs[i]];
+        yused[p][ys[i]] = 1;
+    }
+    // Per plane, the LDS columns (digit-reversed y) that hold plane waves: only these need
+    // the z-transform.
+    std::vector<int> ystart(h.nplanes + 1, 0);
+    std::vector<short> ycols;
+    for (int p = 0; p < h.nplanes; ++p) {
+        for (int yy = 0; yy < h.ny; ++yy)
+            if (yused[p][yy]) ycols.push_back((short)py[yy]);
+        ystart[p + 1] = (int)ycols.size();
     }
     std::vector<int> xpos(h.nplanes);
     for (int p = 0; p < h.nplanes; ++p) xpos[p] = px[xval[p]];
@@ -343,6 +391,8 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
     upload(h.plane_pw, pw);
     upload(h.plane_pos, pos);
     upload(h.plane_xpos, xpos);
+    upload(h.plane_ystart, ystart);
+    upload(h.plane_ycols, ycols);
 
     std::vector<double> kin(d.kinetic, d.kinetic + h.npw);
     std::vector<float> kin32(kin.begin(), kin.end());
@@ -435,7 +485,6 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
             dpos.release();
         }
     }
-    for (auto& e : h.ev) DZ_HIP(hipEventCreate(&e));
     if (d.local_r) ham_set_local(h, d.local_r);
 }
 
@@ -450,13 +499,17 @@ void ham_set_local(Ham& h, const double* local_r) {
 }
 
 void ham_release(Ham& h) {
-    DevBuf* bufs[] = {&h.plane_start, &h.plane_pw, &h.plane_pos, &h.plane_xpos, &h.kinetic64,
+    DevBuf* bufs[] = {&h.plane_start, &h.plane_pw, &h.plane_pos, &h.plane_xpos, &h.plane_ystart,
+                      &h.plane_ycols, &h.kinetic64,
                       &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.dm64,
                       &h.dm32, &h.xstage, &h.ystage, &h.x32, &h.y32, &h.tbuf, &h.partial, &h.qbuf};
     for (DevBuf* b : bufs) b->release();
     h.tab.release();
-    for (auto& e : h.ev)
-        if (e) (void)hipEventDestroy(e);
+    for (auto& set : h.ev_sets)
+        for (auto& e : set)
+            if (e) (void)hipEventDestroy(e);
+    h.ev_sets.clear();
+    h.ev_used = 0;
 }
 
 template <typename T> struct Sel;
@@ -475,8 +528,32 @@ template <> struct Sel<float> {
     static const float2* tw(const Ham& h) { return h.tab.tw32; }
 };
 
-static void mark(Ham& h, int i) {
-    if (h.profile) DZ_HIP(hipEventRecord(h.ev[i], h.stream()));
+// Event set for the current column pass (nullptr when profiling is off).
+static hipEvent_t* next_event_set(Ham& h) {
+    if (!h.profile) return nullptr;
+    if (h.ev_used == h.ev_sets.size()) {
+        std::vector<hipEvent_t> set(DZHIP_PHASE_COUNT + 1);
+        for (auto& e : set) DZ_HIP(hipEventCreate(&e));
+        h.ev_sets.push_back(std::move(set));
+    }
+    return h.ev_sets[h.ev_used++].data();
+}
+
+static void mark(Ham& h, hipEvent_t* set, int i) {
+    if (set) DZ_HIP(hipEventRecord(set[i], h.stream()));
+}
+
+void Ham::collect_profile() {
+    for (size_t k = 0; k < ev_used; ++k) {
+        hipEvent_t* set = ev_sets[k].data();
+        DZ_HIP(hipEventSynchronize(set[DZHIP_PHASE_COUNT]));
+        for (int i = 0; i < DZHIP_PHASE_COUNT; ++i) {
+            float ms = 0.f;
+            DZ_HIP(hipEventElapsedTime(&ms, set[i], set[i + 1]));
+            phase_ms[i] += ms;
+        }
+    }
+    ev_used = 0;
 }
 
 constexpr size_t kMaxColsPerPass = 256;
@@ -497,7 +574,8 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
         const size_t nc = std::min(kMaxColsPerPass, ncols - c0);
         const C* xc = x + c0 * ld;
         C* yc = y + c0 * ld;
-        mark(h, 0);
+        hipEvent_t* evs = next_event_set(h);
+        mark(h, evs, 0);
         // Nonlocal projection first (reads x only).
         int nks = 1;
         if (do_nl) {
@@ -518,39 +596,42 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
                                Sel<T>::dm(h), h.nproj, (int)nc, (T)h.inv_volume);
             DZ_HIP(hipGetLastError());
         }
-        mark(h, 1);
+        mark(h, evs, 1);
         if (local) {
             const size_t plane = (size_t)h.ny * h.nz;
             h.tbuf.ensure(nc * h.nplanes * plane * sizeof(C));
             C* tb = h.tbuf.as<C>();
-            const size_t lds_p = plane * sizeof(C);
-            const size_t lds_x = (size_t)h.nx * h.ny * sizeof(C);
+            const size_t lds_p = plane_lds_bytes<C>(h.ny, h.nz);
+            const size_t lds_x = xline_lds_bytes<C>(h.nx, h.ny);
+            constexpr int nt = TileThreads<T>::value;
             set_max_dynamic_lds((const void*)k_plane_inv<T>, lds_p);
             set_max_dynamic_lds((const void*)k_plane_fwd<T>, lds_p);
             set_max_dynamic_lds((const void*)k_xline<T>, lds_x);
-            hipLaunchKernelGGL(k_plane_inv<T>, dim3(h.nplanes, (unsigned)nc), dim3(kPlaneThreads), lds_p, s,
+            hipLaunchKernelGGL(k_plane_inv<T>, dim3(h.nplanes, (unsigned)nc), dim3(nt), lds_p, s,
                                xc, ld, tb, h.plane_start.as<int>(), h.plane_pw.as<int>(),
-                               h.plane_pos.as<int>(), h.ny, h.nz, h.nplanes, h.tab.ax[1], h.tab.ax[2], tw);
+                               h.plane_pos.as<int>(), h.plane_ystart.as<int>(), h.plane_ycols.as<short>(),
+                               h.ny, h.nz, h.nplanes, h.tab.ax[1], h.tab.ax[2], tw);
             DZ_HIP(hipGetLastError());
-            mark(h, 2);
-            hipLaunchKernelGGL(k_xline<T>, dim3(h.nz, (unsigned)nc), dim3(kPlaneThreads), lds_x, s, tb,
+            mark(h, evs, 2);
+            hipLaunchKernelGGL(k_xline<T>, dim3(h.nz, (unsigned)nc), dim3(nt), lds_x, s, tb,
                                h.plane_xpos.as<int>(), h.nplanes, Sel<T>::vloc(h), h.nx, h.ny, h.nz,
                                h.tab.ax[0], tw);
             DZ_HIP(hipGetLastError());
-            mark(h, 3);
-            hipLaunchKernelGGL(k_plane_fwd<T>, dim3(h.nplanes, (unsigned)nc), dim3(kPlaneThreads), lds_p, s,
+            mark(h, evs, 3);
+            hipLaunchKernelGGL(k_plane_fwd<T>, dim3(h.nplanes, (unsigned)nc), dim3(nt), lds_p, s,
                                tb, xc, ld, yc, ld, Sel<T>::kin(h), h.plane_start.as<int>(),
-                               h.plane_pw.as<int>(), h.plane_pos.as<int>(), h.ny, h.nz, h.nplanes,
-                               h.tab.ax[1], h.tab.ax[2], tw, 1);
+                               h.plane_pw.as<int>(), h.plane_pos.as<int>(), h.plane_ystart.as<int>(),
+                               h.plane_ycols.as<short>(), h.ny, h.nz, h.nplanes, h.tab.ax[1],
+                               h.tab.ax[2], tw, 1);
             DZ_HIP(hipGetLastError());
-            mark(h, 4);
+            mark(h, evs, 4);
         } else {
             const size_t n = nc * ld;
             hipLaunchKernelGGL(k_fill_zero<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, yc, n);
             DZ_HIP(hipGetLastError());
-            mark(h, 2);
-            mark(h, 3);
-            mark(h, 4);
+            mark(h, evs, 2);
+            mark(h, evs, 3);
+            mark(h, evs, 4);
         }
         if (do_nl) {
             hipLaunchKernelGGL(k_nl_back<T>, dim3((unsigned)((h.npw + 63) / 64), (unsigned)((nc + 15) / 16)),
@@ -558,15 +639,9 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
                                (int)h.npw, (int)nc);
             DZ_HIP(hipGetLastError());
         }
-        mark(h, 5);
-        if (h.profile) {
-            DZ_HIP(hipEventSynchronize(h.ev[5]));
-            for (int i = 0; i < DZHIP_PHASE_COUNT; ++i) {
-                float ms = 0.f;
-                DZ_HIP(hipEventElapsedTime(&ms, h.ev[i], h.ev[i + 1]));
-                h.phase_ms[i] += ms;
-            }
-        }
+        mark(h, evs, 5);
+        // Bound the number of outstanding event sets.
+        if (h.profile && h.ev_used >= 4096) h.collect_profile();
     }
     if (h.profile) h.calls++;
 }

@@ -139,9 +139,10 @@ int dzhip_ham_set_stream(dzhip_ham* ham, void* stream);
 int dzhip_ham_sync(dzhip_ham* ham);
 
 /* Phase timers (the names of ScfProfile, src/features/scf/logging.zig:114-131):
- * when enabled, each apply records HIP events around its phases and
- * dzhip_ham_phase_times() returns the accumulated milliseconds in
- * out[DZHIP_PHASE_COUNT] and the number of applies in *calls. */
+ * when enabled, each apply records HIP events around its phases on the
+ * context's stream without waiting; dzhip_ham_phase_times() waits for them
+ * and returns the accumulated milliseconds in out[DZHIP_PHASE_COUNT] and the
+ * number of applies in *calls.  Enabling or disabling resets the totals. */
 #define DZHIP_PHASE_NL_PROJECT 0 /* <beta|psi> projection + D_ij */
 #define DZHIP_PHASE_PLANE_INV 1  /* scatter + inverse FFT along z, y (per x-plane) */
 #define DZHIP_PHASE_XLINE 2      /* inverse FFT along x, V_loc multiply, forward FFT along x */

@@ -5,12 +5,9 @@ import numpy as np
 
 
 def factor(n):
-    """factor_fft_len: radix 4 while m % 4 == 0 and m > 4, then 4, 2, 3, 5."""
+    """factor_fft_len: register radices 9, 8, 4, 2, 3, 5, largest first."""
     out, m = [], n
-    while m % 4 == 0 and m > 4:
-        out.append(4)
-        m //= 4
-    for r in (4, 2, 3, 5):
+    for r in (9, 8, 4, 2, 3, 5):
         while m % r == 0:
             out.append(r)
             m //= r

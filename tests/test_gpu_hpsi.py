@@ -153,16 +153,17 @@ def test_apply_dev_with_torch_buffers(dz, ctx):
     ref = ham.apply(w.psi, 0)
     x = torch.from_numpy(w.psi.view(np.float64).copy()).cuda()
     y = torch.empty_like(x)
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()
+    assert stream.cuda_stream != 0
     ham.set_stream(stream.cuda_stream)
     ham.apply_dev(x.data_ptr(), y.data_ptr(), 12, 0)
-    torch.cuda.synchronize()
+    stream.synchronize()
     got = y.cpu().numpy().view(np.complex128).reshape(12, -1)
     assert relerr(got, ref) < 1e-15
     x32 = torch.from_numpy(w.psi.astype(np.complex64).view(np.float32).copy()).cuda()
     y32 = torch.empty_like(x32)
     ham.apply_dev(x32.data_ptr(), y32.data_ptr(), 12, 1)
-    torch.cuda.synchronize()
+    stream.synchronize()
     got32 = y32.cpu().numpy().view(np.complex64).reshape(12, -1)
     assert relerr(got32.astype(np.complex128), ref) < TOL32
     ham.set_stream(None)

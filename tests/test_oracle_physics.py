@@ -5,7 +5,7 @@ import math
 import numpy as np
 import pytest
 
-from oracle import density, eigensolver, hamiltonian, projectors, pw
+from oracle import adapters, density, eigensolver, hamiltonian, projectors, pw
 import dzhip.tables as tables
 from dzhip import workloads
 
@@ -73,14 +73,7 @@ def test_radial_table_interpolation():
 
 # ---------------------------------------------------------------- H.psi
 def oracle_ctx(w):
-    sps = []
-    for sp in w.species:
-        rad = [projectors.gaussian_beta_radial(l, rc, projectors.gaussian_beta_norm(l, rc))
-               for l, rc in zip(sp.l_list, sp.rc)]
-        sps.append(projectors.Species(list(sp.l_list), rad, sp.dij))
-    b = pw.Basis(w.basis.h, w.basis.k, w.basis.l, w.basis.cart, w.basis.kpg, w.basis.kinetic)
-    atoms = [hamiltonian.Atom(w.atom_pos[i], int(w.atom_species[i])) for i in range(len(w.atom_species))]
-    return hamiltonian.HamContext(w.grid, b, w.local_r, sps, atoms, w.inv_volume)
+    return adapters.context_from_workload(w)
 
 
 @pytest.fixture(scope="module")
