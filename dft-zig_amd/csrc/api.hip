@@ -307,6 +307,50 @@ int dzhip_ham_phase_times(dzhip_ham* ham, double* out_ms, uint64_t n_out, uint64
     });
 }
 
+int dzhip_eig_solve(dzhip_ham* ham, const double* diag, uint64_t n, uint64_t nbands,
+                    const dzhip_eig_opts* opts, const double* init_vecs, uint64_t init_cols,
+                    double* values_out, double* vectors_out, dzhip_eig_stats* stats) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        DZ_REQUIRE(n == ham->npw, DZHIP_ERR_SIZE, "n != n_pw (InvalidMatrixSize)");
+        DZ_REQUIRE(diag && opts && values_out, DZHIP_ERR_ARG, "NULL diag, opts or values_out");
+        DZ_REQUIRE(ham->has_local, DZHIP_ERR_STATE, "local potential not set");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        eig_solve(*ham, diag, nbands, *opts, init_vecs, init_cols, values_out, vectors_out, stats);
+    });
+}
+
+int dzhip_density_accumulate(dzhip_ham* ham, const double* psi, uint64_t n, uint64_t nbands,
+                             const double* weights, double* rho, uint64_t count, int accumulate) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        DZ_REQUIRE(n == ham->npw, DZHIP_ERR_SIZE, "n != n_pw (InvalidMatrixSize)");
+        DZ_REQUIRE(count == ham->ngrid, DZHIP_ERR_SIZE, "count != nx*ny*nz");
+        DZ_REQUIRE(rho && (nbands == 0 || (psi && weights)), DZHIP_ERR_ARG, "NULL psi, weights or rho");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        hipStream_t s = ham->stream();
+        DevBuf dpsi, dw, drho;
+        drho.ensure(count * sizeof(double));
+        if (accumulate) DZ_HIP(hipMemcpyAsync(drho.ptr, rho, count * sizeof(double), hipMemcpyHostToDevice, s));
+        if (nbands > 0) {
+            dpsi.ensure(n * nbands * sizeof(double2));
+            dw.ensure(nbands * sizeof(double));
+            DZ_HIP(hipMemcpyAsync(dpsi.ptr, psi, n * nbands * sizeof(double2), hipMemcpyHostToDevice, s));
+            DZ_HIP(hipMemcpyAsync(dw.ptr, weights, nbands * sizeof(double), hipMemcpyHostToDevice, s));
+            ham_density_dev(*ham, dpsi.as<double2>(), nbands, dw.as<double>(), drho.as<double>(), accumulate != 0);
+        } else if (!accumulate) {
+            DZ_HIP(hipMemsetAsync(drho.ptr, 0, count * sizeof(double), s));
+        }
+        DZ_HIP(hipMemcpyAsync(rho, drho.ptr, count * sizeof(double), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipStreamSynchronize(s));
+        dpsi.release();
+        dw.release();
+        drho.release();
+    });
+}
+
 int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out) {
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");

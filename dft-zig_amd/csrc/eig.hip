@@ -1,0 +1,493 @@
+// Device-resident block eigensolver for H x = lambda x (S = I) on one k-point.
+//
+// Follows the reference's block-Davidson ("LOBPCG" there,
+// src/features/linalg/lobpcg/serial.zig:55-307) in its outer semantics: subspace grows from
+// block = nbands + 4 by appending preconditioned residuals of the unconverged bands, up to
+// max_subspace = min(n, 4 nbands + 8); converged when max_b ||H x_b - lambda_b x_b||_2 < tol
+// or when no vector can be added; kinetic-shift preconditioner r / (diag - lambda) with
+// |denominator| clamped at 1e-8 (common.zig:214-226); random start from the reference LCG
+// (common.zig:287-319, seed 0x6f3d3cbe).  The block operations are re-designed for the GPU:
+//   * new vectors are orthogonalised against the basis with two passes of block classical
+//     Gram-Schmidt (GEMMs) and orthonormalised with Cholesky QR (in place of one-vector MGS);
+//   * the projected matrix T = V^H H V is extended by the new columns only;
+//   * Rayleigh-Ritz diagonalises T with rocSOLVER zheevd on the device;
+//   * a full subspace restarts from [Ritz vectors, new block], reusing H x = W c (no H applies
+//     are discarded).
+// All subspace algebra is FP64.  In mixed mode the H apply runs in FP32 until the largest
+// residual falls below `mixed_switch_tol`, then the basis restarts from the Ritz vectors with an
+// FP64 apply, so the final eigenpairs are those of the FP64 operator.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "ham.h"
+
+namespace dzhip {
+
+#define DZ_BLAS(call)                                                                           \
+    do {                                                                                        \
+        rocblas_status st_ = (call);                                                            \
+        if (st_ != rocblas_status_success)                                                      \
+            throw Error{DZHIP_ERR_HIP, std::string(#call) + ": " + rocblas_status_to_string(st_)}; \
+    } while (0)
+
+using zc = rocblas_double_complex;
+
+// ----------------------------------------------------------------- kernels
+__global__ void k_col_norm2(const double2* __restrict__ v, size_t ld, int n, double* __restrict__ out) {
+    __shared__ double red[256];
+    const double2* c = v + (size_t)blockIdx.x * ld;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += c[i].x * c[i].x + c[i].y * c[i].y;
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// R[:, b] = HX[:, b] - lambda_b X[:, b]
+__global__ void k_residual(const double2* __restrict__ hx, const double2* __restrict__ x,
+                           const double* __restrict__ lam, double2* __restrict__ r, size_t ld, int n) {
+    const int b = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t o = (size_t)b * ld + i;
+    const double l = lam[b];
+    r[o] = make_double2(hx[o].x - l * x[o].x, hx[o].y - l * x[o].y);
+}
+
+// Q[:, j] = P(R[:, sel_j]): kinetic shift r / (diag - lambda) with the reference's 1e-8 clamp,
+// or Teter-Payne-Allan K(x) r with x = diag / ekin_b (ekin = <x_b|T|x_b>).
+__global__ void k_precondition(const double2* __restrict__ r, const int* __restrict__ sel,
+                               const double* __restrict__ lam, const double* __restrict__ ekin,
+                               const double* __restrict__ diag, double2* __restrict__ q, size_t ld,
+                               int n, int mode) {
+    const int j = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = sel[j];
+    const double2 v = r[(size_t)b * ld + i];
+    double f;
+    if (mode == 0) {
+        double den = diag[i] - lam[b];
+        if (fabs(den) < 1e-8) den = den >= 0.0 ? 1e-8 : -1e-8;
+        f = 1.0 / den;
+    } else {
+        const double x = diag[i] / ekin[b];
+        const double p = 27.0 + x * (18.0 + x * (12.0 + x * 8.0));
+        f = p / (p + 16.0 * x * x * x * x);
+    }
+    q[(size_t)j * ld + i] = make_double2(v.x * f, v.y * f);
+}
+
+// <x_b|T|x_b> = sum_G |k+G|^2 |x_bG|^2 (one workgroup per band)
+__global__ void k_kinetic_energy(const double2* __restrict__ x, const double* __restrict__ diag, size_t ld,
+                                 int n, double* __restrict__ out) {
+    __shared__ double red[256];
+    const double2* c = x + (size_t)blockIdx.x * ld;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += diag[i] * (c[i].x * c[i].x + c[i].y * c[i].y);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// dst[:, j] = src[:, idx_j] * scale_j
+__global__ void k_gather_cols(const double2* __restrict__ src, const int* __restrict__ idx,
+                              const double* __restrict__ scale, double2* __restrict__ dst, size_t ld, int n) {
+    const int j = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s = scale[j];
+    const double2 v = src[(size_t)idx[j] * ld + i];
+    dst[(size_t)j * ld + i] = make_double2(v.x * s, v.y * s);
+}
+
+// T[m0 + i, j] = conj(T[j, m0 + i]) for j < m0 (mirror of the new columns), and the new
+// a x a diagonal block made Hermitian with a real diagonal (common.zig:247-258).
+__global__ void k_mirror_new(double2* __restrict__ t, int ld, int m0, int a) {
+    const int i = blockIdx.y;  // new column index 0..a-1
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m0 + a) return;
+    const int c = m0 + i;
+    if (j < m0) {
+        const double2 v = t[(size_t)c * ld + j];
+        t[(size_t)j * ld + c] = make_double2(v.x, -v.y);
+    } else {
+        const int jj = j - m0;
+        if (jj > i) return;
+        const double2 u = t[(size_t)c * ld + j];  // T[j][c]
+        const double2 w = t[(size_t)j * ld + c];  // T[c][j]
+        if (jj == i) {
+            t[(size_t)c * ld + j] = make_double2(u.x, 0.0);
+        } else {
+            const double re = 0.5 * (u.x + w.x), im = 0.5 * (u.y - w.y);
+            t[(size_t)c * ld + j] = make_double2(re, im);
+            t[(size_t)j * ld + c] = make_double2(re, -im);
+        }
+    }
+}
+
+__global__ void k_copy_block(const double2* __restrict__ src, int lds, double2* __restrict__ dst, int ldd,
+                             int rows, int cols) {
+    const int c = blockIdx.y;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < rows && c < cols) dst[(size_t)c * ldd + r] = src[(size_t)c * lds + r];
+}
+
+// ----------------------------------------------------------------- solver
+struct EigWork {
+    rocblas_handle blas = nullptr;
+    hipStream_t stream = nullptr;
+    size_t n = 0;
+    int max_sub = 0, nb = 0;
+    DevBuf V, W, X, HX, R, Q, HQ, T, Tw, S, G, evals, ework, info, norms, lam, ekin, diag, sel, scl, idx;
+    DevBuf x32, y32;
+    ~EigWork() {
+        if (blas) rocblas_destroy_handle(blas);
+        DevBuf* bufs[] = {&V, &W, &X, &HX, &R, &Q, &HQ, &T, &Tw, &S, &G, &evals, &ework, &info,
+                          &norms, &lam, &ekin, &diag, &sel, &scl, &idx, &x32, &y32};
+        for (DevBuf* b : bufs) b->release();
+    }
+};
+
+static void eig_work_release(Ham& h) {
+    delete static_cast<EigWork*>(h.eig_work);
+    h.eig_work = nullptr;
+}
+
+struct Solver {
+    Ham& h;
+    EigWork& w;
+    const dzhip_eig_opts& o;
+    hipStream_t s;
+    size_t n;
+    int nb, max_sub;
+    bool fp64_apply;
+    dzhip_eig_stats st{};
+    unsigned long long seed;
+
+    double2* V() { return w.V.as<double2>(); }
+    double2* W() { return w.W.as<double2>(); }
+    double2* T() { return w.T.as<double2>(); }
+
+    dim3 cols_grid(int cols) const { return dim3((unsigned)((n + 255) / 256), (unsigned)cols); }
+
+    // W_out[:, 0:cols] = H V_in[:, 0:cols] in the current apply precision.
+    void apply(const double2* vin, double2* wout, int cols) {
+        if (cols <= 0) return;
+        if (fp64_apply) {
+            ham_apply_dev<double>(h, vin, wout, cols, true, true);
+            st.fp64_applies += cols;
+        } else {
+            const size_t cnt = n * cols;
+            w.x32.ensure(cnt * sizeof(float2));
+            w.y32.ensure(cnt * sizeof(float2));
+            convert_to_float(vin, w.x32.as<float2>(), cnt, s);
+            ham_apply_dev<float>(h, w.x32.ptr, w.y32.ptr, cols, true, true);
+            convert_to_double(w.y32.as<float2>(), wout, cnt, s);
+            st.fp32_applies += cols;
+        }
+        st.apply_calls += cols;
+    }
+
+    void gemm(rocblas_operation ta, rocblas_operation tb, int m, int nn, int k, zc alpha, const double2* A,
+              int lda, const double2* B, int ldb, zc beta, double2* C, int ldc) {
+        DZ_BLAS(rocblas_zgemm(w.blas, ta, tb, m, nn, k, &alpha, reinterpret_cast<const zc*>(A), lda,
+                              reinterpret_cast<const zc*>(B), ldb, &beta, reinterpret_cast<zc*>(C), ldc));
+    }
+
+    std::vector<double> col_norms(const double2* v, int cols) {
+        std::vector<double> out(cols);
+        if (cols == 0) return out;
+        hipLaunchKernelGGL(k_col_norm2, dim3(cols), dim3(256), 0, s, v, n, (int)n, w.norms.as<double>());
+        DZ_HIP(hipGetLastError());
+        DZ_HIP(hipMemcpyAsync(out.data(), w.norms.ptr, cols * sizeof(double), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipStreamSynchronize(s));
+        for (double& v2 : out) v2 = std::sqrt(v2);
+        return out;
+    }
+
+    // Orthogonalise Q[:, 0:a] against V[:, 0:m] (two block CGS passes) and orthonormalise it
+    // with Cholesky QR.  A column is dropped when projection leaves less than 1e-8 of its norm
+    // (the reference's MGS threshold, common.zig:263-284, applied relative to the direction's
+    // own norm so that tolerances below 1e-8 Ry remain reachable).  Returns the number of
+    // columns kept (compacted to the front).
+    int orthonormalize_new(double2* Q, int a, int m) {
+        const int ldn = (int)n;
+        const std::vector<double> nrm0 = col_norms(Q, a);
+        for (int pass = 0; pass < 2 && m > 0; ++pass) {
+            // S = V^H Q ; Q -= V S
+            gemm(rocblas_operation_conjugate_transpose, rocblas_operation_none, m, a, ldn, zc(1.0, 0.0), V(), ldn,
+                 Q, ldn, zc(0.0, 0.0), w.S.as<double2>(), max_sub);
+            gemm(rocblas_operation_none, rocblas_operation_none, ldn, a, m, zc(-1.0, 0.0), V(), ldn,
+                 w.S.as<double2>(), max_sub, zc(1.0, 0.0), Q, ldn);
+        }
+        std::vector<double> nrm = col_norms(Q, a);
+        std::vector<int> keep;
+        std::vector<double> inv;
+        for (int j = 0; j < a; ++j)
+            if (nrm0[j] > 0.0 && nrm[j] > 1e-8 * nrm0[j]) {
+                keep.push_back(j);
+                inv.push_back(1.0 / nrm[j]);
+            }
+        const int k = (int)keep.size();
+        if (k == 0) return 0;
+        // Compact and normalise into HQ (scratch), then copy back to Q.
+        DZ_HIP(hipMemcpyAsync(w.idx.ptr, keep.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
+        DZ_HIP(hipMemcpyAsync(w.scl.ptr, inv.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_gather_cols, cols_grid(k), dim3(256), 0, s, Q, w.idx.as<int>(), w.scl.as<double>(),
+                           w.HQ.as<double2>(), n, (int)n);
+        DZ_HIP(hipGetLastError());
+        DZ_HIP(hipMemcpyAsync(Q, w.HQ.ptr, n * k * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        for (int pass = 0; pass < 2; ++pass) cholqr(Q, k);
+        return k;
+    }
+
+    // Q <- Q R^{-1} with G = Q^H Q = R^H R.  Throws if G is not positive definite.
+    void cholqr(double2* Q, int k) {
+        const int ldn = (int)n;
+        gemm(rocblas_operation_conjugate_transpose, rocblas_operation_none, k, k, ldn, zc(1.0, 0.0), Q, ldn, Q, ldn,
+             zc(0.0, 0.0), w.G.as<double2>(), k);
+        DZ_BLAS(rocsolver_zpotrf(w.blas, rocblas_fill_upper, k, w.G.as<zc>(), k, w.info.as<rocblas_int>()));
+        rocblas_int info = 0;
+        DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipStreamSynchronize(s));
+        DZ_REQUIRE(info == 0, DZHIP_ERR_HIP, "Cholesky QR: Gram matrix not positive definite");
+        const zc one(1.0, 0.0);
+        DZ_BLAS(rocblas_ztrsm(w.blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                              rocblas_diagonal_non_unit, ldn, k, &one, w.G.as<zc>(), k, reinterpret_cast<zc*>(Q), ldn));
+    }
+
+    // T[0:m+a, m:m+a] = V[:, 0:m+a]^H W[:, m:m+a], mirrored into the rows.
+    void extend_T(int m, int a) {
+        const int ldn = (int)n;
+        gemm(rocblas_operation_conjugate_transpose, rocblas_operation_none, m + a, a, ldn, zc(1.0, 0.0), V(), ldn,
+             W() + (size_t)m * n, ldn, zc(0.0, 0.0), T() + (size_t)m * max_sub, max_sub);
+        hipLaunchKernelGGL(k_mirror_new, dim3((unsigned)((m + a + 255) / 256), (unsigned)a), dim3(256), 0, s, T(),
+                           max_sub, m, a);
+        DZ_HIP(hipGetLastError());
+    }
+
+    // Orthonormal start block, its H image and T.
+    void start(int m, const double* init, uint64_t init_cols, const double* diag_host) {
+        std::vector<double2> v((size_t)n * m);
+        if (o.init_diagonal) {
+            std::vector<size_t> order(n);
+            for (size_t i = 0; i < n; ++i) order[i] = i;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](size_t a, size_t b) { return diag_host[a] < diag_host[b]; });
+            std::fill(v.begin(), v.end(), make_double2(0.0, 0.0));
+            for (int c = 0; c < m; ++c) v[(size_t)c * n + order[c]] = make_double2(1.0, 0.0);
+        } else {
+            unsigned long long x = seed;
+            auto next = [&]() {
+                x = x * 6364136223846793005ULL + 1ULL;
+                return (double)(x >> 11) / 9007199254740992.0;
+            };
+            for (int c = 0; c < m; ++c)
+                for (size_t i = 0; i < n; ++i) {
+                    const double re = next() - 0.5;
+                    const double im = next() - 0.5;
+                    v[(size_t)c * n + i] = make_double2(re, im);
+                }
+        }
+        if (init && init_cols > 0) {
+            const size_t cols = std::min<uint64_t>(init_cols, (uint64_t)m);
+            std::memcpy(v.data(), init, cols * n * sizeof(double2));
+        }
+        DZ_HIP(hipMemcpyAsync(V(), v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+        cholqr(V(), m);
+        cholqr(V(), m);
+        apply(V(), W(), m);
+        extend_T(0, m);
+    }
+
+    // Eigen-decomposition of T[0:m, 0:m]; eigenvectors land in Tw, eigenvalues (host) returned.
+    std::vector<double> rayleigh_ritz(int m) {
+        hipLaunchKernelGGL(k_copy_block, dim3((unsigned)((m + 255) / 256), (unsigned)m), dim3(256), 0, s, T(),
+                           max_sub, w.Tw.as<double2>(), max_sub, m, m);
+        DZ_HIP(hipGetLastError());
+        DZ_BLAS(rocsolver_zheevd(w.blas, rocblas_evect_original, rocblas_fill_lower, m, w.Tw.as<zc>(), max_sub,
+                                 w.evals.as<double>(), w.ework.as<double>(), w.info.as<rocblas_int>()));
+        std::vector<double> ev(m);
+        rocblas_int info = 0;
+        DZ_HIP(hipMemcpyAsync(ev.data(), w.evals.ptr, m * sizeof(double), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipStreamSynchronize(s));
+        DZ_REQUIRE(info == 0, DZHIP_ERR_HIP, "zheevd did not converge");
+        return ev;
+    }
+
+    void ritz(int m) {
+        const int ldn = (int)n;
+        gemm(rocblas_operation_none, rocblas_operation_none, ldn, nb, m, zc(1.0, 0.0), V(), ldn, w.Tw.as<double2>(),
+             max_sub, zc(0.0, 0.0), w.X.as<double2>(), ldn);
+        gemm(rocblas_operation_none, rocblas_operation_none, ldn, nb, m, zc(1.0, 0.0), W(), ldn, w.Tw.as<double2>(),
+             max_sub, zc(0.0, 0.0), w.HX.as<double2>(), ldn);
+    }
+
+    // Basis <- [X, Qnew] (already orthonormal), W <- [HX, HQnew], T rebuilt.
+    void restart_with(int a) {
+        DZ_HIP(hipMemcpyAsync(V(), w.X.ptr, n * nb * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        DZ_HIP(hipMemcpyAsync(W(), w.HX.ptr, n * nb * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        if (a > 0) {
+            DZ_HIP(hipMemcpyAsync(V() + (size_t)nb * n, w.Q.ptr, n * a * sizeof(double2), hipMemcpyDeviceToDevice, s));
+            DZ_HIP(hipMemcpyAsync(W() + (size_t)nb * n, w.HQ.ptr, n * a * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        }
+        extend_T(0, nb + a);
+    }
+};
+
+static void ensure_work(Ham& h, int max_sub, int nb) {
+    auto* w = static_cast<EigWork*>(h.eig_work);
+    if (!w) {
+        w = new EigWork();
+        h.eig_work = w;
+        if (rocblas_create_handle(&w->blas) != rocblas_status_success) {
+            w->blas = nullptr;
+            throw Error{DZHIP_ERR_HIP, "rocblas_create_handle failed"};
+        }
+    }
+    DZ_BLAS(rocblas_set_stream(w->blas, h.stream()));
+    w->stream = h.stream();
+    const size_t n = h.npw;
+    const size_t c = sizeof(double2);
+    w->V.ensure(n * max_sub * c);
+    w->W.ensure(n * max_sub * c);
+    w->X.ensure(n * nb * c);
+    w->HX.ensure(n * nb * c);
+    w->R.ensure(n * nb * c);
+    w->Q.ensure(n * max_sub * c);
+    w->HQ.ensure(n * max_sub * c);
+    w->T.ensure((size_t)max_sub * max_sub * c);
+    w->Tw.ensure((size_t)max_sub * max_sub * c);
+    w->S.ensure((size_t)max_sub * max_sub * c);
+    w->G.ensure((size_t)max_sub * max_sub * c);
+    w->evals.ensure(max_sub * sizeof(double));
+    w->ework.ensure(max_sub * sizeof(double));
+    w->info.ensure(64);
+    w->norms.ensure(max_sub * sizeof(double));
+    w->lam.ensure(max_sub * sizeof(double));
+    w->ekin.ensure(max_sub * sizeof(double));
+    w->diag.ensure(n * sizeof(double));
+    w->sel.ensure(max_sub * sizeof(int));
+    w->scl.ensure(max_sub * sizeof(double));
+    w->idx.ensure(max_sub * sizeof(int));
+}
+
+void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts& o, const double* init,
+               uint64_t init_cols, double* values, double* vectors, dzhip_eig_stats* stats_out) {
+    const size_t n = h.npw;
+    DZ_REQUIRE(nbands > 0 && nbands <= n, DZHIP_ERR_ARG, "nbands must be in 1..n");
+    DZ_REQUIRE(o.precision == DZHIP_FP64 || o.precision == DZHIP_FP32, DZHIP_ERR_ARG, "bad precision");
+    const int nb = (int)nbands;
+    const int max_sub = (int)(o.max_subspace == 0 ? std::min<uint64_t>(n, 4 * nbands + 8)
+                                                   : std::min<uint64_t>(n, std::max<uint64_t>(o.max_subspace, nbands)));
+    const int block = (int)(o.block_size == 0 ? std::min<uint64_t>(max_sub, nbands + 4)
+                                              : std::min<uint64_t>(max_sub, std::max<uint64_t>(o.block_size, nbands)));
+    ensure_work(h, max_sub, nb);
+    EigWork& w = *static_cast<EigWork*>(h.eig_work);
+    Solver S{h, w, o, h.stream(), n, nb, max_sub, o.precision == DZHIP_FP64, {}, o.seed ? o.seed : 0x6f3d3cbeULL};
+    hipStream_t s = S.s;
+    DZ_HIP(hipMemcpyAsync(w.diag.ptr, diag, n * sizeof(double), hipMemcpyHostToDevice, s));
+    const double switch_tol = o.mixed_switch_tol > 0 ? o.mixed_switch_tol : 1e-5;
+
+    int m = block;
+    // Leave the FP32 apply: restart from the Ritz vectors with an FP64 image.
+    auto switch_to_fp64 = [&](uint64_t it) {
+        S.fp64_apply = true;
+        S.st.precision_switch_iter = it + 1;
+        DZ_HIP(hipMemcpyAsync(S.V(), w.X.ptr, n * nb * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        S.cholqr(S.V(), nb);
+        S.apply(S.V(), S.W(), nb);
+        S.extend_T(0, nb);
+        m = nb;
+    };
+    S.start(m, init, init_cols, diag);
+    std::vector<double> lam(nb), best(nb);
+    bool have_best = false;
+    uint64_t iter = 0;
+    for (; iter < o.max_iter; ++iter) {
+        std::vector<double> ev = S.rayleigh_ritz(m);
+        for (int b = 0; b < nb; ++b) lam[b] = ev[b];
+        best = lam;
+        have_best = true;
+        S.ritz(m);
+        DZ_HIP(hipMemcpyAsync(w.lam.ptr, lam.data(), nb * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_residual, S.cols_grid(nb), dim3(256), 0, s, w.HX.as<double2>(), w.X.as<double2>(),
+                           w.lam.as<double>(), w.R.as<double2>(), n, (int)n);
+        DZ_HIP(hipGetLastError());
+        std::vector<double> rn = S.col_norms(w.R.as<double2>(), nb);
+        const double max_res = *std::max_element(rn.begin(), rn.end());
+        S.st.max_residual = max_res;
+        if (max_res < o.tol && S.fp64_apply) {
+            S.st.converged = 1;
+            break;
+        }
+        if (!S.fp64_apply && (max_res < switch_tol || max_res < o.tol)) {
+            switch_to_fp64(iter);
+            continue;
+        }
+        // New directions for the unconverged bands, in band order (serial.zig:249-261).
+        const int room = std::min(nb, max_sub - nb);
+        std::vector<int> sel;
+        for (int b = 0; b < nb && (int)sel.size() < room; ++b)
+            if (rn[b] >= o.tol) sel.push_back(b);
+        int a = 0;
+        if (!sel.empty()) {
+            DZ_HIP(hipMemcpyAsync(w.sel.ptr, sel.data(), sel.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            if (o.preconditioner == DZHIP_PRECOND_TPA) {
+                hipLaunchKernelGGL(k_kinetic_energy, dim3(nb), dim3(256), 0, s, w.X.as<double2>(), w.diag.as<double>(),
+                                   n, (int)n, w.ekin.as<double>());
+                DZ_HIP(hipGetLastError());
+            }
+            hipLaunchKernelGGL(k_precondition, S.cols_grid((int)sel.size()), dim3(256), 0, s, w.R.as<double2>(),
+                               w.sel.as<int>(), w.lam.as<double>(), w.ekin.as<double>(), w.diag.as<double>(),
+                               w.Q.as<double2>(), n, (int)n, o.preconditioner == DZHIP_PRECOND_TPA ? 1 : 0);
+            DZ_HIP(hipGetLastError());
+            a = S.orthonormalize_new(w.Q.as<double2>(), (int)sel.size(), m);
+        }
+        if (a == 0) {
+            // Nothing could be added: every residual is individually below tol
+            // (serial.zig:262-273) or the new directions lie in the basis.
+            if (!S.fp64_apply) {
+                switch_to_fp64(iter);
+                continue;
+            }
+            S.st.converged = max_res < o.tol ? 1 : 0;
+            break;
+        }
+        S.apply(w.Q.as<double2>(), w.HQ.as<double2>(), a);
+        if (m + a > max_sub) {
+            // Thick restart from [Ritz vectors, new block]; their H images are W c and H Q.
+            S.restart_with(a);
+            m = nb + a;
+            S.st.restarts++;
+            continue;
+        }
+        DZ_HIP(hipMemcpyAsync(S.V() + (size_t)m * n, w.Q.ptr, n * a * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        DZ_HIP(hipMemcpyAsync(S.W() + (size_t)m * n, w.HQ.ptr, n * a * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        S.extend_T(m, a);
+        m += a;
+    }
+    S.st.iterations = iter;
+    if (have_best) std::memcpy(values, best.data(), nb * sizeof(double));
+    if (vectors) DZ_HIP(hipMemcpyAsync(vectors, w.X.ptr, n * nb * sizeof(double2), hipMemcpyDeviceToHost, s));
+    DZ_HIP(hipStreamSynchronize(s));
+    if (stats_out) *stats_out = S.st;
+}
+
+void eig_release(Ham& h) { eig_work_release(h); }
+
+}  // namespace dzhip

@@ -52,7 +52,8 @@ struct Ham {
 
     // Nonlocal: complex projector matrix A[p][G] = phi(G) e^{iG.tau_a} (P x n_pw), D tables.
     int nproj = 0;
-    DevBuf a64, a32, meta, dm64, dm32;
+    int nblocks = 0;  // atom blocks (first projector row of each atom in block_rows)
+    DevBuf a64, a32, meta, block_rows, dm64, dm32;
 
     // Work buffers.
     DevBuf xstage, ystage, x32, y32, tbuf, partial, qbuf;
@@ -66,9 +67,19 @@ struct Ham {
     uint64_t calls = 0;
     void collect_profile();
 
+    // Eigensolver workspace (created on first solve).
+    void* eig_work = nullptr;
+
     size_t device_bytes() const;
     std::mutex mu;
 };
+
+// rho[r] (+)= inv_volume * sum_b w_b |psi_b(r)|^2 (FP64), psi and rho on the device.
+void ham_density_dev(Ham& h, const double2* psi, size_t nbands, const double* wts_dev, double* rho_dev,
+                     bool accumulate);
+void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts& o, const double* init,
+               uint64_t init_cols, double* values, double* vectors, dzhip_eig_stats* stats);
+void eig_release(Ham& h);
 
 void ham_build(Ham& h, const dzhip_ham_desc& d);
 void ham_set_local(Ham& h, const double* local_r);

@@ -164,6 +164,71 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_fwd(
     }
 }
 
+// Density: one workgroup per (z, band group).  For each band of the group, the x-lines of the
+// z-plane are gathered from T (sphere planes after the z/y inverse transforms), inverse
+// transformed along x, and w_b |psi_b(r)|^2 is accumulated in FP64 registers; each thread owns
+// fixed grid points of the plane.  Groups are summed by k_density_sum (no atomics).
+constexpr int kDensThreads = 512;
+constexpr int kDensMaxPts = 32;  // nx*ny <= kDensThreads * kDensMaxPts
+
+__global__ __launch_bounds__(kDensThreads) void k_xline_density(
+    const double2* __restrict__ tbuf, const int* __restrict__ pxpos, int nplanes,
+    const double* __restrict__ wts, int nbands, int per_group, int nx, int ny, int nz, FftAxis ax,
+    const double2* __restrict__ tw, double inv_vol, double* __restrict__ partial) {
+    extern __shared__ unsigned char smem_raw[];
+    const int pitch = ny + 1;
+    double2* buf = reinterpret_cast<double2*>(smem_raw);
+    double2* twx = buf + nx * pitch;
+    const int z = blockIdx.x, grp = blockIdx.y;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int b0 = grp * per_group, b1 = min(nbands, b0 + per_group);
+    const int npts = nx * ny;
+    load_twiddles(twx, tw + ax.tw_off, nx, tid, nthr);
+    double acc[kDensMaxPts];
+#pragma unroll
+    for (int k = 0; k < kDensMaxPts; ++k) acc[k] = 0.0;
+    const unsigned my = div_magic(ny), mx = div_magic(nx);
+    const LineSet xl{ny, pitch, 1, nullptr};
+    for (int b = b0; b < b1; ++b) {
+        const double wb = wts[b];
+        if (wb == 0.0) continue;  // same value for the whole workgroup
+        for (int i = tid; i < nx * pitch; i += nthr) buf[i] = make_double2(0.0, 0.0);
+        __syncthreads();
+        const double2* tcol = tbuf + (size_t)b * nplanes * ny * nz;
+        for (int i = tid; i < nplanes * ny; i += nthr) {
+            const int p = fdiv(i, ny, my), y = i - p * ny;
+            buf[y + pitch * pxpos[p]] = tcol[((size_t)p * nz + z) * ny + y];
+        }
+        __syncthreads();
+        lds_fft_lines<double, double2, true>(buf, ax, twx, xl, 1.0, tid, nthr);
+#pragma unroll
+        for (int k = 0; k < kDensMaxPts; ++k) {
+            const int i = tid + k * nthr;
+            if (i < npts) {
+                const int y = fdiv(i, nx, mx), x = i - y * nx;
+                const double2 e = buf[y + pitch * x];
+                acc[k] += wb * (e.x * e.x + e.y * e.y);
+            }
+        }
+        __syncthreads();
+    }
+    double* out = partial + (size_t)grp * nx * ny * nz + (size_t)nx * ny * z;
+#pragma unroll
+    for (int k = 0; k < kDensMaxPts; ++k) {
+        const int i = tid + k * nthr;
+        if (i < npts) out[i] = acc[k] * inv_vol;
+    }
+}
+
+__global__ void k_density_sum(const double* __restrict__ partial, int ngroups, size_t n,
+                              double* __restrict__ rho, int accumulate) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = accumulate ? rho[i] : 0.0;
+    for (int g = 0; g < ngroups; ++g) s += partial[(size_t)g * n + i];
+    rho[i] = s;
+}
+
 // ----------------------------------------------------------------- nonlocal
 __global__ void k_build_a(double2* __restrict__ a, const double* __restrict__ phi_all,
                           const int* __restrict__ row_of, const int* __restrict__ atom_of,
@@ -183,111 +248,308 @@ __global__ void k_build_a(double2* __restrict__ a, const double* __restrict__ ph
     a[i] = make_double2(ph * c, ph * s);
 }
 
-constexpr int kProjTileP = 16, kProjTileB = 16, kProjTileK = 64;
+// MFMA 16x16x4 (f32: v_mfma_f32_16x16x4f32, f64: v_mfma_f64_16x16x4f64).  Per wavefront:
+// A operand lane l holds A[i = l%16][k = l/16], B operand B[k = l/16][j = l%16].
+// Accumulator element r of lane l is D[row(l/16, r)][j = l%16] with
+//   f32: row = 4*(l/16) + r,   f64: row = (l/16) + 4*r
+// (both measured on gfx950 with a one-wavefront probe of the builtins).
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+template <typename T> struct Mfma;
+template <> struct Mfma<float> {
+    using acc = v4f;
+    static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int k4, int r) { return 4 * k4 + r; }
+};
+template <> struct Mfma<double> {
+    using acc = v4d;
+    static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int k4, int r) { return k4 + 4 * r; }
+};
+
+// Workgroup tiles of the nonlocal GEMMs: 5 wavefronts, each owning 32 bands, so one
+// workgroup covers a 160-band block and each projector row is read once per K-chunk.
+constexpr int kNlWaves = 5;
+constexpr int kNlThreads = 64 * kNlWaves;
+constexpr int kNlBands = 32 * kNlWaves;  // bands per workgroup tile
+constexpr int kNlK = 32;                 // K (G or p) per LDS stage
+constexpr int kNlPitch = kNlK + 1;       // odd pitches (in complex elements) against bank conflicts
+// 16-wide MFMA blocks per wavefront along the projector (projection) and G (back-projection)
+// axes: 4 in FP32; 2 in FP64, whose accumulators are twice as wide (keeps them in registers).
+template <typename T> struct NlTile { static constexpr int kBlocks = 4; };
+template <> struct NlTile<double> { static constexpr int kBlocks = 2; };
+template <typename T> constexpr int nl_p_tile() { return 16 * NlTile<T>::kBlocks; }  // projectors / tile
+template <typename T> constexpr int nl_g_tile() { return 16 * NlTile<T>::kBlocks; }  // G / tile
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_nl_proj(const typename C2T<T>::type* __restrict__ a,
-                                                 const typename C2T<T>::type* __restrict__ x,
-                                                 size_t ldx, typename C2T<T>::type* __restrict__ partial,
-                                                 int P, int npw, int ncols, int kchunk) {
+__host__ __device__ inline size_t nl_proj_lds_bytes() {
+    return (size_t)(kNlBands + nl_p_tile<T>()) * kNlPitch * sizeof(typename C2T<T>::type);
+}
+template <typename T>
+__host__ __device__ inline size_t nl_back_lds_bytes() {
+    return ((size_t)kNlBands * kNlPitch + (size_t)kNlK * (nl_g_tile<T>() + 1)) * sizeof(typename C2T<T>::type);
+}
+
+// Block index -> logical tile so that consecutive logical tiles (which share operands)
+// land on the same XCD: hardware dispatches block i to XCD i % 8.
+__device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
+    constexpr int kXcd = 8;
+    const int per = nblocks / kXcd;
+    if (per == 0 || bid >= per * kXcd) return bid;
+    return (bid % kXcd) * per + bid / kXcd;
+}
+
+// partial[ks][b][p] = sum_{G in chunk ks} X[b][G] * A[p][G]   (= <beta_p|psi_b>, no 1/Omega yet)
+template <typename T>
+__global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::type* __restrict__ a,
+                                                        const typename C2T<T>::type* __restrict__ x,
+                                                        size_t ldx, typename C2T<T>::type* __restrict__ partial,
+                                                        int P, int npw, int ncols, int kchunk, int ptiles) {
     using C = typename C2T<T>::type;
-    __shared__ C as[kProjTileP][kProjTileK + 1];
-    __shared__ C xs[kProjTileB][kProjTileK + 1];
-    const int p0 = blockIdx.x * kProjTileP, b0 = blockIdx.y * kProjTileB, ks = blockIdx.z;
+    using M = Mfma<T>;
+    extern __shared__ unsigned char smem_raw[];
+    C* xs = reinterpret_cast<C*>(smem_raw);  // [kNlBands][kNlPitch]
+    constexpr int kNlP = nl_p_tile<T>();
+    constexpr int PB = NlTile<T>::kBlocks;
+    C* as = xs + kNlBands * kNlPitch;        // [kNlP][kNlPitch]
+    const int lin = xcd_tile(blockIdx.x, gridDim.x);
+    const int pt = lin % ptiles, ks = lin / ptiles;
+    const int p0 = pt * kNlP;
+    const int b0 = blockIdx.z * kNlBands;
     const int k0 = ks * kchunk;
     const int k1 = min(npw, k0 + kchunk);
-    const int tid = threadIdx.x;
-    const int tp = tid & 15, tb = tid >> 4;
-    C acc = C{T(0), T(0)};
-    for (int kt = k0; kt < k1; kt += kProjTileK) {
-        for (int e = tid; e < kProjTileP * kProjTileK; e += 256) {
-            const int r = e / kProjTileK, kk = e - r * kProjTileK, k = kt + kk;
-            as[r][kk] = (p0 + r < P && k < k1) ? a[(size_t)(p0 + r) * npw + k] : C{T(0), T(0)};
-            xs[r][kk] = (b0 + r < ncols && k < k1) ? x[(size_t)(b0 + r) * ldx + k] : C{T(0), T(0)};
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int i16 = lane & 15, k4 = lane >> 4;
+    const C zero = C{T(0), T(0)};
+    constexpr int kXPer = (kNlBands * kNlK + kNlThreads - 1) / kNlThreads;
+    constexpr int kAPer = (kNlP * kNlK + kNlThreads - 1) / kNlThreads;
+    T xre[kXPer], xim[kXPer], are[kAPer], aim[kAPer];
+#define DZ_PROJ_FETCH(KT)                                                                     \
+    _Pragma("unroll") for (int i = 0; i < kXPer; ++i) {                                      \
+        const int e = tid + i * kNlThreads;                                                   \
+        const int r = e >> 5, c = e & 31, g = (KT) + c, b = b0 + r;                           \
+        C v = zero;                                                                           \
+        if (r < kNlBands && b < ncols && g < k1) v = x[(size_t)b * ldx + g];                  \
+        xre[i] = v.x;                                                                         \
+        xim[i] = v.y;                                                                         \
+    }                                                                                         \
+    _Pragma("unroll") for (int i = 0; i < kAPer; ++i) {                                      \
+        const int e = tid + i * kNlThreads;                                                   \
+        const int r = e >> 5, c = e & 31, g = (KT) + c, p = p0 + r;                           \
+        C v = zero;                                                                           \
+        if (r < kNlP && p < P && g < k1) v = a[(size_t)p * npw + g];                          \
+        are[i] = v.x;                                                                         \
+        aim[i] = v.y;                                                                         \
+    }
+#define DZ_PROJ_STASH()                                                                       \
+    _Pragma("unroll") for (int i = 0; i < kXPer; ++i) {                                      \
+        const int e = tid + i * kNlThreads;                                                   \
+        if (e < kNlBands * kNlK) xs[(e >> 5) * kNlPitch + (e & 31)] = C{xre[i], xim[i]};     \
+    }                                                                                         \
+    _Pragma("unroll") for (int i = 0; i < kAPer; ++i) {                                      \
+        const int e = tid + i * kNlThreads;                                                   \
+        if (e < kNlP * kNlK) as[(e >> 5) * kNlPitch + (e & 31)] = C{are[i], aim[i]};         \
+    }
+    typename M::acc accr[2][PB], acci[2][PB];
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int pb = 0; pb < PB; ++pb) {
+            accr[bb][pb] = typename M::acc{0, 0, 0, 0};
+            acci[bb][pb] = typename M::acc{0, 0, 0, 0};
+        }
+    if (k0 < k1) {
+        DZ_PROJ_FETCH(k0)
+        DZ_PROJ_STASH()
+    }
+    __syncthreads();
+    for (int kt = k0; kt < k1; kt += kNlK) {
+        const bool more = kt + kNlK < k1;
+        if (more) {
+            DZ_PROJ_FETCH(kt + kNlK)
+        }
+#pragma unroll
+        for (int kk = 0; kk < kNlK; kk += 4) {
+            C xa[2], ab[PB];
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) xa[bb] = xs[(wave * 32 + bb * 16 + i16) * kNlPitch + kk + k4];
+#pragma unroll
+            for (int pb = 0; pb < PB; ++pb) ab[pb] = as[(pb * 16 + i16) * kNlPitch + kk + k4];
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+                for (int pb = 0; pb < PB; ++pb) {
+                    // (xr + i xi)(ar + i ai)
+                    accr[bb][pb] = M::mma(xa[bb].x, ab[pb].x, accr[bb][pb]);
+                    accr[bb][pb] = M::mma(-xa[bb].y, ab[pb].y, accr[bb][pb]);
+                    acci[bb][pb] = M::mma(xa[bb].x, ab[pb].y, acci[bb][pb]);
+                    acci[bb][pb] = M::mma(xa[bb].y, ab[pb].x, acci[bb][pb]);
+                }
         }
         __syncthreads();
-#pragma unroll 8
-        for (int kk = 0; kk < kProjTileK; ++kk) {
-            const C av = as[tp][kk], xv = xs[tb][kk];
-            acc.x += av.x * xv.x - av.y * xv.y;
-            acc.y += av.x * xv.y + av.y * xv.x;
+        if (more) {
+            DZ_PROJ_STASH()
         }
         __syncthreads();
     }
-    if (p0 + tp < P && b0 + tb < ncols) partial[((size_t)ks * ncols + b0 + tb) * P + p0 + tp] = acc;
+#undef DZ_PROJ_FETCH
+#undef DZ_PROJ_STASH
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int pb = 0; pb < PB; ++pb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int b = b0 + wave * 32 + bb * 16 + M::row(k4, r);
+                const int p = p0 + pb * 16 + i16;
+                if (b < ncols && p < P)
+                    partial[((size_t)ks * ncols + b) * P + p] = C{accr[bb][pb][r], acci[bb][pb][r]};
+            }
 }
 
+// One workgroup per band: sum the split-K partials of every projector into LDS (each partial
+// read once, coalesced over p), then q[b][p] = inv_vol * sum_r D[m][r] * sum[base + r] within
+// the projector's atom block.
 template <typename T>
-__global__ void k_nl_dij(const typename C2T<T>::type* __restrict__ partial, int nks,
-                         typename C2T<T>::type* __restrict__ q, const ProjMeta* __restrict__ meta,
-                         const T* __restrict__ dm, int P, int ncols, T inv_vol) {
+__global__ __launch_bounds__(256) void k_nl_dij(const typename C2T<T>::type* __restrict__ partial,
+                                                int nks, typename C2T<T>::type* __restrict__ q,
+                                                const ProjMeta* __restrict__ meta,
+                                                const T* __restrict__ dm, int P, int ncols, T inv_vol) {
     using C = typename C2T<T>::type;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P * ncols) return;
-    const int b = i / P, p = i - b * P;
-    const ProjMeta mt = meta[p];
-    C acc = C{T(0), T(0)};
-    for (int r = 0; r < mt.mt; ++r) {
-        const T d = dm[mt.dm_off + mt.m * mt.mt + r];
-        if (d == T(0)) continue;
+    extern __shared__ unsigned char smem_raw[];
+    C* sum = reinterpret_cast<C*>(smem_raw);  // [P]
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int p = tid; p < P; p += nthr) {
         C s = C{T(0), T(0)};
         for (int ks = 0; ks < nks; ++ks) {
-            const C v = partial[((size_t)ks * ncols + b) * P + mt.base + r];
+            const C v = partial[((size_t)ks * ncols + b) * P + p];
             s.x += v.x;
             s.y += v.y;
         }
-        acc.x += d * s.x;
-        acc.y += d * s.y;
+        sum[p] = s;
     }
-    q[(size_t)b * P + p] = C{acc.x * inv_vol, acc.y * inv_vol};
+    __syncthreads();
+    for (int p = tid; p < P; p += nthr) {
+        const ProjMeta mt = meta[p];
+        const T* drow = dm + mt.dm_off + mt.m * mt.mt;
+        C acc = C{T(0), T(0)};
+        for (int r = 0; r < mt.mt; ++r) {
+            const C v = sum[mt.base + r];
+            acc.x += drow[r] * v.x;
+            acc.y += drow[r] * v.y;
+        }
+        q[(size_t)b * P + p] = C{acc.x * inv_vol, acc.y * inv_vol};
+    }
 }
 
+// y[b][G] += sum_p q[b][p] * conj(A[p][G])
 template <typename T>
-__global__ __launch_bounds__(256) void k_nl_back(const typename C2T<T>::type* __restrict__ a,
-                                                 const typename C2T<T>::type* __restrict__ q,
-                                                 typename C2T<T>::type* __restrict__ y, size_t ldy,
-                                                 int P, int npw, int ncols) {
+__global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::type* __restrict__ a,
+                                                        const typename C2T<T>::type* __restrict__ q,
+                                                        typename C2T<T>::type* __restrict__ y, size_t ldy,
+                                                        int P, int npw, int ncols) {
     using C = typename C2T<T>::type;
-    __shared__ C qs[16][33];
-    const int tid = threadIdx.x;
-    const int g = blockIdx.x * 64 + (tid & 63);
-    const int bq = tid >> 6;
-    const int b0 = blockIdx.y * 16;
-    C acc[4];
+    using M = Mfma<T>;
+    extern __shared__ unsigned char smem_raw[];
+    C* qs = reinterpret_cast<C*>(smem_raw);  // [kNlBands][kNlPitch]  (band, p)
+    constexpr int kNlG = nl_g_tile<T>();
+    constexpr int kNlGPitch = kNlG + 1;
+    constexpr int GB = NlTile<T>::kBlocks;
+    C* as = qs + kNlBands * kNlPitch;        // [kNlK][kNlGPitch]     (p, G)
+    const int g0 = blockIdx.x * kNlG;
+    const int b0 = blockIdx.z * kNlBands;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int i16 = lane & 15, k4 = lane >> 4;
+    const C zero = C{T(0), T(0)};
+    constexpr int kQPer = (kNlBands * kNlK + kNlThreads - 1) / kNlThreads;
+    constexpr int kAPer = (kNlK * kNlG + kNlThreads - 1) / kNlThreads;
+    T qre[kQPer], qim[kQPer], are[kAPer], aim[kAPer];
+#define DZ_BACK_FETCH(PT)                                                                     \
+    _Pragma("unroll") for (int i = 0; i < kQPer; ++i) {                                      \
+        const int e = tid + i * kNlThreads;                                                   \
+        const int r = e >> 5, c = e & 31, b = b0 + r, p = (PT) + c;                           \
+        C v = zero;                                                                           \
+        if (r < kNlBands && b < ncols && p < P) v = q[(size_t)b * P + p];                     \
+        qre[i] = v.x;                                                                         \
+        qim[i] = v.y;                                                                         \
+    }                                                                                         \
+    _Pragma("unroll") for (int i = 0; i < kAPer; ++i) {                                      \
+        const int e = tid + i * kNlThreads;                                                   \
+        const int r = e / kNlG, c = e % kNlG, p = (PT) + r, g = g0 + c;                       \
+        C v = zero;                                                                           \
+        if (r < kNlK && p < P && g < npw) v = a[(size_t)p * npw + g];                         \
+        are[i] = v.x;                                                                         \
+        aim[i] = v.y;                                                                         \
+    }
+#define DZ_BACK_STASH()                                                                       \
+    _Pragma("unroll") for (int i = 0; i < kQPer; ++i) {                                      \
+        const int e = tid + i * kNlThreads;                                                   \
+        if (e < kNlBands * kNlK) qs[(e >> 5) * kNlPitch + (e & 31)] = C{qre[i], qim[i]};     \
+    }                                                                                         \
+    _Pragma("unroll") for (int i = 0; i < kAPer; ++i) {                                      \
+        const int e = tid + i * kNlThreads;                                                   \
+        if (e < kNlK * kNlG) as[(e / kNlG) * kNlGPitch + (e % kNlG)] = C{are[i], aim[i]};   \
+    }
+    typename M::acc accr[2][GB], acci[2][GB];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] = C{T(0), T(0)};
-    for (int pt = 0; pt < P; pt += 32) {
-        for (int e = tid; e < 16 * 32; e += 256) {
-            const int bb = e >> 5, pp = e & 31;
-            qs[bb][pp] = (b0 + bb < ncols && pt + pp < P) ? q[(size_t)(b0 + bb) * P + pt + pp]
-                                                           : C{T(0), T(0)};
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int gb = 0; gb < GB; ++gb) {
+            accr[bb][gb] = typename M::acc{0, 0, 0, 0};
+            acci[bb][gb] = typename M::acc{0, 0, 0, 0};
+        }
+    DZ_BACK_FETCH(0)
+    DZ_BACK_STASH()
+    __syncthreads();
+    for (int pt = 0; pt < P; pt += kNlK) {
+        const bool more = pt + kNlK < P;
+        if (more) {
+            DZ_BACK_FETCH(pt + kNlK)
+        }
+#pragma unroll
+        for (int kk = 0; kk < kNlK; kk += 4) {
+            C qa[2], ab[GB];
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) qa[bb] = qs[(wave * 32 + bb * 16 + i16) * kNlPitch + kk + k4];
+#pragma unroll
+            for (int gb = 0; gb < GB; ++gb) ab[gb] = as[(kk + k4) * kNlGPitch + gb * 16 + i16];
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+                for (int gb = 0; gb < GB; ++gb) {
+                    // (qr + i qi)(ar - i ai)
+                    accr[bb][gb] = M::mma(qa[bb].x, ab[gb].x, accr[bb][gb]);
+                    accr[bb][gb] = M::mma(qa[bb].y, ab[gb].y, accr[bb][gb]);
+                    acci[bb][gb] = M::mma(qa[bb].y, ab[gb].x, acci[bb][gb]);
+                    acci[bb][gb] = M::mma(-qa[bb].x, ab[gb].y, acci[bb][gb]);
+                }
         }
         __syncthreads();
-        if (g < npw) {
-            const int pe = min(32, P - pt);
-            for (int pp = 0; pp < pe; ++pp) {
-                const C av = a[(size_t)(pt + pp) * npw + g];
+        if (more) {
+            DZ_BACK_STASH()
+        }
+        __syncthreads();
+    }
+#undef DZ_BACK_FETCH
+#undef DZ_BACK_STASH
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const C qv = qs[bq + 4 * i][pp];
-                    // conj(a) * q
-                    acc[i].x += av.x * qv.x + av.y * qv.y;
-                    acc[i].y += av.x * qv.y - av.y * qv.x;
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int gb = 0; gb < GB; ++gb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int b = b0 + wave * 32 + bb * 16 + M::row(k4, r);
+                const int g = g0 + gb * 16 + i16;
+                if (b < ncols && g < npw) {
+                    C& o = y[(size_t)b * ldy + g];
+                    o = C{o.x + accr[bb][gb][r], o.y + acci[bb][gb][r]};
                 }
             }
-        }
-        __syncthreads();
-    }
-    if (g < npw) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int b = b0 + bq + 4 * i;
-            if (b < ncols) {
-                C& o = y[(size_t)b * ldy + g];
-                o = C{o.x + acc[i].x, o.y + acc[i].y};
-            }
-        }
-    }
 }
 
 template <typename T>
@@ -306,7 +568,7 @@ __global__ void k_d2f(const double* __restrict__ in, float* __restrict__ out, si
 size_t Ham::device_bytes() const {
     const DevBuf* bufs[] = {&plane_start, &plane_pw, &plane_pos, &plane_xpos, &plane_ystart,
                             &plane_ycols, &kinetic64,
-                            &kinetic32, &vloc64, &vloc32, &a64, &a32, &meta, &dm64, &dm32,
+                            &kinetic32, &vloc64, &vloc32, &a64, &a32, &meta, &block_rows, &dm64, &dm32,
                             &xstage, &ystage, &x32, &y32, &tbuf, &partial, &qbuf};
     size_t s = tab.bytes;
     for (const DevBuf* b : bufs) s += b->bytes;
@@ -415,6 +677,7 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
                 mt += 2 * sd.l_list[b] + 1;
             }
             DZ_REQUIRE(mt == sd.m_total, DZHIP_ERR_ARG, "m_total != sum(2l+1)");
+
             DZ_REQUIRE(mt == 0 || sd.phi, DZHIP_ERR_ARG, "species phi table missing");
             DZ_REQUIRE(sd.n_beta == 0 || sd.dij || sd.dij_m, DZHIP_ERR_ARG, "species D missing");
             sp_row0[s] = (int)rows;
@@ -457,6 +720,13 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
             }
         }
         h.nproj = (int)row_of.size();
+        DZ_REQUIRE((size_t)h.nproj * sizeof(double2) <= kMaxLdsBytes, DZHIP_ERR_ARG,
+                   "more than 10240 projectors per k-point is not supported");
+        std::vector<int> block_rows;
+        for (int p = 0; p < h.nproj; ++p)
+            if (meta[p].m == 0) block_rows.push_back(p);
+        h.nblocks = (int)block_rows.size();
+        upload(h.block_rows, block_rows);
         if (h.nproj > 0) {
             DevBuf dphi, drow, datom, dg, dpos;
             upload(dphi, phi_all);
@@ -499,9 +769,10 @@ void ham_set_local(Ham& h, const double* local_r) {
 }
 
 void ham_release(Ham& h) {
+    eig_release(h);
     DevBuf* bufs[] = {&h.plane_start, &h.plane_pw, &h.plane_pos, &h.plane_xpos, &h.plane_ystart,
                       &h.plane_ycols, &h.kinetic64,
-                      &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.dm64,
+                      &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.block_rows, &h.dm64,
                       &h.dm32, &h.xstage, &h.ystage, &h.x32, &h.y32, &h.tbuf, &h.partial, &h.qbuf};
     for (DevBuf* b : bufs) b->release();
     h.tab.release();
@@ -577,23 +848,28 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
         hipEvent_t* evs = next_event_set(h);
         mark(h, evs, 0);
         // Nonlocal projection first (reads x only).
-        int nks = 1;
         if (do_nl) {
-            const int gp = (h.nproj + kProjTileP - 1) / kProjTileP;
-            const int gb = (int)((nc + kProjTileB - 1) / kProjTileB);
-            const int ktiles = (int)((h.npw + kProjTileK - 1) / kProjTileK);
-            nks = std::max(1, std::min(ktiles, (2048 + gp * gb - 1) / (gp * gb)));
-            const int kchunk = (int)round_up((h.npw + nks - 1) / nks, kProjTileK);
+            constexpr int kNlP = nl_p_tile<T>();
+            const int ptiles = (h.nproj + kNlP - 1) / kNlP;
+            const int btiles = (int)((nc + kNlBands - 1) / kNlBands);
+            const int ktiles = (int)((h.npw + kNlK - 1) / kNlK);
+            // Split K so that about one workgroup per CU runs; each K-chunk a multiple of kNlK.
+            int nks = std::max(1, std::min(ktiles, (256 + ptiles * btiles - 1) / (ptiles * btiles)));
+            const int kchunk = (int)round_up((h.npw + nks - 1) / nks, kNlK);
             nks = (int)((h.npw + kchunk - 1) / kchunk);
             h.partial.ensure((size_t)nks * nc * h.nproj * sizeof(C));
             h.qbuf.ensure(nc * h.nproj * sizeof(C));
-            hipLaunchKernelGGL(k_nl_proj<T>, dim3(gp, gb, nks), dim3(256), 0, s, Sel<T>::a(h), xc, ld,
-                               h.partial.as<C>(), h.nproj, (int)h.npw, (int)nc, kchunk);
+            const size_t lds = nl_proj_lds_bytes<T>();
+            set_max_dynamic_lds((const void*)k_nl_proj<T>, lds);
+            hipLaunchKernelGGL(k_nl_proj<T>, dim3(ptiles * nks, 1, btiles), dim3(kNlThreads), lds, s,
+                               Sel<T>::a(h), xc, ld, h.partial.as<C>(), h.nproj, (int)h.npw, (int)nc,
+                               kchunk, ptiles);
             DZ_HIP(hipGetLastError());
-            const int tot = (int)(nc * h.nproj);
-            hipLaunchKernelGGL(k_nl_dij<T>, dim3((tot + 255) / 256), dim3(256), 0, s,
-                               h.partial.as<C>(), nks, h.qbuf.as<C>(), h.meta.as<ProjMeta>(),
-                               Sel<T>::dm(h), h.nproj, (int)nc, (T)h.inv_volume);
+            const size_t lds_d = (size_t)h.nproj * sizeof(C);
+            set_max_dynamic_lds((const void*)k_nl_dij<T>, lds_d);
+            hipLaunchKernelGGL(k_nl_dij<T>, dim3((unsigned)nc), dim3(256), lds_d, s, h.partial.as<C>(), nks,
+                               h.qbuf.as<C>(), h.meta.as<ProjMeta>(), Sel<T>::dm(h), h.nproj, (int)nc,
+                               (T)h.inv_volume);
             DZ_HIP(hipGetLastError());
         }
         mark(h, evs, 1);
@@ -634,8 +910,12 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
             mark(h, evs, 4);
         }
         if (do_nl) {
-            hipLaunchKernelGGL(k_nl_back<T>, dim3((unsigned)((h.npw + 63) / 64), (unsigned)((nc + 15) / 16)),
-                               dim3(256), 0, s, Sel<T>::a(h), h.qbuf.as<C>(), yc, ld, h.nproj,
+            const size_t lds = nl_back_lds_bytes<T>();
+            constexpr int kNlG = nl_g_tile<T>();
+            set_max_dynamic_lds((const void*)k_nl_back<T>, lds);
+            hipLaunchKernelGGL(k_nl_back<T>, dim3((unsigned)((h.npw + kNlG - 1) / kNlG), 1,
+                                                  (unsigned)((nc + kNlBands - 1) / kNlBands)),
+                               dim3(kNlThreads), lds, s, Sel<T>::a(h), h.qbuf.as<C>(), yc, ld, h.nproj,
                                (int)h.npw, (int)nc);
             DZ_HIP(hipGetLastError());
         }
@@ -644,6 +924,41 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
         if (h.profile && h.ev_used >= 4096) h.collect_profile();
     }
     if (h.profile) h.calls++;
+}
+
+void ham_density_dev(Ham& h, const double2* psi, size_t nbands, const double* wts_dev, double* rho_dev,
+                     bool accumulate) {
+    hipStream_t s = h.stream();
+    DZ_REQUIRE((size_t)h.nx * h.ny <= (size_t)kDensThreads * kDensMaxPts, DZHIP_ERR_ARG,
+               "x*y plane too large for the density kernel");
+    const size_t plane = (size_t)h.ny * h.nz;
+    const size_t lds_p = plane_lds_bytes<double2>(h.ny, h.nz);
+    const size_t lds_x = xline_lds_bytes<double2>(h.nx, h.ny);
+    set_max_dynamic_lds((const void*)k_plane_inv<double>, lds_p);
+    set_max_dynamic_lds((const void*)k_xline_density, lds_x);
+    // Band groups: enough workgroups to fill the device, deterministic sum afterwards.
+    const int ngroups = (int)std::max<size_t>(1, std::min<size_t>(nbands, (512 + h.nz - 1) / h.nz));
+    const int per_group = (int)((nbands + ngroups - 1) / ngroups);
+    h.partial.ensure((size_t)ngroups * h.ngrid * sizeof(double));
+    for (size_t c0 = 0; c0 < nbands; c0 += kMaxColsPerPass) {
+        const size_t nc = std::min(kMaxColsPerPass, nbands - c0);
+        h.tbuf.ensure(nc * h.nplanes * plane * sizeof(double2));
+        hipLaunchKernelGGL(k_plane_inv<double>, dim3(h.nplanes, (unsigned)nc), dim3(TileThreads<double>::value),
+                           lds_p, s, psi + c0 * h.npw, h.npw, h.tbuf.as<double2>(), h.plane_start.as<int>(),
+                           h.plane_pw.as<int>(), h.plane_pos.as<int>(), h.plane_ystart.as<int>(),
+                           h.plane_ycols.as<short>(), h.ny, h.nz, h.nplanes, h.tab.ax[1], h.tab.ax[2], h.tab.tw64);
+        DZ_HIP(hipGetLastError());
+        const int ng = (int)std::max<size_t>(1, std::min<size_t>(nc, (size_t)ngroups));
+        const int pg = (int)((nc + ng - 1) / ng);
+        hipLaunchKernelGGL(k_xline_density, dim3(h.nz, ng), dim3(kDensThreads), lds_x, s, h.tbuf.as<double2>(),
+                           h.plane_xpos.as<int>(), h.nplanes, wts_dev + c0, (int)nc, pg, h.nx, h.ny, h.nz,
+                           h.tab.ax[0], h.tab.tw64, h.inv_volume, h.partial.as<double>());
+        DZ_HIP(hipGetLastError());
+        const bool acc = accumulate || c0 > 0;
+        hipLaunchKernelGGL(k_density_sum, dim3((unsigned)((h.ngrid + 255) / 256)), dim3(256), 0, s,
+                           h.partial.as<double>(), ng, h.ngrid, rho_dev, acc ? 1 : 0);
+        DZ_HIP(hipGetLastError());
+    }
 }
 
 template void ham_apply_dev<double>(Ham&, const void*, void*, size_t, bool, bool);

@@ -32,7 +32,7 @@ EXPORTS = (
     "dzhip_ham_create", "dzhip_ham_destroy", "dzhip_ham_set_local", "dzhip_ham_apply",
     "dzhip_ham_apply_dev", "dzhip_ham_apply_nonlocal", "dzhip_ham_apply_overlap",
     "dzhip_ham_set_stream", "dzhip_ham_sync", "dzhip_ham_profile", "dzhip_ham_phase_times",
-    "dzhip_ham_info",
+    "dzhip_ham_info", "dzhip_eig_solve", "dzhip_density_accumulate",
 )
 
 
@@ -61,6 +61,28 @@ class BadArgument(DzhipError):
 
 
 _ERRS = {-1: InvalidMatrixSize, -2: HipFailure, -3: OutOfMemory, -4: BadState, -5: BadArgument}
+
+PRECOND_KINETIC = 0
+PRECOND_TPA = 1
+
+
+class EigOpts(ct.Structure):
+    _fields_ = [
+        ("max_iter", ct.c_uint64), ("tol", ct.c_double), ("max_subspace", ct.c_uint64),
+        ("block_size", ct.c_uint64), ("init_diagonal", ct.c_int), ("precision", ct.c_int),
+        ("mixed_switch_tol", ct.c_double), ("preconditioner", ct.c_int), ("seed", ct.c_uint64),
+    ]
+
+
+class EigStats(ct.Structure):
+    _fields_ = [
+        ("iterations", ct.c_uint64), ("apply_calls", ct.c_uint64), ("fp32_applies", ct.c_uint64),
+        ("fp64_applies", ct.c_uint64), ("restarts", ct.c_uint64), ("precision_switch_iter", ct.c_uint64),
+        ("converged", ct.c_uint64), ("max_residual", ct.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class SpeciesDesc(ct.Structure):
@@ -109,6 +131,8 @@ def _load():
         "dzhip_ham_profile": ([vp, i32], i32),
         "dzhip_ham_phase_times": ([vp, dp, u64, ct.POINTER(ct.c_uint64)], i32),
         "dzhip_ham_info": ([vp, ct.POINTER(ct.c_uint64), u64], i32),
+        "dzhip_eig_solve": ([vp, dp, u64, u64, ct.POINTER(EigOpts), dp, u64, dp, dp, ct.POINTER(EigStats)], i32),
+        "dzhip_density_accumulate": ([vp, dp, u64, u64, dp, dp, u64, i32], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -222,6 +246,7 @@ class Hamiltonian:
 
         fidx = arr(fft_index, np.int32)
         kin = arr(kinetic, np.float64)
+        self._kinetic = kin
         gc = arr(gcart, np.float64).reshape(-1)
         sp_arr = (SpeciesDesc * max(1, len(species)))()
         for i, sp in enumerate(species):
@@ -322,6 +347,46 @@ class Hamiltonian:
         calls = ct.c_uint64()
         check(L().dzhip_ham_phase_times(self.handle, out, len(PHASES), ct.byref(calls)))
         return dict(zip(PHASES, list(out))), int(calls.value)
+
+    def eig_solve(self, nbands: int, diag=None, tol: float = 1e-6, max_iter: int = 40, precision: int = FP64,
+                  preconditioner: int = PRECOND_KINETIC, init=None, max_subspace: int = 0, block_size: int = 0,
+                  init_diagonal: bool = False, mixed_switch_tol: float = 0.0, seed: int = 0,
+                  want_vectors: bool = True):
+        """Lowest eigenpairs (hermitian_eigen_decomp_iterative counterpart).  diag defaults to the
+        kinetic energies the context was built with; init is an (init_cols, n) block of start vectors.
+        Returns (values[nbands], vectors (nbands, n) or None, stats dict)."""
+        if diag is None:
+            diag = self._kinetic
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        o = EigOpts(max_iter, tol, max_subspace, block_size, 1 if init_diagonal else 0, precision,
+                    mixed_switch_tol, preconditioner, seed)
+        vals = np.zeros(nbands)
+        vecs = np.empty((nbands, self.n), dtype=np.complex128) if want_vectors else None
+        st = EigStats()
+        if init is not None:
+            init = np.ascontiguousarray(init, dtype=np.complex128)
+            ip, ic = _dp(init.view(np.float64)), init.shape[0]
+        else:
+            ip, ic = ct.POINTER(ct.c_double)(), 0
+        vp = _dp(vecs.view(np.float64)) if vecs is not None else ct.POINTER(ct.c_double)()
+        check(L().dzhip_eig_solve(self.handle, _dp(diag), diag.shape[0], nbands, ct.byref(o), ip, ic,
+                                  _dp(vals), vp, ct.byref(st)))
+        return vals, vecs, st.as_dict()
+
+    def density(self, psi, weights, rho=None):
+        """rho(r) (+)= (1/Omega) sum_b w_b |psi_b(r)|^2 in FFT order (accumulate_band_density_fft)."""
+        psi = _as_c128(psi)
+        if psi.ndim == 1:
+            psi = psi[None, :]
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        if w.shape[0] != psi.shape[0]:
+            raise InvalidMatrixSize("one weight per band")
+        total = int(np.prod(self.grid))
+        acc = rho is not None
+        out = np.ascontiguousarray(rho, dtype=np.float64).copy() if acc else np.zeros(total)
+        check(L().dzhip_density_accumulate(self.handle, _dp(psi.view(np.float64)), psi.shape[1], psi.shape[0],
+                                           _dp(w), _dp(out), total, 1 if acc else 0))
+        return out
 
     def info(self):
         out = (ct.c_uint64 * 5)()

@@ -152,6 +152,58 @@ int dzhip_ham_sync(dzhip_ham* ham);
 int dzhip_ham_profile(dzhip_ham* ham, int enable);
 int dzhip_ham_phase_times(dzhip_ham* ham, double* out_ms, uint64_t n_out, uint64_t* calls);
 
+/* ------------------------------------------------------------------ eigensolver
+ * Lowest nbands eigenpairs of H on this k-point, S = I.  Replaces
+ * hermitian_eigen_decomp_iterative (src/features/linalg/iterative.zig:67-76) /
+ * lobpcg_serial.solve (src/features/linalg/lobpcg/serial.zig:55-307) with the same outer
+ * semantics: block = nbands + 4, subspace up to 4 nbands + 8, converged when
+ * max_b ||H x_b - lambda_b x_b|| < tol (Ry), kinetic-shift preconditioner, LCG start with seed
+ * 0x6f3d3cbe.  Orthonormalisation is block CGS2 + Cholesky QR, Rayleigh-Ritz is a device
+ * zheevd, restarts keep [Ritz vectors, new block].  All subspace algebra is FP64; with
+ * precision = DZHIP_FP32 the H apply runs in FP32 until the largest residual is below
+ * mixed_switch_tol, then the basis restarts with an FP64 apply. */
+#define DZHIP_PRECOND_KINETIC 0 /* r / (|k+G|^2 - lambda), |den| >= 1e-8 (common.zig:214-226) */
+#define DZHIP_PRECOND_TPA 1     /* Teter-Payne-Allan K(x) r, x = |k+G|^2 / <x_b|T|x_b> */
+
+typedef struct dzhip_eig_opts {
+    uint64_t max_iter;       /* Options.max_iter (reference default 40) */
+    double tol;              /* Options.tol: max residual 2-norm, Ry */
+    uint64_t max_subspace;   /* 0 = min(n, 4 nbands + 8) */
+    uint64_t block_size;     /* 0 = min(max_subspace, nbands + 4) */
+    int init_diagonal;       /* Options.init_diagonal: unit vectors at the smallest diag entries */
+    int precision;           /* DZHIP_FP64 or DZHIP_FP32 (mixed: FP32 apply, FP64 at the end) */
+    double mixed_switch_tol; /* residual below which the FP32 apply switches to FP64; 0 = 1e-5 */
+    int preconditioner;      /* DZHIP_PRECOND_* */
+    uint64_t seed;           /* LCG seed; 0 = 0x6f3d3cbe */
+} dzhip_eig_opts;
+
+typedef struct dzhip_eig_stats {
+    uint64_t iterations;            /* Rayleigh-Ritz steps */
+    uint64_t apply_calls;           /* H applied to this many vectors in total */
+    uint64_t fp32_applies, fp64_applies;
+    uint64_t restarts;
+    uint64_t precision_switch_iter; /* iteration at which the FP32 apply switched to FP64 (0: never) */
+    uint64_t converged;             /* 1 if max residual < tol at return */
+    double max_residual;
+} dzhip_eig_stats;
+
+/* diag: [n] preconditioner diagonal (|k+G|^2, kpoint/solve.zig:362-367).  init_vecs: NULL or
+ * [n * init_cols] start vectors for the first columns (warm start, solve.zig:542-557).
+ * values_out: [nbands] ascending; vectors_out: NULL or [n * nbands] column-major.  stats may
+ * be NULL.  When not converged after max_iter the best estimate is returned (serial.zig:298-306). */
+int dzhip_eig_solve(dzhip_ham* ham, const double* diag, uint64_t n, uint64_t nbands,
+                    const dzhip_eig_opts* opts, const double* init_vecs, uint64_t init_cols,
+                    double* values_out, double* vectors_out, dzhip_eig_stats* stats);
+
+/* ------------------------------------------------------------------ density
+ * rho(r) (+)= (1/Omega) sum_b w_b |psi_b(r)|^2 with psi_b(r) = sum_G c_bG e^{iG.r}, FP64, rho in
+ * FFT order [nx*ny*nz].  Replaces the per-band loop of accumulate_band_contributions /
+ * accumulate_band_density_fft (src/features/scf/kpoint/density.zig:90-143, 444-474); the
+ * caller supplies the weights w_b = w_k * occ_b * 2 (density.zig:98-107), zero weights skip
+ * the band.  accumulate = 0 overwrites rho.  Host pointers, blocking. */
+int dzhip_density_accumulate(dzhip_ham* ham, const double* psi, uint64_t n, uint64_t nbands,
+                             const double* weights, double* rho, uint64_t count, int accumulate);
+
 /* Sizes the library uses for this context (for roofline accounting):
  * out[0] = n_pw, [1] = grid points, [2] = x-planes holding the G-sphere,
  * [3] = projectors P, [4] = device bytes held. */
