@@ -9,6 +9,7 @@
 //   pass 2: one workgroup per y (nz rows of nx contiguous values): z-lines in LDS.
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "fft_lds.h"
 
@@ -100,10 +101,10 @@ void set_max_dynamic_lds(const void* func, size_t bytes) {
 // ----------------------------------------------------------------- kernels
 // Pass 1: z-plane = contiguous [y][x] block; LDS tile rows of pitch nx+1; transform along
 // x, then y.
-template <typename T>
+template <typename T, int S>
 __global__ __launch_bounds__(TileThreads<T>::value) void k_fft_xy(
     typename C2T<T>::type* __restrict__ data, FftAxis axx, FftAxis axy,
-    const typename C2T<T>::type* __restrict__ tw, const int* __restrict__ perm, T sign) {
+    const typename C2T<T>::type* __restrict__ tw, const int* __restrict__ perm) {
     using C = typename C2T<T>::type;
     extern __shared__ unsigned char smem_raw[];
     const int nx = axx.n, ny = axy.n, pitch = nx + 1;
@@ -122,8 +123,8 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_fft_xy(
         buf[px[x] + pitch * py[y]] = plane[i];
     }
     __syncthreads();
-    lds_fft_lines<T, C, true>(buf, axx, twx, LineSet{ny, 1, pitch, nullptr}, sign, tid, nthr);
-    lds_fft_lines<T, C, true>(buf, axy, twy, LineSet{nx, pitch, 1, nullptr}, sign, tid, nthr);
+    lds_fft_lines<T, C, true, S>(buf, axx, twx, LineSet{ny, 1, pitch, nullptr}, tid, nthr);
+    lds_fft_lines<T, C, true, S>(buf, axy, twy, LineSet{nx, pitch, 1, nullptr}, tid, nthr);
     for (int i = tid; i < nx * ny; i += nthr) {
         const int y = fdiv(i, nx, mx), x = i - y * nx;
         plane[i] = buf[x + pitch * y];
@@ -131,10 +132,10 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_fft_xy(
 }
 
 // Pass 2: fixed y, LDS tile [z][x] of pitch nx+1; transform along z; optional scale.
-template <typename T>
+template <typename T, int S>
 __global__ __launch_bounds__(TileThreads<T>::value) void k_fft_z(
     typename C2T<T>::type* __restrict__ data, int nx, int ny, FftAxis axz,
-    const typename C2T<T>::type* __restrict__ tw, const int* __restrict__ perm, T sign, T scale) {
+    const typename C2T<T>::type* __restrict__ tw, const int* __restrict__ perm, T scale) {
     using C = typename C2T<T>::type;
     extern __shared__ unsigned char smem_raw[];
     const int nz = axz.n, pitch = nx + 1;
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_fft_z(
         buf[x + pitch * pz[z]] = data[x + (size_t)nx * (y + (size_t)ny * z)];
     }
     __syncthreads();
-    lds_fft_lines<T, C, true>(buf, axz, twz, LineSet{nx, pitch, 1, nullptr}, sign, tid, nthr);
+    lds_fft_lines<T, C, true, S>(buf, axz, twz, LineSet{nx, pitch, 1, nullptr}, tid, nthr);
     for (int i = tid; i < nx * nz; i += nthr) {
         const int z = fdiv(i, nx, mx), x = i - z * nx;
         const C v = buf[x + pitch * z];
@@ -185,19 +186,23 @@ void fft3d_device(typename C2T<T>::type* data, const AxisTables& tab, bool inver
     using C = typename C2T<T>::type;
     const int nx = tab.ax[0].n, ny = tab.ax[1].n, nz = tab.ax[2].n;
     const C* tw = TwSel<T>::get(tab.tw64, tab.tw32);
-    const T sign = inverse ? T(1) : T(-1);
     const T scale = inverse ? T(1.0 / ((double)nx * ny * nz)) : T(1);
     const size_t lds1 = fft_xy_lds_bytes(nx, ny, sizeof(C));
     const size_t lds2 = fft_z_lds_bytes(nx, nz, sizeof(C));
-    set_max_dynamic_lds((const void*)k_fft_xy<T>, lds1);
-    set_max_dynamic_lds((const void*)k_fft_z<T>, lds2);
     constexpr int nt = TileThreads<T>::value;
-    hipLaunchKernelGGL(k_fft_xy<T>, dim3(nz), dim3(nt), lds1, s, data, tab.ax[0], tab.ax[1], tw,
-                       tab.perm, sign);
-    DZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_fft_z<T>, dim3(ny), dim3(nt), lds2, s, data, nx, ny, tab.ax[2], tw,
-                       tab.perm, sign, scale);
-    DZ_HIP(hipGetLastError());
+    auto run = [&](auto sign_tag) {
+        constexpr int S = decltype(sign_tag)::value;
+        set_max_dynamic_lds((const void*)k_fft_xy<T, S>, lds1);
+        set_max_dynamic_lds((const void*)k_fft_z<T, S>, lds2);
+        hipLaunchKernelGGL((k_fft_xy<T, S>), dim3(nz), dim3(nt), lds1, s, data, tab.ax[0], tab.ax[1], tw, tab.perm);
+        DZ_HIP(hipGetLastError());
+        hipLaunchKernelGGL((k_fft_z<T, S>), dim3(ny), dim3(nt), lds2, s, data, nx, ny, tab.ax[2], tw, tab.perm, scale);
+        DZ_HIP(hipGetLastError());
+    };
+    if (inverse)
+        run(std::integral_constant<int, 1>{});
+    else
+        run(std::integral_constant<int, -1>{});
 }
 
 template void fft3d_device<double>(double2*, const AxisTables&, bool, hipStream_t);

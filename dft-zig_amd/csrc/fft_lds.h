@@ -170,11 +170,11 @@ __host__ __device__ __forceinline__ void dft_r(C* v, T sign) {
     if constexpr (R == 9) dft9<T>(v, sign);
 }
 
-// w^t for the transform sign: the table holds exp(-2 pi i t / n).
-template <typename C, typename T>
-__host__ __device__ __forceinline__ C tw_get(const C* tw, int t, T sign) {
+// w^t for the transform sign S (+1 inverse, -1 forward): the table holds exp(-2 pi i t / n).
+template <int S, typename C>
+__host__ __device__ __forceinline__ C tw_get(const C* tw, int t) {
     C w = tw[t];
-    if (sign > T(0)) w.y = -w.y;
+    if (S > 0) w.y = -w.y;
     return w;
 }
 
@@ -194,9 +194,10 @@ struct LineSet {
 };
 
 // One DIT stage (span Lp -> L = Lp*R) or DIF stage (span L -> L/R).
-template <typename T, typename C, int R, bool DIT>
+template <typename T, typename C, int R, bool DIT, int S>
 __host__ __device__ __forceinline__ void fft_stage(C* buf, int n, int span, const LineSet& ls,
-                                                   const C* tw, T sign, int tid, int nthr) {
+                                                   const C* tw, int tid, int nthr) {
+    const T sign = T(S);
     const int sub = DIT ? span : span / R;  // distance between the R inputs of a butterfly (in elements)
     const int L = DIT ? span * R : span;
     const int tstep = n / L;
@@ -216,43 +217,43 @@ __host__ __device__ __forceinline__ void fft_stage(C* buf, int n, int span, cons
         for (int r = 0; r < R; ++r) v[r] = p[r * st];
         if (DIT && k != 0) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[r] = cmul_hd(v[r], tw_get(tw, k * r * tstep, sign));
+            for (int r = 1; r < R; ++r) v[r] = cmul_hd(v[r], tw_get<S>(tw, k * r * tstep));
         }
         dft_r<T, C, R>(v, sign);
         if (!DIT && k != 0) {
 #pragma unroll
-            for (int q = 1; q < R; ++q) v[q] = cmul_hd(v[q], tw_get(tw, k * q * tstep, sign));
+            for (int q = 1; q < R; ++q) v[q] = cmul_hd(v[q], tw_get<S>(tw, k * q * tstep));
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) p[r * st] = v[r];
     }
 }
 
-template <typename T, typename C, bool DIT>
+template <typename T, typename C, bool DIT, int S>
 __host__ __device__ __forceinline__ void fft_stage_dispatch(int R, C* buf, int n, int span,
-                                                            const LineSet& ls, const C* tw, T sign,
-                                                            int tid, int nthr) {
+                                                            const LineSet& ls, const C* tw, int tid,
+                                                            int nthr) {
     switch (R) {
-        case 2: fft_stage<T, C, 2, DIT>(buf, n, span, ls, tw, sign, tid, nthr); break;
-        case 3: fft_stage<T, C, 3, DIT>(buf, n, span, ls, tw, sign, tid, nthr); break;
-        case 4: fft_stage<T, C, 4, DIT>(buf, n, span, ls, tw, sign, tid, nthr); break;
-        case 5: fft_stage<T, C, 5, DIT>(buf, n, span, ls, tw, sign, tid, nthr); break;
-        case 8: fft_stage<T, C, 8, DIT>(buf, n, span, ls, tw, sign, tid, nthr); break;
-        default: fft_stage<T, C, 9, DIT>(buf, n, span, ls, tw, sign, tid, nthr); break;
+        case 2: fft_stage<T, C, 2, DIT, S>(buf, n, span, ls, tw, tid, nthr); break;
+        case 3: fft_stage<T, C, 3, DIT, S>(buf, n, span, ls, tw, tid, nthr); break;
+        case 4: fft_stage<T, C, 4, DIT, S>(buf, n, span, ls, tw, tid, nthr); break;
+        case 5: fft_stage<T, C, 5, DIT, S>(buf, n, span, ls, tw, tid, nthr); break;
+        case 8: fft_stage<T, C, 8, DIT, S>(buf, n, span, ls, tw, tid, nthr); break;
+        default: fft_stage<T, C, 9, DIT, S>(buf, n, span, ls, tw, tid, nthr); break;
     }
 }
 
-// Full in-place transform of the lines.  Ends with a barrier.
+// Full in-place transform of the lines, sign S (+1 inverse, -1 forward).  Ends with a barrier.
 // DIT: input digit-reversed -> natural output.  DIF: natural input -> digit-reversed.
-template <typename T, typename C, bool DIT>
+template <typename T, typename C, bool DIT, int S>
 __host__ __device__ void lds_fft_lines(C* buf, const FftAxis& ax, const C* tw, const LineSet& ls,
-                                       T sign, int tid, int nthr) {
+                                       int tid, int nthr) {
     const int n = ax.n;
     if (DIT) {
         int Lp = 1;
         for (int s = 0; s < ax.nstages; ++s) {
             const int R = ax.radix[s];
-            fft_stage_dispatch<T, C, true>(R, buf, n, Lp, ls, tw, sign, tid, nthr);
+            fft_stage_dispatch<T, C, true, S>(R, buf, n, Lp, ls, tw, tid, nthr);
             Lp *= R;
             DZ_BARRIER();
         }
@@ -260,7 +261,7 @@ __host__ __device__ void lds_fft_lines(C* buf, const FftAxis& ax, const C* tw, c
         int L = n;
         for (int s = ax.nstages - 1; s >= 0; --s) {
             const int R = ax.radix[s];
-            fft_stage_dispatch<T, C, false>(R, buf, n, L, ls, tw, sign, tid, nthr);
+            fft_stage_dispatch<T, C, false, S>(R, buf, n, L, ls, tw, tid, nthr);
             L /= R;
             DZ_BARRIER();
         }
@@ -283,7 +284,7 @@ template <> struct TwSel<float> {
 
 // Threads per workgroup for the tile kernels: FP64 tiles are twice as large, so fewer fit
 // per CU; more threads per workgroup keep the CU occupied.
-template <typename T> struct TileThreads { static constexpr int value = 256; };
-template <> struct TileThreads<double> { static constexpr int value = 512; };
+template <typename T> struct TileThreads { static constexpr int value = 512; };
+template <> struct TileThreads<double> { static constexpr int value = 1024; };
 
 }  // namespace dzhip

@@ -43,8 +43,9 @@ struct Ham {
     //   plane_xpos[p]: LDS row px[x] of plane p in the x-line tile
     //   plane_ystart[p]..plane_ystart[p+1]: plane_ycols entries = LDS columns py[y] of the
     //                  y values present in plane p (the z-transforms that are not all zero)
-    int nplanes = 0;
-    DevBuf plane_start, plane_pw, plane_pos, plane_xpos, plane_ystart, plane_ycols;
+    //   plane_xabsent: LDS rows px[x] of the x values with no plane wave (zeroed in the x-line tile)
+    int nplanes = 0, nxabsent = 0;
+    DevBuf plane_start, plane_pw, plane_pos, plane_xpos, plane_ystart, plane_ycols, plane_xabsent;
 
     DevBuf kinetic64, kinetic32;
     DevBuf vloc64, vloc32;  // V_loc(r)/N in Ry

@@ -13,11 +13,12 @@
 //               tile, inverse FFT along x, multiply by V_loc(r)/N, forward FFT along x,
 //               write back only the rows of sphere x-planes.
 //   plane_fwd : one workgroup per (plane, band): forward FFT along y and z, gather the
-//               sphere points, add |k+G|^2 x.
+//               sphere points, add |k+G|^2 x (and the nonlocal term already in y).
 // HBM traffic per band: psi + H psi on the sphere and two round trips of half a grid.
 //
 // Nonlocal part: complex projector matrix A[p][G] = phi_p(G) e^{+iG.tau_a} held on the
-// device; proj = A x (split-K), q = D proj / Omega per atom block, y += A^H q.
+// device; proj = A x (split-K), q = D proj / Omega per atom block, y = A^H q (before the
+// local part, which then accumulates into y).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -26,6 +27,9 @@
 #include "ham.h"
 
 namespace dzhip {
+
+// Columns (bands) per launch sequence of the apply; larger blocks are processed in passes.
+constexpr size_t kMaxColsPerPass = 256;
 
 // LDS layout of the tile kernels: tile rows of pitch ny+1 (odd in elements), then the
 // twiddle tables of the axes used, then (plane kernels) the list of active y columns.
@@ -65,9 +69,9 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_inv(
     for (int j = pstart[p] + tid; j < j1; j += nthr) buf[ppos[j]] = xc[ppw[j]];
     __syncthreads();
     // Inverse (e^{+i}) along z, only on the y-columns that hold plane waves.
-    lds_fft_lines<T, C, true>(buf, az, twz, LineSet{nact, pitch, 0, cols}, T(1), tid, nthr);
+    lds_fft_lines<T, C, true, 1>(buf, az, twz, LineSet{nact, pitch, 0, cols}, tid, nthr);
     // Inverse along y on every z-row.
-    lds_fft_lines<T, C, true>(buf, ay, twy, LineSet{nz, 1, pitch, nullptr}, T(1), tid, nthr);
+    lds_fft_lines<T, C, true, 1>(buf, ay, twy, LineSet{nz, 1, pitch, nullptr}, tid, nthr);
     C* out = tbuf + (col * nplanes + p) * (size_t)ny * nz;
     const unsigned my = div_magic(ny);
     for (int i = tid; i < ny * nz; i += nthr) {
@@ -79,8 +83,8 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_inv(
 template <typename T>
 __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
     typename C2T<T>::type* __restrict__ tbuf, const int* __restrict__ pxpos, int nplanes,
-    const T* __restrict__ vloc, int nx, int ny, int nz, FftAxis ax,
-    const typename C2T<T>::type* __restrict__ tw) {
+    const int* __restrict__ xabsent, int nabsent, const T* __restrict__ vloc, int nx, int ny, int nz,
+    FftAxis ax, const typename C2T<T>::type* __restrict__ tw) {
     using C = typename C2T<T>::type;
     extern __shared__ unsigned char smem_raw[];
     const int pitch = ny + 1;
@@ -89,18 +93,21 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
     const int z = blockIdx.x;
     const size_t col = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    for (int i = tid; i < nx * pitch; i += nthr) buf[i] = C{T(0), T(0)};
-    load_twiddles(twx, tw + ax.tw_off, nx, tid, nthr);
-    __syncthreads();
-    C* tcol = tbuf + col * (size_t)nplanes * ny * nz;
     const unsigned my = div_magic(ny);
+    // Rows of x values that hold no plane wave start (and stay) zero before the inverse.
+    for (int i = tid; i < nabsent * ny; i += nthr) {
+        const int r = fdiv(i, ny, my), y = i - r * ny;
+        buf[y + pitch * xabsent[r]] = C{T(0), T(0)};
+    }
+    load_twiddles(twx, tw + ax.tw_off, nx, tid, nthr);
+    C* tcol = tbuf + col * (size_t)nplanes * ny * nz;
     for (int i = tid; i < nplanes * ny; i += nthr) {
         const int p = fdiv(i, ny, my), y = i - p * ny;
         buf[y + pitch * pxpos[p]] = tcol[((size_t)p * nz + z) * ny + y];
     }
     __syncthreads();
     const LineSet xl{ny, pitch, 1, nullptr};
-    lds_fft_lines<T, C, true>(buf, ax, twx, xl, T(1), tid, nthr);
+    lds_fft_lines<T, C, true, 1>(buf, ax, twx, xl, tid, nthr);
     const T* vz = vloc + (size_t)nx * ny * z;
     const unsigned mx = div_magic(nx);
     for (int i = tid; i < nx * ny; i += nthr) {
@@ -110,7 +117,7 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
         e = C{e.x * v, e.y * v};
     }
     __syncthreads();
-    lds_fft_lines<T, C, false>(buf, ax, twx, xl, T(-1), tid, nthr);
+    lds_fft_lines<T, C, false, -1>(buf, ax, twx, xl, tid, nthr);
     for (int i = tid; i < nplanes * ny; i += nthr) {
         const int p = fdiv(i, ny, my), y = i - p * ny;
         tcol[((size_t)p * nz + z) * ny + y] = buf[y + pitch * pxpos[p]];
@@ -123,7 +130,7 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_fwd(
     size_t ldx, typename C2T<T>::type* __restrict__ y, size_t ldy, const T* __restrict__ kinetic,
     const int* __restrict__ pstart, const int* __restrict__ ppw, const int* __restrict__ ppos,
     const int* __restrict__ ystart, const short* __restrict__ ycols, int ny, int nz, int nplanes,
-    FftAxis ay, FftAxis az, const typename C2T<T>::type* __restrict__ tw, int add_kinetic) {
+    FftAxis ay, FftAxis az, const typename C2T<T>::type* __restrict__ tw, int accumulate) {
     using C = typename C2T<T>::type;
     extern __shared__ unsigned char smem_raw[];
     const int pitch = ny + 1;
@@ -147,20 +154,22 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_fwd(
     __syncthreads();
     // Forward along y on every z-row (frequency y lands at py[y]), then along z only on the
     // columns whose frequencies are gathered.
-    lds_fft_lines<T, C, false>(buf, ay, twy, LineSet{nz, 1, pitch, nullptr}, T(-1), tid, nthr);
-    lds_fft_lines<T, C, false>(buf, az, twz, LineSet{nact, pitch, 0, cols}, T(-1), tid, nthr);
+    lds_fft_lines<T, C, false, -1>(buf, ay, twy, LineSet{nz, 1, pitch, nullptr}, tid, nthr);
+    lds_fft_lines<T, C, false, -1>(buf, az, twz, LineSet{nact, pitch, 0, cols}, tid, nthr);
     const C* xc = x + col * ldx;
     C* yc = y + col * ldy;
     const int j1 = pstart[p + 1];
     for (int j = pstart[p] + tid; j < j1; j += nthr) {
         const int g = ppw[j];
-        C v = buf[ppos[j]];
-        if (add_kinetic) {
-            const T kin = kinetic[g];
-            const C xv = xc[g];
-            v = C{v.x + kin * xv.x, v.y + kin * xv.y};
+        const C v = buf[ppos[j]];
+        const T kin = kinetic[g];
+        const C xv = xc[g];
+        C out = C{v.x + kin * xv.x, v.y + kin * xv.y};
+        if (accumulate) {
+            const C prev = yc[g];
+            out = C{out.x + prev.x, out.y + prev.y};
         }
-        yc[g] = v;
+        yc[g] = out;
     }
 }
 
@@ -173,6 +182,7 @@ constexpr int kDensMaxPts = 32;  // nx*ny <= kDensThreads * kDensMaxPts
 
 __global__ __launch_bounds__(kDensThreads) void k_xline_density(
     const double2* __restrict__ tbuf, const int* __restrict__ pxpos, int nplanes,
+    const int* __restrict__ xabsent, int nabsent,
     const double* __restrict__ wts, int nbands, int per_group, int nx, int ny, int nz, FftAxis ax,
     const double2* __restrict__ tw, double inv_vol, double* __restrict__ partial) {
     extern __shared__ unsigned char smem_raw[];
@@ -192,15 +202,17 @@ __global__ __launch_bounds__(kDensThreads) void k_xline_density(
     for (int b = b0; b < b1; ++b) {
         const double wb = wts[b];
         if (wb == 0.0) continue;  // same value for the whole workgroup
-        for (int i = tid; i < nx * pitch; i += nthr) buf[i] = make_double2(0.0, 0.0);
-        __syncthreads();
+        for (int i = tid; i < nabsent * ny; i += nthr) {
+            const int r = fdiv(i, ny, my), y = i - r * ny;
+            buf[y + pitch * xabsent[r]] = make_double2(0.0, 0.0);
+        }
         const double2* tcol = tbuf + (size_t)b * nplanes * ny * nz;
         for (int i = tid; i < nplanes * ny; i += nthr) {
             const int p = fdiv(i, ny, my), y = i - p * ny;
             buf[y + pitch * pxpos[p]] = tcol[((size_t)p * nz + z) * ny + y];
         }
         __syncthreads();
-        lds_fft_lines<double, double2, true>(buf, ax, twx, xl, 1.0, tid, nthr);
+        lds_fft_lines<double, double2, true, 1>(buf, ax, twx, xl, tid, nthr);
 #pragma unroll
         for (int k = 0; k < kDensMaxPts; ++k) {
             const int i = tid + k * nthr;
@@ -332,18 +344,18 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::t
     _Pragma("unroll") for (int i = 0; i < kXPer; ++i) {                                      \
         const int e = tid + i * kNlThreads;                                                   \
         const int r = e >> 5, c = e & 31, g = (KT) + c, b = b0 + r;                           \
-        C v = zero;                                                                           \
-        if (r < kNlBands && b < ncols && g < k1) v = x[(size_t)b * ldx + g];                  \
-        xre[i] = v.x;                                                                         \
-        xim[i] = v.y;                                                                         \
+        const bool ok = r < kNlBands && b < ncols && g < k1;                                  \
+        const C v = x[ok ? b * (int)ldx + g : 0];                                             \
+        xre[i] = ok ? v.x : T(0);                                                             \
+        xim[i] = ok ? v.y : T(0);                                                             \
     }                                                                                         \
     _Pragma("unroll") for (int i = 0; i < kAPer; ++i) {                                      \
         const int e = tid + i * kNlThreads;                                                   \
         const int r = e >> 5, c = e & 31, g = (KT) + c, p = p0 + r;                           \
-        C v = zero;                                                                           \
-        if (r < kNlP && p < P && g < k1) v = a[(size_t)p * npw + g];                          \
-        are[i] = v.x;                                                                         \
-        aim[i] = v.y;                                                                         \
+        const bool ok = r < kNlP && p < P && g < k1;                                          \
+        const C v = a[ok ? p * npw + g : 0];                                                  \
+        are[i] = ok ? v.x : T(0);                                                             \
+        aim[i] = ok ? v.y : T(0);                                                             \
     }
 #define DZ_PROJ_STASH()                                                                       \
     _Pragma("unroll") for (int i = 0; i < kXPer; ++i) {                                      \
@@ -447,7 +459,7 @@ __global__ __launch_bounds__(256) void k_nl_dij(const typename C2T<T>::type* __r
     }
 }
 
-// y[b][G] += sum_p q[b][p] * conj(A[p][G])
+// y[b][G] = sum_p q[b][p] * conj(A[p][G])   (written, not accumulated: runs before k_plane_fwd)
 template <typename T>
 __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::type* __restrict__ a,
                                                         const typename C2T<T>::type* __restrict__ q,
@@ -473,18 +485,18 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::t
     _Pragma("unroll") for (int i = 0; i < kQPer; ++i) {                                      \
         const int e = tid + i * kNlThreads;                                                   \
         const int r = e >> 5, c = e & 31, b = b0 + r, p = (PT) + c;                           \
-        C v = zero;                                                                           \
-        if (r < kNlBands && b < ncols && p < P) v = q[(size_t)b * P + p];                     \
-        qre[i] = v.x;                                                                         \
-        qim[i] = v.y;                                                                         \
+        const bool ok = r < kNlBands && b < ncols && p < P;                                   \
+        const C v = q[ok ? b * P + p : 0];                                                    \
+        qre[i] = ok ? v.x : T(0);                                                             \
+        qim[i] = ok ? v.y : T(0);                                                             \
     }                                                                                         \
     _Pragma("unroll") for (int i = 0; i < kAPer; ++i) {                                      \
         const int e = tid + i * kNlThreads;                                                   \
         const int r = e / kNlG, c = e % kNlG, p = (PT) + r, g = g0 + c;                       \
-        C v = zero;                                                                           \
-        if (r < kNlK && p < P && g < npw) v = a[(size_t)p * npw + g];                         \
-        are[i] = v.x;                                                                         \
-        aim[i] = v.y;                                                                         \
+        const bool ok = r < kNlK && p < P && g < npw;                                         \
+        const C v = a[ok ? p * npw + g : 0];                                                  \
+        are[i] = ok ? v.x : T(0);                                                             \
+        aim[i] = ok ? v.y : T(0);                                                             \
     }
 #define DZ_BACK_STASH()                                                                       \
     _Pragma("unroll") for (int i = 0; i < kQPer; ++i) {                                      \
@@ -545,10 +557,7 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::t
             for (int r = 0; r < 4; ++r) {
                 const int b = b0 + wave * 32 + bb * 16 + M::row(k4, r);
                 const int g = g0 + gb * 16 + i16;
-                if (b < ncols && g < npw) {
-                    C& o = y[(size_t)b * ldy + g];
-                    o = C{o.x + accr[bb][gb][r], o.y + acci[bb][gb][r]};
-                }
+                if (b < ncols && g < npw) y[(size_t)b * ldy + g] = C{accr[bb][gb][r], acci[bb][gb][r]};
             }
 }
 
@@ -567,7 +576,7 @@ __global__ void k_d2f(const double* __restrict__ in, float* __restrict__ out, si
 // ----------------------------------------------------------------- host side
 size_t Ham::device_bytes() const {
     const DevBuf* bufs[] = {&plane_start, &plane_pw, &plane_pos, &plane_xpos, &plane_ystart,
-                            &plane_ycols, &kinetic64,
+                            &plane_ycols, &plane_xabsent, &kinetic64,
                             &kinetic32, &vloc64, &vloc32, &a64, &a32, &meta, &block_rows, &dm64, &dm32,
                             &xstage, &ystage, &x32, &y32, &tbuf, &partial, &qbuf};
     size_t s = tab.bytes;
@@ -655,6 +664,11 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
     upload(h.plane_xpos, xpos);
     upload(h.plane_ystart, ystart);
     upload(h.plane_ycols, ycols);
+    std::vector<int> xabsent;
+    for (int xx = 0; xx < h.nx; ++xx)
+        if (plane_of_x[xx] < 0) xabsent.push_back(px[xx]);
+    h.nxabsent = (int)xabsent.size();
+    upload(h.plane_xabsent, xabsent);
 
     std::vector<double> kin(d.kinetic, d.kinetic + h.npw);
     std::vector<float> kin32(kin.begin(), kin.end());
@@ -720,6 +734,8 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
             }
         }
         h.nproj = (int)row_of.size();
+        DZ_REQUIRE((size_t)h.nproj * h.npw < ((size_t)1 << 31) && h.npw * kMaxColsPerPass < ((size_t)1 << 31),
+                   DZHIP_ERR_ARG, "projector matrix or column block exceeds 2^31 elements");
         DZ_REQUIRE((size_t)h.nproj * sizeof(double2) <= kMaxLdsBytes, DZHIP_ERR_ARG,
                    "more than 10240 projectors per k-point is not supported");
         std::vector<int> block_rows;
@@ -771,7 +787,7 @@ void ham_set_local(Ham& h, const double* local_r) {
 void ham_release(Ham& h) {
     eig_release(h);
     DevBuf* bufs[] = {&h.plane_start, &h.plane_pw, &h.plane_pos, &h.plane_xpos, &h.plane_ystart,
-                      &h.plane_ycols, &h.kinetic64,
+                      &h.plane_ycols, &h.plane_xabsent, &h.kinetic64,
                       &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.block_rows, &h.dm64,
                       &h.dm32, &h.xstage, &h.ystage, &h.x32, &h.y32, &h.tbuf, &h.partial, &h.qbuf};
     for (DevBuf* b : bufs) b->release();
@@ -827,7 +843,6 @@ void Ham::collect_profile() {
     ev_used = 0;
 }
 
-constexpr size_t kMaxColsPerPass = 256;
 
 template <typename T>
 void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, bool nonlocal) {
@@ -873,6 +888,17 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
             DZ_HIP(hipGetLastError());
         }
         mark(h, evs, 1);
+        if (do_nl) {
+            const size_t lds = nl_back_lds_bytes<T>();
+            constexpr int kNlG = nl_g_tile<T>();
+            set_max_dynamic_lds((const void*)k_nl_back<T>, lds);
+            hipLaunchKernelGGL(k_nl_back<T>, dim3((unsigned)((h.npw + kNlG - 1) / kNlG), 1,
+                                                  (unsigned)((nc + kNlBands - 1) / kNlBands)),
+                               dim3(kNlThreads), lds, s, Sel<T>::a(h), h.qbuf.as<C>(), yc, ld, h.nproj,
+                               (int)h.npw, (int)nc);
+            DZ_HIP(hipGetLastError());
+        }
+        mark(h, evs, 2);
         if (local) {
             const size_t plane = (size_t)h.ny * h.nz;
             h.tbuf.ensure(nc * h.nplanes * plane * sizeof(C));
@@ -888,36 +914,26 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
                                h.plane_pos.as<int>(), h.plane_ystart.as<int>(), h.plane_ycols.as<short>(),
                                h.ny, h.nz, h.nplanes, h.tab.ax[1], h.tab.ax[2], tw);
             DZ_HIP(hipGetLastError());
-            mark(h, evs, 2);
-            hipLaunchKernelGGL(k_xline<T>, dim3(h.nz, (unsigned)nc), dim3(nt), lds_x, s, tb,
-                               h.plane_xpos.as<int>(), h.nplanes, Sel<T>::vloc(h), h.nx, h.ny, h.nz,
-                               h.tab.ax[0], tw);
-            DZ_HIP(hipGetLastError());
             mark(h, evs, 3);
+            hipLaunchKernelGGL(k_xline<T>, dim3(h.nz, (unsigned)nc), dim3(nt), lds_x, s, tb,
+                               h.plane_xpos.as<int>(), h.nplanes, h.plane_xabsent.as<int>(), h.nxabsent,
+                               Sel<T>::vloc(h), h.nx, h.ny, h.nz, h.tab.ax[0], tw);
+            DZ_HIP(hipGetLastError());
+            mark(h, evs, 4);
             hipLaunchKernelGGL(k_plane_fwd<T>, dim3(h.nplanes, (unsigned)nc), dim3(nt), lds_p, s,
                                tb, xc, ld, yc, ld, Sel<T>::kin(h), h.plane_start.as<int>(),
                                h.plane_pw.as<int>(), h.plane_pos.as<int>(), h.plane_ystart.as<int>(),
                                h.plane_ycols.as<short>(), h.ny, h.nz, h.nplanes, h.tab.ax[1],
-                               h.tab.ax[2], tw, 1);
+                               h.tab.ax[2], tw, do_nl ? 1 : 0);
             DZ_HIP(hipGetLastError());
-            mark(h, evs, 4);
         } else {
-            const size_t n = nc * ld;
-            hipLaunchKernelGGL(k_fill_zero<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, yc, n);
-            DZ_HIP(hipGetLastError());
-            mark(h, evs, 2);
+            if (!do_nl) {
+                const size_t n = nc * ld;
+                hipLaunchKernelGGL(k_fill_zero<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, yc, n);
+                DZ_HIP(hipGetLastError());
+            }
             mark(h, evs, 3);
             mark(h, evs, 4);
-        }
-        if (do_nl) {
-            const size_t lds = nl_back_lds_bytes<T>();
-            constexpr int kNlG = nl_g_tile<T>();
-            set_max_dynamic_lds((const void*)k_nl_back<T>, lds);
-            hipLaunchKernelGGL(k_nl_back<T>, dim3((unsigned)((h.npw + kNlG - 1) / kNlG), 1,
-                                                  (unsigned)((nc + kNlBands - 1) / kNlBands)),
-                               dim3(kNlThreads), lds, s, Sel<T>::a(h), h.qbuf.as<C>(), yc, ld, h.nproj,
-                               (int)h.npw, (int)nc);
-            DZ_HIP(hipGetLastError());
         }
         mark(h, evs, 5);
         // Bound the number of outstanding event sets.
@@ -951,7 +967,7 @@ void ham_density_dev(Ham& h, const double2* psi, size_t nbands, const double* wt
         const int ng = (int)std::max<size_t>(1, std::min<size_t>(nc, (size_t)ngroups));
         const int pg = (int)((nc + ng - 1) / ng);
         hipLaunchKernelGGL(k_xline_density, dim3(h.nz, ng), dim3(kDensThreads), lds_x, s, h.tbuf.as<double2>(),
-                           h.plane_xpos.as<int>(), h.nplanes, wts_dev + c0, (int)nc, pg, h.nx, h.ny, h.nz,
+                           h.plane_xpos.as<int>(), h.nplanes, h.plane_xabsent.as<int>(), h.nxabsent, wts_dev + c0, (int)nc, pg, h.nx, h.ny, h.nz,
                            h.tab.ax[0], h.tab.tw64, h.inv_volume, h.partial.as<double>());
         DZ_HIP(hipGetLastError());
         const bool acc = accumulate || c0 > 0;

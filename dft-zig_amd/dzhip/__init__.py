@@ -24,7 +24,7 @@ LIB_PATH = _HERE / "libdzhip.so"
 FP64 = 0
 FP32 = 1
 
-PHASES = ("nl_project", "plane_inv", "xline", "plane_fwd", "nl_back")
+PHASES = ("nl_project", "nl_back", "plane_inv", "xline", "plane_fwd")
 
 EXPORTS = (
     "dzhip_version", "dzhip_last_error", "dzhip_ctx_create", "dzhip_ctx_destroy",

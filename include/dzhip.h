@@ -144,10 +144,10 @@ int dzhip_ham_sync(dzhip_ham* ham);
  * and returns the accumulated milliseconds in out[DZHIP_PHASE_COUNT] and the
  * number of applies in *calls.  Enabling or disabling resets the totals. */
 #define DZHIP_PHASE_NL_PROJECT 0 /* <beta|psi> projection + D_ij */
-#define DZHIP_PHASE_PLANE_INV 1  /* scatter + inverse FFT along z, y (per x-plane) */
-#define DZHIP_PHASE_XLINE 2      /* inverse FFT along x, V_loc multiply, forward FFT along x */
-#define DZHIP_PHASE_PLANE_FWD 3  /* forward FFT along y, z + gather + kinetic */
-#define DZHIP_PHASE_NL_BACK 4    /* back-projection accumulate */
+#define DZHIP_PHASE_NL_BACK 1    /* back-projection, written into y */
+#define DZHIP_PHASE_PLANE_INV 2  /* scatter + inverse FFT along z, y (per x-plane) */
+#define DZHIP_PHASE_XLINE 3      /* inverse FFT along x, V_loc multiply, forward FFT along x */
+#define DZHIP_PHASE_PLANE_FWD 4  /* forward FFT along y, z + gather + kinetic, added to y */
 #define DZHIP_PHASE_COUNT 5
 int dzhip_ham_profile(dzhip_ham* ham, int enable);
 int dzhip_ham_phase_times(dzhip_ham* ham, double* out_ms, uint64_t n_out, uint64_t* calls);

@@ -46,8 +46,10 @@ static double run(int n, bool dit, double sign, int nlines, int es, int ls, bool
         }
     }
     LineSet lsd{nlines, es, ls, use_list ? lines.data() : nullptr};
-    if (dit) lds_fft_lines<T, C, true>(buf.data(), ax, tw.data(), lsd, (T)sign, 0, 1);
-    else lds_fft_lines<T, C, false>(buf.data(), ax, tw.data(), lsd, (T)sign, 0, 1);
+    if (dit && sign > 0) lds_fft_lines<T, C, true, 1>(buf.data(), ax, tw.data(), lsd, 0, 1);
+    else if (dit) lds_fft_lines<T, C, true, -1>(buf.data(), ax, tw.data(), lsd, 0, 1);
+    else if (sign > 0) lds_fft_lines<T, C, false, 1>(buf.data(), ax, tw.data(), lsd, 0, 1);
+    else lds_fft_lines<T, C, false, -1>(buf.data(), ax, tw.data(), lsd, 0, 1);
     double err = 0, mx = 0;
     for (int l = 0; l < nlines; ++l) {
         const int b = use_list ? lines[l] : l * ls;
