@@ -111,12 +111,55 @@ def cpu_baseline(w, budget_s=12.0):
                       f"{t_used / done * 1e3:.1f} ms/band -> {t_used / done * w.nbands:.1f} s per 160-band block"}
 
 
+def scf_leg(name, max_iter, log=sys.stderr):
+    """Time-to-1e-6 Ha of the SCF: plain FP64 (k-points one after another on one stream) against
+    the mixed form (FP32 H apply, FP64 subspace algebra and FP64 apply below a 1e-5 residual,
+    every k-point on its own stream and host thread).  E_inf is the plain FP64 run's converged
+    energy (the driver is tied to the FP64 oracle SCF on the small Si case by the parity tests)."""
+    import dzhip
+    from dzhip import scf as dscf
+
+    system = {"si64": dscf.si64, "si8": dscf.si8_small}[name]()
+    variants = [("fp64", dzhip.FP64, False), ("mixed_batched", dzhip.FP32, True)]
+    runs, setup = {}, {}
+    for tag, prec, conc in variants:
+        drv = dscf.ScfDriver(system, concurrent_k=conc)
+        setup[tag] = drv.setup_s
+
+        def show(r, tag=tag):
+            print(f"scf {name} {tag} it {r.iteration} E {r.energy_ry:.10f} drho {r.drho:.3e} t {r.wall_s:.2f}s "
+                  f"eig {r.eig_s:.2f}s eig_it {r.eig_iterations} applies {r.applies} fp32 {r.fp32_applies}",
+                  file=log, flush=True)
+        try:
+            runs[tag] = drv.run(max_iter=max_iter, conv=1e-8, precision=prec, eig_tol_ratio=1.0, eig_tol_min=1e-7,
+                                progress=show)
+        finally:
+            drv.close()
+    e_inf = runs["fp64"].energies[-1]
+    out = {"system": system.name, "grid": list(system.grid), "k_points": len(system.kpoints), "bands": system.nbands,
+           "e_inf_ry": e_inf, "e_inf_source": "plain FP64 GPU run, converged to RMS drho < 1e-8"}
+    for tag, run in runs.items():
+        t = run.time_to(e_inf)
+        out[tag] = {"t_1e-6_Ha_s": None if t is None else round(t, 3), "iterations": len(run.records),
+                    "converged": run.converged, "final_dE_Ha": (run.energies[-1] - e_inf) * dscf.HA_PER_RY,
+                    "wall_s": round(run.records[-1].wall_s, 3), "setup_s": round(setup[tag], 3),
+                    "eig_s": round(sum(r.eig_s for r in run.records), 3),
+                    "applies": sum(r.applies for r in run.records),
+                    "fp32_applies": sum(r.fp32_applies for r in run.records)}
+    a, b = out["fp64"]["t_1e-6_Ha_s"], out["mixed_batched"]["t_1e-6_Ha_s"]
+    out["speedup"] = None if not a or not b else round(a / b, 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--scf", default="none", choices=("none", "si8", "si64"),
+                    help="also time the SCF to 1e-6 Ha (plain FP64 vs mixed+batched)")
+    ap.add_argument("--scf-max-iter", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (k-point sharding is "
@@ -198,7 +241,7 @@ def main():
         "fp32_vs_fp64_err": {"max_band_rel_l2": err, "max_abs": abserr},
         "phases_ms_fp32": {k: round(v, 5) for k, v in ph32.items()},
         "phases_ms_fp64": {k: round(v, 5) for k, v in ph64.items()},
-        "scf_time_to_1e-6_Ha": None,
+        "scf_time_to_1e-6_Ha": None if args.scf == "none" else scf_leg(args.scf, args.scf_max_iter),
     }
     print(json.dumps(out), flush=True)
 

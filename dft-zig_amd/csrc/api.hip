@@ -2,6 +2,7 @@
 #include <cstring>
 #include <exception>
 #include <new>
+#include <vector>
 
 #include "ham.h"
 
@@ -47,6 +48,27 @@ using namespace dzhip;
 struct dzhip_ctx : Context {};
 struct dzhip_fft_plan : FftPlan {};
 struct dzhip_ham : Ham {};
+
+namespace dzhip {
+// rho (host, ngrid) (+)= density of the nbands device columns psi, through the context's
+// staging buffers; blocking.
+static void density_host_rho(Ham& h, const double2* psi, uint64_t nbands, const double* weights, double* rho,
+                             bool accumulate) {
+    hipStream_t s = h.stream();
+    const size_t count = h.ngrid;
+    h.dens_rho.ensure(count * sizeof(double));
+    if (accumulate) DZ_HIP(hipMemcpyAsync(h.dens_rho.ptr, rho, count * sizeof(double), hipMemcpyHostToDevice, s));
+    if (nbands > 0) {
+        h.dens_w.ensure(nbands * sizeof(double));
+        DZ_HIP(hipMemcpyAsync(h.dens_w.ptr, weights, nbands * sizeof(double), hipMemcpyHostToDevice, s));
+        ham_density_dev(h, psi, nbands, h.dens_w.as<double>(), h.dens_rho.as<double>(), accumulate);
+    } else if (!accumulate) {
+        DZ_HIP(hipMemsetAsync(h.dens_rho.ptr, 0, count * sizeof(double), s));
+    }
+    DZ_HIP(hipMemcpyAsync(rho, h.dens_rho.ptr, count * sizeof(double), hipMemcpyDeviceToHost, s));
+    DZ_HIP(hipStreamSynchronize(s));
+}
+}  // namespace dzhip
 
 extern "C" {
 
@@ -331,23 +353,136 @@ int dzhip_density_accumulate(dzhip_ham* ham, const double* psi, uint64_t n, uint
         std::lock_guard<std::mutex> lk(ham->mu);
         DeviceGuard g(ham->ctx->device);
         hipStream_t s = ham->stream();
-        DevBuf dpsi, dw, drho;
-        drho.ensure(count * sizeof(double));
-        if (accumulate) DZ_HIP(hipMemcpyAsync(drho.ptr, rho, count * sizeof(double), hipMemcpyHostToDevice, s));
+        DevBuf dpsi;
         if (nbands > 0) {
             dpsi.ensure(n * nbands * sizeof(double2));
-            dw.ensure(nbands * sizeof(double));
             DZ_HIP(hipMemcpyAsync(dpsi.ptr, psi, n * nbands * sizeof(double2), hipMemcpyHostToDevice, s));
-            DZ_HIP(hipMemcpyAsync(dw.ptr, weights, nbands * sizeof(double), hipMemcpyHostToDevice, s));
-            ham_density_dev(*ham, dpsi.as<double2>(), nbands, dw.as<double>(), drho.as<double>(), accumulate != 0);
-        } else if (!accumulate) {
-            DZ_HIP(hipMemsetAsync(drho.ptr, 0, count * sizeof(double), s));
         }
-        DZ_HIP(hipMemcpyAsync(rho, drho.ptr, count * sizeof(double), hipMemcpyDeviceToHost, s));
-        DZ_HIP(hipStreamSynchronize(s));
+        density_host_rho(*ham, dpsi.as<double2>(), nbands, weights, rho, accumulate != 0);
         dpsi.release();
-        dw.release();
-        drho.release();
+    });
+}
+
+int dzhip_density_accumulate_solution(dzhip_ham* ham, uint64_t nbands, const double* weights, double* rho,
+                                      uint64_t count, int accumulate) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        DZ_REQUIRE(count == ham->ngrid, DZHIP_ERR_SIZE, "count != nx*ny*nz");
+        DZ_REQUIRE(rho && (nbands == 0 || weights), DZHIP_ERR_ARG, "NULL weights or rho");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        uint64_t cols = 0;
+        const double2* x = eig_solution(*ham, &cols);
+        DZ_REQUIRE(nbands == 0 || (x && nbands <= cols), DZHIP_ERR_STATE,
+                   "no eigensolver solution with that many bands on this context");
+        density_host_rho(*ham, x, nbands, weights, rho, accumulate != 0);
+    });
+}
+
+int dzhip_ham_set_precond_model(dzhip_ham* ham, const float* params, uint64_t n_params, double energy_scale) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        DZ_REQUIRE(params && n_params == (uint64_t)kMlpParams, DZHIP_ERR_SIZE, "need DZHIP_MLP_PARAMS parameters");
+        DZ_REQUIRE(energy_scale > 0.0, DZHIP_ERR_ARG, "energy_scale must be > 0");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        ham->mlp_params.ensure(kMlpParams * sizeof(float));
+        DZ_HIP(hipMemcpyAsync(ham->mlp_params.ptr, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice,
+                              ham->stream()));
+        DZ_HIP(hipStreamSynchronize(ham->stream()));
+        ham->mlp_energy_scale = energy_scale;
+        ham->has_mlp = true;
+    });
+}
+
+int dzhip_ham_training_samples(dzhip_ham* ham, double energy_scale, uint64_t stride, float* out, uint64_t capacity,
+                               uint64_t* count) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        DZ_REQUIRE(out && count, DZHIP_ERR_ARG, "NULL out or count");
+        DZ_REQUIRE(stride >= 1 && stride < (1u << 30), DZHIP_ERR_ARG, "bad stride");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        *count = eig_training_samples(*ham, energy_scale, (int)stride, out, capacity);
+    });
+}
+
+namespace {
+void upload_samples(MlpTrainer& t, const float* samples, uint64_t n, hipStream_t s) {
+    DZ_HIP(hipMemcpyAsync(t.samples.ptr, samples, 6 * n * sizeof(float), hipMemcpyHostToDevice, s));
+}
+}  // namespace
+
+int dzhip_mlp_loss_grad(dzhip_ctx* ctx, const float* samples, uint64_t n, const float* params, double* grad_out,
+                        double* loss_out) {
+    return guarded([&] {
+        DZ_REQUIRE(ctx, DZHIP_ERR_STATE, "NULL ctx");
+        DZ_REQUIRE(samples && params && n > 0, DZHIP_ERR_ARG, "NULL samples/params or n = 0");
+        DZ_REQUIRE(n < (1ull << 31), DZHIP_ERR_SIZE, "n must be below 2^31");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        hipStream_t s = ctx->stream;
+        MlpTrainer t(n);
+        upload_samples(t, samples, n, s);
+        DZ_HIP(hipMemcpyAsync(t.prm.ptr, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, s));
+        t.loss_grad(n, 1, 0, 0, n, s);
+        std::vector<double> gh(kMlpParams + 1);
+        DZ_HIP(hipMemcpyAsync(gh.data(), t.g.ptr, gh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipStreamSynchronize(s));
+        if (grad_out) std::memcpy(grad_out, gh.data(), kMlpParams * sizeof(double));
+        if (loss_out) *loss_out = gh[kMlpParams];
+    });
+}
+
+int dzhip_mlp_train(dzhip_ctx* ctx, const float* samples, uint64_t n, float* params, uint64_t epochs, uint64_t batch,
+                    double lr, uint64_t seed, double* epoch_loss) {
+    return guarded([&] {
+        DZ_REQUIRE(ctx, DZHIP_ERR_STATE, "NULL ctx");
+        DZ_REQUIRE(samples && params && n > 0 && batch > 0, DZHIP_ERR_ARG, "NULL samples/params, n = 0 or batch = 0");
+        DZ_REQUIRE(n < (1ull << 31), DZHIP_ERR_SIZE, "n must be below 2^31");
+        DZ_REQUIRE(lr > 0.0, DZHIP_ERR_ARG, "lr must be > 0");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        hipStream_t s = ctx->stream;
+        MlpTrainer t(n);
+        upload_samples(t, samples, n, s);
+        DZ_HIP(hipMemcpyAsync(t.prm.ptr, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, s));
+        DZ_HIP(hipMemsetAsync(t.m.ptr, 0, kMlpParams * sizeof(float), s));
+        DZ_HIP(hipMemsetAsync(t.v.ptr, 0, kMlpParams * sizeof(float), s));
+        unsigned long long x = seed ? seed : 0x6f3d3cbeULL;
+        auto next = [&]() {
+            x = x * 6364136223846793005ULL + 1442695040888963407ULL;
+            return x >> 17;
+        };
+        auto gcd = [](unsigned long long a, unsigned long long b) {
+            while (b) {
+                const unsigned long long r = a % b;
+                a = b;
+                b = r;
+            }
+            return a;
+        };
+        int step = 0;
+        for (uint64_t e = 0; e < epochs; ++e) {
+            // Visit order idx(j) = (mul j + add) mod n, a permutation since gcd(mul, n) = 1.
+            unsigned long long mul = 1 + next() % n;
+            while (gcd(mul, n) != 1) mul = 1 + next() % n;
+            const unsigned long long add = next() % n;
+            for (uint64_t start = 0; start < n; start += batch) {
+                const uint64_t cnt = std::min<uint64_t>(batch, n - start);
+                t.loss_grad(n, mul, add, start, cnt, s);
+                t.adam(++step, lr, s);
+            }
+            if (epoch_loss) {
+                t.loss_grad(n, 1, 0, 0, n, s);
+                double l = 0.0;
+                DZ_HIP(hipMemcpyAsync(&l, t.g.as<double>() + kMlpParams, sizeof(double), hipMemcpyDeviceToHost, s));
+                DZ_HIP(hipStreamSynchronize(s));
+                epoch_loss[e] = l;
+            }
+        }
+        DZ_HIP(hipMemcpyAsync(params, t.prm.ptr, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipStreamSynchronize(s));
     });
 }
 

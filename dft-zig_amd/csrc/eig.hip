@@ -153,10 +153,17 @@ struct EigWork {
     int max_sub = 0, nb = 0;
     DevBuf V, W, X, HX, R, Q, HQ, T, Tw, S, G, evals, ework, info, norms, lam, ekin, diag, sel, scl, idx;
     DevBuf x32, y32;
+    uint64_t x_cols = 0;  // columns of X holding the last solve's Ritz vectors (0: none)
+    bool x_converged = false;
+    // Learned preconditioner: residual norms of the current iteration; training snapshots
+    // (Ritz vectors, residuals, Ritz values, residual norms, <T> per band) of the last solve.
+    DevBuf rnorm, snapX, snapR, snapLam, snapRn, snapEk, bandsum, samples;
+    int snaps = 0, snap_nb = 0;
     ~EigWork() {
         if (blas) rocblas_destroy_handle(blas);
         DevBuf* bufs[] = {&V, &W, &X, &HX, &R, &Q, &HQ, &T, &Tw, &S, &G, &evals, &ework, &info,
-                          &norms, &lam, &ekin, &diag, &sel, &scl, &idx, &x32, &y32};
+                          &norms, &lam, &ekin, &diag, &sel, &scl, &idx, &x32, &y32, &rnorm,
+                          &snapX, &snapR, &snapLam, &snapRn, &snapEk, &bandsum, &samples};
         for (DevBuf* b : bufs) b->release();
     }
 };
@@ -279,34 +286,54 @@ struct Solver {
         DZ_HIP(hipGetLastError());
     }
 
-    // Orthonormal start block, its H image and T.
-    void start(int m, const double* init, uint64_t init_cols, const double* diag_host) {
-        std::vector<double2> v((size_t)n * m);
+    // Orthonormal start block, its H image and T.  Columns [0, cols) come from the host start
+    // vectors or, with reuse, from the previous solve's Ritz vectors still in X; the rest are
+    // unit vectors at the smallest diag entries or LCG values (common.zig:287-319), with the
+    // generator advanced past the columns that were supplied.
+    void start(int m, const double* init, uint64_t init_cols, bool reuse, const double* diag_host) {
+        size_t cols = 0;
+        if (init && init_cols > 0) cols = std::min<uint64_t>(init_cols, (uint64_t)m);
+        else if (reuse) cols = std::min<uint64_t>(w.x_cols, (uint64_t)m);
+        const size_t rest = (size_t)m - cols;
+        std::vector<double2> v(n * rest);
         if (o.init_diagonal) {
             std::vector<size_t> order(n);
             for (size_t i = 0; i < n; ++i) order[i] = i;
             std::stable_sort(order.begin(), order.end(),
                              [&](size_t a, size_t b) { return diag_host[a] < diag_host[b]; });
             std::fill(v.begin(), v.end(), make_double2(0.0, 0.0));
-            for (int c = 0; c < m; ++c) v[(size_t)c * n + order[c]] = make_double2(1.0, 0.0);
+            for (size_t c = 0; c < rest; ++c) v[c * n + order[cols + c]] = make_double2(1.0, 0.0);
         } else {
-            unsigned long long x = seed;
+            const unsigned long long A = 6364136223846793005ULL, C = 1ULL;
+            // x_{k+j} = Aj x_k + Cj: affine map of j steps by repeated squaring.
+            unsigned long long aj = 1, cj = 0, ap = A, cp = C;
+            for (unsigned long long j = 2ULL * n * cols; j; j >>= 1) {
+                if (j & 1) {
+                    aj = ap * aj;
+                    cj = ap * cj + cp;
+                }
+                cp = ap * cp + cp;
+                ap = ap * ap;
+            }
+            unsigned long long x = aj * seed + cj;
             auto next = [&]() {
-                x = x * 6364136223846793005ULL + 1ULL;
+                x = x * A + C;
                 return (double)(x >> 11) / 9007199254740992.0;
             };
-            for (int c = 0; c < m; ++c)
+            for (size_t c = 0; c < rest; ++c)
                 for (size_t i = 0; i < n; ++i) {
                     const double re = next() - 0.5;
                     const double im = next() - 0.5;
-                    v[(size_t)c * n + i] = make_double2(re, im);
+                    v[c * n + i] = make_double2(re, im);
                 }
         }
-        if (init && init_cols > 0) {
-            const size_t cols = std::min<uint64_t>(init_cols, (uint64_t)m);
-            std::memcpy(v.data(), init, cols * n * sizeof(double2));
-        }
-        DZ_HIP(hipMemcpyAsync(V(), v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+        if (init && init_cols > 0)
+            DZ_HIP(hipMemcpyAsync(V(), init, cols * n * sizeof(double2), hipMemcpyHostToDevice, s));
+        else if (cols > 0)
+            DZ_HIP(hipMemcpyAsync(V(), w.X.ptr, cols * n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        if (rest > 0)
+            DZ_HIP(hipMemcpyAsync(V() + cols * n, v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+        DZ_HIP(hipStreamSynchronize(s));
         cholqr(V(), m);
         cholqr(V(), m);
         apply(V(), W(), m);
@@ -363,6 +390,7 @@ static void ensure_work(Ham& h, int max_sub, int nb) {
     w->stream = h.stream();
     const size_t n = h.npw;
     const size_t c = sizeof(double2);
+    if (w->X.bytes < n * nb * c) w->x_cols = 0;  // X is reallocated below
     w->V.ensure(n * max_sub * c);
     w->W.ensure(n * max_sub * c);
     w->X.ensure(n * nb * c);
@@ -384,6 +412,7 @@ static void ensure_work(Ham& h, int max_sub, int nb) {
     w->sel.ensure(max_sub * sizeof(int));
     w->scl.ensure(max_sub * sizeof(double));
     w->idx.ensure(max_sub * sizeof(int));
+    w->rnorm.ensure(max_sub * sizeof(double));
 }
 
 void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts& o, const double* init,
@@ -391,6 +420,10 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
     const size_t n = h.npw;
     DZ_REQUIRE(nbands > 0 && nbands <= n, DZHIP_ERR_ARG, "nbands must be in 1..n");
     DZ_REQUIRE(o.precision == DZHIP_FP64 || o.precision == DZHIP_FP32, DZHIP_ERR_ARG, "bad precision");
+    DZ_REQUIRE(o.preconditioner >= DZHIP_PRECOND_KINETIC && o.preconditioner <= DZHIP_PRECOND_MLP, DZHIP_ERR_ARG,
+               "bad preconditioner");
+    DZ_REQUIRE(o.preconditioner != DZHIP_PRECOND_MLP || h.has_mlp, DZHIP_ERR_STATE,
+               "DZHIP_PRECOND_MLP without a model (dzhip_ham_set_precond_model)");
     const int nb = (int)nbands;
     const int max_sub = (int)(o.max_subspace == 0 ? std::min<uint64_t>(n, 4 * nbands + 8)
                                                    : std::min<uint64_t>(n, std::max<uint64_t>(o.max_subspace, nbands)));
@@ -414,7 +447,23 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
         S.extend_T(0, nb);
         m = nb;
     };
-    S.start(m, init, init_cols, diag);
+    const bool reuse = o.reuse_vectors != 0 && !(init && init_cols > 0) && w.x_cols > 0;
+    S.start(m, init, init_cols, reuse, diag);
+    w.x_cols = 0;
+    w.x_converged = false;
+    const int nsnap = (int)std::min<uint64_t>(o.collect_snapshots, 64);
+    if (nsnap > 0) {
+        const size_t blk = n * nb * sizeof(double2);
+        w.snapX.ensure(nsnap * blk);
+        w.snapR.ensure(nsnap * blk);
+        w.snapLam.ensure(nsnap * nb * sizeof(double));
+        w.snapRn.ensure(nsnap * nb * sizeof(double));
+        w.snapEk.ensure(nsnap * nb * sizeof(double));
+    }
+    w.snaps = 0;
+    w.snap_nb = nb;
+    int precond = o.preconditioner;
+    std::vector<double> res_hist;
     std::vector<double> lam(nb), best(nb);
     bool have_best = false;
     uint64_t iter = 0;
@@ -431,11 +480,38 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
         std::vector<double> rn = S.col_norms(w.R.as<double2>(), nb);
         const double max_res = *std::max_element(rn.begin(), rn.end());
         S.st.max_residual = max_res;
-        if (max_res < o.tol && S.fp64_apply) {
+        if (iter == 0) S.st.initial_residual = max_res;
+        res_hist.push_back(max_res);
+        if (w.snaps < nsnap && S.fp64_apply) {
+            const size_t blk = n * nb;
+            double2* sx = w.snapX.as<double2>() + w.snaps * blk;
+            double2* sr = w.snapR.as<double2>() + w.snaps * blk;
+            DZ_HIP(hipMemcpyAsync(sx, w.X.ptr, blk * sizeof(double2), hipMemcpyDeviceToDevice, s));
+            DZ_HIP(hipMemcpyAsync(sr, w.R.ptr, blk * sizeof(double2), hipMemcpyDeviceToDevice, s));
+            DZ_HIP(hipMemcpyAsync(w.snapLam.as<double>() + w.snaps * nb, lam.data(), nb * sizeof(double),
+                                  hipMemcpyHostToDevice, s));
+            DZ_HIP(hipMemcpyAsync(w.snapRn.as<double>() + w.snaps * nb, rn.data(), nb * sizeof(double),
+                                  hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_kinetic_energy, dim3(nb), dim3(256), 0, s, w.X.as<double2>(), w.diag.as<double>(), n,
+                               (int)n, w.snapEk.as<double>() + w.snaps * nb);
+            DZ_HIP(hipGetLastError());
+            DZ_HIP(hipStreamSynchronize(s));  // lam / rn are host vectors rebuilt next iteration
+            w.snaps++;
+        }
+        // Learned preconditioner: fall back to TPA for the rest of the solve when the largest
+        // residual fell by less than 10% over the last three iterations.
+        if (precond == DZHIP_PRECOND_MLP && res_hist.size() > 3 &&
+            max_res > 0.9 * res_hist[res_hist.size() - 4]) {
+            precond = DZHIP_PRECOND_TPA;
+            S.st.precond_fallback = iter + 1;
+        }
+        // FP32 apply: a tolerance looser than the switch threshold is met in FP32; tighter
+        // tolerances switch the apply to FP64 once the residual is below the threshold.
+        if (max_res < o.tol && (S.fp64_apply || o.tol >= switch_tol)) {
             S.st.converged = 1;
             break;
         }
-        if (!S.fp64_apply && (max_res < switch_tol || max_res < o.tol)) {
+        if (!S.fp64_apply && max_res < switch_tol) {
             switch_to_fp64(iter);
             continue;
         }
@@ -447,21 +523,28 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
         int a = 0;
         if (!sel.empty()) {
             DZ_HIP(hipMemcpyAsync(w.sel.ptr, sel.data(), sel.size() * sizeof(int), hipMemcpyHostToDevice, s));
-            if (o.preconditioner == DZHIP_PRECOND_TPA) {
+            if (precond != DZHIP_PRECOND_KINETIC) {
                 hipLaunchKernelGGL(k_kinetic_energy, dim3(nb), dim3(256), 0, s, w.X.as<double2>(), w.diag.as<double>(),
                                    n, (int)n, w.ekin.as<double>());
                 DZ_HIP(hipGetLastError());
             }
-            hipLaunchKernelGGL(k_precondition, S.cols_grid((int)sel.size()), dim3(256), 0, s, w.R.as<double2>(),
-                               w.sel.as<int>(), w.lam.as<double>(), w.ekin.as<double>(), w.diag.as<double>(),
-                               w.Q.as<double2>(), n, (int)n, o.preconditioner == DZHIP_PRECOND_TPA ? 1 : 0);
-            DZ_HIP(hipGetLastError());
+            if (precond == DZHIP_PRECOND_MLP) {
+                DZ_HIP(hipMemcpyAsync(w.rnorm.ptr, rn.data(), nb * sizeof(double), hipMemcpyHostToDevice, s));
+                precondition_mlp(w.R.as<double2>(), w.sel.as<int>(), (int)sel.size(), w.lam.as<double>(),
+                                 w.ekin.as<double>(), w.rnorm.as<double>(), w.diag.as<double>(), h.mlp_params.as<float>(),
+                                 h.mlp_energy_scale, w.Q.as<double2>(), n, (int)n, s);
+            } else {
+                hipLaunchKernelGGL(k_precondition, S.cols_grid((int)sel.size()), dim3(256), 0, s, w.R.as<double2>(),
+                                   w.sel.as<int>(), w.lam.as<double>(), w.ekin.as<double>(), w.diag.as<double>(),
+                                   w.Q.as<double2>(), n, (int)n, precond == DZHIP_PRECOND_TPA ? 1 : 0);
+                DZ_HIP(hipGetLastError());
+            }
             a = S.orthonormalize_new(w.Q.as<double2>(), (int)sel.size(), m);
         }
         if (a == 0) {
             // Nothing could be added: every residual is individually below tol
             // (serial.zig:262-273) or the new directions lie in the basis.
-            if (!S.fp64_apply) {
+            if (!S.fp64_apply && o.tol < switch_tol) {
                 switch_to_fp64(iter);
                 continue;
             }
@@ -482,12 +565,63 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
         m += a;
     }
     S.st.iterations = iter;
-    if (have_best) std::memcpy(values, best.data(), nb * sizeof(double));
+    if (have_best) {
+        std::memcpy(values, best.data(), nb * sizeof(double));
+        w.x_cols = nb;
+        w.x_converged = S.st.converged != 0;
+    }
     if (vectors) DZ_HIP(hipMemcpyAsync(vectors, w.X.ptr, n * nb * sizeof(double2), hipMemcpyDeviceToHost, s));
     DZ_HIP(hipStreamSynchronize(s));
     if (stats_out) *stats_out = S.st;
 }
 
 void eig_release(Ham& h) { eig_work_release(h); }
+
+size_t eig_training_samples(Ham& h, double energy_scale, int stride, float* out_host, size_t cap) {
+    auto* w = static_cast<EigWork*>(h.eig_work);
+    DZ_REQUIRE(w && w->snaps > 0, DZHIP_ERR_STATE, "no training snapshots (solve with collect_snapshots > 0)");
+    DZ_REQUIRE(w->x_converged && w->x_cols == (uint64_t)w->snap_nb, DZHIP_ERR_STATE,
+               "the recording solve did not converge");
+    DZ_REQUIRE(stride >= 1 && energy_scale > 0.0, DZHIP_ERR_ARG, "stride >= 1 and energy_scale > 0");
+    hipStream_t s = h.stream();
+    DZ_BLAS(rocblas_set_stream(w->blas, s));
+    const int nb = w->snap_nb;
+    const size_t n = h.npw, blk = n * nb;
+    const int per_col = (int)((n + stride - 1) / stride);
+    const size_t total = std::min<size_t>(cap, (size_t)w->snaps * nb * per_col);
+    if (total == 0) return 0;
+    w->samples.ensure(6 * total * sizeof(float));
+    w->bandsum.ensure(3 * nb * sizeof(double));
+    w->Q.ensure(blk * sizeof(double2));
+    w->G.ensure((size_t)nb * nb * sizeof(double2));
+    const zc one(1.0, 0.0), zero(0.0, 0.0), mone(-1.0, 0.0);
+    size_t done = 0;
+    for (int i = 0; i < w->snaps && done < total; ++i) {
+        const double2* xi = w->snapX.as<double2>() + i * blk;
+        // C = X^H x_i, E = X C - x_i (the part of x_i outside the converged subspace, negated).
+        DZ_BLAS(rocblas_zgemm(w->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, nb, nb, (int)n,
+                              &one, reinterpret_cast<const zc*>(w->X.ptr), (int)n, reinterpret_cast<const zc*>(xi),
+                              (int)n, &zero, w->G.as<zc>(), nb));
+        DZ_HIP(hipMemcpyAsync(w->Q.ptr, xi, blk * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        DZ_BLAS(rocblas_zgemm(w->blas, rocblas_operation_none, rocblas_operation_none, (int)n, nb, nb, &one,
+                              reinterpret_cast<const zc*>(w->X.ptr), (int)n, w->G.as<zc>(), nb, &mone,
+                              w->Q.as<zc>(), (int)n));
+        done += mlp_write_samples(w->Q.as<double2>(), w->snapR.as<double2>() + i * blk, w->snapLam.as<double>() + i * nb,
+                                  w->snapEk.as<double>() + i * nb, w->snapRn.as<double>() + i * nb,
+                                  w->diag.as<double>(), n, (int)n, nb, stride, energy_scale, w->bandsum.as<double>(),
+                                  w->samples.as<float>(), total, done, s);
+    }
+    for (int f = 0; f < 6; ++f)
+        DZ_HIP(hipMemcpyAsync(out_host + (size_t)f * cap, w->samples.as<float>() + (size_t)f * total,
+                              done * sizeof(float), hipMemcpyDeviceToHost, s));
+    DZ_HIP(hipStreamSynchronize(s));
+    return done;
+}
+
+const double2* eig_solution(const Ham& h, uint64_t* cols) {
+    const auto* w = static_cast<const EigWork*>(h.eig_work);
+    *cols = w ? w->x_cols : 0;
+    return (w && w->x_cols > 0) ? w->X.as<double2>() : nullptr;
+}
 
 }  // namespace dzhip

@@ -68,11 +68,45 @@ struct Ham {
     uint64_t calls = 0;
     void collect_profile();
 
-    // Eigensolver workspace (created on first solve).
+    // Eigensolver workspace (created on first solve); it keeps the last solve's Ritz vectors.
     void* eig_work = nullptr;
+    // Learned preconditioner model (kMlpParams floats) and its feature energy scale (Ha).
+    DevBuf mlp_params;
+    double mlp_energy_scale = 0.0;
+    bool has_mlp = false;
+    // Density staging (grow-only).
+    DevBuf dens_rho, dens_w;
 
     size_t device_bytes() const;
     std::mutex mu;
+};
+
+// ----------------------------------------------------------------- learned preconditioner
+constexpr int kMlpHidden = 32;
+constexpr int kMlpParams = 3 * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden + kMlpHidden + 1;
+
+// Q[:, j] = s(G, sel_j) R[:, sel_j] with the network's scaling (device pointers).
+void precondition_mlp(const double2* r, const int* sel, int nsel, const double* lam, const double* ekin,
+                      const double* rnorm, const double* diag, const float* prm, double energy_scale, double2* q,
+                      size_t ld, int n, hipStream_t s);
+// Training samples (SoA [6][cap] floats: x1 x2 x3 kx target weight) of `cols` residual/correction
+// column pairs, every `stride`-th coefficient, written from index base; returns how many.
+// bandsum: 3 * cols doubles of device scratch.
+size_t mlp_write_samples(const double2* e, const double2* r, const double* lam, const double* ekin,
+                         const double* rnorm, const double* diag, size_t ld, int n, int cols, int stride,
+                         double energy_scale, double* bandsum, float* out, size_t cap, size_t base, hipStream_t s);
+
+// Device state of one training run: samples (SoA [6][cap]), parameters, Adam moments, gradient.
+struct MlpTrainer {
+    size_t cap;
+    DevBuf samples, prm, m, v, g, part;
+    explicit MlpTrainer(size_t cap);
+    ~MlpTrainer();
+    // g[0..kMlpParams) = gradient, g[kMlpParams] = loss of the minibatch of `count` samples
+    // idx(j) = (mul (start + j) + add) mod n.
+    double loss_grad(size_t n, unsigned long long mul, unsigned long long add, size_t start, size_t count,
+                     hipStream_t s);
+    void adam(int step, double lr, hipStream_t s);
 };
 
 // rho[r] (+)= inv_volume * sum_b w_b |psi_b(r)|^2 (FP64), psi and rho on the device.
@@ -81,6 +115,11 @@ void ham_density_dev(Ham& h, const double2* psi, size_t nbands, const double* wt
 void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts& o, const double* init,
                uint64_t init_cols, double* values, double* vectors, dzhip_eig_stats* stats);
 void eig_release(Ham& h);
+// Ritz vectors [npw x cols] (FP64, device) the last eig_solve on h left, or nullptr.
+const double2* eig_solution(const Ham& h, uint64_t* cols);
+// Training samples from the snapshots the last solve recorded (opts.collect_snapshots) against
+// its converged Ritz vectors; host output SoA [6][cap]; returns the count written.
+size_t eig_training_samples(Ham& h, double energy_scale, int stride, float* out_host, size_t cap);
 
 void ham_build(Ham& h, const dzhip_ham_desc& d);
 void ham_set_local(Ham& h, const double* local_r);

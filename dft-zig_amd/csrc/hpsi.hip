@@ -283,15 +283,19 @@ template <> struct Mfma<double> {
     static __device__ __forceinline__ int row(int k4, int r) { return k4 + 4 * r; }
 };
 
-// Workgroup tiles of the nonlocal GEMMs: 5 wavefronts, each owning 32 bands, so one
-// workgroup covers a 160-band block and each projector row is read once per K-chunk.
-constexpr int kNlWaves = 5;
+// Workgroup tiles of the nonlocal GEMMs: 10 wavefronts = 5 band groups of 32 bands x 2 halves
+// of the projector (projection) or G (back-projection) tile, so one workgroup covers a 160-band
+// block, each projector row is read once per K-chunk, and two workgroups fit on a CU with the
+// wavefronts spread evenly over its four SIMDs.
+constexpr int kNlBandGroups = 5;
+constexpr int kNlWaves = 2 * kNlBandGroups;
 constexpr int kNlThreads = 64 * kNlWaves;
-constexpr int kNlBands = 32 * kNlWaves;  // bands per workgroup tile
+constexpr int kNlBands = 32 * kNlBandGroups;  // bands per workgroup tile
 constexpr int kNlK = 32;                 // K (G or p) per LDS stage
 constexpr int kNlPitch = kNlK + 1;       // odd pitches (in complex elements) against bank conflicts
-// 16-wide MFMA blocks per wavefront along the projector (projection) and G (back-projection)
-// axes: 4 in FP32; 2 in FP64, whose accumulators are twice as wide (keeps them in registers).
+// 16-wide MFMA blocks per workgroup tile along the projector (projection) and G
+// (back-projection) axes, split over the two wavefront halves: 4 in FP32; 2 in FP64, whose
+// accumulators are twice as wide.
 template <typename T> struct NlTile { static constexpr int kBlocks = 4; };
 template <> struct NlTile<double> { static constexpr int kBlocks = 2; };
 template <typename T> constexpr int nl_p_tile() { return 16 * NlTile<T>::kBlocks; }  // projectors / tile
@@ -326,7 +330,7 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::t
     extern __shared__ unsigned char smem_raw[];
     C* xs = reinterpret_cast<C*>(smem_raw);  // [kNlBands][kNlPitch]
     constexpr int kNlP = nl_p_tile<T>();
-    constexpr int PB = NlTile<T>::kBlocks;
+    constexpr int PB = NlTile<T>::kBlocks / 2;  // projector blocks per wavefront
     C* as = xs + kNlBands * kNlPitch;        // [kNlP][kNlPitch]
     const int lin = xcd_tile(blockIdx.x, gridDim.x);
     const int pt = lin % ptiles, ks = lin / ptiles;
@@ -335,6 +339,7 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::t
     const int k0 = ks * kchunk;
     const int k1 = min(npw, k0 + kchunk);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int bg = wave % kNlBandGroups, half = wave / kNlBandGroups;
     const int i16 = lane & 15, k4 = lane >> 4;
     const C zero = C{T(0), T(0)};
     constexpr int kXPer = (kNlBands * kNlK + kNlThreads - 1) / kNlThreads;
@@ -388,9 +393,9 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::t
         for (int kk = 0; kk < kNlK; kk += 4) {
             C xa[2], ab[PB];
 #pragma unroll
-            for (int bb = 0; bb < 2; ++bb) xa[bb] = xs[(wave * 32 + bb * 16 + i16) * kNlPitch + kk + k4];
+            for (int bb = 0; bb < 2; ++bb) xa[bb] = xs[(bg * 32 + bb * 16 + i16) * kNlPitch + kk + k4];
 #pragma unroll
-            for (int pb = 0; pb < PB; ++pb) ab[pb] = as[(pb * 16 + i16) * kNlPitch + kk + k4];
+            for (int pb = 0; pb < PB; ++pb) ab[pb] = as[((half * PB + pb) * 16 + i16) * kNlPitch + kk + k4];
 #pragma unroll
             for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
@@ -416,8 +421,8 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::t
         for (int pb = 0; pb < PB; ++pb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int b = b0 + wave * 32 + bb * 16 + M::row(k4, r);
-                const int p = p0 + pb * 16 + i16;
+                const int b = b0 + bg * 32 + bb * 16 + M::row(k4, r);
+                const int p = p0 + (half * PB + pb) * 16 + i16;
                 if (b < ncols && p < P)
                     partial[((size_t)ks * ncols + b) * P + p] = C{accr[bb][pb][r], acci[bb][pb][r]};
             }
@@ -471,11 +476,12 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::t
     C* qs = reinterpret_cast<C*>(smem_raw);  // [kNlBands][kNlPitch]  (band, p)
     constexpr int kNlG = nl_g_tile<T>();
     constexpr int kNlGPitch = kNlG + 1;
-    constexpr int GB = NlTile<T>::kBlocks;
+    constexpr int GB = NlTile<T>::kBlocks / 2;  // G blocks per wavefront
     C* as = qs + kNlBands * kNlPitch;        // [kNlK][kNlGPitch]     (p, G)
     const int g0 = blockIdx.x * kNlG;
     const int b0 = blockIdx.z * kNlBands;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int bg = wave % kNlBandGroups, half = wave / kNlBandGroups;
     const int i16 = lane & 15, k4 = lane >> 4;
     const C zero = C{T(0), T(0)};
     constexpr int kQPer = (kNlBands * kNlK + kNlThreads - 1) / kNlThreads;
@@ -527,9 +533,9 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::t
         for (int kk = 0; kk < kNlK; kk += 4) {
             C qa[2], ab[GB];
 #pragma unroll
-            for (int bb = 0; bb < 2; ++bb) qa[bb] = qs[(wave * 32 + bb * 16 + i16) * kNlPitch + kk + k4];
+            for (int bb = 0; bb < 2; ++bb) qa[bb] = qs[(bg * 32 + bb * 16 + i16) * kNlPitch + kk + k4];
 #pragma unroll
-            for (int gb = 0; gb < GB; ++gb) ab[gb] = as[(kk + k4) * kNlGPitch + gb * 16 + i16];
+            for (int gb = 0; gb < GB; ++gb) ab[gb] = as[(kk + k4) * kNlGPitch + (half * GB + gb) * 16 + i16];
 #pragma unroll
             for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
@@ -555,8 +561,8 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::t
         for (int gb = 0; gb < GB; ++gb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int b = b0 + wave * 32 + bb * 16 + M::row(k4, r);
-                const int g = g0 + gb * 16 + i16;
+                const int b = b0 + bg * 32 + bb * 16 + M::row(k4, r);
+                const int g = g0 + (half * GB + gb) * 16 + i16;
                 if (b < ncols && g < npw) y[(size_t)b * ldy + g] = C{accr[bb][gb][r], acci[bb][gb][r]};
             }
 }
@@ -789,7 +795,8 @@ void ham_release(Ham& h) {
     DevBuf* bufs[] = {&h.plane_start, &h.plane_pw, &h.plane_pos, &h.plane_xpos, &h.plane_ystart,
                       &h.plane_ycols, &h.plane_xabsent, &h.kinetic64,
                       &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.block_rows, &h.dm64,
-                      &h.dm32, &h.xstage, &h.ystage, &h.x32, &h.y32, &h.tbuf, &h.partial, &h.qbuf};
+                      &h.dm32, &h.xstage, &h.ystage, &h.x32, &h.y32, &h.tbuf, &h.partial, &h.qbuf,
+                      &h.dens_rho, &h.dens_w};
     for (DevBuf* b : bufs) b->release();
     h.tab.release();
     for (auto& set : h.ev_sets)

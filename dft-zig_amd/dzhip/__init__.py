@@ -33,6 +33,8 @@ EXPORTS = (
     "dzhip_ham_apply_dev", "dzhip_ham_apply_nonlocal", "dzhip_ham_apply_overlap",
     "dzhip_ham_set_stream", "dzhip_ham_sync", "dzhip_ham_profile", "dzhip_ham_phase_times",
     "dzhip_ham_info", "dzhip_eig_solve", "dzhip_density_accumulate",
+    "dzhip_density_accumulate_solution", "dzhip_ham_set_precond_model", "dzhip_ham_training_samples",
+    "dzhip_mlp_loss_grad", "dzhip_mlp_train",
 )
 
 
@@ -64,6 +66,8 @@ _ERRS = {-1: InvalidMatrixSize, -2: HipFailure, -3: OutOfMemory, -4: BadState, -
 
 PRECOND_KINETIC = 0
 PRECOND_TPA = 1
+PRECOND_MLP = 2
+MLP_PARAMS = 1217
 
 
 class EigOpts(ct.Structure):
@@ -71,6 +75,7 @@ class EigOpts(ct.Structure):
         ("max_iter", ct.c_uint64), ("tol", ct.c_double), ("max_subspace", ct.c_uint64),
         ("block_size", ct.c_uint64), ("init_diagonal", ct.c_int), ("precision", ct.c_int),
         ("mixed_switch_tol", ct.c_double), ("preconditioner", ct.c_int), ("seed", ct.c_uint64),
+        ("reuse_vectors", ct.c_int), ("collect_snapshots", ct.c_uint64),
     ]
 
 
@@ -78,7 +83,8 @@ class EigStats(ct.Structure):
     _fields_ = [
         ("iterations", ct.c_uint64), ("apply_calls", ct.c_uint64), ("fp32_applies", ct.c_uint64),
         ("fp64_applies", ct.c_uint64), ("restarts", ct.c_uint64), ("precision_switch_iter", ct.c_uint64),
-        ("converged", ct.c_uint64), ("max_residual", ct.c_double),
+        ("converged", ct.c_uint64), ("max_residual", ct.c_double), ("initial_residual", ct.c_double),
+        ("precond_fallback", ct.c_uint64),
     ]
 
     def as_dict(self):
@@ -116,6 +122,7 @@ def _load():
         pass
     lib = ct.CDLL(str(LIB_PATH))
     vp, u64, i32, dp = ct.c_void_p, ct.c_uint64, ct.c_int, ct.POINTER(ct.c_double)
+    fp = ct.POINTER(ct.c_float)
     sig = {
         "dzhip_version": ([], i32), "dzhip_last_error": ([], ct.c_char_p),
         "dzhip_ctx_create": ([i32], vp), "dzhip_ctx_destroy": ([vp], None),
@@ -133,7 +140,15 @@ def _load():
         "dzhip_ham_info": ([vp, ct.POINTER(ct.c_uint64), u64], i32),
         "dzhip_eig_solve": ([vp, dp, u64, u64, ct.POINTER(EigOpts), dp, u64, dp, dp, ct.POINTER(EigStats)], i32),
         "dzhip_density_accumulate": ([vp, dp, u64, u64, dp, dp, u64, i32], i32),
+        "dzhip_density_accumulate_solution": ([vp, u64, dp, dp, u64, i32], i32),
+        "dzhip_ham_set_precond_model": ([vp, fp, u64, ct.c_double], i32),
+        "dzhip_ham_training_samples": ([vp, ct.c_double, u64, fp, u64, ct.POINTER(ct.c_uint64)], i32),
+        "dzhip_mlp_loss_grad": ([vp, fp, u64, fp, dp, dp], i32),
+        "dzhip_mlp_train": ([vp, fp, u64, fp, u64, u64, ct.c_double, u64, dp], i32),
     }
+    missing = set(EXPORTS) - set(sig)
+    if missing:
+        raise ImportError(f"no ctypes signature for {sorted(missing)}")
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
         fn.argtypes = args
@@ -161,6 +176,10 @@ def check(rc: int) -> None:
         raise _ERRS.get(rc, DzhipError)(f"dzhip error {rc}: {last_error()}")
 
 
+def _fp(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_float))
+
+
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(ct.POINTER(ct.c_double))
 
@@ -181,6 +200,23 @@ class Context:
         if not self.handle:
             raise HipFailure(f"dzhip_ctx_create({device}) failed: {last_error()}")
         self.device = device
+
+    def mlp_loss_grad(self, samples, params):
+        """Full-batch loss and gradient (FP64) of the learned preconditioner on (6, n) samples."""
+        smp = np.ascontiguousarray(samples, dtype=np.float32)
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        g = np.zeros(MLP_PARAMS)
+        loss = ct.c_double(0.0)
+        check(L().dzhip_mlp_loss_grad(self.handle, _fp(smp), smp.shape[1], _fp(p), _dp(g), ct.byref(loss)))
+        return loss.value, g
+
+    def mlp_train(self, samples, params, epochs: int, batch: int = 4096, lr: float = 1e-3, seed: int = 1):
+        """Adam on the device; returns (params, loss after each epoch)."""
+        smp = np.ascontiguousarray(samples, dtype=np.float32)
+        p = np.ascontiguousarray(params, dtype=np.float32).copy()
+        hist = np.zeros(max(1, epochs))
+        check(L().dzhip_mlp_train(self.handle, _fp(smp), smp.shape[1], _fp(p), epochs, batch, lr, seed, _dp(hist)))
+        return p, hist[:epochs]
 
     def close(self):
         if self.handle:
@@ -351,15 +387,16 @@ class Hamiltonian:
     def eig_solve(self, nbands: int, diag=None, tol: float = 1e-6, max_iter: int = 40, precision: int = FP64,
                   preconditioner: int = PRECOND_KINETIC, init=None, max_subspace: int = 0, block_size: int = 0,
                   init_diagonal: bool = False, mixed_switch_tol: float = 0.0, seed: int = 0,
-                  want_vectors: bool = True):
+                  want_vectors: bool = True, reuse: bool = False, collect_snapshots: int = 0):
         """Lowest eigenpairs (hermitian_eigen_decomp_iterative counterpart).  diag defaults to the
-        kinetic energies the context was built with; init is an (init_cols, n) block of start vectors.
+        kinetic energies the context was built with; init is an (init_cols, n) block of start vectors;
+        reuse (without init) starts from the previous solve's vectors kept on the device.
         Returns (values[nbands], vectors (nbands, n) or None, stats dict)."""
         if diag is None:
             diag = self._kinetic
         diag = np.ascontiguousarray(diag, dtype=np.float64)
         o = EigOpts(max_iter, tol, max_subspace, block_size, 1 if init_diagonal else 0, precision,
-                    mixed_switch_tol, preconditioner, seed)
+                    mixed_switch_tol, preconditioner, seed, 1 if reuse else 0, collect_snapshots)
         vals = np.zeros(nbands)
         vecs = np.empty((nbands, self.n), dtype=np.complex128) if want_vectors else None
         st = EigStats()
@@ -386,6 +423,30 @@ class Hamiltonian:
         out = np.ascontiguousarray(rho, dtype=np.float64).copy() if acc else np.zeros(total)
         check(L().dzhip_density_accumulate(self.handle, _dp(psi.view(np.float64)), psi.shape[1], psi.shape[0],
                                            _dp(w), _dp(out), total, 1 if acc else 0))
+        return out
+
+    def set_precond_model(self, params, energy_scale: float):
+        """Install the learned preconditioner's parameters (MLP_PARAMS floats)."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        check(L().dzhip_ham_set_precond_model(self.handle, _fp(p), p.shape[0], float(energy_scale)))
+
+    def training_samples(self, energy_scale: float, stride: int = 1, capacity: int = 1 << 20):
+        """Residual/correction training samples of the last solve (collect_snapshots > 0), as a
+        (6, count) float32 array: x1, x2, x3, kx, target, weight."""
+        out = np.zeros((6, capacity), dtype=np.float32)
+        cnt = ct.c_uint64(0)
+        check(L().dzhip_ham_training_samples(self.handle, float(energy_scale), stride, _fp(out), capacity,
+                                             ct.byref(cnt)))
+        return np.ascontiguousarray(out[:, :cnt.value])
+
+    def density_solution(self, weights, rho=None):
+        """density() of the first len(weights) vectors of the last eig_solve, read on the device."""
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        total = int(np.prod(self.grid))
+        acc = rho is not None
+        out = np.ascontiguousarray(rho, dtype=np.float64).copy() if acc else np.zeros(total)
+        check(L().dzhip_density_accumulate_solution(self.handle, w.shape[0], _dp(w), _dp(out), total,
+                                                    1 if acc else 0))
         return out
 
     def info(self):

@@ -1,0 +1,71 @@
+"""Learned preconditioner: sample collection, training and model files (host orchestration).
+
+The network and its kernels live in the library (csrc/precond.hip, C ABI dzhip_ham_set_precond_model,
+dzhip_ham_training_samples, dzhip_mlp_train, dzhip_mlp_loss_grad).  A model is the 1217 FP32
+parameters plus the feature energy scale e_s (Ha):
+
+    s(G, b) = K_tpa(|k+G|^2 / <x_b|T|x_b>) exp(z),  z = MLP(|k+G|^2/2 / e_s, lambda_b/2 / e_s, log10|r_b| / 4)
+
+Training data are residual/correction pairs of FP64 Davidson solves that converged: for the Ritz
+vector x_b of an early iteration, the correction is the part of x_b outside the converged
+invariant subspace (negated) and the target is the per-coefficient real factor that maps the
+residual onto it, relative to the band's best TPA scale.  With a zero output layer the model is
+exactly TPA, which is where training starts.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+
+from . import FP64, MLP_PARAMS, PRECOND_TPA, Context, Hamiltonian
+
+HIDDEN = 32
+ENERGY_SCALE = 10.0  # Ha
+MODEL_DIR = Path(__file__).resolve().parent / "models"
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """Hidden layers U(+-1/sqrt(fan_in)), output layer zero (the model starts as TPA)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    w1 = rng.uniform(-1, 1, (HIDDEN, 3)) / np.sqrt(3.0)
+    b1 = rng.uniform(-1, 1, HIDDEN) / np.sqrt(3.0)
+    w2 = rng.uniform(-1, 1, (HIDDEN, HIDDEN)) / np.sqrt(HIDDEN)
+    b2 = rng.uniform(-1, 1, HIDDEN) / np.sqrt(HIDDEN)
+    p = np.concatenate([w1.ravel(), b1, w2.ravel(), b2, np.zeros(HIDDEN), [0.0]]).astype(np.float32)
+    assert p.shape[0] == MLP_PARAMS
+    return p
+
+
+def collect(ham: Hamiltonian, nbands: int, snapshots: int = 16, stride: int = 1, tol: float = 1e-9,
+            max_iter: int = 300, seed: int = 0, energy_scale: float = ENERGY_SCALE, capacity: int = 1 << 22):
+    """Samples (6, n) of one cold FP64 TPA solve of `ham` (weight-0 samples dropped)."""
+    vals, _, st = ham.eig_solve(nbands, tol=tol, max_iter=max_iter, precision=FP64, preconditioner=PRECOND_TPA,
+                                want_vectors=False, seed=seed, collect_snapshots=snapshots)
+    if not st["converged"]:
+        raise RuntimeError(f"training solve did not converge: {st}")
+    smp = ham.training_samples(energy_scale, stride, capacity)
+    return smp[:, smp[5] > 0]
+
+
+def normalise_weights(samples: np.ndarray) -> np.ndarray:
+    """Scale the weights to mean 1 (the loss is a mean over each minibatch)."""
+    out = np.array(samples, dtype=np.float32, copy=True)
+    out[5] /= max(float(np.mean(out[5])), 1e-30)
+    return out
+
+
+def train(ctx: Context, samples: np.ndarray, epochs: int = 30, batch: int = 8192, lr: float = 2e-3,
+          seed: int = 1, init: np.ndarray | None = None):
+    """Device Adam training; returns (params, loss after each epoch)."""
+    p0 = init_params(seed) if init is None else np.asarray(init, dtype=np.float32)
+    return ctx.mlp_train(normalise_weights(samples), p0, epochs, batch, lr, seed)
+
+
+def save(path, params, energy_scale=ENERGY_SCALE, **meta):
+    np.savez(path, params=np.asarray(params, dtype=np.float32), energy_scale=np.array(energy_scale), **meta)
+
+
+def load(path):
+    with np.load(path, allow_pickle=False) as d:
+        return d["params"].astype(np.float32), float(d["energy_scale"])

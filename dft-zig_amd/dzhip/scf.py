@@ -1,0 +1,426 @@
+"""SCF driver around the device eigensolver (host orchestration).
+
+Follows the reference's SCF loop (src/features/scf/scf.zig:489-521: compute_density ->
+build_potential_grid -> mix) with its defaults for the measured systems: fixed occupations
+(src/features/scf/kpoint/density.zig:98-107), LDA-PZ (src/features/xc/xc.zig:195-210, 344-384),
+Hartree (src/features/scf/potential.zig:34-44), potential mixing, linear (mixing.zig:10-18) then
+Pulay/DIIS with history 8 from iteration 4 (mixing.zig:183-234, scf.zig:147-188, defaults
+config/defaults.zig:42-45), convergence on the RMS density change (util.zig:17-24), iterative
+eigensolver schedule of kpoint/solve.zig:294-402 (warm-up: 2 SCF steps at tol 1e-3 and 10
+iterations, then tol 1e-4 and 20 iterations, vectors reused from the previous SCF step;
+config/defaults.zig:32-40).
+
+Per k-point the eigenproblem and the |psi|^2 density run on the GPU through libdzhip
+(dzhip_eig_solve with the device warm-start cache, dzhip_density_accumulate_solution), so no
+wave function crosses PCIe inside an SCF iteration; the potentials and the mixing run on the host
+in FP64 as in the reference.  Each k-point may get its own context (own HIP stream and rocBLAS
+handle) and run on its own host thread, the k-point worker scheme of kpoint/workers.zig:82-188.
+
+Systems use a GTH-form synthetic pseudopotential (no UPF files ship with the reference): local
+part v(q) = 2[-4 pi Z/q^2 + C1 (2 pi)^{3/2} rloc^3] e^{-q^2 rloc^2/2} Ry per atom, G = 0 dropped
+(hamiltonian.zig:457-461), and Gaussian s/p Kleinman-Bylander projectors with D in Ry.  Total
+energies leave out the ion-ion and psp-core constants of the structure.
+"""
+from __future__ import annotations
+
+import math
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import FP64, PRECOND_KINETIC, Context, Hamiltonian, tables
+from .workloads import SpeciesDesc
+
+HA_PER_RY = 0.5
+
+
+@dataclass
+class ScfSystem:
+    name: str
+    cell: np.ndarray
+    recip: np.ndarray
+    volume: float
+    grid: tuple
+    ecut_ry: float
+    atom_pos: np.ndarray
+    atom_species: np.ndarray
+    species: list
+    z_ion: float
+    rloc: float
+    c1: float  # Ha
+    nelec: float
+    nbands: int
+    kpoints: list  # Cartesian, bohr^-1
+    kweights: list
+
+
+# GTH-form parameters of a four-electron Si-like pseudo-atom (Hartree units for C1 and h): the
+# local part, s-projector radius and strength of the Goedecker-Teter-Hutter Si LDA set, one
+# Gaussian p projector of radius r_p with h_p = -0.3 Ha.  With these the diamond cell is an
+# insulator with 16 valence bands (degeneracies 1, 6, 6, 3 at the cubic-cell Gamma point) and a
+# gap of 0.08 Ry over a 4x4x4 mesh of the cubic cell, so fixed occupations are well defined.
+SI_GTH = {"z": 4.0, "rloc": 0.44, "c1": -7.336103, "r_s": 0.422738, "h_s": 5.906928,
+          "r_p": 0.484278, "h_p": -0.3}
+SI_A_ANGSTROM = 5.431
+DIAMOND_FRAC = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0],
+                         [.25, .25, .25], [.25, .75, .75], [.75, .25, .75], [.75, .75, .25]])
+
+
+def mp_mesh(recip, mesh):
+    """Unshifted Monkhorst-Pack points i/n (Cartesian), equal weights; without symmetry reduction
+    (what remains of the mesh for a structure with only the identity, kpoints/mesh.zig:5-47)."""
+    m1, m2, m3 = mesh
+    pts = [np.array([i / m1, j / m2, k / m3]) @ recip for i in range(m1) for j in range(m2) for k in range(m3)]
+    return pts, [1.0 / (m1 * m2 * m3)] * len(pts)
+
+
+def silicon(cells=(1, 1, 1), ecut_ry=20.0, kmesh=(1, 1, 1), nbands=None, displacement_angstrom=0.0,
+            seed=64, grid=None, kpoints=None) -> ScfSystem:
+    """Diamond Si in n1 x n2 x n3 conventional cubic cells (8 atoms each, a = 5.431 A), atoms
+    optionally displaced by N(0, displacement) (numpy PCG64(seed)); k-points from an unshifted
+    MP mesh unless given explicitly as (Cartesian points, weights)."""
+    a = SI_A_ANGSTROM * tables.ANGSTROM_TO_BOHR
+    n1, n2, n3 = cells
+    cell = np.diag([a * n1, a * n2, a * n3])
+    recip = tables.reciprocal(cell)
+    pos = np.array([(f + np.array([i, j, k])) * a for i in range(n1) for j in range(n2) for k in range(n3)
+                    for f in DIAMOND_FRAC])
+    if displacement_angstrom > 0:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        pos = pos + rng.normal(0.0, displacement_angstrom * tables.ANGSTROM_TO_BOHR, size=pos.shape)
+    g = SI_GTH
+    sp = SpeciesDesc([0, 1], [g["r_s"], g["r_p"]], np.diag([2.0 * g["h_s"], 2.0 * g["h_p"]]))
+    nat = pos.shape[0]
+    nelec = g["z"] * nat
+    kpts, kw = kpoints if kpoints is not None else mp_mesh(recip, kmesh)
+    grid = tuple(grid) if grid is not None else tuple(tables.auto_grid(ecut_ry, recip))
+    nb = nbands or int(math.ceil(nelec / 2.0))
+    return ScfSystem(f"si{nat}", cell, recip, abs(float(np.linalg.det(cell))), grid, ecut_ry, pos,
+                     np.zeros(nat, dtype=np.int32), [sp], g["z"], g["rloc"], g["c1"], nelec, nb,
+                     [np.asarray(k, dtype=np.float64) for k in kpts], list(kw))
+
+
+def si8_small(ecut_ry=8.0, nbands=20):
+    """Small SCF parity case: the 8-atom cubic cell at 8 Ry (24^3 grid, about 420 plane waves),
+    k = Gamma and (1/4, 1/4, 1/4) in reciprocal-lattice units, weights 1/2 each."""
+    a = SI_A_ANGSTROM * tables.ANGSTROM_TO_BOHR
+    recip = tables.reciprocal(np.diag([a, a, a]))
+    kpts = [np.zeros(3), np.array([0.25, 0.25, 0.25]) @ recip]
+    return silicon((1, 1, 1), ecut_ry, nbands=nbands, kpoints=(kpts, [0.5, 0.5]))
+
+
+def si64(ecut_ry=30.0, nbands=160):
+    """2x2x2 supercell of the cubic cell, atoms displaced by N(0, 0.02 A), unshifted 2x2x2 mesh
+    (8 k-points: the displacements leave only the identity), 160 bands."""
+    return silicon((2, 2, 2), ecut_ry, (2, 2, 2), nbands, displacement_angstrom=0.02, seed=64)
+
+
+# ----------------------------------------------------------------- grid potentials (host, FP64)
+def _freqs(n):
+    i = np.arange(n)
+    return np.where(i <= (n - 1) // 2, i, i - n)
+
+
+def g_vectors(grid, recip):
+    """Cartesian G and |G|^2 on the FFT-order grid (index x + nx (y + ny z), wrapped frequencies)."""
+    nx, ny, nz = grid
+    fz, fy, fx = np.meshgrid(_freqs(nz), _freqs(ny), _freqs(nx), indexing="ij")
+    g = (fx.reshape(-1, 1) * recip[0] + fy.reshape(-1, 1) * recip[1] + fz.reshape(-1, 1) * recip[2])
+    return g, np.einsum("ij,ij->i", g, g)
+
+
+def to_g(v_r, grid):
+    nx, ny, nz = grid
+    return np.fft.fftn(v_r.reshape(nz, ny, nx)).reshape(-1)
+
+
+def to_r(v_g, grid):
+    nx, ny, nz = grid
+    return np.fft.ifftn(v_g.reshape(nz, ny, nx)).reshape(-1)
+
+
+def lda_pz(rho):
+    """Energy density f (Ry per volume) and potential df/drho (Ry), Perdew-Zunger 1981 fit of
+    Ceperley-Alder correlation plus Slater exchange; zero where rho <= 1e-12."""
+    rho = np.asarray(rho, dtype=np.float64)
+    f = np.zeros_like(rho)
+    v = np.zeros_like(rho)
+    on = rho > 1e-12
+    n = rho[on]
+    cbrt_n = np.cbrt(n)
+    cx = -0.75 * (3.0 / math.pi) ** (1.0 / 3.0)
+    rs = np.cbrt(3.0 / (4.0 * math.pi * n))
+    dense = rs < 1.0
+    ec = np.empty_like(n)
+    vc = np.empty_like(n)
+    r = rs[dense]
+    ln = np.log(r)
+    ec[dense] = 0.0311 * ln - 0.048 + 0.0020 * r * ln - 0.0116 * r
+    vc[dense] = ec[dense] - (r / 3.0) * (0.0311 / r + 0.0020 * ln + 0.0020 - 0.0116)
+    r = rs[~dense]
+    sq = np.sqrt(r)
+    den = 1.0 + 1.0529 * sq + 0.3334 * r
+    ec[~dense] = -0.1423 / den
+    vc[~dense] = ec[~dense] - (r / 3.0) * (0.1423 * (0.5 * 1.0529 / sq + 0.3334) / (den * den))
+    f[on] = 2.0 * n * (cx * cbrt_n + ec)
+    v[on] = 2.0 * (4.0 / 3.0 * cx * cbrt_n + vc)
+    return f, v
+
+
+def hartree(rho, grid, g2, volume):
+    """V_H(r) (Ry, V(G) = 8 pi rho(G)/G^2, G = 0 dropped) and E_H = (1/2) int V_H rho."""
+    n = rho.shape[0]
+    rg = to_g(rho, grid)
+    safe = np.where(g2 > 1e-12, g2, 1.0)
+    vh = to_r(np.where(g2 > 1e-12, rg * (8.0 * math.pi) / safe, 0.0), grid).real
+    return vh, 0.5 * float(np.dot(vh, rho)) * volume / n
+
+
+def ionic_potential(system, g, g2):
+    """V_ion(r) = sum_G e^{iG.r} (1/Omega) sum_a e^{-iG.tau_a} v(|G|), v of the GTH local form."""
+    s = system
+    sf = np.zeros(g.shape[0], dtype=np.complex128)
+    for tau in s.atom_pos:
+        sf += np.exp(-1j * (g @ tau))
+    vq = np.zeros(g.shape[0])
+    nz = g2 > 1e-20
+    q2 = g2[nz]
+    vq[nz] = 2.0 * (-4.0 * math.pi * s.z_ion / q2 + s.c1 * (2.0 * math.pi) ** 1.5 * s.rloc ** 3) \
+        * np.exp(-0.5 * q2 * s.rloc ** 2)
+    return to_r(sf * vq * (g.shape[0] / s.volume), s.grid).real
+
+
+def band_weights(nelec, nbands, kweight):
+    """w_b = w_k occ_b 2: the first ceil(nelec/2) bands are occupied, the last one partially."""
+    nocc = int(math.ceil(nelec / 2.0))
+    w = np.zeros(nbands)
+    for b in range(min(nocc, nbands)):
+        w[b] = 2.0 * kweight * ((nelec / 2.0 - (nocc - 1)) if b + 1 == nocc else 1.0)
+    return w
+
+
+class Pulay:
+    """DIIS on (V_in, R = V_out - V_in): keep the last `history` pairs, linear step while fewer
+    than two, else V = sum_i c_i (V_i + beta R_i) with sum_i c_i = 1 minimising |sum_i c_i R_i|
+    (mixing.zig:183-234); a singular system gives equal weights (mixing.zig:340-400).
+
+    Two deliberate departures keep the iteration converging below 1e-6 Ry residuals.  The
+    overlap matrix is divided by its largest diagonal entry before the elimination: the
+    reference tests its pivots against an absolute 1e-12, so once <R|R> (its G-space norm, the
+    real-space sum / N) is below about 1e-12 it takes the equal-weights branch on every step.
+    And a singular system keeps only the newest pair and takes the linear step, where the
+    reference averages the whole history.  Whenever the reference's system is regular the
+    coefficients are the same (they are invariant under a scaling of the matrix)."""
+
+    def __init__(self, history: int):
+        self.history = history
+        self.hist_v: list = []
+        self.hist_r: list = []
+
+    def step(self, v_in, resid, beta):
+        self.hist_v.append(np.array(v_in))
+        self.hist_r.append(np.array(resid))
+        if len(self.hist_v) > self.history:
+            del self.hist_v[0], self.hist_r[0]
+        m = len(self.hist_v)
+        if m < 2:
+            return v_in + beta * resid
+        r = np.stack(self.hist_r)
+        a = np.zeros((m + 1, m + 1))
+        a[:m, :m] = r @ r.T
+        a[:m, :m] /= max(float(np.max(np.diag(a)[:m])), 1e-300)
+        a[:m, m] = -1.0
+        a[m, :m] = -1.0
+        rhs = np.zeros(m + 1)
+        rhs[m] = -1.0
+        c = _gauss_solve(a, rhs)
+        if c is None:
+            del self.hist_v[:-1], self.hist_r[:-1]
+            return v_in + beta * resid
+        out = np.zeros_like(v_in)
+        for i in range(m):
+            out += c[i] * (self.hist_v[i] + beta * self.hist_r[i])
+        return out
+
+
+def _gauss_solve(a, b):
+    """Gaussian elimination with partial pivoting; None when a pivot is below 1e-12."""
+    a = a.copy()
+    b = b.copy()
+    size = a.shape[0]
+    for col in range(size):
+        p = col + int(np.argmax(np.abs(a[col:, col])))
+        if p != col:
+            a[[col, p]] = a[[p, col]]
+            b[[col, p]] = b[[p, col]]
+        if abs(a[col, col]) < 1e-12:
+            return None
+        f = a[col + 1:, col] / a[col, col]
+        a[col + 1:, col:] -= np.outer(f, a[col, col:])
+        b[col + 1:] -= f * b[col]
+    x = np.zeros(size)
+    for i in range(size - 1, -1, -1):
+        x[i] = (b[i] - a[i, i + 1:] @ x[i + 1:]) / a[i, i]
+    return x
+
+
+# ----------------------------------------------------------------- driver
+@dataclass
+class ScfRecord:
+    iteration: int
+    wall_s: float  # since the start of the first iteration
+    energy_ry: float
+    band_ry: float
+    drho: float
+    eig_iterations: int
+    applies: int
+    fp32_applies: int
+    eig_s: float  # wall time inside the k-point solves and densities
+
+
+@dataclass
+class ScfRun:
+    records: list = field(default_factory=list)
+    eigenvalues: list = field(default_factory=list)
+    rho: np.ndarray | None = None
+    v_in: np.ndarray | None = None
+    converged: bool = False
+    setup_s: float = 0.0
+
+    @property
+    def energies(self):
+        return [r.energy_ry for r in self.records]
+
+    def time_to(self, e_ref_ry: float, tol_ha: float = 1e-6):
+        """Wall time to the end of the first iteration after which every iteration's total energy
+        stays within tol (Ha) of e_ref; None if the run never gets there."""
+        first = None
+        for i, r in enumerate(self.records):
+            if abs(r.energy_ry - e_ref_ry) * HA_PER_RY <= tol_ha:
+                first = i if first is None else first
+            else:
+                first = None
+        return None if first is None else self.records[first].wall_s
+
+
+class ScfDriver:
+    """One Hamiltonian context per k-point, built once and kept across SCF iterations."""
+
+    def __init__(self, system: ScfSystem, concurrent_k: bool = False, device: int = 0):
+        t0 = time.perf_counter()
+        s = self.sys = system
+        self.concurrent = concurrent_k and len(s.kpoints) > 1
+        self.ctxs = [Context(device) for _ in s.kpoints] if self.concurrent else [Context(device)]
+        self.g, self.g2 = g_vectors(s.grid, s.recip)
+        self.v_ion = ionic_potential(s, self.g, self.g2)
+        zero = np.zeros(int(np.prod(s.grid)))
+        self.hams = []
+        for i, kp in enumerate(s.kpoints):
+            b = tables.generate_basis(s.recip, s.ecut_ry, kp)
+            species = [{"l_list": sp.l_list, "dij": sp.dij,
+                        "phi": tables.build_phi(b.kpg, sp.l_list, sp.radial_fns())} for sp in s.species]
+            ctx = self.ctxs[i if self.concurrent else 0]
+            self.hams.append(Hamiltonian(ctx, s.grid, tables.fft_indices(b, s.grid), b.kinetic, b.cart, zero,
+                                         1.0 / s.volume, species, s.atom_pos, s.atom_species))
+        self.npw = [h.n for h in self.hams]
+        self.pool = ThreadPoolExecutor(len(self.hams)) if self.concurrent else None
+        self.setup_s = time.perf_counter() - t0
+
+    def _solve_k(self, k, local_r, nb, opts, weights):
+        ham = self.hams[k]
+        ham.set_local(local_r)
+        vals, _, st = ham.eig_solve(nb, want_vectors=False, reuse=True, **opts)
+        rho_k = ham.density_solution(weights)
+        return vals, rho_k, st
+
+    def run(self, max_iter=60, beta=0.3, conv=1e-9, precision=FP64, preconditioner=PRECOND_KINETIC,
+            pulay_history=8, pulay_start=4, eig_tol=1e-4, eig_max_iter=20, warmup_steps=2, warmup_tol=1e-3,
+            warmup_max_iter=10, mixed_switch_tol=0.0, final_tol=None, energy_tol_ry=None, eig_tol_ratio=None,
+            eig_tol_min=1e-8, progress=None):
+        """SCF from the uniform start density nelec/Omega.  Stops when the RMS density change is
+        below conv, or (with energy_tol_ry) when two successive total energies after the warm-up
+        differ by less than energy_tol_ry.
+
+        eig_tol_ratio (None = the reference's fixed eig_tol) tightens the eigensolver tolerance with
+        the SCF: tol = min(eig_tol, max(eig_tol_min, eig_tol_ratio * drho of the previous step)).
+        A fixed 1e-4 Ry residual leaves band-energy errors of order N_b r^2 / gap, about 1e-5 Ry
+        for 128 bands across a 0.08 Ry gap, which a converged total energy cannot carry.
+        progress, if given, is called with each ScfRecord."""
+        s = self.sys
+        n = int(np.prod(s.grid))
+        vol = s.volume
+        rho = np.full(n, s.nelec / vol)
+        v_in = hartree(rho, s.grid, self.g2, vol)[0] + lda_pz(rho)[1]
+        mixer = Pulay(pulay_history) if pulay_history > 0 else None
+        weights = [band_weights(s.nelec, s.nbands, wk) for wk in s.kweights]
+        run = ScfRun(setup_s=self.setup_s)
+        t0 = time.perf_counter()
+        drho_prev = None
+        for it in range(max_iter):
+            warm = it < warmup_steps
+            tol = warmup_tol if warm else eig_tol
+            if not warm and eig_tol_ratio is not None and drho_prev is not None:
+                tol = min(eig_tol, max(eig_tol_min, eig_tol_ratio * drho_prev))
+            opts = dict(tol=tol, max_iter=warmup_max_iter if warm else eig_max_iter,
+                        precision=precision, preconditioner=preconditioner, mixed_switch_tol=mixed_switch_tol)
+            local_r = self.v_ion + v_in
+            te = time.perf_counter()
+            ks = range(len(self.hams))
+            if self.pool is not None:
+                res = list(self.pool.map(lambda k: self._solve_k(k, local_r, s.nbands, opts, weights[k]), ks))
+            else:
+                res = [self._solve_k(k, local_r, s.nbands, opts, weights[k]) for k in ks]
+            eig_s = time.perf_counter() - te
+            rho_out = np.sum([r[1] for r in res], axis=0)
+            e_band = float(sum(np.dot(w, r[0]) for w, r in zip(weights, res)))
+            vh_out, e_h = hartree(rho_out, s.grid, self.g2, vol)
+            f_xc, vxc_out = lda_pz(rho_out)
+            e_xc = float(np.sum(f_xc)) * vol / n
+            # Kohn-Sham energy of rho_out at the input potential (equals the reference's
+            # band + double counting at self-consistency).
+            e_tot = e_band - float(np.dot(v_in, rho_out)) * vol / n + e_h + e_xc
+            drho = math.sqrt(float(np.mean((rho_out - rho) ** 2)))
+            run.records.append(ScfRecord(it, 0.0, e_tot, e_band, drho,
+                                         sum(r[2]["iterations"] for r in res), sum(r[2]["apply_calls"] for r in res),
+                                         sum(r[2]["fp32_applies"] for r in res), eig_s))
+            run.eigenvalues = [r[0] for r in res]
+            rho = rho_out
+            drho_prev = drho
+            done = drho < conv
+            if energy_tol_ry is not None and not warm and len(run.records) > 1:
+                done = done or abs(run.records[-1].energy_ry - run.records[-2].energy_ry) < energy_tol_ry
+            if not done:
+                v_out = vh_out + vxc_out
+                if mixer is not None and it >= pulay_start:
+                    v_in = mixer.step(v_in, v_out - v_in, beta)
+                else:
+                    v_in = v_in + beta * (v_out - v_in)
+            run.records[-1].wall_s = time.perf_counter() - t0
+            if progress is not None:
+                progress(run.records[-1])
+            if done:
+                run.converged = True
+                break
+        run.rho = rho
+        run.v_in = v_in
+        if final_tol is not None:
+            # Tight FP64 eigenvalues on the last input potential.
+            run.eigenvalues = []
+            for ham in self.hams:
+                vals, _, _ = ham.eig_solve(s.nbands, tol=final_tol, max_iter=400, precision=FP64,
+                                           preconditioner=preconditioner, want_vectors=False, reuse=True)
+                run.eigenvalues.append(vals)
+        return run
+
+    def close(self):
+        if self.pool is not None:
+            self.pool.shutdown()
+            self.pool = None
+        for h in self.hams:
+            h.close()
+        self.hams = []
+        for c in self.ctxs:
+            c.close()
+        self.ctxs = []

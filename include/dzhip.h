@@ -160,10 +160,14 @@ int dzhip_ham_phase_times(dzhip_ham* ham, double* out_ms, uint64_t n_out, uint64
  * max_b ||H x_b - lambda_b x_b|| < tol (Ry), kinetic-shift preconditioner, LCG start with seed
  * 0x6f3d3cbe.  Orthonormalisation is block CGS2 + Cholesky QR, Rayleigh-Ritz is a device
  * zheevd, restarts keep [Ritz vectors, new block].  All subspace algebra is FP64; with
- * precision = DZHIP_FP32 the H apply runs in FP32 until the largest residual is below
- * mixed_switch_tol, then the basis restarts with an FP64 apply. */
+ * precision = DZHIP_FP32 the H apply runs in FP32; when tol < mixed_switch_tol the basis
+ * restarts with an FP64 apply once the largest residual is below mixed_switch_tol (looser
+ * tolerances are met in FP32). */
 #define DZHIP_PRECOND_KINETIC 0 /* r / (|k+G|^2 - lambda), |den| >= 1e-8 (common.zig:214-226) */
 #define DZHIP_PRECOND_TPA 1     /* Teter-Payne-Allan K(x) r, x = |k+G|^2 / <x_b|T|x_b> */
+#define DZHIP_PRECOND_MLP 2     /* learned: K_tpa(x) exp(MLP(features)) r, model set with
+                                   dzhip_ham_set_precond_model; falls back to TPA within a solve
+                                   when the residual stops decreasing (stats.precond_fallback) */
 
 typedef struct dzhip_eig_opts {
     uint64_t max_iter;       /* Options.max_iter (reference default 40) */
@@ -175,6 +179,11 @@ typedef struct dzhip_eig_opts {
     double mixed_switch_tol; /* residual below which the FP32 apply switches to FP64; 0 = 1e-5 */
     int preconditioner;      /* DZHIP_PRECOND_* */
     uint64_t seed;           /* LCG seed; 0 = 0x6f3d3cbe */
+    int reuse_vectors;       /* with init_vecs NULL: start from the Ritz vectors of this context's
+                                previous solve, kept on the device (the warm-start cache of
+                                kpoint/solve.zig:542-557, iterative_reuse_vectors) */
+    uint64_t collect_snapshots; /* record (Ritz vectors, residuals) of the first this-many FP64
+                                   iterations for dzhip_ham_training_samples; 0 = off */
 } dzhip_eig_opts;
 
 typedef struct dzhip_eig_stats {
@@ -185,6 +194,9 @@ typedef struct dzhip_eig_stats {
     uint64_t precision_switch_iter; /* iteration at which the FP32 apply switched to FP64 (0: never) */
     uint64_t converged;             /* 1 if max residual < tol at return */
     double max_residual;
+    double initial_residual;        /* largest residual of the start block */
+    uint64_t precond_fallback;      /* DZHIP_PRECOND_MLP: iteration at which the solve fell back
+                                       to TPA (0: never) */
 } dzhip_eig_stats;
 
 /* diag: [n] preconditioner diagonal (|k+G|^2, kpoint/solve.zig:362-367).  init_vecs: NULL or
@@ -203,6 +215,43 @@ int dzhip_eig_solve(dzhip_ham* ham, const double* diag, uint64_t n, uint64_t nba
  * the band.  accumulate = 0 overwrites rho.  Host pointers, blocking. */
 int dzhip_density_accumulate(dzhip_ham* ham, const double* psi, uint64_t n, uint64_t nbands,
                              const double* weights, double* rho, uint64_t count, int accumulate);
+
+/* The same accumulation for the vectors the last dzhip_eig_solve on this context returned
+ * (still on the device; the first nbands of them), so an SCF iteration moves no wave
+ * functions over PCIe.  DZHIP_ERR_STATE if there is no solution with nbands columns. */
+int dzhip_density_accumulate_solution(dzhip_ham* ham, uint64_t nbands, const double* weights,
+                                      double* rho, uint64_t count, int accumulate);
+
+/* ------------------------------------------------------------------ learned preconditioner
+ * No counterpart in the reference (its Davidson uses the kinetic-shift diagonal,
+ * src/features/linalg/lobpcg/common.zig:214-226).  A 3 -> 32 -> 32 -> 1 SiLU network (FP32)
+ * rescales the Teter-Payne-Allan diagonal per coefficient:
+ *   s(G, b) = K_tpa(|k+G|^2 / <x_b|T|x_b>) exp(z),
+ *   z = MLP(|k+G|^2/2 / e_s, lambda_b/2 / e_s, log10(|r_b|) / 4)     (energies in Ha)
+ * Parameters: DZHIP_MLP_PARAMS floats, W1[32][3], b1[32], W2[32][32], b2[32], w3[32], b3.
+ * Samples: structure of arrays [6][n] floats: x1, x2, x3, kx = |k+G|^2/<T>, target, weight;
+ * loss = sum_i w_i (s_i - t_i)^2 / n over a minibatch of n samples. */
+#define DZHIP_MLP_PARAMS 1217
+
+int dzhip_ham_set_precond_model(dzhip_ham* ham, const float* params, uint64_t n_params, double energy_scale);
+
+/* Residual/correction pairs of the snapshots the last solve recorded (collect_snapshots > 0,
+ * FP64, converged): correction e = X X^H x_i - x_i against the converged Ritz vectors X,
+ * target t = Re(e conj r) / (alpha |r|^2) and weight alpha^2 |r|^2 / |e|^2 with alpha the
+ * band's least-squares TPA scale; every stride-th coefficient.  Writes min(capacity, available)
+ * samples to out ([6][capacity]) and their count.  Weight-0 samples mark unusable bands. */
+int dzhip_ham_training_samples(dzhip_ham* ham, double energy_scale, uint64_t stride, float* out,
+                               uint64_t capacity, uint64_t* count);
+
+/* Full-batch loss and gradient (FP64 out) of the network on n samples ([6][n]). */
+int dzhip_mlp_loss_grad(dzhip_ctx* ctx, const float* samples, uint64_t n, const float* params,
+                        double* grad_out, double* loss_out);
+
+/* Adam training on the device: `epochs` passes over n samples in minibatches of `batch`, each
+ * pass in a different pseudo-random order (seeded); params in/out; epoch_loss (may be NULL)
+ * receives the full-batch loss after each epoch. */
+int dzhip_mlp_train(dzhip_ctx* ctx, const float* samples, uint64_t n, float* params, uint64_t epochs,
+                    uint64_t batch, double lr, uint64_t seed, double* epoch_loss);
 
 /* Sizes the library uses for this context (for roofline accounting):
  * out[0] = n_pw, [1] = grid points, [2] = x-planes holding the G-sphere,

@@ -18,3 +18,29 @@ def context_from_workload(w) -> hamiltonian.HamContext:
     b = pw.Basis(w.basis.h, w.basis.k, w.basis.l, w.basis.cart, w.basis.kpg, w.basis.kinetic)
     atoms = [hamiltonian.Atom(w.atom_pos[i], int(w.atom_species[i])) for i in range(len(w.atom_species))]
     return hamiltonian.HamContext(tuple(w.grid), b, w.local_r, sps, atoms, w.inv_volume)
+
+
+def gth_local_q(z_ion: float, rloc: float, c1: float):
+    """v(q) (Ry, per atom, without 1/Omega) of the local potential
+    V(r) = 2 [ -Z erf(r / (sqrt2 rloc)) / r + C1 exp(-r^2 / (2 rloc^2)) ]  (Hartree -> Ry):
+    v(q) = 2 [ -4 pi Z / q^2 + C1 (2 pi)^{3/2} rloc^3 ] exp(-q^2 rloc^2 / 2),  q > 0."""
+    import math
+
+    def fn(q: float) -> float:
+        e = math.exp(-0.5 * q * q * rloc * rloc)
+        return 2.0 * (-4.0 * math.pi * z_ion / (q * q) + c1 * (2.0 * math.pi) ** 1.5 * rloc ** 3) * e
+
+    return fn
+
+
+def context_for_k(system, k_cart, local_r) -> hamiltonian.HamContext:
+    """Oracle context of one k-point of an SCF system (raw parameters only)."""
+    basis = pw.generate_basis(system.recip, system.ecut_ry, k_cart)
+    sps = []
+    for sp in system.species:
+        rad = [projectors.gaussian_beta_radial(l, rc, projectors.gaussian_beta_norm(l, rc))
+               for l, rc in zip(sp.l_list, sp.rc)]
+        sps.append(projectors.Species(list(sp.l_list), rad, sp.dij))
+    atoms = [hamiltonian.Atom(system.atom_pos[i], int(system.atom_species[i]))
+             for i in range(len(system.atom_species))]
+    return hamiltonian.HamContext(tuple(system.grid), basis, local_r, sps, atoms, 1.0 / system.volume)

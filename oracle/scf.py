@@ -1,0 +1,239 @@
+"""SCF algebra around the eigensolver, FP64 (numpy), for small cells.
+
+Restates (paths relative to the dft-zig source tree):
+  hartree_potential_at_g: V_H(G) = 8 pi rho(G) / G^2, G = 0 dropped   src/features/scf/potential.zig:34-44
+  lda_pz (Ry), PZ81 correlation energy/potential                      src/features/xc/xc.zig:195-210, 344-384
+  assemble_energy_terms: total = band + (-E_H - int v_xc rho + E_xc) + ion terms
+                                                                     src/features/scf/energy.zig:372-396
+  ionic local potential sum_a e^{-iG.tau} v(|G|) / Omega, G = 0 skipped
+                                                                     src/features/hamiltonian/hamiltonian.zig:421-474
+  local_r = Re IFFT(N (V_ion + V_Hxc)(G)) in FFT order              src/features/scf/potential.zig:320-334
+  occupations nocc = ceil(nelec/2), weight w_k occ 2                 src/features/scf/kpoint/density.zig:98-107
+  linear potential mixing V_in <- V_in + beta (V_out - V_in)         src/features/scf/mixing.zig:10-18
+  Pulay/DIIS on (V_in, R = V_out - V_in), history 8, from iteration 4 src/features/scf/mixing.zig:183-234,
+                                                                     src/features/scf/scf.zig:147-188,
+                                                                     src/features/config/defaults.zig:42-45
+  Pulay system by Gaussian elimination, |pivot| < 1e-12 -> equal weights
+                                                                     src/features/scf/mixing.zig:340-400
+  convergence on the RMS density change                              src/features/scf/util.zig:17-24
+
+The reference mixes V_Hxc(G) over the full FFT grid; with diemac = 1 (its default) the Pulay
+inner products are Parseval multiples of the real-space ones used here, so the coefficients and the
+mixed potential are the same.  Energies per iteration use the Kohn-Sham functional of rho_out at the
+input potential, band - int V_Hxc^in rho_out + E_H + E_xc, which equals assemble_energy_terms at
+self-consistency.  The ion-ion (Ewald) and psp_core terms are constants of the structure and are left out of the
+energies compared here; the eigenproblems use dense numpy.linalg.eigh.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import adapters
+from . import density as odens
+from . import fftconv, hamiltonian, pw
+
+PI = math.pi
+
+
+def lda_pz(n: np.ndarray):
+    """(f, df/dn) in Ry per volume, element-wise; zero below n = 1e-12 (xc.zig:195-210)."""
+    n = np.asarray(n, dtype=np.float64)
+    f = np.zeros_like(n)
+    v = np.zeros_like(n)
+    m = n > 1e-12
+    nm = n[m]
+    n13 = np.power(nm, 1.0 / 3.0)
+    c_x = -0.75 * math.pow(3.0 / PI, 1.0 / 3.0)
+    rs = np.power(3.0 / (4.0 * PI * nm), 1.0 / 3.0)
+    eps_c = np.empty_like(nm)
+    v_c = np.empty_like(nm)
+    lo = rs < 1.0
+    a, b, c, d = 0.0311, -0.048, 0.0020, -0.0116
+    lr = np.log(rs[lo])
+    e_lo = a * lr + b + c * rs[lo] * lr + d * rs[lo]
+    eps_c[lo] = e_lo
+    v_c[lo] = e_lo - (rs[lo] / 3.0) * (a / rs[lo] + c * lr + c + d)
+    hi = ~lo
+    g, b1, b2 = -0.1423, 1.0529, 0.3334
+    sr = np.sqrt(rs[hi])
+    den = 1.0 + b1 * sr + b2 * rs[hi]
+    e_hi = g / den
+    eps_c[hi] = e_hi
+    v_c[hi] = e_hi - (rs[hi] / 3.0) * (-g * (0.5 * b1 / sr + b2) / (den * den))
+    f[m] = 2.0 * (nm * c_x * n13 + nm * eps_c)
+    v[m] = 2.0 * (4.0 / 3.0 * c_x * n13 + v_c)
+    return f, v
+
+
+def g_squared_grid(grid, recip):
+    """|G|^2 on the FFT-order grid (frequencies wrapped, index_to_freq)."""
+    nx, ny, nz = grid
+    fx = np.array([pw.index_to_freq(i, nx) for i in range(nx)])
+    fy = np.array([pw.index_to_freq(i, ny) for i in range(ny)])
+    fz = np.array([pw.index_to_freq(i, nz) for i in range(nz)])
+    Z, Y, X = np.meshgrid(fz, fy, fx, indexing="ij")
+    g = X.ravel()[:, None] * recip[0] + Y.ravel()[:, None] * recip[1] + Z.ravel()[:, None] * recip[2]
+    return g, np.sum(g * g, axis=1)
+
+
+def hartree(rho_r, grid, recip, volume):
+    """V_H(r) and E_H = (1/2) int V_H rho (Ry)."""
+    n = rho_r.shape[0]
+    rho_g = fftconv.forward3d(rho_r, grid) / n
+    _, g2 = g_squared_grid(grid, recip)
+    vg = np.zeros_like(rho_g)
+    nz = g2 > 1e-12
+    vg[nz] = rho_g[nz] * 8.0 * PI / g2[nz]
+    vh = (fftconv.inverse3d(vg * n, grid)).real
+    e_h = 0.5 * float(np.sum(vh * rho_r)) * volume / n
+    return vh, e_h
+
+
+def ionic_potential_r(grid, recip, volume, positions, vq_fn):
+    """V_ion(r) = Re sum_G V_ion(G) e^{iG.r}, V_ion(G) = (1/Omega) sum_a e^{-iG.tau_a} v(|G|)."""
+    n = int(np.prod(grid))
+    g, g2 = g_squared_grid(grid, recip)
+    q = np.sqrt(g2)
+    vg = np.zeros(n, dtype=np.complex128)
+    nz = q > 1e-10
+    sf = np.zeros(n, dtype=np.complex128)
+    for tau in positions:
+        sf += np.exp(-1j * (g @ tau))
+    vg[nz] = sf[nz] * np.array([vq_fn(x) for x in q[nz]]) / volume
+    return (fftconv.inverse3d(vg * n, grid)).real
+
+
+def solve_pulay(b: np.ndarray, rhs: np.ndarray):
+    """Gaussian elimination with partial pivoting; a pivot below 1e-12 gives equal weights
+    (mixing.zig:340-400).  Returns (coefficients, singular)."""
+    a = np.array(b, dtype=np.float64)
+    r = np.array(rhs, dtype=np.float64)
+    size = a.shape[0]
+    for col in range(size):
+        piv = col + int(np.argmax(np.abs(a[col:, col])))
+        if piv != col:
+            a[[col, piv]] = a[[piv, col]]
+            r[[col, piv]] = r[[piv, col]]
+        if abs(a[col, col]) < 1e-12:
+            out = np.full(size, 1.0 / (size - 1))
+            out[-1] = 0.0
+            return out, True
+        for row in range(col + 1, size):
+            f = a[row, col] / a[col, col]
+            a[row, col:] -= f * a[col, col:]
+            r[row] -= f * r[col]
+    x = np.zeros(size)
+    for i in range(size - 1, -1, -1):
+        x[i] = (r[i] - np.dot(a[i, i + 1:], x[i + 1:])) / a[i, i]
+    return x, False
+
+
+class PulayMixer:
+    """mix_with_residual (mixing.zig:183-234): append (v_in, residual), drop the oldest beyond
+    `history`, linear step while fewer than 2 entries, else v = sum_i c_i (v_i + beta R_i) with
+    [B -1; -1 0] c = [0; -1], B_ij = <R_i|R_j>.
+
+    relative=False is the reference as written: its absolute pivot threshold (1e-12) takes the
+    equal-weights branch once <R|R> is below about 1e-12 (an RMS potential residual near 1e-6
+    Ry), after which the iteration wanders instead of converging.  relative=True is the form
+    the GPU driver uses: B is divided by its largest diagonal entry (the coefficients of a
+    regular system are invariant under a scaling of B), and a singular system drops the history
+    except the newest pair and takes the linear step instead of averaging the history."""
+
+    def __init__(self, history: int, relative: bool = True):
+        self.history = history
+        self.relative = relative
+        self.v, self.r = [], []
+
+    def mix(self, v_in: np.ndarray, residual: np.ndarray, beta: float) -> np.ndarray:
+        self.v.append(v_in.copy())
+        self.r.append(residual.copy())
+        while len(self.v) > self.history:
+            self.v.pop(0)
+            self.r.pop(0)
+        m = len(self.v)
+        if m < 2:
+            return v_in + beta * residual
+        # <R_i|R_j> over the reference's V(G) = (1/N) sum_r V(r) e^{-iG.r} on the full grid
+        # (fft_grid.zig:27-36) is (1/N) sum_r R_i(r) R_j(r) by Parseval; the scale matters for
+        # the absolute singular-pivot threshold of solve_pulay.
+        n = self.r[0].shape[0]
+        b = np.zeros((m + 1, m + 1))
+        for i in range(m):
+            for j in range(m):
+                b[i, j] = float(np.dot(self.r[i], self.r[j])) / n
+        if self.relative:
+            b[:m, :m] /= max(float(np.max(np.diag(b)[:m])), 1e-300)
+        b[:m, m] = -1.0
+        b[m, :m] = -1.0
+        rhs = np.zeros(m + 1)
+        rhs[m] = -1.0
+        c, singular = solve_pulay(b, rhs)
+        if singular and self.relative:
+            del self.v[:-1], self.r[:-1]
+            return v_in + beta * residual
+        out = np.zeros_like(v_in)
+        for i in range(m):
+            out += c[i] * (self.v[i] + beta * self.r[i])
+        return out
+
+
+@dataclass
+class ScfResult:
+    energies: list = field(default_factory=list)  # total energy (Ry) per iteration
+    drho: list = field(default_factory=list)
+    eigenvalues: list = field(default_factory=list)  # per k, final
+    rho: np.ndarray | None = None
+    converged: bool = False
+
+
+def run_scf(system, n_iter=40, beta=0.3, conv=1e-8, pulay_history=0, pulay_start=4, pulay_relative=True):
+    """Fixed-occupation SCF with potential mixing (linear, or Pulay from iteration `pulay_start`
+    when pulay_history > 0, scf.zig:164); dense eigh per k."""
+    grid, recip, vol = system.grid, system.recip, system.volume
+    n = int(np.prod(grid))
+    v_ion = ionic_potential_r(grid, recip, vol, system.atom_pos,
+                              adapters.gth_local_q(system.z_ion, system.rloc, system.c1))
+    rho = np.full(n, system.nelec / vol)
+    vh, _ = hartree(rho, grid, recip, vol)
+    _, vxc = lda_pz(rho)
+    v_in = vh + vxc
+    res = ScfResult()
+    mixer = PulayMixer(pulay_history, pulay_relative) if pulay_history > 0 else None
+    for it in range(n_iter):
+        local_r = v_ion + v_in
+        rho_out = np.zeros(n)
+        e_band = 0.0
+        eigs = []
+        for kp, wk in zip(system.kpoints, system.kweights):
+            ctx = adapters.context_for_k(system, kp, local_r)
+            h = hamiltonian.dense_hamiltonian(ctx)
+            ev, vec = np.linalg.eigh(h)
+            nb = system.nbands
+            wts = odens.band_weights(system.nelec, nb, wk)
+            e_band += float(np.sum(wts * ev[:nb]))
+            rho_out = odens.accumulate_density(ctx.fft_idx, grid, vec[:, :nb].T, wts, 1.0 / vol, rho_out)
+            eigs.append(ev[:nb])
+        vh_out, e_h = hartree(rho_out, grid, recip, vol)
+        f_xc, vxc_out = lda_pz(rho_out)
+        e_xc = float(np.sum(f_xc)) * vol / n
+        # Kohn-Sham energy with the input potential: band - int V_Hxc^in rho_out + E_H + E_xc.
+        e_tot = e_band - float(np.sum(v_in * rho_out)) * vol / n + e_h + e_xc
+        drho = math.sqrt(float(np.mean((rho_out - rho) ** 2)))
+        res.energies.append(e_tot)
+        res.drho.append(drho)
+        res.eigenvalues = eigs
+        rho = rho_out
+        v_out = vh_out + vxc_out
+        if drho < conv:
+            res.converged = True
+            break
+        if mixer is not None and it >= pulay_start:
+            v_in = mixer.mix(v_in, v_out - v_in, beta)
+        else:
+            v_in = v_in + beta * (v_out - v_in)
+    res.rho = rho
+    return res

@@ -55,7 +55,27 @@ def fft_file():
     print("fft", list(out)[:3])
 
 
+def scf_file():
+    """Oracle SCF of the small Si case: energies per iteration, final eigenvalues, density."""
+    from dzhip import scf as dscf
+    from oracle import scf as oscf
+
+    system = dscf.si8_small()
+    res = oscf.run_scf(system, n_iter=40, beta=0.3, conv=1e-10, pulay_history=8, pulay_start=4)
+    assert res.converged
+    key = np.concatenate([np.asarray(system.atom_pos).ravel(), np.ravel(system.kpoints),
+                          [system.ecut_ry, system.nbands, *system.grid]])
+    np.savez_compressed(HERE / "scf_si8_small.npz", energies=np.array(res.energies), drho=np.array(res.drho),
+                        eigenvalues=np.array(res.eigenvalues), rho=res.rho, key=key)
+    print("scf", len(res.energies), res.energies[-1])
+
+
 if __name__ == "__main__":
-    hpsi_file("tiny")
-    hpsi_file("si8")
-    fft_file()
+    which = sys.argv[1:] or ["hpsi", "fft", "scf"]
+    if "hpsi" in which:
+        hpsi_file("tiny")
+        hpsi_file("si8")
+    if "fft" in which:
+        fft_file()
+    if "scf" in which:
+        scf_file()

@@ -1,0 +1,135 @@
+"""GPU tests of the learned preconditioner: the device forward/backward against the FP64 CPU
+oracle (oracle/mlp.py), Adam against the oracle's update, the zero-output-layer model reducing
+exactly to Teter-Payne-Allan inside the eigensolver, sample collection, and what the trained
+network gains over TPA on problems it was not trained on.
+
+Tolerances: gradient 1e-4 relative (2-norm) for FP32 arithmetic against FP64, loss 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+from dzhip import FP64, PRECOND_MLP, PRECOND_TPA, precond, workloads
+from dzhip import scf as dscf
+from oracle import mlp as omlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_samples(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x1 = rng.uniform(0.0, 1.5, n)
+    x2 = rng.uniform(-0.1, 0.2, n)
+    x3 = rng.uniform(-2.5, -0.25, n)
+    kx = rng.uniform(0.0, 6.0, n)
+    t = omlp.tpa(kx) * np.exp(rng.normal(0.0, 0.3, n))
+    w = rng.uniform(0.1, 2.0, n)
+    return np.stack([x1, x2, x3, kx, t, w]).astype(np.float32)
+
+
+def _random_params(seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = omlp.init_params(seed)
+    p[-(omlp.H + 1):] = rng.normal(0.0, 0.3, omlp.H + 1)
+    return p.astype(np.float32)
+
+
+@pytest.mark.parametrize("n", [1, 127, 5000])
+def test_mlp_loss_grad_matches_oracle(dz, ctx, n):
+    smp = _random_samples(n, 5 + n)
+    p = _random_params(3)
+    loss, g = ctx.mlp_loss_grad(smp, p)
+    s64 = smp.astype(np.float64)
+    lo, go = omlp.loss_and_grad(p.astype(np.float64), s64[:3].T, s64[3], s64[4], s64[5])
+    # The library's loss is sum w (s-t)^2 / n; the oracle's divides by sum w.
+    scale = float(np.sum(s64[5])) / n
+    print(n, loss, lo * scale, np.linalg.norm(g - go * scale) / np.linalg.norm(go * scale))
+    assert abs(loss - lo * scale) <= 1e-5 * abs(lo * scale)
+    assert np.linalg.norm(g - go * scale) <= 1e-4 * np.linalg.norm(go * scale)
+
+
+def test_adam_first_step_matches_oracle(dz, ctx):
+    n = 4096
+    smp = _random_samples(n, 21)
+    p = _random_params(5)
+    out, hist = ctx.mlp_train(smp, p, epochs=1, batch=n, lr=1e-3, seed=3)
+    s64 = smp.astype(np.float64)
+    _, g = omlp.loss_and_grad(p.astype(np.float64), s64[:3].T, s64[3], s64[4], s64[5])
+    ref, _, _ = omlp.adam_step(p.astype(np.float64), g, np.zeros_like(g), np.zeros_like(g), 1, 1e-3)
+    # First Adam step: p - lr * g/|g| elementwise (up to eps), so a sign check plus magnitude.
+    big = np.abs(g) > 1e-6 * np.max(np.abs(g))
+    assert np.max(np.abs(out[big] - ref[big])) < 1e-6
+    assert hist.shape == (1,) and np.isfinite(hist[0])
+
+
+def test_zero_output_layer_is_exactly_tpa(dz, ctx):
+    w = workloads.si8(nbands=1)
+    ham = dz.Hamiltonian.from_workload(ctx, w)
+    try:
+        p = precond.init_params(7)
+        ham.set_precond_model(p, precond.ENERGY_SCALE)
+        v_t, _, st_t = ham.eig_solve(12, tol=1e-8, max_iter=200, precision=FP64, preconditioner=PRECOND_TPA,
+                                     want_vectors=False)
+        v_m, _, st_m = ham.eig_solve(12, tol=1e-8, max_iter=200, precision=FP64, preconditioner=PRECOND_MLP,
+                                     want_vectors=False)
+        print(st_t, st_m)
+        assert st_t["iterations"] == st_m["iterations"] and st_t["apply_calls"] == st_m["apply_calls"]
+        assert np.max(np.abs(v_t - v_m)) < 1e-12
+        assert st_m["precond_fallback"] == 0
+    finally:
+        ham.close()
+
+
+def test_mlp_requires_model(dz, ctx):
+    ham = dz.Hamiltonian.from_workload(ctx, workloads.tiny())
+    try:
+        with pytest.raises(dz.BadState):
+            ham.eig_solve(4, tol=1e-8, preconditioner=PRECOND_MLP)
+    finally:
+        ham.close()
+
+
+def test_samples_train_and_gain(dz, ctx):
+    """Train on residual/correction pairs of converged FP64 solves of the Si-8 cell at two k-points
+    and the bare ionic potential; evaluate on other k-points and on the SCF potential of a
+    different cutoff.  Reports Davidson iterations and H applies with TPA and with the network."""
+    train_sys = dscf.silicon(ecut_ry=12.0, nbands=20, kpoints=([np.zeros(3), np.array([0.3, 0.1, 0.2])], [0.5, 0.5]))
+    drv = dscf.ScfDriver(train_sys)
+    chunks = []
+    try:
+        for ham in drv.hams:
+            ham.set_local(drv.v_ion)
+            smp = precond.collect(ham, 20, snapshots=24, stride=1)
+            assert smp.shape[0] == 6 and smp.shape[1] > 1000
+            assert np.all(np.isfinite(smp)) and np.all(smp[5] > 0)
+            chunks.append(smp)
+    finally:
+        drv.close()
+    samples = np.concatenate(chunks, axis=1)
+    p0 = precond.init_params(1)
+    l0, _ = ctx.mlp_loss_grad(precond.normalise_weights(samples), p0)
+    params, hist = precond.train(ctx, samples, epochs=40, batch=8192, lr=2e-3, seed=1, init=p0)
+    print("samples", samples.shape[1], "loss TPA", l0, "trained", hist[-1], hist[::8])
+    assert hist[-1] < 0.9 * l0
+
+    test_sys = dscf.silicon(ecut_ry=16.0, nbands=24, kpoints=([np.array([0.15, 0.25, -0.1]), np.array([0.5, 0.5, 0.0])],
+                                                             [0.5, 0.5]))
+    drv = dscf.ScfDriver(test_sys)
+    res = {}
+    try:
+        for i, ham in enumerate(drv.hams):
+            ham.set_local(drv.v_ion)
+            ham.set_precond_model(params, precond.ENERGY_SCALE)
+            for name, pc in (("tpa", PRECOND_TPA), ("mlp", PRECOND_MLP)):
+                vals, _, st = ham.eig_solve(24, tol=1e-8, max_iter=300, precision=FP64, preconditioner=pc,
+                                            want_vectors=False, seed=11)
+                res[(i, name)] = (vals, st)
+    finally:
+        drv.close()
+    tot = {name: sum(res[(i, name)][1]["apply_calls"] for i in range(2)) for name in ("tpa", "mlp")}
+    its = {name: sum(res[(i, name)][1]["iterations"] for i in range(2)) for name in ("tpa", "mlp")}
+    print("held-out applies", tot, "iterations", its,
+          "fallbacks", [res[(i, "mlp")][1]["precond_fallback"] for i in range(2)])
+    for i in range(2):
+        assert res[(i, "mlp")][1]["converged"]
+        assert np.max(np.abs(res[(i, "mlp")][0] - res[(i, "tpa")][0])) < 2e-8
+    assert tot["mlp"] <= 1.1 * tot["tpa"]
