@@ -127,9 +127,10 @@ def scf_leg(name, max_iter, log=sys.stderr):
         setup[tag] = drv.setup_s
 
         def show(r, tag=tag):
+            sec = " ".join(f"{k[2:-3]} {v:.0f}" for k, v in r.sections_ms.items())
             print(f"scf {name} {tag} it {r.iteration} E {r.energy_ry:.10f} drho {r.drho:.3e} t {r.wall_s:.2f}s "
-                  f"eig {r.eig_s:.2f}s eig_it {r.eig_iterations} applies {r.applies} fp32 {r.fp32_applies}",
-                  file=log, flush=True)
+                  f"eig {r.eig_s:.2f}s eig_it {r.eig_iterations} applies {r.applies} fp32 {r.fp32_applies} "
+                  f"gpu_ms[{sec}]", file=log, flush=True)
         try:
             runs[tag] = drv.run(max_iter=max_iter, conv=1e-8, precision=prec, eig_tol_ratio=1.0, eig_tol_min=1e-7,
                                 progress=show)
@@ -145,6 +146,8 @@ def scf_leg(name, max_iter, log=sys.stderr):
                     "wall_s": round(run.records[-1].wall_s, 3), "setup_s": round(setup[tag], 3),
                     "eig_s": round(sum(r.eig_s for r in run.records), 3),
                     "applies": sum(r.applies for r in run.records),
+                    "solver_gpu_ms": {k[2:-3]: round(sum(r.sections_ms.get(k, 0.0) for r in run.records), 1)
+                                      for k in dscf.SECTIONS},
                     "fp32_applies": sum(r.fp32_applies for r in run.records)}
     a, b = out["fp64"]["t_1e-6_Ha_s"], out["mixed_batched"]["t_1e-6_Ha_s"]
     out["speedup"] = None if not a or not b else round(a / b, 3)

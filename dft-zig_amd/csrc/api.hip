@@ -343,6 +343,20 @@ int dzhip_eig_solve(dzhip_ham* ham, const double* diag, uint64_t n, uint64_t nba
     });
 }
 
+int dzhip_cg_solve(dzhip_ham* ham, const double* diag, uint64_t n, uint64_t nbands, const dzhip_eig_opts* opts,
+                   const double* init_vecs, uint64_t init_cols, double* values_out, double* vectors_out,
+                   dzhip_eig_stats* stats) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        DZ_REQUIRE(n == ham->npw, DZHIP_ERR_SIZE, "n != n_pw (InvalidMatrixSize)");
+        DZ_REQUIRE(diag && opts && values_out, DZHIP_ERR_ARG, "NULL diag, opts or values_out");
+        DZ_REQUIRE(ham->has_local, DZHIP_ERR_STATE, "local potential not set");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        cg_solve(*ham, diag, nbands, *opts, init_vecs, init_cols, values_out, vectors_out, stats);
+    });
+}
+
 int dzhip_density_accumulate(dzhip_ham* ham, const double* psi, uint64_t n, uint64_t nbands,
                              const double* weights, double* rho, uint64_t count, int accumulate) {
     return guarded([&] {

@@ -21,9 +21,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
+#include <vector>
 
 #include "ham.h"
+#include "herm3.h"
 
 namespace dzhip {
 
@@ -84,6 +87,34 @@ __global__ void k_precondition(const double2* __restrict__ r, const int* __restr
         f = p / (p + 16.0 * x * x * x * x);
     }
     q[(size_t)j * ld + i] = make_double2(v.x * f, v.y * f);
+}
+
+// p = r / (diag - lambda) with the 1e-8 clamp (single vector, CG).
+__global__ void k_precondition_one(const double2* __restrict__ r, const double* __restrict__ diag, double lam,
+                                   double2* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double den = diag[i] - lam;
+    if (fabs(den) < 1e-8) den = den >= 0.0 ? 1e-8 : -1e-8;
+    const double f = 1.0 / den;
+    p[i] = make_double2(r[i].x * f, r[i].y * f);
+}
+
+// out = c0 a + c1 b (+ c2 c), complex coefficients (CG Ritz update of psi and H psi).
+__global__ void k_combine3(const double2* __restrict__ a, const double2* __restrict__ b,
+                           const double2* __restrict__ c, double2 c0, double2 c1, double2 c2, int use_c,
+                           double2* __restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 x = a[i], y = b[i];
+    double re = c0.x * x.x - c0.y * x.y + c1.x * y.x - c1.y * y.y;
+    double im = c0.x * x.y + c0.y * x.x + c1.x * y.y + c1.y * y.x;
+    if (use_c) {
+        const double2 z = c[i];
+        re += c2.x * z.x - c2.y * z.y;
+        im += c2.x * z.y + c2.y * z.x;
+    }
+    out[i] = make_double2(re, im);
 }
 
 // <x_b|T|x_b> = sum_G |k+G|^2 |x_bG|^2 (one workgroup per band)
@@ -159,7 +190,12 @@ struct EigWork {
     // (Ritz vectors, residuals, Ritz values, residual norms, <T> per band) of the last solve.
     DevBuf rnorm, snapX, snapR, snapLam, snapRn, snapEk, bandsum, samples;
     int snaps = 0, snap_nb = 0;
+    // Section timing: events recorded on the stream at section starts (pooled).
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_tag;
+    size_t ev_used = 0;
     ~EigWork() {
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         if (blas) rocblas_destroy_handle(blas);
         DevBuf* bufs[] = {&V, &W, &X, &HX, &R, &Q, &HQ, &T, &Tw, &S, &G, &evals, &ework, &info,
                           &norms, &lam, &ekin, &diag, &sel, &scl, &idx, &x32, &y32, &rnorm,
@@ -167,6 +203,35 @@ struct EigWork {
         for (DevBuf* b : bufs) b->release();
     }
 };
+
+// Sections of a solve timed with events: a mark starts a section and ends the previous one.
+enum EigSection { SEC_RR = 0, SEC_RITZ, SEC_PRECOND, SEC_ORTHO, SEC_APPLY, SEC_EXTEND, SEC_COUNT, SEC_END = SEC_COUNT };
+
+static void mark(EigWork& w, hipStream_t s, int tag) {
+    if (w.ev_used == w.ev_pool.size()) {
+        hipEvent_t e;
+        DZ_HIP(hipEventCreate(&e));
+        w.ev_pool.push_back(e);
+        w.ev_tag.push_back(0);
+    }
+    DZ_HIP(hipEventRecord(w.ev_pool[w.ev_used], s));
+    w.ev_tag[w.ev_used++] = tag;
+}
+
+static void section_times(EigWork& w, double* out) {
+    for (int i = 0; i < SEC_COUNT; ++i) out[i] = 0.0;
+    if (w.ev_used < 2) {
+        w.ev_used = 0;
+        return;
+    }
+    DZ_HIP(hipEventSynchronize(w.ev_pool[w.ev_used - 1]));
+    for (size_t i = 0; i + 1 < w.ev_used; ++i) {
+        float ms = 0.f;
+        DZ_HIP(hipEventElapsedTime(&ms, w.ev_pool[i], w.ev_pool[i + 1]));
+        if (w.ev_tag[i] < SEC_COUNT) out[w.ev_tag[i]] += ms;
+    }
+    w.ev_used = 0;
+}
 
 static void eig_work_release(Ham& h) {
     delete static_cast<EigWork*>(h.eig_work);
@@ -467,11 +532,14 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
     std::vector<double> lam(nb), best(nb);
     bool have_best = false;
     uint64_t iter = 0;
+    w.ev_used = 0;
     for (; iter < o.max_iter; ++iter) {
+        mark(w, s, SEC_RR);
         std::vector<double> ev = S.rayleigh_ritz(m);
         for (int b = 0; b < nb; ++b) lam[b] = ev[b];
         best = lam;
         have_best = true;
+        mark(w, s, SEC_RITZ);
         S.ritz(m);
         DZ_HIP(hipMemcpyAsync(w.lam.ptr, lam.data(), nb * sizeof(double), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_residual, S.cols_grid(nb), dim3(256), 0, s, w.HX.as<double2>(), w.X.as<double2>(),
@@ -522,6 +590,7 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
             if (rn[b] >= o.tol) sel.push_back(b);
         int a = 0;
         if (!sel.empty()) {
+            mark(w, s, SEC_PRECOND);
             DZ_HIP(hipMemcpyAsync(w.sel.ptr, sel.data(), sel.size() * sizeof(int), hipMemcpyHostToDevice, s));
             if (precond != DZHIP_PRECOND_KINETIC) {
                 hipLaunchKernelGGL(k_kinetic_energy, dim3(nb), dim3(256), 0, s, w.X.as<double2>(), w.diag.as<double>(),
@@ -539,6 +608,7 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
                                    w.Q.as<double2>(), n, (int)n, precond == DZHIP_PRECOND_TPA ? 1 : 0);
                 DZ_HIP(hipGetLastError());
             }
+            mark(w, s, SEC_ORTHO);
             a = S.orthonormalize_new(w.Q.as<double2>(), (int)sel.size(), m);
         }
         if (a == 0) {
@@ -551,7 +621,9 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
             S.st.converged = max_res < o.tol ? 1 : 0;
             break;
         }
+        mark(w, s, SEC_APPLY);
         S.apply(w.Q.as<double2>(), w.HQ.as<double2>(), a);
+        mark(w, s, SEC_EXTEND);
         if (m + a > max_sub) {
             // Thick restart from [Ritz vectors, new block]; their H images are W c and H Q.
             S.restart_with(a);
@@ -564,6 +636,7 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
         S.extend_T(m, a);
         m += a;
     }
+    mark(w, s, SEC_END);
     S.st.iterations = iter;
     if (have_best) {
         std::memcpy(values, best.data(), nb * sizeof(double));
@@ -572,7 +645,263 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
     }
     if (vectors) DZ_HIP(hipMemcpyAsync(vectors, w.X.ptr, n * nb * sizeof(double2), hipMemcpyDeviceToHost, s));
     DZ_HIP(hipStreamSynchronize(s));
+    double sec[SEC_COUNT];
+    section_times(w, sec);
+    S.st.t_rr_ms = sec[SEC_RR];
+    S.st.t_ritz_ms = sec[SEC_RITZ];
+    S.st.t_precond_ms = sec[SEC_PRECOND];
+    S.st.t_ortho_ms = sec[SEC_ORTHO];
+    S.st.t_apply_ms = sec[SEC_APPLY];
+    S.st.t_extend_ms = sec[SEC_EXTEND];
     if (stats_out) *stats_out = S.st;
+}
+
+// ----------------------------------------------------------------- band-by-band CG
+namespace {
+
+struct CgRun {
+    Ham& h;
+    EigWork& w;
+    hipStream_t s;
+    size_t n;
+    int ldn;
+    dzhip_eig_stats& st;
+
+    dim3 grid1() const { return dim3((unsigned)((n + 255) / 256)); }
+    cplx dot(const double2* a, const double2* b) {  // <a|b>
+        zc r;
+        DZ_BLAS(rocblas_zdotc(w.blas, ldn, reinterpret_cast<const zc*>(a), 1, reinterpret_cast<const zc*>(b), 1, &r));
+        return cplx(r.real(), r.imag());
+    }
+    double nrm(const double2* a) {
+        double r = 0.0;
+        DZ_BLAS(rocblas_dznrm2(w.blas, ldn, reinterpret_cast<const zc*>(a), 1, &r));
+        return r;
+    }
+    void axpy(cplx alpha, const double2* x, double2* y) {
+        const zc al(alpha.real(), alpha.imag());
+        DZ_BLAS(rocblas_zaxpy(w.blas, ldn, &al, reinterpret_cast<const zc*>(x), 1, reinterpret_cast<zc*>(y), 1));
+    }
+    void scal(double a, double2* x) {
+        DZ_BLAS(rocblas_zdscal(w.blas, ldn, &a, reinterpret_cast<zc*>(x), 1));
+    }
+    void copy(double2* dst, const double2* src) {
+        DZ_HIP(hipMemcpyAsync(dst, src, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+    }
+    void apply(const double2* x, double2* y) {
+        ham_apply_dev<double>(h, x, y, 1, true, true);
+        st.apply_calls++;
+        st.fp64_applies++;
+    }
+    // v -= sum_{j<m} <V_j|v> V_j, one column at a time in order (modified Gram-Schmidt).
+    void orth(double2* v, const double2* V, int m) {
+        for (int j = 0; j < m; ++j) axpy(-dot(V + (size_t)j * n, v), V + (size_t)j * n, v);
+    }
+    void combine(const double2* a, const double2* b, const double2* c, cplx c0, cplx c1, cplx c2, bool use_c,
+                 double2* out) {
+        hipLaunchKernelGGL(k_combine3, grid1(), dim3(256), 0, s, a, b, c, make_double2(c0.real(), c0.imag()),
+                           make_double2(c1.real(), c1.imag()), make_double2(c2.real(), c2.imag()), use_c ? 1 : 0, out,
+                           (int)n);
+        DZ_HIP(hipGetLastError());
+    }
+};
+
+}  // namespace
+
+void cg_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts& o, const double* init,
+              uint64_t init_cols, double* values, double* vectors, dzhip_eig_stats* stats_out) {
+    const size_t n = h.npw;
+    DZ_REQUIRE(nbands > 0 && nbands <= n, DZHIP_ERR_ARG, "nbands must be in 1..n");
+    const int nb = (int)nbands;
+    const uint64_t max_iter = o.max_iter ? o.max_iter : 20;
+    const double tol = o.tol > 0.0 ? o.tol : 1e-6;
+    ensure_work(h, nb, nb);
+    EigWork& w = *static_cast<EigWork*>(h.eig_work);
+    // Work vectors: psi block in V, H psi in W[:,0], r/p/d/Hd/psi_old/Hpsi_old/psi_prev/Hpsi_prev
+    // in Q columns 0..7 (Q holds n x max_sub >= n x 8 when nb >= 8; else grow it).
+    w.Q.ensure(n * std::max(nb, 8) * sizeof(double2));
+    w.W.ensure(n * std::max(nb, 8) * sizeof(double2));
+    dzhip_eig_stats st{};
+    hipStream_t s = h.stream();
+    CgRun R{h, w, s, n, (int)n, st};
+    DZ_BLAS(rocblas_set_pointer_mode(w.blas, rocblas_pointer_mode_host));
+    DZ_HIP(hipMemcpyAsync(w.diag.ptr, diag, n * sizeof(double), hipMemcpyHostToDevice, s));
+
+    // Start: copied vectors, reused device vectors, or the LCG; orthonormalised (MGS).
+    double2* V = w.V.as<double2>();
+    {
+        const bool reuse = o.reuse_vectors != 0 && !(init && init_cols > 0) && w.x_cols > 0;
+        size_t cols = 0;
+        if (init && init_cols > 0) cols = std::min<uint64_t>(init_cols, nbands);
+        else if (reuse) cols = std::min<uint64_t>(w.x_cols, nbands);
+        std::vector<double2> v(n * (nb - cols));
+        unsigned long long x = o.seed ? o.seed : 0x6f3d3cbeULL;
+        for (size_t i = 0; i < v.size(); ++i) {
+            auto nxt = [&]() {
+                x = x * 6364136223846793005ULL + 1ULL;
+                return (double)(x >> 11) / 9007199254740992.0;
+            };
+            const double re = nxt() - 0.5;
+            const double im = nxt() - 0.5;
+            v[i] = make_double2(re, im);
+        }
+        if (init && init_cols > 0) DZ_HIP(hipMemcpyAsync(V, init, cols * n * sizeof(double2), hipMemcpyHostToDevice, s));
+        else if (cols > 0) DZ_HIP(hipMemcpyAsync(V, w.X.ptr, cols * n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        if (!v.empty())
+            DZ_HIP(hipMemcpyAsync(V + cols * n, v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+        DZ_HIP(hipStreamSynchronize(s));
+        for (int b = 0; b < nb; ++b) {
+            double2* vb = V + (size_t)b * n;
+            R.orth(vb, V, b);
+            const double nv = R.nrm(vb);
+            DZ_REQUIRE(nv > 1e-8, DZHIP_ERR_ARG, "start vectors are linearly dependent");
+            R.scal(1.0 / nv, vb);
+        }
+    }
+    w.x_cols = 0;
+    w.x_converged = false;
+    double2* Q = w.Q.as<double2>();
+    double2 *hpsi = w.W.as<double2>(), *r = Q, *p = Q + n, *d = Q + 2 * n, *hd = Q + 3 * n, *pold = Q + 4 * n,
+            *hpold = Q + 5 * n, *pp = Q + 6 * n, *hpp = Q + 7 * n;
+    std::vector<double> vals(nb);
+    bool all_conv = true;
+    for (int b = 0; b < nb; ++b) {
+        double2* psi = V + (size_t)b * n;
+        if (b > 0) {
+            R.orth(psi, V, b);
+            R.scal(1.0 / R.nrm(psi), psi);
+        }
+        R.apply(psi, hpsi);
+        double lam = R.dot(psi, hpsi).real();
+        double prev_zr = 0.0;
+        bool has_prev = false, conv = false;
+        for (uint64_t it = 0; it < max_iter; ++it) {
+            st.iterations++;
+            R.copy(r, hpsi);
+            R.axpy(cplx(-lam, 0.0), psi, r);
+            R.orth(r, V, b);
+            if (R.nrm(r) < tol) {
+                conv = true;
+                break;
+            }
+            hipLaunchKernelGGL(k_precondition_one, R.grid1(), dim3(256), 0, s, r, w.diag.as<double>(), lam, p, (int)n);
+            DZ_HIP(hipGetLastError());
+            R.orth(p, V, b + 1);
+            const double zr = R.dot(p, r).real();
+            if (has_prev && prev_zr > 1e-30) {
+                R.scal(std::max(0.0, zr / prev_zr), d);
+                R.axpy(cplx(1.0, 0.0), p, d);
+            } else {
+                R.copy(d, p);
+            }
+            prev_zr = zr;
+            R.orth(d, V, b + 1);
+            const double dn = R.nrm(d);
+            if (dn < 1e-14) break;
+            R.scal(1.0 / dn, d);
+            R.apply(d, hd);
+            // Three-term Rayleigh-Ritz on {psi, d, psi_prev} when the previous psi adds a direction.
+            bool three = false;
+            double lam_new = lam;
+            cplx c0, c1, c2;
+            if (has_prev) {
+                R.copy(pp, pold);
+                R.copy(hpp, hpold);
+                const cplx ov = R.dot(psi, pp);
+                R.axpy(-ov, psi, pp);
+                R.axpy(-ov, hpsi, hpp);
+                const cplx ovd = R.dot(d, pp);
+                R.axpy(-ovd, d, pp);
+                R.axpy(-ovd, hd, hpp);
+                const double pn = R.nrm(pp);
+                if (pn > 1e-6) {
+                    R.scal(1.0 / pn, pp);
+                    R.scal(1.0 / pn, hpp);
+                    const double a22 = R.dot(d, hd).real(), a33 = R.dot(pp, hpp).real();
+                    const cplx a12 = R.dot(psi, hd), a13 = R.dot(psi, hpp), a23 = R.dot(d, hpp);
+                    const cplx m3[9] = {lam, a12, a13, std::conj(a12), a22, a23, std::conj(a13), std::conj(a23), a33};
+                    cplx vec[3];
+                    herm3_lowest(m3, &lam_new, vec);
+                    // Phase convention: the psi coefficient real and non-negative.
+                    if (std::abs(vec[0]) > 0.0) {
+                        const cplx ph = std::abs(vec[0]) / vec[0];
+                        for (auto& c : vec) c *= ph;
+                    }
+                    c0 = vec[0];
+                    c1 = vec[1];
+                    c2 = vec[2];
+                    three = true;
+                }
+            }
+            if (!three) {
+                const double a11 = lam, a22 = R.dot(d, hd).real();
+                const cplx a12 = R.dot(psi, hd);
+                const double avg = 0.5 * (a11 + a22), diff = 0.5 * (a11 - a22), off = std::norm(a12);
+                lam_new = avg - std::sqrt(diff * diff + off);
+                if (off > 1e-30) {
+                    c0 = -a12;
+                    c1 = cplx(a11 - lam_new, 0.0);
+                } else if (a11 <= a22) {
+                    c0 = 1.0;
+                    c1 = 0.0;
+                } else {
+                    c0 = 0.0;
+                    c1 = 1.0;
+                }
+                const double cn = std::sqrt(std::norm(c0) + std::norm(c1));
+                if (cn > 1e-30) {
+                    c0 /= cn;
+                    c1 /= cn;
+                }
+                c2 = 0.0;
+            }
+            R.copy(pold, psi);
+            R.copy(hpold, hpsi);
+            has_prev = true;
+            R.combine(pold, d, pp, c0, c1, c2, three, psi);
+            R.combine(hpold, hd, hpp, c0, c1, c2, three, hpsi);
+            const double pn2 = R.nrm(psi);
+            if (pn2 > 1e-14) {
+                R.scal(1.0 / pn2, psi);
+                R.scal(1.0 / pn2, hpsi);
+            }
+            const double change = std::fabs(lam_new - lam);
+            lam = lam_new;
+            if (change < tol) {
+                conv = true;
+                break;
+            }
+        }
+        vals[b] = lam;
+        all_conv = all_conv && conv;
+        if (b > 0) {
+            R.orth(psi, V, b);
+            R.scal(1.0 / R.nrm(psi), psi);
+        }
+    }
+    // Final subspace rotation: T = V^H H V, eigen-decomposition, X = V C (cg.zig:450-482).
+    double2* HV = w.HX.as<double2>();
+    ham_apply_dev<double>(h, V, HV, nb, true, true);
+    st.apply_calls += nb;
+    st.fp64_applies += nb;
+    const zc one(1.0, 0.0), zero(0.0, 0.0);
+    DZ_BLAS(rocblas_zgemm(w.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, nb, nb, (int)n, &one,
+                          reinterpret_cast<const zc*>(V), (int)n, reinterpret_cast<const zc*>(HV), (int)n, &zero,
+                          w.Tw.as<zc>(), nb));
+    DZ_BLAS(rocsolver_zheevd(w.blas, rocblas_evect_original, rocblas_fill_lower, nb, w.Tw.as<zc>(), nb,
+                             w.evals.as<double>(), w.ework.as<double>(), w.info.as<rocblas_int>()));
+    DZ_BLAS(rocblas_zgemm(w.blas, rocblas_operation_none, rocblas_operation_none, (int)n, nb, nb, &one,
+                          reinterpret_cast<const zc*>(V), (int)n, w.Tw.as<zc>(), nb, &zero, w.X.as<zc>(), (int)n));
+    rocblas_int info = 0;
+    DZ_HIP(hipMemcpyAsync(values, w.evals.ptr, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+    DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
+    if (vectors) DZ_HIP(hipMemcpyAsync(vectors, w.X.ptr, n * nb * sizeof(double2), hipMemcpyDeviceToHost, s));
+    DZ_HIP(hipStreamSynchronize(s));
+    DZ_REQUIRE(info == 0, DZHIP_ERR_HIP, "zheevd did not converge");
+    w.x_cols = nb;
+    w.x_converged = all_conv;
+    st.converged = all_conv ? 1 : 0;
+    if (stats_out) *stats_out = st;
 }
 
 void eig_release(Ham& h) { eig_work_release(h); }

@@ -115,6 +115,9 @@ void ham_density_dev(Ham& h, const double2* psi, size_t nbands, const double* wt
 void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts& o, const double* init,
                uint64_t init_cols, double* values, double* vectors, dzhip_eig_stats* stats);
 void eig_release(Ham& h);
+// Band-by-band CG with three-term Rayleigh-Ritz and a final subspace rotation (FP64).
+void cg_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts& o, const double* init,
+              uint64_t init_cols, double* values, double* vectors, dzhip_eig_stats* stats);
 // Ritz vectors [npw x cols] (FP64, device) the last eig_solve on h left, or nullptr.
 const double2* eig_solution(const Ham& h, uint64_t* cols);
 // Training samples from the snapshots the last solve recorded (opts.collect_snapshots) against

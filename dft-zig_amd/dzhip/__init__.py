@@ -34,7 +34,7 @@ EXPORTS = (
     "dzhip_ham_set_stream", "dzhip_ham_sync", "dzhip_ham_profile", "dzhip_ham_phase_times",
     "dzhip_ham_info", "dzhip_eig_solve", "dzhip_density_accumulate",
     "dzhip_density_accumulate_solution", "dzhip_ham_set_precond_model", "dzhip_ham_training_samples",
-    "dzhip_mlp_loss_grad", "dzhip_mlp_train",
+    "dzhip_mlp_loss_grad", "dzhip_mlp_train", "dzhip_cg_solve",
 )
 
 
@@ -84,7 +84,9 @@ class EigStats(ct.Structure):
         ("iterations", ct.c_uint64), ("apply_calls", ct.c_uint64), ("fp32_applies", ct.c_uint64),
         ("fp64_applies", ct.c_uint64), ("restarts", ct.c_uint64), ("precision_switch_iter", ct.c_uint64),
         ("converged", ct.c_uint64), ("max_residual", ct.c_double), ("initial_residual", ct.c_double),
-        ("precond_fallback", ct.c_uint64),
+        ("precond_fallback", ct.c_uint64), ("t_rr_ms", ct.c_double), ("t_ritz_ms", ct.c_double),
+        ("t_precond_ms", ct.c_double), ("t_ortho_ms", ct.c_double), ("t_apply_ms", ct.c_double),
+        ("t_extend_ms", ct.c_double),
     ]
 
     def as_dict(self):
@@ -145,6 +147,7 @@ def _load():
         "dzhip_ham_training_samples": ([vp, ct.c_double, u64, fp, u64, ct.POINTER(ct.c_uint64)], i32),
         "dzhip_mlp_loss_grad": ([vp, fp, u64, fp, dp, dp], i32),
         "dzhip_mlp_train": ([vp, fp, u64, fp, u64, u64, ct.c_double, u64, dp], i32),
+        "dzhip_cg_solve": ([vp, dp, u64, u64, ct.POINTER(EigOpts), dp, u64, dp, dp, ct.POINTER(EigStats)], i32),
     }
     missing = set(EXPORTS) - set(sig)
     if missing:
@@ -424,6 +427,27 @@ class Hamiltonian:
         check(L().dzhip_density_accumulate(self.handle, _dp(psi.view(np.float64)), psi.shape[1], psi.shape[0],
                                            _dp(w), _dp(out), total, 1 if acc else 0))
         return out
+
+    def cg_solve(self, nbands: int, diag=None, tol: float = 0.0, max_iter: int = 0, init=None, seed: int = 0,
+                 want_vectors: bool = True, reuse: bool = False):
+        """Band-by-band CG (cg.solve counterpart; tol 0 = 1e-6, max_iter 0 = 20).
+        Returns (values[nbands], vectors (nbands, n) or None, stats dict)."""
+        if diag is None:
+            diag = self._kinetic
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        o = EigOpts(max_iter, tol, 0, 0, 0, FP64, 0.0, PRECOND_KINETIC, seed, 1 if reuse else 0, 0)
+        vals = np.zeros(nbands)
+        vecs = np.empty((nbands, self.n), dtype=np.complex128) if want_vectors else None
+        st = EigStats()
+        if init is not None:
+            init = np.ascontiguousarray(init, dtype=np.complex128)
+            ip, ic = _dp(init.view(np.float64)), init.shape[0]
+        else:
+            ip, ic = ct.POINTER(ct.c_double)(), 0
+        vp = _dp(vecs.view(np.float64)) if vecs is not None else ct.POINTER(ct.c_double)()
+        check(L().dzhip_cg_solve(self.handle, _dp(diag), diag.shape[0], nbands, ct.byref(o), ip, ic,
+                                 _dp(vals), vp, ct.byref(st)))
+        return vals, vecs, st.as_dict()
 
     def set_precond_model(self, params, energy_scale: float):
         """Install the learned preconditioner's parameters (MLP_PARAMS floats)."""

@@ -34,6 +34,7 @@ from . import FP64, PRECOND_KINETIC, Context, Hamiltonian, tables
 from .workloads import SpeciesDesc
 
 HA_PER_RY = 0.5
+SECTIONS = ("t_rr_ms", "t_ritz_ms", "t_precond_ms", "t_ortho_ms", "t_apply_ms", "t_extend_ms")
 
 
 @dataclass
@@ -278,6 +279,7 @@ class ScfRecord:
     applies: int
     fp32_applies: int
     eig_s: float  # wall time inside the k-point solves and densities
+    sections_ms: dict = field(default_factory=dict)  # solver GPU time per section, summed over k
 
 
 @dataclass
@@ -384,7 +386,8 @@ class ScfDriver:
             drho = math.sqrt(float(np.mean((rho_out - rho) ** 2)))
             run.records.append(ScfRecord(it, 0.0, e_tot, e_band, drho,
                                          sum(r[2]["iterations"] for r in res), sum(r[2]["apply_calls"] for r in res),
-                                         sum(r[2]["fp32_applies"] for r in res), eig_s))
+                                         sum(r[2]["fp32_applies"] for r in res), eig_s,
+                                         {k: sum(r[2][k] for r in res) for k in SECTIONS}))
             run.eigenvalues = [r[0] for r in res]
             rho = rho_out
             drho_prev = drho

@@ -197,6 +197,11 @@ typedef struct dzhip_eig_stats {
     double initial_residual;        /* largest residual of the start block */
     uint64_t precond_fallback;      /* DZHIP_PRECOND_MLP: iteration at which the solve fell back
                                        to TPA (0: never) */
+    /* GPU time (ms, HIP events on the context's stream) per section of the solve:
+     * Rayleigh-Ritz (T copy + zheevd), Ritz vectors + residuals + norms, preconditioner,
+     * orthonormalisation (CGS2 + Cholesky QR), H apply of new vectors, T extension/restart.
+     * Host waits between sections (small downloads) fall into the section that follows. */
+    double t_rr_ms, t_ritz_ms, t_precond_ms, t_ortho_ms, t_apply_ms, t_extend_ms;
 } dzhip_eig_stats;
 
 /* diag: [n] preconditioner diagonal (|k+G|^2, kpoint/solve.zig:362-367).  init_vecs: NULL or
@@ -206,6 +211,16 @@ typedef struct dzhip_eig_stats {
 int dzhip_eig_solve(dzhip_ham* ham, const double* diag, uint64_t n, uint64_t nbands,
                     const dzhip_eig_opts* opts, const double* init_vecs, uint64_t init_cols,
                     double* values_out, double* vectors_out, dzhip_eig_stats* stats);
+
+/* Band-by-band conjugate-gradient eigensolver, the `solver = "cg"` path: replaces cg.solve
+ * (src/features/linalg/cg/cg.zig:79-116): per band, preconditioned (kinetic shift) CG with a
+ * three-term Rayleigh-Ritz on {psi, d, psi_prev} (two-term closed form when psi_prev adds
+ * nothing), band converged when |r| < tol or |delta lambda| < tol; then a subspace rotation of
+ * the whole block (cg.zig:450-482).  Uses opts->max_iter (0 = 20), tol (0 = 1e-6), seed and
+ * reuse_vectors; FP64 apply.  stats->iterations counts CG steps over all bands. */
+int dzhip_cg_solve(dzhip_ham* ham, const double* diag, uint64_t n, uint64_t nbands,
+                   const dzhip_eig_opts* opts, const double* init_vecs, uint64_t init_cols,
+                   double* values_out, double* vectors_out, dzhip_eig_stats* stats);
 
 /* ------------------------------------------------------------------ density
  * rho(r) (+)= (1/Omega) sum_b w_b |psi_b(r)|^2 with psi_b(r) = sum_G c_bG e^{iG.r}, FP64, rho in

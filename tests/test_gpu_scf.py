@@ -57,24 +57,48 @@ def test_scf_trajectory_matches_oracle(dz, ctx, gold, system):
     assert l1 < RHO_L1
 
 
-@pytest.mark.parametrize("precision,concurrent,precond", [(FP64, False, 0), (FP32, True, 0), (FP32, True, PRECOND_TPA)])
-def test_scf_reference_schedule(dz, ctx, gold, system, precision, concurrent, precond):
-    """The reference's eigensolver schedule (warm-up tol 1e-3, then 1e-4 with reused vectors),
-    FP64 or mixed FP32-apply, k-points serial or on their own streams: same fixed point."""
-    drv = dscf.ScfDriver(system, concurrent_k=concurrent)
+def test_scf_reference_fixed_tolerance(dz, ctx, gold, system):
+    """The reference's eigensolver schedule as written (warm-up tol 1e-3, then a fixed 1e-4 Ry with
+    reused vectors): the energy reaches the oracle's within 1e-6 Ha, but once the start residual
+    is below 1e-4 the solver returns at once, so the density is built from vectors with 1e-4
+    residuals and the fixed point (eigenvalues) is only good to about 1e-5 Ry."""
+    drv = dscf.ScfDriver(system)
     try:
-        run = drv.run(max_iter=60, conv=1e-9, precision=precision, preconditioner=precond, final_tol=1e-10)
+        run = drv.run(max_iter=60, conv=1e-9, precision=FP64, final_tol=1e-10)
     finally:
         drv.close()
     e_ref = gold["energies"][-1]
-    print("iterations", len(run.records), "E - E_ref", run.energies[-1] - e_ref,
-          "t(1e-6 Ha)", run.time_to(e_ref), [(r.eig_iterations, r.fp32_applies) for r in run.records])
+    ev_err = np.max(np.abs(np.array(run.eigenvalues) - gold["eigenvalues"]))
+    print("iterations", len(run.records), "E - E_ref", run.energies[-1] - e_ref, "eigenvalue error", ev_err,
+          [r.eig_iterations for r in run.records])
     assert run.converged
     assert abs(run.energies[-1] - e_ref) < E_TOL
     assert run.time_to(e_ref) is not None
+    assert ev_err < 1e-4
+
+
+@pytest.mark.parametrize("precision,concurrent,precond", [(FP64, False, 0), (FP32, True, 0), (FP32, True, PRECOND_TPA)])
+def test_scf_adaptive_tolerance(dz, ctx, gold, system, precision, concurrent, precond):
+    """Eigensolver tolerance tied to the density change (min(1e-4, max(1e-9, drho))): FP64 or
+    mixed FP32-apply (FP64 below 1e-5), k-points serial or on their own streams, kinetic or TPA
+    preconditioner: the oracle's fixed point within the eigenvalue, energy and density bars."""
+    drv = dscf.ScfDriver(system, concurrent_k=concurrent)
+    try:
+        run = drv.run(max_iter=60, conv=1e-9, precision=precision, preconditioner=precond, final_tol=1e-10,
+                      eig_tol_ratio=1.0, eig_tol_min=1e-9)
+    finally:
+        drv.close()
+    e_ref = gold["energies"][-1]
     ev = np.array(run.eigenvalues)
+    l1 = _l1(system, run.rho, gold["rho"])
+    print("iterations", len(run.records), "E - E_ref", run.energies[-1] - e_ref, "t(1e-6 Ha)", run.time_to(e_ref),
+          "eigenvalue error", np.max(np.abs(ev - gold["eigenvalues"])), "density L1", l1,
+          [(r.eig_iterations, r.fp32_applies) for r in run.records])
+    assert run.converged
+    assert abs(run.energies[-1] - e_ref) < E_TOL
+    assert run.time_to(e_ref) is not None
     assert np.max(np.abs(ev - gold["eigenvalues"])) < EV_TOL
-    assert _l1(system, run.rho, gold["rho"]) < 1e-6
+    assert l1 < RHO_L1
     if precision == FP32:
         assert sum(r.fp32_applies for r in run.records) > 0
 

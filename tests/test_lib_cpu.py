@@ -61,3 +61,32 @@ def test_fft_core_runs_on_host_against_dft(tmp_path):
     assert r.returncode == 0, r.stderr[-2000:]
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and "FFT core ok" in r.stdout, r.stdout[-3000:]
+
+
+def test_herm3_jacobi_host(tmp_path):
+    """The CG solver's 3x3 Hermitian eigensolver (csrc/herm3.h, host code) against numpy eigh,
+    including degenerate and already-diagonal matrices."""
+    import subprocess
+
+    exe = tmp_path / "herm3_host"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", str(ROOT / "dft-zig_amd" / "csrc"),
+                    str(ROOT / "tests" / "native" / "herm3_host.cpp"), "-o", str(exe)], check=True)
+    rng = np.random.default_rng(3)
+    mats = []
+    for _ in range(200):
+        a = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+        mats.append(a + a.conj().T)
+    mats.append(np.diag([3.0, -1.0, 2.0]).astype(complex))
+    mats.append(np.diag([1.0, 1.0, 2.0]).astype(complex))
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3)))
+    mats.append(q @ np.diag([0.5, 0.5, 4.0]) @ q.conj().T)
+    inp = "\n".join(" ".join(f"{z.real:.17g} {z.imag:.17g}" for z in m.ravel()) for m in mats)
+    out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, check=True).stdout.split("\n")
+    for m, line in zip(mats, out):
+        vals = [float(t) for t in line.split()]
+        lam = vals[0]
+        v = np.array([vals[1] + 1j * vals[2], vals[3] + 1j * vals[4], vals[5] + 1j * vals[6]])
+        ev = np.linalg.eigvalsh(m)
+        assert abs(lam - ev[0]) < 1e-12 * max(1.0, abs(ev[0]))
+        assert abs(np.linalg.norm(v) - 1.0) < 1e-12
+        assert np.linalg.norm(m @ v - lam * v) < 1e-11 * max(1.0, np.max(np.abs(ev)))
