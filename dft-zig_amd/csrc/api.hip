@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "ham.h"
+#include "potential.h"
 
 namespace dzhip {
 
@@ -48,6 +49,7 @@ using namespace dzhip;
 struct dzhip_ctx : Context {};
 struct dzhip_fft_plan : FftPlan {};
 struct dzhip_ham : Ham {};
+struct dzhip_scf_grid : ScfGrid {};
 
 namespace dzhip {
 // rho (host, ngrid) (+)= density of the nbands device columns psi, through the context's
@@ -497,6 +499,123 @@ int dzhip_mlp_train(dzhip_ctx* ctx, const float* samples, uint64_t n, float* par
         }
         DZ_HIP(hipMemcpyAsync(params, t.prm.ptr, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, s));
         DZ_HIP(hipStreamSynchronize(s));
+    });
+}
+
+dzhip_scf_grid* dzhip_scf_grid_create(dzhip_ctx* ctx, uint64_t nx, uint64_t ny, uint64_t nz, const double* recip,
+                                      double volume, uint64_t pulay_history) {
+    dzhip_scf_grid* out = nullptr;
+    const int rc = guarded([&] {
+        DZ_REQUIRE(ctx, DZHIP_ERR_STATE, "NULL ctx");
+        DZ_REQUIRE(recip && volume > 0.0, DZHIP_ERR_ARG, "NULL recip or volume <= 0");
+        DZ_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && nx <= 1024 && ny <= 1024 && nz <= 1024, DZHIP_ERR_SIZE,
+                   "grid dimensions must be in 1..1024");
+        DZ_REQUIRE(pulay_history <= 64, DZHIP_ERR_ARG, "pulay_history <= 64");
+        DeviceGuard g(ctx->device);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        out = static_cast<dzhip_scf_grid*>(
+            scf_grid_create(ctx, (int)nx, (int)ny, (int)nz, recip, volume, (int)pulay_history));
+    });
+    return rc == DZHIP_OK ? out : nullptr;
+}
+
+void dzhip_scf_grid_destroy(dzhip_scf_grid* grid) {
+    if (!grid) return;
+    DeviceGuard g(grid->ctx->device);
+    scf_grid_destroy(grid);
+}
+
+int dzhip_scf_grid_set_ion(dzhip_scf_grid* grid, const double* v_ion, uint64_t count) {
+    return guarded([&] {
+        DZ_REQUIRE(grid, DZHIP_ERR_STATE, "NULL grid");
+        DZ_REQUIRE(v_ion && count == grid->n, DZHIP_ERR_SIZE, "count != nx*ny*nz");
+        std::lock_guard<std::mutex> lk(grid->mu);
+        DeviceGuard g(grid->ctx->device);
+        DZ_HIP(hipMemcpyAsync(grid->v_ion.ptr, v_ion, count * sizeof(double), hipMemcpyHostToDevice, grid->stream()));
+        DZ_HIP(hipStreamSynchronize(grid->stream()));
+    });
+}
+
+int dzhip_scf_grid_init(dzhip_scf_grid* grid, double nelec) {
+    return guarded([&] {
+        DZ_REQUIRE(grid, DZHIP_ERR_STATE, "NULL grid");
+        std::lock_guard<std::mutex> lk(grid->mu);
+        DeviceGuard g(grid->ctx->device);
+        scf_grid_init(*grid, nelec);
+    });
+}
+
+int dzhip_ham_set_local_grid(dzhip_ham* ham, dzhip_scf_grid* grid) {
+    return guarded([&] {
+        DZ_REQUIRE(ham && grid, DZHIP_ERR_STATE, "NULL ham or grid");
+        DZ_REQUIRE(ham->ngrid == grid->n && ham->nx == grid->nx && ham->ny == grid->ny, DZHIP_ERR_SIZE,
+                   "grid dimensions differ");
+        DZ_REQUIRE(ham->ctx->device == grid->ctx->device, DZHIP_ERR_ARG, "ham and grid on different devices");
+        std::lock_guard<std::mutex> lg(grid->mu);
+        std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        scf_grid_local(*grid);
+        DZ_HIP(hipStreamSynchronize(grid->stream()));
+        ham_set_local_dev(*ham, grid->local.as<double>());
+        DZ_HIP(hipStreamSynchronize(ham->stream()));
+    });
+}
+
+int dzhip_scf_grid_add_solution_density(dzhip_scf_grid* grid, dzhip_ham* ham, const double* weights,
+                                        uint64_t nbands) {
+    return guarded([&] {
+        DZ_REQUIRE(ham && grid, DZHIP_ERR_STATE, "NULL ham or grid");
+        DZ_REQUIRE(ham->ngrid == grid->n, DZHIP_ERR_SIZE, "grid dimensions differ");
+        DZ_REQUIRE(nbands == 0 || weights, DZHIP_ERR_ARG, "NULL weights");
+        if (nbands == 0) return;
+        std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        uint64_t cols = 0;
+        const double2* x = eig_solution(*ham, &cols);
+        DZ_REQUIRE(x && nbands <= cols, DZHIP_ERR_STATE, "no eigensolver solution with that many bands");
+        hipStream_t s = ham->stream();
+        ham->dens_rho.ensure(ham->ngrid * sizeof(double));
+        ham->dens_w.ensure(nbands * sizeof(double));
+        DZ_HIP(hipMemcpyAsync(ham->dens_w.ptr, weights, nbands * sizeof(double), hipMemcpyHostToDevice, s));
+        ham_density_dev(*ham, x, nbands, ham->dens_w.as<double>(), ham->dens_rho.as<double>(), false);
+        DZ_HIP(hipStreamSynchronize(s));
+        std::lock_guard<std::mutex> lg(grid->mu);
+        scf_grid_add(*grid, ham->dens_rho.as<double>());
+        DZ_HIP(hipStreamSynchronize(grid->stream()));
+    });
+}
+
+int dzhip_scf_grid_hxc(dzhip_scf_grid* grid, double* out, uint64_t n_out) {
+    return guarded([&] {
+        DZ_REQUIRE(grid, DZHIP_ERR_STATE, "NULL grid");
+        DZ_REQUIRE(out && n_out >= 4, DZHIP_ERR_ARG, "out needs 4 entries");
+        std::lock_guard<std::mutex> lk(grid->mu);
+        DeviceGuard g(grid->ctx->device);
+        scf_grid_hxc(*grid, out);
+    });
+}
+
+int dzhip_scf_grid_mix(dzhip_scf_grid* grid, double beta, int pulay) {
+    return guarded([&] {
+        DZ_REQUIRE(grid, DZHIP_ERR_STATE, "NULL grid");
+        std::lock_guard<std::mutex> lk(grid->mu);
+        DeviceGuard g(grid->ctx->device);
+        scf_grid_mix(*grid, beta, pulay);
+        DZ_HIP(hipStreamSynchronize(grid->stream()));
+    });
+}
+
+int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t count) {
+    return guarded([&] {
+        DZ_REQUIRE(grid, DZHIP_ERR_STATE, "NULL grid");
+        DZ_REQUIRE(out && count == grid->n, DZHIP_ERR_SIZE, "count != nx*ny*nz");
+        DevBuf* src[] = {&grid->rho_in, &grid->v_in, &grid->v_out, &grid->local, &grid->v_ion};
+        DZ_REQUIRE(which >= 0 && which < 5, DZHIP_ERR_ARG, "which must be DZHIP_GRID_*");
+        std::lock_guard<std::mutex> lk(grid->mu);
+        DeviceGuard g(grid->ctx->device);
+        if (which == DZHIP_GRID_LOCAL) scf_grid_local(*grid);
+        DZ_HIP(hipMemcpyAsync(out, src[which]->ptr, count * sizeof(double), hipMemcpyDeviceToHost, grid->stream()));
+        DZ_HIP(hipStreamSynchronize(grid->stream()));
     });
 }
 

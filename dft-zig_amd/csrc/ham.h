@@ -126,6 +126,18 @@ size_t eig_training_samples(Ham& h, double energy_scale, int stride, float* out_
 
 void ham_build(Ham& h, const dzhip_ham_desc& d);
 void ham_set_local(Ham& h, const double* local_r);
+// local_r already on the device (read on h's stream).
+void ham_set_local_dev(Ham& h, const double* local_dev);
+
+// ----------------------------------------------------------------- SCF grid (potential.hip)
+struct ScfGrid;
+ScfGrid* scf_grid_create(Context* ctx, int nx, int ny, int nz, const double* recip, double volume, int history);
+void scf_grid_destroy(ScfGrid* g);
+void scf_grid_init(ScfGrid& g, double nelec);
+void scf_grid_local(ScfGrid& g);
+void scf_grid_add(ScfGrid& g, const double* rho_k);
+void scf_grid_hxc(ScfGrid& g, double out[4]);
+void scf_grid_mix(ScfGrid& g, double beta, int pulay);
 void ham_release(Ham& h);
 // Device apply; x/y hold C2T<T> elements, column-major with leading dimension npw.
 template <typename T>

@@ -790,6 +790,24 @@ void ham_set_local(Ham& h, const double* local_r) {
     h.has_local = true;
 }
 
+__global__ void k_scale_local(const double* __restrict__ in, double inv_n, double* __restrict__ v64,
+                              float* __restrict__ v32, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = in[i] * inv_n;
+    v64[i] = v;
+    v32[i] = (float)v;
+}
+
+void ham_set_local_dev(Ham& h, const double* local_dev) {
+    h.vloc64.ensure(h.ngrid * sizeof(double));
+    h.vloc32.ensure(h.ngrid * sizeof(float));
+    hipLaunchKernelGGL(k_scale_local, dim3((unsigned)((h.ngrid + 255) / 256)), dim3(256), 0, h.stream(), local_dev,
+                       1.0 / (double)h.ngrid, h.vloc64.as<double>(), h.vloc32.as<float>(), h.ngrid);
+    DZ_HIP(hipGetLastError());
+    h.has_local = true;
+}
+
 void ham_release(Ham& h) {
     eig_release(h);
     DevBuf* bufs[] = {&h.plane_start, &h.plane_pw, &h.plane_pos, &h.plane_xpos, &h.plane_ystart,

@@ -1,0 +1,30 @@
+// Device-resident SCF grid state (density, potentials, Pulay history); see potential.hip.
+#pragma once
+
+#include <mutex>
+#include <vector>
+
+#include "ham.h"
+
+namespace dzhip {
+
+struct ScfGrid {
+    Context* ctx = nullptr;
+    int nx = 0, ny = 0, nz = 0;
+    size_t n = 0;
+    double volume = 0.0;
+    AxisTables tab;
+    DevBuf g2, work, rho_in, rho_out, v_in, v_out, v_ion, local, part, hv, hr, slots, coef;
+    int hist_cap = 0;
+    std::vector<int> order;              // Pulay ring slots, oldest first
+    std::vector<std::vector<double>> b;  // overlaps <R_i|R_j>/N by slot
+    std::mutex mu;
+
+    hipStream_t stream() const { return ctx->stream; }
+    double reduce(const double* part_dev, int nblocks, int stride, int comp);
+    double dot(const double* a, const double* c);
+    // v_out = V_H[rho] + v_xc[rho]; out = E_H, E_xc, int v_in rho (Ry), RMS(rho - rho_ref).
+    void hxc(const double* rho, const double* rho_ref, const double* vin, double* vout, double out[4]);
+};
+
+}  // namespace dzhip

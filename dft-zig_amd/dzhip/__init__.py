@@ -35,6 +35,9 @@ EXPORTS = (
     "dzhip_ham_info", "dzhip_eig_solve", "dzhip_density_accumulate",
     "dzhip_density_accumulate_solution", "dzhip_ham_set_precond_model", "dzhip_ham_training_samples",
     "dzhip_mlp_loss_grad", "dzhip_mlp_train", "dzhip_cg_solve",
+    "dzhip_scf_grid_create", "dzhip_scf_grid_destroy", "dzhip_scf_grid_set_ion", "dzhip_scf_grid_init",
+    "dzhip_ham_set_local_grid", "dzhip_scf_grid_add_solution_density", "dzhip_scf_grid_hxc",
+    "dzhip_scf_grid_mix", "dzhip_scf_grid_read",
 )
 
 
@@ -148,6 +151,15 @@ def _load():
         "dzhip_mlp_loss_grad": ([vp, fp, u64, fp, dp, dp], i32),
         "dzhip_mlp_train": ([vp, fp, u64, fp, u64, u64, ct.c_double, u64, dp], i32),
         "dzhip_cg_solve": ([vp, dp, u64, u64, ct.POINTER(EigOpts), dp, u64, dp, dp, ct.POINTER(EigStats)], i32),
+        "dzhip_scf_grid_create": ([vp, u64, u64, u64, dp, ct.c_double, u64], vp),
+        "dzhip_scf_grid_destroy": ([vp], None),
+        "dzhip_scf_grid_set_ion": ([vp, dp, u64], i32),
+        "dzhip_scf_grid_init": ([vp, ct.c_double], i32),
+        "dzhip_ham_set_local_grid": ([vp, vp], i32),
+        "dzhip_scf_grid_add_solution_density": ([vp, vp, dp, u64], i32),
+        "dzhip_scf_grid_hxc": ([vp, dp, u64], i32),
+        "dzhip_scf_grid_mix": ([vp, ct.c_double, i32], i32),
+        "dzhip_scf_grid_read": ([vp, i32, dp, u64], i32),
     }
     missing = set(EXPORTS) - set(sig)
     if missing:
@@ -482,6 +494,61 @@ class Hamiltonian:
     def close(self):
         if self.handle:
             L().dzhip_ham_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+GRID_RHO, GRID_VIN, GRID_VOUT, GRID_LOCAL, GRID_VION = range(5)
+
+
+class ScfGrid:
+    """Device-resident SCF grid state (dzhip_scf_grid): density sum, V_in/V_out, Pulay history."""
+
+    def __init__(self, ctx: Context, grid, recip, volume: float, pulay_history: int = 8):
+        self.grid = tuple(int(v) for v in grid)
+        self.n = int(np.prod(self.grid))
+        r = np.ascontiguousarray(np.asarray(recip, dtype=np.float64).reshape(9))
+        self.handle = L().dzhip_scf_grid_create(ctx.handle, *self.grid, _dp(r), float(volume), pulay_history)
+        if not self.handle:
+            raise BadArgument(f"dzhip_scf_grid_create failed: {last_error()}")
+        self._ctx = ctx
+
+    def set_ion(self, v_ion):
+        v = np.ascontiguousarray(v_ion, dtype=np.float64)
+        check(L().dzhip_scf_grid_set_ion(self.handle, _dp(v), v.shape[0]))
+
+    def init(self, nelec: float):
+        check(L().dzhip_scf_grid_init(self.handle, float(nelec)))
+
+    def set_local(self, ham: "Hamiltonian"):
+        check(L().dzhip_ham_set_local_grid(ham.handle, self.handle))
+
+    def add_density(self, ham: "Hamiltonian", weights):
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        check(L().dzhip_scf_grid_add_solution_density(self.handle, ham.handle, _dp(w), w.shape[0]))
+
+    def hxc(self):
+        """(E_H, E_xc, int V_in rho, RMS drho) of the density sum, which becomes rho."""
+        out = np.zeros(4)
+        check(L().dzhip_scf_grid_hxc(self.handle, _dp(out), 4))
+        return tuple(float(v) for v in out)
+
+    def mix(self, beta: float, pulay: bool):
+        check(L().dzhip_scf_grid_mix(self.handle, float(beta), 1 if pulay else 0))
+
+    def read(self, which: int):
+        out = np.zeros(self.n)
+        check(L().dzhip_scf_grid_read(self.handle, which, _dp(out), self.n))
+        return out
+
+    def close(self):
+        if self.handle:
+            L().dzhip_scf_grid_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -13,7 +13,7 @@ from pathlib import Path
 PKG_DIR = Path(__file__).resolve().parent
 CSRC = PKG_DIR.parent / "csrc"
 LIB_PATH = PKG_DIR / "libdzhip.so"
-SOURCES = ["api.hip", "fft.hip", "hpsi.hip", "eig.hip", "precond.hip"]
+SOURCES = ["api.hip", "fft.hip", "hpsi.hip", "eig.hip", "precond.hip", "potential.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -30,7 +30,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import FP64, PRECOND_KINETIC, Context, Hamiltonian, tables
+from . import FP64, PRECOND_KINETIC, Context, Hamiltonian, ScfGrid, tables
 from .workloads import SpeciesDesc
 
 HA_PER_RY = 0.5
@@ -310,7 +310,10 @@ class ScfRun:
 class ScfDriver:
     """One Hamiltonian context per k-point, built once and kept across SCF iterations."""
 
-    def __init__(self, system: ScfSystem, concurrent_k: bool = False, device: int = 0):
+    def __init__(self, system: ScfSystem, concurrent_k: bool = False, device: int = 0, device_grid: bool = False,
+                 pulay_history: int = 8):
+        """device_grid keeps rho, V and the Pulay history on the GPU (dzhip_scf_grid); otherwise
+        potentials and mixing run on the host in FP64 numpy."""
         t0 = time.perf_counter()
         s = self.sys = system
         self.concurrent = concurrent_k and len(s.kpoints) > 1
@@ -327,15 +330,67 @@ class ScfDriver:
             self.hams.append(Hamiltonian(ctx, s.grid, tables.fft_indices(b, s.grid), b.kinetic, b.cart, zero,
                                          1.0 / s.volume, species, s.atom_pos, s.atom_species))
         self.npw = [h.n for h in self.hams]
+        self.grid = None
+        if device_grid:
+            self.grid = ScfGrid(self.ctxs[0], s.grid, s.recip, s.volume, pulay_history)
+            self.grid.set_ion(self.v_ion)
         self.pool = ThreadPoolExecutor(len(self.hams)) if self.concurrent else None
         self.setup_s = time.perf_counter() - t0
 
     def _solve_k(self, k, local_r, nb, opts, weights):
         ham = self.hams[k]
-        ham.set_local(local_r)
+        if self.grid is not None:
+            self.grid.set_local(ham)
+        else:
+            ham.set_local(local_r)
         vals, _, st = ham.eig_solve(nb, want_vectors=False, reuse=True, **opts)
-        rho_k = ham.density_solution(weights)
-        return vals, rho_k, st
+        if self.grid is not None:
+            self.grid.add_density(ham, weights)
+            return vals, None, st
+        return vals, ham.density_solution(weights), st
+
+    def _run_device(self, max_iter, beta, conv, pulay_history, pulay_start, warmup_steps, solve_opts, progress,
+                    energy_tol_ry):
+        """The SCF loop with rho, V and mixing on the device (same algorithm as run())."""
+        s = self.sys
+        g = self.grid
+        g.init(s.nelec)
+        weights = [band_weights(s.nelec, s.nbands, wk) for wk in s.kweights]
+        run = ScfRun(setup_s=self.setup_s)
+        t0 = time.perf_counter()
+        drho_prev = None
+        for it in range(max_iter):
+            opts = solve_opts(it, drho_prev)
+            te = time.perf_counter()
+            ks = range(len(self.hams))
+            if self.pool is not None:
+                res = list(self.pool.map(lambda k: self._solve_k(k, None, s.nbands, opts, weights[k]), ks))
+            else:
+                res = [self._solve_k(k, None, s.nbands, opts, weights[k]) for k in ks]
+            eig_s = time.perf_counter() - te
+            e_band = float(sum(np.dot(w, r[0]) for w, r in zip(weights, res)))
+            e_h, e_xc, vin_rho, drho = g.hxc()
+            e_tot = e_band - vin_rho + e_h + e_xc
+            run.records.append(ScfRecord(it, 0.0, e_tot, e_band, drho,
+                                         sum(r[2]["iterations"] for r in res), sum(r[2]["apply_calls"] for r in res),
+                                         sum(r[2]["fp32_applies"] for r in res), eig_s,
+                                         {k: sum(r[2][k] for r in res) for k in SECTIONS}))
+            run.eigenvalues = [r[0] for r in res]
+            drho_prev = drho
+            done = drho < conv
+            if energy_tol_ry is not None and it >= warmup_steps and len(run.records) > 1:
+                done = done or abs(run.records[-1].energy_ry - run.records[-2].energy_ry) < energy_tol_ry
+            if not done:
+                g.mix(beta, pulay_history > 0 and it >= pulay_start)
+            run.records[-1].wall_s = time.perf_counter() - t0
+            if progress is not None:
+                progress(run.records[-1])
+            if done:
+                run.converged = True
+                break
+        run.rho = g.read(0)
+        run.v_in = g.read(1)
+        return run
 
     def run(self, max_iter=60, beta=0.3, conv=1e-9, precision=FP64, preconditioner=PRECOND_KINETIC,
             pulay_history=8, pulay_start=4, eig_tol=1e-4, eig_max_iter=20, warmup_steps=2, warmup_tol=1e-3,
@@ -351,6 +406,20 @@ class ScfDriver:
         for 128 bands across a 0.08 Ry gap, which a converged total energy cannot carry.
         progress, if given, is called with each ScfRecord."""
         s = self.sys
+
+        def solve_opts(it, drho_prev):
+            warm = it < warmup_steps
+            tol = warmup_tol if warm else eig_tol
+            if not warm and eig_tol_ratio is not None and drho_prev is not None:
+                tol = min(eig_tol, max(eig_tol_min, eig_tol_ratio * drho_prev))
+            return dict(tol=tol, max_iter=warmup_max_iter if warm else eig_max_iter, precision=precision,
+                        preconditioner=preconditioner, mixed_switch_tol=mixed_switch_tol)
+
+        if self.grid is not None:
+            run = self._run_device(max_iter, beta, conv, pulay_history, pulay_start, warmup_steps, solve_opts,
+                                   progress, energy_tol_ry)
+            self._final(run, final_tol, preconditioner)
+            return run
         n = int(np.prod(s.grid))
         vol = s.volume
         rho = np.full(n, s.nelec / vol)
@@ -362,11 +431,7 @@ class ScfDriver:
         drho_prev = None
         for it in range(max_iter):
             warm = it < warmup_steps
-            tol = warmup_tol if warm else eig_tol
-            if not warm and eig_tol_ratio is not None and drho_prev is not None:
-                tol = min(eig_tol, max(eig_tol_min, eig_tol_ratio * drho_prev))
-            opts = dict(tol=tol, max_iter=warmup_max_iter if warm else eig_max_iter,
-                        precision=precision, preconditioner=preconditioner, mixed_switch_tol=mixed_switch_tol)
+            opts = solve_opts(it, drho_prev)
             local_r = self.v_ion + v_in
             te = time.perf_counter()
             ks = range(len(self.hams))
@@ -408,16 +473,23 @@ class ScfDriver:
                 break
         run.rho = rho
         run.v_in = v_in
-        if final_tol is not None:
-            # Tight FP64 eigenvalues on the last input potential.
-            run.eigenvalues = []
-            for ham in self.hams:
-                vals, _, _ = ham.eig_solve(s.nbands, tol=final_tol, max_iter=400, precision=FP64,
-                                           preconditioner=preconditioner, want_vectors=False, reuse=True)
-                run.eigenvalues.append(vals)
+        self._final(run, final_tol, preconditioner)
         return run
 
+    def _final(self, run, final_tol, preconditioner):
+        """Tight FP64 eigenvalues on the last input potential (already set on every context)."""
+        if final_tol is None:
+            return
+        run.eigenvalues = []
+        for ham in self.hams:
+            vals, _, _ = ham.eig_solve(self.sys.nbands, tol=final_tol, max_iter=400, precision=FP64,
+                                       preconditioner=preconditioner, want_vectors=False, reuse=True)
+            run.eigenvalues.append(vals)
+
     def close(self):
+        if self.grid is not None:
+            self.grid.close()
+            self.grid = None
         if self.pool is not None:
             self.pool.shutdown()
             self.pool = None

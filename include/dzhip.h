@@ -268,6 +268,43 @@ int dzhip_mlp_loss_grad(dzhip_ctx* ctx, const float* samples, uint64_t n, const 
 int dzhip_mlp_train(dzhip_ctx* ctx, const float* samples, uint64_t n, float* params, uint64_t epochs,
                     uint64_t batch, double lr, uint64_t seed, double* epoch_loss);
 
+/* ------------------------------------------------------------------ SCF grid
+ * Device-resident density, potential and mixing state of one SCF run, so an iteration moves
+ * no grid or wave-function data over PCIe.  Replaces, on the device: build_potential_grid /
+ * build_local_potential_real (src/features/scf/potential.zig:144-203, 320-334) with Hartree
+ * (potential.zig:34-44) and LDA-PZ (src/features/xc/xc.zig:195-210, 344-384); the potential-mixing
+ * branch (src/features/scf/scf.zig:147-188) with linear (mixing.zig:10-18) and Pulay
+ * (mixing.zig:183-234) mixing; the density sum over k and its RMS change (util.zig:17-24).
+ * recip: reciprocal lattice rows b1, b2, b3 (bohr^-1, 9 doubles); volume in bohr^3.
+ * Pulay overlaps are real-space sums / N (the reference's G-space norm by Parseval), scaled by
+ * the largest diagonal; a singular system restarts the history with a linear step. */
+typedef struct dzhip_scf_grid dzhip_scf_grid;
+
+#define DZHIP_GRID_RHO 0   /* density of the last iteration (rho_in of the next) */
+#define DZHIP_GRID_VIN 1   /* V_Hxc input of the next iteration */
+#define DZHIP_GRID_VOUT 2  /* V_Hxc[rho] of the last iteration */
+#define DZHIP_GRID_LOCAL 3 /* V_ion + V_in */
+#define DZHIP_GRID_VION 4
+
+dzhip_scf_grid* dzhip_scf_grid_create(dzhip_ctx* ctx, uint64_t nx, uint64_t ny, uint64_t nz,
+                                      const double* recip, double volume, uint64_t pulay_history);
+void dzhip_scf_grid_destroy(dzhip_scf_grid* grid);
+/* V_ion(r) (Ry, FFT order, host). */
+int dzhip_scf_grid_set_ion(dzhip_scf_grid* grid, const double* v_ion, uint64_t count);
+/* rho = nelec / volume, V_in = V_Hxc[rho], Pulay history cleared, density sum cleared. */
+int dzhip_scf_grid_init(dzhip_scf_grid* grid, double nelec);
+/* local_r of ham = V_ion + V_in (dzhip_ham_set_local without the host copy). */
+int dzhip_ham_set_local_grid(dzhip_ham* ham, dzhip_scf_grid* grid);
+/* Add (1/Omega) sum_b w_b |psi_b(r)|^2 of ham's last eigensolver solution to the density sum. */
+int dzhip_scf_grid_add_solution_density(dzhip_scf_grid* grid, dzhip_ham* ham, const double* weights,
+                                        uint64_t nbands);
+/* From the density sum: V_out = V_H + V_xc; out[0] = E_H, out[1] = E_xc, out[2] = int V_in rho,
+ * out[3] = RMS(rho - rho_previous) (energies in Ry); then the sum becomes rho and is cleared. */
+int dzhip_scf_grid_hxc(dzhip_scf_grid* grid, double* out, uint64_t n_out);
+/* V_in <- linear (pulay = 0) or Pulay (pulay != 0) mix of V_in and V_out. */
+int dzhip_scf_grid_mix(dzhip_scf_grid* grid, double beta, int pulay);
+int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t count);
+
 /* Sizes the library uses for this context (for roofline accounting):
  * out[0] = n_pw, [1] = grid points, [2] = x-planes holding the G-sphere,
  * [3] = projectors P, [4] = device bytes held. */
