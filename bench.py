@@ -191,6 +191,7 @@ def main():
     ms32, ph32 = time_gpu(ham, x32, y32, nb, dzhip.FP32, args.steps, args.warmup, torch)
     ms64, ph64 = time_gpu(ham, x64, y64, nb, dzhip.FP64, args.steps, args.warmup, torch)
 
+    tuned = ham.info()
     h32 = y32.cpu().numpy().view(np.complex64).reshape(nb, npw).astype(np.complex128)
     h64 = y64.cpu().numpy().view(np.complex128).reshape(nb, npw)
     err = float(np.max(np.linalg.norm(h32 - h64, axis=1) / np.linalg.norm(h64, axis=1)))
@@ -242,6 +243,7 @@ def main():
                  "frac_hbm": round(gbs64 / HBM_PEAK_GBS, 4)},
         "fp32_frac_hbm": round(gbs32 / HBM_PEAK_GBS, 4),
         "fp32_vs_fp64_err": {"max_band_rel_l2": err, "max_abs": abserr},
+        "tuning": {"xline_split_fp32": tuned["xline_split_fp32"], "xline_split_fp64": tuned["xline_split_fp64"]},
         "phases_ms_fp32": {k: round(v, 5) for k, v in ph32.items()},
         "phases_ms_fp64": {k: round(v, 5) for k, v in ph64.items()},
         "scf_time_to_1e-6_Ha": None if args.scf == "none" else scf_leg(args.scf, args.scf_max_iter),

@@ -89,6 +89,117 @@ __global__ void k_precondition(const double2* __restrict__ r, const int* __restr
     q[(size_t)j * ld + i] = make_double2(v.x * f, v.y * f);
 }
 
+// ----------------------------------------------------------------- Gram products C = A^H B
+// For the tall-skinny products of the solver (K = n_pw ~ 24k, M, N <= a few hundred), where the
+// library GEMM runs at a few TF/s: split-K over workgroups of a 64 x 64 output tile, FP64 MFMA
+// 16x16x4 with the complex product as four real MFMAs, operands staged through LDS with a
+// register prefetch of the next K-stage, partials reduced in a fixed order by a second kernel.
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+constexpr int kGramTile = 64, kGramK = 16, kGramPitch = kGramK + 1, kGramThreads = 256;
+constexpr int kGramPer = kGramTile * kGramK / kGramThreads;  // complex elements per thread per operand
+
+__global__ __launch_bounds__(kGramThreads) void k_gram_cn(const double2* __restrict__ A, int lda, int M,
+                                                          const double2* __restrict__ B, int ldb, int N, int K,
+                                                          int kchunk, int mtiles, double2* __restrict__ partial) {
+    __shared__ double2 As[kGramTile * kGramPitch];
+    __shared__ double2 Bs[kGramTile * kGramPitch];
+    const int mt = blockIdx.x % mtiles, ks = blockIdx.x / mtiles;
+    const int i_base = mt * kGramTile, j_base = blockIdx.y * kGramTile;
+    const int k0 = ks * kchunk, k1 = min(K, k0 + kchunk);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wi = wave & 1, wj = wave >> 1;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    double ar[kGramPer], ai[kGramPer], br[kGramPer], bi[kGramPer];
+    auto fetch = [&](int kt) {
+#pragma unroll
+        for (int q = 0; q < kGramPer; ++q) {
+            const int e = tid + q * kGramThreads;
+            const int c = e / kGramK, kk = e % kGramK, k = kt + kk;
+            const int i = i_base + c, j = j_base + c;
+            const bool oka = i < M && k < k1, okb = j < N && k < k1;
+            const double2 va = A[oka ? (size_t)i * lda + k : 0];
+            const double2 vb = B[okb ? (size_t)j * ldb + k : 0];
+            ar[q] = oka ? va.x : 0.0;
+            ai[q] = oka ? va.y : 0.0;
+            br[q] = okb ? vb.x : 0.0;
+            bi[q] = okb ? vb.y : 0.0;
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < kGramPer; ++q) {
+            const int e = tid + q * kGramThreads;
+            const int c = e / kGramK, kk = e % kGramK;
+            As[c * kGramPitch + kk] = make_double2(ar[q], ai[q]);
+            Bs[c * kGramPitch + kk] = make_double2(br[q], bi[q]);
+        }
+    };
+    v4d_t cr[2][2], ci[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            cr[x][y] = v4d_t{0, 0, 0, 0};
+            ci[x][y] = v4d_t{0, 0, 0, 0};
+        }
+    if (k0 < k1) {
+        fetch(k0);
+        stash();
+    }
+    __syncthreads();
+    for (int kt = k0; kt < k1; kt += kGramK) {
+        const bool more = kt + kGramK < k1;
+        if (more) fetch(kt + kGramK);
+#pragma unroll
+        for (int kk = 0; kk < kGramK; kk += 4) {
+            double2 av[2], bv[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) av[x] = As[(wi * 32 + x * 16 + l16) * kGramPitch + kk + l4];
+#pragma unroll
+            for (int y = 0; y < 2; ++y) bv[y] = Bs[(wj * 32 + y * 16 + l16) * kGramPitch + kk + l4];
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    // conj(a) b = (ar br + ai bi) + i (ar bi - ai br)
+                    cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x].x, bv[y].x, cr[x][y], 0, 0, 0);
+                    cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x].y, bv[y].y, cr[x][y], 0, 0, 0);
+                    ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x].x, bv[y].y, ci[x][y], 0, 0, 0);
+                    ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[x].y, bv[y].x, ci[x][y], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+        if (more) stash();
+        __syncthreads();
+    }
+    // D slot r of a 16x16 block: row (l/16) + 4 r, column l % 16.
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i_base + wi * 32 + x * 16 + l4 + 4 * r;
+                const int j = j_base + wj * 32 + y * 16 + l16;
+                if (i < M && j < N) partial[((size_t)ks * N + j) * M + i] = make_double2(cr[x][y][r], ci[x][y][r]);
+            }
+}
+
+// C[j * ldc + i] = sum_ks partial[ks][j][i]
+__global__ void k_gram_reduce(const double2* __restrict__ partial, int nks, int M, int N, double2* __restrict__ C,
+                              int ldc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= M) return;
+    double re = 0.0, im = 0.0;
+    for (int ks = 0; ks < nks; ++ks) {
+        const double2 v = partial[((size_t)ks * N + j) * M + i];
+        re += v.x;
+        im += v.y;
+    }
+    C[(size_t)j * ldc + i] = make_double2(re, im);
+}
+
 // p = r / (diag - lambda) with the 1e-8 clamp (single vector, CG).
 __global__ void k_precondition_one(const double2* __restrict__ r, const double* __restrict__ diag, double lam,
                                    double2* __restrict__ p, int n) {
@@ -188,7 +299,7 @@ struct EigWork {
     bool x_converged = false;
     // Learned preconditioner: residual norms of the current iteration; training snapshots
     // (Ritz vectors, residuals, Ritz values, residual norms, <T> per band) of the last solve.
-    DevBuf rnorm, snapX, snapR, snapLam, snapRn, snapEk, bandsum, samples;
+    DevBuf rnorm, snapX, snapR, snapLam, snapRn, snapEk, bandsum, samples, gram_part;
     int snaps = 0, snap_nb = 0;
     // Section timing: events recorded on the stream at section starts (pooled).
     std::vector<hipEvent_t> ev_pool;
@@ -199,7 +310,7 @@ struct EigWork {
         if (blas) rocblas_destroy_handle(blas);
         DevBuf* bufs[] = {&V, &W, &X, &HX, &R, &Q, &HQ, &T, &Tw, &S, &G, &evals, &ework, &info,
                           &norms, &lam, &ekin, &diag, &sel, &scl, &idx, &x32, &y32, &rnorm,
-                          &snapX, &snapR, &snapLam, &snapRn, &snapEk, &bandsum, &samples};
+                          &snapX, &snapR, &snapLam, &snapRn, &snapEk, &bandsum, &samples, &gram_part};
         for (DevBuf* b : bufs) b->release();
     }
 };
@@ -273,6 +384,25 @@ struct Solver {
         st.apply_calls += cols;
     }
 
+    // C[0:M, 0:N] (ld ldc) = A[:, 0:M]^H B[:, 0:N], columns of length n with leading dimension n.
+    void gram(const double2* A, int M, const double2* B, int N, double2* C, int ldc) {
+        if (M <= 0 || N <= 0) return;
+        const int K = (int)n;
+        const int mtiles = (M + kGramTile - 1) / kGramTile, ntiles = (N + kGramTile - 1) / kGramTile;
+        // K-chunks (multiples of the stage depth) so that about three workgroups per CU run.
+        const int want = std::max(1, 768 / (mtiles * ntiles));
+        int kchunk = (K + want - 1) / want;
+        kchunk = std::max(kGramK * 8, (kchunk + kGramK - 1) / kGramK * kGramK);
+        const int nks = (K + kchunk - 1) / kchunk;
+        w.gram_part.ensure((size_t)nks * M * N * sizeof(double2));
+        hipLaunchKernelGGL(k_gram_cn, dim3((unsigned)(mtiles * nks), (unsigned)ntiles), dim3(kGramThreads), 0, s, A,
+                           K, M, B, K, N, K, kchunk, mtiles, w.gram_part.as<double2>());
+        DZ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((M + 255) / 256), (unsigned)N), dim3(256), 0, s,
+                           w.gram_part.as<double2>(), nks, M, N, C, ldc);
+        DZ_HIP(hipGetLastError());
+    }
+
     void gemm(rocblas_operation ta, rocblas_operation tb, int m, int nn, int k, zc alpha, const double2* A,
               int lda, const double2* B, int ldb, zc beta, double2* C, int ldc) {
         DZ_BLAS(rocblas_zgemm(w.blas, ta, tb, m, nn, k, &alpha, reinterpret_cast<const zc*>(A), lda,
@@ -300,8 +430,7 @@ struct Solver {
         const std::vector<double> nrm0 = col_norms(Q, a);
         for (int pass = 0; pass < 2 && m > 0; ++pass) {
             // S = V^H Q ; Q -= V S
-            gemm(rocblas_operation_conjugate_transpose, rocblas_operation_none, m, a, ldn, zc(1.0, 0.0), V(), ldn,
-                 Q, ldn, zc(0.0, 0.0), w.S.as<double2>(), max_sub);
+            gram(V(), m, Q, a, w.S.as<double2>(), max_sub);
             gemm(rocblas_operation_none, rocblas_operation_none, ldn, a, m, zc(-1.0, 0.0), V(), ldn,
                  w.S.as<double2>(), max_sub, zc(1.0, 0.0), Q, ldn);
         }
@@ -322,15 +451,63 @@ struct Solver {
                            w.HQ.as<double2>(), n, (int)n);
         DZ_HIP(hipGetLastError());
         DZ_HIP(hipMemcpyAsync(Q, w.HQ.ptr, n * k * sizeof(double2), hipMemcpyDeviceToDevice, s));
-        for (int pass = 0; pass < 2; ++pass) cholqr(Q, k);
-        return k;
+        int kk = k;
+        for (int pass = 0; pass < 2 && kk > 0; ++pass) kk = ortho_block(Q, kk);
+        return kk;
+    }
+
+    // Orthonormalise Q[:, 0:k] in place: Cholesky QR, or, when the Gram matrix is not
+    // numerically positive definite, its eigen-decomposition G = U L U^H with the directions of
+    // L < 1e-12 max(L) dropped and Q <- Q U L^{-1/2} (compacted to the front).  Returns the
+    // number of columns kept.
+    int ortho_block(double2* Q, int k) {
+        const int ldn = (int)n;
+        gram(Q, k, Q, k, w.G.as<double2>(), k);
+        DZ_HIP(hipMemcpyAsync(w.S.ptr, w.G.ptr, (size_t)k * k * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        DZ_BLAS(rocsolver_zpotrf(w.blas, rocblas_fill_upper, k, w.G.as<zc>(), k, w.info.as<rocblas_int>()));
+        rocblas_int info = 0;
+        DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipStreamSynchronize(s));
+        const zc one(1.0, 0.0), zero(0.0, 0.0);
+        if (info == 0) {
+            DZ_BLAS(rocblas_ztrsm(w.blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                  rocblas_diagonal_non_unit, ldn, k, &one, w.G.as<zc>(), k, reinterpret_cast<zc*>(Q), ldn));
+            return k;
+        }
+        st.ortho_fallbacks++;
+        DZ_BLAS(rocsolver_zheevd(w.blas, rocblas_evect_original, rocblas_fill_upper, k, w.S.as<zc>(), k,
+                                 w.evals.as<double>(), w.ework.as<double>(), w.info.as<rocblas_int>()));
+        std::vector<double> lam(k);
+        DZ_HIP(hipMemcpyAsync(lam.data(), w.evals.ptr, k * sizeof(double), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipStreamSynchronize(s));
+        DZ_REQUIRE(info == 0, DZHIP_ERR_HIP, "zheevd of the Gram matrix did not converge");
+        const double thr = 1e-12 * std::max(lam[k - 1], 0.0);
+        int first = 0;
+        while (first < k && !(lam[first] > thr)) ++first;
+        const int kk = k - first;
+        if (kk == 0) return 0;
+        gemm(rocblas_operation_none, rocblas_operation_none, ldn, kk, k, one, Q, ldn,
+             w.S.as<double2>() + (size_t)first * k, k, zero, w.HQ.as<double2>(), ldn);
+        std::vector<int> idx(kk);
+        std::vector<double> scl(kk);
+        for (int j = 0; j < kk; ++j) {
+            idx[j] = j;
+            scl[j] = 1.0 / std::sqrt(lam[first + j]);
+        }
+        DZ_HIP(hipMemcpyAsync(w.idx.ptr, idx.data(), kk * sizeof(int), hipMemcpyHostToDevice, s));
+        DZ_HIP(hipMemcpyAsync(w.scl.ptr, scl.data(), kk * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_gather_cols, cols_grid(kk), dim3(256), 0, s, w.HQ.as<double2>(), w.idx.as<int>(),
+                           w.scl.as<double>(), Q, n, (int)n);
+        DZ_HIP(hipGetLastError());
+        DZ_HIP(hipStreamSynchronize(s));  // idx / scl are host vectors
+        return kk;
     }
 
     // Q <- Q R^{-1} with G = Q^H Q = R^H R.  Throws if G is not positive definite.
     void cholqr(double2* Q, int k) {
         const int ldn = (int)n;
-        gemm(rocblas_operation_conjugate_transpose, rocblas_operation_none, k, k, ldn, zc(1.0, 0.0), Q, ldn, Q, ldn,
-             zc(0.0, 0.0), w.G.as<double2>(), k);
+        gram(Q, k, Q, k, w.G.as<double2>(), k);
         DZ_BLAS(rocsolver_zpotrf(w.blas, rocblas_fill_upper, k, w.G.as<zc>(), k, w.info.as<rocblas_int>()));
         rocblas_int info = 0;
         DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
@@ -343,9 +520,7 @@ struct Solver {
 
     // T[0:m+a, m:m+a] = V[:, 0:m+a]^H W[:, m:m+a], mirrored into the rows.
     void extend_T(int m, int a) {
-        const int ldn = (int)n;
-        gemm(rocblas_operation_conjugate_transpose, rocblas_operation_none, m + a, a, ldn, zc(1.0, 0.0), V(), ldn,
-             W() + (size_t)m * n, ldn, zc(0.0, 0.0), T() + (size_t)m * max_sub, max_sub);
+        gram(V(), m + a, W() + (size_t)m * n, a, T() + (size_t)m * max_sub, max_sub);
         hipLaunchKernelGGL(k_mirror_new, dim3((unsigned)((m + a + 255) / 256), (unsigned)a), dim3(256), 0, s, T(),
                            max_sub, m, a);
         DZ_HIP(hipGetLastError());

@@ -58,6 +58,10 @@ struct Ham {
 
     // Work buffers.
     DevBuf xstage, ystage, x32, y32, tbuf, partial, qbuf;
+    // Tuned launch shapes: y-slab count of the x-line kernel per precision (0: not yet tuned;
+    // [0] FP64, [1] FP32); tune_fixed != 0 forces a slab count (sweeps).
+    int xline_split[2] = {0, 0};
+    int tune_fixed = 0;
 
     // Profiling: one set of phase-boundary events per column pass, recorded on the stream
     // without waiting; read (and recycled) by collect_profile().

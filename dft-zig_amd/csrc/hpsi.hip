@@ -23,6 +23,9 @@
 #include <cmath>
 #include <cstring>
 
+#include <type_traits>
+#include <vector>
+
 #include "fft_lds.h"
 #include "ham.h"
 
@@ -84,33 +87,37 @@ template <typename T>
 __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
     typename C2T<T>::type* __restrict__ tbuf, const int* __restrict__ pxpos, int nplanes,
     const int* __restrict__ xabsent, int nabsent, const T* __restrict__ vloc, int nx, int ny, int nz,
-    FftAxis ax, const typename C2T<T>::type* __restrict__ tw) {
+    int ylen, FftAxis ax, const typename C2T<T>::type* __restrict__ tw) {
+    // One workgroup: the x-lines of ylen consecutive y values (y0 = blockIdx.z * ylen) at one z of
+    // one band; the x-transforms of different y are independent, so the slab is the tuning knob
+    // between LDS per workgroup (occupancy) and the length of the contiguous row segments.
     using C = typename C2T<T>::type;
     extern __shared__ unsigned char smem_raw[];
-    const int pitch = ny + 1;
+    const int pitch = ylen + 1;
     C* buf = reinterpret_cast<C*>(smem_raw);  // [x][y]: element (x, y) at y + pitch*x
     C* twx = buf + nx * pitch;
     const int z = blockIdx.x;
     const size_t col = blockIdx.y;
+    const int y0 = blockIdx.z * ylen;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const unsigned my = div_magic(ny);
+    const unsigned my = div_magic(ylen);
     // Rows of x values that hold no plane wave start (and stay) zero before the inverse.
-    for (int i = tid; i < nabsent * ny; i += nthr) {
-        const int r = fdiv(i, ny, my), y = i - r * ny;
+    for (int i = tid; i < nabsent * ylen; i += nthr) {
+        const int r = fdiv(i, ylen, my), y = i - r * ylen;
         buf[y + pitch * xabsent[r]] = C{T(0), T(0)};
     }
     load_twiddles(twx, tw + ax.tw_off, nx, tid, nthr);
-    C* tcol = tbuf + col * (size_t)nplanes * ny * nz;
-    for (int i = tid; i < nplanes * ny; i += nthr) {
-        const int p = fdiv(i, ny, my), y = i - p * ny;
+    C* tcol = tbuf + col * (size_t)nplanes * ny * nz + y0;
+    for (int i = tid; i < nplanes * ylen; i += nthr) {
+        const int p = fdiv(i, ylen, my), y = i - p * ylen;
         buf[y + pitch * pxpos[p]] = tcol[((size_t)p * nz + z) * ny + y];
     }
     __syncthreads();
-    const LineSet xl{ny, pitch, 1, nullptr};
+    const LineSet xl{ylen, pitch, 1, nullptr};
     lds_fft_lines<T, C, true, 1>(buf, ax, twx, xl, tid, nthr);
-    const T* vz = vloc + (size_t)nx * ny * z;
+    const T* vz = vloc + (size_t)nx * (ny * (size_t)z + y0);
     const unsigned mx = div_magic(nx);
-    for (int i = tid; i < nx * ny; i += nthr) {
+    for (int i = tid; i < nx * ylen; i += nthr) {
         const int y = fdiv(i, nx, mx), xx = i - y * nx;
         C& e = buf[y + pitch * xx];
         const T v = vz[i];
@@ -118,8 +125,8 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
     }
     __syncthreads();
     lds_fft_lines<T, C, false, -1>(buf, ax, twx, xl, tid, nthr);
-    for (int i = tid; i < nplanes * ny; i += nthr) {
-        const int p = fdiv(i, ny, my), y = i - p * ny;
+    for (int i = tid; i < nplanes * ylen; i += nthr) {
+        const int p = fdiv(i, ylen, my), y = i - p * ylen;
         tcol[((size_t)p * nz + z) * ny + y] = buf[y + pitch * pxpos[p]];
     }
 }
@@ -870,6 +877,56 @@ void Ham::collect_profile() {
 
 
 template <typename T>
+void launch_xline(Ham& h, typename C2T<T>::type* tb, size_t nc, int split, hipStream_t s) {
+    using C = typename C2T<T>::type;
+    const int ylen = h.ny / split;
+    const size_t lds = xline_lds_bytes<C>(h.nx, ylen);
+    set_max_dynamic_lds((const void*)k_xline<T>, lds);
+    hipLaunchKernelGGL(k_xline<T>, dim3(h.nz, (unsigned)nc, (unsigned)split), dim3(TileThreads<T>::value), lds, s, tb,
+                       h.plane_xpos.as<int>(), h.nplanes, h.plane_xabsent.as<int>(), h.nxabsent, Sel<T>::vloc(h), h.nx,
+                       h.ny, h.nz, ylen, h.tab.ax[0], Sel<T>::tw(h));
+    DZ_HIP(hipGetLastError());
+}
+
+// y-slab count of the x-line kernel, chosen once per context and precision by timing every
+// divisor d of ny with ny/d >= 8 on the real shapes.  The trial launches transform tbuf in
+// place, so this runs before plane_inv writes the pass's data into it.
+template <typename T>
+int xline_split(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
+    int& sp = h.xline_split[std::is_same<T, float>::value ? 1 : 0];
+    if (sp > 0) return sp;
+    std::vector<int> cand;
+    for (int d = 1; d <= 12; ++d)
+        if (h.ny % d == 0 && h.ny / d >= 8) cand.push_back(d);
+    if (cand.size() <= 1 || h.tune_fixed) {
+        sp = h.tune_fixed ? h.tune_fixed : 1;
+        return sp;
+    }
+    hipEvent_t e0, e1;
+    DZ_HIP(hipEventCreate(&e0));
+    DZ_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_d = 1;
+    for (int d : cand) {
+        launch_xline<T>(h, tb, nc, d, s);
+        DZ_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r) launch_xline<T>(h, tb, nc, d, s);
+        DZ_HIP(hipEventRecord(e1, s));
+        DZ_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        DZ_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_d = d;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    sp = best_d;
+    return sp;
+}
+
+template <typename T>
 void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, bool nonlocal) {
     using C = typename C2T<T>::type;
     const C* x = reinterpret_cast<const C*>(xv);
@@ -928,22 +985,19 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
             const size_t plane = (size_t)h.ny * h.nz;
             h.tbuf.ensure(nc * h.nplanes * plane * sizeof(C));
             C* tb = h.tbuf.as<C>();
+            // Tuned before plane_inv fills tbuf: the trial launches transform tbuf in place.
+            const int split = xline_split<T>(h, tb, nc, s);
             const size_t lds_p = plane_lds_bytes<C>(h.ny, h.nz);
-            const size_t lds_x = xline_lds_bytes<C>(h.nx, h.ny);
             constexpr int nt = TileThreads<T>::value;
             set_max_dynamic_lds((const void*)k_plane_inv<T>, lds_p);
             set_max_dynamic_lds((const void*)k_plane_fwd<T>, lds_p);
-            set_max_dynamic_lds((const void*)k_xline<T>, lds_x);
             hipLaunchKernelGGL(k_plane_inv<T>, dim3(h.nplanes, (unsigned)nc), dim3(nt), lds_p, s,
                                xc, ld, tb, h.plane_start.as<int>(), h.plane_pw.as<int>(),
                                h.plane_pos.as<int>(), h.plane_ystart.as<int>(), h.plane_ycols.as<short>(),
                                h.ny, h.nz, h.nplanes, h.tab.ax[1], h.tab.ax[2], tw);
             DZ_HIP(hipGetLastError());
             mark(h, evs, 3);
-            hipLaunchKernelGGL(k_xline<T>, dim3(h.nz, (unsigned)nc), dim3(nt), lds_x, s, tb,
-                               h.plane_xpos.as<int>(), h.nplanes, h.plane_xabsent.as<int>(), h.nxabsent,
-                               Sel<T>::vloc(h), h.nx, h.ny, h.nz, h.tab.ax[0], tw);
-            DZ_HIP(hipGetLastError());
+            launch_xline<T>(h, tb, nc, split, s);
             mark(h, evs, 4);
             hipLaunchKernelGGL(k_plane_fwd<T>, dim3(h.nplanes, (unsigned)nc), dim3(nt), lds_p, s,
                                tb, xc, ld, yc, ld, Sel<T>::kin(h), h.plane_start.as<int>(),

@@ -37,7 +37,7 @@ EXPORTS = (
     "dzhip_mlp_loss_grad", "dzhip_mlp_train", "dzhip_cg_solve",
     "dzhip_scf_grid_create", "dzhip_scf_grid_destroy", "dzhip_scf_grid_set_ion", "dzhip_scf_grid_init",
     "dzhip_ham_set_local_grid", "dzhip_scf_grid_add_solution_density", "dzhip_scf_grid_hxc",
-    "dzhip_scf_grid_mix", "dzhip_scf_grid_read",
+    "dzhip_scf_grid_mix", "dzhip_scf_grid_read", "dzhip_ham_set_tuning",
 )
 
 
@@ -89,7 +89,7 @@ class EigStats(ct.Structure):
         ("converged", ct.c_uint64), ("max_residual", ct.c_double), ("initial_residual", ct.c_double),
         ("precond_fallback", ct.c_uint64), ("t_rr_ms", ct.c_double), ("t_ritz_ms", ct.c_double),
         ("t_precond_ms", ct.c_double), ("t_ortho_ms", ct.c_double), ("t_apply_ms", ct.c_double),
-        ("t_extend_ms", ct.c_double),
+        ("t_extend_ms", ct.c_double), ("ortho_fallbacks", ct.c_uint64),
     ]
 
     def as_dict(self):
@@ -160,6 +160,7 @@ def _load():
         "dzhip_scf_grid_hxc": ([vp, dp, u64], i32),
         "dzhip_scf_grid_mix": ([vp, ct.c_double, i32], i32),
         "dzhip_scf_grid_read": ([vp, i32, dp, u64], i32),
+        "dzhip_ham_set_tuning": ([vp, i32], i32),
     }
     missing = set(EXPORTS) - set(sig)
     if missing:
@@ -485,10 +486,15 @@ class Hamiltonian:
                                                     1 if acc else 0))
         return out
 
+    def set_tuning(self, xline_split: int = 0):
+        """Force the x-line kernel's y-slab count (0: autotune on the next apply)."""
+        check(L().dzhip_ham_set_tuning(self.handle, int(xline_split)))
+
     def info(self):
-        out = (ct.c_uint64 * 5)()
-        check(L().dzhip_ham_info(self.handle, out, 5))
-        keys = ("n_pw", "grid_points", "planes", "projectors", "device_bytes")
+        out = (ct.c_uint64 * 7)()
+        check(L().dzhip_ham_info(self.handle, out, 7))
+        keys = ("n_pw", "grid_points", "planes", "projectors", "device_bytes", "xline_split_fp32",
+                "xline_split_fp64")
         return dict(zip(keys, [int(v) for v in out]))
 
     def close(self):

@@ -202,6 +202,8 @@ typedef struct dzhip_eig_stats {
      * orthonormalisation (CGS2 + Cholesky QR), H apply of new vectors, T extension/restart.
      * Host waits between sections (small downloads) fall into the section that follows. */
     double t_rr_ms, t_ritz_ms, t_precond_ms, t_ortho_ms, t_apply_ms, t_extend_ms;
+    uint64_t ortho_fallbacks;       /* new blocks whose Cholesky QR failed and that were
+                                       orthonormalised through the Gram eigen-decomposition */
 } dzhip_eig_stats;
 
 /* diag: [n] preconditioner diagonal (|k+G|^2, kpoint/solve.zig:362-367).  init_vecs: NULL or
@@ -305,9 +307,15 @@ int dzhip_scf_grid_hxc(dzhip_scf_grid* grid, double* out, uint64_t n_out);
 int dzhip_scf_grid_mix(dzhip_scf_grid* grid, double beta, int pulay);
 int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t count);
 
+/* Launch-shape tuning (no reference counterpart): the x-line kernel processes the y-lines of
+ * one (z, band) in xline_split slabs; 0 (default) times every divisor of ny with at least 8
+ * lines per slab on the first apply of each precision and keeps the fastest. */
+int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split);
+
 /* Sizes the library uses for this context (for roofline accounting):
  * out[0] = n_pw, [1] = grid points, [2] = x-planes holding the G-sphere,
- * [3] = projectors P, [4] = device bytes held. */
+ * [3] = projectors P, [4] = device bytes held, [5] = x-line slabs chosen for FP32,
+ * [6] = for FP64 (0 = not tuned yet). */
 int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out);
 
 #ifdef __cplusplus

@@ -167,3 +167,21 @@ def test_apply_dev_with_torch_buffers(dz, ctx):
     got32 = y32.cpu().numpy().view(np.complex64).reshape(12, -1)
     assert relerr(got32.astype(np.complex128), ref) < TOL32
     ham.set_stream(None)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hpsi_every_xline_split(dz, ctx, prec):
+    """Every y-slab count the x-line tuner may pick gives the golden H.psi (Si-8, ny = 32)."""
+    w = workloads.si8()
+    g = np.load(GOLDEN / "hpsi_si8.npz")
+    ham = make_ham(dz, ctx, w)
+    tol = TOL64 if prec == 0 else TOL32
+    errs = {}
+    for d in (1, 2, 4):
+        ham.set_tuning(d)
+        errs[d] = relerr(ham.apply(w.psi, precision=prec), g["y"])
+        assert ham.info()["xline_split_fp32" if prec else "xline_split_fp64"] == d
+    ham.set_tuning(0)
+    errs["auto"] = relerr(ham.apply(w.psi, precision=prec), g["y"])
+    print(prec, errs, ham.info())
+    assert all(e < tol for e in errs.values()), errs
