@@ -724,7 +724,17 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
         const double max_res = *std::max_element(rn.begin(), rn.end());
         S.st.max_residual = max_res;
         if (iter == 0) S.st.initial_residual = max_res;
-        res_hist.push_back(max_res);
+        {
+            // Convergence progress: mean log10 residual of the bands still above tol.
+            double sl = 0.0;
+            int cnt = 0;
+            for (double r : rn)
+                if (r >= o.tol && r > 0.0) {
+                    sl += std::log10(r);
+                    ++cnt;
+                }
+            res_hist.push_back(cnt ? sl / cnt : std::log10(std::max(max_res, 1e-300)));
+        }
         if (w.snaps < nsnap && S.fp64_apply) {
             const size_t blk = n * nb;
             double2* sx = w.snapX.as<double2>() + w.snaps * blk;
@@ -741,12 +751,17 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
             DZ_HIP(hipStreamSynchronize(s));  // lam / rn are host vectors rebuilt next iteration
             w.snaps++;
         }
-        // Learned preconditioner: fall back to TPA for the rest of the solve when the largest
-        // residual fell by less than 10% over the last three iterations.
-        if (precond == DZHIP_PRECOND_MLP && res_hist.size() > 3 &&
-            max_res > 0.9 * res_hist[res_hist.size() - 4]) {
-            precond = DZHIP_PRECOND_TPA;
-            S.st.precond_fallback = iter + 1;
+        // Learned preconditioner: fall back to TPA for the rest of the solve when, over the last
+        // three iterations, the unconverged bands' mean log residual fell more slowly than 80% of
+        // the rate TPA solves on this context achieved (or by less than 0.05 decades per
+        // iteration before any TPA rate is known).
+        if (precond == DZHIP_PRECOND_MLP && res_hist.size() > 3) {
+            const double rate = (res_hist[res_hist.size() - 4] - res_hist.back()) / 3.0;
+            const double need = h.tpa_rate > 0.0 ? 0.8 * h.tpa_rate : 0.05;
+            if (rate < need) {
+                precond = DZHIP_PRECOND_TPA;
+                S.st.precond_fallback = iter + 1;
+            }
         }
         // FP32 apply: a tolerance looser than the switch threshold is met in FP32; tighter
         // tolerances switch the apply to FP64 once the residual is below the threshold.
@@ -813,6 +828,12 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
     }
     mark(w, s, SEC_END);
     S.st.iterations = iter;
+    // Decades of residual per iteration achieved by TPA solves on this context (running mean),
+    // the yardstick of the learned preconditioner's fallback.
+    if (o.preconditioner == DZHIP_PRECOND_TPA && res_hist.size() >= 3) {
+        const double r = (res_hist.front() - res_hist.back()) / (double)(res_hist.size() - 1);
+        if (r > 0.0) h.tpa_rate = h.tpa_rate > 0.0 ? 0.5 * (h.tpa_rate + r) : r;
+    }
     if (have_best) {
         std::memcpy(values, best.data(), nb * sizeof(double));
         w.x_cols = nb;

@@ -78,6 +78,7 @@ struct Ham {
     DevBuf mlp_params;
     double mlp_energy_scale = 0.0;
     bool has_mlp = false;
+    double tpa_rate = 0.0;  // decades of residual per iteration of TPA solves (0: none yet)
     // Density staging (grow-only).
     DevBuf dens_rho, dens_w;
 

@@ -55,6 +55,7 @@ class ScfSystem:
     nbands: int
     kpoints: list  # Cartesian, bohr^-1
     kweights: list
+    smearing: float = 0.0  # Fermi-Dirac width (Ry); 0 = fixed occupations
 
 
 # GTH-form parameters of a four-electron Si-like pseudo-atom (Hartree units for C1 and h): the
@@ -101,6 +102,46 @@ def silicon(cells=(1, 1, 1), ecut_ry=20.0, kmesh=(1, 1, 1), nbands=None, displac
     return ScfSystem(f"si{nat}", cell, recip, abs(float(np.linalg.det(cell))), grid, ecut_ry, pos,
                      np.zeros(nat, dtype=np.int32), [sp], g["z"], g["rloc"], g["c1"], nelec, nb,
                      [np.asarray(k, dtype=np.float64) for k in kpts], list(kw))
+
+
+def mp_mesh_time_reversal(recip, mesh):
+    """Unshifted Monkhorst-Pack mesh with k and -k merged (time reversal, the only symmetry used
+    here; kpoints/kpoints.zig:30-93 also applies the crystal point group)."""
+    m = np.array(mesh)
+    seen, pts, wts = {}, [], []
+    for i in range(m[0]):
+        for j in range(m[1]):
+            for k in range(m[2]):
+                key = (i, j, k)
+                neg = tuple(int(v) for v in (-np.array(key)) % m)
+                if neg in seen:
+                    wts[seen[neg]] += 1.0
+                    continue
+                seen[key] = len(pts)
+                pts.append(np.array(key) / m @ recip)
+                wts.append(1.0)
+    tot = float(np.prod(m))
+    return pts, [w / tot for w in wts]
+
+
+AL_GTH = {"z": 3.0, "rloc": 0.45, "c1": -8.491351, "r_s": 0.460104, "h_s": 5.088340, "r_p": 0.536744,
+          "h_p": 2.193438}
+
+
+def aluminium(ecut_ry=30.0, kmesh=(4, 4, 4), nbands=12, smearing_ry=0.01, grid=None) -> ScfSystem:
+    """fcc Al in the 4-atom cubic cell (a = 4.05 A), GTH-form local part and s/p Gaussian
+    projectors (Goedecker-Teter-Hutter Al LDA parameters), Fermi-Dirac smearing; k-points from an
+    unshifted mesh reduced by time reversal (4x4x4 -> 36 points with uneven plane-wave counts)."""
+    a = 4.05 * tables.ANGSTROM_TO_BOHR
+    cell = np.diag([a, a, a])
+    recip = tables.reciprocal(cell)
+    pos = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]]) * a
+    g = AL_GTH
+    sp = SpeciesDesc([0, 1], [g["r_s"], g["r_p"]], np.diag([2.0 * g["h_s"], 2.0 * g["h_p"]]))
+    kpts, kw = mp_mesh_time_reversal(recip, kmesh)
+    grid = tuple(grid) if grid is not None else tuple(tables.auto_grid(ecut_ry, recip))
+    return ScfSystem("al4", cell, recip, abs(float(np.linalg.det(cell))), grid, ecut_ry, pos, np.zeros(4, np.int32),
+                     [sp], g["z"], g["rloc"], g["c1"], 12.0, nbands, kpts, kw, smearing_ry)
 
 
 def si8_small(ecut_ry=8.0, nbands=20):
@@ -200,6 +241,63 @@ def band_weights(nelec, nbands, kweight):
     for b in range(min(nocc, nbands)):
         w[b] = 2.0 * kweight * ((nelec / 2.0 - (nocc - 1)) if b + 1 == nocc else 1.0)
     return w
+
+
+def fermi_dirac(e, mu, sigma):
+    """Occupation 1/(1 + e^{(e - mu)/sigma}) (fermi_level.zig:7-11)."""
+    x = np.clip((np.asarray(e, dtype=np.float64) - mu) / sigma, -700.0, 700.0)
+    return 1.0 / (1.0 + np.exp(x))
+
+
+def fd_entropy(f):
+    """-(f ln f + (1 - f) ln(1 - f)), zero within 1e-12 of 0 and 1 (fermi_level.zig:15-19)."""
+    f = np.asarray(f, dtype=np.float64)
+    inner = (f > 1e-12) & (f < 1.0 - 1e-12)
+    g = np.where(inner, f, 0.5)
+    return np.where(inner, -(g * np.log(g) + (1.0 - g) * np.log1p(-g)), 0.0)
+
+
+def fermi_level(nelec, sigma, eigenvalues, kweights):
+    """mu with sum_k 2 w_k sum_b f(e_kb) = nelec by bisection on a padded bracket (pad
+    max(10 sigma, 1e-3), doubled up to 12 times), 80 halvings, |dN| < 1e-8 stops
+    (fermi_level.zig:54-124)."""
+    ev = np.concatenate([np.asarray(v, dtype=np.float64) for v in eigenvalues])
+    wt = np.concatenate([np.full(len(v), 2.0 * w) for v, w in zip(eigenvalues, kweights)])
+
+    def count(mu):
+        return float(np.dot(wt, fermi_dirac(ev, mu, sigma)))
+
+    e0, e1 = float(ev.min()), float(ev.max())
+    pad = max(10.0 * sigma, 1e-3)
+    for _ in range(13):
+        lo, hi = e0 - pad, e1 + pad
+        if count(lo) <= nelec <= count(hi):
+            break
+        pad *= 2.0
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        c = count(mid)
+        if abs(c - nelec) < 1e-8:
+            return mid
+        if c > nelec:
+            hi = mid
+        else:
+            lo = mid
+    return 0.5 * (lo + hi)
+
+
+def occupations(system, eigenvalues):
+    """Per-k band weights w_k occ 2 and the -TS energy (Ry) for fixed or Fermi-Dirac occupations."""
+    s = system
+    if s.smearing <= 0.0:
+        return [band_weights(s.nelec, len(v), wk) for v, wk in zip(eigenvalues, s.kweights)], 0.0, None
+    mu = fermi_level(s.nelec, s.smearing, eigenvalues, s.kweights)
+    ws, ts = [], 0.0
+    for v, wk in zip(eigenvalues, s.kweights):
+        f = fermi_dirac(v, mu, s.smearing)
+        ws.append(2.0 * wk * f)
+        ts += float(np.sum(2.0 * wk * s.smearing * fd_entropy(f)))
+    return ws, ts, mu
 
 
 class Pulay:
@@ -337,17 +435,34 @@ class ScfDriver:
         self.pool = ThreadPoolExecutor(len(self.hams)) if self.concurrent else None
         self.setup_s = time.perf_counter() - t0
 
-    def _solve_k(self, k, local_r, nb, opts, weights):
+    def _solve_k(self, k, local_r, nb, opts):
         ham = self.hams[k]
         if self.grid is not None:
             self.grid.set_local(ham)
         else:
             ham.set_local(local_r)
         vals, _, st = ham.eig_solve(nb, want_vectors=False, reuse=True, **opts)
-        if self.grid is not None:
-            self.grid.add_density(ham, weights)
-            return vals, None, st
-        return vals, ham.density_solution(weights), st
+        return vals, st
+
+    def _solve_all(self, local_r, opts):
+        """Every k-point's eigenproblem (concurrently when each has its own context), then the
+        occupations (Fermi level when smearing) and the densities of the device solutions."""
+        s = self.sys
+        ks = range(len(self.hams))
+        if self.pool is not None:
+            res = list(self.pool.map(lambda k: self._solve_k(k, local_r, s.nbands, opts), ks))
+        else:
+            res = [self._solve_k(k, local_r, s.nbands, opts) for k in ks]
+        vals = [r[0] for r in res]
+        weights, ts, mu = occupations(s, vals)
+        rho = None
+        for ham, w in zip(self.hams, weights):
+            if self.grid is not None:
+                self.grid.add_density(ham, w)
+            else:
+                rho = ham.density_solution(w, rho=rho)
+        e_band = float(sum(np.dot(w, v) for w, v in zip(weights, vals)))
+        return vals, [r[1] for r in res], rho, e_band, ts
 
     def _run_device(self, max_iter, beta, conv, pulay_history, pulay_start, warmup_steps, solve_opts, progress,
                     energy_tol_ry):
@@ -355,27 +470,18 @@ class ScfDriver:
         s = self.sys
         g = self.grid
         g.init(s.nelec)
-        weights = [band_weights(s.nelec, s.nbands, wk) for wk in s.kweights]
         run = ScfRun(setup_s=self.setup_s)
         t0 = time.perf_counter()
         drho_prev = None
         for it in range(max_iter):
             opts = solve_opts(it, drho_prev)
             te = time.perf_counter()
-            ks = range(len(self.hams))
-            if self.pool is not None:
-                res = list(self.pool.map(lambda k: self._solve_k(k, None, s.nbands, opts, weights[k]), ks))
-            else:
-                res = [self._solve_k(k, None, s.nbands, opts, weights[k]) for k in ks]
+            vals, stats, _, e_band, ts = self._solve_all(None, opts)
             eig_s = time.perf_counter() - te
-            e_band = float(sum(np.dot(w, r[0]) for w, r in zip(weights, res)))
             e_h, e_xc, vin_rho, drho = g.hxc()
-            e_tot = e_band - vin_rho + e_h + e_xc
-            run.records.append(ScfRecord(it, 0.0, e_tot, e_band, drho,
-                                         sum(r[2]["iterations"] for r in res), sum(r[2]["apply_calls"] for r in res),
-                                         sum(r[2]["fp32_applies"] for r in res), eig_s,
-                                         {k: sum(r[2][k] for r in res) for k in SECTIONS}))
-            run.eigenvalues = [r[0] for r in res]
+            e_tot = e_band - vin_rho + e_h + e_xc - ts
+            run.records.append(self._record(it, e_tot, e_band, drho, stats, eig_s))
+            run.eigenvalues = vals
             drho_prev = drho
             done = drho < conv
             if energy_tol_ry is not None and it >= warmup_steps and len(run.records) > 1:
@@ -425,7 +531,6 @@ class ScfDriver:
         rho = np.full(n, s.nelec / vol)
         v_in = hartree(rho, s.grid, self.g2, vol)[0] + lda_pz(rho)[1]
         mixer = Pulay(pulay_history) if pulay_history > 0 else None
-        weights = [band_weights(s.nelec, s.nbands, wk) for wk in s.kweights]
         run = ScfRun(setup_s=self.setup_s)
         t0 = time.perf_counter()
         drho_prev = None
@@ -434,26 +539,17 @@ class ScfDriver:
             opts = solve_opts(it, drho_prev)
             local_r = self.v_ion + v_in
             te = time.perf_counter()
-            ks = range(len(self.hams))
-            if self.pool is not None:
-                res = list(self.pool.map(lambda k: self._solve_k(k, local_r, s.nbands, opts, weights[k]), ks))
-            else:
-                res = [self._solve_k(k, local_r, s.nbands, opts, weights[k]) for k in ks]
+            vals, stats, rho_out, e_band, ts = self._solve_all(local_r, opts)
             eig_s = time.perf_counter() - te
-            rho_out = np.sum([r[1] for r in res], axis=0)
-            e_band = float(sum(np.dot(w, r[0]) for w, r in zip(weights, res)))
             vh_out, e_h = hartree(rho_out, s.grid, self.g2, vol)
             f_xc, vxc_out = lda_pz(rho_out)
             e_xc = float(np.sum(f_xc)) * vol / n
             # Kohn-Sham energy of rho_out at the input potential (equals the reference's
             # band + double counting at self-consistency).
-            e_tot = e_band - float(np.dot(v_in, rho_out)) * vol / n + e_h + e_xc
+            e_tot = e_band - float(np.dot(v_in, rho_out)) * vol / n + e_h + e_xc - ts
             drho = math.sqrt(float(np.mean((rho_out - rho) ** 2)))
-            run.records.append(ScfRecord(it, 0.0, e_tot, e_band, drho,
-                                         sum(r[2]["iterations"] for r in res), sum(r[2]["apply_calls"] for r in res),
-                                         sum(r[2]["fp32_applies"] for r in res), eig_s,
-                                         {k: sum(r[2][k] for r in res) for k in SECTIONS}))
-            run.eigenvalues = [r[0] for r in res]
+            run.records.append(self._record(it, e_tot, e_band, drho, stats, eig_s))
+            run.eigenvalues = vals
             rho = rho_out
             drho_prev = drho
             done = drho < conv
@@ -475,6 +571,12 @@ class ScfDriver:
         run.v_in = v_in
         self._final(run, final_tol, preconditioner)
         return run
+
+    @staticmethod
+    def _record(it, e_tot, e_band, drho, stats, eig_s):
+        return ScfRecord(it, 0.0, e_tot, e_band, drho, sum(st["iterations"] for st in stats),
+                         sum(st["apply_calls"] for st in stats), sum(st["fp32_applies"] for st in stats), eig_s,
+                         {k: sum(st[k] for st in stats) for k in SECTIONS})
 
     def _final(self, run, final_tol, preconditioner):
         """Tight FP64 eigenvalues on the last input potential (already set on every context)."""

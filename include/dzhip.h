@@ -167,7 +167,8 @@ int dzhip_ham_phase_times(dzhip_ham* ham, double* out_ms, uint64_t n_out, uint64
 #define DZHIP_PRECOND_TPA 1     /* Teter-Payne-Allan K(x) r, x = |k+G|^2 / <x_b|T|x_b> */
 #define DZHIP_PRECOND_MLP 2     /* learned: K_tpa(x) exp(MLP(features)) r, model set with
                                    dzhip_ham_set_precond_model; falls back to TPA within a solve
-                                   when the residual stops decreasing (stats.precond_fallback) */
+                                   when the residuals fall more slowly than TPA's rate on this
+                                   context (stats.precond_fallback) */
 
 typedef struct dzhip_eig_opts {
     uint64_t max_iter;       /* Options.max_iter (reference default 40) */

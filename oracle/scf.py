@@ -9,6 +9,8 @@ Restates (paths relative to the dft-zig source tree):
                                                                      src/features/hamiltonian/hamiltonian.zig:421-474
   local_r = Re IFFT(N (V_ion + V_Hxc)(G)) in FFT order              src/features/scf/potential.zig:320-334
   occupations nocc = ceil(nelec/2), weight w_k occ 2                 src/features/scf/kpoint/density.zig:98-107
+  Fermi-Dirac occupations, entropy -TS, bisection for mu            src/features/scf/fermi_level.zig:7-124,
+                                                                     src/features/scf/kpoint/density.zig:398-404
   linear potential mixing V_in <- V_in + beta (V_out - V_in)         src/features/scf/mixing.zig:10-18
   Pulay/DIIS on (V_in, R = V_out - V_in), history 8, from iteration 4 src/features/scf/mixing.zig:183-234,
                                                                      src/features/scf/scf.zig:147-188,
@@ -181,6 +183,53 @@ class PulayMixer:
         return out
 
 
+def smearing_occ(e, mu, sigma):
+    """Fermi-Dirac occupation 1/(1 + e^{(e - mu)/sigma}); a step for sigma <= 0 (fermi_level.zig:7-11)."""
+    e = np.asarray(e, dtype=np.float64)
+    if sigma <= 0.0:
+        return (e <= mu).astype(np.float64)
+    return 1.0 / (1.0 + np.exp((e - mu) / sigma))
+
+
+def smearing_entropy(f):
+    """-(f ln f + (1 - f) ln(1 - f)), zero within 1e-12 of 0 and 1 (fermi_level.zig:15-19)."""
+    f = np.asarray(f, dtype=np.float64)
+    ok = (f > 1e-12) & (f < 1.0 - 1e-12)
+    fs = np.where(ok, f, 0.5)
+    return np.where(ok, -(fs * np.log(fs) + (1.0 - fs) * np.log(1.0 - fs)), 0.0)
+
+
+def find_fermi_level(nelec, sigma, eigenvalues, kweights):
+    """Bisection for mu with sum_k 2 w_k sum_b f(e_kb) = nelec: bracket [min - pad, max + pad],
+    pad = max(10 sigma, 1e-3) doubled up to 12 times, 80 halvings, early exit at |dN| < 1e-8
+    (fermi_level.zig:54-124)."""
+    vals = [np.asarray(v, dtype=np.float64) for v in eigenvalues]
+    lo_e = min(float(v.min()) for v in vals)
+    hi_e = max(float(v.max()) for v in vals)
+
+    def count(mu):
+        return sum(2.0 * w * float(np.sum(smearing_occ(v, mu, sigma))) for v, w in zip(vals, kweights))
+
+    pad = max(10.0 * sigma, 1e-3)
+    low, high = lo_e - pad, hi_e + pad
+    expand = 0
+    while (count(low) > nelec or count(high) < nelec) and expand < 12:
+        pad *= 2.0
+        low, high = lo_e - pad, hi_e + pad
+        expand += 1
+    left, right = low, high
+    for _ in range(80):
+        mid = 0.5 * (left + right)
+        c = count(mid)
+        if abs(c - nelec) < 1e-8:
+            return mid
+        if c > nelec:
+            right = mid
+        else:
+            left = mid
+    return 0.5 * (left + right)
+
+
 @dataclass
 class ScfResult:
     energies: list = field(default_factory=list)  # total energy (Ry) per iteration
@@ -203,25 +252,36 @@ def run_scf(system, n_iter=40, beta=0.3, conv=1e-8, pulay_history=0, pulay_start
     v_in = vh + vxc
     res = ScfResult()
     mixer = PulayMixer(pulay_history, pulay_relative) if pulay_history > 0 else None
+    sigma = float(getattr(system, "smearing", 0.0))
     for it in range(n_iter):
         local_r = v_ion + v_in
         rho_out = np.zeros(n)
         e_band = 0.0
+        e_ts = 0.0
         eigs = []
-        for kp, wk in zip(system.kpoints, system.kweights):
+        nb = system.nbands
+        # Solve every k first (the smearing path needs all eigenvalues for mu, smearing.zig:107-169).
+        solved = []
+        for kp in system.kpoints:
             ctx = adapters.context_for_k(system, kp, local_r)
-            h = hamiltonian.dense_hamiltonian(ctx)
-            ev, vec = np.linalg.eigh(h)
-            nb = system.nbands
-            wts = odens.band_weights(system.nelec, nb, wk)
-            e_band += float(np.sum(wts * ev[:nb]))
-            rho_out = odens.accumulate_density(ctx.fft_idx, grid, vec[:, :nb].T, wts, 1.0 / vol, rho_out)
-            eigs.append(ev[:nb])
+            ev, vec = np.linalg.eigh(hamiltonian.dense_hamiltonian(ctx))
+            solved.append((ctx, ev[:nb], vec[:, :nb]))
+        mu = find_fermi_level(system.nelec, sigma, [e for _, e, _ in solved], system.kweights) if sigma > 0 else None
+        for (ctx, ev, vec), wk in zip(solved, system.kweights):
+            if sigma > 0:
+                occ = smearing_occ(ev, mu, sigma)
+                wts = 2.0 * wk * occ
+                e_ts += float(np.sum(2.0 * wk * sigma * smearing_entropy(occ)))
+            else:
+                wts = odens.band_weights(system.nelec, nb, wk)
+            e_band += float(np.sum(wts * ev))
+            rho_out = odens.accumulate_density(ctx.fft_idx, grid, vec.T, wts, 1.0 / vol, rho_out)
+            eigs.append(ev)
         vh_out, e_h = hartree(rho_out, grid, recip, vol)
         f_xc, vxc_out = lda_pz(rho_out)
         e_xc = float(np.sum(f_xc)) * vol / n
         # Kohn-Sham energy with the input potential: band - int V_Hxc^in rho_out + E_H + E_xc.
-        e_tot = e_band - float(np.sum(v_in * rho_out)) * vol / n + e_h + e_xc
+        e_tot = e_band - float(np.sum(v_in * rho_out)) * vol / n + e_h + e_xc - e_ts
         drho = math.sqrt(float(np.mean((rho_out - rho) ** 2)))
         res.energies.append(e_tot)
         res.drho.append(drho)

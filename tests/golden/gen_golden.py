@@ -70,8 +70,21 @@ def scf_file():
     print("scf", len(res.energies), res.energies[-1])
 
 
+def scf_al_file():
+    """Oracle SCF of the small Al case with Fermi-Dirac smearing (0.01 Ry)."""
+    from dzhip import scf as dscf
+    from oracle import scf as oscf
+
+    system = dscf.aluminium(ecut_ry=8.0, kmesh=(2, 2, 2), nbands=10)
+    res = oscf.run_scf(system, n_iter=60, beta=0.3, conv=1e-10, pulay_history=8, pulay_start=4)
+    assert res.converged
+    np.savez_compressed(HERE / "scf_al4_small.npz", energies=np.array(res.energies), drho=np.array(res.drho),
+                        eigenvalues=np.array(res.eigenvalues), rho=res.rho)
+    print("scf al", len(res.energies), res.energies[-1])
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["hpsi", "fft", "scf"]
+    which = sys.argv[1:] or ["hpsi", "fft", "scf", "scf_al"]
     if "hpsi" in which:
         hpsi_file("tiny")
         hpsi_file("si8")
@@ -79,3 +92,5 @@ if __name__ == "__main__":
         fft_file()
     if "scf" in which:
         scf_file()
+    if "scf_al" in which:
+        scf_al_file()

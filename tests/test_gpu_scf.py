@@ -186,3 +186,25 @@ def test_scf_grid_potential_matches_host(dz, ctx, system):
         np.testing.assert_allclose(g.read(dz.GRID_RHO), rho_h, rtol=0, atol=1e-14)
     finally:
         drv.close()
+
+
+@pytest.mark.parametrize("concurrent,device_grid", [(False, False), (True, True)])
+def test_scf_smearing_al(dz, ctx, concurrent, device_grid):
+    """Fermi-Dirac smearing (fermi_level.zig, kpoint/density.zig:398-404) on fcc Al, 8 k-points
+    with different plane-wave counts: the oracle's trajectory and fixed point."""
+    gold = np.load(GOLD.parent / "scf_al4_small.npz")
+    s = dscf.aluminium(ecut_ry=8.0, kmesh=(2, 2, 2), nbands=10)
+    drv = dscf.ScfDriver(s, concurrent_k=concurrent, device_grid=device_grid)
+    try:
+        run = drv.run(max_iter=60, conv=1e-10, eig_tol=1e-10, eig_max_iter=400, warmup_steps=0, final_tol=1e-10)
+    finally:
+        drv.close()
+    e = np.array(run.energies)
+    m = min(len(e), len(gold["energies"]))
+    print("iterations", len(e), len(gold["energies"]), "dE", np.max(np.abs(e[:m] - gold["energies"][:m])),
+          "eig", np.max(np.abs(np.array(run.eigenvalues) - gold["eigenvalues"])))
+    assert run.converged
+    assert abs(len(e) - len(gold["energies"])) <= 3
+    assert np.max(np.abs(e[:m] - gold["energies"][:m])) < 1e-8
+    assert np.max(np.abs(np.array(run.eigenvalues) - gold["eigenvalues"])) < EV_TOL
+    assert float(np.sum(np.abs(run.rho - gold["rho"]))) * s.volume / run.rho.size < RHO_L1

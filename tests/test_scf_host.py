@@ -148,3 +148,43 @@ def test_oracle_scf_golden_and_energy_identity():
     f, vxc = oscf.lda_pz(rho)
     e_ref_form = band - e_h - float(np.dot(vxc, rho)) * s.volume / n + float(np.sum(f)) * s.volume / n
     assert abs(e_ref_form - res.energies[-1]) < 1e-8
+
+
+def test_fermi_level_and_entropy():
+    rng = np.random.default_rng(5)
+    ev = [np.sort(rng.normal(0.0, 0.4, 10)) for _ in range(6)]
+    w = list(rng.dirichlet(np.ones(6)))
+    for sigma in (0.005, 0.01, 0.05):
+        mu_o = oscf.find_fermi_level(9.0, sigma, ev, w)
+        mu_p = dscf.fermi_level(9.0, sigma, ev, w)
+        assert abs(mu_o - mu_p) < 1e-12
+        n = sum(2 * wk * float(np.sum(oscf.smearing_occ(v, mu_o, sigma))) for v, wk in zip(ev, w))
+        assert abs(n - 9.0) < 1e-8
+    # Entropy closed forms: S(1/2) = ln 2, S(0) = S(1) = 0, symmetric.
+    assert abs(oscf.smearing_entropy(np.array([0.5]))[0] - math.log(2.0)) < 1e-15
+    assert oscf.smearing_entropy(np.array([0.0, 1.0])).tolist() == [0.0, 0.0]
+    f = rng.uniform(0, 1, 50)
+    assert np.max(np.abs(oscf.smearing_entropy(f) - oscf.smearing_entropy(1 - f))) < 1e-14
+    assert np.max(np.abs(dscf.fd_entropy(f) - oscf.smearing_entropy(f))) < 1e-14
+    # Fermi-Dirac at e = mu is 1/2; sigma -> 0 is a step.
+    assert oscf.smearing_occ(np.array([0.3]), 0.3, 0.01)[0] == 0.5
+    assert oscf.smearing_occ(np.array([0.2, 0.4]), 0.3, 0.0).tolist() == [1.0, 0.0]
+
+
+def test_time_reversal_mesh():
+    s = dscf.aluminium(ecut_ry=8.0, kmesh=(4, 4, 4))
+    assert len(s.kpoints) == 36 and abs(sum(s.kweights) - 1.0) < 1e-15
+    frac = [np.linalg.solve(s.recip.T, k) for k in s.kpoints]
+    keys = {tuple(np.round(np.mod(f * 4, 4)).astype(int)) for f in frac}
+    for kk in keys:  # no point together with its time-reversed partner unless self-inverse
+        neg = tuple(int(v) for v in np.mod(-np.array(kk), 4))
+        assert neg == kk or neg not in keys
+
+
+def test_oracle_smearing_scf_golden():
+    gold = np.load(GOLD.parent / "scf_al4_small.npz")
+    s = dscf.aluminium(ecut_ry=8.0, kmesh=(2, 2, 2), nbands=10)
+    res = oscf.run_scf(s, n_iter=60, beta=0.3, conv=1e-10, pulay_history=8, pulay_start=4)
+    assert res.converged and len(res.energies) == len(gold["energies"])
+    assert np.max(np.abs(np.array(res.energies) - gold["energies"])) < 1e-10
+    assert abs(np.sum(res.rho) * s.volume / res.rho.size - s.nelec) < 1e-8
