@@ -595,6 +595,16 @@ int dzhip_scf_grid_hxc(dzhip_scf_grid* grid, double* out, uint64_t n_out) {
     });
 }
 
+int dzhip_scf_grid_set_dielectric(dzhip_scf_grid* grid, double diemac, double dielng) {
+    return guarded([&] {
+        DZ_REQUIRE(grid, DZHIP_ERR_STATE, "NULL grid");
+        DZ_REQUIRE(diemac >= 1.0 && dielng > 0.0, DZHIP_ERR_ARG, "diemac >= 1 and dielng > 0");
+        std::lock_guard<std::mutex> lk(grid->mu);
+        grid->diemac = diemac;
+        grid->dielng = dielng;
+    });
+}
+
 int dzhip_scf_grid_mix(dzhip_scf_grid* grid, double beta, int pulay) {
     return guarded([&] {
         DZ_REQUIRE(grid, DZHIP_ERR_STATE, "NULL grid");

@@ -16,6 +16,7 @@ struct ScfGrid {
     AxisTables tab;
     DevBuf g2, work, rho_in, rho_out, v_in, v_out, v_ion, local, part, hv, hr, slots, coef;
     int hist_cap = 0;
+    double diemac = 1.0, dielng = 1.0;  // model dielectric preconditioner of the Pulay residual
     std::vector<int> order;              // Pulay ring slots, oldest first
     std::vector<std::vector<double>> b;  // overlaps <R_i|R_j>/N by slot
     std::mutex mu;

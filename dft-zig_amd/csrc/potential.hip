@@ -141,6 +141,24 @@ __global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b
     if (i < n) out[i] = a[i] - b[i];
 }
 
+// w(G) *= P(G), P = (l^2 G^2 + 1/eps) / (l^2 G^2 + 1), P(0) = 1/eps (mixing.zig:309-338); the
+// forward transform is unnormalised and the inverse carries the 1/N, so only P is applied.
+__global__ void k_dielectric(double2* __restrict__ w, const double* __restrict__ g2, double inv_eps, double l2,
+                             size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double g = g2[i];
+    const double x = l2 * g;
+    const double p = g < 1e-12 ? inv_eps : (x + inv_eps) / (x + 1.0);
+    w[i] = make_double2(w[i].x * p, w[i].y * p);
+}
+
+// out = Re(w)
+__global__ void k_real_part(const double2* __restrict__ w, double* __restrict__ out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = w[i].x;
+}
+
 // v = sum_j c_j (hv[slot_j] + beta hr[slot_j])
 __global__ void k_pulay_combine(const double* __restrict__ hv, const double* __restrict__ hr, const int* __restrict__ slot,
                                 const double* __restrict__ c, int m, double beta, double* __restrict__ v, size_t n) {
@@ -308,6 +326,18 @@ void scf_grid_mix(ScfGrid& g, double beta, int pulay) {
     DZ_HIP(hipMemcpyAsync(hv, g.v_in.ptr, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_sub, dim3(grid_blocks(n)), dim3(256), 0, s, g.v_out.as<double>(), g.v_in.as<double>(), hr, n);
     DZ_HIP(hipGetLastError());
+    if (g.diemac > 1.0) {
+        // R <- P R through the FFT (scf.zig:164-178 applies it in G space before the Pulay step).
+        hipLaunchKernelGGL(k_real_to_complex, dim3(grid_blocks(n)), dim3(256), 0, s, hr, g.work.as<double2>(), n);
+        DZ_HIP(hipGetLastError());
+        fft3d_device<double>(g.work.as<double2>(), g.tab, false, s);
+        hipLaunchKernelGGL(k_dielectric, dim3(grid_blocks(n)), dim3(256), 0, s, g.work.as<double2>(), g.g2.as<double>(),
+                           1.0 / g.diemac, g.dielng * g.dielng, n);
+        DZ_HIP(hipGetLastError());
+        fft3d_device<double>(g.work.as<double2>(), g.tab, true, s);
+        hipLaunchKernelGGL(k_real_part, dim3(grid_blocks(n)), dim3(256), 0, s, g.work.as<double2>(), hr, n);
+        DZ_HIP(hipGetLastError());
+    }
     for (int j : g.order) {
         const double d = g.dot(hr, g.hr.as<double>() + (size_t)j * n) / (double)n;
         g.b[slot][j] = d;

@@ -37,7 +37,7 @@ EXPORTS = (
     "dzhip_mlp_loss_grad", "dzhip_mlp_train", "dzhip_cg_solve",
     "dzhip_scf_grid_create", "dzhip_scf_grid_destroy", "dzhip_scf_grid_set_ion", "dzhip_scf_grid_init",
     "dzhip_ham_set_local_grid", "dzhip_scf_grid_add_solution_density", "dzhip_scf_grid_hxc",
-    "dzhip_scf_grid_mix", "dzhip_scf_grid_read", "dzhip_ham_set_tuning",
+    "dzhip_scf_grid_mix", "dzhip_scf_grid_read", "dzhip_ham_set_tuning", "dzhip_scf_grid_set_dielectric",
 )
 
 
@@ -161,6 +161,7 @@ def _load():
         "dzhip_scf_grid_mix": ([vp, ct.c_double, i32], i32),
         "dzhip_scf_grid_read": ([vp, i32, dp, u64], i32),
         "dzhip_ham_set_tuning": ([vp, i32], i32),
+        "dzhip_scf_grid_set_dielectric": ([vp, ct.c_double, ct.c_double], i32),
     }
     missing = set(EXPORTS) - set(sig)
     if missing:
@@ -543,6 +544,9 @@ class ScfGrid:
         out = np.zeros(4)
         check(L().dzhip_scf_grid_hxc(self.handle, _dp(out), 4))
         return tuple(float(v) for v in out)
+
+    def set_dielectric(self, diemac: float = 1.0, dielng: float = 1.0):
+        check(L().dzhip_scf_grid_set_dielectric(self.handle, float(diemac), float(dielng)))
 
     def mix(self, beta: float, pulay: bool):
         check(L().dzhip_scf_grid_mix(self.handle, float(beta), 1 if pulay else 0))

@@ -300,6 +300,14 @@ def occupations(system, eigenvalues):
     return ws, ts, mu
 
 
+def model_dielectric(resid, grid, g2, diemac, dielng):
+    """P(G) R(G), P = (dielng^2 G^2 + 1/diemac) / (dielng^2 G^2 + 1), P(0) = 1/diemac: the model
+    dielectric preconditioner applied to the Pulay residual (mixing.zig:309-338, scf.zig:164-178)."""
+    x = dielng * dielng * g2
+    p = np.where(g2 < 1e-12, 1.0 / diemac, (x + 1.0 / diemac) / (x + 1.0))
+    return to_r(to_g(resid, grid) * p, grid).real
+
+
 class Pulay:
     """DIIS on (V_in, R = V_out - V_in): keep the last `history` pairs, linear step while fewer
     than two, else V = sum_i c_i (V_i + beta R_i) with sum_i c_i = 1 minimising |sum_i c_i R_i|
@@ -501,7 +509,7 @@ class ScfDriver:
     def run(self, max_iter=60, beta=0.3, conv=1e-9, precision=FP64, preconditioner=PRECOND_KINETIC,
             pulay_history=8, pulay_start=4, eig_tol=1e-4, eig_max_iter=20, warmup_steps=2, warmup_tol=1e-3,
             warmup_max_iter=10, mixed_switch_tol=0.0, final_tol=None, energy_tol_ry=None, eig_tol_ratio=None,
-            eig_tol_min=1e-8, progress=None):
+            eig_tol_min=1e-8, progress=None, diemac=1.0, dielng=1.0, max_subspace=0, init_diagonal=False):
         """SCF from the uniform start density nelec/Omega.  Stops when the RMS density change is
         below conv, or (with energy_tol_ry) when two successive total energies after the warm-up
         differ by less than energy_tol_ry.
@@ -519,9 +527,11 @@ class ScfDriver:
             if not warm and eig_tol_ratio is not None and drho_prev is not None:
                 tol = min(eig_tol, max(eig_tol_min, eig_tol_ratio * drho_prev))
             return dict(tol=tol, max_iter=warmup_max_iter if warm else eig_max_iter, precision=precision,
-                        preconditioner=preconditioner, mixed_switch_tol=mixed_switch_tol)
+                        preconditioner=preconditioner, mixed_switch_tol=mixed_switch_tol,
+                        max_subspace=max_subspace, init_diagonal=init_diagonal and it == 0)
 
         if self.grid is not None:
+            self.grid.set_dielectric(diemac, dielng)
             run = self._run_device(max_iter, beta, conv, pulay_history, pulay_start, warmup_steps, solve_opts,
                                    progress, energy_tol_ry)
             self._final(run, final_tol, preconditioner)
@@ -558,7 +568,10 @@ class ScfDriver:
             if not done:
                 v_out = vh_out + vxc_out
                 if mixer is not None and it >= pulay_start:
-                    v_in = mixer.step(v_in, v_out - v_in, beta)
+                    resid = v_out - v_in
+                    if diemac > 1.0:
+                        resid = model_dielectric(resid, s.grid, self.g2, diemac, dielng)
+                    v_in = mixer.step(v_in, resid, beta)
                 else:
                     v_in = v_in + beta * (v_out - v_in)
             run.records[-1].wall_s = time.perf_counter() - t0

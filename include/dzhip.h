@@ -304,6 +304,9 @@ int dzhip_scf_grid_add_solution_density(dzhip_scf_grid* grid, dzhip_ham* ham, co
 /* From the density sum: V_out = V_H + V_xc; out[0] = E_H, out[1] = E_xc, out[2] = int V_in rho,
  * out[3] = RMS(rho - rho_previous) (energies in Ry); then the sum becomes rho and is cleared. */
 int dzhip_scf_grid_hxc(dzhip_scf_grid* grid, double* out, uint64_t n_out);
+/* Model dielectric preconditioner of the Pulay residual, P(G) = (dielng^2 G^2 + 1/diemac) /
+ * (dielng^2 G^2 + 1) (src/features/scf/mixing.zig:309-338); diemac = 1 (default) disables it. */
+int dzhip_scf_grid_set_dielectric(dzhip_scf_grid* grid, double diemac, double dielng);
 /* V_in <- linear (pulay = 0) or Pulay (pulay != 0) mix of V_in and V_out. */
 int dzhip_scf_grid_mix(dzhip_scf_grid* grid, double beta, int pulay);
 int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t count);

@@ -15,6 +15,7 @@ Restates (paths relative to the dft-zig source tree):
   Pulay/DIIS on (V_in, R = V_out - V_in), history 8, from iteration 4 src/features/scf/mixing.zig:183-234,
                                                                      src/features/scf/scf.zig:147-188,
                                                                      src/features/config/defaults.zig:42-45
+  model dielectric preconditioner of the Pulay residual (diemac > 1)  src/features/scf/mixing.zig:309-338
   Pulay system by Gaussian elimination, |pivot| < 1e-12 -> equal weights
                                                                      src/features/scf/mixing.zig:340-400
   convergence on the RMS density change                              src/features/scf/util.zig:17-24
@@ -239,7 +240,18 @@ class ScfResult:
     converged: bool = False
 
 
-def run_scf(system, n_iter=40, beta=0.3, conv=1e-8, pulay_history=0, pulay_start=4, pulay_relative=True):
+def model_dielectric(residual_r, grid, recip, diemac, dielng):
+    """P(G) R(G) with P = (dielng^2 G^2 + 1/diemac) / (dielng^2 G^2 + 1), P(0) = 1/diemac
+    (mixing.zig:309-338), applied to a real-space residual through the FFT."""
+    n = residual_r.shape[0]
+    _, g2 = g_squared_grid(grid, recip)
+    x = dielng * dielng * g2
+    p = np.where(g2 < 1e-12, 1.0 / diemac, (x + 1.0 / diemac) / (x + 1.0))
+    return fftconv.inverse3d(fftconv.forward3d(residual_r.astype(np.complex128), grid) * p, grid).real
+
+
+def run_scf(system, n_iter=40, beta=0.3, conv=1e-8, pulay_history=0, pulay_start=4, pulay_relative=True,
+            diemac=1.0, dielng=1.0):
     """Fixed-occupation SCF with potential mixing (linear, or Pulay from iteration `pulay_start`
     when pulay_history > 0, scf.zig:164); dense eigh per k."""
     grid, recip, vol = system.grid, system.recip, system.volume
@@ -292,7 +304,10 @@ def run_scf(system, n_iter=40, beta=0.3, conv=1e-8, pulay_history=0, pulay_start
             res.converged = True
             break
         if mixer is not None and it >= pulay_start:
-            v_in = mixer.mix(v_in, v_out - v_in, beta)
+            resid = v_out - v_in
+            if diemac > 1.0:
+                resid = model_dielectric(resid, grid, recip, diemac, dielng)
+            v_in = mixer.mix(v_in, resid, beta)
         else:
             v_in = v_in + beta * (v_out - v_in)
     res.rho = rho

@@ -208,3 +208,23 @@ def test_scf_smearing_al(dz, ctx, concurrent, device_grid):
     assert np.max(np.abs(e[:m] - gold["energies"][:m])) < 1e-8
     assert np.max(np.abs(np.array(run.eigenvalues) - gold["eigenvalues"])) < EV_TOL
     assert float(np.sum(np.abs(run.rho - gold["rho"]))) * s.volume / run.rho.size < RHO_L1
+
+
+def test_scf_dielectric_pulay_host_and_device_vs_oracle(dz, ctx, system):
+    """diemac = 12 model-dielectric Pulay (mixing.zig:309-338): host and device drivers follow
+    the oracle's trajectory with tight eigen-solves."""
+    from oracle import scf as oscf
+
+    ref = oscf.run_scf(system, n_iter=40, beta=0.3, conv=1e-10, pulay_history=8, pulay_start=4, diemac=12.0)
+    assert ref.converged
+    for device_grid in (False, True):
+        drv = dscf.ScfDriver(system, device_grid=device_grid)
+        try:
+            run = drv.run(max_iter=40, conv=1e-10, eig_tol=1e-10, eig_max_iter=400, warmup_steps=0, diemac=12.0)
+        finally:
+            drv.close()
+        e = np.array(run.energies)
+        m = min(len(e), len(ref.energies))
+        print(device_grid, len(e), len(ref.energies), np.max(np.abs(e[:m] - np.array(ref.energies[:m]))))
+        assert run.converged and abs(len(e) - len(ref.energies)) <= 3
+        assert np.max(np.abs(e[:m] - np.array(ref.energies[:m]))) < 1e-8

@@ -188,3 +188,20 @@ def test_oracle_smearing_scf_golden():
     assert res.converged and len(res.energies) == len(gold["energies"])
     assert np.max(np.abs(np.array(res.energies) - gold["energies"])) < 1e-10
     assert abs(np.sum(res.rho) * s.volume / res.rho.size - s.nelec) < 1e-8
+
+
+def test_model_dielectric_single_mode():
+    """P(G) on a single cosine: scaled by (l^2 G^2 + 1/eps) / (l^2 G^2 + 1); a constant by 1/eps."""
+    s = dscf.si8_small()
+    nx, ny, nz = s.grid
+    z, y, x = np.meshgrid(np.arange(nz) / nz, np.arange(ny) / ny, np.arange(nx) / nx, indexing="ij")
+    r = np.stack([x.ravel(), y.ravel(), z.ravel()], 1) @ s.cell
+    gv = 1 * s.recip[0] + 2 * s.recip[1]
+    res = 0.3 + np.cos(r @ gv)
+    g2 = float(gv @ gv)
+    for eps, l in ((12.0, 1.0), (4.0, 2.5)):
+        want = 0.3 / eps + (l * l * g2 + 1 / eps) / (l * l * g2 + 1) * np.cos(r @ gv)
+        got = oscf.model_dielectric(res, s.grid, s.recip, eps, l)
+        assert np.max(np.abs(got - want)) < 1e-12
+        _, g2g = dscf.g_vectors(s.grid, s.recip)
+        assert np.max(np.abs(dscf.model_dielectric(res, s.grid, g2g, eps, l) - want)) < 1e-12
