@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--scf", default="none", choices=("none", "si8", "si64"),
                     help="also time the SCF to 1e-6 Ha (plain FP64 vs mixed+batched)")
     ap.add_argument("--scf-max-iter", type=int, default=40)
+    ap.add_argument("--xline-split", type=int, default=0,
+                    help="fix the x-line kernel's y-slab count (0 = autotune on the first apply)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (k-point sharding is "
@@ -180,6 +182,8 @@ def main():
     ngrid = int(np.prod(w.grid))
     ctx = dzhip.Context(0)
     ham = dzhip.Hamiltonian.from_workload(ctx, w)
+    if args.xline_split:
+        ham.set_tuning(args.xline_split)
     info = ham.info()
     nb, npw = w.nbands, w.npw
 

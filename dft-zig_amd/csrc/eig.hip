@@ -704,6 +704,7 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
     w.snap_nb = nb;
     int precond = o.preconditioner;
     std::vector<double> res_hist;
+    const int nconv = (o.nconv == 0 || o.nconv > (uint64_t)nb) ? nb : (int)o.nconv;
     std::vector<double> lam(nb), best(nb);
     bool have_best = false;
     uint64_t iter = 0;
@@ -721,7 +722,8 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
                            w.lam.as<double>(), w.R.as<double2>(), n, (int)n);
         DZ_HIP(hipGetLastError());
         std::vector<double> rn = S.col_norms(w.R.as<double2>(), nb);
-        const double max_res = *std::max_element(rn.begin(), rn.end());
+        // Convergence is judged on the first nconv bands (all by default); the others are a buffer.
+        const double max_res = *std::max_element(rn.begin(), rn.begin() + nconv);
         S.st.max_residual = max_res;
         if (iter == 0) S.st.initial_residual = max_res;
         {

@@ -78,7 +78,7 @@ class EigOpts(ct.Structure):
         ("max_iter", ct.c_uint64), ("tol", ct.c_double), ("max_subspace", ct.c_uint64),
         ("block_size", ct.c_uint64), ("init_diagonal", ct.c_int), ("precision", ct.c_int),
         ("mixed_switch_tol", ct.c_double), ("preconditioner", ct.c_int), ("seed", ct.c_uint64),
-        ("reuse_vectors", ct.c_int), ("collect_snapshots", ct.c_uint64),
+        ("reuse_vectors", ct.c_int), ("collect_snapshots", ct.c_uint64), ("nconv", ct.c_uint64),
     ]
 
 
@@ -404,7 +404,7 @@ class Hamiltonian:
     def eig_solve(self, nbands: int, diag=None, tol: float = 1e-6, max_iter: int = 40, precision: int = FP64,
                   preconditioner: int = PRECOND_KINETIC, init=None, max_subspace: int = 0, block_size: int = 0,
                   init_diagonal: bool = False, mixed_switch_tol: float = 0.0, seed: int = 0,
-                  want_vectors: bool = True, reuse: bool = False, collect_snapshots: int = 0):
+                  want_vectors: bool = True, reuse: bool = False, collect_snapshots: int = 0, nconv: int = 0):
         """Lowest eigenpairs (hermitian_eigen_decomp_iterative counterpart).  diag defaults to the
         kinetic energies the context was built with; init is an (init_cols, n) block of start vectors;
         reuse (without init) starts from the previous solve's vectors kept on the device.
@@ -413,7 +413,7 @@ class Hamiltonian:
             diag = self._kinetic
         diag = np.ascontiguousarray(diag, dtype=np.float64)
         o = EigOpts(max_iter, tol, max_subspace, block_size, 1 if init_diagonal else 0, precision,
-                    mixed_switch_tol, preconditioner, seed, 1 if reuse else 0, collect_snapshots)
+                    mixed_switch_tol, preconditioner, seed, 1 if reuse else 0, collect_snapshots, nconv)
         vals = np.zeros(nbands)
         vecs = np.empty((nbands, self.n), dtype=np.complex128) if want_vectors else None
         st = EigStats()
@@ -449,7 +449,7 @@ class Hamiltonian:
         if diag is None:
             diag = self._kinetic
         diag = np.ascontiguousarray(diag, dtype=np.float64)
-        o = EigOpts(max_iter, tol, 0, 0, 0, FP64, 0.0, PRECOND_KINETIC, seed, 1 if reuse else 0, 0)
+        o = EigOpts(max_iter, tol, 0, 0, 0, FP64, 0.0, PRECOND_KINETIC, seed, 1 if reuse else 0, 0, 0)
         vals = np.zeros(nbands)
         vecs = np.empty((nbands, self.n), dtype=np.complex128) if want_vectors else None
         st = EigStats()

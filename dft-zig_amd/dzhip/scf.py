@@ -509,7 +509,8 @@ class ScfDriver:
     def run(self, max_iter=60, beta=0.3, conv=1e-9, precision=FP64, preconditioner=PRECOND_KINETIC,
             pulay_history=8, pulay_start=4, eig_tol=1e-4, eig_max_iter=20, warmup_steps=2, warmup_tol=1e-3,
             warmup_max_iter=10, mixed_switch_tol=0.0, final_tol=None, energy_tol_ry=None, eig_tol_ratio=None,
-            eig_tol_min=1e-8, progress=None, diemac=1.0, dielng=1.0, max_subspace=0, init_diagonal=False):
+            eig_tol_min=1e-8, progress=None, diemac=1.0, dielng=1.0, max_subspace=0, init_diagonal=False,
+            nconv=0):
         """SCF from the uniform start density nelec/Omega.  Stops when the RMS density change is
         below conv, or (with energy_tol_ry) when two successive total energies after the warm-up
         differ by less than energy_tol_ry.
@@ -528,7 +529,7 @@ class ScfDriver:
                 tol = min(eig_tol, max(eig_tol_min, eig_tol_ratio * drho_prev))
             return dict(tol=tol, max_iter=warmup_max_iter if warm else eig_max_iter, precision=precision,
                         preconditioner=preconditioner, mixed_switch_tol=mixed_switch_tol,
-                        max_subspace=max_subspace, init_diagonal=init_diagonal and it == 0)
+                        max_subspace=max_subspace, init_diagonal=init_diagonal and it == 0, nconv=nconv)
 
         if self.grid is not None:
             self.grid.set_dielectric(diemac, dielng)

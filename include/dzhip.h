@@ -185,6 +185,9 @@ typedef struct dzhip_eig_opts {
                                 kpoint/solve.zig:542-557, iterative_reuse_vectors) */
     uint64_t collect_snapshots; /* record (Ritz vectors, residuals) of the first this-many FP64
                                    iterations for dzhip_ham_training_samples; 0 = off */
+    uint64_t nconv;          /* converged when the largest residual of the lowest nconv bands is
+                                below tol (the rest are a buffer that still gets new directions);
+                                0 = all nbands, the reference's test (serial.zig:238-283) */
 } dzhip_eig_opts;
 
 typedef struct dzhip_eig_stats {
