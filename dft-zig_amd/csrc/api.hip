@@ -541,7 +541,17 @@ int dzhip_scf_grid_init(dzhip_scf_grid* grid, double nelec) {
         DZ_REQUIRE(grid, DZHIP_ERR_STATE, "NULL grid");
         std::lock_guard<std::mutex> lk(grid->mu);
         DeviceGuard g(grid->ctx->device);
-        scf_grid_init(*grid, nelec);
+        scf_grid_init(*grid, nelec, nullptr);
+    });
+}
+
+int dzhip_scf_grid_init_rho(dzhip_scf_grid* grid, const double* rho, uint64_t count) {
+    return guarded([&] {
+        DZ_REQUIRE(grid, DZHIP_ERR_STATE, "NULL grid");
+        DZ_REQUIRE(rho && count == grid->n, DZHIP_ERR_SIZE, "count != nx*ny*nz");
+        std::lock_guard<std::mutex> lk(grid->mu);
+        DeviceGuard g(grid->ctx->device);
+        scf_grid_init(*grid, 0.0, rho);
     });
 }
 

@@ -138,7 +138,7 @@ void ham_set_local_dev(Ham& h, const double* local_dev);
 struct ScfGrid;
 ScfGrid* scf_grid_create(Context* ctx, int nx, int ny, int nz, const double* recip, double volume, int history);
 void scf_grid_destroy(ScfGrid* g);
-void scf_grid_init(ScfGrid& g, double nelec);
+void scf_grid_init(ScfGrid& g, double nelec, const double* rho_host);  // rho_host NULL: uniform
 void scf_grid_local(ScfGrid& g);
 void scf_grid_add(ScfGrid& g, const double* rho_k);
 void scf_grid_hxc(ScfGrid& g, double out[4]);

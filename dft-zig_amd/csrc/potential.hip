@@ -265,10 +265,14 @@ void scf_grid_destroy(ScfGrid* g) {
     delete g;
 }
 
-void scf_grid_init(ScfGrid& g, double nelec) {
+void scf_grid_init(ScfGrid& g, double nelec, const double* rho_host) {
     hipStream_t s = g.stream();
-    hipLaunchKernelGGL(k_fill, dim3(grid_blocks(g.n)), dim3(256), 0, s, g.rho_in.as<double>(), nelec / g.volume, g.n);
-    DZ_HIP(hipGetLastError());
+    if (rho_host) {
+        DZ_HIP(hipMemcpyAsync(g.rho_in.ptr, rho_host, g.n * sizeof(double), hipMemcpyHostToDevice, s));
+    } else {
+        hipLaunchKernelGGL(k_fill, dim3(grid_blocks(g.n)), dim3(256), 0, s, g.rho_in.as<double>(), nelec / g.volume, g.n);
+        DZ_HIP(hipGetLastError());
+    }
     double e[4];
     // v_in = V_Hxc[rho_in]; the reference arguments for rho_ref/vin only feed the unused sums.
     g.hxc(g.rho_in.as<double>(), g.rho_in.as<double>(), g.rho_in.as<double>(), g.v_in.as<double>(), e);

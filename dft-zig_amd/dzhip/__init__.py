@@ -38,6 +38,7 @@ EXPORTS = (
     "dzhip_scf_grid_create", "dzhip_scf_grid_destroy", "dzhip_scf_grid_set_ion", "dzhip_scf_grid_init",
     "dzhip_ham_set_local_grid", "dzhip_scf_grid_add_solution_density", "dzhip_scf_grid_hxc",
     "dzhip_scf_grid_mix", "dzhip_scf_grid_read", "dzhip_ham_set_tuning", "dzhip_scf_grid_set_dielectric",
+    "dzhip_scf_grid_init_rho",
 )
 
 
@@ -162,6 +163,7 @@ def _load():
         "dzhip_scf_grid_read": ([vp, i32, dp, u64], i32),
         "dzhip_ham_set_tuning": ([vp, i32], i32),
         "dzhip_scf_grid_set_dielectric": ([vp, ct.c_double, ct.c_double], i32),
+        "dzhip_scf_grid_init_rho": ([vp, dp, u64], i32),
     }
     missing = set(EXPORTS) - set(sig)
     if missing:
@@ -531,6 +533,10 @@ class ScfGrid:
 
     def init(self, nelec: float):
         check(L().dzhip_scf_grid_init(self.handle, float(nelec)))
+
+    def init_rho(self, rho):
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        check(L().dzhip_scf_grid_init_rho(self.handle, _dp(r), r.shape[0]))
 
     def set_local(self, ham: "Hamiltonian"):
         check(L().dzhip_ham_set_local_grid(ham.handle, self.handle))

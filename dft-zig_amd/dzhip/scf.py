@@ -56,6 +56,7 @@ class ScfSystem:
     kpoints: list  # Cartesian, bohr^-1
     kweights: list
     smearing: float = 0.0  # Fermi-Dirac width (Ry); 0 = fixed occupations
+    atom_rho_width: float = 1.2  # bohr: Gaussian valence charge per atom of the atomic start density
 
 
 # GTH-form parameters of a four-electron Si-like pseudo-atom (Hartree units for C1 and h): the
@@ -232,6 +233,18 @@ def ionic_potential(system, g, g2):
     vq[nz] = 2.0 * (-4.0 * math.pi * s.z_ion / q2 + s.c1 * (2.0 * math.pi) ** 1.5 * s.rloc ** 3) \
         * np.exp(-0.5 * q2 * s.rloc ** 2)
     return to_r(sf * vq * (g.shape[0] / s.volume), s.grid).real
+
+
+def atomic_density(system, g, g2):
+    """Superposition of atomic valence charges (core_density.zig:128-178): rho(G) =
+    (1/Omega) sum_a Z e^{-G^2 a^2 / 2} e^{-iG.tau_a}, a Gaussian of width a per atom (the
+    synthetic species carry no PP_RHOATOM), so int rho = nelec."""
+    s = system
+    sf = np.zeros(g.shape[0], dtype=np.complex128)
+    for tau in s.atom_pos:
+        sf += np.exp(-1j * (g @ tau))
+    rho_g = s.z_ion * np.exp(-0.5 * g2 * s.atom_rho_width ** 2) * sf / s.volume
+    return to_r(rho_g * g.shape[0], s.grid).real
 
 
 def band_weights(nelec, nbands, kweight):
@@ -473,11 +486,14 @@ class ScfDriver:
         return vals, [r[1] for r in res], rho, e_band, ts
 
     def _run_device(self, max_iter, beta, conv, pulay_history, pulay_start, warmup_steps, solve_opts, progress,
-                    energy_tol_ry):
+                    energy_tol_ry, rho0=None):
         """The SCF loop with rho, V and mixing on the device (same algorithm as run())."""
         s = self.sys
         g = self.grid
-        g.init(s.nelec)
+        if rho0 is None:
+            g.init(s.nelec)
+        else:
+            g.init_rho(rho0)
         run = ScfRun(setup_s=self.setup_s)
         t0 = time.perf_counter()
         drho_prev = None
@@ -510,8 +526,9 @@ class ScfDriver:
             pulay_history=8, pulay_start=4, eig_tol=1e-4, eig_max_iter=20, warmup_steps=2, warmup_tol=1e-3,
             warmup_max_iter=10, mixed_switch_tol=0.0, final_tol=None, energy_tol_ry=None, eig_tol_ratio=None,
             eig_tol_min=1e-8, progress=None, diemac=1.0, dielng=1.0, max_subspace=0, init_diagonal=False,
-            nconv=0):
-        """SCF from the uniform start density nelec/Omega.  Stops when the RMS density change is
+            nconv=0, start="uniform"):
+        """SCF from the uniform start density nelec/Omega (or start="atomic": a superposition of
+        atomic valence charges).  Stops when the RMS density change is
         below conv, or (with energy_tol_ry) when two successive total energies after the warm-up
         differ by less than energy_tol_ry.
 
@@ -531,15 +548,16 @@ class ScfDriver:
                         preconditioner=preconditioner, mixed_switch_tol=mixed_switch_tol,
                         max_subspace=max_subspace, init_diagonal=init_diagonal and it == 0, nconv=nconv)
 
+        rho0 = atomic_density(s, self.g, self.g2) if start == "atomic" else None
         if self.grid is not None:
             self.grid.set_dielectric(diemac, dielng)
             run = self._run_device(max_iter, beta, conv, pulay_history, pulay_start, warmup_steps, solve_opts,
-                                   progress, energy_tol_ry)
+                                   progress, energy_tol_ry, rho0)
             self._final(run, final_tol, preconditioner)
             return run
         n = int(np.prod(s.grid))
         vol = s.volume
-        rho = np.full(n, s.nelec / vol)
+        rho = np.full(n, s.nelec / vol) if rho0 is None else rho0
         v_in = hartree(rho, s.grid, self.g2, vol)[0] + lda_pz(rho)[1]
         mixer = Pulay(pulay_history) if pulay_history > 0 else None
         run = ScfRun(setup_s=self.setup_s)

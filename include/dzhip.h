@@ -299,6 +299,9 @@ void dzhip_scf_grid_destroy(dzhip_scf_grid* grid);
 int dzhip_scf_grid_set_ion(dzhip_scf_grid* grid, const double* v_ion, uint64_t count);
 /* rho = nelec / volume, V_in = V_Hxc[rho], Pulay history cleared, density sum cleared. */
 int dzhip_scf_grid_init(dzhip_scf_grid* grid, double nelec);
+/* The same from a given start density (host, FFT order), e.g. the superposition of atomic
+ * charges of core_density.zig:128-178. */
+int dzhip_scf_grid_init_rho(dzhip_scf_grid* grid, const double* rho, uint64_t count);
 /* local_r of ham = V_ion + V_in (dzhip_ham_set_local without the host copy). */
 int dzhip_ham_set_local_grid(dzhip_ham* ham, dzhip_scf_grid* grid);
 /* Add (1/Omega) sum_b w_b |psi_b(r)|^2 of ham's last eigensolver solution to the density sum. */
