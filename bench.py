@@ -111,20 +111,44 @@ def cpu_baseline(w, budget_s=12.0):
                       f"{t_used / done * 1e3:.1f} ms/band -> {t_used / done * w.nbands:.1f} s per 160-band block"}
 
 
+# SCF settings shared by every variant (only precision, k-point concurrency and the
+# preconditioner differ): Pulay history 8 from iteration 4, beta 0.3, model-dielectric
+# preconditioner diemac 12 (a Si setting of the reference's config), atomic start density,
+# diagonal start vectors, eigensolver tolerance min(1e-4, drho / 10) >= 1e-7 on the occupied bands.
+SCF_COMMON = dict(conv=1e-8, diemac=12.0, start="atomic", init_diagonal=True, eig_tol_ratio=0.1, eig_tol_min=1e-7)
+
+
 def scf_leg(name, max_iter, log=sys.stderr):
-    """Time-to-1e-6 Ha of the SCF: plain FP64 (k-points one after another on one stream) against
-    the mixed form (FP32 H apply, FP64 subspace algebra and FP64 apply below a 1e-5 residual,
-    every k-point on its own stream and host thread).  E_inf is the plain FP64 run's converged
-    energy (the driver is tied to the FP64 oracle SCF on the small Si case by the parity tests)."""
+    """Time-to-1e-6 Ha of the SCF for three forms: plain FP64 (k-points one after another on one
+    stream), mixed + batched (FP32 H apply with FP64 subspace algebra and an FP64 apply below a
+    1e-5 residual, every k-point on its own stream and host thread), and the same with the
+    learned preconditioner (trained first, on the device, on residual/correction pairs of Si-8 and
+    Al solves; its training time is reported, not counted).  E_inf is the plain FP64 run's final
+    energy (the GPU SCF is tied to the FP64 oracle SCF on the small Si cell by the parity tests)."""
     import dzhip
+    from dzhip import precond
     from dzhip import scf as dscf
 
     system = {"si64": dscf.si64, "si8": dscf.si8_small}[name]()
-    variants = [("fp64", dzhip.FP64, False), ("mixed_batched", dzhip.FP32, True)]
+    nocc = int(np.ceil(system.nelec / 2))
+    t0 = time.perf_counter()
+    tctx = dzhip.Context(0)
+    params, hist, nsamp = precond.train_on_systems(
+        tctx, [dscf.si8_small(), dscf.aluminium(ecut_ry=8.0, kmesh=(2, 2, 2), nbands=10)], epochs=40)
+    tctx.close()
+    train_s = time.perf_counter() - t0
+    print(f"learned preconditioner: {nsamp} samples, loss {hist[0]:.4f} -> {hist[-1]:.4f}, {train_s:.1f} s",
+          file=log, flush=True)
+    variants = [("fp64", dzhip.FP64, False, dzhip.PRECOND_KINETIC),
+                ("mixed_batched", dzhip.FP32, True, dzhip.PRECOND_KINETIC),
+                ("mixed_batched_learned", dzhip.FP32, True, dzhip.PRECOND_MLP)]
     runs, setup = {}, {}
-    for tag, prec, conc in variants:
-        drv = dscf.ScfDriver(system, concurrent_k=conc)
+    for tag, prec, conc, pc in variants:
+        drv = dscf.ScfDriver(system, concurrent_k=conc, device_grid=True)
         setup[tag] = drv.setup_s
+        if pc == dzhip.PRECOND_MLP:
+            for h in drv.hams:
+                h.set_precond_model(params, precond.ENERGY_SCALE)
 
         def show(r, tag=tag):
             sec = " ".join(f"{k[2:-3]} {v:.0f}" for k, v in r.sections_ms.items())
@@ -132,25 +156,31 @@ def scf_leg(name, max_iter, log=sys.stderr):
                   f"eig {r.eig_s:.2f}s eig_it {r.eig_iterations} applies {r.applies} fp32 {r.fp32_applies} "
                   f"gpu_ms[{sec}]", file=log, flush=True)
         try:
-            runs[tag] = drv.run(max_iter=max_iter, conv=1e-8, precision=prec, eig_tol_ratio=1.0, eig_tol_min=1e-7,
-                                progress=show)
+            runs[tag] = drv.run(max_iter=max_iter, precision=prec, preconditioner=pc, nconv=nocc, progress=show,
+                                **SCF_COMMON)
         finally:
             drv.close()
     e_inf = runs["fp64"].energies[-1]
     out = {"system": system.name, "grid": list(system.grid), "k_points": len(system.kpoints), "bands": system.nbands,
-           "e_inf_ry": e_inf, "e_inf_source": "plain FP64 GPU run, converged to RMS drho < 1e-8"}
+           "e_inf_ry": e_inf, "e_inf_source": "plain FP64 GPU run, final iteration",
+           "settings": {k: v for k, v in SCF_COMMON.items()},
+           "learned_training": {"samples": nsamp, "loss_start": float(hist[0]), "loss_end": float(hist[-1]),
+                                "seconds": round(train_s, 2)}}
     for tag, run in runs.items():
         t = run.time_to(e_inf)
         out[tag] = {"t_1e-6_Ha_s": None if t is None else round(t, 3), "iterations": len(run.records),
                     "converged": run.converged, "final_dE_Ha": (run.energies[-1] - e_inf) * dscf.HA_PER_RY,
                     "wall_s": round(run.records[-1].wall_s, 3), "setup_s": round(setup[tag], 3),
                     "eig_s": round(sum(r.eig_s for r in run.records), 3),
+                    "davidson_iterations": sum(r.eig_iterations for r in run.records),
                     "applies": sum(r.applies for r in run.records),
                     "solver_gpu_ms": {k[2:-3]: round(sum(r.sections_ms.get(k, 0.0) for r in run.records), 1)
                                       for k in dscf.SECTIONS},
                     "fp32_applies": sum(r.fp32_applies for r in run.records)}
-    a, b = out["fp64"]["t_1e-6_Ha_s"], out["mixed_batched"]["t_1e-6_Ha_s"]
-    out["speedup"] = None if not a or not b else round(a / b, 3)
+    a = out["fp64"]["t_1e-6_Ha_s"]
+    for tag in ("mixed_batched", "mixed_batched_learned"):
+        b = out[tag]["t_1e-6_Ha_s"]
+        out[f"speedup_{tag}"] = None if not a or not b else round(a / b, 3)
     return out
 
 
@@ -162,7 +192,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--scf", default="none", choices=("none", "si8", "si64"),
                     help="also time the SCF to 1e-6 Ha (plain FP64 vs mixed+batched)")
-    ap.add_argument("--scf-max-iter", type=int, default=40)
+    ap.add_argument("--scf-max-iter", type=int, default=45)
     ap.add_argument("--xline-split", type=int, default=0,
                     help="fix the x-line kernel's y-slab count (0 = autotune on the first apply)")
     args = ap.parse_args()

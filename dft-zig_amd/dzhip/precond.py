@@ -69,3 +69,23 @@ def save(path, params, energy_scale=ENERGY_SCALE, **meta):
 def load(path):
     with np.load(path, allow_pickle=False) as d:
         return d["params"].astype(np.float32), float(d["energy_scale"])
+
+
+def train_on_systems(ctx: Context, systems, nbands=None, snapshots=16, epochs=40, seed=1, lr=2e-3):
+    """Collect residual/correction pairs from one cold FP64 TPA solve per k-point of each SCF
+    system (bare ionic potential) and train a model from the TPA start.  Returns (params,
+    losses, sample count)."""
+    from . import scf as dscf
+
+    chunks = []
+    for system in systems:
+        drv = dscf.ScfDriver(system)
+        try:
+            for ham in drv.hams:
+                ham.set_local(drv.v_ion)
+                chunks.append(collect(ham, nbands or system.nbands, snapshots=snapshots))
+        finally:
+            drv.close()
+    samples = np.concatenate(chunks, axis=1)
+    params, hist = train(ctx, samples, epochs=epochs, lr=lr, seed=seed)
+    return params, hist, samples.shape[1]
