@@ -108,20 +108,47 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
     }
     load_twiddles(twx, tw + ax.tw_off, nx, tid, nthr);
     C* tcol = tbuf + col * (size_t)nplanes * ny * nz + y0;
-    for (int i = tid; i < nplanes * ylen; i += nthr) {
-        const int p = fdiv(i, ylen, my), y = i - p * ylen;
-        buf[y + pitch * pxpos[p]] = tcol[((size_t)p * nz + z) * ny + y];
+    // Loads are issued kBatch at a time before their LDS stores, so each thread keeps several
+    // global loads in flight (the kernel is latency-bound at one outstanding load per thread).
+    constexpr int kBatch = 8;
+    const int ntot = nplanes * ylen;
+    for (int base = 0; base < ntot; base += kBatch * nthr) {
+        C v[kBatch];
+        int dst[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int i = base + u * nthr + tid;
+            const int ic = i < ntot ? i : 0;
+            const int p = fdiv(ic, ylen, my), y = ic - p * ylen;
+            v[u] = tcol[((size_t)p * nz + z) * ny + y];
+            dst[u] = y + pitch * pxpos[p];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (base + u * nthr + tid < ntot) buf[dst[u]] = v[u];
     }
     __syncthreads();
     const LineSet xl{ylen, pitch, 1, nullptr};
     lds_fft_lines<T, C, true, 1>(buf, ax, twx, xl, tid, nthr);
     const T* vz = vloc + (size_t)nx * (ny * (size_t)z + y0);
     const unsigned mx = div_magic(nx);
-    for (int i = tid; i < nx * ylen; i += nthr) {
-        const int y = fdiv(i, nx, mx), xx = i - y * nx;
-        C& e = buf[y + pitch * xx];
-        const T v = vz[i];
-        e = C{e.x * v, e.y * v};
+    const int nv = nx * ylen;
+    for (int base = 0; base < nv; base += kBatch * nthr) {
+        T vv[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int i = base + u * nthr + tid;
+            vv[u] = vz[i < nv ? i : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int i = base + u * nthr + tid;
+            if (i < nv) {
+                const int y = fdiv(i, nx, mx), xx = i - y * nx;
+                C& e = buf[y + pitch * xx];
+                e = C{e.x * vv[u], e.y * vv[u]};
+            }
+        }
     }
     __syncthreads();
     lds_fft_lines<T, C, false, -1>(buf, ax, twx, xl, tid, nthr);
@@ -154,9 +181,24 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_fwd(
     for (int i = tid; i < nact; i += nthr) cols[i] = ycols[c0 + i];
     const C* in = tbuf + (col * nplanes + p) * (size_t)ny * nz;
     const unsigned my = div_magic(ny);
-    for (int i = tid; i < ny * nz; i += nthr) {
-        const int z = fdiv(i, ny, my), yy = i - z * ny;
-        buf[yy + pitch * z] = in[i];
+    // Tile load in batches of loads per thread (several in flight before the LDS stores).
+    constexpr int kBatch = 8;
+    const int ntot = ny * nz;
+    for (int base = 0; base < ntot; base += kBatch * nthr) {
+        C v[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int i = base + u * nthr + tid;
+            v[u] = in[i < ntot ? i : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int i = base + u * nthr + tid;
+            if (i < ntot) {
+                const int z = fdiv(i, ny, my), yy = i - z * ny;
+                buf[yy + pitch * z] = v[u];
+            }
+        }
     }
     __syncthreads();
     // Forward along y on every z-row (frequency y lands at py[y]), then along z only on the

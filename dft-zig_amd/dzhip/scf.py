@@ -30,7 +30,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import FP64, PRECOND_KINETIC, Context, Hamiltonian, ScfGrid, tables
+from . import FP64, PRECOND_KINETIC, PRECOND_MLP, PRECOND_TPA, Context, Hamiltonian, ScfGrid, tables
 from .workloads import SpeciesDesc
 
 HA_PER_RY = 0.5
@@ -544,8 +544,11 @@ class ScfDriver:
             tol = warmup_tol if warm else eig_tol
             if not warm and eig_tol_ratio is not None and drho_prev is not None:
                 tol = min(eig_tol, max(eig_tol_min, eig_tol_ratio * drho_prev))
+            # The learned preconditioner's fallback compares against TPA's convergence rate on
+            # each context, so the warm-up solves run TPA and measure it.
+            pc = PRECOND_TPA if (preconditioner == PRECOND_MLP and warm) else preconditioner
             return dict(tol=tol, max_iter=warmup_max_iter if warm else eig_max_iter, precision=precision,
-                        preconditioner=preconditioner, mixed_switch_tol=mixed_switch_tol,
+                        preconditioner=pc, mixed_switch_tol=mixed_switch_tol,
                         max_subspace=max_subspace, init_diagonal=init_diagonal and it == 0, nconv=nconv)
 
         rho0 = atomic_density(s, self.g, self.g2) if start == "atomic" else None
