@@ -286,5 +286,10 @@ template <> struct TwSel<float> {
 // per CU; more threads per workgroup keep the CU occupied.
 template <typename T> struct TileThreads { static constexpr int value = 512; };
 template <> struct TileThreads<double> { static constexpr int value = 1024; };
+// Global loads each thread issues before its LDS stores in the tile kernels' load loops.  FP32
+// tiles run 512 threads and gain from 8 in flight; FP64 tiles run 1024 threads at a 128-VGPR
+// budget, where 8 double2 loads per thread raise register pressure more than they hide latency.
+template <typename T> struct LoadBatch { static constexpr int value = 8; };
+template <> struct LoadBatch<double> { static constexpr int value = 2; };
 
 }  // namespace dzhip

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
     C* tcol = tbuf + col * (size_t)nplanes * ny * nz + y0;
     // Loads are issued kBatch at a time before their LDS stores, so each thread keeps several
     // global loads in flight (the kernel is latency-bound at one outstanding load per thread).
-    constexpr int kBatch = 8;
+    constexpr int kBatch = LoadBatch<T>::value;
     const int ntot = nplanes * ylen;
     for (int base = 0; base < ntot; base += kBatch * nthr) {
         C v[kBatch];
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_fwd(
     const C* in = tbuf + (col * nplanes + p) * (size_t)ny * nz;
     const unsigned my = div_magic(ny);
     // Tile load in batches of loads per thread (several in flight before the LDS stores).
-    constexpr int kBatch = 8;
+    constexpr int kBatch = LoadBatch<T>::value;
     const int ntot = ny * nz;
     for (int base = 0; base < ntot; base += kBatch * nthr) {
         C v[kBatch];

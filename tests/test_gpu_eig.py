@@ -101,3 +101,16 @@ def test_density_matches_oracle(dz, ctx, name):
     # zero weights skip bands
     rz = ham.density(w.psi, np.zeros(nb))
     assert np.all(rz == 0.0)
+
+
+def test_eig_nconv_converges_lowest_bands(dz, ctx, si8_exact):
+    """nconv = 8 of 12 bands: converged once the lowest 8 residuals are below tol; those match
+    the dense spectrum, and the solve needs no more applies than converging all 12."""
+    w, exact = si8_exact
+    ham = dz.Hamiltonian.from_workload(ctx, w)
+    v_all, _, st_all = ham.eig_solve(12, tol=1e-9, max_iter=300)
+    v8, _, st8 = ham.eig_solve(12, tol=1e-9, max_iter=300, nconv=8)
+    print(st_all["apply_calls"], st8["apply_calls"])
+    assert st8["converged"] == 1
+    np.testing.assert_allclose(v8[:8], exact[:8], atol=EV_TOL)
+    assert st8["apply_calls"] <= st_all["apply_calls"]

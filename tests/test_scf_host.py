@@ -205,3 +205,11 @@ def test_model_dielectric_single_mode():
         assert np.max(np.abs(got - want)) < 1e-12
         _, g2g = dscf.g_vectors(s.grid, s.recip)
         assert np.max(np.abs(dscf.model_dielectric(res, s.grid, g2g, eps, l) - want)) < 1e-12
+
+
+def test_atomic_start_density():
+    for s in (dscf.si8_small(), dscf.si64()):
+        g, g2 = dscf.g_vectors(s.grid, s.recip)
+        rho = dscf.atomic_density(s, g, g2)
+        assert abs(np.sum(rho) * s.volume / rho.size - s.nelec) < 1e-9 * s.nelec
+        assert rho.min() > -1e-6
