@@ -395,11 +395,14 @@ int dzhip_density_accumulate_solution(dzhip_ham* ham, uint64_t nbands, const dou
     });
 }
 
-int dzhip_ham_set_precond_model(dzhip_ham* ham, const float* params, uint64_t n_params, double energy_scale) {
+int dzhip_ham_set_precond_model(dzhip_ham* ham, const float* params, uint64_t n_params, double energy_scale,
+                                int base) {
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
         DZ_REQUIRE(params && n_params == (uint64_t)kMlpParams, DZHIP_ERR_SIZE, "need DZHIP_MLP_PARAMS parameters");
         DZ_REQUIRE(energy_scale > 0.0, DZHIP_ERR_ARG, "energy_scale must be > 0");
+        DZ_REQUIRE(base == DZHIP_PRECOND_KINETIC || base == DZHIP_PRECOND_TPA, DZHIP_ERR_ARG,
+                   "base must be DZHIP_PRECOND_KINETIC or DZHIP_PRECOND_TPA");
         std::lock_guard<std::mutex> lk(ham->mu);
         DeviceGuard g(ham->ctx->device);
         ham->mlp_params.ensure(kMlpParams * sizeof(float));
@@ -407,19 +410,22 @@ int dzhip_ham_set_precond_model(dzhip_ham* ham, const float* params, uint64_t n_
                               ham->stream()));
         DZ_HIP(hipStreamSynchronize(ham->stream()));
         ham->mlp_energy_scale = energy_scale;
+        ham->mlp_base = base;
         ham->has_mlp = true;
+        ham->mlp_fail_streak = 0;
+        ham->mlp_rest = 0;
     });
 }
 
-int dzhip_ham_training_samples(dzhip_ham* ham, double energy_scale, uint64_t stride, float* out, uint64_t capacity,
-                               uint64_t* count) {
+int dzhip_ham_training_samples(dzhip_ham* ham, double energy_scale, int base, uint64_t stride, float* out,
+                               uint64_t capacity, uint64_t* count) {
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
         DZ_REQUIRE(out && count, DZHIP_ERR_ARG, "NULL out or count");
         DZ_REQUIRE(stride >= 1 && stride < (1u << 30), DZHIP_ERR_ARG, "bad stride");
         std::lock_guard<std::mutex> lk(ham->mu);
         DeviceGuard g(ham->ctx->device);
-        *count = eig_training_samples(*ham, energy_scale, (int)stride, out, capacity);
+        *count = eig_training_samples(*ham, energy_scale, base, (int)stride, out, capacity);
     });
 }
 

@@ -703,6 +703,15 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
     w.snaps = 0;
     w.snap_nb = nb;
     int precond = o.preconditioner;
+    // A learned model that fell back in two solves in a row on this context sits out the next
+    // kMlpRest solves (the base preconditioner runs; precond_fallback reports iteration 1).
+    constexpr int kMlpRest = 8;
+    if (precond == DZHIP_PRECOND_MLP && h.mlp_rest > 0) {
+        h.mlp_rest--;
+        precond = h.mlp_base;
+        S.st.precond_fallback = 1;
+    }
+    const bool mlp_ran = precond == DZHIP_PRECOND_MLP;
     std::vector<double> res_hist;
     const int nconv = (o.nconv == 0 || o.nconv > (uint64_t)nb) ? nb : (int)o.nconv;
     std::vector<double> lam(nb), best(nb);
@@ -753,15 +762,16 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
             DZ_HIP(hipStreamSynchronize(s));  // lam / rn are host vectors rebuilt next iteration
             w.snaps++;
         }
-        // Learned preconditioner: fall back to TPA for the rest of the solve when, over the last
-        // three iterations, the unconverged bands' mean log residual fell more slowly than 80% of
-        // the rate TPA solves on this context achieved (or by less than 0.05 decades per
-        // iteration before any TPA rate is known).
+        // Learned preconditioner: fall back to its base preconditioner for the rest of the solve
+        // when, over the last three iterations, the unconverged bands' mean log residual fell
+        // more slowly than 80% of the rate solves with the base preconditioner achieved on this
+        // context (or by less than 0.05 decades per iteration before any such rate is known).
         if (precond == DZHIP_PRECOND_MLP && res_hist.size() > 3) {
             const double rate = (res_hist[res_hist.size() - 4] - res_hist.back()) / 3.0;
-            const double need = h.tpa_rate > 0.0 ? 0.8 * h.tpa_rate : 0.05;
+            const double base_rate = h.precond_rate[h.mlp_base == DZHIP_PRECOND_TPA ? 1 : 0];
+            const double need = base_rate > 0.0 ? 0.8 * base_rate : 0.05;
             if (rate < need) {
-                precond = DZHIP_PRECOND_TPA;
+                precond = h.mlp_base;
                 S.st.precond_fallback = iter + 1;
             }
         }
@@ -793,7 +803,7 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
                 DZ_HIP(hipMemcpyAsync(w.rnorm.ptr, rn.data(), nb * sizeof(double), hipMemcpyHostToDevice, s));
                 precondition_mlp(w.R.as<double2>(), w.sel.as<int>(), (int)sel.size(), w.lam.as<double>(),
                                  w.ekin.as<double>(), w.rnorm.as<double>(), w.diag.as<double>(), h.mlp_params.as<float>(),
-                                 h.mlp_energy_scale, w.Q.as<double2>(), n, (int)n, s);
+                                 h.mlp_energy_scale, h.mlp_base, w.Q.as<double2>(), n, (int)n, s);
             } else {
                 hipLaunchKernelGGL(k_precondition, S.cols_grid((int)sel.size()), dim3(256), 0, s, w.R.as<double2>(),
                                    w.sel.as<int>(), w.lam.as<double>(), w.ekin.as<double>(), w.diag.as<double>(),
@@ -830,11 +840,20 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
     }
     mark(w, s, SEC_END);
     S.st.iterations = iter;
-    // Decades of residual per iteration achieved by TPA solves on this context (running mean),
-    // the yardstick of the learned preconditioner's fallback.
-    if (o.preconditioner == DZHIP_PRECOND_TPA && res_hist.size() >= 3) {
+    if (mlp_ran) {
+        h.mlp_fail_streak = S.st.precond_fallback > 0 ? h.mlp_fail_streak + 1 : 0;
+        if (h.mlp_fail_streak >= 2) {
+            h.mlp_rest = kMlpRest;
+            h.mlp_fail_streak = 0;
+        }
+    }
+    // Decades of residual per iteration achieved by kinetic-shift and TPA solves on this context
+    // (running mean), the yardstick of the learned preconditioner's fallback.
+    if ((o.preconditioner == DZHIP_PRECOND_KINETIC || o.preconditioner == DZHIP_PRECOND_TPA) &&
+        res_hist.size() >= 3) {
         const double r = (res_hist.front() - res_hist.back()) / (double)(res_hist.size() - 1);
-        if (r > 0.0) h.tpa_rate = h.tpa_rate > 0.0 ? 0.5 * (h.tpa_rate + r) : r;
+        double& rate = h.precond_rate[o.preconditioner == DZHIP_PRECOND_TPA ? 1 : 0];
+        if (r > 0.0) rate = rate > 0.0 ? 0.5 * (rate + r) : r;
     }
     if (have_best) {
         std::memcpy(values, best.data(), nb * sizeof(double));
@@ -1104,12 +1123,14 @@ void cg_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts&
 
 void eig_release(Ham& h) { eig_work_release(h); }
 
-size_t eig_training_samples(Ham& h, double energy_scale, int stride, float* out_host, size_t cap) {
+size_t eig_training_samples(Ham& h, double energy_scale, int base, int stride, float* out_host, size_t cap) {
     auto* w = static_cast<EigWork*>(h.eig_work);
     DZ_REQUIRE(w && w->snaps > 0, DZHIP_ERR_STATE, "no training snapshots (solve with collect_snapshots > 0)");
     DZ_REQUIRE(w->x_converged && w->x_cols == (uint64_t)w->snap_nb, DZHIP_ERR_STATE,
                "the recording solve did not converge");
     DZ_REQUIRE(stride >= 1 && energy_scale > 0.0, DZHIP_ERR_ARG, "stride >= 1 and energy_scale > 0");
+    DZ_REQUIRE(base == DZHIP_PRECOND_KINETIC || base == DZHIP_PRECOND_TPA, DZHIP_ERR_ARG,
+               "base must be DZHIP_PRECOND_KINETIC or DZHIP_PRECOND_TPA");
     hipStream_t s = h.stream();
     DZ_BLAS(rocblas_set_stream(w->blas, s));
     const int nb = w->snap_nb;
@@ -1135,7 +1156,7 @@ size_t eig_training_samples(Ham& h, double energy_scale, int stride, float* out_
                               w->Q.as<zc>(), (int)n));
         done += mlp_write_samples(w->Q.as<double2>(), w->snapR.as<double2>() + i * blk, w->snapLam.as<double>() + i * nb,
                                   w->snapEk.as<double>() + i * nb, w->snapRn.as<double>() + i * nb,
-                                  w->diag.as<double>(), n, (int)n, nb, stride, energy_scale, w->bandsum.as<double>(),
+                                  w->diag.as<double>(), n, (int)n, nb, stride, energy_scale, base, w->bandsum.as<double>(),
                                   w->samples.as<float>(), total, done, s);
     }
     for (int f = 0; f < 6; ++f)

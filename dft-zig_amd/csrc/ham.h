@@ -74,11 +74,17 @@ struct Ham {
 
     // Eigensolver workspace (created on first solve); it keeps the last solve's Ritz vectors.
     void* eig_work = nullptr;
-    // Learned preconditioner model (kMlpParams floats) and its feature energy scale (Ha).
+    // Learned preconditioner model (kMlpParams floats), its feature energy scale (Ha) and the
+    // base preconditioner it rescales (DZHIP_PRECOND_KINETIC or DZHIP_PRECOND_TPA).
     DevBuf mlp_params;
     double mlp_energy_scale = 0.0;
+    int mlp_base = DZHIP_PRECOND_TPA;
     bool has_mlp = false;
-    double tpa_rate = 0.0;  // decades of residual per iteration of TPA solves (0: none yet)
+    // Decades of residual per iteration that solves with the kinetic-shift [0] and TPA [1]
+    // preconditioners achieved on this context (running mean; 0 = none yet).
+    double precond_rate[2] = {0.0, 0.0};
+    // Learned-model solves that fell back in a row, and solves the model still sits out.
+    int mlp_fail_streak = 0, mlp_rest = 0;
     // Density staging (grow-only).
     DevBuf dens_rho, dens_w;
 
@@ -92,14 +98,15 @@ constexpr int kMlpParams = 3 * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden
 
 // Q[:, j] = s(G, sel_j) R[:, sel_j] with the network's scaling (device pointers).
 void precondition_mlp(const double2* r, const int* sel, int nsel, const double* lam, const double* ekin,
-                      const double* rnorm, const double* diag, const float* prm, double energy_scale, double2* q,
-                      size_t ld, int n, hipStream_t s);
-// Training samples (SoA [6][cap] floats: x1 x2 x3 kx target weight) of `cols` residual/correction
-// column pairs, every `stride`-th coefficient, written from index base; returns how many.
-// bandsum: 3 * cols doubles of device scratch.
+                      const double* rnorm, const double* diag, const float* prm, double energy_scale, int base,
+                      double2* q, size_t ld, int n, hipStream_t s);
+// Training samples (SoA [6][cap] floats: x1 x2 x3 base-factor target weight) of `cols`
+// residual/correction column pairs for base preconditioner pbase, every `stride`-th coefficient,
+// written from index base; returns how many.  bandsum: 3 * cols doubles of device scratch.
 size_t mlp_write_samples(const double2* e, const double2* r, const double* lam, const double* ekin,
                          const double* rnorm, const double* diag, size_t ld, int n, int cols, int stride,
-                         double energy_scale, double* bandsum, float* out, size_t cap, size_t base, hipStream_t s);
+                         double energy_scale, int pbase, double* bandsum, float* out, size_t cap, size_t base,
+                         hipStream_t s);
 
 // Device state of one training run: samples (SoA [6][cap]), parameters, Adam moments, gradient.
 struct MlpTrainer {
@@ -126,8 +133,9 @@ void cg_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts&
 // Ritz vectors [npw x cols] (FP64, device) the last eig_solve on h left, or nullptr.
 const double2* eig_solution(const Ham& h, uint64_t* cols);
 // Training samples from the snapshots the last solve recorded (opts.collect_snapshots) against
-// its converged Ritz vectors; host output SoA [6][cap]; returns the count written.
-size_t eig_training_samples(Ham& h, double energy_scale, int stride, float* out_host, size_t cap);
+// its converged Ritz vectors, relative to base preconditioner `base`; host output SoA [6][cap];
+// returns the count written.
+size_t eig_training_samples(Ham& h, double energy_scale, int base, int stride, float* out_host, size_t cap);
 
 void ham_build(Ham& h, const dzhip_ham_desc& d);
 void ham_set_local(Ham& h, const double* local_r);

@@ -1,8 +1,10 @@
 // Learned diagonal preconditioner for the Davidson residuals: a 3 -> 32 -> 32 -> 1 SiLU network
-// (FP32) that rescales the Teter-Payne-Allan diagonal per coefficient,
-//     s(G, b) = K_tpa(|k+G|^2 / T_b) * exp(z),   z = MLP(|k+G|^2/2 / e_s, lambda_b/2 / e_s, log10|r_b| / 4),
-// with T_b = <x_b|T|x_b> and e_s the model's feature energy scale (Ha).  The reference has no
-// learned preconditioner (it uses the kinetic shift of src/features/linalg/lobpcg/common.zig:214-226).
+// (FP32) that rescales a base diagonal preconditioner per coefficient,
+//     s(G, b) = B(G, b) * exp(z),   z = MLP(|k+G|^2/2 / e_s, lambda_b/2 / e_s, log10|r_b| / 4),
+// with e_s the model's feature energy scale (Ha) and B either the reference's kinetic shift
+// 1/(|k+G|^2 - lambda_b) (src/features/linalg/lobpcg/common.zig:214-226, 1e-8 clamp) or the
+// Teter-Payne-Allan K(|k+G|^2 / T_b), T_b = <x_b|T|x_b>.  The reference has no learned
+// preconditioner.
 //
 // Kernels:
 //   k_precondition_mlp  inference, one thread per coefficient of one selected band; the weights
@@ -36,6 +38,15 @@ __device__ __forceinline__ double tpa(double x) {
     const double p = 27.0 + x * (18.0 + x * (12.0 + x * 8.0));
     return p / (p + 16.0 * x * x * x * x);
 }
+// Base factor B(G, b) for diag = |k+G|^2: kinetic shift (base DZHIP_PRECOND_KINETIC) or TPA.
+__device__ __forceinline__ double base_factor(int base, double diag, double lam, double ekin) {
+    if (base == DZHIP_PRECOND_KINETIC) {
+        double den = diag - lam;
+        if (fabs(den) < 1e-8) den = den >= 0.0 ? 1e-8 : -1e-8;
+        return 1.0 / den;
+    }
+    return tpa(diag / ekin);
+}
 
 constexpr int kInferThreads = 256;
 
@@ -43,7 +54,7 @@ constexpr int kInferThreads = 256;
 __global__ __launch_bounds__(kInferThreads) void k_precondition_mlp(
     const double2* __restrict__ r, const int* __restrict__ sel, const double* __restrict__ lam,
     const double* __restrict__ ekin, const double* __restrict__ rnorm, const double* __restrict__ diag,
-    const float* __restrict__ prm, float inv_es, double2* __restrict__ q, size_t ld, int n) {
+    const float* __restrict__ prm, float inv_es, int base, double2* __restrict__ q, size_t ld, int n) {
     __shared__ float w2[H * H];
     __shared__ float b2[H], w3[H], w1x[H], c1[H];
     const int j = blockIdx.y;
@@ -74,24 +85,24 @@ __global__ __launch_bounds__(kInferThreads) void k_precondition_mlp(
         for (int k = 0; k < H; ++k) u = fmaf(w2[o * H + k], h1[k], u);
         z = fmaf(w3[o], silu(u), z);
     }
-    const double f = tpa(d / ekin[b]) * (double)__expf(fminf(z, 30.0f));
+    const double f = base_factor(base, d, lam[b], ekin[b]) * (double)__expf(fminf(z, 30.0f));
     const double2 v = r[(size_t)b * ld + i];
     q[(size_t)j * ld + i] = make_double2(v.x * f, v.y * f);
 }
 
 // Per (snapshot i, band b), with e the correction and r the residual of that band:
-// out[0] = <K r, e> (real part), out[1] = |K r|^2, out[2] = |e|^2; K = K_tpa(diag / ekin_b).
+// out[0] = <K r, e> (real part), out[1] = |K r|^2, out[2] = |e|^2; K = B(G, b), the base factor.
 __global__ void k_sample_band(const double2* __restrict__ e, const double2* __restrict__ r,
-                              const double* __restrict__ ekin, const double* __restrict__ diag, size_t ld, int n,
-                              double* __restrict__ out) {
+                              const double* __restrict__ lam, const double* __restrict__ ekin,
+                              const double* __restrict__ diag, size_t ld, int n, int base, double* __restrict__ out) {
     __shared__ double red[3][256];
     const int col = blockIdx.x;
     const double2* ec = e + (size_t)col * ld;
     const double2* rc = r + (size_t)col * ld;
-    const double ek = ekin[col];
+    const double ek = ekin[col], lb = lam[col];
     double a = 0.0, kk = 0.0, ee = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double k = tpa(diag[i] / ek);
+        const double k = base_factor(base, diag[i], lb, ek);
         const double2 rv = rc[i], ev = ec[i];
         a += k * (rv.x * ev.x + rv.y * ev.y);
         kk += k * k * (rv.x * rv.x + rv.y * rv.y);
@@ -110,14 +121,15 @@ __global__ void k_sample_band(const double2* __restrict__ e, const double2* __re
 }
 
 // Samples for coefficients i = 0, stride, 2 stride, ... of each (snapshot, band) column:
-// features x1..x3, kx = diag/ekin, target t = Re(e conj r) / (alpha |r|^2) and weight
-// w = alpha^2 |r|^2 / |e|^2 with alpha = <K r, e> / |K r|^2 (the best TPA scale of the band);
-// columns with alpha <= 0 or vanishing norms get weight 0.  Output SoA [6][cap].
+// features x1..x3, the base factor B(G, b), target t = Re(e conj r) / (alpha |r|^2) and weight
+// w = alpha^2 |r|^2 / |e|^2 with alpha = <K r, e> / |K r|^2 (the best scale of the base
+// preconditioner for the band); columns with alpha <= 0 or vanishing norms get weight 0.
+// Output SoA [6][cap].
 __global__ void k_sample_write(const double2* __restrict__ e, const double2* __restrict__ r,
                                const double* __restrict__ lam, const double* __restrict__ ekin,
                                const double* __restrict__ rnorm, const double* __restrict__ bandsum,
                                const double* __restrict__ diag, size_t ld, int n, int stride, int per_col,
-                               double inv_es, float* __restrict__ out, size_t cap, size_t base) {
+                               double inv_es, int pbase, float* __restrict__ out, size_t cap, size_t base) {
     const int col = blockIdx.y;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= per_col) return;
@@ -134,7 +146,7 @@ __global__ void k_sample_write(const double2* __restrict__ e, const double2* __r
     out[0 * cap + o] = (float)(0.5 * diag[i] * inv_es);
     out[1 * cap + o] = (float)(0.5 * lam[col] * inv_es);
     out[2 * cap + o] = (float)(log10(fmax(rnorm[col], 1e-30)) * 0.25);
-    out[3 * cap + o] = (float)(diag[i] / ekin[col]);
+    out[3 * cap + o] = (float)base_factor(pbase, diag[i], lam[col], ekin[col]);
     out[4 * cap + o] = (float)tgt;
     out[5 * cap + o] = (float)wgt;
 }
@@ -172,7 +184,7 @@ __global__ __launch_bounds__(kGradThreads) void k_mlp_grad(
         x1 = smp[s];
         x2 = smp[cap + s];
         x3 = smp[2 * cap + s];
-        const float kx = smp[3 * cap + s], tg = smp[4 * cap + s], wg = smp[5 * cap + s];
+        const float base = smp[3 * cap + s], tg = smp[4 * cap + s], wg = smp[5 * cap + s];
         float h1[H];
 #pragma unroll
         for (int k = 0; k < H; ++k) {
@@ -192,8 +204,6 @@ __global__ __launch_bounds__(kGradThreads) void k_mlp_grad(
             d2r[o] = dsilu(u);
             z = fmaf(prm[OW3 + o], h2, z);
         }
-        const double p = 27.0 + kx * (18.0 + kx * (12.0 + kx * 8.0));
-        const float base = (float)(p / (p + 16.0 * (double)kx * kx * kx * kx));
         const float sv = base * __expf(fminf(z, 30.0f));
         const float diff = sv - tg;
         loss = wg * diff * diff * inv_count;
@@ -272,24 +282,27 @@ size_t grad_lds_bytes() { return sizeof(float) * (4 * kGradThreads * kP + 4 * kG
 }  // namespace
 
 void precondition_mlp(const double2* r, const int* sel, int nsel, const double* lam, const double* ekin,
-                      const double* rnorm, const double* diag, const float* prm, double energy_scale, double2* q,
-                      size_t ld, int n, hipStream_t s) {
+                      const double* rnorm, const double* diag, const float* prm, double energy_scale, int base,
+                      double2* q, size_t ld, int n, hipStream_t s) {
     if (nsel <= 0) return;
     hipLaunchKernelGGL(k_precondition_mlp, dim3((unsigned)((n + kInferThreads - 1) / kInferThreads), (unsigned)nsel),
-                       dim3(kInferThreads), 0, s, r, sel, lam, ekin, rnorm, diag, prm, (float)(1.0 / energy_scale), q,
-                       ld, n);
+                       dim3(kInferThreads), 0, s, r, sel, lam, ekin, rnorm, diag, prm, (float)(1.0 / energy_scale), base,
+                       q, ld, n);
     DZ_HIP(hipGetLastError());
 }
 
 size_t mlp_write_samples(const double2* e, const double2* r, const double* lam, const double* ekin,
                          const double* rnorm, const double* diag, size_t ld, int n, int cols, int stride,
-                         double energy_scale, double* bandsum, float* out, size_t cap, size_t base, hipStream_t s) {
+                         double energy_scale, int pbase, double* bandsum, float* out, size_t cap, size_t base,
+                         hipStream_t s) {
     if (cols <= 0) return 0;
-    hipLaunchKernelGGL(k_sample_band, dim3((unsigned)cols), dim3(256), 0, s, e, r, ekin, diag, ld, n, bandsum);
+    hipLaunchKernelGGL(k_sample_band, dim3((unsigned)cols), dim3(256), 0, s, e, r, lam, ekin, diag, ld, n, pbase,
+                       bandsum);
     DZ_HIP(hipGetLastError());
     const int per_col = (n + stride - 1) / stride;
     hipLaunchKernelGGL(k_sample_write, dim3((unsigned)((per_col + 255) / 256), (unsigned)cols), dim3(256), 0, s, e, r,
-                       lam, ekin, rnorm, bandsum, diag, ld, n, stride, per_col, 1.0 / energy_scale, out, cap, base);
+                       lam, ekin, rnorm, bandsum, diag, ld, n, stride, per_col, 1.0 / energy_scale, pbase, out, cap,
+                       base);
     DZ_HIP(hipGetLastError());
     return std::min<size_t>((size_t)cols * per_col, cap > base ? cap - base : 0);
 }

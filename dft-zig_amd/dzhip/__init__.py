@@ -147,8 +147,8 @@ def _load():
         "dzhip_eig_solve": ([vp, dp, u64, u64, ct.POINTER(EigOpts), dp, u64, dp, dp, ct.POINTER(EigStats)], i32),
         "dzhip_density_accumulate": ([vp, dp, u64, u64, dp, dp, u64, i32], i32),
         "dzhip_density_accumulate_solution": ([vp, u64, dp, dp, u64, i32], i32),
-        "dzhip_ham_set_precond_model": ([vp, fp, u64, ct.c_double], i32),
-        "dzhip_ham_training_samples": ([vp, ct.c_double, u64, fp, u64, ct.POINTER(ct.c_uint64)], i32),
+        "dzhip_ham_set_precond_model": ([vp, fp, u64, ct.c_double, i32], i32),
+        "dzhip_ham_training_samples": ([vp, ct.c_double, i32, u64, fp, u64, ct.POINTER(ct.c_uint64)], i32),
         "dzhip_mlp_loss_grad": ([vp, fp, u64, fp, dp, dp], i32),
         "dzhip_mlp_train": ([vp, fp, u64, fp, u64, u64, ct.c_double, u64, dp], i32),
         "dzhip_cg_solve": ([vp, dp, u64, u64, ct.POINTER(EigOpts), dp, u64, dp, dp, ct.POINTER(EigStats)], i32),
@@ -465,18 +465,20 @@ class Hamiltonian:
                                  _dp(vals), vp, ct.byref(st)))
         return vals, vecs, st.as_dict()
 
-    def set_precond_model(self, params, energy_scale: float):
-        """Install the learned preconditioner's parameters (MLP_PARAMS floats)."""
+    def set_precond_model(self, params, energy_scale: float, base: int = PRECOND_TPA):
+        """Install the learned preconditioner's parameters (MLP_PARAMS floats) and the base
+        preconditioner they rescale (PRECOND_KINETIC or PRECOND_TPA)."""
         p = np.ascontiguousarray(params, dtype=np.float32)
-        check(L().dzhip_ham_set_precond_model(self.handle, _fp(p), p.shape[0], float(energy_scale)))
+        check(L().dzhip_ham_set_precond_model(self.handle, _fp(p), p.shape[0], float(energy_scale), int(base)))
 
-    def training_samples(self, energy_scale: float, stride: int = 1, capacity: int = 1 << 20):
+    def training_samples(self, energy_scale: float, stride: int = 1, capacity: int = 1 << 20,
+                         base: int = PRECOND_TPA):
         """Residual/correction training samples of the last solve (collect_snapshots > 0), as a
-        (6, count) float32 array: x1, x2, x3, kx, target, weight."""
+        (6, count) float32 array: x1, x2, x3, base factor B, target, weight."""
         out = np.zeros((6, capacity), dtype=np.float32)
         cnt = ct.c_uint64(0)
-        check(L().dzhip_ham_training_samples(self.handle, float(energy_scale), stride, _fp(out), capacity,
-                                             ct.byref(cnt)))
+        check(L().dzhip_ham_training_samples(self.handle, float(energy_scale), int(base), stride, _fp(out),
+                                             capacity, ct.byref(cnt)))
         return np.ascontiguousarray(out[:, :cnt.value])
 
     def density_solution(self, weights, rho=None):

@@ -129,14 +129,22 @@ AL_GTH = {"z": 3.0, "rloc": 0.45, "c1": -8.491351, "r_s": 0.460104, "h_s": 5.088
           "h_p": 2.193438}
 
 
-def aluminium(ecut_ry=30.0, kmesh=(4, 4, 4), nbands=12, smearing_ry=0.01, grid=None) -> ScfSystem:
+def aluminium(ecut_ry=30.0, kmesh=(4, 4, 4), nbands=12, smearing_ry=0.01, grid=None, displace=None) -> ScfSystem:
     """fcc Al in the 4-atom cubic cell (a = 4.05 A), GTH-form local part and s/p Gaussian
     projectors (Goedecker-Teter-Hutter Al LDA parameters), Fermi-Dirac smearing; k-points from an
-    unshifted mesh reduced by time reversal (4x4x4 -> 36 points with uneven plane-wave counts)."""
+    unshifted mesh reduced by time reversal (4x4x4 -> 36 points with uneven plane-wave counts).
+
+    `displace` (bohr, 3-vector) moves the second atom off its site.  In the perfect crystal,
+    partially occupied levels are symmetry-degenerate, and the density then depends at the 1e-9
+    level on how an eigensolver rotates each degenerate set (occupations of numerically split
+    copies differ by ~1e-7), which leaves any SCF at a noise floor of drho ~ 1e-9.  A small
+    displacement lifts the degeneracies so the fixed point is defined to round-off."""
     a = 4.05 * tables.ANGSTROM_TO_BOHR
     cell = np.diag([a, a, a])
     recip = tables.reciprocal(cell)
     pos = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]]) * a
+    if displace is not None:
+        pos[1] += np.asarray(displace, dtype=np.float64)
     g = AL_GTH
     sp = SpeciesDesc([0, 1], [g["r_s"], g["r_p"]], np.diag([2.0 * g["h_s"], 2.0 * g["h_p"]]))
     kpts, kw = mp_mesh_time_reversal(recip, kmesh)
@@ -152,6 +160,16 @@ def si8_small(ecut_ry=8.0, nbands=20):
     recip = tables.reciprocal(np.diag([a, a, a]))
     kpts = [np.zeros(3), np.array([0.25, 0.25, 0.25]) @ recip]
     return silicon((1, 1, 1), ecut_ry, nbands=nbands, kpoints=(kpts, [0.5, 0.5]))
+
+
+AL_SMALL_DISPLACEMENT = (0.11, 0.07, 0.05)  # bohr
+
+
+def al4_small(ecut_ry=8.0, nbands=10):
+    """Small smearing parity case: the 4-atom Al cell at 8 Ry (15^3 grid), 2x2x2 mesh reduced by
+    time reversal (8 k-points with different plane-wave counts), second atom displaced by
+    AL_SMALL_DISPLACEMENT so no partially occupied level is degenerate."""
+    return aluminium(ecut_ry=ecut_ry, kmesh=(2, 2, 2), nbands=nbands, displace=AL_SMALL_DISPLACEMENT)
 
 
 def si64(ecut_ry=30.0, nbands=160):
@@ -454,7 +472,17 @@ class ScfDriver:
             self.grid = ScfGrid(self.ctxs[0], s.grid, s.recip, s.volume, pulay_history)
             self.grid.set_ion(self.v_ion)
         self.pool = ThreadPoolExecutor(len(self.hams)) if self.concurrent else None
+        # Base preconditioner of the installed learned model (set_precond_model) and an optional
+        # callback hook(k, ham, stats) after every k-point solve.
+        self.mlp_base = PRECOND_TPA
+        self.solve_hook = None
         self.setup_s = time.perf_counter() - t0
+
+    def set_precond_model(self, params, energy_scale, base=PRECOND_TPA):
+        """Install a learned preconditioner model on every k-point's context."""
+        for h in self.hams:
+            h.set_precond_model(params, energy_scale, base)
+        self.mlp_base = base
 
     def _solve_k(self, k, local_r, nb, opts):
         ham = self.hams[k]
@@ -463,6 +491,8 @@ class ScfDriver:
         else:
             ham.set_local(local_r)
         vals, _, st = ham.eig_solve(nb, want_vectors=False, reuse=True, **opts)
+        if self.solve_hook is not None:
+            self.solve_hook(k, ham, st)
         return vals, st
 
     def _solve_all(self, local_r, opts):
@@ -526,7 +556,7 @@ class ScfDriver:
             pulay_history=8, pulay_start=4, eig_tol=1e-4, eig_max_iter=20, warmup_steps=2, warmup_tol=1e-3,
             warmup_max_iter=10, mixed_switch_tol=0.0, final_tol=None, energy_tol_ry=None, eig_tol_ratio=None,
             eig_tol_min=1e-8, progress=None, diemac=1.0, dielng=1.0, max_subspace=0, init_diagonal=False,
-            nconv=0, start="uniform"):
+            nconv=0, start="uniform", collect_snapshots=0):
         """SCF from the uniform start density nelec/Omega (or start="atomic": a superposition of
         atomic valence charges).  Stops when the RMS density change is
         below conv, or (with energy_tol_ry) when two successive total energies after the warm-up
@@ -534,6 +564,8 @@ class ScfDriver:
 
         eig_tol_ratio (None = the reference's fixed eig_tol) tightens the eigensolver tolerance with
         the SCF: tol = min(eig_tol, max(eig_tol_min, eig_tol_ratio * drho of the previous step)).
+        collect_snapshots > 0 makes every solve record that many snapshots for training samples
+        (read them in solve_hook).
         A fixed 1e-4 Ry residual leaves band-energy errors of order N_b r^2 / gap, about 1e-5 Ry
         for 128 bands across a 0.08 Ry gap, which a converged total energy cannot carry.
         progress, if given, is called with each ScfRecord."""
@@ -544,12 +576,13 @@ class ScfDriver:
             tol = warmup_tol if warm else eig_tol
             if not warm and eig_tol_ratio is not None and drho_prev is not None:
                 tol = min(eig_tol, max(eig_tol_min, eig_tol_ratio * drho_prev))
-            # The learned preconditioner's fallback compares against TPA's convergence rate on
-            # each context, so the warm-up solves run TPA and measure it.
-            pc = PRECOND_TPA if (preconditioner == PRECOND_MLP and warm) else preconditioner
+            # The learned preconditioner's fallback compares against its base preconditioner's
+            # convergence rate on each context, so the warm-up solves run the base and measure it.
+            pc = self.mlp_base if (preconditioner == PRECOND_MLP and warm) else preconditioner
             return dict(tol=tol, max_iter=warmup_max_iter if warm else eig_max_iter, precision=precision,
                         preconditioner=pc, mixed_switch_tol=mixed_switch_tol,
-                        max_subspace=max_subspace, init_diagonal=init_diagonal and it == 0, nconv=nconv)
+                        max_subspace=max_subspace, init_diagonal=init_diagonal and it == 0, nconv=nconv,
+                        collect_snapshots=collect_snapshots)
 
         rho0 = atomic_density(s, self.g, self.g2) if start == "atomic" else None
         if self.grid is not None:

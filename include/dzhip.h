@@ -165,10 +165,11 @@ int dzhip_ham_phase_times(dzhip_ham* ham, double* out_ms, uint64_t n_out, uint64
  * tolerances are met in FP32). */
 #define DZHIP_PRECOND_KINETIC 0 /* r / (|k+G|^2 - lambda), |den| >= 1e-8 (common.zig:214-226) */
 #define DZHIP_PRECOND_TPA 1     /* Teter-Payne-Allan K(x) r, x = |k+G|^2 / <x_b|T|x_b> */
-#define DZHIP_PRECOND_MLP 2     /* learned: K_tpa(x) exp(MLP(features)) r, model set with
-                                   dzhip_ham_set_precond_model; falls back to TPA within a solve
-                                   when the residuals fall more slowly than TPA's rate on this
-                                   context (stats.precond_fallback) */
+#define DZHIP_PRECOND_MLP 2     /* learned: B(G, b) exp(MLP(features)) r with B the model's base
+                                   (kinetic shift or TPA), model set with
+                                   dzhip_ham_set_precond_model; falls back to the base within a
+                                   solve when the residuals fall more slowly than the base's rate
+                                   on this context (stats.precond_fallback) */
 
 typedef struct dzhip_eig_opts {
     uint64_t max_iter;       /* Options.max_iter (reference default 40) */
@@ -200,7 +201,9 @@ typedef struct dzhip_eig_stats {
     double max_residual;
     double initial_residual;        /* largest residual of the start block */
     uint64_t precond_fallback;      /* DZHIP_PRECOND_MLP: iteration at which the solve fell back
-                                       to TPA (0: never) */
+                                       to the model's base preconditioner (0: never; 1 also when
+                                       the model sits out this solve: after two solves in a row
+                                       that fell back it sits out the next 8 on this context) */
     /* GPU time (ms, HIP events on the context's stream) per section of the solve:
      * Rayleigh-Ritz (T copy + zheevd), Ritz vectors + residuals + norms, preconditioner,
      * orthonormalisation (CGS2 + Cholesky QR), H apply of new vectors, T extension/restart.
@@ -246,22 +249,26 @@ int dzhip_density_accumulate_solution(dzhip_ham* ham, uint64_t nbands, const dou
 /* ------------------------------------------------------------------ learned preconditioner
  * No counterpart in the reference (its Davidson uses the kinetic-shift diagonal,
  * src/features/linalg/lobpcg/common.zig:214-226).  A 3 -> 32 -> 32 -> 1 SiLU network (FP32)
- * rescales the Teter-Payne-Allan diagonal per coefficient:
- *   s(G, b) = K_tpa(|k+G|^2 / <x_b|T|x_b>) exp(z),
+ * rescales a base diagonal preconditioner B per coefficient:
+ *   s(G, b) = B(G, b) exp(z),
  *   z = MLP(|k+G|^2/2 / e_s, lambda_b/2 / e_s, log10(|r_b|) / 4)     (energies in Ha)
+ *   B = 1 / (|k+G|^2 - lambda_b), |den| >= 1e-8            base DZHIP_PRECOND_KINETIC
+ *   B = K_tpa(|k+G|^2 / <x_b|T|x_b>)                         base DZHIP_PRECOND_TPA
  * Parameters: DZHIP_MLP_PARAMS floats, W1[32][3], b1[32], W2[32][32], b2[32], w3[32], b3.
- * Samples: structure of arrays [6][n] floats: x1, x2, x3, kx = |k+G|^2/<T>, target, weight;
+ * Samples: structure of arrays [6][n] floats: x1, x2, x3, B, target, weight;
  * loss = sum_i w_i (s_i - t_i)^2 / n over a minibatch of n samples. */
 #define DZHIP_MLP_PARAMS 1217
 
-int dzhip_ham_set_precond_model(dzhip_ham* ham, const float* params, uint64_t n_params, double energy_scale);
+int dzhip_ham_set_precond_model(dzhip_ham* ham, const float* params, uint64_t n_params, double energy_scale,
+                                int base);
 
 /* Residual/correction pairs of the snapshots the last solve recorded (collect_snapshots > 0,
  * FP64, converged): correction e = X X^H x_i - x_i against the converged Ritz vectors X,
  * target t = Re(e conj r) / (alpha |r|^2) and weight alpha^2 |r|^2 / |e|^2 with alpha the
- * band's least-squares TPA scale; every stride-th coefficient.  Writes min(capacity, available)
- * samples to out ([6][capacity]) and their count.  Weight-0 samples mark unusable bands. */
-int dzhip_ham_training_samples(dzhip_ham* ham, double energy_scale, uint64_t stride, float* out,
+ * band's least-squares scale of base preconditioner `base` (DZHIP_PRECOND_KINETIC or
+ * DZHIP_PRECOND_TPA); every stride-th coefficient.  Writes min(capacity, available) samples to
+ * out ([6][capacity]) and their count.  Weight-0 samples mark unusable bands. */
+int dzhip_ham_training_samples(dzhip_ham* ham, double energy_scale, int base, uint64_t stride, float* out,
                                uint64_t capacity, uint64_t* count);
 
 /* Full-batch loss and gradient (FP64 out) of the network on n samples ([6][n]). */

@@ -7,14 +7,16 @@ gradients and optimiser step can be checked:
 
   features  x = (|k+G|^2/2 / e_s, lambda_b/2 / e_s, log10 |r_b|)    e_s = feature energy scale (Ha)
   network   z = w3 . silu(W2 silu(W1 x + b1) + b2) + b3             3 -> 32 -> 32 -> 1, silu(u) = u/(1+e^-u)
-  scaling   s = K_tpa(|k+G|^2 / T_b) * exp(z)                       K_tpa: Teter-Payne-Allan polynomial
+  scaling   s = B * exp(z)                                          B: base preconditioner factor,
+            B = K_tpa(|k+G|^2 / T_b)  (Teter-Payne-Allan polynomial) or
+            B = 1 / (|k+G|^2 - lambda_b), |den| >= 1e-8  (the reference's kinetic shift)
   loss      L = sum_i w_i (s_i - t_i)^2 / sum_i w_i                 weighted least squares on samples i
   Adam      m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2, p -= lr mhat/(sqrt(vhat)+eps)
 
 Parameter layout (1217 floats): W1[32][3], b1[32], W2[32][32], b2[32], w3[32], b3.
 
 Pinned by: finite differences of the loss (central, FP64) for the gradient; z = 0 (zero last
-layer) reducing s to the TPA scaling exactly; silu closed forms.
+layer) reducing s to the base factor exactly; silu and TPA closed forms.
 """
 from __future__ import annotations
 
@@ -28,6 +30,13 @@ def tpa(x):
     """Teter-Payne-Allan K(x) = (27 + 18x + 12x^2 + 8x^3) / (27 + 18x + 12x^2 + 8x^3 + 16x^4)."""
     p = 27.0 + 18.0 * x + 12.0 * x * x + 8.0 * x ** 3
     return p / (p + 16.0 * x ** 4)
+
+
+def kinetic_shift(diag, lam):
+    """1 / (diag - lambda) with |den| clamped to >= 1e-8, sign kept (common.zig:214-226)."""
+    den = np.asarray(diag, dtype=np.float64) - lam
+    den = np.where(np.abs(den) < 1e-8, np.where(den >= 0.0, 1e-8, -1e-8), den)
+    return 1.0 / den
 
 
 def unpack(p):
@@ -72,18 +81,17 @@ def forward(p, x):
     return z, (x, u1, h1, u2, h2)
 
 
-def scaling(p, x, kx):
-    """s = K_tpa(kx) exp(z(x)); kx = |k+G|^2 / T_b per sample."""
+def scaling(p, x, base):
+    """s = B exp(z(x)) for the base factors B of the samples."""
     z, _ = forward(p, x)
-    return tpa(kx) * np.exp(z)
+    return np.asarray(base, dtype=np.float64) * np.exp(z)
 
 
-def loss_and_grad(p, x, kx, t, w):
-    """Weighted least-squares loss of s against targets t and its gradient."""
+def loss_and_grad(p, x, base, t, w):
+    """Weighted least-squares loss of s = B exp(z) against targets t and its gradient."""
     w1, b1, w2, b2, w3, b3 = unpack(p)
     z, (x_, u1, h1, u2, h2) = forward(p, x)
-    base = tpa(kx)
-    s = base * np.exp(z)
+    s = np.asarray(base, dtype=np.float64) * np.exp(z)
     wsum = float(np.sum(w))
     diff = s - t
     loss = float(np.sum(w * diff * diff)) / wsum

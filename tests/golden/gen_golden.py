@@ -71,12 +71,12 @@ def scf_file():
 
 
 def scf_al_file():
-    """Oracle SCF of the small Al case with Fermi-Dirac smearing (0.01 Ry)."""
+    """Oracle SCF of the small Al case (displaced atom) with Fermi-Dirac smearing (0.01 Ry)."""
     from dzhip import scf as dscf
     from oracle import scf as oscf
 
-    system = dscf.aluminium(ecut_ry=8.0, kmesh=(2, 2, 2), nbands=10)
-    res = oscf.run_scf(system, n_iter=60, beta=0.3, conv=1e-10, pulay_history=8, pulay_start=4)
+    system = dscf.al4_small()
+    res = oscf.run_scf(system, n_iter=60, beta=0.3, conv=1e-12, pulay_history=8, pulay_start=4)
     assert res.converged
     np.savez_compressed(HERE / "scf_al4_small.npz", energies=np.array(res.energies), drho=np.array(res.drho),
                         eigenvalues=np.array(res.eigenvalues), rho=res.rho)

@@ -1,14 +1,15 @@
 """GPU tests of the learned preconditioner: the device forward/backward against the FP64 CPU
 oracle (oracle/mlp.py), Adam against the oracle's update, the zero-output-layer model reducing
-exactly to Teter-Payne-Allan inside the eigensolver, sample collection, and what the trained
-network gains over TPA on problems it was not trained on.
+exactly to its base preconditioner (TPA or the kinetic shift) inside the eigensolver, sample
+collection from cold solves and from an SCF's warm-started solves, and what the trained network
+gains over TPA on cold solves it was not trained on.
 
 Tolerances: gradient 1e-4 relative (2-norm) for FP32 arithmetic against FP64, loss 1e-5 relative.
 """
 import numpy as np
 import pytest
 
-from dzhip import FP64, PRECOND_MLP, PRECOND_TPA, precond, workloads
+from dzhip import FP64, PRECOND_KINETIC, PRECOND_MLP, PRECOND_TPA, precond, workloads
 from dzhip import scf as dscf
 from oracle import mlp as omlp
 
@@ -20,10 +21,14 @@ def _random_samples(n, seed):
     x1 = rng.uniform(0.0, 1.5, n)
     x2 = rng.uniform(-0.1, 0.2, n)
     x3 = rng.uniform(-2.5, -0.25, n)
+    # Base factors of both kinds: TPA values in (0, 1] and kinetic shifts of either sign.
     kx = rng.uniform(0.0, 6.0, n)
-    t = omlp.tpa(kx) * np.exp(rng.normal(0.0, 0.3, n))
+    d = rng.uniform(0.0, 6.0, n)
+    lam = d - rng.choice([-1.0, 1.0], n) * rng.uniform(0.05, 3.0, n)
+    base = np.where(np.arange(n) % 2 == 0, omlp.tpa(kx), omlp.kinetic_shift(d, lam))
+    t = base * np.exp(rng.normal(0.0, 0.3, n))
     w = rng.uniform(0.1, 2.0, n)
-    return np.stack([x1, x2, x3, kx, t, w]).astype(np.float32)
+    return np.stack([x1, x2, x3, base, t, w]).astype(np.float32)
 
 
 def _random_params(seed):
@@ -62,13 +67,14 @@ def test_adam_first_step_matches_oracle(dz, ctx):
     assert hist.shape == (1,) and np.isfinite(hist[0])
 
 
-def test_zero_output_layer_is_exactly_tpa(dz, ctx):
+@pytest.mark.parametrize("base", [PRECOND_TPA, PRECOND_KINETIC])
+def test_zero_output_layer_is_exactly_the_base(dz, ctx, base):
     w = workloads.si8(nbands=1)
     ham = dz.Hamiltonian.from_workload(ctx, w)
     try:
         p = precond.init_params(7)
-        ham.set_precond_model(p, precond.ENERGY_SCALE)
-        v_t, _, st_t = ham.eig_solve(12, tol=1e-8, max_iter=200, precision=FP64, preconditioner=PRECOND_TPA,
+        ham.set_precond_model(p, precond.ENERGY_SCALE, base)
+        v_t, _, st_t = ham.eig_solve(12, tol=1e-8, max_iter=200, precision=FP64, preconditioner=base,
                                      want_vectors=False)
         v_m, _, st_m = ham.eig_solve(12, tol=1e-8, max_iter=200, precision=FP64, preconditioner=PRECOND_MLP,
                                      want_vectors=False)
@@ -133,3 +139,25 @@ def test_samples_train_and_gain(dz, ctx):
         assert res[(i, "mlp")][1]["converged"]
         assert np.max(np.abs(res[(i, "mlp")][0] - res[(i, "tpa")][0])) < 2e-8
     assert tot["mlp"] <= 1.1 * tot["tpa"]
+
+
+def test_scf_samples_kinetic_base(dz, ctx):
+    """Samples from the warm-started solves of a short FP64 SCF (kinetic-shift base): finite,
+    positive weights, base factors equal to 1/(|k+G|^2 - lambda) with the 1e-8 clamp, and
+    training lowers the loss from the zero-output start (which is the kinetic shift itself)."""
+    s = dscf.si8_small()
+    smp = precond.collect_scf(s, base=PRECOND_KINETIC, steps=5, snapshots=8, stride=2)
+    print("scf samples", smp.shape)
+    assert smp.shape[0] == 6 and smp.shape[1] > 1000
+    assert np.all(np.isfinite(smp)) and np.all(smp[5] > 0)
+    # Row 0 is |k+G|^2/2/e_s and row 1 lambda/2/e_s, so |k+G|^2 - lambda = 2 e_s (x1 - x2).
+    d = 2.0 * precond.ENERGY_SCALE * (smp[0].astype(np.float64) - smp[1].astype(np.float64))
+    # (float32 features: |d| is known to ~2e-6, so compare where |d| > 0.05 at 1e-3.)
+    ok = np.abs(d) > 0.05
+    assert np.count_nonzero(ok) > 0.9 * d.size
+    np.testing.assert_allclose(smp[3][ok], 1.0 / d[ok], rtol=1e-3)
+    p0 = precond.init_params(1)
+    l0, _ = ctx.mlp_loss_grad(precond.normalise_weights(smp), p0)
+    params, hist = precond.train(ctx, smp, epochs=20, init=p0)
+    print("loss kinetic", l0, "trained", hist[-1])
+    assert hist[-1] < l0

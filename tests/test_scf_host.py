@@ -183,8 +183,8 @@ def test_time_reversal_mesh():
 
 def test_oracle_smearing_scf_golden():
     gold = np.load(GOLD.parent / "scf_al4_small.npz")
-    s = dscf.aluminium(ecut_ry=8.0, kmesh=(2, 2, 2), nbands=10)
-    res = oscf.run_scf(s, n_iter=60, beta=0.3, conv=1e-10, pulay_history=8, pulay_start=4)
+    s = dscf.al4_small()
+    res = oscf.run_scf(s, n_iter=60, beta=0.3, conv=1e-12, pulay_history=8, pulay_start=4)
     assert res.converged and len(res.energies) == len(gold["energies"])
     assert np.max(np.abs(np.array(res.energies) - gold["energies"])) < 1e-10
     assert abs(np.sum(res.rho) * s.volume / res.rho.size - s.nelec) < 1e-8
