@@ -706,6 +706,7 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
     // A learned model that fell back in two solves in a row on this context sits out the next
     // kMlpRest solves (the base preconditioner runs; precond_fallback reports iteration 1).
     constexpr int kMlpRest = 8;
+    constexpr size_t kCurveMax = 64;  // iterations of the base preconditioner's progress record
     if (precond == DZHIP_PRECOND_MLP && h.mlp_rest > 0) {
         h.mlp_rest--;
         precond = h.mlp_base;
@@ -736,15 +737,10 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
         S.st.max_residual = max_res;
         if (iter == 0) S.st.initial_residual = max_res;
         {
-            // Convergence progress: mean log10 residual of the bands still above tol.
+            // Convergence progress: mean over the bands that must converge of log10 max(r_b, tol).
             double sl = 0.0;
-            int cnt = 0;
-            for (double r : rn)
-                if (r >= o.tol && r > 0.0) {
-                    sl += std::log10(r);
-                    ++cnt;
-                }
-            res_hist.push_back(cnt ? sl / cnt : std::log10(std::max(max_res, 1e-300)));
+            for (int b = 0; b < nconv; ++b) sl += std::log10(std::max(rn[b], std::max(o.tol, 1e-300)));
+            res_hist.push_back(sl / nconv);
         }
         if (w.snaps < nsnap && S.fp64_apply) {
             const size_t blk = n * nb;
@@ -763,14 +759,17 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
             w.snaps++;
         }
         // Learned preconditioner: fall back to its base preconditioner for the rest of the solve
-        // when, over the last three iterations, the unconverged bands' mean log residual fell
-        // more slowly than 80% of the rate solves with the base preconditioner achieved on this
-        // context (or by less than 0.05 decades per iteration before any such rate is known).
+        // when, from the fourth iteration on, the progress so far (drop of the mean log residual
+        // since the first iteration) trails the base's mean progress at the same iteration on
+        // this context by more than 0.25 decades, or (no base record for this iteration yet)
+        // when the last three iterations gained less than 0.05 decades per iteration.
         if (precond == DZHIP_PRECOND_MLP && res_hist.size() > 3) {
-            const double rate = (res_hist[res_hist.size() - 4] - res_hist.back()) / 3.0;
-            const double base_rate = h.precond_rate[h.mlp_base == DZHIP_PRECOND_TPA ? 1 : 0];
-            const double need = base_rate > 0.0 ? 0.8 * base_rate : 0.05;
-            if (rate < need) {
+            const size_t i = res_hist.size() - 1;
+            const auto& curve = h.precond_curve[h.mlp_base == DZHIP_PRECOND_TPA ? 1 : 0];
+            const double drop = res_hist.front() - res_hist.back();
+            const bool behind = i < curve.size() ? drop < curve[i] - 0.25
+                                                 : (res_hist[i - 3] - res_hist[i]) / 3.0 < 0.05;
+            if (behind) {
                 precond = h.mlp_base;
                 S.st.precond_fallback = iter + 1;
             }
@@ -847,13 +846,25 @@ void eig_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts
             h.mlp_fail_streak = 0;
         }
     }
-    // Decades of residual per iteration achieved by kinetic-shift and TPA solves on this context
-    // (running mean), the yardstick of the learned preconditioner's fallback.
+    // Mean progress per iteration index of kinetic-shift and TPA solves on this context (drop of
+    // the mean log residual since the first iteration), the yardstick of the learned
+    // preconditioner's fallback.  A converged solve's last value stands for the iterations it
+    // did not need.
     if ((o.preconditioner == DZHIP_PRECOND_KINETIC || o.preconditioner == DZHIP_PRECOND_TPA) &&
-        res_hist.size() >= 3) {
-        const double r = (res_hist.front() - res_hist.back()) / (double)(res_hist.size() - 1);
-        double& rate = h.precond_rate[o.preconditioner == DZHIP_PRECOND_TPA ? 1 : 0];
-        if (r > 0.0) rate = rate > 0.0 ? 0.5 * (rate + r) : r;
+        res_hist.size() >= 2) {
+        const int c = o.preconditioner == DZHIP_PRECOND_TPA ? 1 : 0;
+        auto& curve = h.precond_curve[c];
+        auto& cnt = h.precond_curve_n[c];
+        const size_t len = std::max(curve.size(), S.st.converged ? (size_t)kCurveMax : res_hist.size());
+        const size_t keep = std::min(len, (size_t)kCurveMax);
+        curve.resize(keep, 0.0);
+        cnt.resize(keep, 0);
+        for (size_t i = 0; i < keep; ++i) {
+            if (i >= res_hist.size() && !S.st.converged) break;
+            const double d = res_hist.front() - res_hist[std::min(i, res_hist.size() - 1)];
+            cnt[i]++;
+            curve[i] += (d - curve[i]) / cnt[i];
+        }
     }
     if (have_best) {
         std::memcpy(values, best.data(), nb * sizeof(double));

@@ -80,9 +80,10 @@ struct Ham {
     double mlp_energy_scale = 0.0;
     int mlp_base = DZHIP_PRECOND_TPA;
     bool has_mlp = false;
-    // Decades of residual per iteration that solves with the kinetic-shift [0] and TPA [1]
-    // preconditioners achieved on this context (running mean; 0 = none yet).
-    double precond_rate[2] = {0.0, 0.0};
+    // Mean drop of the log residual since the first iteration, per iteration index, of the solves
+    // with the kinetic-shift [0] and TPA [1] preconditioners on this context (with sample counts).
+    std::vector<double> precond_curve[2];
+    std::vector<int> precond_curve_n[2];
     // Learned-model solves that fell back in a row, and solves the model still sits out.
     int mlp_fail_streak = 0, mlp_rest = 0;
     // Density staging (grow-only).

@@ -168,8 +168,8 @@ int dzhip_ham_phase_times(dzhip_ham* ham, double* out_ms, uint64_t n_out, uint64
 #define DZHIP_PRECOND_MLP 2     /* learned: B(G, b) exp(MLP(features)) r with B the model's base
                                    (kinetic shift or TPA), model set with
                                    dzhip_ham_set_precond_model; falls back to the base within a
-                                   solve when the residuals fall more slowly than the base's rate
-                                   on this context (stats.precond_fallback) */
+                                   solve when its progress trails the base's mean progress at the
+                                   same iteration on this context (stats.precond_fallback) */
 
 typedef struct dzhip_eig_opts {
     uint64_t max_iter;       /* Options.max_iter (reference default 40) */
