@@ -621,6 +621,17 @@ int dzhip_scf_grid_set_dielectric(dzhip_scf_grid* grid, double diemac, double di
     });
 }
 
+int dzhip_scf_grid_set_xc(dzhip_scf_grid* grid, int functional) {
+    return guarded([&] {
+        DZ_REQUIRE(grid, DZHIP_ERR_STATE, "NULL grid");
+        DZ_REQUIRE(functional == DZHIP_XC_LDA_PZ || functional == DZHIP_XC_PBE, DZHIP_ERR_ARG,
+                   "functional must be DZHIP_XC_LDA_PZ or DZHIP_XC_PBE");
+        std::lock_guard<std::mutex> lk(grid->mu);
+        DeviceGuard g(grid->ctx->device);
+        scf_grid_set_xc(*grid, functional);
+    });
+}
+
 int dzhip_scf_grid_mix(dzhip_scf_grid* grid, double beta, int pulay) {
     return guarded([&] {
         DZ_REQUIRE(grid, DZHIP_ERR_STATE, "NULL grid");

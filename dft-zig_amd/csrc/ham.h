@@ -152,6 +152,7 @@ void scf_grid_local(ScfGrid& g);
 void scf_grid_add(ScfGrid& g, const double* rho_k);
 void scf_grid_hxc(ScfGrid& g, double out[4]);
 void scf_grid_mix(ScfGrid& g, double beta, int pulay);
+void scf_grid_set_xc(ScfGrid& g, int functional);  // DZHIP_XC_LDA_PZ or DZHIP_XC_PBE
 void ham_release(Ham& h);
 // Device apply; x/y hold C2T<T> elements, column-major with leading dimension npw.
 template <typename T>

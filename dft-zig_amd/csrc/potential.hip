@@ -1,9 +1,12 @@
-// Device-resident SCF grid state: density, Hartree + LDA-PZ potential, energies and potential
-// mixing (linear, then Pulay/DIIS), so an SCF iteration keeps rho(r) and V(r) in HBM.
+// Device-resident SCF grid state: density, Hartree + LDA-PZ or PBE potential, energies and
+// potential mixing (linear, then Pulay/DIIS), so an SCF iteration keeps rho(r) and V(r) in HBM.
 //
 // Follows (paths relative to the dft-zig source tree):
 //   Hartree V_H(G) = 8 pi rho(G) / G^2, G = 0 dropped              src/features/scf/potential.zig:34-44
 //   LDA-PZ energy density and potential (Ry)                       src/features/xc/xc.zig:195-210, 344-384
+//   PBE (Ry): exchange, correlation H(t^2, A) on PW92               src/features/xc/xc.zig:213-342
+//   GGA fields: grad n = IFFT(i G n(G)), b = df/d|grad n|^2 grad n,
+//   v_xc = df/dn - 2 div b, div by i G . b(G), real parts kept     src/features/scf/xc_fields.zig:87-140, 306-375
 //   local_r = V_ion + V_Hxc on the FFT-order grid                   src/features/scf/potential.zig:320-334
 //   linear potential mixing V <- V + beta (V_out - V)              src/features/scf/mixing.zig:10-18
 //   Pulay on (V_in, R = V_out - V_in), history h, from iteration s  src/features/scf/mixing.zig:183-234,
@@ -78,15 +81,138 @@ __device__ inline void lda_pz(double n, double* f, double* v) {
     *v = 2.0 * (4.0 / 3.0 * cx * cbrt_n + vc);
 }
 
+// PW92 unpolarised correlation (Ha): eps, d eps / d rs, v_c = eps - rs/3 d eps/d rs.
+__device__ inline void pw92(double rs, double* eps, double* deps, double* vc) {
+    constexpr double a = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+    const double sr = sqrt(rs);
+    const double f = b1 * sr + b2 * rs + b3 * rs * sr + b4 * rs * rs;
+    const double g = 1.0 + 1.0 / (2.0 * a * f);
+    const double lg = log(g);
+    const double df = 0.5 * b1 / sr + b2 + 1.5 * b3 * sr + 2.0 * b4 * rs;
+    const double dlg = -(1.0 / (2.0 * a * f * f * g)) * df;
+    *eps = -2.0 * a * (1.0 + a1 * rs) * lg;
+    *deps = -2.0 * a * (a1 * lg + (1.0 + a1 * rs) * dlg);
+    *vc = *eps - (rs / 3.0) * *deps;
+}
+
+// PBE at density n and g2 = |grad n|^2: f, df/dn, df/dg2 (Ry); zero below n = 1e-12.
+__device__ inline void pbe(double n, double g2, double* f, double* fn, double* fg) {
+    if (!(n > 1e-12)) {
+        *f = *fn = *fg = 0.0;
+        return;
+    }
+    // Exchange: F_x = 1 + kappa - kappa / (1 + mu s^2 / kappa).
+    constexpr double mu = 0.2195149727645171, kappa = 0.804;
+    const double cx = -0.75 * cbrt(3.0 / M_PI);
+    const double n13 = cbrt(n);
+    const double ex0 = n * cx * n13;
+    const double kf = cbrt(3.0 * M_PI * M_PI * n);
+    const double den_x = 4.0 * kf * kf * n * n;
+    const double s2 = g2 > 0.0 ? g2 / den_x : 0.0;
+    const double t = 1.0 + mu * s2 / kappa;
+    const double fx = 1.0 + kappa - kappa / t;
+    const double dfx = mu / (t * t);
+    const double fx_n = 4.0 / 3.0 * cx * n13 * fx + ex0 * dfx * (-8.0 / 3.0 * s2 / n);
+    const double fx_g = ex0 * dfx / den_x;
+    // Correlation: H = gamma ln(1 + (beta/gamma) t^2 (1 + A t^2) / (1 + A t^2 + A^2 t^4)).
+    constexpr double beta = 0.06672455060314922, gamma = 0.031090690869654895, bg = beta / gamma;
+    const double rs = cbrt(3.0 / (4.0 * M_PI * n));
+    double ec, deps, vc;
+    pw92(rs, &ec, &deps, &vc);
+    const double deps_dn = deps * (-rs / (3.0 * n));
+    const double den_c = 4.0 * (4.0 * kf / M_PI) * n * n;
+    const double t2 = g2 > 0.0 ? g2 / den_c : 0.0;
+    const double x = bg * t2;
+    const double u = exp(-ec / gamma);
+    const double um = u - 1.0;
+    const bool big = fabs(um) > 1e-12;
+    const double A = big ? bg / um : 0.0;
+    const double y = A * t2;
+    const double d2 = 1.0 + y + y * y;
+    const double q = (1.0 + y) / d2;
+    const double arg = 1.0 + x * q;
+    const bool pos = arg > 0.0;
+    const double H = pos ? gamma * log(arg) : 0.0;
+    const double dq = -A * y * (2.0 + y) / (d2 * d2);
+    const double dH_dt2 = pos ? gamma * (bg * q + x * dq) / arg : 0.0;
+    const double dH_dA = pos ? -gamma * x * t2 * y * (2.0 + y) / (arg * d2 * d2) : 0.0;
+    const double dA_dn = big ? bg * u / (gamma * um * um) * deps_dn : 0.0;
+    const double dH_dn = dH_dt2 * (-7.0 / 3.0 * t2 / n) + dH_dA * dA_dn;
+    *f = 2.0 * (ex0 * fx + n * (ec + H));
+    *fn = 2.0 * (fx_n + vc + H + n * dH_dn);
+    *fg = 2.0 * (fx_g + n * dH_dt2 / den_c);
+}
+
+// Cartesian G of FFT-order point i (wrapped frequencies); b = reciprocal rows.
+__device__ inline double3 g_at(size_t i, int nx, int ny, int nz, double3 b1, double3 b2, double3 b3) {
+    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
+    const int fx = x <= (nx - 1) / 2 ? x : x - nx;
+    const int fy = y <= (ny - 1) / 2 ? y : y - ny;
+    const int fz = z <= (nz - 1) / 2 ? z : z - nz;
+    return make_double3(fx * b1.x + fy * b2.x + fz * b3.x, fx * b1.y + fy * b2.y + fz * b3.y,
+                        fx * b1.z + fy * b2.z + fz * b3.z);
+}
+
+__device__ inline double comp(double3 v, int c) { return c == 0 ? v.x : (c == 1 ? v.y : v.z); }
+
+// out(G) = i G_c w(G)
+__global__ void k_grad_comp(const double2* __restrict__ w, double2* __restrict__ out, int c, int nx, int ny, int nz,
+                            double3 b1, double3 b2, double3 b3) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)nx * ny * nz;
+    if (i >= n) return;
+    const double g = comp(g_at(i, nx, ny, nz, b1, b2, b3), c);
+    out[i] = make_double2(-g * w[i].y, g * w[i].x);
+}
+
+// acc(G) (+)= i G_c w(G)
+__global__ void k_div_acc(const double2* __restrict__ w, double2* __restrict__ acc, int c, int first, int nx, int ny,
+                          int nz, double3 b1, double3 b2, double3 b3) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)nx * ny * nz;
+    if (i >= n) return;
+    const double g = comp(g_at(i, nx, ny, nz, b1, b2, b3), c);
+    const double2 t = make_double2(-g * w[i].y, g * w[i].x);
+    acc[i] = first ? t : make_double2(acc[i].x + t.x, acc[i].y + t.y);
+}
+
+// f, df/dn and b_c = df/dg2 * (grad n)_c (overwriting the gradient components).
+__global__ void k_pbe_point(const double* __restrict__ rho, double* __restrict__ gx, double* __restrict__ gy,
+                            double* __restrict__ gz, double* __restrict__ fxc, double* __restrict__ vn, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ax = gx[i], ay = gy[i], az = gz[i];
+    double f, fn, fg;
+    pbe(rho[i], ax * ax + ay * ay + az * az, &f, &fn, &fg);
+    fxc[i] = f;
+    vn[i] = fn;
+    gx[i] = fg * ax;
+    gy[i] = fg * ay;
+    gz[i] = fg * az;
+}
+
+// vn -= 2 Re(div)
+__global__ void k_gga_combine(double* __restrict__ vn, const double2* __restrict__ div, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) vn[i] -= 2.0 * div[i].x;
+}
+
 // V_out = Re(V_H) + v_xc; per-block partial sums of [V_H rho, f_xc, v_in rho, (rho - rho_in)^2].
+// With gga_f/gga_v (PBE fields computed before) those replace the point-wise LDA-PZ.
 __global__ void k_hxc(const double2* __restrict__ vh, const double* __restrict__ rho, const double* __restrict__ rho_in,
-                      const double* __restrict__ vin, double* __restrict__ vout, double* __restrict__ part, size_t n) {
+                      const double* __restrict__ vin, const double* __restrict__ gga_f, const double* __restrict__ gga_v,
+                      double* __restrict__ vout, double* __restrict__ part, size_t n) {
     __shared__ double red[4][kRedThreads];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const double r = rho[i];
         double f, v;
-        lda_pz(r, &f, &v);
+        if (gga_f) {
+            f = gga_f[i];
+            v = gga_v[i];
+        } else {
+            lda_pz(r, &f, &v);
+        }
         const double h = vh[i].x;
         vout[i] = h + v;
         s0 += h * r;
@@ -193,8 +319,44 @@ double ScfGrid::dot(const double* a, const double* c) {
     return reduce(part.as<double>(), kRedBlocks, 1, 0);
 }
 
+void ScfGrid::gga(const double* rho) {
+    hipStream_t s = stream();
+    const unsigned nb = grid_blocks(n);
+    double* gc[3] = {grad.as<double>(), grad.as<double>() + n, grad.as<double>() + 2 * n};
+    double2* w = work.as<double2>();
+    double2* w2 = work2.as<double2>();
+    // grad n = IFFT(i G FFT(n)), real part kept.
+    hipLaunchKernelGGL(k_real_to_complex, dim3(nb), dim3(256), 0, s, rho, w, n);
+    DZ_HIP(hipGetLastError());
+    fft3d_device<double>(w, tab, false, s);
+    for (int c = 0; c < 3; ++c) {
+        hipLaunchKernelGGL(k_grad_comp, dim3(nb), dim3(256), 0, s, w, w2, c, nx, ny, nz, recip_rows[0], recip_rows[1],
+                           recip_rows[2]);
+        DZ_HIP(hipGetLastError());
+        fft3d_device<double>(w2, tab, true, s);
+        hipLaunchKernelGGL(k_real_part, dim3(nb), dim3(256), 0, s, w2, gc[c], n);
+        DZ_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_pbe_point, dim3(nb), dim3(256), 0, s, rho, gc[0], gc[1], gc[2], gga_f.as<double>(),
+                       gga_v.as<double>(), n);
+    DZ_HIP(hipGetLastError());
+    // div b = IFFT(sum_c i G_c FFT(b_c)), then v_xc = df/dn - 2 Re(div b).
+    for (int c = 0; c < 3; ++c) {
+        hipLaunchKernelGGL(k_real_to_complex, dim3(nb), dim3(256), 0, s, gc[c], w, n);
+        DZ_HIP(hipGetLastError());
+        fft3d_device<double>(w, tab, false, s);
+        hipLaunchKernelGGL(k_div_acc, dim3(nb), dim3(256), 0, s, w, w2, c, c == 0 ? 1 : 0, nx, ny, nz,
+                           recip_rows[0], recip_rows[1], recip_rows[2]);
+        DZ_HIP(hipGetLastError());
+    }
+    fft3d_device<double>(w2, tab, true, s);
+    hipLaunchKernelGGL(k_gga_combine, dim3(nb), dim3(256), 0, s, gga_v.as<double>(), w2, n);
+    DZ_HIP(hipGetLastError());
+}
+
 void ScfGrid::hxc(const double* rho, const double* rho_ref, const double* vin, double* vout, double out[4]) {
     hipStream_t s = stream();
+    if (xc == DZHIP_XC_PBE) gga(rho);
     hipLaunchKernelGGL(k_real_to_complex, dim3(grid_blocks(n)), dim3(256), 0, s, rho, work.as<double2>(), n);
     DZ_HIP(hipGetLastError());
     fft3d_device<double>(work.as<double2>(), tab, false, s);
@@ -202,7 +364,9 @@ void ScfGrid::hxc(const double* rho, const double* rho_ref, const double* vin, d
     DZ_HIP(hipGetLastError());
     fft3d_device<double>(work.as<double2>(), tab, true, s);
     part.ensure(4 * kRedBlocks * sizeof(double));
-    hipLaunchKernelGGL(k_hxc, dim3(kRedBlocks), dim3(kRedThreads), 0, s, work.as<double2>(), rho, rho_ref, vin, vout,
+    const bool is_gga = xc == DZHIP_XC_PBE;
+    hipLaunchKernelGGL(k_hxc, dim3(kRedBlocks), dim3(kRedThreads), 0, s, work.as<double2>(), rho, rho_ref, vin,
+                       is_gga ? gga_f.as<double>() : nullptr, is_gga ? gga_v.as<double>() : nullptr, vout,
                        part.as<double>(), n);
     DZ_HIP(hipGetLastError());
     std::vector<double> h((size_t)kRedBlocks * 4);
@@ -228,6 +392,7 @@ ScfGrid* scf_grid_create(Context* ctx, int nx, int ny, int nz, const double* rec
         g->n = (size_t)nx * ny * nz;
         g->volume = volume;
         g->hist_cap = history;
+        for (int r = 0; r < 3; ++r) g->recip_rows[r] = make_double3(recip[3 * r], recip[3 * r + 1], recip[3 * r + 2]);
         g->tab.build(nx, ny, nz);
         const size_t n = g->n;
         g->g2.ensure(n * sizeof(double));
@@ -259,7 +424,7 @@ ScfGrid* scf_grid_create(Context* ctx, int nx, int ny, int nz, const double* rec
 void scf_grid_destroy(ScfGrid* g) {
     if (!g) return;
     DevBuf* bufs[] = {&g->g2, &g->work, &g->rho_in, &g->rho_out, &g->v_in, &g->v_out, &g->v_ion, &g->local,
-                      &g->part, &g->hv, &g->hr, &g->slots, &g->coef};
+                      &g->part, &g->hv, &g->hr, &g->slots, &g->coef, &g->work2, &g->grad, &g->gga_f, &g->gga_v};
     for (DevBuf* b : bufs) b->release();
     g->tab.release();
     delete g;
@@ -406,6 +571,16 @@ void scf_grid_mix(ScfGrid& g, double beta, int pulay) {
                        g.slots.as<int>(), g.coef.as<double>(), m, beta, g.v_in.as<double>(), n);
     DZ_HIP(hipGetLastError());
     DZ_HIP(hipStreamSynchronize(s));  // slots/coef are host vectors
+}
+
+void scf_grid_set_xc(ScfGrid& g, int functional) {
+    if (functional == DZHIP_XC_PBE) {
+        g.work2.ensure(g.n * sizeof(double2));
+        g.grad.ensure(3 * g.n * sizeof(double));
+        g.gga_f.ensure(g.n * sizeof(double));
+        g.gga_v.ensure(g.n * sizeof(double));
+    }
+    g.xc = functional;
 }
 
 }  // namespace dzhip

@@ -38,6 +38,7 @@ EXPORTS = (
     "dzhip_scf_grid_create", "dzhip_scf_grid_destroy", "dzhip_scf_grid_set_ion", "dzhip_scf_grid_init",
     "dzhip_ham_set_local_grid", "dzhip_scf_grid_add_solution_density", "dzhip_scf_grid_hxc",
     "dzhip_scf_grid_mix", "dzhip_scf_grid_read", "dzhip_ham_set_tuning", "dzhip_scf_grid_set_dielectric",
+    "dzhip_scf_grid_set_xc",
     "dzhip_scf_grid_init_rho",
 )
 
@@ -71,6 +72,8 @@ _ERRS = {-1: InvalidMatrixSize, -2: HipFailure, -3: OutOfMemory, -4: BadState, -
 PRECOND_KINETIC = 0
 PRECOND_TPA = 1
 PRECOND_MLP = 2
+XC_LDA_PZ = 0
+XC_PBE = 1
 MLP_PARAMS = 1217
 
 
@@ -163,6 +166,7 @@ def _load():
         "dzhip_scf_grid_read": ([vp, i32, dp, u64], i32),
         "dzhip_ham_set_tuning": ([vp, i32], i32),
         "dzhip_scf_grid_set_dielectric": ([vp, ct.c_double, ct.c_double], i32),
+        "dzhip_scf_grid_set_xc": ([vp, i32], i32),
         "dzhip_scf_grid_init_rho": ([vp, dp, u64], i32),
     }
     missing = set(EXPORTS) - set(sig)
@@ -555,6 +559,11 @@ class ScfGrid:
 
     def set_dielectric(self, diemac: float = 1.0, dielng: float = 1.0):
         check(L().dzhip_scf_grid_set_dielectric(self.handle, float(diemac), float(dielng)))
+
+    def set_xc(self, functional: str):
+        """Exchange-correlation functional of V_xc and E_xc: "lda_pz" (default) or "pbe"."""
+        code = {"lda_pz": XC_LDA_PZ, "pbe": XC_PBE}[functional]
+        check(L().dzhip_scf_grid_set_xc(self.handle, code))
 
     def mix(self, beta: float, pulay: bool):
         check(L().dzhip_scf_grid_mix(self.handle, float(beta), 1 if pulay else 0))

@@ -57,6 +57,7 @@ class ScfSystem:
     kweights: list
     smearing: float = 0.0  # Fermi-Dirac width (Ry); 0 = fixed occupations
     atom_rho_width: float = 1.2  # bohr: Gaussian valence charge per atom of the atomic start density
+    xc: str = "lda_pz"  # "lda_pz" or "pbe" (PBE runs on the device grid)
 
 
 # GTH-form parameters of a four-electron Si-like pseudo-atom (Hartree units for C1 and h): the
@@ -129,7 +130,8 @@ AL_GTH = {"z": 3.0, "rloc": 0.45, "c1": -8.491351, "r_s": 0.460104, "h_s": 5.088
           "h_p": 2.193438}
 
 
-def aluminium(ecut_ry=30.0, kmesh=(4, 4, 4), nbands=12, smearing_ry=0.01, grid=None, displace=None) -> ScfSystem:
+def aluminium(ecut_ry=30.0, kmesh=(4, 4, 4), nbands=12, smearing_ry=0.01, grid=None, displace=None,
+              xc="lda_pz") -> ScfSystem:
     """fcc Al in the 4-atom cubic cell (a = 4.05 A), GTH-form local part and s/p Gaussian
     projectors (Goedecker-Teter-Hutter Al LDA parameters), Fermi-Dirac smearing; k-points from an
     unshifted mesh reduced by time reversal (4x4x4 -> 36 points with uneven plane-wave counts).
@@ -150,7 +152,7 @@ def aluminium(ecut_ry=30.0, kmesh=(4, 4, 4), nbands=12, smearing_ry=0.01, grid=N
     kpts, kw = mp_mesh_time_reversal(recip, kmesh)
     grid = tuple(grid) if grid is not None else tuple(tables.auto_grid(ecut_ry, recip))
     return ScfSystem("al4", cell, recip, abs(float(np.linalg.det(cell))), grid, ecut_ry, pos, np.zeros(4, np.int32),
-                     [sp], g["z"], g["rloc"], g["c1"], 12.0, nbands, kpts, kw, smearing_ry)
+                     [sp], g["z"], g["rloc"], g["c1"], 12.0, nbands, kpts, kw, smearing_ry, xc=xc)
 
 
 def si8_small(ecut_ry=8.0, nbands=20):
@@ -165,11 +167,11 @@ def si8_small(ecut_ry=8.0, nbands=20):
 AL_SMALL_DISPLACEMENT = (0.11, 0.07, 0.05)  # bohr
 
 
-def al4_small(ecut_ry=8.0, nbands=10):
+def al4_small(ecut_ry=8.0, nbands=10, xc="lda_pz"):
     """Small smearing parity case: the 4-atom Al cell at 8 Ry (15^3 grid), 2x2x2 mesh reduced by
     time reversal (8 k-points with different plane-wave counts), second atom displaced by
     AL_SMALL_DISPLACEMENT so no partially occupied level is degenerate."""
-    return aluminium(ecut_ry=ecut_ry, kmesh=(2, 2, 2), nbands=nbands, displace=AL_SMALL_DISPLACEMENT)
+    return aluminium(ecut_ry=ecut_ry, kmesh=(2, 2, 2), nbands=nbands, displace=AL_SMALL_DISPLACEMENT, xc=xc)
 
 
 def si64(ecut_ry=30.0, nbands=160):
@@ -453,6 +455,8 @@ class ScfDriver:
         potentials and mixing run on the host in FP64 numpy."""
         t0 = time.perf_counter()
         s = self.sys = system
+        if s.xc != "lda_pz" and not device_grid:
+            raise ValueError(f"xc = {s.xc!r} runs on the device grid (device_grid=True)")
         self.concurrent = concurrent_k and len(s.kpoints) > 1
         self.ctxs = [Context(device) for _ in s.kpoints] if self.concurrent else [Context(device)]
         self.g, self.g2 = g_vectors(s.grid, s.recip)
@@ -471,6 +475,7 @@ class ScfDriver:
         if device_grid:
             self.grid = ScfGrid(self.ctxs[0], s.grid, s.recip, s.volume, pulay_history)
             self.grid.set_ion(self.v_ion)
+            self.grid.set_xc(s.xc)
         self.pool = ThreadPoolExecutor(len(self.hams)) if self.concurrent else None
         # Base preconditioner of the installed learned model (set_precond_model) and an optional
         # callback hook(k, ham, stats) after every k-point solve.

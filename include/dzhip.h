@@ -320,6 +320,14 @@ int dzhip_scf_grid_hxc(dzhip_scf_grid* grid, double* out, uint64_t n_out);
 /* Model dielectric preconditioner of the Pulay residual, P(G) = (dielng^2 G^2 + 1/diemac) /
  * (dielng^2 G^2 + 1) (src/features/scf/mixing.zig:309-338); diemac = 1 (default) disables it. */
 int dzhip_scf_grid_set_dielectric(dzhip_scf_grid* grid, double diemac, double dielng);
+/* Exchange-correlation functional of the grid's V_xc and E_xc (the reference's [scf] xc):
+ * LDA-PZ point-wise (src/features/xc/xc.zig:195-210) or PBE with the gradient and the
+ * divergence term taken spectrally on the grid (xc.zig:213-342,
+ * src/features/scf/xc_fields.zig:87-140, 306-375).  Default DZHIP_XC_LDA_PZ; takes effect at the
+ * next dzhip_scf_grid_init / _init_rho / _hxc. */
+#define DZHIP_XC_LDA_PZ 0
+#define DZHIP_XC_PBE 1
+int dzhip_scf_grid_set_xc(dzhip_scf_grid* grid, int functional);
 /* V_in <- linear (pulay = 0) or Pulay (pulay != 0) mix of V_in and V_out. */
 int dzhip_scf_grid_mix(dzhip_scf_grid* grid, double beta, int pulay);
 int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t count);

@@ -3,6 +3,9 @@
 Restates (paths relative to the dft-zig source tree):
   hartree_potential_at_g: V_H(G) = 8 pi rho(G) / G^2, G = 0 dropped   src/features/scf/potential.zig:34-44
   lda_pz (Ry), PZ81 correlation energy/potential                      src/features/xc/xc.zig:195-210, 344-384
+  pbe (Ry): PBE exchange, PBE correlation on PW92                     src/features/xc/xc.zig:213-342
+  GGA fields: grad n by i G, b = df/d|grad n|^2 grad n, v = df/dn - 2 div b
+                                                                     src/features/scf/xc_fields.zig:87-140, 306-375
   assemble_energy_terms: total = band + (-E_H - int v_xc rho + E_xc) + ion terms
                                                                      src/features/scf/energy.zig:372-396
   ionic local potential sum_a e^{-iG.tau} v(|G|) / Omega, G = 0 skipped
@@ -69,6 +72,109 @@ def lda_pz(n: np.ndarray):
     f[m] = 2.0 * (nm * c_x * n13 + nm * eps_c)
     v[m] = 2.0 * (4.0 / 3.0 * c_x * n13 + v_c)
     return f, v
+
+
+def pw92(rs: np.ndarray):
+    """PW92 unpolarised correlation: (eps_c, d eps_c / d rs, v_c) in Ha (xc.zig:320-342)."""
+    a, a1, b1, b2, b3, b4 = 0.031091, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294
+    sr = np.sqrt(rs)
+    f = b1 * sr + b2 * rs + b3 * rs * sr + b4 * rs * rs
+    g = 1.0 + 1.0 / (2.0 * a * f)
+    lg = np.log(g)
+    eps = -2.0 * a * (1.0 + a1 * rs) * lg
+    df = 0.5 * b1 / sr + b2 + 1.5 * b3 * sr + 2.0 * b4 * rs
+    dlg = -(1.0 / (2.0 * a * f * f * g)) * df
+    deps = -2.0 * a * (a1 * lg + (1.0 + a1 * rs) * dlg)
+    return eps, deps, eps - (rs / 3.0) * deps
+
+
+def pbe(n: np.ndarray, g2: np.ndarray):
+    """(f, df/dn, df/d|grad n|^2) of PBE in Ry, element-wise; zero below n = 1e-12
+    (xc.zig:213-318: exchange with mu = 0.2195149727645171, kappa = 0.804; correlation
+    H(t^2, A) on PW92 with beta = 0.06672455060314922, gamma = 0.031090690869654895)."""
+    n = np.asarray(n, dtype=np.float64)
+    g2 = np.broadcast_to(np.asarray(g2, dtype=np.float64), n.shape)
+    f = np.zeros_like(n)
+    vn = np.zeros_like(n)
+    vg = np.zeros_like(n)
+    m = n > 1e-12
+    nm, gm = n[m], g2[m]
+    # Exchange.
+    mu, kappa = 0.2195149727645171, 0.804
+    c_x = -0.75 * math.pow(3.0 / PI, 1.0 / 3.0)
+    n13 = np.power(nm, 1.0 / 3.0)
+    ex0 = nm * c_x * n13
+    kf = np.power(3.0 * PI * PI * nm, 1.0 / 3.0)
+    den_x = 4.0 * kf * kf * nm * nm
+    s2 = np.where(gm > 0.0, gm / den_x, 0.0)
+    t = 1.0 + mu * s2 / kappa
+    fx = 1.0 + kappa - kappa / t
+    dfx = mu / (t * t)
+    fx_n = 4.0 / 3.0 * c_x * n13 * fx + ex0 * dfx * (-8.0 / 3.0 * s2 / nm)
+    fx_g = ex0 * dfx / den_x
+    # Correlation.
+    beta, gamma = 0.06672455060314922, 0.031090690869654895
+    rs = np.power(3.0 / (4.0 * PI * nm), 1.0 / 3.0)
+    eps_c, deps_drs, v_c = pw92(rs)
+    deps_dn = deps_drs * (-rs / (3.0 * nm))
+    ks2 = 4.0 * kf / PI
+    den_c = 4.0 * ks2 * nm * nm
+    t2 = np.where(gm > 0.0, gm / den_c, 0.0)
+    bg = beta / gamma
+    x = bg * t2
+    u = np.exp(-eps_c / gamma)
+    um = u - 1.0
+    big = np.abs(um) > 1e-12
+    A = np.where(big, bg / np.where(big, um, 1.0), 0.0)
+    y = A * t2
+    d2 = 1.0 + y + y * y
+    q = (1.0 + y) / d2
+    arg = 1.0 + x * q
+    pos = arg > 0.0
+    H = np.where(pos, gamma * np.log(np.where(pos, arg, 1.0)), 0.0)
+    dq_dt2 = -A * y * (2.0 + y) / (d2 * d2)
+    dH_dt2 = np.where(pos, gamma * (bg * q + x * dq_dt2) / np.where(pos, arg, 1.0), 0.0)
+    dH_dA = np.where(pos, -gamma * x * t2 * y * (2.0 + y) / (np.where(pos, arg, 1.0) * d2 * d2), 0.0)
+    dA_dn = np.where(big, bg * u / (gamma * np.where(big, um, 1.0) ** 2) * deps_dn, 0.0)
+    dH_dn = dH_dt2 * (-7.0 / 3.0 * t2 / nm) + dH_dA * dA_dn
+    fc = nm * (eps_c + H)
+    fc_n = v_c + H + nm * dH_dn
+    fc_g = nm * dH_dt2 / den_c
+    f[m] = 2.0 * (ex0 * fx + fc)
+    vn[m] = 2.0 * (fx_n + fc_n)
+    vg[m] = 2.0 * (fx_g + fc_g)
+    return f, vn, vg
+
+
+def gradient(values, grid, recip):
+    """(d/dx, d/dy, d/dz) of a real grid function by i G in reciprocal space, real part kept
+    (xc_fields.zig:306-339)."""
+    n = values.shape[0]
+    vg = fftconv.forward3d(values.astype(np.complex128), grid) / n
+    g, _ = g_squared_grid(grid, recip)
+    return [fftconv.inverse3d(1j * vg * g[:, c] * n, grid).real for c in range(3)]
+
+
+def divergence(bx, by, bz, grid, recip):
+    """div b by i G . b(G), real part kept (xc_fields.zig:341-375)."""
+    n = bx.shape[0]
+    g, _ = g_squared_grid(grid, recip)
+    acc = np.zeros(n, dtype=np.complex128)
+    for c, comp in enumerate((bx, by, bz)):
+        acc += fftconv.forward3d(comp.astype(np.complex128), grid) / n * g[:, c]
+    return fftconv.inverse3d(1j * acc * n, grid).real
+
+
+def xc_fields(rho, grid, recip, functional="lda_pz"):
+    """(f_xc, v_xc) on the grid in Ry: LDA point-wise, PBE with v = df/dn - 2 div(df/dg2 grad n)
+    (xc_fields.zig:64-140, 377-386)."""
+    if functional == "lda_pz":
+        return lda_pz(rho)
+    if functional != "pbe":
+        raise ValueError(functional)
+    gx, gy, gz = gradient(rho, grid, recip)
+    f, vn, vg = pbe(rho, gx * gx + gy * gy + gz * gz)
+    return f, vn - 2.0 * divergence(vg * gx, vg * gy, vg * gz, grid, recip)
 
 
 def g_squared_grid(grid, recip):
@@ -253,14 +359,16 @@ def model_dielectric(residual_r, grid, recip, diemac, dielng):
 def run_scf(system, n_iter=40, beta=0.3, conv=1e-8, pulay_history=0, pulay_start=4, pulay_relative=True,
             diemac=1.0, dielng=1.0):
     """Fixed-occupation SCF with potential mixing (linear, or Pulay from iteration `pulay_start`
-    when pulay_history > 0, scf.zig:164); dense eigh per k."""
+    when pulay_history > 0, scf.zig:164); dense eigh per k; system.xc ("lda_pz" default, "pbe")
+    picks the functional."""
+    xc = getattr(system, "xc", "lda_pz")
     grid, recip, vol = system.grid, system.recip, system.volume
     n = int(np.prod(grid))
     v_ion = ionic_potential_r(grid, recip, vol, system.atom_pos,
                               adapters.gth_local_q(system.z_ion, system.rloc, system.c1))
     rho = np.full(n, system.nelec / vol)
     vh, _ = hartree(rho, grid, recip, vol)
-    _, vxc = lda_pz(rho)
+    _, vxc = xc_fields(rho, grid, recip, xc)
     v_in = vh + vxc
     res = ScfResult()
     mixer = PulayMixer(pulay_history, pulay_relative) if pulay_history > 0 else None
@@ -290,7 +398,7 @@ def run_scf(system, n_iter=40, beta=0.3, conv=1e-8, pulay_history=0, pulay_start
             rho_out = odens.accumulate_density(ctx.fft_idx, grid, vec.T, wts, 1.0 / vol, rho_out)
             eigs.append(ev)
         vh_out, e_h = hartree(rho_out, grid, recip, vol)
-        f_xc, vxc_out = lda_pz(rho_out)
+        f_xc, vxc_out = xc_fields(rho_out, grid, recip, xc)
         e_xc = float(np.sum(f_xc)) * vol / n
         # Kohn-Sham energy with the input potential: band - int V_Hxc^in rho_out + E_H + E_xc.
         e_tot = e_band - float(np.sum(v_in * rho_out)) * vol / n + e_h + e_xc - e_ts

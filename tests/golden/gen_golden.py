@@ -83,8 +83,21 @@ def scf_al_file():
     print("scf al", len(res.energies), res.energies[-1])
 
 
+def scf_al_pbe_file():
+    """Oracle SCF of the small Al case (displaced atom) with PBE and Fermi-Dirac smearing."""
+    from dzhip import scf as dscf
+    from oracle import scf as oscf
+
+    system = dscf.al4_small(xc="pbe")
+    res = oscf.run_scf(system, n_iter=60, beta=0.3, conv=1e-12, pulay_history=8, pulay_start=4)
+    assert res.converged
+    np.savez_compressed(HERE / "scf_al4_pbe_small.npz", energies=np.array(res.energies), drho=np.array(res.drho),
+                        eigenvalues=np.array(res.eigenvalues), rho=res.rho)
+    print("scf al pbe", len(res.energies), res.energies[-1])
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["hpsi", "fft", "scf", "scf_al"]
+    which = sys.argv[1:] or ["hpsi", "fft", "scf", "scf_al", "scf_al_pbe"]
     if "hpsi" in which:
         hpsi_file("tiny")
         hpsi_file("si8")
@@ -94,3 +107,5 @@ if __name__ == "__main__":
         scf_file()
     if "scf_al" in which:
         scf_al_file()
+    if "scf_al_pbe" in which:
+        scf_al_pbe_file()

@@ -13,6 +13,7 @@ import pytest
 
 from dzhip import FP32, FP64, PRECOND_TPA
 from dzhip import scf as dscf
+from oracle import scf as oscf
 
 pytestmark = pytest.mark.gpu
 GOLD = Path(__file__).resolve().parent / "golden" / "scf_si8_small.npz"
@@ -229,3 +230,58 @@ def test_scf_dielectric_pulay_host_and_device_vs_oracle(dz, ctx, system):
         print(device_grid, len(e), len(ref.energies), np.max(np.abs(e[:m] - np.array(ref.energies[:m]))))
         assert run.converged and abs(len(e) - len(ref.energies)) <= 3
         assert np.max(np.abs(e[:m] - np.array(ref.energies[:m]))) < 1e-8
+
+
+def test_scf_grid_pbe_matches_oracle(dz, ctx):
+    """PBE on the device grid (gradient, point-wise PBE, divergence term) against the oracle's
+    xc_fields on an atomic-superposition density and on a solved density (1e-11 relative)."""
+    s = dscf.al4_small(xc="pbe")
+    drv = dscf.ScfDriver(s, device_grid=True)
+    try:
+        g = drv.grid
+        rho0 = dscf.atomic_density(s, drv.g, drv.g2)
+        g.init_rho(rho0)
+        n = rho0.shape[0]
+        vh, _ = oscf.hartree(rho0, s.grid, s.recip, s.volume)
+        _, vx = oscf.xc_fields(rho0, s.grid, s.recip, "pbe")
+        _, vx_lda = oscf.xc_fields(rho0, s.grid, s.recip, "lda_pz")
+        assert np.max(np.abs(vx - vx_lda)) > 1e-3  # the gradient terms are not negligible here
+        np.testing.assert_allclose(g.read(dz.GRID_VIN), vh + vx, rtol=0, atol=1e-11 * np.max(np.abs(vh + vx)))
+        ham = drv.hams[0]
+        g.set_local(ham)
+        ham.eig_solve(s.nbands, tol=1e-9, max_iter=300, want_vectors=False)
+        w = dscf.band_weights(s.nelec, s.nbands, 1.0)
+        g.add_density(ham, w)
+        rho_h = ham.density_solution(w)
+        e_h, e_xc, _, _ = g.hxc()
+        vh2, eh2 = oscf.hartree(rho_h, s.grid, s.recip, s.volume)
+        f2, vx2 = oscf.xc_fields(rho_h, s.grid, s.recip, "pbe")
+        print("E_xc", e_xc, float(np.sum(f2)) * s.volume / n, "max |dV|", np.max(np.abs(g.read(dz.GRID_VOUT) - vh2 - vx2)))
+        assert abs(e_h - eh2) < 1e-11 * abs(eh2)
+        assert abs(e_xc - float(np.sum(f2)) * s.volume / n) < 1e-11 * abs(e_xc)
+        np.testing.assert_allclose(g.read(dz.GRID_VOUT), vh2 + vx2, rtol=0, atol=1e-11 * np.max(np.abs(vh2 + vx2)))
+    finally:
+        drv.close()
+
+
+@pytest.mark.parametrize("concurrent", [False, True])
+def test_scf_pbe_al(dz, ctx, concurrent):
+    """PBE + Fermi-Dirac SCF on the device grid (xc_fields.zig:104-140): the oracle's trajectory
+    and fixed point on the small displaced Al cell."""
+    gold = np.load(GOLD.parent / "scf_al4_pbe_small.npz")
+    s = dscf.al4_small(xc="pbe")
+    drv = dscf.ScfDriver(s, concurrent_k=concurrent, device_grid=True)
+    try:
+        run = drv.run(max_iter=60, conv=1e-12, eig_tol=1e-11, eig_max_iter=600, warmup_steps=0, final_tol=1e-11)
+    finally:
+        drv.close()
+    e = np.array(run.energies)
+    m = min(len(e), len(gold["energies"]))
+    print("iterations", len(e), len(gold["energies"]), "dE", np.max(np.abs(e[:m] - gold["energies"][:m])),
+          "eig", np.max(np.abs(np.array(run.eigenvalues) - gold["eigenvalues"])),
+          "L1", float(np.sum(np.abs(run.rho - gold["rho"]))) * s.volume / run.rho.size)
+    assert run.converged
+    assert abs(len(e) - len(gold["energies"])) <= 3
+    assert np.max(np.abs(e[:m] - gold["energies"][:m])) < 1e-8
+    assert np.max(np.abs(np.array(run.eigenvalues) - gold["eigenvalues"])) < EV_TOL
+    assert float(np.sum(np.abs(run.rho - gold["rho"]))) * s.volume / run.rho.size < RHO_L1

@@ -213,3 +213,19 @@ def test_atomic_start_density():
         rho = dscf.atomic_density(s, g, g2)
         assert abs(np.sum(rho) * s.volume / rho.size - s.nelec) < 1e-9 * s.nelec
         assert rho.min() > -1e-6
+
+
+def test_oracle_pbe_scf_golden():
+    gold = np.load(GOLD.parent / "scf_al4_pbe_small.npz")
+    s = dscf.al4_small(xc="pbe")
+    res = oscf.run_scf(s, n_iter=60, beta=0.3, conv=1e-12, pulay_history=8, pulay_start=4)
+    assert res.converged and len(res.energies) == len(gold["energies"])
+    assert np.max(np.abs(np.array(res.energies) - gold["energies"])) < 1e-10
+    # PBE differs from LDA-PZ on the same cell (the golden files are not the same run).
+    lda = np.load(GOLD.parent / "scf_al4_small.npz")
+    assert abs(gold["energies"][-1] - lda["energies"][-1]) > 1e-4
+
+
+def test_pbe_needs_the_device_grid():
+    with pytest.raises(ValueError):
+        dscf.ScfDriver(dscf.al4_small(xc="pbe"), device_grid=False)
