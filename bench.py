@@ -209,7 +209,7 @@ def main():
                     help="also time the SCF to 1e-6 Ha (plain FP64 vs mixed+batched)")
     ap.add_argument("--scf-max-iter", type=int, default=45)
     ap.add_argument("--xline-split", type=int, default=0,
-                    help="fix the x-line kernel's y-slab count (0 = autotune on the first apply)")
+                    help="fix the x-line kernel's y-slab count, -1 = register kernel (0 = autotune on the first apply)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (k-point sharding is "
@@ -292,7 +292,8 @@ def main():
                  "frac_hbm": round(gbs64 / HBM_PEAK_GBS, 4)},
         "fp32_frac_hbm": round(gbs32 / HBM_PEAK_GBS, 4),
         "fp32_vs_fp64_err": {"max_band_rel_l2": err, "max_abs": abserr},
-        "tuning": {"xline_split_fp32": tuned["xline_split_fp32"], "xline_split_fp64": tuned["xline_split_fp64"]},
+        "tuning": {"xline_split_fp32": tuned["xline_split_fp32"], "xline_split_fp64": tuned["xline_split_fp64"],
+                   "xline_register_fp32": tuned["xline_register_fp32"]},
         "phases_ms_fp32": {k: round(v, 5) for k, v in ph32.items()},
         "phases_ms_fp64": {k: round(v, 5) for k, v in ph64.items()},
         "scf_time_to_1e-6_Ha": None if args.scf == "none" else scf_leg(args.scf, args.scf_max_iter),

@@ -1,0 +1,139 @@
+// Register-resident FFT of one line per thread, for line lengths known at compile time.
+//
+// Mixed-radix Stockham autosort (decimation in time): stage s with radix R and span Ns (the
+// product of the earlier radices) reads a[j + r N/R], multiplies by w^(r (j mod Ns) N/(Ns R)),
+// does a radix-R butterfly and writes b[(j / Ns) Ns R + (j mod Ns) + r Ns]; natural order in
+// and out.  Radices are taken as 4s, then 2, then 3s, then 5s.  Every index is a compile-time
+// constant after unrolling, so the line lives in VGPRs and no LDS or barrier is involved; the
+// twiddles w^k = e^{-2 pi i k / N} come from a table with compile-time offsets (wave-uniform
+// loads).  Sign S = -1 is the forward transform e^{-i...}, S = +1 the unnormalised inverse.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace dzhip {
+namespace freg {
+
+constexpr int next_radix(int n) { return n % 4 == 0 ? 4 : (n % 2 == 0 ? 2 : (n % 3 == 0 ? 3 : 5)); }
+
+constexpr bool supported(int n) {
+    if (n < 2) return false;
+    while (n % 2 == 0) n /= 2;
+    while (n % 3 == 0) n /= 3;
+    while (n % 5 == 0) n /= 5;
+    return n == 1;
+}
+
+constexpr int num_stages(int n) {
+    int c = 0;
+    while (n > 1) {
+        n /= next_radix(n);
+        ++c;
+    }
+    return c;
+}
+
+constexpr int radix_at(int n, int k) {
+    for (int i = 0; i < k; ++i) n /= next_radix(n);
+    return next_radix(n);
+}
+
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C, typename T>
+__device__ __forceinline__ C cscale(C a, T s) { return C{a.x * s, a.y * s}; }
+// i^(S) * a with S = +-1: +1 -> i a, -1 -> -i a
+template <int S, typename C>
+__device__ __forceinline__ C cmuli(C a) { return S > 0 ? C{-a.y, a.x} : C{a.y, -a.x}; }
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C w) { return C{a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x}; }
+
+// In-place DFT of size R with kernel e^{S 2 pi i q s / R}.
+template <int R, int S, typename C>
+__device__ __forceinline__ void butterfly(C (&v)[R]) {
+    using T = decltype(v[0].x);
+    if constexpr (R == 2) {
+        const C a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        const C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const C s13 = cadd(v[1], v[3]), d13 = cmuli<S>(csub(v[1], v[3]));
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[3] = csub(d02, d13);
+    } else if constexpr (R == 3) {
+        constexpr T c = T(-0.5);
+        constexpr T sn = T(0.86602540378443864676);  // sin(2 pi / 3)
+        const C t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+        const C m = C{v[0].x + c * t1.x, v[0].y + c * t1.y};
+        const C n = cscale(cmuli<S>(t2), sn);
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(m, n);
+        v[2] = csub(m, n);
+    } else {
+        static_assert(R == 5, "radix");
+        constexpr T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+        constexpr T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);
+        const C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        const C t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        const C m1 = C{v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
+        const C m2 = C{v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
+        const C n1 = cmuli<S>(C{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y});
+        const C n2 = cmuli<S>(C{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y});
+        v[0] = cadd(v[0], cadd(t1, t2));
+        v[1] = cadd(m1, n1);
+        v[4] = csub(m1, n1);
+        v[2] = cadd(m2, n2);
+        v[3] = csub(m2, n2);
+    }
+}
+
+template <typename C, int N, int R, int Ns, int S>
+__device__ __forceinline__ void stage(const C (&in)[N], C (&out)[N], const C* __restrict__ tw) {
+#pragma unroll
+    for (int j = 0; j < N / R; ++j) {
+        C v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = in[j + r * (N / R)];
+        const int k = j % Ns;
+        if (k != 0) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const int e = (r * k * (N / (Ns * R))) % N;
+                C w = tw[e];
+                if (S > 0) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        butterfly<R, S>(v);
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[base + r * Ns] = v[r];
+    }
+}
+
+template <typename C, int N, int S, int K, int Ns>
+__device__ __forceinline__ void stages(C (&a)[N], const C* __restrict__ tw) {
+    if constexpr (K < num_stages(N)) {
+        constexpr int R = radix_at(N, K);
+        C b[N];
+        stage<C, N, R, Ns, S>(a, b, tw);
+#pragma unroll
+        for (int i = 0; i < N; ++i) a[i] = b[i];
+        stages<C, N, S, K + 1, Ns * R>(a, tw);
+    }
+}
+
+// a <- DFT_S(a); tw[k] = e^{-2 pi i k / N}, k < N.
+template <int N, int S, typename C>
+__device__ __forceinline__ void fft(C (&a)[N], const C* __restrict__ tw) {
+    static_assert(supported(N), "line length must be 2^a 3^b 5^c");
+    stages<C, N, S, 0, 1>(a, tw);
+}
+
+}  // namespace freg
+}  // namespace dzhip

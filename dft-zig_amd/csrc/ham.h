@@ -49,6 +49,9 @@ struct Ham {
 
     DevBuf kinetic64, kinetic32;
     DevBuf vloc64, vloc32;  // V_loc(r)/N in Ry
+    // Register x-line kernel: plane index of each x (-1 = no sphere plane), e^{-2 pi i k/nx}
+    // (FP32), and V_loc/N in x-major order [x][z][y] (FP32).
+    DevBuf plane_of_x, xreg_tw, vloc32_t;
     bool has_local = false;
 
     // Nonlocal: complex projector matrix A[p][G] = phi(G) e^{iG.tau_a} (P x n_pw), D tables.
@@ -58,8 +61,9 @@ struct Ham {
 
     // Work buffers.
     DevBuf xstage, ystage, x32, y32, tbuf, partial, qbuf;
-    // Tuned launch shapes: y-slab count of the x-line kernel per precision (0: not yet tuned;
-    // [0] FP64, [1] FP32); tune_fixed != 0 forces a slab count (sweeps).
+    // Tuned launch shapes: x-line variant per precision ([0] FP64, [1] FP32): 0 not yet tuned,
+    // d >= 1 the LDS kernel with d y-slabs, -1 the register kernel (FP32); tune_fixed != 0
+    // forces a variant (sweeps).
     int xline_split[2] = {0, 0};
     int tune_fixed = 0;
 
@@ -139,6 +143,8 @@ const double2* eig_solution(const Ham& h, uint64_t* cols);
 size_t eig_training_samples(Ham& h, double energy_scale, int base, int stride, float* out_host, size_t cap);
 
 void ham_build(Ham& h, const dzhip_ham_desc& d);
+// True when the register-resident FP32 x-line kernel is built for line length nx.
+bool xline_reg_supported(int nx);
 void ham_set_local(Ham& h, const double* local_r);
 // local_r already on the device (read on h's stream).
 void ham_set_local_dev(Ham& h, const double* local_dev);

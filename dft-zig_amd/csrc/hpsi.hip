@@ -11,7 +11,9 @@
 //               hold plane waves exist, so T is half a grid.
 //   xline     : one workgroup per (z, band): gather the planes' rows into an LDS [x][y]
 //               tile, inverse FFT along x, multiply by V_loc(r)/N, forward FFT along x,
-//               write back only the rows of sphere x-planes.
+//               write back only the rows of sphere x-planes.  FP32 with nx = 2^a 3^b 5^c
+//               up to 128 can instead run k_xline_reg, one x-line per thread in registers
+//               (fft_reg.h), when the autotuner finds it faster.
 //   plane_fwd : one workgroup per (plane, band): forward FFT along y and z, gather the
 //               sphere points, add |k+G|^2 x (and the nonlocal term already in y).
 // HBM traffic per band: psi + H psi on the sphere and two round trips of half a grid.
@@ -27,6 +29,7 @@
 #include <vector>
 
 #include "fft_lds.h"
+#include "fft_reg.h"
 #include "ham.h"
 
 namespace dzhip {
@@ -156,6 +159,70 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
         const int p = fdiv(i, ylen, my), y = i - p * ylen;
         tcol[((size_t)p * nz + z) * ny + y] = buf[y + pitch * pxpos[p]];
     }
+}
+
+// x-line pass with one x-line per thread held in registers (FP32, nx fixed at compile time):
+// thread l = z ny + y of band blockIdx.y loads its values of the np sphere planes (consecutive
+// threads read consecutive addresses of each plane), runs the inverse x-transform, multiplies by
+// V_loc/N from the x-major copy vt[x][z][y], runs the forward transform and stores the planes
+// back.  No LDS and no barrier; absent x-planes enter as zeros.
+template <int NX>
+__global__ __launch_bounds__(256) void k_xline_reg(float2* __restrict__ tbuf, const int* __restrict__ plane_of_x,
+                                                   int nplanes, int lines, const float* __restrict__ vt,
+                                                   const float2* __restrict__ tw) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= lines) return;
+    float2* t = tbuf + (size_t)blockIdx.y * nplanes * lines + l;
+    float2 a[NX];
+#pragma unroll
+    for (int x = 0; x < NX; ++x) {
+        const int p = plane_of_x[x];
+        a[x] = p >= 0 ? t[(size_t)p * lines] : make_float2(0.f, 0.f);
+    }
+    freg::fft<NX, 1>(a, tw);
+#pragma unroll
+    for (int x = 0; x < NX; ++x) {
+        const float v = vt[(size_t)x * lines + l];
+        a[x] = make_float2(a[x].x * v, a[x].y * v);
+    }
+    freg::fft<NX, -1>(a, tw);
+#pragma unroll
+    for (int x = 0; x < NX; ++x) {
+        const int p = plane_of_x[x];
+        if (p >= 0) t[(size_t)p * lines] = a[x];
+    }
+}
+
+// Line lengths the register x-line kernel is built for.
+#define DZ_XREG_SIZES(X) X(8) X(12) X(15) X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) \
+    X(45) X(48) X(50) X(54) X(60) X(64) X(72) X(75) X(80) X(81) X(90) X(96) X(100) X(108) X(120) X(125) X(128)
+
+bool xline_reg_supported(int nx) {
+    switch (nx) {
+#define DZ_XREG_CASE(N) case N:
+        DZ_XREG_SIZES(DZ_XREG_CASE)
+        return true;
+#undef DZ_XREG_CASE
+        default:
+            return false;
+    }
+}
+
+void launch_xline_reg(Ham& h, float2* tb, size_t nc, hipStream_t s) {
+    const int lines = h.ny * h.nz;
+    const dim3 grid((unsigned)((lines + 255) / 256), (unsigned)nc);
+    switch (h.nx) {
+#define DZ_XREG_CASE(N)                                                                                        \
+    case N:                                                                                                    \
+        hipLaunchKernelGGL(k_xline_reg<N>, grid, dim3(256), 0, s, tb, h.plane_of_x.as<int>(), h.nplanes, lines, \
+                           h.vloc32_t.as<float>(), h.xreg_tw.as<float2>());                                    \
+        break;
+        DZ_XREG_SIZES(DZ_XREG_CASE)
+#undef DZ_XREG_CASE
+        default:
+            DZ_REQUIRE(false, DZHIP_ERR_STATE, "no register x-line kernel for this nx");
+    }
+    DZ_HIP(hipGetLastError());
 }
 
 template <typename T>
@@ -633,7 +700,8 @@ size_t Ham::device_bytes() const {
     const DevBuf* bufs[] = {&plane_start, &plane_pw, &plane_pos, &plane_xpos, &plane_ystart,
                             &plane_ycols, &plane_xabsent, &kinetic64,
                             &kinetic32, &vloc64, &vloc32, &a64, &a32, &meta, &block_rows, &dm64, &dm32,
-                            &xstage, &ystage, &x32, &y32, &tbuf, &partial, &qbuf};
+                            &xstage, &ystage, &x32, &y32, &tbuf, &partial, &qbuf,
+                            &plane_of_x, &xreg_tw, &vloc32_t};
     size_t s = tab.bytes;
     for (const DevBuf* b : bufs) s += b->bytes;
     return s;
@@ -724,6 +792,16 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
         if (plane_of_x[xx] < 0) xabsent.push_back(px[xx]);
     h.nxabsent = (int)xabsent.size();
     upload(h.plane_xabsent, xabsent);
+    upload(h.plane_of_x, plane_of_x);
+    {
+        // e^{-2 pi i k / nx} for the register x-line kernel (computed in FP64).
+        std::vector<float2> tw(h.nx);
+        for (int k = 0; k < h.nx; ++k) {
+            const double ang = -2.0 * M_PI * (double)k / (double)h.nx;
+            tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        }
+        upload(h.xreg_tw, tw);
+    }
 
     std::vector<double> kin(d.kinetic, d.kinetic + h.npw);
     std::vector<float> kin32(kin.begin(), kin.end());
@@ -829,6 +907,22 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
     if (d.local_r) ham_set_local(h, d.local_r);
 }
 
+// vt[x][z][y] = v[z][y][x]: the x-major FP32 copy the register x-line kernel reads.
+__global__ void k_local_xmajor(const float* __restrict__ v, float* __restrict__ vt, int nx, int ny, int nz) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)nx * ny * nz;
+    if (i >= n) return;
+    const size_t x = i % nx, line = i / nx;  // line = y + ny z
+    vt[x * ((size_t)ny * nz) + line] = v[i];
+}
+
+static void local_xmajor(Ham& h) {
+    h.vloc32_t.ensure(h.ngrid * sizeof(float));
+    hipLaunchKernelGGL(k_local_xmajor, dim3((unsigned)((h.ngrid + 255) / 256)), dim3(256), 0, h.stream(),
+                       h.vloc32.as<float>(), h.vloc32_t.as<float>(), h.nx, h.ny, h.nz);
+    DZ_HIP(hipGetLastError());
+}
+
 void ham_set_local(Ham& h, const double* local_r) {
     std::vector<double> v(h.ngrid);
     const double inv_n = 1.0 / (double)h.ngrid;
@@ -836,6 +930,8 @@ void ham_set_local(Ham& h, const double* local_r) {
     std::vector<float> v32(v.begin(), v.end());
     upload(h.vloc64, v);
     upload(h.vloc32, v32);
+    local_xmajor(h);
+    DZ_HIP(hipStreamSynchronize(h.stream()));
     h.has_local = true;
 }
 
@@ -854,6 +950,7 @@ void ham_set_local_dev(Ham& h, const double* local_dev) {
     hipLaunchKernelGGL(k_scale_local, dim3((unsigned)((h.ngrid + 255) / 256)), dim3(256), 0, h.stream(), local_dev,
                        1.0 / (double)h.ngrid, h.vloc64.as<double>(), h.vloc32.as<float>(), h.ngrid);
     DZ_HIP(hipGetLastError());
+    local_xmajor(h);
     h.has_local = true;
 }
 
@@ -863,7 +960,7 @@ void ham_release(Ham& h) {
                       &h.plane_ycols, &h.plane_xabsent, &h.kinetic64,
                       &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.block_rows, &h.dm64,
                       &h.dm32, &h.xstage, &h.ystage, &h.x32, &h.y32, &h.tbuf, &h.partial, &h.qbuf,
-                      &h.dens_rho, &h.dens_w};
+                      &h.dens_rho, &h.dens_w, &h.plane_of_x, &h.xreg_tw, &h.vloc32_t};
     for (DevBuf* b : bufs) b->release();
     h.tab.release();
     for (auto& set : h.ev_sets)
@@ -930,29 +1027,56 @@ void launch_xline(Ham& h, typename C2T<T>::type* tb, size_t nc, int split, hipSt
     DZ_HIP(hipGetLastError());
 }
 
-// y-slab count of the x-line kernel, chosen once per context and precision by timing every
-// divisor d of ny with ny/d >= 8 on the real shapes.  The trial launches transform tbuf in
-// place, so this runs before plane_inv writes the pass's data into it.
+// x-line variant launched by code: d >= 1 is the LDS kernel with d y-slabs, kXlineReg the
+// register kernel (FP32 only).
+constexpr int kXlineReg = -1;
+
 template <typename T>
-int xline_split(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
-    int& sp = h.xline_split[std::is_same<T, float>::value ? 1 : 0];
-    if (sp > 0) return sp;
+void launch_xline_any(Ham& h, typename C2T<T>::type* tb, size_t nc, int code, hipStream_t s) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (code == kXlineReg) {
+            launch_xline_reg(h, tb, nc, s);
+            return;
+        }
+    }
+    launch_xline<T>(h, tb, nc, code, s);
+}
+
+// x-line variant, chosen once per context and precision by timing every divisor d of ny with
+// ny/d >= 8 on the real shapes and, for FP32 with a supported nx, the register kernel.  The
+// trial launches transform tbuf in place, so this runs before plane_inv writes the pass's data
+// into it.  tune_fixed != 0 forces a variant (d, or -1 for the register kernel).
+template <typename T>
+int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
+    constexpr bool fp32 = std::is_same<T, float>::value;
+    int& sp = h.xline_split[fp32 ? 1 : 0];
+    if (sp != 0) return sp;
+    const bool reg_ok = fp32 && xline_reg_supported(h.nx);
+    if (h.tune_fixed) {
+        DZ_REQUIRE(h.tune_fixed != kXlineReg || xline_reg_supported(h.nx), DZHIP_ERR_ARG,
+                   "no register x-line kernel for this nx");
+        // The register kernel exists for FP32 only; FP64 then keeps one slab.
+        sp = h.tune_fixed == kXlineReg && !fp32 ? 1 : h.tune_fixed;
+        return sp;
+    }
     std::vector<int> cand;
+    if (reg_ok) cand.push_back(kXlineReg);
     for (int d = 1; d <= 12; ++d)
         if (h.ny % d == 0 && h.ny / d >= 8) cand.push_back(d);
-    if (cand.size() <= 1 || h.tune_fixed) {
-        sp = h.tune_fixed ? h.tune_fixed : 1;
+    if (cand.empty()) cand.push_back(1);
+    if (cand.size() == 1) {
+        sp = cand[0];
         return sp;
     }
     hipEvent_t e0, e1;
     DZ_HIP(hipEventCreate(&e0));
     DZ_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    int best_d = 1;
+    int best_d = cand[0];
     for (int d : cand) {
-        launch_xline<T>(h, tb, nc, d, s);
+        launch_xline_any<T>(h, tb, nc, d, s);
         DZ_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < 3; ++r) launch_xline<T>(h, tb, nc, d, s);
+        for (int r = 0; r < 3; ++r) launch_xline_any<T>(h, tb, nc, d, s);
         DZ_HIP(hipEventRecord(e1, s));
         DZ_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -1028,7 +1152,7 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
             h.tbuf.ensure(nc * h.nplanes * plane * sizeof(C));
             C* tb = h.tbuf.as<C>();
             // Tuned before plane_inv fills tbuf: the trial launches transform tbuf in place.
-            const int split = xline_split<T>(h, tb, nc, s);
+            const int split = xline_variant<T>(h, tb, nc, s);
             const size_t lds_p = plane_lds_bytes<C>(h.ny, h.nz);
             constexpr int nt = TileThreads<T>::value;
             set_max_dynamic_lds((const void*)k_plane_inv<T>, lds_p);
@@ -1039,7 +1163,7 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
                                h.ny, h.nz, h.nplanes, h.tab.ax[1], h.tab.ax[2], tw);
             DZ_HIP(hipGetLastError());
             mark(h, evs, 3);
-            launch_xline<T>(h, tb, nc, split, s);
+            launch_xline_any<T>(h, tb, nc, split, s);
             mark(h, evs, 4);
             hipLaunchKernelGGL(k_plane_fwd<T>, dim3(h.nplanes, (unsigned)nc), dim3(nt), lds_p, s,
                                tb, xc, ld, yc, ld, Sel<T>::kin(h), h.plane_start.as<int>(),

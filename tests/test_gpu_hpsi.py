@@ -181,7 +181,30 @@ def test_hpsi_every_xline_split(dz, ctx, prec):
         ham.set_tuning(d)
         errs[d] = relerr(ham.apply(w.psi, precision=prec), g["y"])
         assert ham.info()["xline_split_fp32" if prec else "xline_split_fp64"] == d
+    # -1: the register-resident x-line kernel (FP32; FP64 keeps one slab).
+    ham.set_tuning(-1)
+    errs["register"] = relerr(ham.apply(w.psi, precision=prec), g["y"])
+    assert ham.info()["xline_register_fp32"] == (1 if prec else 0)
     ham.set_tuning(0)
     errs["auto"] = relerr(ham.apply(w.psi, precision=prec), g["y"])
     print(prec, errs, ham.info())
     assert all(e < tol for e in errs.values()), errs
+
+
+@pytest.mark.parametrize("which", ["tiny", "si8", "bench"])
+def test_hpsi_register_xline_vs_oracle(dz, ctx, which):
+    """The register-resident FP32 x-line kernel (one x-line per thread, Stockham in VGPRs) on
+    grids of different factorisations against the FP64 oracle apply, FP32 tolerance."""
+    w = {"tiny": workloads.tiny, "si8": workloads.si8, "bench": workloads.bench}[which]()
+    ham = make_ham(dz, ctx, w)
+    try:
+        ham.set_tuning(-1)
+        nb = min(w.nbands, 8)
+        got = ham.apply(w.psi[:nb], precision=1)
+        assert ham.info()["xline_register_fp32"] == 1
+        ref = hamiltonian.apply_hamiltonian_batched(oracle_ctx(w), w.psi[:nb])
+        err = relerr(got, ref)
+        print(which, w.grid, err)
+        assert err < TOL32
+    finally:
+        ham.close()
