@@ -660,9 +660,9 @@ int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split) {
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
         DZ_REQUIRE(xline_split == 0 || (xline_split > 0 && ham->ny % xline_split == 0 && ham->ny / xline_split >= 1) ||
-                       (xline_split == -1 && xline_reg_supported(ham->nx)),
+                       (xline_split >= -3 && xline_split <= -1 && xline_reg_usable(*ham)),
                    DZHIP_ERR_ARG,
-                   "xline_split must divide ny, or be -1 with a register kernel for nx (0 = autotune)");
+                   "xline_split must divide ny, or be -1..-3 with a register kernel for this context (0 = autotune)");
         std::lock_guard<std::mutex> lk(ham->mu);
         ham->tune_fixed = xline_split;
         ham->xline_split[0] = ham->xline_split[1] = 0;
@@ -675,7 +675,7 @@ int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out) {
         auto slabs = [](int code) { return code > 0 ? (uint64_t)code : 0; };
         const uint64_t v[8] = {ham->npw, ham->ngrid, (uint64_t)ham->nplanes, (uint64_t)ham->nproj,
                                ham->device_bytes(), slabs(ham->xline_split[1]), slabs(ham->xline_split[0]),
-                               ham->xline_split[1] == -1 ? 1u : 0u};
+                               ham->xline_split[1] < 0 ? (uint64_t)(256 >> (-ham->xline_split[1] - 1)) : 0u};
         for (uint64_t i = 0; i < n_out && i < 8; ++i) out[i] = v[i];
     });
 }

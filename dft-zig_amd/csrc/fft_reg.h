@@ -92,7 +92,7 @@ __device__ __forceinline__ void butterfly(C (&v)[R]) {
     }
 }
 
-template <typename C, int N, int R, int Ns, int S>
+template <typename C, int N, int R, int Ns, int S, int TS>
 __device__ __forceinline__ void stage(const C (&in)[N], C (&out)[N], const C* __restrict__ tw) {
 #pragma unroll
     for (int j = 0; j < N / R; ++j) {
@@ -104,7 +104,7 @@ __device__ __forceinline__ void stage(const C (&in)[N], C (&out)[N], const C* __
 #pragma unroll
             for (int r = 1; r < R; ++r) {
                 const int e = (r * k * (N / (Ns * R))) % N;
-                C w = tw[e];
+                C w = tw[e * TS];
                 if (S > 0) w.y = -w.y;
                 v[r] = cmul(v[r], w);
             }
@@ -116,23 +116,24 @@ __device__ __forceinline__ void stage(const C (&in)[N], C (&out)[N], const C* __
     }
 }
 
-template <typename C, int N, int S, int K, int Ns>
+template <typename C, int N, int S, int K, int Ns, int TS>
 __device__ __forceinline__ void stages(C (&a)[N], const C* __restrict__ tw) {
     if constexpr (K < num_stages(N)) {
         constexpr int R = radix_at(N, K);
         C b[N];
-        stage<C, N, R, Ns, S>(a, b, tw);
+        stage<C, N, R, Ns, S, TS>(a, b, tw);
 #pragma unroll
         for (int i = 0; i < N; ++i) a[i] = b[i];
-        stages<C, N, S, K + 1, Ns * R>(a, tw);
+        stages<C, N, S, K + 1, Ns * R, TS>(a, tw);
     }
 }
 
-// a <- DFT_S(a); tw[k] = e^{-2 pi i k / N}, k < N.
-template <int N, int S, typename C>
+// a <- DFT_S(a); tw[k * TS] = e^{-2 pi i k / N}, k < N (TS > 1 reads the table of a multiple
+// length N TS).
+template <int N, int S, int TS = 1, typename C>
 __device__ __forceinline__ void fft(C (&a)[N], const C* __restrict__ tw) {
     static_assert(supported(N), "line length must be 2^a 3^b 5^c");
-    stages<C, N, S, 0, 1>(a, tw);
+    stages<C, N, S, 0, 1, TS>(a, tw);
 }
 
 }  // namespace freg

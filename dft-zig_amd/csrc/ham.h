@@ -49,9 +49,11 @@ struct Ham {
 
     DevBuf kinetic64, kinetic32;
     DevBuf vloc64, vloc32;  // V_loc(r)/N in Ry
-    // Register x-line kernel: plane index of each x (-1 = no sphere plane), e^{-2 pi i k/nx}
-    // (FP32), and V_loc/N in x-major order [x][z][y] (FP32).
-    DevBuf plane_of_x, xreg_tw, vloc32_t;
+    // Register x-line kernel: the sphere's x-planes as the cyclic run starting at plane_xs
+    // (-1: not a run) with plane_xw planes wrapped past nx, e^{-2 pi i k/nx} (FP32), and V_loc/N
+    // in x-major order [x][z][y] (FP32).
+    int plane_xs = -1, plane_xw = 0;
+    DevBuf xreg_tw, vloc32_t;
     bool has_local = false;
 
     // Nonlocal: complex projector matrix A[p][G] = phi(G) e^{iG.tau_a} (P x n_pw), D tables.
@@ -143,8 +145,10 @@ const double2* eig_solution(const Ham& h, uint64_t* cols);
 size_t eig_training_samples(Ham& h, double energy_scale, int base, int stride, float* out_host, size_t cap);
 
 void ham_build(Ham& h, const dzhip_ham_desc& d);
-// True when the register-resident FP32 x-line kernel is built for line length nx.
+// True when the register-resident FP32 x-line kernel is built for line length nx, and when
+// it can run on this context (its x-planes form a cyclic run).
 bool xline_reg_supported(int nx);
+bool xline_reg_usable(const Ham& h);
 void ham_set_local(Ham& h, const double* local_r);
 // local_r already on the device (read on h's stream).
 void ham_set_local_dev(Ham& h, const double* local_dev);

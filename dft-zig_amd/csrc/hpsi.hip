@@ -165,19 +165,27 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
 // thread l = z ny + y of band blockIdx.y loads its values of the np sphere planes (consecutive
 // threads read consecutive addresses of each plane), runs the inverse x-transform, multiplies by
 // V_loc/N from the x-major copy vt[x][z][y], runs the forward transform and stores the planes
-// back.  No LDS and no barrier; absent x-planes enter as zeros.
+// back.  No LDS and no barrier; absent x-planes enter as zeros.  The sphere's x-planes are the
+// cyclic run x = xs, xs + 1, ... (np of them, mod NX), numbered in increasing x, so the plane of
+// x is x for x < xw (the part that wrapped past NX) and x - xs + xw for xs <= x < xs + np - xw:
+// scalar arithmetic on kernel arguments, with no table load in front of each row's load.
 template <int NX>
-__global__ __launch_bounds__(256) void k_xline_reg(float2* __restrict__ tbuf, const int* __restrict__ plane_of_x,
-                                                   int nplanes, int lines, const float* __restrict__ vt,
+__global__ __launch_bounds__(256) void k_xline_reg(float2* __restrict__ tbuf, int nplanes, int xs, int xw,
+                                                   int lines, const float* __restrict__ vt,
                                                    const float2* __restrict__ tw) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= lines) return;
     float2* t = tbuf + (size_t)blockIdx.y * nplanes * lines + l;
+    const int hi_end = xs + nplanes - xw;
     float2 a[NX];
+    // Every row is loaded (absent ones from plane 0, a cache hit) and masked, so the loads issue
+    // back to back with no branch between them.
 #pragma unroll
     for (int x = 0; x < NX; ++x) {
-        const int p = plane_of_x[x];
-        a[x] = p >= 0 ? t[(size_t)p * lines] : make_float2(0.f, 0.f);
+        const bool lo = x < xw, hi = x >= xs && x < hi_end;
+        const int p = lo ? x : (hi ? x - xs + xw : 0);
+        const float2 v = t[(size_t)p * lines];
+        a[x] = (lo || hi) ? v : make_float2(0.f, 0.f);
     }
     freg::fft<NX, 1>(a, tw);
 #pragma unroll
@@ -188,8 +196,9 @@ __global__ __launch_bounds__(256) void k_xline_reg(float2* __restrict__ tbuf, co
     freg::fft<NX, -1>(a, tw);
 #pragma unroll
     for (int x = 0; x < NX; ++x) {
-        const int p = plane_of_x[x];
-        if (p >= 0) t[(size_t)p * lines] = a[x];
+        const bool lo = x < xw, hi = x >= xs && x < hi_end;
+        const int p = lo ? x : x - xs + xw;
+        if (lo || hi) t[(size_t)p * lines] = a[x];
     }
 }
 
@@ -208,13 +217,16 @@ bool xline_reg_supported(int nx) {
     }
 }
 
-void launch_xline_reg(Ham& h, float2* tb, size_t nc, hipStream_t s) {
+bool xline_reg_usable(const Ham& h) { return xline_reg_supported(h.nx) && h.plane_xs >= 0; }
+
+void launch_xline_reg(Ham& h, float2* tb, size_t nc, int threads, hipStream_t s) {
     const int lines = h.ny * h.nz;
-    const dim3 grid((unsigned)((lines + 255) / 256), (unsigned)nc);
+    const dim3 grid((unsigned)((lines + threads - 1) / threads), (unsigned)nc);
     switch (h.nx) {
 #define DZ_XREG_CASE(N)                                                                                        \
     case N:                                                                                                    \
-        hipLaunchKernelGGL(k_xline_reg<N>, grid, dim3(256), 0, s, tb, h.plane_of_x.as<int>(), h.nplanes, lines, \
+        hipLaunchKernelGGL(k_xline_reg<N>, grid, dim3(threads), 0, s, tb, h.nplanes, h.plane_xs, h.plane_xw,      \
+                           lines,                                                                              \
                            h.vloc32_t.as<float>(), h.xreg_tw.as<float2>());                                    \
         break;
         DZ_XREG_SIZES(DZ_XREG_CASE)
@@ -701,7 +713,7 @@ size_t Ham::device_bytes() const {
                             &plane_ycols, &plane_xabsent, &kinetic64,
                             &kinetic32, &vloc64, &vloc32, &a64, &a32, &meta, &block_rows, &dm64, &dm32,
                             &xstage, &ystage, &x32, &y32, &tbuf, &partial, &qbuf,
-                            &plane_of_x, &xreg_tw, &vloc32_t};
+                            &xreg_tw, &vloc32_t};
     size_t s = tab.bytes;
     for (const DevBuf* b : bufs) s += b->bytes;
     return s;
@@ -792,7 +804,23 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
         if (plane_of_x[xx] < 0) xabsent.push_back(px[xx]);
     h.nxabsent = (int)xabsent.size();
     upload(h.plane_xabsent, xabsent);
-    upload(h.plane_of_x, plane_of_x);
+    {
+        // The sphere's x-planes as a cyclic run xs, xs + 1, ... (mod nx) numbered in increasing x:
+        // xw = planes that wrapped past nx.  plane_xs = -1 when the planes are not such a run.
+        int x0 = -1;
+        for (int xx = 0; xx < h.nx && x0 < 0; ++xx)
+            if (plane_of_x[xx] >= 0 && plane_of_x[(xx + h.nx - 1) % h.nx] < 0) x0 = xx;
+        if (h.nplanes == h.nx) x0 = 0;
+        const int xw = x0 < 0 ? 0 : std::max(0, x0 + h.nplanes - h.nx);
+        bool ok = x0 >= 0;
+        for (int xx = 0; xx < h.nx && ok; ++xx) {
+            const bool lo = xx < xw, hi = xx >= x0 && xx < x0 + h.nplanes - xw;
+            const int want = lo ? xx : (hi ? xx - x0 + xw : -1);
+            ok = plane_of_x[xx] == want;
+        }
+        h.plane_xs = ok ? x0 : -1;
+        h.plane_xw = ok ? xw : 0;
+    }
     {
         // e^{-2 pi i k / nx} for the register x-line kernel (computed in FP64).
         std::vector<float2> tw(h.nx);
@@ -960,7 +988,7 @@ void ham_release(Ham& h) {
                       &h.plane_ycols, &h.plane_xabsent, &h.kinetic64,
                       &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.block_rows, &h.dm64,
                       &h.dm32, &h.xstage, &h.ystage, &h.x32, &h.y32, &h.tbuf, &h.partial, &h.qbuf,
-                      &h.dens_rho, &h.dens_w, &h.plane_of_x, &h.xreg_tw, &h.vloc32_t};
+                      &h.dens_rho, &h.dens_w, &h.xreg_tw, &h.vloc32_t};
     for (DevBuf* b : bufs) b->release();
     h.tab.release();
     for (auto& set : h.ev_sets)
@@ -1027,15 +1055,16 @@ void launch_xline(Ham& h, typename C2T<T>::type* tb, size_t nc, int split, hipSt
     DZ_HIP(hipGetLastError());
 }
 
-// x-line variant launched by code: d >= 1 is the LDS kernel with d y-slabs, kXlineReg the
-// register kernel (FP32 only).
-constexpr int kXlineReg = -1;
+// x-line variant launched by code: d >= 1 is the LDS kernel with d y-slabs, -1 / -2 / -3 the
+// register kernel (FP32 only) in workgroups of 256 / 128 / 64 threads.
+constexpr int kXlineReg = -1, kXlineRegLast = -3;
+constexpr int xline_reg_threads(int code) { return 256 >> (-code - 1); }
 
 template <typename T>
 void launch_xline_any(Ham& h, typename C2T<T>::type* tb, size_t nc, int code, hipStream_t s) {
     if constexpr (std::is_same<T, float>::value) {
-        if (code == kXlineReg) {
-            launch_xline_reg(h, tb, nc, s);
+        if (code <= kXlineReg && code >= kXlineRegLast) {
+            launch_xline_reg(h, tb, nc, xline_reg_threads(code), s);
             return;
         }
     }
@@ -1051,16 +1080,17 @@ int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
     constexpr bool fp32 = std::is_same<T, float>::value;
     int& sp = h.xline_split[fp32 ? 1 : 0];
     if (sp != 0) return sp;
-    const bool reg_ok = fp32 && xline_reg_supported(h.nx);
+    const bool reg_ok = fp32 && xline_reg_usable(h);
     if (h.tune_fixed) {
-        DZ_REQUIRE(h.tune_fixed != kXlineReg || xline_reg_supported(h.nx), DZHIP_ERR_ARG,
+        DZ_REQUIRE(h.tune_fixed > 0 || xline_reg_usable(h), DZHIP_ERR_ARG,
                    "no register x-line kernel for this nx");
         // The register kernel exists for FP32 only; FP64 then keeps one slab.
-        sp = h.tune_fixed == kXlineReg && !fp32 ? 1 : h.tune_fixed;
+        sp = h.tune_fixed < 0 && !fp32 ? 1 : h.tune_fixed;
         return sp;
     }
     std::vector<int> cand;
-    if (reg_ok) cand.push_back(kXlineReg);
+    if (reg_ok)
+        for (int c = kXlineReg; c >= kXlineRegLast; --c) cand.push_back(c);
     for (int d = 1; d <= 12; ++d)
         if (h.ny % d == 0 && h.ny / d >= 8) cand.push_back(d);
     if (cand.empty()) cand.push_back(1);

@@ -182,29 +182,36 @@ def test_hpsi_every_xline_split(dz, ctx, prec):
         errs[d] = relerr(ham.apply(w.psi, precision=prec), g["y"])
         assert ham.info()["xline_split_fp32" if prec else "xline_split_fp64"] == d
     # -1: the register-resident x-line kernel (FP32; FP64 keeps one slab).
-    ham.set_tuning(-1)
-    errs["register"] = relerr(ham.apply(w.psi, precision=prec), g["y"])
-    assert ham.info()["xline_register_fp32"] == (1 if prec else 0)
+    for v, threads in ((-1, 256), (-2, 128), (-3, 64)):
+        ham.set_tuning(v)
+        errs[f"register{threads}"] = relerr(ham.apply(w.psi, precision=prec), g["y"])
+        assert ham.info()["xline_register_fp32"] == (threads if prec else 0)
     ham.set_tuning(0)
     errs["auto"] = relerr(ham.apply(w.psi, precision=prec), g["y"])
     print(prec, errs, ham.info())
     assert all(e < tol for e in errs.values()), errs
 
 
-@pytest.mark.parametrize("which", ["tiny", "si8", "bench"])
+@pytest.mark.parametrize("which", ["tiny", "si8", "bench", "bench75", "bench90", "bench80"])
 def test_hpsi_register_xline_vs_oracle(dz, ctx, which):
     """The register-resident FP32 x-line kernel (one x-line per thread, Stockham in VGPRs) on
-    grids of different factorisations against the FP64 oracle apply, FP32 tolerance."""
-    w = {"tiny": workloads.tiny, "si8": workloads.si8, "bench": workloads.bench}[which]()
+    grids of different factorisations (nx = 8, 32, 72, 75, 80, 90; radices 4/2/3/5) against the
+    FP64 oracle apply, FP32 tolerance."""
+    variant = -1
+    make = {"tiny": workloads.tiny, "si8": workloads.si8, "bench": workloads.bench,
+            "bench75": lambda: workloads.bench(nbands=8, grid=(75, 75, 75)),
+            "bench80": lambda: workloads.bench(nbands=8, grid=(80, 72, 75)),
+            "bench90": lambda: workloads.bench(nbands=8, grid=(90, 90, 90))}
+    w = make[which]()
     ham = make_ham(dz, ctx, w)
     try:
-        ham.set_tuning(-1)
+        ham.set_tuning(variant)
         nb = min(w.nbands, 8)
         got = ham.apply(w.psi[:nb], precision=1)
-        assert ham.info()["xline_register_fp32"] == 1
+        assert ham.info()["xline_register_fp32"] == 256
         ref = hamiltonian.apply_hamiltonian_batched(oracle_ctx(w), w.psi[:nb])
         err = relerr(got, ref)
-        print(which, w.grid, err)
+        print(which, variant, w.grid, err)
         assert err < TOL32
     finally:
         ham.close()
