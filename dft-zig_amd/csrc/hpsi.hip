@@ -162,7 +162,7 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
 }
 
 // x-line pass with one x-line per thread held in registers (FP32, nx fixed at compile time):
-// thread l = z ny + y of band blockIdx.y loads its values of the np sphere planes (consecutive
+// thread l = z ny + y of band blockIdx.x loads its values of the np sphere planes (consecutive
 // threads read consecutive addresses of each plane), runs the inverse x-transform, multiplies by
 // V_loc/N from the x-major copy vt[x][z][y], runs the forward transform and stores the planes
 // back.  No LDS and no barrier; absent x-planes enter as zeros.  The sphere's x-planes are the
@@ -173,9 +173,11 @@ template <int NX>
 __global__ __launch_bounds__(256) void k_xline_reg(float2* __restrict__ tbuf, int nplanes, int xs, int xw,
                                                    int lines, const float* __restrict__ vt,
                                                    const float2* __restrict__ tw) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    // Bands run fastest over the grid (blockIdx.x), so the workgroups that follow each other
+    // read the same V_loc lines while they are still in cache.
+    const int l = blockIdx.y * blockDim.x + threadIdx.x;
     if (l >= lines) return;
-    float2* t = tbuf + (size_t)blockIdx.y * nplanes * lines + l;
+    float2* t = tbuf + (size_t)blockIdx.x * nplanes * lines + l;
     const int hi_end = xs + nplanes - xw;
     float2 a[NX];
     // Every row is loaded (absent ones from plane 0, a cache hit) and masked, so the loads issue
@@ -221,7 +223,7 @@ bool xline_reg_usable(const Ham& h) { return xline_reg_supported(h.nx) && h.plan
 
 void launch_xline_reg(Ham& h, float2* tb, size_t nc, int threads, hipStream_t s) {
     const int lines = h.ny * h.nz;
-    const dim3 grid((unsigned)((lines + threads - 1) / threads), (unsigned)nc);
+    const dim3 grid((unsigned)nc, (unsigned)((lines + threads - 1) / threads));
     switch (h.nx) {
 #define DZ_XREG_CASE(N)                                                                                        \
     case N:                                                                                                    \
