@@ -501,13 +501,17 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::t
         const int e = tid + i * kNlThreads;                                                   \
         if (e < kNlP * kNlK) as[(e >> 5) * kNlPitch + (e & 31)] = C{are[i], aim[i]};         \
     }
-    typename M::acc accr[2][PB], acci[2][PB];
+    // Complex products by three real MFMAs (Gauss): with x = xr + i xi, a = ar + i ai,
+    // acc1 = sum xr ar, acc2 = sum xi ai, acc3 = sum (xr + xi)(ar + ai);
+    // Re = acc1 - acc2, Im = acc3 - acc1 - acc2.
+    typename M::acc acc1[2][PB], acc2[2][PB], acc3[2][PB];
 #pragma unroll
     for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
         for (int pb = 0; pb < PB; ++pb) {
-            accr[bb][pb] = typename M::acc{0, 0, 0, 0};
-            acci[bb][pb] = typename M::acc{0, 0, 0, 0};
+            acc1[bb][pb] = typename M::acc{0, 0, 0, 0};
+            acc2[bb][pb] = typename M::acc{0, 0, 0, 0};
+            acc3[bb][pb] = typename M::acc{0, 0, 0, 0};
         }
     if (k0 < k1) {
         DZ_PROJ_FETCH(k0)
@@ -526,15 +530,18 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::t
             for (int bb = 0; bb < 2; ++bb) xa[bb] = xs[(bg * 32 + bb * 16 + i16) * kNlPitch + kk + k4];
 #pragma unroll
             for (int pb = 0; pb < PB; ++pb) ab[pb] = as[((half * PB + pb) * 16 + i16) * kNlPitch + kk + k4];
+            T xsum[2], asum[PB];
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) xsum[bb] = xa[bb].x + xa[bb].y;
+#pragma unroll
+            for (int pb = 0; pb < PB; ++pb) asum[pb] = ab[pb].x + ab[pb].y;
 #pragma unroll
             for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
                 for (int pb = 0; pb < PB; ++pb) {
-                    // (xr + i xi)(ar + i ai)
-                    accr[bb][pb] = M::mma(xa[bb].x, ab[pb].x, accr[bb][pb]);
-                    accr[bb][pb] = M::mma(-xa[bb].y, ab[pb].y, accr[bb][pb]);
-                    acci[bb][pb] = M::mma(xa[bb].x, ab[pb].y, acci[bb][pb]);
-                    acci[bb][pb] = M::mma(xa[bb].y, ab[pb].x, acci[bb][pb]);
+                    acc1[bb][pb] = M::mma(xa[bb].x, ab[pb].x, acc1[bb][pb]);
+                    acc2[bb][pb] = M::mma(xa[bb].y, ab[pb].y, acc2[bb][pb]);
+                    acc3[bb][pb] = M::mma(xsum[bb], asum[pb], acc3[bb][pb]);
                 }
         }
         __syncthreads();
@@ -553,8 +560,9 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::t
             for (int r = 0; r < 4; ++r) {
                 const int b = b0 + bg * 32 + bb * 16 + M::row(k4, r);
                 const int p = p0 + (half * PB + pb) * 16 + i16;
+                const T c1 = acc1[bb][pb][r], c2 = acc2[bb][pb][r];
                 if (b < ncols && p < P)
-                    partial[((size_t)ks * ncols + b) * P + p] = C{accr[bb][pb][r], acci[bb][pb][r]};
+                    partial[((size_t)ks * ncols + b) * P + p] = C{c1 - c2, acc3[bb][pb][r] - c1 - c2};
             }
 }
 
@@ -643,13 +651,16 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::t
         const int e = tid + i * kNlThreads;                                                   \
         if (e < kNlK * kNlG) as[(e / kNlG) * kNlGPitch + (e % kNlG)] = C{are[i], aim[i]};   \
     }
-    typename M::acc accr[2][GB], acci[2][GB];
+    // q conj(a) by three real MFMAs (Gauss) with a' = ar - i ai: acc1 = sum qr ar,
+    // acc2 = sum qi (-ai), acc3 = sum (qr + qi)(ar - ai); Re = acc1 - acc2, Im = acc3 - acc1 - acc2.
+    typename M::acc acc1[2][GB], acc2[2][GB], acc3[2][GB];
 #pragma unroll
     for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
         for (int gb = 0; gb < GB; ++gb) {
-            accr[bb][gb] = typename M::acc{0, 0, 0, 0};
-            acci[bb][gb] = typename M::acc{0, 0, 0, 0};
+            acc1[bb][gb] = typename M::acc{0, 0, 0, 0};
+            acc2[bb][gb] = typename M::acc{0, 0, 0, 0};
+            acc3[bb][gb] = typename M::acc{0, 0, 0, 0};
         }
     DZ_BACK_FETCH(0)
     DZ_BACK_STASH()
@@ -666,15 +677,18 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::t
             for (int bb = 0; bb < 2; ++bb) qa[bb] = qs[(bg * 32 + bb * 16 + i16) * kNlPitch + kk + k4];
 #pragma unroll
             for (int gb = 0; gb < GB; ++gb) ab[gb] = as[(kk + k4) * kNlGPitch + (half * GB + gb) * 16 + i16];
+            T qsum[2], adif[GB];
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) qsum[bb] = qa[bb].x + qa[bb].y;
+#pragma unroll
+            for (int gb = 0; gb < GB; ++gb) adif[gb] = ab[gb].x - ab[gb].y;
 #pragma unroll
             for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
                 for (int gb = 0; gb < GB; ++gb) {
-                    // (qr + i qi)(ar - i ai)
-                    accr[bb][gb] = M::mma(qa[bb].x, ab[gb].x, accr[bb][gb]);
-                    accr[bb][gb] = M::mma(qa[bb].y, ab[gb].y, accr[bb][gb]);
-                    acci[bb][gb] = M::mma(qa[bb].y, ab[gb].x, acci[bb][gb]);
-                    acci[bb][gb] = M::mma(-qa[bb].x, ab[gb].y, acci[bb][gb]);
+                    acc1[bb][gb] = M::mma(qa[bb].x, ab[gb].x, acc1[bb][gb]);
+                    acc2[bb][gb] = M::mma(qa[bb].y, -ab[gb].y, acc2[bb][gb]);
+                    acc3[bb][gb] = M::mma(qsum[bb], adif[gb], acc3[bb][gb]);
                 }
         }
         __syncthreads();
@@ -693,7 +707,8 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::t
             for (int r = 0; r < 4; ++r) {
                 const int b = b0 + bg * 32 + bb * 16 + M::row(k4, r);
                 const int g = g0 + (half * GB + gb) * 16 + i16;
-                if (b < ncols && g < npw) y[(size_t)b * ldy + g] = C{accr[bb][gb][r], acci[bb][gb][r]};
+                const T c1 = acc1[bb][gb][r], c2 = acc2[bb][gb][r];
+                if (b < ncols && g < npw) y[(size_t)b * ldy + g] = C{c1 - c2, acc3[bb][gb][r] - c1 - c2};
             }
 }
 
