@@ -567,28 +567,56 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::t
 }
 
 // One workgroup per band: sum the split-K partials of every projector into LDS (each partial
-// read once, coalesced over p), then q[b][p] = inv_vol * sum_r D[m][r] * sum[base + r] within
-// the projector's atom block.
+// read once, coalesced over p; `split` interleaved subsets of the K-chunks per projector so
+// more loads are in flight, combined in a fixed order), then
+// q[b][p] = inv_vol * sum_r D[m][r] * sum[base + r] within the projector's atom block.
+constexpr int kDijSplit = 4;
+constexpr int kDijThreads = 1024;
+// K-chunk subsets the D kernel can stage for P projectors of element size esz, and its LDS bytes.
+inline int dij_split(int P, size_t esz) {
+    for (int sp = kDijSplit; sp > 1; --sp)
+        if ((size_t)(sp + 1) * P * esz <= kMaxLdsBytes) return sp;
+    return 1;
+}
+inline size_t dij_lds_bytes(int P, size_t esz) {
+    const int sp = dij_split(P, esz);
+    return (size_t)(sp == 1 ? 1 : sp + 1) * P * esz;
+}
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_nl_dij(const typename C2T<T>::type* __restrict__ partial,
-                                                int nks, typename C2T<T>::type* __restrict__ q,
-                                                const ProjMeta* __restrict__ meta,
-                                                const T* __restrict__ dm, int P, int ncols, T inv_vol) {
+__global__ __launch_bounds__(kDijThreads) void k_nl_dij(const typename C2T<T>::type* __restrict__ partial,
+                                                        int nks, typename C2T<T>::type* __restrict__ q,
+                                                        const ProjMeta* __restrict__ meta,
+                                                        const T* __restrict__ dm, int P, int ncols, T inv_vol,
+                                                        int split) {
     using C = typename C2T<T>::type;
     extern __shared__ unsigned char smem_raw[];
-    C* sum = reinterpret_cast<C*>(smem_raw);  // [P]
+    C* part = reinterpret_cast<C*>(smem_raw);                     // [split][P]
+    C* sum = split == 1 ? part : part + (size_t)split * P;       // [P]
     const int b = blockIdx.x;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    for (int p = tid; p < P; p += nthr) {
+    for (int idx = tid; idx < split * P; idx += nthr) {
+        const int j = idx / P, p = idx - j * P;
         C s = C{T(0), T(0)};
-        for (int ks = 0; ks < nks; ++ks) {
+        for (int ks = j; ks < nks; ks += split) {
             const C v = partial[((size_t)ks * ncols + b) * P + p];
             s.x += v.x;
             s.y += v.y;
         }
-        sum[p] = s;
+        part[(size_t)j * P + p] = s;
     }
     __syncthreads();
+    if (split > 1) {
+        for (int p = tid; p < P; p += nthr) {
+            C s = part[p];
+            for (int j = 1; j < split; ++j) {
+                s.x += part[(size_t)j * P + p].x;
+                s.y += part[(size_t)j * P + p].y;
+            }
+            sum[p] = s;
+        }
+        __syncthreads();
+    }
     for (int p = tid; p < P; p += nthr) {
         const ProjMeta mt = meta[p];
         const T* drow = dm + mt.dm_off + mt.m * mt.mt;
@@ -1175,11 +1203,11 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
                                Sel<T>::a(h), xc, ld, h.partial.as<C>(), h.nproj, (int)h.npw, (int)nc,
                                kchunk, ptiles);
             DZ_HIP(hipGetLastError());
-            const size_t lds_d = (size_t)h.nproj * sizeof(C);
+            const size_t lds_d = dij_lds_bytes(h.nproj, sizeof(C));
             set_max_dynamic_lds((const void*)k_nl_dij<T>, lds_d);
-            hipLaunchKernelGGL(k_nl_dij<T>, dim3((unsigned)nc), dim3(256), lds_d, s, h.partial.as<C>(), nks,
+            hipLaunchKernelGGL(k_nl_dij<T>, dim3((unsigned)nc), dim3(kDijThreads), lds_d, s, h.partial.as<C>(), nks,
                                h.qbuf.as<C>(), h.meta.as<ProjMeta>(), Sel<T>::dm(h), h.nproj, (int)nc,
-                               (T)h.inv_volume);
+                               (T)h.inv_volume, dij_split(h.nproj, sizeof(C)));
             DZ_HIP(hipGetLastError());
         }
         mark(h, evs, 1);
