@@ -22,8 +22,11 @@
 // device; proj = A x (split-K), q = D proj / Omega per atom block, y = A^H q (before the
 // local part, which then accumulates into y).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include <type_traits>
 #include <vector>
@@ -1116,10 +1119,20 @@ void launch_xline_any(Ham& h, typename C2T<T>::type* tb, size_t nc, int code, hi
     launch_xline<T>(h, tb, nc, code, s);
 }
 
-// x-line variant, chosen once per context and precision by timing every divisor d of ny with
-// ny/d >= 8 on the real shapes and, for FP32 with a supported nx, the register kernel.  The
-// trial launches transform tbuf in place, so this runs before plane_inv writes the pass's data
-// into it.  tune_fixed != 0 forces a variant (d, or -1 for the register kernel).
+// Process-wide x-line tuning results by (device, grid, sphere x-run, precision): contexts of the
+// same shape (the k-points of one SCF, solved concurrently) share one tuning, and the mutex keeps
+// the other contexts' work off the GPU while the trials are timed.
+std::mutex g_xline_tune_mu;
+std::map<std::array<int, 8>, int> g_xline_tune;
+
+// Below this many columns a pass is launch-bound and its timings would not carry over.
+constexpr size_t kXlineTuneMinCols = 16;
+
+// x-line variant, chosen once per shape and precision by timing every divisor d of ny with
+// ny/d >= 8 on the real shapes and, for FP32 with a supported nx, the register kernel in three
+// workgroup sizes.  The trial launches transform tbuf in place, so this runs before plane_inv
+// writes the pass's data into it.  tune_fixed != 0 forces a variant (d, or -1..-3 for the
+// register kernel).
 template <typename T>
 int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
     constexpr bool fp32 = std::is_same<T, float>::value;
@@ -1143,6 +1156,15 @@ int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
         sp = cand[0];
         return sp;
     }
+    // Narrow passes run the first candidate without caching it; the next wide pass tunes.
+    if (nc < kXlineTuneMinCols) return cand[0];
+    const std::array<int, 8> key = {h.ctx->device, h.nx, h.ny, h.nz, h.nplanes, h.plane_xs, h.plane_xw, fp32 ? 1 : 0};
+    std::lock_guard<std::mutex> lk(g_xline_tune_mu);
+    const auto it = g_xline_tune.find(key);
+    if (it != g_xline_tune.end()) {
+        sp = it->second;
+        return sp;
+    }
     hipEvent_t e0, e1;
     DZ_HIP(hipEventCreate(&e0));
     DZ_HIP(hipEventCreate(&e1));
@@ -1164,6 +1186,7 @@ int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     sp = best_d;
+    g_xline_tune[key] = best_d;
     return sp;
 }
 

@@ -337,7 +337,8 @@ int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t c
  * kernel with one x-line per thread (nx = 2^a 3^b 5^c, 8 <= nx <= 128) in workgroups of
  * 256 / 128 / 64 threads; FP64 then keeps one slab.  0 (default) times every divisor of ny with
  * at least 8 lines per slab, and the register kernel's shapes where it exists, on the first
- * apply of each precision and keeps the fastest. */
+ * apply of each precision with at least 16 columns and keeps the fastest; the result is shared
+ * by every context of the same device, grid and sphere shape in the process. */
 int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split);
 
 /* Sizes the library uses for this context (for roofline accounting):
