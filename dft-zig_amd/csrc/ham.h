@@ -50,10 +50,10 @@ struct Ham {
     DevBuf kinetic64, kinetic32;
     DevBuf vloc64, vloc32;  // V_loc(r)/N in Ry
     // Register x-line kernel: the sphere's x-planes as the cyclic run starting at plane_xs
-    // (-1: not a run) with plane_xw planes wrapped past nx, e^{-2 pi i k/nx} (FP32), and V_loc/N
-    // in x-major order [x][z][y] (FP32).
+    // (-1: not a run) with plane_xw planes wrapped past nx, e^{-2 pi i k/nx} (FP32, FP64), and
+    // V_loc/N in x-major order [x][z][y] (FP32, FP64).
     int plane_xs = -1, plane_xw = 0;
-    DevBuf xreg_tw, vloc32_t;
+    DevBuf xreg_tw, vloc32_t, xreg_tw64, vloc64_t;
     bool has_local = false;
 
     // Nonlocal: complex projector matrix A[p][G] = phi(G) e^{iG.tau_a} (P x n_pw), D tables.
@@ -64,8 +64,8 @@ struct Ham {
     // Work buffers.
     DevBuf xstage, ystage, x32, y32, tbuf, partial, qbuf;
     // Tuned launch shapes: x-line variant per precision ([0] FP64, [1] FP32): 0 not yet tuned,
-    // d >= 1 the LDS kernel with d y-slabs, -1 the register kernel (FP32); tune_fixed != 0
-    // forces a variant (sweeps).
+    // d >= 1 the LDS kernel with d y-slabs, -1..-3 the register kernel (256/128/64 threads);
+    // tune_fixed != 0 forces a variant (sweeps).
     int xline_split[2] = {0, 0};
     int tune_fixed = 0;
 
@@ -145,10 +145,10 @@ const double2* eig_solution(const Ham& h, uint64_t* cols);
 size_t eig_training_samples(Ham& h, double energy_scale, int base, int stride, float* out_host, size_t cap);
 
 void ham_build(Ham& h, const dzhip_ham_desc& d);
-// True when the register-resident FP32 x-line kernel is built for line length nx, and when
-// it can run on this context (its x-planes form a cyclic run).
-bool xline_reg_supported(int nx);
-bool xline_reg_usable(const Ham& h);
+// True when the register-resident x-line kernel is built for line length nx in the given
+// precision, and when it can run on this context (its x-planes form a cyclic run).
+bool xline_reg_supported(int nx, bool fp64);
+bool xline_reg_usable(const Ham& h, bool fp64);
 void ham_set_local(Ham& h, const double* local_r);
 // local_r already on the device (read on h's stream).
 void ham_set_local_dev(Ham& h, const double* local_dev);

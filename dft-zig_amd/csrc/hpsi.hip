@@ -172,31 +172,32 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
 // cyclic run x = xs, xs + 1, ... (np of them, mod NX), numbered in increasing x, so the plane of
 // x is x for x < xw (the part that wrapped past NX) and x - xs + xw for xs <= x < xs + np - xw:
 // scalar arithmetic on kernel arguments, with no table load in front of each row's load.
-template <int NX>
-__global__ __launch_bounds__(256) void k_xline_reg(float2* __restrict__ tbuf, int nplanes, int xs, int xw,
-                                                   int lines, const float* __restrict__ vt,
-                                                   const float2* __restrict__ tw) {
+template <typename T, int NX>
+__global__ __launch_bounds__(256) void k_xline_reg(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int xs,
+                                                   int xw, int lines, const T* __restrict__ vt,
+                                                   const typename C2T<T>::type* __restrict__ tw) {
+    using C = typename C2T<T>::type;
     // Bands run fastest over the grid (blockIdx.x), so the workgroups that follow each other
     // read the same V_loc lines while they are still in cache.
     const int l = blockIdx.y * blockDim.x + threadIdx.x;
     if (l >= lines) return;
-    float2* t = tbuf + (size_t)blockIdx.x * nplanes * lines + l;
+    C* t = tbuf + (size_t)blockIdx.x * nplanes * lines + l;
     const int hi_end = xs + nplanes - xw;
-    float2 a[NX];
+    C a[NX];
     // Every row is loaded (absent ones from plane 0, a cache hit) and masked, so the loads issue
     // back to back with no branch between them.
 #pragma unroll
     for (int x = 0; x < NX; ++x) {
         const bool lo = x < xw, hi = x >= xs && x < hi_end;
         const int p = lo ? x : (hi ? x - xs + xw : 0);
-        const float2 v = t[(size_t)p * lines];
-        a[x] = (lo || hi) ? v : make_float2(0.f, 0.f);
+        const C v = t[(size_t)p * lines];
+        a[x] = (lo || hi) ? v : C{T(0), T(0)};
     }
     freg::fft<NX, 1>(a, tw);
 #pragma unroll
     for (int x = 0; x < NX; ++x) {
-        const float v = vt[(size_t)x * lines + l];
-        a[x] = make_float2(a[x].x * v, a[x].y * v);
+        const T v = vt[(size_t)x * lines + l];
+        a[x] = C{a[x].x * v, a[x].y * v};
     }
     freg::fft<NX, -1>(a, tw);
 #pragma unroll
@@ -207,14 +208,26 @@ __global__ __launch_bounds__(256) void k_xline_reg(float2* __restrict__ tbuf, in
     }
 }
 
-// Line lengths the register x-line kernel is built for.
+// Line lengths the register x-line kernel is built for: FP32 from 8 to 128, FP64 (twice the
+// registers per line) up to 50, beyond which the line no longer fits the register file.
 #define DZ_XREG_SIZES(X) X(8) X(12) X(15) X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) \
     X(45) X(48) X(50) X(54) X(60) X(64) X(72) X(75) X(80) X(81) X(90) X(96) X(100) X(108) X(120) X(125) X(128)
+#define DZ_XREG64_SIZES(X) X(8) X(12) X(15) X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) \
+    X(45) X(48) X(50)
 
-bool xline_reg_supported(int nx) {
+bool xline_reg_supported(int nx, bool fp64) {
     switch (nx) {
 #define DZ_XREG_CASE(N) case N:
         DZ_XREG_SIZES(DZ_XREG_CASE)
+#undef DZ_XREG_CASE
+        break;
+        default:
+            return false;
+    }
+    if (!fp64) return true;
+    switch (nx) {
+#define DZ_XREG_CASE(N) case N:
+        DZ_XREG64_SIZES(DZ_XREG_CASE)
         return true;
 #undef DZ_XREG_CASE
         default:
@@ -222,23 +235,44 @@ bool xline_reg_supported(int nx) {
     }
 }
 
-bool xline_reg_usable(const Ham& h) { return xline_reg_supported(h.nx) && h.plane_xs >= 0; }
+bool xline_reg_usable(const Ham& h, bool fp64) { return xline_reg_supported(h.nx, fp64) && h.plane_xs >= 0; }
 
-void launch_xline_reg(Ham& h, float2* tb, size_t nc, int threads, hipStream_t s) {
+template <typename T>
+void launch_xline_reg(Ham& h, typename C2T<T>::type* tb, size_t nc, int threads, hipStream_t s) {
+    using C = typename C2T<T>::type;
     const int lines = h.ny * h.nz;
     const dim3 grid((unsigned)nc, (unsigned)((lines + threads - 1) / threads));
-    switch (h.nx) {
-#define DZ_XREG_CASE(N)                                                                                        \
-    case N:                                                                                                    \
-        hipLaunchKernelGGL(k_xline_reg<N>, grid, dim3(threads), 0, s, tb, h.nplanes, h.plane_xs, h.plane_xw,      \
-                           lines,                                                                              \
-                           h.vloc32_t.as<float>(), h.xreg_tw.as<float2>());                                    \
+    const T* vt = std::is_same<T, float>::value ? (const T*)h.vloc32_t.ptr : (const T*)h.vloc64_t.ptr;
+    const C* tw = std::is_same<T, float>::value ? (const C*)h.xreg_tw.ptr : (const C*)h.xreg_tw64.ptr;
+    bool done = false;
+    if constexpr (std::is_same<T, float>::value) {
+        switch (h.nx) {
+#define DZ_XREG_CASE(N)                                                                                          \
+    case N:                                                                                                      \
+        hipLaunchKernelGGL((k_xline_reg<T, N>), grid, dim3(threads), 0, s, tb, h.nplanes, h.plane_xs, h.plane_xw, \
+                           lines, vt, tw);                                                                       \
+        done = true;                                                                                             \
         break;
-        DZ_XREG_SIZES(DZ_XREG_CASE)
+            DZ_XREG_SIZES(DZ_XREG_CASE)
 #undef DZ_XREG_CASE
-        default:
-            DZ_REQUIRE(false, DZHIP_ERR_STATE, "no register x-line kernel for this nx");
+            default:
+                break;
+        }
+    } else {
+        switch (h.nx) {
+#define DZ_XREG_CASE(N)                                                                                          \
+    case N:                                                                                                      \
+        hipLaunchKernelGGL((k_xline_reg<T, N>), grid, dim3(threads), 0, s, tb, h.nplanes, h.plane_xs, h.plane_xw, \
+                           lines, vt, tw);                                                                       \
+        done = true;                                                                                             \
+        break;
+            DZ_XREG64_SIZES(DZ_XREG_CASE)
+#undef DZ_XREG_CASE
+            default:
+                break;
+        }
     }
+    DZ_REQUIRE(done, DZHIP_ERR_STATE, "no register x-line kernel for this nx");
     DZ_HIP(hipGetLastError());
 }
 
@@ -761,7 +795,7 @@ size_t Ham::device_bytes() const {
                             &plane_ycols, &plane_xabsent, &kinetic64,
                             &kinetic32, &vloc64, &vloc32, &a64, &a32, &meta, &block_rows, &dm64, &dm32,
                             &xstage, &ystage, &x32, &y32, &tbuf, &partial, &qbuf,
-                            &xreg_tw, &vloc32_t};
+                            &xreg_tw, &vloc32_t, &xreg_tw64, &vloc64_t};
     size_t s = tab.bytes;
     for (const DevBuf* b : bufs) s += b->bytes;
     return s;
@@ -872,11 +906,14 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
     {
         // e^{-2 pi i k / nx} for the register x-line kernel (computed in FP64).
         std::vector<float2> tw(h.nx);
+        std::vector<double2> tw64(h.nx);
         for (int k = 0; k < h.nx; ++k) {
             const double ang = -2.0 * M_PI * (double)k / (double)h.nx;
-            tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+            tw64[k] = make_double2(std::cos(ang), std::sin(ang));
+            tw[k] = make_float2((float)tw64[k].x, (float)tw64[k].y);
         }
         upload(h.xreg_tw, tw);
+        upload(h.xreg_tw64, tw64);
     }
 
     std::vector<double> kin(d.kinetic, d.kinetic + h.npw);
@@ -983,19 +1020,24 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
     if (d.local_r) ham_set_local(h, d.local_r);
 }
 
-// vt[x][z][y] = v[z][y][x]: the x-major FP32 copy the register x-line kernel reads.
-__global__ void k_local_xmajor(const float* __restrict__ v, float* __restrict__ vt, int nx, int ny, int nz) {
+// vt[x][z][y] = v[z][y][x]: the x-major copies the register x-line kernel reads.
+__global__ void k_local_xmajor(const float* __restrict__ v, float* __restrict__ vt, const double* __restrict__ v64,
+                               double* __restrict__ vt64, int nx, int ny, int nz) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t n = (size_t)nx * ny * nz;
     if (i >= n) return;
     const size_t x = i % nx, line = i / nx;  // line = y + ny z
-    vt[x * ((size_t)ny * nz) + line] = v[i];
+    const size_t o = x * ((size_t)ny * nz) + line;
+    vt[o] = v[i];
+    vt64[o] = v64[i];
 }
 
 static void local_xmajor(Ham& h) {
     h.vloc32_t.ensure(h.ngrid * sizeof(float));
+    h.vloc64_t.ensure(h.ngrid * sizeof(double));
     hipLaunchKernelGGL(k_local_xmajor, dim3((unsigned)((h.ngrid + 255) / 256)), dim3(256), 0, h.stream(),
-                       h.vloc32.as<float>(), h.vloc32_t.as<float>(), h.nx, h.ny, h.nz);
+                       h.vloc32.as<float>(), h.vloc32_t.as<float>(), h.vloc64.as<double>(), h.vloc64_t.as<double>(),
+                       h.nx, h.ny, h.nz);
     DZ_HIP(hipGetLastError());
 }
 
@@ -1036,7 +1078,7 @@ void ham_release(Ham& h) {
                       &h.plane_ycols, &h.plane_xabsent, &h.kinetic64,
                       &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.block_rows, &h.dm64,
                       &h.dm32, &h.xstage, &h.ystage, &h.x32, &h.y32, &h.tbuf, &h.partial, &h.qbuf,
-                      &h.dens_rho, &h.dens_w, &h.xreg_tw, &h.vloc32_t};
+                      &h.dens_rho, &h.dens_w, &h.xreg_tw, &h.vloc32_t, &h.xreg_tw64, &h.vloc64_t};
     for (DevBuf* b : bufs) b->release();
     h.tab.release();
     for (auto& set : h.ev_sets)
@@ -1104,17 +1146,15 @@ void launch_xline(Ham& h, typename C2T<T>::type* tb, size_t nc, int split, hipSt
 }
 
 // x-line variant launched by code: d >= 1 is the LDS kernel with d y-slabs, -1 / -2 / -3 the
-// register kernel (FP32 only) in workgroups of 256 / 128 / 64 threads.
+// register kernel in workgroups of 256 / 128 / 64 threads.
 constexpr int kXlineReg = -1, kXlineRegLast = -3;
 constexpr int xline_reg_threads(int code) { return 256 >> (-code - 1); }
 
 template <typename T>
 void launch_xline_any(Ham& h, typename C2T<T>::type* tb, size_t nc, int code, hipStream_t s) {
-    if constexpr (std::is_same<T, float>::value) {
-        if (code <= kXlineReg && code >= kXlineRegLast) {
-            launch_xline_reg(h, tb, nc, xline_reg_threads(code), s);
-            return;
-        }
+    if (code <= kXlineReg && code >= kXlineRegLast) {
+        launch_xline_reg<T>(h, tb, nc, xline_reg_threads(code), s);
+        return;
     }
     launch_xline<T>(h, tb, nc, code, s);
 }
@@ -1129,7 +1169,7 @@ std::map<std::array<int, 8>, int> g_xline_tune;
 constexpr size_t kXlineTuneMinCols = 16;
 
 // x-line variant, chosen once per shape and precision by timing every divisor d of ny with
-// ny/d >= 8 on the real shapes and, for FP32 with a supported nx, the register kernel in three
+// ny/d >= 8 on the real shapes and, with a supported nx, the register kernel in three
 // workgroup sizes.  The trial launches transform tbuf in place, so this runs before plane_inv
 // writes the pass's data into it.  tune_fixed != 0 forces a variant (d, or -1..-3 for the
 // register kernel).
@@ -1138,12 +1178,10 @@ int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
     constexpr bool fp32 = std::is_same<T, float>::value;
     int& sp = h.xline_split[fp32 ? 1 : 0];
     if (sp != 0) return sp;
-    const bool reg_ok = fp32 && xline_reg_usable(h);
+    const bool reg_ok = xline_reg_usable(h, !fp32);
     if (h.tune_fixed) {
-        DZ_REQUIRE(h.tune_fixed > 0 || xline_reg_usable(h), DZHIP_ERR_ARG,
-                   "no register x-line kernel for this nx");
-        // The register kernel exists for FP32 only; FP64 then keeps one slab.
-        sp = h.tune_fixed < 0 && !fp32 ? 1 : h.tune_fixed;
+        // A register variant forced for an nx that has none in this precision keeps one slab.
+        sp = h.tune_fixed < 0 && !reg_ok ? 1 : h.tune_fixed;
         return sp;
     }
     std::vector<int> cand;
@@ -1173,7 +1211,7 @@ int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
     for (int d : cand) {
         launch_xline_any<T>(h, tb, nc, d, s);
         DZ_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < 3; ++r) launch_xline_any<T>(h, tb, nc, d, s);
+        for (int r = 0; r < 5; ++r) launch_xline_any<T>(h, tb, nc, d, s);
         DZ_HIP(hipEventRecord(e1, s));
         DZ_HIP(hipEventSynchronize(e1));
         float ms = 0.f;

@@ -500,10 +500,10 @@ class Hamiltonian:
         check(L().dzhip_ham_set_tuning(self.handle, int(xline_split)))
 
     def info(self):
-        out = (ct.c_uint64 * 8)()
-        check(L().dzhip_ham_info(self.handle, out, 8))
+        out = (ct.c_uint64 * 9)()
+        check(L().dzhip_ham_info(self.handle, out, 9))
         keys = ("n_pw", "grid_points", "planes", "projectors", "device_bytes", "xline_split_fp32",
-                "xline_split_fp64", "xline_register_fp32")
+                "xline_split_fp64", "xline_register_fp32", "xline_register_fp64")
         return dict(zip(keys, [int(v) for v in out]))
 
     def close(self):

@@ -333,9 +333,9 @@ int dzhip_scf_grid_mix(dzhip_scf_grid* grid, double beta, int pulay);
 int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t count);
 
 /* Launch-shape tuning (no reference counterpart): the LDS x-line kernel processes the y-lines
- * of one (z, band) in xline_split slabs; -1 / -2 / -3 select the register-resident FP32 x-line
- * kernel with one x-line per thread (nx = 2^a 3^b 5^c, 8 <= nx <= 128) in workgroups of
- * 256 / 128 / 64 threads; FP64 then keeps one slab.  0 (default) times every divisor of ny with
+ * of one (z, band) in xline_split slabs; -1 / -2 / -3 select the register-resident x-line
+ * kernel with one x-line per thread (nx = 2^a 3^b 5^c, 8 <= nx <= 128 in FP32, <= 50 in FP64)
+ * in workgroups of 256 / 128 / 64 threads (a precision without it for this nx keeps one slab).  0 (default) times every divisor of ny with
  * at least 8 lines per slab, and the register kernel's shapes where it exists, on the first
  * apply of each precision with at least 16 columns and keeps the fastest; the result is shared
  * by every context of the same device, grid and sphere shape in the process. */
@@ -345,7 +345,8 @@ int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split);
  * out[0] = n_pw, [1] = grid points, [2] = x-planes holding the G-sphere,
  * [3] = projectors P, [4] = device bytes held, [5] = LDS x-line slabs chosen for FP32,
  * [6] = for FP64 (0 = not tuned yet, or the register kernel), [7] = workgroup size when FP32
- * runs the register x-line kernel (0 = the LDS kernel or not tuned yet). */
+ * runs the register x-line kernel (0 = the LDS kernel or not tuned yet), [8] = the same for
+ * FP64. */
 int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out);
 
 #ifdef __cplusplus
