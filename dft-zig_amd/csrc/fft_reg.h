@@ -3,18 +3,22 @@
 // Mixed-radix Stockham autosort (decimation in time): stage s with radix R and span Ns (the
 // product of the earlier radices) reads a[j + r N/R], multiplies by w^(r (j mod Ns) N/(Ns R)),
 // does a radix-R butterfly and writes b[(j / Ns) Ns R + (j mod Ns) + r Ns]; natural order in
-// and out.  Radices are taken as 4s, then 2, then 3s, then 5s.  Every index is a compile-time
+// and out.  Radices are taken as 8s, then 9s, 4, 2, 3, 5.  Every index is a compile-time
 // constant after unrolling, so the line lives in VGPRs and no LDS or barrier is involved; the
 // twiddles w^k = e^{-2 pi i k / N} come from a table with compile-time offsets (wave-uniform
 // loads).  Sign S = -1 is the forward transform e^{-i...}, S = +1 the unnormalised inverse.
 #pragma once
 
+#if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+#endif
 
 namespace dzhip {
 namespace freg {
 
-constexpr int next_radix(int n) { return n % 4 == 0 ? 4 : (n % 2 == 0 ? 2 : (n % 3 == 0 ? 3 : 5)); }
+constexpr int next_radix(int n) {
+    return n % 8 == 0 ? 8 : (n % 9 == 0 ? 9 : (n % 4 == 0 ? 4 : (n % 2 == 0 ? 2 : (n % 3 == 0 ? 3 : 5))));
+}
 
 constexpr bool supported(int n) {
     if (n < 2) return false;
@@ -74,6 +78,50 @@ __device__ __forceinline__ void butterfly(C (&v)[R]) {
         v[0] = cadd(v[0], t1);
         v[1] = cadd(m, n);
         v[2] = csub(m, n);
+    } else if constexpr (R == 8) {
+        // Two radix-4 transforms of the even and odd samples, odd ones turned by w8^(S k).
+        C e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        butterfly<4, S>(e);
+        butterfly<4, S>(o);
+        constexpr T h = T(0.70710678118654752440);  // 1 / sqrt(2)
+        const C o1 = C{h * (o[1].x - S * o[1].y), h * (S * o[1].x + o[1].y)};
+        const C o2 = cmuli<S>(o[2]);
+        const C o3 = C{h * (-o[3].x - S * o[3].y), h * (S * o[3].x - o[3].y)};
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    } else if constexpr (R == 9) {
+        // 3 x 3: radix-3 over n1 for each n2 (samples n2 + 3 n1), twiddle w9^(S n2 k1), radix-3
+        // over n2; output k1 + 3 k2.
+        C t[3][3];
+#pragma unroll
+        for (int n2 = 0; n2 < 3; ++n2) {
+            C u[3] = {v[n2], v[n2 + 3], v[n2 + 6]};
+            butterfly<3, S>(u);
+            t[n2][0] = u[0];
+            t[n2][1] = u[1];
+            t[n2][2] = u[2];
+        }
+        constexpr T c1 = T(0.76604444311897803520), s1 = T(0.64278760968653932632);   // 40 deg
+        constexpr T c2 = T(0.17364817766693034885), s2 = T(0.98480775301220805936);   // 80 deg
+        constexpr T c4 = T(-0.93969262078590838405), s4 = T(0.34202014332566873304);  // 160 deg
+        t[1][1] = cmul(t[1][1], C{c1, S * s1});
+        t[1][2] = cmul(t[1][2], C{c2, S * s2});
+        t[2][1] = cmul(t[2][1], C{c2, S * s2});
+        t[2][2] = cmul(t[2][2], C{c4, S * s4});
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) {
+            C u[3] = {t[0][k1], t[1][k1], t[2][k1]};
+            butterfly<3, S>(u);
+            v[k1] = u[0];
+            v[k1 + 3] = u[1];
+            v[k1 + 6] = u[2];
+        }
     } else {
         static_assert(R == 5, "radix");
         constexpr T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);

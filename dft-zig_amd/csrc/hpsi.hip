@@ -12,7 +12,7 @@
 //   xline     : one workgroup per (z, band): gather the planes' rows into an LDS [x][y]
 //               tile, inverse FFT along x, multiply by V_loc(r)/N, forward FFT along x,
 //               write back only the rows of sphere x-planes.  FP32 with nx = 2^a 3^b 5^c
-//               up to 128 can instead run k_xline_reg, one x-line per thread in registers
+//               up to 100 can instead run k_xline_reg, one x-line per thread in registers
 //               (fft_reg.h), when the autotuner finds it faster.
 //   plane_fwd : one workgroup per (plane, band): forward FFT along y and z, gather the
 //               sphere points, add |k+G|^2 x (and the nonlocal term already in y).
@@ -208,10 +208,10 @@ __global__ __launch_bounds__(256) void k_xline_reg(typename C2T<T>::type* __rest
     }
 }
 
-// Line lengths the register x-line kernel is built for: FP32 from 8 to 128, FP64 (twice the
+// Line lengths the register x-line kernel is built for: FP32 from 8 to 100, FP64 (twice the
 // registers per line) up to 50, beyond which the line no longer fits the register file.
 #define DZ_XREG_SIZES(X) X(8) X(12) X(15) X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) \
-    X(45) X(48) X(50) X(54) X(60) X(64) X(72) X(75) X(80) X(81) X(90) X(96) X(100) X(108) X(120) X(125) X(128)
+    X(45) X(48) X(50) X(54) X(60) X(64) X(72) X(75) X(80) X(81) X(90) X(96) X(100)
 #define DZ_XREG64_SIZES(X) X(8) X(12) X(15) X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) \
     X(45) X(48) X(50)
 

@@ -334,7 +334,7 @@ int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t c
 
 /* Launch-shape tuning (no reference counterpart): the LDS x-line kernel processes the y-lines
  * of one (z, band) in xline_split slabs; -1 / -2 / -3 select the register-resident x-line
- * kernel with one x-line per thread (nx = 2^a 3^b 5^c, 8 <= nx <= 128 in FP32, <= 50 in FP64)
+ * kernel with one x-line per thread (nx = 2^a 3^b 5^c, 8 <= nx <= 100 in FP32, <= 50 in FP64)
  * in workgroups of 256 / 128 / 64 threads (a precision without it for this nx keeps one slab).  0 (default) times every divisor of ny with
  * at least 8 lines per slab, and the register kernel's shapes where it exists, on the first
  * apply of each precision with at least 16 columns and keeps the fastest; the result is shared
