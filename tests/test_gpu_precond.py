@@ -113,7 +113,7 @@ def test_samples_train_and_gain(dz, ctx):
     samples = np.concatenate(chunks, axis=1)
     p0 = precond.init_params(1)
     l0, _ = ctx.mlp_loss_grad(precond.normalise_weights(samples), p0)
-    params, hist = precond.train(ctx, samples, epochs=40, batch=8192, lr=2e-3, seed=1, init=p0)
+    params, hist = precond.train(ctx, samples, epochs=60, batch=8192, lr=2e-3, seed=1, init=p0)
     print("samples", samples.shape[1], "loss TPA", l0, "trained", hist[-1], hist[::8])
     assert hist[-1] < 0.9 * l0
 
@@ -138,7 +138,9 @@ def test_samples_train_and_gain(dz, ctx):
     for i in range(2):
         assert res[(i, "mlp")][1]["converged"]
         assert np.max(np.abs(res[(i, "mlp")][0] - res[(i, "tpa")][0])) < 2e-8
-    assert tot["mlp"] <= 1.1 * tot["tpa"]
+    # The network's first iterations (before its fallback hands over to TPA) carry the gain:
+    # measured 1,639 against 1,820 H applies (seeded training and solves).
+    assert tot["mlp"] <= 0.95 * tot["tpa"]
 
 
 def test_scf_samples_kinetic_base(dz, ctx):
