@@ -669,6 +669,14 @@ int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split) {
     });
 }
 
+int dzhip_ham_set_overlap(dzhip_ham* ham, int on) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        ham->overlap = on != 0;
+    });
+}
+
 int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out) {
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");

@@ -94,6 +94,12 @@ struct Ham {
     int mlp_fail_streak = 0, mlp_rest = 0;
     // Density staging (grow-only).
     DevBuf dens_rho, dens_w;
+    // Overlapped apply: the nonlocal chain on a second stream (created on first use), joined
+    // before plane_fwd; only when enabled by dzhip_ham_set_overlap and not profiling.  Off by
+    // default: at the bench size the co-running kernels slow each other down to no net gain.
+    bool overlap = false;
+    hipStream_t nl_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     size_t device_bytes() const;
     std::mutex mu;
@@ -145,6 +151,7 @@ const double2* eig_solution(const Ham& h, uint64_t* cols);
 size_t eig_training_samples(Ham& h, double energy_scale, int base, int stride, float* out_host, size_t cap);
 
 void ham_build(Ham& h, const dzhip_ham_desc& d);
+void ham_ensure_side_stream(Ham& h);
 // True when the register-resident x-line kernel is built for line length nx in the given
 // precision, and when it can run on this context (its x-planes form a cyclic run).
 bool xline_reg_supported(int nx, bool fp64);

@@ -1072,8 +1072,24 @@ void ham_set_local_dev(Ham& h, const double* local_dev) {
     h.has_local = true;
 }
 
+// The second stream and fork/join events of the overlapped apply (created on first use, default
+// priority, on the context's device).
+void ham_ensure_side_stream(Ham& h) {
+    if (h.nl_stream) return;
+    DZ_HIP(hipStreamCreateWithFlags(&h.nl_stream, hipStreamNonBlocking));
+    DZ_HIP(hipEventCreateWithFlags(&h.ev_fork, hipEventDisableTiming));
+    DZ_HIP(hipEventCreateWithFlags(&h.ev_join, hipEventDisableTiming));
+}
+
 void ham_release(Ham& h) {
     eig_release(h);
+    if (h.nl_stream) {
+        (void)hipStreamSynchronize(h.nl_stream);
+        (void)hipStreamDestroy(h.nl_stream);
+        (void)hipEventDestroy(h.ev_fork);
+        (void)hipEventDestroy(h.ev_join);
+        h.nl_stream = nullptr;
+    }
     DevBuf* bufs[] = {&h.plane_start, &h.plane_pw, &h.plane_pos, &h.plane_xpos, &h.plane_ystart,
                       &h.plane_ycols, &h.plane_xabsent, &h.kinetic64,
                       &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.block_rows, &h.dm64,
@@ -1244,30 +1260,53 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
         const size_t nc = std::min(kMaxColsPerPass, ncols - c0);
         const C* xc = x + c0 * ld;
         C* yc = y + c0 * ld;
-        hipEvent_t* evs = next_event_set(h);
-        mark(h, evs, 0);
-        // Nonlocal projection first (reads x only).
+        // Launch shapes and buffers first: nothing may reallocate once two streams run.
+        constexpr int kNlP = nl_p_tile<T>();
+        const int ptiles = (h.nproj + kNlP - 1) / kNlP;
+        const int btiles = (int)((nc + kNlBands - 1) / kNlBands);
+        const int ktiles = (int)((h.npw + kNlK - 1) / kNlK);
+        // Split K so that about one workgroup per CU runs; each K-chunk a multiple of kNlK.
+        int nks = std::max(1, std::min(ktiles, (256 + ptiles * btiles - 1) / std::max(1, ptiles * btiles)));
+        const int kchunk = (int)round_up((h.npw + nks - 1) / nks, kNlK);
+        nks = (int)((h.npw + kchunk - 1) / kchunk);
         if (do_nl) {
-            constexpr int kNlP = nl_p_tile<T>();
-            const int ptiles = (h.nproj + kNlP - 1) / kNlP;
-            const int btiles = (int)((nc + kNlBands - 1) / kNlBands);
-            const int ktiles = (int)((h.npw + kNlK - 1) / kNlK);
-            // Split K so that about one workgroup per CU runs; each K-chunk a multiple of kNlK.
-            int nks = std::max(1, std::min(ktiles, (256 + ptiles * btiles - 1) / (ptiles * btiles)));
-            const int kchunk = (int)round_up((h.npw + nks - 1) / nks, kNlK);
-            nks = (int)((h.npw + kchunk - 1) / kchunk);
             h.partial.ensure((size_t)nks * nc * h.nproj * sizeof(C));
             h.qbuf.ensure(nc * h.nproj * sizeof(C));
+        }
+        C* tb = nullptr;
+        int split = 1;
+        if (local) {
+            h.tbuf.ensure(nc * h.nplanes * (size_t)h.ny * h.nz * sizeof(C));
+            tb = h.tbuf.as<C>();
+            // Tuned before plane_inv fills tbuf (the trial launches transform tbuf in place) and
+            // before the nonlocal kernels start, so the trials run alone.
+            split = xline_variant<T>(h, tb, nc, s);
+        }
+        // The nonlocal chain (matrix cores) and plane_inv + x-line (memory and latency bound) are
+        // independent: they run on two streams and join before plane_fwd, which adds the local
+        // and kinetic terms to the nonlocal y.  Profiling keeps them in order on one stream so
+        // each phase is timed alone.
+        const bool overlap = do_nl && local && h.overlap && !h.profile;
+        hipStream_t snl = s;
+        if (overlap) {
+            ham_ensure_side_stream(h);
+            snl = h.nl_stream;
+            DZ_HIP(hipEventRecord(h.ev_fork, s));
+            DZ_HIP(hipStreamWaitEvent(snl, h.ev_fork, 0));
+        }
+        hipEvent_t* evs = next_event_set(h);
+        mark(h, evs, 0);
+        if (do_nl) {
             const size_t lds = nl_proj_lds_bytes<T>();
             set_max_dynamic_lds((const void*)k_nl_proj<T>, lds);
-            hipLaunchKernelGGL(k_nl_proj<T>, dim3(ptiles * nks, 1, btiles), dim3(kNlThreads), lds, s,
+            hipLaunchKernelGGL(k_nl_proj<T>, dim3(ptiles * nks, 1, btiles), dim3(kNlThreads), lds, snl,
                                Sel<T>::a(h), xc, ld, h.partial.as<C>(), h.nproj, (int)h.npw, (int)nc,
                                kchunk, ptiles);
             DZ_HIP(hipGetLastError());
             const size_t lds_d = dij_lds_bytes(h.nproj, sizeof(C));
             set_max_dynamic_lds((const void*)k_nl_dij<T>, lds_d);
-            hipLaunchKernelGGL(k_nl_dij<T>, dim3((unsigned)nc), dim3(kDijThreads), lds_d, s, h.partial.as<C>(), nks,
-                               h.qbuf.as<C>(), h.meta.as<ProjMeta>(), Sel<T>::dm(h), h.nproj, (int)nc,
+            hipLaunchKernelGGL(k_nl_dij<T>, dim3((unsigned)nc), dim3(kDijThreads), lds_d, snl, h.partial.as<C>(),
+                               nks, h.qbuf.as<C>(), h.meta.as<ProjMeta>(), Sel<T>::dm(h), h.nproj, (int)nc,
                                (T)h.inv_volume, dij_split(h.nproj, sizeof(C)));
             DZ_HIP(hipGetLastError());
         }
@@ -1278,17 +1317,13 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
             set_max_dynamic_lds((const void*)k_nl_back<T>, lds);
             hipLaunchKernelGGL(k_nl_back<T>, dim3((unsigned)((h.npw + kNlG - 1) / kNlG), 1,
                                                   (unsigned)((nc + kNlBands - 1) / kNlBands)),
-                               dim3(kNlThreads), lds, s, Sel<T>::a(h), h.qbuf.as<C>(), yc, ld, h.nproj,
+                               dim3(kNlThreads), lds, snl, Sel<T>::a(h), h.qbuf.as<C>(), yc, ld, h.nproj,
                                (int)h.npw, (int)nc);
             DZ_HIP(hipGetLastError());
         }
+        if (overlap) DZ_HIP(hipEventRecord(h.ev_join, snl));
         mark(h, evs, 2);
         if (local) {
-            const size_t plane = (size_t)h.ny * h.nz;
-            h.tbuf.ensure(nc * h.nplanes * plane * sizeof(C));
-            C* tb = h.tbuf.as<C>();
-            // Tuned before plane_inv fills tbuf: the trial launches transform tbuf in place.
-            const int split = xline_variant<T>(h, tb, nc, s);
             const size_t lds_p = plane_lds_bytes<C>(h.ny, h.nz);
             constexpr int nt = TileThreads<T>::value;
             set_max_dynamic_lds((const void*)k_plane_inv<T>, lds_p);
@@ -1301,6 +1336,7 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
             mark(h, evs, 3);
             launch_xline_any<T>(h, tb, nc, split, s);
             mark(h, evs, 4);
+            if (overlap) DZ_HIP(hipStreamWaitEvent(s, h.ev_join, 0));
             hipLaunchKernelGGL(k_plane_fwd<T>, dim3(h.nplanes, (unsigned)nc), dim3(nt), lds_p, s,
                                tb, xc, ld, yc, ld, Sel<T>::kin(h), h.plane_start.as<int>(),
                                h.plane_pw.as<int>(), h.plane_pos.as<int>(), h.plane_ystart.as<int>(),

@@ -38,7 +38,7 @@ EXPORTS = (
     "dzhip_scf_grid_create", "dzhip_scf_grid_destroy", "dzhip_scf_grid_set_ion", "dzhip_scf_grid_init",
     "dzhip_ham_set_local_grid", "dzhip_scf_grid_add_solution_density", "dzhip_scf_grid_hxc",
     "dzhip_scf_grid_mix", "dzhip_scf_grid_read", "dzhip_ham_set_tuning", "dzhip_scf_grid_set_dielectric",
-    "dzhip_scf_grid_set_xc",
+    "dzhip_scf_grid_set_xc", "dzhip_ham_set_overlap",
     "dzhip_scf_grid_init_rho",
 )
 
@@ -167,6 +167,7 @@ def _load():
         "dzhip_ham_set_tuning": ([vp, i32], i32),
         "dzhip_scf_grid_set_dielectric": ([vp, ct.c_double, ct.c_double], i32),
         "dzhip_scf_grid_set_xc": ([vp, i32], i32),
+        "dzhip_ham_set_overlap": ([vp, i32], i32),
         "dzhip_scf_grid_init_rho": ([vp, dp, u64], i32),
     }
     missing = set(EXPORTS) - set(sig)
@@ -498,6 +499,10 @@ class Hamiltonian:
     def set_tuning(self, xline_split: int = 0):
         """Force the x-line kernel's y-slab count (0: autotune on the next apply)."""
         check(L().dzhip_ham_set_tuning(self.handle, int(xline_split)))
+
+    def set_overlap(self, on: bool = True):
+        """Run the nonlocal kernels on a second stream, concurrently with the local ones."""
+        check(L().dzhip_ham_set_overlap(self.handle, 1 if on else 0))
 
     def info(self):
         out = (ct.c_uint64 * 9)()

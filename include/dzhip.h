@@ -341,6 +341,13 @@ int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t c
  * by every context of the same device, grid and sphere shape in the process. */
 int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split);
 
+/* Overlap (no reference counterpart; default off): with on != 0 the nonlocal projection / D /
+ * back-projection kernels run on a second stream of the context, concurrently with the local
+ * plane_inv and x-line kernels, and join before plane_fwd.  Results are identical either way;
+ * profiling (dzhip_ham_profile) runs the phases in order.  Measured on the bench block, the
+ * co-running kernels slow each other down (back-projection 105 -> 290 us) to no net gain. */
+int dzhip_ham_set_overlap(dzhip_ham* ham, int on);
+
 /* Sizes the library uses for this context (for roofline accounting):
  * out[0] = n_pw, [1] = grid points, [2] = x-planes holding the G-sphere,
  * [3] = projectors P, [4] = device bytes held, [5] = LDS x-line slabs chosen for FP32,

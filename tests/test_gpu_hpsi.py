@@ -217,3 +217,21 @@ def test_hpsi_register_xline_vs_oracle(dz, ctx, which, prec):
         assert err < (TOL32 if prec else TOL64)
     finally:
         ham.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hpsi_overlap_is_bitwise_identical(dz, ctx, prec):
+    """The overlapped apply (nonlocal kernels on the context's second stream, joined before
+    plane_fwd) gives bit for bit the one-stream result, repeatedly and over two column passes
+    (300 bands: 256 + 44)."""
+    w = workloads.bench(nbands=300)
+    ham = make_ham(dz, ctx, w)
+    try:
+        ham.set_overlap(False)
+        ref = ham.apply(w.psi, precision=prec)
+        ham.set_overlap(True)
+        for _ in range(3):
+            got = ham.apply(w.psi, precision=prec)
+            assert np.array_equal(got, ref)
+    finally:
+        ham.close()
