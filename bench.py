@@ -216,7 +216,8 @@ def main():
                     help="also time the SCF to 1e-6 Ha (plain FP64 vs mixed+batched)")
     ap.add_argument("--scf-max-iter", type=int, default=45)
     ap.add_argument("--xline-split", type=int, default=0,
-                    help="fix the x-line kernel's y-slab count, -1 = register kernel (0 = autotune on the first apply)")
+                    help="fix the x-line variant: d > 0 LDS y-slabs, -1..-3 register kernel, -4..-6 lane-pair register kernel "
+                         "(0 = autotune on the first apply)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (k-point sharding is "
@@ -301,7 +302,8 @@ def main():
         "fp32_vs_fp64_err": {"max_band_rel_l2": err, "max_abs": abserr},
         "tuning": {"xline_split_fp32": tuned["xline_split_fp32"], "xline_split_fp64": tuned["xline_split_fp64"],
                    "xline_register_fp32": tuned["xline_register_fp32"],
-                   "xline_register_fp64": tuned["xline_register_fp64"]},
+                   "xline_register_fp64": tuned["xline_register_fp64"],
+                   "xline_lanes_fp32": tuned["xline_lanes_fp32"], "xline_lanes_fp64": tuned["xline_lanes_fp64"]},
         # Per-phase times of the profiled pass (HIP events between the phases).
         "phases_ms_fp32": {k: round(v, 5) for k, v in ph32.items()},
         "phases_ms_fp64": {k: round(v, 5) for k, v in ph64.items()},

@@ -659,10 +659,9 @@ int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t c
 int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split) {
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
-        DZ_REQUIRE(xline_split == 0 || (xline_split > 0 && ham->ny % xline_split == 0 && ham->ny / xline_split >= 1) ||
-                       (xline_split >= -3 && xline_split <= -1 && xline_reg_usable(*ham, false)),
-                   DZHIP_ERR_ARG,
-                   "xline_split must divide ny, or be -1..-3 with a register kernel for this context (0 = autotune)");
+        DZ_REQUIRE(xline_split == 0 || xline_code_usable(*ham, xline_split, false), DZHIP_ERR_ARG,
+                   "xline_split must divide ny, or be -1..-3 (-4..-6) with a (lane-pair) register kernel for this "
+                   "context (0 = autotune)");
         std::lock_guard<std::mutex> lk(ham->mu);
         ham->tune_fixed = xline_split;
         ham->xline_split[0] = ham->xline_split[1] = 0;
@@ -681,11 +680,13 @@ int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out) {
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
         auto slabs = [](int code) { return code > 0 ? (uint64_t)code : 0; };
-        auto reg = [](int code) { return code < 0 ? (uint64_t)(256 >> (-code - 1)) : 0u; };
-        const uint64_t v[9] = {ham->npw, ham->ngrid, (uint64_t)ham->nplanes, (uint64_t)ham->nproj,
-                               ham->device_bytes(), slabs(ham->xline_split[1]), slabs(ham->xline_split[0]),
-                               reg(ham->xline_split[1]), reg(ham->xline_split[0])};
-        for (uint64_t i = 0; i < n_out && i < 9; ++i) out[i] = v[i];
+        auto reg = [](int code) { return code < 0 ? (uint64_t)xline_reg_threads(code) : 0u; };
+        auto waves = [](int code) { return code < 0 ? (uint64_t)(xline_reg_pair(code) ? 2 : 1) : 0u; };
+        const uint64_t v[11] = {ham->npw, ham->ngrid, (uint64_t)ham->nplanes, (uint64_t)ham->nproj,
+                                ham->device_bytes(), slabs(ham->xline_split[1]), slabs(ham->xline_split[0]),
+                                reg(ham->xline_split[1]), reg(ham->xline_split[0]), waves(ham->xline_split[1]),
+                                waves(ham->xline_split[0])};
+        for (uint64_t i = 0; i < n_out && i < 11; ++i) out[i] = v[i];
     });
 }
 

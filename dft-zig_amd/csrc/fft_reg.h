@@ -42,6 +42,61 @@ constexpr int radix_at(int n, int k) {
     return next_radix(n);
 }
 
+// cos and sin of 2 pi k / n as compile-time constants: the angle is reduced exactly in integers to
+// a quadrant and to [0, pi/4] (by the complement), where the Taylor series of sin and cos, summed
+// in Horner form to x^29, are exact to a few units in the last place of a double.
+struct CosSin {
+    double c, s;
+};
+
+constexpr double sin_series(double x) {
+    double r = 1.0;
+    for (int n = 14; n >= 1; --n) r = 1.0 - x * x / ((2.0 * n) * (2.0 * n + 1.0)) * r;
+    return x * r;
+}
+
+constexpr double cos_series(double x) {
+    double r = 1.0;
+    for (int n = 14; n >= 1; --n) r = 1.0 - x * x / ((2.0 * n - 1.0) * (2.0 * n)) * r;
+    return r;
+}
+
+constexpr CosSin unit_root(long k, long n) {
+    k = ((k % n) + n) % n;
+    const long quad = 4 * k / n, rem = 4 * k - quad * n;  // 2 pi k / n = (pi / 2) (quad + rem / n)
+    const bool comp = 2 * rem > n;
+    const double x = 1.57079632679489661923 * (double)(comp ? n - rem : rem) / (double)n;
+    double s = sin_series(x), c = cos_series(x);
+    if (comp) {
+        const double t = s;
+        s = c;
+        c = t;
+    }
+    switch (quad) {
+        case 0: return CosSin{c, s};
+        case 1: return CosSin{-s, c};
+        case 2: return CosSin{-c, -s};
+        default: return CosSin{s, -c};
+    }
+}
+
+template <int N>
+struct RootTable {
+    double c[N], s[N];
+    constexpr RootTable() : c{}, s{} {
+        for (int k = 0; k < N; ++k) {
+            const CosSin v = unit_root(k, N);
+            c[k] = v.c;
+            s[k] = v.s;
+        }
+    }
+};
+
+// cos / sin of 2 pi k / N, k < N; indexed with indices that are constant after unrolling, the
+// loads fold into literal operands.
+template <int N>
+inline constexpr RootTable<N> kRoots{};
+
 template <typename C>
 __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
 template <typename C>
@@ -152,8 +207,14 @@ __device__ __forceinline__ void stage(const C (&in)[N], C (&out)[N], const C* __
 #pragma unroll
             for (int r = 1; r < R; ++r) {
                 const int e = (r * k * (N / (Ns * R))) % N;
-                C w = tw[e * TS];
-                if (S > 0) w.y = -w.y;
+                C w;
+                if constexpr (TS == 0) {
+                    using T = decltype(w.x);
+                    w = C{T(kRoots<N>.c[e]), T(S * kRoots<N>.s[e])};
+                } else {
+                    w = tw[e * TS];
+                    if (S > 0) w.y = -w.y;
+                }
                 v[r] = cmul(v[r], w);
             }
         }
@@ -182,6 +243,13 @@ template <int N, int S, int TS = 1, typename C>
 __device__ __forceinline__ void fft(C (&a)[N], const C* __restrict__ tw) {
     static_assert(supported(N), "line length must be 2^a 3^b 5^c");
     stages<C, N, S, 0, 1, TS>(a, tw);
+}
+
+// a <- DFT_S(a) with the twiddles as compile-time constants (kRoots).
+template <int N, int S, typename C>
+__device__ __forceinline__ void fft(C (&a)[N]) {
+    static_assert(supported(N), "line length must be 2^a 3^b 5^c");
+    stages<C, N, S, 0, 1, 0>(a, nullptr);
 }
 
 }  // namespace freg

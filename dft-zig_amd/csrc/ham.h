@@ -49,11 +49,10 @@ struct Ham {
 
     DevBuf kinetic64, kinetic32;
     DevBuf vloc64, vloc32;  // V_loc(r)/N in Ry
-    // Register x-line kernel: the sphere's x-planes as the cyclic run starting at plane_xs
-    // (-1: not a run) with plane_xw planes wrapped past nx, e^{-2 pi i k/nx} (FP32, FP64), and
-    // V_loc/N in x-major order [x][z][y] (FP32, FP64).
-    int plane_xs = -1, plane_xw = 0;
-    DevBuf xreg_tw, vloc32_t, xreg_tw64, vloc64_t;
+    // Register x-line kernel: plane p is x = plane_xs + p (mod nx) when the sphere's x-planes form
+    // one cyclic run (plane_xs = -1 otherwise), and V_loc/N in x-major order [x][z][y] (FP32, FP64).
+    int plane_xs = -1;
+    DevBuf vloc32_t, vloc64_t;
     bool has_local = false;
 
     // Nonlocal: complex projector matrix A[p][G] = phi(G) e^{iG.tau_a} (P x n_pw), D tables.
@@ -64,8 +63,8 @@ struct Ham {
     // Work buffers.
     DevBuf xstage, ystage, x32, y32, tbuf, partial, qbuf;
     // Tuned launch shapes: x-line variant per precision ([0] FP64, [1] FP32): 0 not yet tuned,
-    // d >= 1 the LDS kernel with d y-slabs, -1..-3 the register kernel (256/128/64 threads);
-    // tune_fixed != 0 forces a variant (sweeps).
+    // d >= 1 the LDS kernel with d y-slabs, -1..-6 a register kernel (xline_reg_threads,
+    // xline_reg_pair); tune_fixed != 0 forces a variant (sweeps).
     int xline_split[2] = {0, 0};
     int tune_fixed = 0;
 
@@ -156,6 +155,17 @@ void ham_ensure_side_stream(Ham& h);
 // precision, and when it can run on this context (its x-planes form a cyclic run).
 bool xline_reg_supported(int nx, bool fp64);
 bool xline_reg_usable(const Ham& h, bool fp64);
+// ... and when the lane-pair register kernel (two lanes per x-line, half-length transforms) is
+// built for nx and can run here: the sphere spans at most half the line.
+bool xline_pair_supported(int nx);
+bool xline_pair_usable(const Ham& h);
+// Register x-line variants: -1 / -2 / -3 = one lane per x-line in workgroups of 256 / 128 / 64
+// threads, -4 / -5 / -6 = the lane-pair kernel in the same workgroup sizes.
+constexpr int kXlineReg = -1, kXlineRegLast = -6;
+constexpr int xline_reg_threads(int code) { return 256 >> ((-code - 1) % 3); }
+constexpr bool xline_reg_pair(int code) { return code <= -4; }
+// Whether variant `code` (d >= 1 LDS slabs, or a register variant) can run on h in a precision.
+bool xline_code_usable(const Ham& h, int code, bool fp64);
 void ham_set_local(Ham& h, const double* local_r);
 // local_r already on the device (read on h's stream).
 void ham_set_local_dev(Ham& h, const double* local_dev);

@@ -164,47 +164,151 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
     }
 }
 
-// x-line pass with one x-line per thread held in registers (FP32, nx fixed at compile time):
-// thread l = z ny + y of band blockIdx.x loads its values of the np sphere planes (consecutive
-// threads read consecutive addresses of each plane), runs the inverse x-transform, multiplies by
-// V_loc/N from the x-major copy vt[x][z][y], runs the forward transform and stores the planes
-// back.  No LDS and no barrier; absent x-planes enter as zeros.  The sphere's x-planes are the
-// cyclic run x = xs, xs + 1, ... (np of them, mod NX), numbered in increasing x, so the plane of
-// x is x for x < xw (the part that wrapped past NX) and x - xs + xw for xs <= x < xs + np - xw:
-// scalar arithmetic on kernel arguments, with no table load in front of each row's load.
+// x-line pass with one x-line per thread held in registers (nx fixed at compile time): thread
+// l = z ny + y of band blockIdx.x loads its values of the np sphere planes (consecutive threads
+// read consecutive addresses of each plane), runs the inverse x-transform, multiplies by V_loc/N
+// from the x-major copy vt[x][z][y], runs the forward transform and stores the planes back.  No
+// LDS and no barrier.  The planes are the cyclic run x = xs + p (mod NX), p < np, and the line is
+// held rotated, a[u] = value at x = xs + u: the x-part of H is a convolution over the frequency
+// index (a product in real space), so the shift passes through it unchanged and the result lands
+// at the same rotated positions.  Row u is then plane u for u < np and zero above (never loaded),
+// and the twiddles are compile-time constants.
+// Raw buffer loads and stores (gfx9 buffer resource: base, byte size, no stride): the row offset
+// is a scalar operand and the per-thread offset one VGPR for every row, so no 64-bit address is
+// formed per access; accesses at or past the size read zero and write nothing.
+using BufU2 = unsigned int __attribute__((ext_vector_type(2)));
+using BufU4 = unsigned int __attribute__((ext_vector_type(4)));
+constexpr int kBufRsrcWord3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, kBufRsrcWord3);
+}
+
+template <typename X>
+__device__ __forceinline__ X buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+    if constexpr (sizeof(X) == 4) return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    else if constexpr (sizeof(X) == 8) return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    else return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+
+template <typename X>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff, X v) {
+    if constexpr (sizeof(X) == 8) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(BufU2, v), r, voff, soff, 0);
+    else __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(BufU4, v), r, voff, soff, 0);
+}
+
 template <typename T, int NX>
-__global__ __launch_bounds__(256) void k_xline_reg(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int xs,
-                                                   int xw, int lines, const T* __restrict__ vt,
-                                                   const typename C2T<T>::type* __restrict__ tw) {
+__global__ __launch_bounds__(256) void k_xline_reg(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
+                                                   const T* __restrict__ vt) {
     using C = typename C2T<T>::type;
     // Bands run fastest over the grid (blockIdx.x), so the workgroups that follow each other
     // read the same V_loc lines while they are still in cache.
-    const int l = blockIdx.y * blockDim.x + threadIdx.x;
-    if (l >= lines) return;
-    C* t = tbuf + (size_t)blockIdx.x * nplanes * lines + l;
-    const int hi_end = xs + nplanes - xw;
+    const unsigned l = blockIdx.y * blockDim.x + threadIdx.x;
+    if (l >= (unsigned)lines) return;
+    const int row = lines * (int)sizeof(C), vrow = lines * (int)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tbuf + (size_t)blockIdx.x * nplanes * lines, nplanes * row);
+    const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vt, NX * vrow);
+    const unsigned vo = l * (unsigned)sizeof(C), vvo = l * (unsigned)sizeof(T);
     C a[NX];
-    // Every row is loaded (absent ones from plane 0, a cache hit) and masked, so the loads issue
-    // back to back with no branch between them.
 #pragma unroll
-    for (int x = 0; x < NX; ++x) {
-        const bool lo = x < xw, hi = x >= xs && x < hi_end;
-        const int p = lo ? x : (hi ? x - xs + xw : 0);
-        const C v = t[(size_t)p * lines];
-        a[x] = (lo || hi) ? v : C{T(0), T(0)};
+    for (int u = 0; u < NX; ++u) {
+        a[u] = C{T(0), T(0)};
+        if (u < nplanes) a[u] = buf_load<C>(rt, vo, u * row);
     }
-    freg::fft<NX, 1>(a, tw);
+    freg::fft<NX, 1>(a);
 #pragma unroll
     for (int x = 0; x < NX; ++x) {
-        const T v = vt[(size_t)x * lines + l];
+        const T v = buf_load<T>(rv, vvo, x * vrow);
         a[x] = C{a[x].x * v, a[x].y * v};
     }
-    freg::fft<NX, -1>(a, tw);
+    freg::fft<NX, -1>(a);
+    // Fresh copies of the scalars keep the compiler from holding the load side's row offsets and
+    // comparisons live across both transforms.
+    int np = nplanes, rs = row;
+    asm volatile("" : "+s"(np), "+s"(rs));
 #pragma unroll
-    for (int x = 0; x < NX; ++x) {
-        const bool lo = x < xw, hi = x >= xs && x < hi_end;
-        const int p = lo ? x : x - xs + xw;
-        if (lo || hi) t[(size_t)p * lines] = a[x];
+    for (int u = 0; u < NX; ++u)
+        if (u < np) buf_store<C>(rt, vo, u * rs, a[u]);
+}
+
+// Lane-pair variant for spheres that span at most half the line (np <= NX / 2, NX even): the
+// two lanes 2l, 2l + 1 share x-line l and each runs transforms of length M = NX / 2 only.  With
+// the rotated rows b[u] (u < M, zero above), the real-space values at r = 2m + t are
+//   psi(2m + t) = sum_u (b[u] e^{+2 pi i u t / NX}) e^{+2 pi i u m / M},
+// an inverse length-M transform of b twisted by w^{-u t}; lane t multiplies its values by
+// V_loc(2m + t)/N, and the forward result at rows k < np is
+//   X[k] = F_0[k] + e^{-2 pi i k / NX} F_1[k],  F_t = forward length-M transform of lane t,
+// so lane 1 twists its transform, one DPP exchange adds it to lane 0's, and lane 0 stores.
+// Both lanes load the same rows (one memory request per cache line).  Half the registers per lane
+// of k_xline_reg: several waves per SIMD, and FP64 lines up to 100 fit.
+template <typename X>
+__device__ __forceinline__ X pair_partner(X v) {
+    // quad_perm [1, 0, 3, 2]: every lane reads its pair partner's value.
+    constexpr int kSwap = 1 | (0 << 2) | (3 << 4) | (2 << 6);
+    if constexpr (sizeof(X) == 4) {
+        return __builtin_bit_cast(X, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), kSwap, 0xf, 0xf, false));
+    } else {
+        const BufU2 w = __builtin_bit_cast(BufU2, v);
+        BufU2 r;
+        r.x = (unsigned)__builtin_amdgcn_mov_dpp((int)w.x, kSwap, 0xf, 0xf, false);
+        r.y = (unsigned)__builtin_amdgcn_mov_dpp((int)w.y, kSwap, 0xf, 0xf, false);
+        return __builtin_bit_cast(X, r);
+    }
+}
+
+template <typename T, int NX>
+__global__ __launch_bounds__(256) void k_xline_pair(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
+                                                    const T* __restrict__ vt) {
+    using C = typename C2T<T>::type;
+    constexpr int M = NX / 2;
+    static_assert(NX % 2 == 0, "pair kernel needs an even line");
+    // Pairs never straddle a wavefront and exit together (blockDim is even).
+    const unsigned g = blockIdx.y * blockDim.x + threadIdx.x;
+    const unsigned l = g >> 1;
+    const bool odd = (g & 1u) != 0;
+    if (l >= (unsigned)lines) return;
+    const int row = lines * (int)sizeof(C), vrow = lines * (int)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tbuf + (size_t)blockIdx.x * nplanes * lines, nplanes * row);
+    const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vt, NX * vrow);
+    const unsigned vo = l * (unsigned)sizeof(C);
+    const unsigned vvo = l * (unsigned)sizeof(T) + (odd ? (unsigned)vrow : 0u);
+    C a[M];
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        a[u] = C{T(0), T(0)};
+        if (u < nplanes) a[u] = buf_load<C>(rt, vo, u * row);
+    }
+    // Lane 1: b[u] e^{+2 pi i u / NX} (the product with literal twiddles, then a per-lane select).
+#pragma unroll
+    for (int u = 1; u < M; ++u) {
+        const T c = T(freg::kRoots<NX>.c[u]), sn = T(freg::kRoots<NX>.s[u]);
+        const T px = a[u].x * c - a[u].y * sn, py = a[u].x * sn + a[u].y * c;
+        a[u].x = odd ? px : a[u].x;
+        a[u].y = odd ? py : a[u].y;
+    }
+    freg::fft<M, 1>(a);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const T v = buf_load<T>(rv, vvo, 2 * m * vrow);
+        a[m] = C{a[m].x * v, a[m].y * v};
+    }
+    freg::fft<M, -1>(a);
+    // Lane 1: e^{-2 pi i k / NX} F_1[k]; lane 0 adds it to F_0[k].
+#pragma unroll
+    for (int k = 1; k < M; ++k) {
+        const T c = T(freg::kRoots<NX>.c[k]), sn = T(freg::kRoots<NX>.s[k]);
+        const T px = a[k].x * c + a[k].y * sn, py = a[k].y * c - a[k].x * sn;
+        a[k].x = odd ? px : a[k].x;
+        a[k].y = odd ? py : a[k].y;
+    }
+    int np = nplanes, rs = row;
+    asm volatile("" : "+s"(np), "+s"(rs));
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        if (k < np) {
+            const C o = C{pair_partner(a[k].x), pair_partner(a[k].y)};
+            if (!odd) buf_store<C>(rt, vo, k * rs, C{a[k].x + o.x, a[k].y + o.y});
+        }
     }
 }
 
@@ -214,6 +318,21 @@ __global__ __launch_bounds__(256) void k_xline_reg(typename C2T<T>::type* __rest
     X(45) X(48) X(50) X(54) X(60) X(64) X(72) X(75) X(80) X(81) X(90) X(96) X(100)
 #define DZ_XREG64_SIZES(X) X(8) X(12) X(15) X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) \
     X(45) X(48) X(50)
+
+// Even lines whose half length the register FFT takes, for the lane-pair kernel (both precisions).
+#define DZ_XPAIR_SIZES(X) X(8) X(12) X(16) X(18) X(20) X(24) X(30) X(32) X(36) X(40) X(48) X(50) X(54) X(60) \
+    X(64) X(72) X(80) X(90) X(96) X(100)
+
+bool xline_pair_supported(int nx) {
+    switch (nx) {
+#define DZ_XREG_CASE(N) case N:
+        DZ_XPAIR_SIZES(DZ_XREG_CASE)
+#undef DZ_XREG_CASE
+        return true;
+        default:
+            return false;
+    }
+}
 
 bool xline_reg_supported(int nx, bool fp64) {
     switch (nx) {
@@ -235,23 +354,47 @@ bool xline_reg_supported(int nx, bool fp64) {
     }
 }
 
-bool xline_reg_usable(const Ham& h, bool fp64) { return xline_reg_supported(h.nx, fp64) && h.plane_xs >= 0; }
+bool xline_reg_usable(const Ham& h, bool fp64) {
+    // The kernel's buffer resources hold a band's planes and V_loc in 32-bit byte ranges.
+    return xline_reg_supported(h.nx, fp64) && h.plane_xs >= 0 && h.ngrid * 2 * sizeof(double) < ((size_t)1 << 31);
+}
+
+bool xline_pair_usable(const Ham& h) {
+    return xline_pair_supported(h.nx) && h.plane_xs >= 0 && 2 * h.nplanes <= h.nx &&
+           h.ngrid * 2 * sizeof(double) < ((size_t)1 << 31);
+}
+
+bool xline_code_usable(const Ham& h, int code, bool fp64) {
+    if (code > 0) return h.ny % code == 0;
+    if (code < kXlineRegLast || code > kXlineReg) return false;
+    return xline_reg_pair(code) ? xline_pair_usable(h) : xline_reg_usable(h, fp64);
+}
 
 template <typename T>
-void launch_xline_reg(Ham& h, typename C2T<T>::type* tb, size_t nc, int threads, hipStream_t s) {
-    using C = typename C2T<T>::type;
+void launch_xline_reg(Ham& h, typename C2T<T>::type* tb, size_t nc, int threads, bool pair, hipStream_t s) {
     const int lines = h.ny * h.nz;
-    const dim3 grid((unsigned)nc, (unsigned)((lines + threads - 1) / threads));
+    const int lanes = pair ? 2 * lines : lines;
+    const dim3 grid((unsigned)nc, (unsigned)((lanes + threads - 1) / threads));
     const T* vt = std::is_same<T, float>::value ? (const T*)h.vloc32_t.ptr : (const T*)h.vloc64_t.ptr;
-    const C* tw = std::is_same<T, float>::value ? (const C*)h.xreg_tw.ptr : (const C*)h.xreg_tw64.ptr;
     bool done = false;
-    if constexpr (std::is_same<T, float>::value) {
+    if (pair) {
         switch (h.nx) {
-#define DZ_XREG_CASE(N)                                                                                          \
-    case N:                                                                                                      \
-        hipLaunchKernelGGL((k_xline_reg<T, N>), grid, dim3(threads), 0, s, tb, h.nplanes, h.plane_xs, h.plane_xw, \
-                           lines, vt, tw);                                                                       \
-        done = true;                                                                                             \
+#define DZ_XREG_CASE(N)                                                                                      \
+    case N:                                                                                                  \
+        hipLaunchKernelGGL((k_xline_pair<T, N>), grid, dim3(threads), 0, s, tb, h.nplanes, lines, vt);      \
+        done = true;                                                                                         \
+        break;
+            DZ_XPAIR_SIZES(DZ_XREG_CASE)
+#undef DZ_XREG_CASE
+            default:
+                break;
+        }
+    } else if constexpr (std::is_same<T, float>::value) {
+        switch (h.nx) {
+#define DZ_XREG_CASE(N)                                                                                      \
+    case N:                                                                                                  \
+        hipLaunchKernelGGL((k_xline_reg<T, N>), grid, dim3(threads), 0, s, tb, h.nplanes, lines, vt);       \
+        done = true;                                                                                         \
         break;
             DZ_XREG_SIZES(DZ_XREG_CASE)
 #undef DZ_XREG_CASE
@@ -260,11 +403,10 @@ void launch_xline_reg(Ham& h, typename C2T<T>::type* tb, size_t nc, int threads,
         }
     } else {
         switch (h.nx) {
-#define DZ_XREG_CASE(N)                                                                                          \
-    case N:                                                                                                      \
-        hipLaunchKernelGGL((k_xline_reg<T, N>), grid, dim3(threads), 0, s, tb, h.nplanes, h.plane_xs, h.plane_xw, \
-                           lines, vt, tw);                                                                       \
-        done = true;                                                                                             \
+#define DZ_XREG_CASE(N)                                                                                      \
+    case N:                                                                                                  \
+        hipLaunchKernelGGL((k_xline_reg<T, N>), grid, dim3(threads), 0, s, tb, h.nplanes, lines, vt);       \
+        done = true;                                                                                         \
         break;
             DZ_XREG64_SIZES(DZ_XREG_CASE)
 #undef DZ_XREG_CASE
@@ -795,7 +937,7 @@ size_t Ham::device_bytes() const {
                             &plane_ycols, &plane_xabsent, &kinetic64,
                             &kinetic32, &vloc64, &vloc32, &a64, &a32, &meta, &block_rows, &dm64, &dm32,
                             &xstage, &ystage, &x32, &y32, &tbuf, &partial, &qbuf,
-                            &xreg_tw, &vloc32_t, &xreg_tw64, &vloc64_t};
+                            &vloc32_t, &vloc64_t};
     size_t s = tab.bytes;
     for (const DevBuf* b : bufs) s += b->bytes;
     return s;
@@ -844,13 +986,36 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
     }
     std::vector<int> count(h.nx, 0);
     for (size_t i = 0; i < h.npw; ++i) count[xs[i]]++;
+    // The x values that hold plane waves normally form one cyclic run x0, x0 + 1, ... (mod nx)
+    // (the G-sphere around k = 0 wraps past nx).  Planes are numbered along that run, so plane p
+    // is x = x0 + p (mod nx) and the register x-line kernel reads its rows as a prefix; any other
+    // pattern is numbered in increasing x and runs the LDS x-line kernel.
+    int x0 = -1, run = 0;
+    {
+        int present = 0, starts = 0;
+        for (int xx = 0; xx < h.nx; ++xx) {
+            if (count[xx] == 0) continue;
+            ++present;
+            if (count[(xx + h.nx - 1) % h.nx] == 0) {
+                ++starts;
+                x0 = xx;
+            }
+        }
+        if (present == h.nx) x0 = 0;
+        else if (starts != 1) x0 = -1;
+        run = x0 >= 0 ? present : 0;
+    }
     std::vector<int> plane_of_x(h.nx, -1), xval;
-    for (int xx = 0; xx < h.nx; ++xx)
+    for (int k = 0; k < h.nx; ++k) {
+        const int xx = x0 >= 0 ? (x0 + k) % h.nx : k;
         if (count[xx] > 0) {
             plane_of_x[xx] = (int)xval.size();
             xval.push_back(xx);
         }
+    }
     h.nplanes = (int)xval.size();
+    h.plane_xs = x0;
+    DZ_REQUIRE(x0 < 0 || run == h.nplanes, DZHIP_ERR_STATE, "x-plane run mismatch");
     std::vector<int> start(h.nplanes + 1, 0);
     for (int p = 0; p < h.nplanes; ++p) start[p + 1] = start[p] + count[xval[p]];
     std::vector<int> fill(start.begin(), start.end() - 1);
@@ -886,36 +1051,6 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
         if (plane_of_x[xx] < 0) xabsent.push_back(px[xx]);
     h.nxabsent = (int)xabsent.size();
     upload(h.plane_xabsent, xabsent);
-    {
-        // The sphere's x-planes as a cyclic run xs, xs + 1, ... (mod nx) numbered in increasing x:
-        // xw = planes that wrapped past nx.  plane_xs = -1 when the planes are not such a run.
-        int x0 = -1;
-        for (int xx = 0; xx < h.nx && x0 < 0; ++xx)
-            if (plane_of_x[xx] >= 0 && plane_of_x[(xx + h.nx - 1) % h.nx] < 0) x0 = xx;
-        if (h.nplanes == h.nx) x0 = 0;
-        const int xw = x0 < 0 ? 0 : std::max(0, x0 + h.nplanes - h.nx);
-        bool ok = x0 >= 0;
-        for (int xx = 0; xx < h.nx && ok; ++xx) {
-            const bool lo = xx < xw, hi = xx >= x0 && xx < x0 + h.nplanes - xw;
-            const int want = lo ? xx : (hi ? xx - x0 + xw : -1);
-            ok = plane_of_x[xx] == want;
-        }
-        h.plane_xs = ok ? x0 : -1;
-        h.plane_xw = ok ? xw : 0;
-    }
-    {
-        // e^{-2 pi i k / nx} for the register x-line kernel (computed in FP64).
-        std::vector<float2> tw(h.nx);
-        std::vector<double2> tw64(h.nx);
-        for (int k = 0; k < h.nx; ++k) {
-            const double ang = -2.0 * M_PI * (double)k / (double)h.nx;
-            tw64[k] = make_double2(std::cos(ang), std::sin(ang));
-            tw[k] = make_float2((float)tw64[k].x, (float)tw64[k].y);
-        }
-        upload(h.xreg_tw, tw);
-        upload(h.xreg_tw64, tw64);
-    }
-
     std::vector<double> kin(d.kinetic, d.kinetic + h.npw);
     std::vector<float> kin32(kin.begin(), kin.end());
     upload(h.kinetic64, kin);
@@ -1094,7 +1229,7 @@ void ham_release(Ham& h) {
                       &h.plane_ycols, &h.plane_xabsent, &h.kinetic64,
                       &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.block_rows, &h.dm64,
                       &h.dm32, &h.xstage, &h.ystage, &h.x32, &h.y32, &h.tbuf, &h.partial, &h.qbuf,
-                      &h.dens_rho, &h.dens_w, &h.xreg_tw, &h.vloc32_t, &h.xreg_tw64, &h.vloc64_t};
+                      &h.dens_rho, &h.dens_w, &h.vloc32_t, &h.vloc64_t};
     for (DevBuf* b : bufs) b->release();
     h.tab.release();
     for (auto& set : h.ev_sets)
@@ -1162,14 +1297,13 @@ void launch_xline(Ham& h, typename C2T<T>::type* tb, size_t nc, int split, hipSt
 }
 
 // x-line variant launched by code: d >= 1 is the LDS kernel with d y-slabs, -1 / -2 / -3 the
-// register kernel in workgroups of 256 / 128 / 64 threads.
-constexpr int kXlineReg = -1, kXlineRegLast = -3;
-constexpr int xline_reg_threads(int code) { return 256 >> (-code - 1); }
+// register kernel and -4 / -5 / -6 the lane-pair register kernel in workgroups of 256 / 128 / 64
+// threads.
 
 template <typename T>
 void launch_xline_any(Ham& h, typename C2T<T>::type* tb, size_t nc, int code, hipStream_t s) {
     if (code <= kXlineReg && code >= kXlineRegLast) {
-        launch_xline_reg<T>(h, tb, nc, xline_reg_threads(code), s);
+        launch_xline_reg<T>(h, tb, nc, xline_reg_threads(code), xline_reg_pair(code), s);
         return;
     }
     launch_xline<T>(h, tb, nc, code, s);
@@ -1185,24 +1319,26 @@ std::map<std::array<int, 8>, int> g_xline_tune;
 constexpr size_t kXlineTuneMinCols = 16;
 
 // x-line variant, chosen once per shape and precision by timing every divisor d of ny with
-// ny/d >= 8 on the real shapes and, with a supported nx, the register kernel in three
-// workgroup sizes.  The trial launches transform tbuf in place, so this runs before plane_inv
-// writes the pass's data into it.  tune_fixed != 0 forces a variant (d, or -1..-3 for the
-// register kernel).
+// ny/d >= 8 on the real shapes and, where they can run, the lane-pair and one-lane register
+// kernels in three workgroup sizes each.  The trial launches transform tbuf in place, so this
+// runs before plane_inv writes the pass's data into it.  tune_fixed != 0 forces a variant (d, or
+// -1..-6 for the register kernels).
 template <typename T>
 int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
     constexpr bool fp32 = std::is_same<T, float>::value;
     int& sp = h.xline_split[fp32 ? 1 : 0];
     if (sp != 0) return sp;
-    const bool reg_ok = xline_reg_usable(h, !fp32);
     if (h.tune_fixed) {
-        // A register variant forced for an nx that has none in this precision keeps one slab.
-        sp = h.tune_fixed < 0 && !reg_ok ? 1 : h.tune_fixed;
+        // A register variant forced where it cannot run in this precision keeps one slab.
+        sp = xline_code_usable(h, h.tune_fixed, !fp32) ? h.tune_fixed : 1;
         return sp;
     }
     std::vector<int> cand;
-    if (reg_ok)
-        for (int c = kXlineReg; c >= kXlineRegLast; --c) cand.push_back(c);
+    // The first candidate is also what narrow passes run before a wide pass has tuned.
+    if (xline_pair_usable(h))
+        for (int c = -4; c >= -6; --c) cand.push_back(c);
+    if (xline_reg_usable(h, !fp32))
+        for (int c = -1; c >= -3; --c) cand.push_back(c);
     for (int d = 1; d <= 12; ++d)
         if (h.ny % d == 0 && h.ny / d >= 8) cand.push_back(d);
     if (cand.empty()) cand.push_back(1);
@@ -1212,7 +1348,7 @@ int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
     }
     // Narrow passes run the first candidate without caching it; the next wide pass tunes.
     if (nc < kXlineTuneMinCols) return cand[0];
-    const std::array<int, 8> key = {h.ctx->device, h.nx, h.ny, h.nz, h.nplanes, h.plane_xs, h.plane_xw, fp32 ? 1 : 0};
+    const std::array<int, 8> key = {h.ctx->device, h.nx, h.ny, h.nz, h.nplanes, h.plane_xs, 0, fp32 ? 1 : 0};
     std::lock_guard<std::mutex> lk(g_xline_tune_mu);
     const auto it = g_xline_tune.find(key);
     if (it != g_xline_tune.end()) {

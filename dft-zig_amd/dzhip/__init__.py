@@ -497,7 +497,7 @@ class Hamiltonian:
         return out
 
     def set_tuning(self, xline_split: int = 0):
-        """Force the x-line kernel's y-slab count (0: autotune on the next apply)."""
+        """Force the x-line variant: d > 0 LDS y-slabs, -1..-6 register kernel (0: autotune)."""
         check(L().dzhip_ham_set_tuning(self.handle, int(xline_split)))
 
     def set_overlap(self, on: bool = True):
@@ -505,10 +505,11 @@ class Hamiltonian:
         check(L().dzhip_ham_set_overlap(self.handle, 1 if on else 0))
 
     def info(self):
-        out = (ct.c_uint64 * 9)()
-        check(L().dzhip_ham_info(self.handle, out, 9))
+        out = (ct.c_uint64 * 11)()
+        check(L().dzhip_ham_info(self.handle, out, 11))
         keys = ("n_pw", "grid_points", "planes", "projectors", "device_bytes", "xline_split_fp32",
-                "xline_split_fp64", "xline_register_fp32", "xline_register_fp64")
+                "xline_split_fp64", "xline_register_fp32", "xline_register_fp64", "xline_lanes_fp32",
+                "xline_lanes_fp64")
         return dict(zip(keys, [int(v) for v in out]))
 
     def close(self):

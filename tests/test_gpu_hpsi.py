@@ -181,40 +181,64 @@ def test_hpsi_every_xline_split(dz, ctx, prec):
         ham.set_tuning(d)
         errs[d] = relerr(ham.apply(w.psi, precision=prec), g["y"])
         assert ham.info()["xline_split_fp32" if prec else "xline_split_fp64"] == d
-    # -1: the register-resident x-line kernel (FP32; FP64 keeps one slab).
-    for v, threads in ((-1, 256), (-2, 128), (-3, 64)):
+    # -1..-3: the register-resident x-line kernel (one lane per line), -4..-6 the lane-pair kernel
+    # (the Si-8 sphere spans 15 of the 32 x values), in 256 / 128 / 64-thread workgroups.
+    for v, threads, lanes in ((-1, 256, 1), (-2, 128, 1), (-3, 64, 1), (-4, 256, 2), (-5, 128, 2), (-6, 64, 2)):
         ham.set_tuning(v)
-        errs[f"register{threads}"] = relerr(ham.apply(w.psi, precision=prec), g["y"])
-        # ny = 32: the register kernel exists in both precisions.
-        assert ham.info()["xline_register_fp32" if prec else "xline_register_fp64"] == threads
+        errs[f"register{threads}x{lanes}"] = relerr(ham.apply(w.psi, precision=prec), g["y"])
+        # nx = 32: both register kernels exist in both precisions.
+        info = ham.info()
+        assert info["xline_register_fp32" if prec else "xline_register_fp64"] == threads
+        assert info["xline_lanes_fp32" if prec else "xline_lanes_fp64"] == lanes
     ham.set_tuning(0)
     errs["auto"] = relerr(ham.apply(w.psi, precision=prec), g["y"])
     print(prec, errs, ham.info())
     assert all(e < tol for e in errs.values()), errs
 
 
+# Which register kernels run per workload and precision: one lane per line for nx <= 100 (FP32)
+# or <= 50 (FP64); the lane-pair kernel for even nx when the sphere spans at most half the line
+# (tiny: 5 of 8 x values, no; nx = 75: odd, no).
+REGISTER_KERNELS = {"tiny": {"one": (0, 1), "pair": ()}, "si8": {"one": (0, 1), "pair": (0, 1)},
+                    "bench": {"one": (1,), "pair": (0, 1)}, "bench75": {"one": (1,), "pair": ()},
+                    "bench80": {"one": (1,), "pair": (0, 1)}, "bench90": {"one": (1,), "pair": (0, 1)}}
+
+
 @pytest.mark.parametrize("which,prec", [("tiny", 1), ("si8", 1), ("bench", 1), ("bench75", 1), ("bench90", 1),
-                                        ("bench80", 1), ("tiny", 0), ("si8", 0)])
+                                        ("bench80", 1), ("tiny", 0), ("si8", 0), ("bench", 0), ("bench90", 0)])
 def test_hpsi_register_xline_vs_oracle(dz, ctx, which, prec):
-    """The register-resident x-line kernel (one x-line per thread, Stockham in VGPRs) on grids
-    of different factorisations (nx = 8, 32, 72, 75, 80, 90; radices 4/2/3/5) against the FP64
-    oracle apply: FP32 tolerance, and 1e-12 for the FP64 kernel (nx <= 50)."""
-    variant = -1
+    """The register-resident x-line kernels (one x-line per thread, or per lane pair with
+    half-length transforms; Stockham in VGPRs) on grids of different factorisations (nx = 8, 32,
+    72, 75, 80, 90; radices 4/2/3/5/9) against the FP64 oracle apply: FP32 tolerance, and 1e-12
+    in FP64.  A variant that cannot run is refused (both precisions) or runs the LDS kernel (one
+    precision only)."""
     make = {"tiny": workloads.tiny, "si8": workloads.si8, "bench": workloads.bench,
             "bench75": lambda: workloads.bench(nbands=8, grid=(75, 75, 75)),
             "bench80": lambda: workloads.bench(nbands=8, grid=(80, 72, 75)),
             "bench90": lambda: workloads.bench(nbands=8, grid=(90, 90, 90))}
     w = make[which]()
     ham = make_ham(dz, ctx, w)
+    key = "fp32" if prec else "fp64"
     try:
-        ham.set_tuning(variant)
         nb = min(w.nbands, 8)
-        got = ham.apply(w.psi[:nb], precision=prec)
-        assert ham.info()["xline_register_fp32" if prec else "xline_register_fp64"] == 256
         ref = hamiltonian.apply_hamiltonian_batched(oracle_ctx(w), w.psi[:nb])
-        err = relerr(got, ref)
-        print(which, prec, w.grid, err)
-        assert err < (TOL32 if prec else TOL64)
+        ran = 0
+        for variant, kind, lanes in ((-1, "one", 1), (-4, "pair", 2)):
+            runs = prec in REGISTER_KERNELS[which][kind]
+            if not REGISTER_KERNELS[which][kind] or (kind == "one" and 1 not in REGISTER_KERNELS[which][kind]):
+                with pytest.raises(dz.BadArgument):
+                    ham.set_tuning(variant)
+                continue
+            ham.set_tuning(variant)
+            got = ham.apply(w.psi[:nb], precision=prec)
+            info = ham.info()
+            assert info[f"xline_register_{key}"] == (256 if runs else 0)
+            assert info[f"xline_lanes_{key}"] == (lanes if runs else 0)
+            err = relerr(got, ref)
+            print(which, prec, w.grid, variant, runs, err)
+            assert err < (TOL32 if prec else TOL64)
+            ran += runs
+        assert ran >= 1
     finally:
         ham.close()
 
