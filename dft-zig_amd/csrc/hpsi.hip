@@ -51,6 +51,11 @@ __host__ __device__ inline size_t xline_lds_bytes(int nx, int ny) {
     return ((size_t)nx * (ny + 1) + nx) * sizeof(C);
 }
 
+// Sphere entries per thread whose global loads the plane kernels issue ahead of their tile work
+// (FP64 runs twice the threads per tile at a tighter register budget).
+template <typename T> struct SpherePrefetch { static constexpr int value = 2; };
+template <> struct SpherePrefetch<double> { static constexpr int value = 1; };
+
 template <typename T>
 __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_inv(
     const typename C2T<T>::type* __restrict__ x, size_t ldx, typename C2T<T>::type* __restrict__ tbuf,
@@ -67,15 +72,31 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_inv(
     const int p = blockIdx.x;
     const size_t col = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    // The plane's first sphere entries are loaded before the tile is cleared, so the latency of
+    // the two dependent loads (index, then coefficient) overlaps the clearing.
+    const C* xc = x + col * ldx;
+    const int j0 = pstart[p], j1 = pstart[p + 1];
+    C pv[SpherePrefetch<T>::value];
+    int pp[SpherePrefetch<T>::value];
+#pragma unroll
+    for (int u = 0; u < SpherePrefetch<T>::value; ++u) {
+        const int j = j0 + tid + u * nthr;
+        pp[u] = -1;
+        if (j < j1) {
+            pp[u] = ppos[j];
+            pv[u] = xc[ppw[j]];
+        }
+    }
     for (int i = tid; i < nz * pitch; i += nthr) buf[i] = C{T(0), T(0)};
     load_twiddles(twy, tw + ay.tw_off, ny, tid, nthr);
     load_twiddles(twz, tw + az.tw_off, nz, tid, nthr);
     const int c0 = ystart[p], nact = ystart[p + 1] - c0;
     for (int i = tid; i < nact; i += nthr) cols[i] = ycols[c0 + i];
     __syncthreads();
-    const C* xc = x + col * ldx;
-    const int j1 = pstart[p + 1];
-    for (int j = pstart[p] + tid; j < j1; j += nthr) buf[ppos[j]] = xc[ppw[j]];
+#pragma unroll
+    for (int u = 0; u < SpherePrefetch<T>::value; ++u)
+        if (pp[u] >= 0) buf[pp[u]] = pv[u];
+    for (int j = j0 + tid + SpherePrefetch<T>::value * nthr; j < j1; j += nthr) buf[ppos[j]] = xc[ppw[j]];
     __syncthreads();
     // Inverse (e^{+i}) along z, only on the y-columns that hold plane waves.
     lds_fft_lines<T, C, true, 1>(buf, az, twz, LineSet{nact, pitch, 0, cols}, tid, nthr);
@@ -435,6 +456,28 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_fwd(
     const int p = blockIdx.x;
     const size_t col = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    // The gather's operands (index, LDS position, kinetic factor, x and the nonlocal y) do not
+    // depend on the transforms: the plane's first entries are loaded before the tile, so their
+    // latency overlaps the tile load and the transforms.
+    const C* xc = x + col * ldx;
+    C* yc = y + col * ldy;
+    const int j0 = pstart[p], j1 = pstart[p + 1];
+    int gg[SpherePrefetch<T>::value], gp[SpherePrefetch<T>::value];
+    T gk[SpherePrefetch<T>::value];
+    C gx[SpherePrefetch<T>::value], gy[SpherePrefetch<T>::value];
+#pragma unroll
+    for (int u = 0; u < SpherePrefetch<T>::value; ++u) {
+        const int j = j0 + tid + u * nthr;
+        gg[u] = -1;
+        if (j < j1) {
+            const int g = ppw[j];
+            gg[u] = g;
+            gp[u] = ppos[j];
+            gk[u] = kinetic[g];
+            gx[u] = xc[g];
+            gy[u] = accumulate ? yc[g] : C{T(0), T(0)};
+        }
+    }
     load_twiddles(twy, tw + ay.tw_off, ny, tid, nthr);
     load_twiddles(twz, tw + az.tw_off, nz, tid, nthr);
     const int c0 = ystart[p], nact = ystart[p + 1] - c0;
@@ -465,10 +508,13 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_fwd(
     // columns whose frequencies are gathered.
     lds_fft_lines<T, C, false, -1>(buf, ay, twy, LineSet{nz, 1, pitch, nullptr}, tid, nthr);
     lds_fft_lines<T, C, false, -1>(buf, az, twz, LineSet{nact, pitch, 0, cols}, tid, nthr);
-    const C* xc = x + col * ldx;
-    C* yc = y + col * ldy;
-    const int j1 = pstart[p + 1];
-    for (int j = pstart[p] + tid; j < j1; j += nthr) {
+#pragma unroll
+    for (int u = 0; u < SpherePrefetch<T>::value; ++u)
+        if (gg[u] >= 0) {
+            const C v = buf[gp[u]];
+            yc[gg[u]] = C{v.x + gk[u] * gx[u].x + gy[u].x, v.y + gk[u] * gx[u].y + gy[u].y};
+        }
+    for (int j = j0 + tid + SpherePrefetch<T>::value * nthr; j < j1; j += nthr) {
         const int g = ppw[j];
         const C v = buf[ppos[j]];
         const T kin = kinetic[g];
