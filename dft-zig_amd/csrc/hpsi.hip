@@ -654,7 +654,10 @@ constexpr int kNlPitch = kNlK + 1;       // odd pitches (in complex elements) ag
 template <typename T> struct NlTile { static constexpr int kBlocks = 4; };
 template <> struct NlTile<double> { static constexpr int kBlocks = 2; };
 template <typename T> constexpr int nl_p_tile() { return 16 * NlTile<T>::kBlocks; }  // projectors / tile
-template <typename T> constexpr int nl_g_tile() { return 16 * NlTile<T>::kBlocks; }  // G / tile
+// Back-projection G tiles of two 16-wide blocks (one per wavefront half): 373 tiles of 64 G at
+// the bench size leave a quarter of two-per-CU slots idle, 746 tiles of 32 G fill three per CU.
+template <typename T> struct NlBackTile { static constexpr int kBlocks = 2; };
+template <typename T> constexpr int nl_g_tile() { return 16 * NlBackTile<T>::kBlocks; }  // G / tile
 
 template <typename T>
 __host__ __device__ inline size_t nl_proj_lds_bytes() {
@@ -867,7 +870,7 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::t
     C* qs = reinterpret_cast<C*>(smem_raw);  // [kNlBands][kNlPitch]  (band, p)
     constexpr int kNlG = nl_g_tile<T>();
     constexpr int kNlGPitch = kNlG + 1;
-    constexpr int GB = NlTile<T>::kBlocks / 2;  // G blocks per wavefront
+    constexpr int GB = NlBackTile<T>::kBlocks / 2;  // G blocks per wavefront
     C* as = qs + kNlBands * kNlPitch;        // [kNlK][kNlGPitch]     (p, G)
     const int g0 = blockIdx.x * kNlG;
     const int b0 = blockIdx.z * kNlBands;
