@@ -215,6 +215,8 @@ def main():
     ap.add_argument("--scf", default="si64", choices=("none", "si8", "si64"),
                     help="also time the SCF to 1e-6 Ha (plain FP64 vs mixed+batched)")
     ap.add_argument("--scf-max-iter", type=int, default=45)
+    ap.add_argument("--overlap", action="store_true",
+                    help="run the nonlocal kernels on a second stream beside the local ones (timed pass only)")
     ap.add_argument("--xline-split", type=int, default=0,
                     help="fix the x-line variant: d > 0 LDS y-slabs, -1..-3 register kernel, -4..-6 lane-pair register kernel "
                          "(0 = autotune on the first apply)")
@@ -237,6 +239,8 @@ def main():
     ham = dzhip.Hamiltonian.from_workload(ctx, w)
     if args.xline_split:
         ham.set_tuning(args.xline_split)
+    if args.overlap:
+        ham.set_overlap(True)
     info = ham.info()
     nb, npw = w.nbands, w.npw
 
