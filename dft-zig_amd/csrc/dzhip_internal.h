@@ -72,6 +72,9 @@ struct FftAxis {
     int radix[kMaxStages];
     int tw_off;   // element offset of this axis' twiddles in the table
     int perm_off; // element offset of this axis' permutation in the table
+    // Division constant (div_magic in fft_lds.h) of each stage's butterfly distance
+    // prod_{i < s} radix[i], so the kernels divide by it without computing it.
+    unsigned sub_magic[kMaxStages];
 };
 
 // Host-side factorisation and tables (fft_core.cpp-like helpers in fft.hip).

@@ -52,6 +52,7 @@ void AxisTables::build(int nx, int ny, int nz) {
         ax[d].n = lens[d];
         ax[d].nstages = (int)rads[d].size();
         for (int s = 0; s < kMaxStages; ++s) ax[d].radix[s] = s < ax[d].nstages ? rads[d][s] : 1;
+        for (int s = 0, sub = 1; s < kMaxStages; sub *= ax[d].radix[s], ++s) ax[d].sub_magic[s] = div_magic(sub);
         ax[d].tw_off = (int)tw_total;
         ax[d].perm_off = (int)perm_total;
         tw_total += lens[d];

@@ -190,21 +190,23 @@ struct LineSet {
     int es;
     int ls;
     const short* lines;  // nullptr: uniform stride
+    unsigned mlines;     // div_magic(nlines), computed once by the constructor
+    __host__ __device__ __forceinline__ LineSet(int n, int e, int l, const short* ln)
+        : nlines(n), es(e), ls(l), lines(ln), mlines(div_magic(n)) {}
     __host__ __device__ __forceinline__ int base(int l) const { return lines ? (int)lines[l] : l * ls; }
 };
 
 // One DIT stage (span Lp -> L = Lp*R) or DIF stage (span L -> L/R).
 template <typename T, typename C, int R, bool DIT, int S>
 __host__ __device__ __forceinline__ void fft_stage(C* buf, int n, int span, const LineSet& ls,
-                                                   const C* tw, int tid, int nthr) {
+                                                   const C* tw, unsigned m_sub, int tid, int nthr) {
     const T sign = T(S);
     const int sub = DIT ? span : span / R;  // distance between the R inputs of a butterfly (in elements)
     const int L = DIT ? span * R : span;
     const int tstep = n / L;
     const int nbf = n / R;
     const int total = ls.nlines * nbf;
-    const unsigned m_lines = div_magic(ls.nlines);
-    const unsigned m_sub = div_magic(sub);
+    const unsigned m_lines = ls.mlines;
     for (int w = tid; w < total; w += nthr) {
         const int j = fdiv(w, ls.nlines, m_lines);
         const int line = w - j * ls.nlines;
@@ -231,15 +233,15 @@ __host__ __device__ __forceinline__ void fft_stage(C* buf, int n, int span, cons
 
 template <typename T, typename C, bool DIT, int S>
 __host__ __device__ __forceinline__ void fft_stage_dispatch(int R, C* buf, int n, int span,
-                                                            const LineSet& ls, const C* tw, int tid,
-                                                            int nthr) {
+                                                            const LineSet& ls, const C* tw, unsigned m_sub,
+                                                            int tid, int nthr) {
     switch (R) {
-        case 2: fft_stage<T, C, 2, DIT, S>(buf, n, span, ls, tw, tid, nthr); break;
-        case 3: fft_stage<T, C, 3, DIT, S>(buf, n, span, ls, tw, tid, nthr); break;
-        case 4: fft_stage<T, C, 4, DIT, S>(buf, n, span, ls, tw, tid, nthr); break;
-        case 5: fft_stage<T, C, 5, DIT, S>(buf, n, span, ls, tw, tid, nthr); break;
-        case 8: fft_stage<T, C, 8, DIT, S>(buf, n, span, ls, tw, tid, nthr); break;
-        default: fft_stage<T, C, 9, DIT, S>(buf, n, span, ls, tw, tid, nthr); break;
+        case 2: fft_stage<T, C, 2, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
+        case 3: fft_stage<T, C, 3, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
+        case 4: fft_stage<T, C, 4, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
+        case 5: fft_stage<T, C, 5, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
+        case 8: fft_stage<T, C, 8, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
+        default: fft_stage<T, C, 9, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
     }
 }
 
@@ -253,7 +255,7 @@ __host__ __device__ void lds_fft_lines(C* buf, const FftAxis& ax, const C* tw, c
         int Lp = 1;
         for (int s = 0; s < ax.nstages; ++s) {
             const int R = ax.radix[s];
-            fft_stage_dispatch<T, C, true, S>(R, buf, n, Lp, ls, tw, tid, nthr);
+            fft_stage_dispatch<T, C, true, S>(R, buf, n, Lp, ls, tw, ax.sub_magic[s], tid, nthr);
             Lp *= R;
             DZ_BARRIER();
         }
@@ -261,7 +263,7 @@ __host__ __device__ void lds_fft_lines(C* buf, const FftAxis& ax, const C* tw, c
         int L = n;
         for (int s = ax.nstages - 1; s >= 0; --s) {
             const int R = ax.radix[s];
-            fft_stage_dispatch<T, C, false, S>(R, buf, n, L, ls, tw, tid, nthr);
+            fft_stage_dispatch<T, C, false, S>(R, buf, n, L, ls, tw, ax.sub_magic[s], tid, nthr);
             L /= R;
             DZ_BARRIER();
         }
