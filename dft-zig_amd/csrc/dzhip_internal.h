@@ -80,6 +80,9 @@ struct FftAxis {
 // Host-side factorisation and tables (fft_core.cpp-like helpers in fft.hip).
 bool factor_fft_len(int n, std::vector<int>& radices);
 std::vector<int> digit_reverse_perm(int n, const std::vector<int>& radices);
+// Axis of length n with the given stage radices: fills n, nstages, radix[] and sub_magic[]
+// (table offsets left 0).
+FftAxis make_fft_axis(int n, const std::vector<int>& radices);
 
 // Device-resident set of axis tables for a grid (up to 3 distinct lengths).
 struct AxisTables {

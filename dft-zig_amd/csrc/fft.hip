@@ -36,6 +36,15 @@ static int dr_pos(int i, int n, const int* rad, int nrad) {
     return (i % R) * m + dr_pos(i / R, m, rad, nrad - 1);
 }
 
+FftAxis make_fft_axis(int n, const std::vector<int>& radices) {
+    FftAxis ax{};
+    ax.n = n;
+    ax.nstages = (int)radices.size();
+    for (int s = 0; s < kMaxStages; ++s) ax.radix[s] = s < ax.nstages ? radices[s] : 1;
+    for (int s = 0, sub = 1; s < kMaxStages; sub *= ax.radix[s], ++s) ax.sub_magic[s] = div_magic(sub);
+    return ax;
+}
+
 std::vector<int> digit_reverse_perm(int n, const std::vector<int>& radices) {
     std::vector<int> p(n);
     for (int i = 0; i < n; ++i) p[i] = dr_pos(i, n, radices.data(), (int)radices.size());
@@ -49,10 +58,7 @@ void AxisTables::build(int nx, int ny, int nz) {
     for (int d = 0; d < 3; ++d) {
         DZ_REQUIRE(factor_fft_len(lens[d], rads[d]), DZHIP_ERR_ARG,
                    "FFT length " + std::to_string(lens[d]) + " is not 2/3/5-smooth or exceeds 256");
-        ax[d].n = lens[d];
-        ax[d].nstages = (int)rads[d].size();
-        for (int s = 0; s < kMaxStages; ++s) ax[d].radix[s] = s < ax[d].nstages ? rads[d][s] : 1;
-        for (int s = 0, sub = 1; s < kMaxStages; sub *= ax[d].radix[s], ++s) ax[d].sub_magic[s] = div_magic(sub);
+        ax[d] = make_fft_axis(lens[d], rads[d]);
         ax[d].tw_off = (int)tw_total;
         ax[d].perm_off = (int)perm_total;
         tw_total += lens[d];

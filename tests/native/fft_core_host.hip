@@ -20,10 +20,7 @@ static double run(int n, bool dit, double sign, int nlines, int es, int ls, bool
     using C = typename C2T<T>::type;
     std::vector<int> rad;
     if (!factor_fft_len(n, rad)) return 1e9;
-    FftAxis ax{};
-    ax.n = n;
-    ax.nstages = (int)rad.size();
-    for (int s = 0; s < ax.nstages; ++s) ax.radix[s] = rad[s];
+    const FftAxis ax = make_fft_axis(n, rad);
     std::vector<int> perm = digit_reverse_perm(n, rad);
     std::vector<C> tw(n);
     for (int t = 0; t < n; ++t) {
