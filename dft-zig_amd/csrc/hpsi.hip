@@ -185,15 +185,6 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_xline(
     }
 }
 
-// x-line pass with one x-line per thread held in registers (nx fixed at compile time): thread
-// l = z ny + y of band blockIdx.x loads its values of the np sphere planes (consecutive threads
-// read consecutive addresses of each plane), runs the inverse x-transform, multiplies by V_loc/N
-// from the x-major copy vt[x][z][y], runs the forward transform and stores the planes back.  No
-// LDS and no barrier.  The planes are the cyclic run x = xs + p (mod NX), p < np, and the line is
-// held rotated, a[u] = value at x = xs + u: the x-part of H is a convolution over the frequency
-// index (a product in real space), so the shift passes through it unchanged and the result lands
-// at the same rotated positions.  Row u is then plane u for u < np and zero above (never loaded),
-// and the twiddles are compile-time constants.
 // Raw buffer loads and stores (gfx9 buffer resource: base, byte size, no stride): the row offset
 // is a scalar operand and the per-thread offset one VGPR for every row, so no 64-bit address is
 // formed per access; accesses at or past the size read zero and write nothing.
@@ -218,6 +209,15 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned vof
     else __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(BufU4, v), r, voff, soff, 0);
 }
 
+// x-line pass with one x-line per thread held in registers (nx fixed at compile time): thread
+// l = z ny + y of band blockIdx.x loads its values of the np sphere planes (consecutive threads
+// read consecutive addresses of each plane), runs the inverse x-transform, multiplies by V_loc/N
+// from the x-major copy vt[x][z][y], runs the forward transform and stores the planes back.  No
+// LDS and no barrier.  The planes are the cyclic run x = xs + p (mod NX), p < np, and the line is
+// held rotated, a[u] = value at x = xs + u: the x-part of H is a convolution over the frequency
+// index (a product in real space), so the shift passes through it unchanged and the result lands
+// at the same rotated positions.  Row u is then plane u for u < np and zero above (never loaded),
+// and the twiddles are compile-time constants.
 template <typename T, int NX>
 __global__ __launch_bounds__(256) void k_xline_reg(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
                                                    const T* __restrict__ vt) {
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void k_xline_reg(typename C2T<T>::type* __rest
 // two lanes 2l, 2l + 1 share x-line l and each runs transforms of length M = NX / 2 only.  With
 // the rotated rows b[u] (u < M, zero above), the real-space values at r = 2m + t are
 //   psi(2m + t) = sum_u (b[u] e^{+2 pi i u t / NX}) e^{+2 pi i u m / M},
-// an inverse length-M transform of b twisted by w^{-u t}; lane t multiplies its values by
+// an inverse length-M transform of b twisted by e^{+2 pi i u t / NX}; lane t multiplies its values by
 // V_loc(2m + t)/N, and the forward result at rows k < np is
 //   X[k] = F_0[k] + e^{-2 pi i k / NX} F_1[k],  F_t = forward length-M transform of lane t,
 // so lane 1 twists its transform, one DPP exchange adds it to lane 0's, and lane 0 stores.
