@@ -654,8 +654,9 @@ constexpr int kNlPitch = kNlK + 1;       // odd pitches (in complex elements) ag
 template <typename T> struct NlTile { static constexpr int kBlocks = 4; };
 template <> struct NlTile<double> { static constexpr int kBlocks = 2; };
 template <typename T> constexpr int nl_p_tile() { return 16 * NlTile<T>::kBlocks; }  // projectors / tile
-// Back-projection G tiles of two 16-wide blocks (one per wavefront half): 373 tiles of 64 G at
-// the bench size leave a quarter of two-per-CU slots idle, 746 tiles of 32 G fill three per CU.
+// Back-projection G tiles of two 16-wide blocks (one per wavefront half): at the bench size 746
+// tiles of 32 G even out the work per CU better than 373 tiles of 64 G on 256 CUs (two
+// workgroups fit per CU either way).
 template <typename T> struct NlBackTile { static constexpr int kBlocks = 2; };
 template <typename T> constexpr int nl_g_tile() { return 16 * NlBackTile<T>::kBlocks; }  // G / tile
 
