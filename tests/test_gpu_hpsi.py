@@ -133,6 +133,14 @@ def test_hpsi_errors(dz, ctx):
         ham.set_local(w.local_r[:-1])
     s = ham.apply_s(w.psi)
     np.testing.assert_array_equal(s, w.psi)
+    # An empty block (0 columns) is a no-op in every precision and entry point; with n wrong it is
+    # still InvalidMatrixSize.
+    empty = np.empty((0, w.npw), dtype=np.complex128)
+    for prec in (0, 1):
+        assert ham.apply(empty, prec).shape == (0, w.npw)
+        assert ham.apply_nonlocal(empty, prec).shape == (0, w.npw)
+    with pytest.raises(dz.InvalidMatrixSize):
+        ham.apply(np.empty((0, w.npw - 1), dtype=np.complex128))
     # No local potential yet -> BadState.
     ham2 = dz.Hamiltonian(ctx, w.grid, tables.fft_indices(w.basis, w.grid), w.basis.kinetic,
                           w.basis.cart, None, w.inv_volume)
@@ -201,21 +209,24 @@ def test_hpsi_every_xline_split(dz, ctx, prec):
 # (tiny: 5 of 8 x values, no; nx = 75: odd, no).
 REGISTER_KERNELS = {"tiny": {"one": (0, 1), "pair": ()}, "si8": {"one": (0, 1), "pair": (0, 1)},
                     "bench": {"one": (1,), "pair": (0, 1)}, "bench75": {"one": (1,), "pair": ()},
-                    "bench80": {"one": (1,), "pair": (0, 1)}, "bench90": {"one": (1,), "pair": (0, 1)}}
+                    "bench80": {"one": (1,), "pair": (0, 1)}, "bench90": {"one": (1,), "pair": (0, 1)},
+                    "bench100": {"one": (1,), "pair": (0, 1)}}
 
 
 @pytest.mark.parametrize("which,prec", [("tiny", 1), ("si8", 1), ("bench", 1), ("bench75", 1), ("bench90", 1),
-                                        ("bench80", 1), ("tiny", 0), ("si8", 0), ("bench", 0), ("bench90", 0)])
+                                        ("bench80", 1), ("bench100", 1), ("tiny", 0), ("si8", 0), ("bench", 0),
+                                        ("bench90", 0), ("bench100", 0)])
 def test_hpsi_register_xline_vs_oracle(dz, ctx, which, prec):
     """The register-resident x-line kernels (one x-line per thread, or per lane pair with
     half-length transforms; Stockham in VGPRs) on grids of different factorisations (nx = 8, 32,
-    72, 75, 80, 90; radices 4/2/3/5/9) against the FP64 oracle apply: FP32 tolerance, and 1e-12
-    in FP64.  A variant that cannot run is refused (both precisions) or runs the LDS kernel (one
-    precision only)."""
+    72, 75, 80, 90 and 100, the largest built, on a 100 x 72 x 72 grid; radices 4/2/3/5/9) against
+    the FP64 oracle apply: FP32 tolerance, and 1e-12 in FP64.  A variant that cannot run is refused
+    (both precisions) or runs the LDS kernel (one precision only)."""
     make = {"tiny": workloads.tiny, "si8": workloads.si8, "bench": workloads.bench,
             "bench75": lambda: workloads.bench(nbands=8, grid=(75, 75, 75)),
             "bench80": lambda: workloads.bench(nbands=8, grid=(80, 72, 75)),
-            "bench90": lambda: workloads.bench(nbands=8, grid=(90, 90, 90))}
+            "bench90": lambda: workloads.bench(nbands=8, grid=(90, 90, 90)),
+            "bench100": lambda: workloads.bench(nbands=8, grid=(100, 72, 72))}
     w = make[which]()
     ham = make_ham(dz, ctx, w)
     key = "fp32" if prec else "fp64"
