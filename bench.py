@@ -206,6 +206,18 @@ def scf_leg(name, max_iter, log=sys.stderr):
     return out
 
 
+def eig_profile(ham):
+    """Section times of one cold and two warm FP64 Davidson solves (128 bands, 12 iterations,
+    diagonal start) on the bench context, one JSON line per solve."""
+    for rep in ("cold", "warm", "warm"):
+        t0 = time.perf_counter()
+        _, _, st = ham.eig_solve(128, tol=1e-4, max_iter=12, init_diagonal=True, want_vectors=False)
+        wall = time.perf_counter() - t0
+        print(json.dumps({"solve": rep, "wall_ms": round(wall * 1e3, 1), "iterations": st["iterations"],
+                          "sections_ms": {k[2:-3]: round(v, 1) for k, v in st.items() if k.startswith("t_")}}),
+              flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +227,9 @@ def main():
     ap.add_argument("--scf", default="si64", choices=("none", "si8", "si64"),
                     help="also time the SCF to 1e-6 Ha (plain FP64 vs mixed+batched)")
     ap.add_argument("--scf-max-iter", type=int, default=45)
+    ap.add_argument("--eig", action="store_true",
+                    help="instead of the bench line: one cold and two warm FP64 Davidson solves of 128 bands "
+                         "on the bench workload (12 iterations), with the solver's section times")
     ap.add_argument("--overlap", action="store_true",
                     help="run the nonlocal kernels on a second stream beside the local ones (timed pass only)")
     ap.add_argument("--xline-split", type=int, default=0,
@@ -241,6 +256,9 @@ def main():
         ham.set_tuning(args.xline_split)
     if args.overlap:
         ham.set_overlap(True)
+    if args.eig:
+        eig_profile(ham)
+        return
     info = ham.info()
     nb, npw = w.nbands, w.npw
 
