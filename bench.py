@@ -1,19 +1,25 @@
 #!/usr/bin/env python3
-"""H.psi block-apply benchmark on one MI355X (the isolated H.psi kernel configuration).
+"""H.psi block-apply and SCF benchmark on one MI355X, on the Si-64 workload.
 
-Workload (synthetic, seeded): cubic box L = 20.5 bohr, ecut 15 Ha, 72^3 grid, N_pw = 23,847,
-160 bands, V_loc = 64 Gaussians, 64 atoms x (s + 3p) Gaussian KB projectors.  One step = one
-apply of H to the whole 160-band block with psi already resident in HBM.
+Headline (`value`): H.psi throughput of one k-point of the Si-64 SCF system in FP64 (the
+reference's precision): 2x2x2 conventional Si cells (64 atoms displaced by N(0, 0.02 A)), 30 Ry,
+75^3 grid, k = (1/2, 1/2, 1/2) of the 2x2x2 mesh (N_pw = 23,984, 36 x-planes), 256 KB projectors,
+160 bands, local_r = V_ion + V_Hxc of the atomic start density, psi random (seeded) with
+coefficients damped by 1/(1 + |k+G|^2).  One step = one apply of H to the 160-band block with psi
+resident in HBM; the FP32 apply is timed the same way and reported beside it, with its error.
 
-value = algorithmic H.psi bytes per block / time per block (GB/s), for the FP32 apply (the
-mixed-precision SCF path); the FP64 apply is timed the same way and reported beside it.
-Algorithmic bytes per block (lower bound for any design that materialises psi(r) once per band):
+value = algorithmic H.psi bytes per block / time per block (GB/s), where
     B = nb*s_c*(2*N_pw + 2*N) + s_r*N + 4*N_pw + s_r*4*N_pw
-with s_c/s_r = 8/4 (FP32) or 16/8 (FP64).
+with s_c/s_r = 16/8 (FP64) or 8/4 (FP32): psi read + H psi written, one psi(r) round trip per band,
+V once, the int32 index map and the projector table (a lower bound for any design that
+materialises psi(r) once per band).
 
-The SCF metric (`--scf si64`, the default; `--scf none` skips it): time to 1e-6 Ha of the Si-64
-SCF (2x2x2 cubic cells, 8 k-points, 160 bands, 75^3 grid) for plain FP64 and the mixed-precision,
-k-concurrent forms with and without the learned preconditioner.
+Also reported: the isolated H.psi bench (L = 20.5 bohr cube, 15 Ha, 72^3, N_pw = 23,847, 160 bands,
+64 Gaussian s/p atoms) in FP32 and FP64; the CPU baseline (the oracle's FP64 H.psi with
+min(n_k, cores) k-threads over the 8 Si-64 k-points, an all-cores single-k variant, and the dense
+algebra of one Davidson iteration); and (`--scf si64`, the default) the SCF time to 1e-6 Ha of
+Si-64 for plain FP64 (k-points in sequence), FP64 with concurrent k-points, mixed precision with
+concurrent k-points, and the same with the learned preconditioner.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--scf si64|si8|none]
 """
@@ -34,7 +40,8 @@ sys.path.insert(0, str(ROOT / "dft-zig_amd"))
 METRIC = "SCF time-to-1e-6 Ha & H·ψ GB/s, Si-64: mixed+batched+learned vs plain FP64"
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
-TRAFFIC_FILE = ROOT / "profiles" / "traffic_bench.txt"
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_si64.txt"
+SI64_K = 7  # k = (1/2, 1/2, 1/2): the largest sphere of the Si-64 mesh
 
 
 def hpsi_bytes(nb, npw, ngrid, prec):
@@ -57,7 +64,8 @@ def kernel_model(phase, nb, npw, ngrid, planes, plane_elems, nproj, prec):
 
 
 def read_traffic():
-    """Bytes per launch of the dominant kernel from the committed PMC summary, if present."""
+    """HBM bytes per launch of each phase from the committed PMC summary, if present
+    (keys: phase name, with a _fp64 suffix for the FP64 apply)."""
     if not TRAFFIC_FILE.exists():
         return {}
     out = {}
@@ -100,68 +108,96 @@ def time_gpu(ham, x, y, ncols, prec_code, steps, warmup, torch):
     return ms, {k: v / max(calls, 1) for k, v in phases.items()}
 
 
-def cpu_baseline(w, budget_s=12.0):
-    """Oracle (numpy FP64 restatement of the reference apply) on one host core."""
-    from threadpoolctl import threadpool_limits
+def cpu_baseline(system, bands_per_k=16, cores_cap=16):
+    """The oracle's FP64 H.psi on the host (oracle/cpu_baseline.py): min(n_k, cores) k-threads over
+    the Si-64 k-points, `bands_per_k` bands of each, one FFT/BLAS thread per k-thread (the
+    reference's k-point worker scheme); an all-cores variant on one k-point; and the dense algebra
+    of one Davidson iteration at the full subspace (m = 648) with every core."""
+    from dzhip import scf as dscf
+    from oracle import adapters, cpu_baseline as cb
 
-    from oracle import adapters, hamiltonian
-
-    ctx_o = adapters.context_from_workload(w)
-    done, t_used = 0, 0.0
-    with threadpool_limits(limits=1):
-        while t_used < budget_s and done < w.nbands:
-            nb = min(4, w.nbands - done)
-            t = time.perf_counter()
-            hamiltonian.apply_hamiltonian_batched(ctx_o, w.psi[done:done + nb])
-            t_used += time.perf_counter() - t
-            done += nb
-    gbs = hpsi_bytes(done, w.npw, int(np.prod(w.grid)), "fp64") / t_used / 1e9
-    return {"value": round(gbs, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP64 H.psi (numpy pocketfft + 1-thread BLAS) on {done} of {w.nbands} bands "
-                      f"of the same workload, {t_used:.1f} s; "
-                      f"{t_used / done * 1e3:.1f} ms/band -> {t_used / done * w.nbands:.1f} s per 160-band block"}
+    host = cb.host_description()
+    cores = max(1, min(host["cores_available"], cores_cap))
+    nk = len(system.kpoints)
+    kthreads = min(nk, cores)
+    ctxs, blocks, nbytes = [], [], 0.0
+    ngrid = int(np.prod(system.grid))
+    local_r = dscf.start_potential(system)
+    for k in range(nk):
+        wk = dscf.kpoint_workload(system, k, nbands=bands_per_k, local_r=local_r)
+        ctxs.append(adapters.context_from_workload(wk))
+        blocks.append(wk.psi)
+        nbytes += hpsi_bytes(bands_per_k, wk.npw, ngrid, "fp64")
+    r = cb.kthread_hpsi(ctxs, blocks, kthreads)
+    gbs = nbytes / r["seconds"] / 1e9
+    one = cb.allcores_hpsi(ctxs[SI64_K], blocks[SI64_K], cores)
+    gbs_all = hpsi_bytes(one["bands"], ctxs[SI64_K].npw, ngrid, "fp64") / one["seconds"] / 1e9
+    npw = ctxs[SI64_K].npw
+    dav = cb.davidson_iteration_algebra(npw, 4 * system.nbands + 8, system.nbands + 4, cores)
+    per_band = r["seconds"] * kthreads / r["bands"]
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": kthreads, "kind": "port",
+            "sample": f"oracle FP64 H.psi (scipy pocketfft + numpy BLAS, one thread per k-thread) on "
+                      f"{bands_per_k} of 160 bands at each of the {nk} Si-64 k-points, {kthreads} k-threads "
+                      f"(min(n_k, cores)), {r['seconds']:.2f} s wall; about {per_band * 160:.1f} s per 160-band "
+                      f"block per k-thread",
+            "host": host,
+            "all_cores_single_k": {"value": round(gbs_all, 3), "unit": "GB/s", "cores": cores,
+                                   "sample": f"{one['bands']} bands of k = {SI64_K}, FFTs and BLAS on {cores} "
+                                             f"threads, {one['seconds']:.2f} s"},
+            "davidson_iteration_algebra_s": {k: round(v, 4) for k, v in dav.items()},
+            "davidson_iteration_note": f"m = {4 * system.nbands + 8} subspace, {system.nbands + 4} new columns, "
+                                       f"N_pw = {npw}, {cores} threads: V^H W, zheev (numpy eigh), V c and W c; "
+                                       f"add the H.psi of {system.nbands + 4} bands for a full iteration"}
 
 
 # SCF settings shared by every variant (only precision, k-point concurrency and the
 # preconditioner differ): Pulay history 8 from iteration 4, beta 0.3, model-dielectric
 # preconditioner diemac 12 (a Si setting of the reference's config), atomic start density,
-# diagonal start vectors, eigensolver tolerance min(1e-4, drho / 10) >= 1e-7 on the occupied bands.
-SCF_COMMON = dict(conv=1e-8, diemac=12.0, start="atomic", init_diagonal=True, eig_tol_ratio=0.1, eig_tol_min=1e-7)
+# diagonal start vectors, eigensolver tolerance min(1e-4, drho / 10) >= eig_tol_min on the occupied
+# bands, converged when the RMS density change is below conv.
+SCF_COMMON = dict(conv=1e-8, diemac=12.0, start="atomic", init_diagonal=True, eig_tol_ratio=0.1, eig_tol_min=1e-9)
 
 
-def scf_leg(name, max_iter, log=sys.stderr):
-    """Time-to-1e-6 Ha of the SCF for three forms: plain FP64 (k-points one after another on one
-    stream), mixed + batched (FP32 H apply with FP64 subspace algebra and an FP64 apply below a
-    1e-5 residual, every k-point on its own stream and host thread), and the same with the
-    learned preconditioner on the kinetic-shift base (trained first, on the device, on
-    residual/correction pairs of the warm-started solves of a 12-step FP64 SCF of the 8-atom Si
-    cell at the same cutoff; its collection and training time is reported, not counted; the
-    warm-up SCF steps run the base to give the network's fallback its yardstick).  E_inf is the
-    plain FP64 run's final energy (the GPU SCF is tied to the FP64 oracle SCF on the small Si
-    cell by the parity tests)."""
+def scf_leg(name, max_iter, arms, log=sys.stderr):
+    """Time-to-1e-6 Ha of the SCF for the forms in `arms`: plain FP64 ("fp64": k-points one after
+    another on one stream), FP64 with every k-point on its own stream and host thread
+    ("fp64_concurrent"), mixed precision with concurrent k-points ("mixed_batched": FP32 H apply,
+    FP64 subspace algebra, FP64 apply below a 1e-5 residual) and the same with the learned
+    preconditioner on the kinetic-shift base ("mixed_batched_learned": trained first, on the
+    device, on residual/correction pairs of the warm-started solves of a 12-step FP64 SCF of the
+    8-atom Si cell at the same cutoff; collection and training time reported, not counted).
+    E_inf is the plain FP64 run's final energy when that run converged (RMS density change below
+    conv); otherwise no time-to-1e-6 Ha or speedup is reported."""
     import dzhip
     from dzhip import precond
     from dzhip import scf as dscf
 
     system = {"si64": dscf.si64, "si8": dscf.si8_small}[name]()
     nocc = int(np.ceil(system.nelec / 2))
-    t0 = time.perf_counter()
-    tctx = dzhip.Context(0)
-    # Training system at the cutoff of the run, so the features cover its |k+G|^2 range.
-    train_systems = [dscf.silicon(ecut_ry=system.ecut_ry, nbands=24,
-                                  kpoints=([np.zeros(3), np.array([0.11, 0.23, 0.05])], [0.5, 0.5]))]
-    params, hist, nsamp = precond.train_on_scf(tctx, train_systems, base=dzhip.PRECOND_KINETIC, steps=12,
-                                               diemac=12.0, start="atomic", init_diagonal=True)
-    tctx.close()
-    train_s = time.perf_counter() - t0
-    print(f"learned preconditioner: {nsamp} samples, loss {hist[0]:.4f} -> {hist[-1]:.4f}, {train_s:.1f} s",
-          file=log, flush=True)
-    variants = [("fp64", dzhip.FP64, False, dzhip.PRECOND_KINETIC),
-                ("mixed_batched", dzhip.FP32, True, dzhip.PRECOND_KINETIC),
-                ("mixed_batched_learned", dzhip.FP32, True, dzhip.PRECOND_MLP)]
+    out = {"system": system.name, "grid": list(system.grid), "k_points": len(system.kpoints),
+           "bands": system.nbands, "settings": dict(SCF_COMMON), "max_iter": max_iter}
+    params = None
+    if "mixed_batched_learned" in arms:
+        t0 = time.perf_counter()
+        tctx = dzhip.Context(0)
+        # Training system at the cutoff of the run, so the features cover its |k+G|^2 range.
+        train_systems = [dscf.silicon(ecut_ry=system.ecut_ry, nbands=24,
+                                      kpoints=([np.zeros(3), np.array([0.11, 0.23, 0.05])], [0.5, 0.5]))]
+        params, hist, nsamp = precond.train_on_scf(tctx, train_systems, base=dzhip.PRECOND_KINETIC, steps=12,
+                                                   diemac=12.0, start="atomic", init_diagonal=True)
+        tctx.close()
+        train_s = time.perf_counter() - t0
+        print(f"learned preconditioner: {nsamp} samples, loss {hist[0]:.4f} -> {hist[-1]:.4f}, {train_s:.1f} s",
+              file=log, flush=True)
+        out["learned_training"] = {"samples": nsamp, "base": "kinetic shift", "loss_start": float(hist[0]),
+                                   "loss_end": float(hist[-1]), "seconds": round(train_s, 2)}
+    forms = {"fp64": (dzhip.FP64, False, dzhip.PRECOND_KINETIC),
+             "fp64_concurrent": (dzhip.FP64, True, dzhip.PRECOND_KINETIC),
+             "mixed_batched": (dzhip.FP32, True, dzhip.PRECOND_KINETIC),
+             "mixed_batched_learned": (dzhip.FP32, True, dzhip.PRECOND_MLP)}
     runs, setup = {}, {}
-    mlp_solves = None
-    for tag, prec, conc, pc in variants:
+    for tag in arms:
+        prec, conc, pc = forms[tag]
         drv = dscf.ScfDriver(system, concurrent_k=conc, device_grid=True)
         setup[tag] = drv.setup_s
         fallbacks = []
@@ -180,18 +216,19 @@ def scf_leg(name, max_iter, log=sys.stderr):
         finally:
             drv.close()
         if pc == dzhip.PRECOND_MLP:
-            mlp_solves = {"solves": len(fallbacks), "fell_back": sum(1 for f in fallbacks if f > 0)}
-    e_inf = runs["fp64"].energies[-1]
-    out = {"system": system.name, "grid": list(system.grid), "k_points": len(system.kpoints), "bands": system.nbands,
-           "e_inf_ry": e_inf, "e_inf_source": "plain FP64 GPU run, final iteration",
-           "settings": {k: v for k, v in SCF_COMMON.items()},
-           "learned_training": {"samples": nsamp, "base": "kinetic shift", "loss_start": float(hist[0]),
-                                "loss_end": float(hist[-1]), "seconds": round(train_s, 2),
-                                "scf_solves_fallback": mlp_solves}}
+            out["learned_training"]["scf_solves_fallback"] = {"solves": len(fallbacks),
+                                                              "fell_back": sum(1 for f in fallbacks if f > 0)}
+    ref = runs.get("fp64")
+    e_inf = ref.energies[-1] if ref is not None and ref.converged else None
+    out["e_inf_ry"] = e_inf
+    out["e_inf_source"] = ("plain FP64 GPU run, converged (RMS density change below conv)" if e_inf is not None
+                           else "none: the plain FP64 run did not converge, so no time-to-1e-6 Ha is reported")
     for tag, run in runs.items():
-        t = run.time_to(e_inf)
+        t = run.time_to(e_inf) if e_inf is not None else None
         out[tag] = {"t_1e-6_Ha_s": None if t is None else round(t, 3), "iterations": len(run.records),
-                    "converged": run.converged, "final_dE_Ha": (run.energies[-1] - e_inf) * dscf.HA_PER_RY,
+                    "converged": run.converged,
+                    "final_dE_Ha": None if e_inf is None else (run.energies[-1] - e_inf) * dscf.HA_PER_RY,
+                    "final_drho": run.records[-1].drho,
                     "wall_s": round(run.records[-1].wall_s, 3), "setup_s": round(setup[tag], 3),
                     "eig_s": round(sum(r.eig_s for r in run.records), 3),
                     "davidson_iterations": sum(r.eig_iterations for r in run.records),
@@ -199,8 +236,10 @@ def scf_leg(name, max_iter, log=sys.stderr):
                     "solver_gpu_ms": {k[2:-3]: round(sum(r.sections_ms.get(k, 0.0) for r in run.records), 1)
                                       for k in dscf.SECTIONS},
                     "fp32_applies": sum(r.fp32_applies for r in run.records)}
-    a = out["fp64"]["t_1e-6_Ha_s"]
-    for tag in ("mixed_batched", "mixed_batched_learned"):
+    a = out.get("fp64", {}).get("t_1e-6_Ha_s")
+    for tag in arms:
+        if tag == "fp64":
+            continue
         b = out[tag]["t_1e-6_Ha_s"]
         out[f"speedup_{tag}"] = None if not a or not b else round(a / b, 3)
     return out
@@ -208,7 +247,7 @@ def scf_leg(name, max_iter, log=sys.stderr):
 
 def eig_profile(ham):
     """Section times of one cold and two warm FP64 Davidson solves (128 bands, 12 iterations,
-    diagonal start) on the bench context, one JSON line per solve."""
+    diagonal start) on the Si-64 k-point context, one JSON line per solve."""
     for rep in ("cold", "warm", "warm"):
         t0 = time.perf_counter()
         _, _, st = ham.eig_solve(128, tol=1e-4, max_iter=12, init_diagonal=True, want_vectors=False)
@@ -218,6 +257,57 @@ def eig_profile(ham):
               flush=True)
 
 
+def hpsi_block(dzhip, torch, ham, w, steps, warmup):
+    """FP64 and FP32 timings of one block apply of workload w (psi resident in HBM), the phases of
+    each, and the FP32-vs-FP64 error."""
+    nb, npw = w.nbands, w.npw
+    ngrid = int(np.prod(w.grid))
+    x64 = torch.from_numpy(w.psi.view(np.float64).copy()).cuda()
+    y64 = torch.empty_like(x64)
+    x32 = torch.from_numpy(w.psi.astype(np.complex64).view(np.float32).copy()).cuda()
+    y32 = torch.empty_like(x32)
+    ms64, ph64 = time_gpu(ham, x64, y64, nb, dzhip.FP64, steps, warmup, torch)
+    ms32, ph32 = time_gpu(ham, x32, y32, nb, dzhip.FP32, steps, warmup, torch)
+    h32 = y32.cpu().numpy().view(np.complex64).reshape(nb, npw).astype(np.complex128)
+    h64 = y64.cpu().numpy().view(np.complex128).reshape(nb, npw)
+    err = float(np.max(np.linalg.norm(h32 - h64, axis=1) / np.linalg.norm(h64, axis=1)))
+    abserr = float(np.max(np.abs(h32 - h64)))
+    return {"ms64": ms64, "ph64": ph64, "ms32": ms32, "ph32": ph32,
+            "gbs64": hpsi_bytes(nb, npw, ngrid, "fp64") / (ms64 * 1e-3) / 1e9,
+            "gbs32": hpsi_bytes(nb, npw, ngrid, "fp32") / (ms32 * 1e-3) / 1e9,
+            "err": {"max_band_rel_l2": err, "max_abs": abserr}}
+
+
+def roofline(w, info, phases, prec):
+    """Roofline of the dominant kernel of an apply (per-phase HIP events on the launch stream)."""
+    nb, npw = w.nbands, w.npw
+    ngrid = int(np.prod(w.grid))
+    dom = max(phases, key=phases.get)
+    plane_elems = w.grid[1] * w.grid[2]
+    bound, amount, _ = kernel_model(dom, nb, npw, ngrid, info["planes"], plane_elems, info["projectors"], prec)
+    t_dom = phases[dom] * 1e-3
+    traffic = read_traffic().get(dom + ("_fp64" if prec == "fp64" else ""))
+    if bound == "hbm":
+        achieved = amount / t_dom / 1e9
+        return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes": amount, "avg_ms": round(phases[dom], 5), "precision": prec}
+    achieved = amount / t_dom / 1e12
+    peak = MFMA_PEAK_TFLOPS[prec]
+    return {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 2), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
+            "algorithmic_flops": amount, "avg_ms": round(phases[dom], 5), "precision": prec}
+
+
+def tuning_info(ham):
+    t = ham.info()
+    return {k: t[k] for k in ("xline_split_fp32", "xline_split_fp64", "xline_register_fp32", "xline_register_fp64",
+                              "xline_lanes_fp32", "xline_lanes_fp64")}
+
+
+SCF_ARMS = ("fp64", "fp64_concurrent", "mixed_batched", "mixed_batched_learned")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,16 +315,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--scf", default="si64", choices=("none", "si8", "si64"),
-                    help="also time the SCF to 1e-6 Ha (plain FP64 vs mixed+batched)")
-    ap.add_argument("--scf-max-iter", type=int, default=45)
+                    help="also time the SCF to 1e-6 Ha (plain FP64 vs the concurrent and mixed forms)")
+    ap.add_argument("--scf-max-iter", type=int, default=60)
+    ap.add_argument("--scf-arms", default=",".join(SCF_ARMS),
+                    help="comma-separated SCF forms to run (fp64 gives E_inf)")
+    ap.add_argument("--no-isolated", action="store_true", help="skip the isolated 72^3 H.psi bench")
     ap.add_argument("--eig", action="store_true",
                     help="instead of the bench line: one cold and two warm FP64 Davidson solves of 128 bands "
-                         "on the bench workload (12 iterations), with the solver's section times")
+                         "on the Si-64 k-point (12 iterations), with the solver's section times")
     ap.add_argument("--overlap", action="store_true",
                     help="run the nonlocal kernels on a second stream beside the local ones (timed pass only)")
     ap.add_argument("--xline-split", type=int, default=0,
-                    help="fix the x-line variant: d > 0 LDS y-slabs, -1..-3 register kernel, -4..-6 lane-pair register kernel "
-                         "(0 = autotune on the first apply)")
+                    help="fix the x-line variant: d > 0 LDS y-slabs, -1..-3 one-lane, -4..-6 lane-pair, "
+                         "-7..-9 lane-triple register kernel (0 = autotune on the first apply)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (k-point sharding is "
@@ -243,13 +336,14 @@ def main():
 
     import torch
     import dzhip
+    from dzhip import scf as dscf
     from dzhip import workloads
 
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
-    w = workloads.bench(nbands=160)
-    ngrid = int(np.prod(w.grid))
+    system = dscf.si64()
+    w = dscf.kpoint_workload(system, SI64_K, nbands=system.nbands)
     ctx = dzhip.Context(0)
     ham = dzhip.Hamiltonian.from_workload(ctx, w)
     if args.xline_split:
@@ -260,76 +354,60 @@ def main():
         eig_profile(ham)
         return
     info = ham.info()
-    nb, npw = w.nbands, w.npw
+    r = hpsi_block(dzhip, torch, ham, w, args.steps, args.warmup)
+    tuned = tuning_info(ham)
+    ham.close()
 
-    x64 = torch.from_numpy(w.psi.view(np.float64).copy()).cuda()
-    y64 = torch.empty_like(x64)
-    x32 = torch.from_numpy(w.psi.astype(np.complex64).view(np.float32).copy()).cuda()
-    y32 = torch.empty_like(x32)
+    isolated = None
+    if not args.no_isolated:
+        wb = workloads.bench(nbands=160)
+        hb = dzhip.Hamiltonian.from_workload(ctx, wb)
+        rb = hpsi_block(dzhip, torch, hb, wb, args.steps, args.warmup)
+        isolated = {"workload": "L=20.5 bohr cube, ecut 15 Ha, 72^3 grid, N_pw=23847, 160 bands, 64 atoms x "
+                                "(s+3p) Gaussian projectors, V_loc = 64 Gaussians (seeded PCG64 20261202)",
+                    "fp64": {"value": round(rb["gbs64"], 2), "unit": "GB/s", "ms_per_step": round(rb["ms64"], 4),
+                             "frac_hbm": round(rb["gbs64"] / HBM_PEAK_GBS, 4)},
+                    "fp32": {"value": round(rb["gbs32"], 2), "unit": "GB/s", "ms_per_step": round(rb["ms32"], 4),
+                             "frac_hbm": round(rb["gbs32"] / HBM_PEAK_GBS, 4)},
+                    "fp32_vs_fp64_err": rb["err"],
+                    "phases_ms_fp64": {k: round(v, 5) for k, v in rb["ph64"].items()},
+                    "phases_ms_fp32": {k: round(v, 5) for k, v in rb["ph32"].items()},
+                    "tuning": tuning_info(hb)}
+        hb.close()
 
-    ms32, ph32 = time_gpu(ham, x32, y32, nb, dzhip.FP32, args.steps, args.warmup, torch)
-    ms64, ph64 = time_gpu(ham, x64, y64, nb, dzhip.FP64, args.steps, args.warmup, torch)
-
-    tuned = ham.info()
-    h32 = y32.cpu().numpy().view(np.complex64).reshape(nb, npw).astype(np.complex128)
-    h64 = y64.cpu().numpy().view(np.complex128).reshape(nb, npw)
-    err = float(np.max(np.linalg.norm(h32 - h64, axis=1) / np.linalg.norm(h64, axis=1)))
-    abserr = float(np.max(np.abs(h32 - h64)))
-
-    gbs32 = hpsi_bytes(nb, npw, ngrid, "fp32") / (ms32 * 1e-3) / 1e9
-    gbs64 = hpsi_bytes(nb, npw, ngrid, "fp64") / (ms64 * 1e-3) / 1e9
-
-    # Roofline of the dominant kernel of the FP32 apply (per-phase HIP events, same stream).
-    dom = max(ph32, key=ph32.get)
-    plane_elems = w.grid[1] * w.grid[2]
-    bound, amount, _ = kernel_model(dom, nb, npw, ngrid, info["planes"], plane_elems, info["projectors"], "fp32")
-    t_dom = ph32[dom] * 1e-3
-    traffic = read_traffic().get(dom)
-    if bound == "hbm":
-        achieved = amount / t_dom / 1e9
-        roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "algorithmic_bytes": amount, "avg_ms": round(ph32[dom], 5)}
-    else:
-        achieved = amount / t_dom / 1e12
-        peak = MFMA_PEAK_TFLOPS["fp32"]
-        roof = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 2), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
-                "algorithmic_flops": amount, "avg_ms": round(ph32[dom], 5)}
-
-    cpu = None if args.no_cpu else cpu_baseline(w)
+    cpu = None if args.no_cpu else cpu_baseline(system)
+    arms = [a for a in args.scf_arms.split(",") if a]
 
     out = {
         "metric": METRIC,
-        "value": round(gbs32, 2),
+        "value": round(r["gbs64"], 2),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms32, 4),
+        "ms_per_step": round(r["ms64"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic (seeded PCG64 20261202: Gaussian V_loc, Gaussian s/p KB projectors, random psi)",
-        "config": {"workload": "H.psi block apply: L=20.5 bohr cube, ecut 15 Ha, 72^3 grid, N_pw=23847, "
-                               "160 bands, 64 atoms x (s+3p) projectors", "n_pw": npw, "grid": list(w.grid),
-                   "bands": nb, "projectors": info["projectors"], "x_planes": info["planes"],
-                   "parallelism": "single GPU"},
-        "roofline": roof,
+        "dtype": "f64",
+        "data": "synthetic (Si-64 structure from seeded displacements, GTH-form local part and Gaussian s/p "
+                "KB projectors, random seeded psi)",
+        "config": {"workload": "Si-64 H.psi block apply at one k-point: 2x2x2 Si cubic cells (64 atoms), 30 Ry, "
+                               "75^3 grid, k = (1/2,1/2,1/2), N_pw = 23984, 160 bands, 256 projectors",
+                   "n_pw": w.npw, "grid": list(w.grid), "bands": w.nbands, "projectors": info["projectors"],
+                   "x_planes": info["planes"], "k_index": SI64_K, "parallelism": "single GPU"},
+        "roofline": roofline(w, info, r["ph64"], "fp64"),
         "cpu_baseline": cpu,
-        "fp64": {"value": round(gbs64, 2), "unit": "GB/s", "ms_per_step": round(ms64, 4),
-                 "frac_hbm": round(gbs64 / HBM_PEAK_GBS, 4)},
-        "fp32_frac_hbm": round(gbs32 / HBM_PEAK_GBS, 4),
-        "fp32_vs_fp64_err": {"max_band_rel_l2": err, "max_abs": abserr},
-        "tuning": {"xline_split_fp32": tuned["xline_split_fp32"], "xline_split_fp64": tuned["xline_split_fp64"],
-                   "xline_register_fp32": tuned["xline_register_fp32"],
-                   "xline_register_fp64": tuned["xline_register_fp64"],
-                   "xline_lanes_fp32": tuned["xline_lanes_fp32"], "xline_lanes_fp64": tuned["xline_lanes_fp64"]},
+        "frac_hbm": round(r["gbs64"] / HBM_PEAK_GBS, 4),
+        "fp32": {"value": round(r["gbs32"], 2), "unit": "GB/s", "ms_per_step": round(r["ms32"], 4),
+                 "frac_hbm": round(r["gbs32"] / HBM_PEAK_GBS, 4), "roofline": roofline(w, info, r["ph32"], "fp32")},
+        "fp32_vs_fp64_err": r["err"],
+        "tuning": tuned,
         # Per-phase times of the profiled pass (HIP events between the phases).
-        "phases_ms_fp32": {k: round(v, 5) for k, v in ph32.items()},
-        "phases_ms_fp64": {k: round(v, 5) for k, v in ph64.items()},
-        "scf_time_to_1e-6_Ha": None if args.scf == "none" else scf_leg(args.scf, args.scf_max_iter),
+        "phases_ms_fp64": {k: round(v, 5) for k, v in r["ph64"].items()},
+        "phases_ms_fp32": {k: round(v, 5) for k, v in r["ph32"].items()},
+        "isolated_hpsi_bench": isolated,
+        "scf_time_to_1e-6_Ha": None if args.scf == "none" else scf_leg(args.scf, args.scf_max_iter, arms),
     }
     print(json.dumps(out), flush=True)
 

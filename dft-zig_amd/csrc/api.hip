@@ -296,7 +296,16 @@ int dzhip_ham_set_stream(dzhip_ham* ham, void* stream) {
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
         std::lock_guard<std::mutex> lk(ham->mu);
+        DeviceGuard g(ham->ctx->device);
+        const hipStream_t prev = ham->stream();
         ham->user_stream = reinterpret_cast<hipStream_t>(stream);
+        const hipStream_t next = ham->stream();
+        if (next == prev) return;
+        // Every apply shares this context's workspaces: the new stream waits for what the old
+        // one still has queued, so the two streams never run this context's kernels at once.
+        if (!ham->ev_switch) DZ_HIP(hipEventCreateWithFlags(&ham->ev_switch, hipEventDisableTiming));
+        DZ_HIP(hipEventRecord(ham->ev_switch, prev));
+        DZ_HIP(hipStreamWaitEvent(next, ham->ev_switch, 0));
     });
 }
 
@@ -567,8 +576,9 @@ int dzhip_ham_set_local_grid(dzhip_ham* ham, dzhip_scf_grid* grid) {
         DZ_REQUIRE(ham->ngrid == grid->n && ham->nx == grid->nx && ham->ny == grid->ny, DZHIP_ERR_SIZE,
                    "grid dimensions differ");
         DZ_REQUIRE(ham->ctx->device == grid->ctx->device, DZHIP_ERR_ARG, "ham and grid on different devices");
-        std::lock_guard<std::mutex> lg(grid->mu);
-        std::lock_guard<std::mutex> lk(ham->mu);
+        // Both calls that take a grid and a ham lock them together (std::scoped_lock orders the
+        // acquisition), so two threads calling them on the same pair cannot deadlock.
+        std::scoped_lock lk(grid->mu, ham->mu);
         DeviceGuard g(ham->ctx->device);
         scf_grid_local(*grid);
         DZ_HIP(hipStreamSynchronize(grid->stream()));
@@ -584,7 +594,7 @@ int dzhip_scf_grid_add_solution_density(dzhip_scf_grid* grid, dzhip_ham* ham, co
         DZ_REQUIRE(ham->ngrid == grid->n, DZHIP_ERR_SIZE, "grid dimensions differ");
         DZ_REQUIRE(nbands == 0 || weights, DZHIP_ERR_ARG, "NULL weights");
         if (nbands == 0) return;
-        std::lock_guard<std::mutex> lk(ham->mu);
+        std::scoped_lock lk(grid->mu, ham->mu);
         DeviceGuard g(ham->ctx->device);
         uint64_t cols = 0;
         const double2* x = eig_solution(*ham, &cols);
@@ -595,7 +605,6 @@ int dzhip_scf_grid_add_solution_density(dzhip_scf_grid* grid, dzhip_ham* ham, co
         DZ_HIP(hipMemcpyAsync(ham->dens_w.ptr, weights, nbands * sizeof(double), hipMemcpyHostToDevice, s));
         ham_density_dev(*ham, x, nbands, ham->dens_w.as<double>(), ham->dens_rho.as<double>(), false);
         DZ_HIP(hipStreamSynchronize(s));
-        std::lock_guard<std::mutex> lg(grid->mu);
         scf_grid_add(*grid, ham->dens_rho.as<double>());
         DZ_HIP(hipStreamSynchronize(grid->stream()));
     });
@@ -660,8 +669,8 @@ int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split) {
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
         DZ_REQUIRE(xline_split == 0 || xline_code_usable(*ham, xline_split, false), DZHIP_ERR_ARG,
-                   "xline_split must divide ny, or be -1..-3 (-4..-6) with a (lane-pair) register kernel for this "
-                   "context (0 = autotune)");
+                   "xline_split must divide ny, or be -1..-3 (-4..-6, -7..-9) with a (lane-pair, lane-triple) "
+                   "register kernel for this context (0 = autotune)");
         std::lock_guard<std::mutex> lk(ham->mu);
         ham->tune_fixed = xline_split;
         ham->xline_split[0] = ham->xline_split[1] = 0;
@@ -681,7 +690,7 @@ int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out) {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
         auto slabs = [](int code) { return code > 0 ? (uint64_t)code : 0; };
         auto reg = [](int code) { return code < 0 ? (uint64_t)xline_reg_threads(code) : 0u; };
-        auto waves = [](int code) { return code < 0 ? (uint64_t)(xline_reg_pair(code) ? 2 : 1) : 0u; };
+        auto waves = [](int code) { return code < 0 ? (uint64_t)xline_reg_lanes(code) : 0u; };
         const uint64_t v[11] = {ham->npw, ham->ngrid, (uint64_t)ham->nplanes, (uint64_t)ham->nproj,
                                 ham->device_bytes(), slabs(ham->xline_split[1]), slabs(ham->xline_split[0]),
                                 reg(ham->xline_split[1]), reg(ham->xline_split[0]), waves(ham->xline_split[1]),

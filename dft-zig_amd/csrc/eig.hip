@@ -459,17 +459,27 @@ struct Solver {
     // Orthonormalise Q[:, 0:k] in place: Cholesky QR, or, when the Gram matrix is not
     // numerically positive definite, its eigen-decomposition G = U L U^H with the directions of
     // L < 1e-12 max(L) dropped and Q <- Q U L^{-1/2} (compacted to the front).  Returns the
-    // number of columns kept.
+    // number of columns kept.  Cholesky counts as failed also when a pivot keeps less than 1e-7 of
+    // its column's norm (R_jj < 1e-7 sqrt(G_jj)): such a column is numerically dependent on the
+    // ones before it, and dividing by its pivot would turn rounding noise into a basis vector.
     int ortho_block(double2* Q, int k) {
         const int ldn = (int)n;
         gram(Q, k, Q, k, w.G.as<double2>(), k);
         DZ_HIP(hipMemcpyAsync(w.S.ptr, w.G.ptr, (size_t)k * k * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        std::vector<double> gdiag(k), rdiag(k);
+        // Real parts of the diagonals: one double every k + 1 complex elements.
+        DZ_HIP(hipMemcpy2DAsync(gdiag.data(), sizeof(double), w.G.ptr, (size_t)(k + 1) * sizeof(double2),
+                                sizeof(double), k, hipMemcpyDeviceToHost, s));
         DZ_BLAS(rocsolver_zpotrf(w.blas, rocblas_fill_upper, k, w.G.as<zc>(), k, w.info.as<rocblas_int>()));
         rocblas_int info = 0;
         DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
+        DZ_HIP(hipMemcpy2DAsync(rdiag.data(), sizeof(double), w.G.ptr, (size_t)(k + 1) * sizeof(double2),
+                                sizeof(double), k, hipMemcpyDeviceToHost, s));
         DZ_HIP(hipStreamSynchronize(s));
         const zc one(1.0, 0.0), zero(0.0, 0.0);
-        if (info == 0) {
+        bool pivots_ok = info == 0;
+        for (int j = 0; j < k && pivots_ok; ++j) pivots_ok = rdiag[j] > 1e-7 * std::sqrt(std::max(gdiag[j], 0.0));
+        if (pivots_ok) {
             DZ_BLAS(rocblas_ztrsm(w.blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                                   rocblas_diagonal_non_unit, ldn, k, &one, w.G.as<zc>(), k, reinterpret_cast<zc*>(Q), ldn));
             return k;
@@ -529,43 +539,50 @@ struct Solver {
     // Orthonormal start block, its H image and T.  Columns [0, cols) come from the host start
     // vectors or, with reuse, from the previous solve's Ritz vectors still in X; the rest are
     // unit vectors at the smallest diag entries or LCG values (common.zig:287-319), with the
-    // generator advanced past the columns that were supplied.
+    // generator advanced past the columns that were supplied.  A start block that is not of full
+    // rank (duplicated or dependent columns) keeps its independent directions and is completed
+    // with fresh LCG vectors orthogonalised against them, up to three rounds, as
+    // orthonormalize_all replaces dependent columns (common.zig:298-312).
     void start(int m, const double* init, uint64_t init_cols, bool reuse, const double* diag_host) {
         size_t cols = 0;
         if (init && init_cols > 0) cols = std::min<uint64_t>(init_cols, (uint64_t)m);
         else if (reuse) cols = std::min<uint64_t>(w.x_cols, (uint64_t)m);
         const size_t rest = (size_t)m - cols;
-        std::vector<double2> v(n * rest);
-        if (o.init_diagonal) {
-            std::vector<size_t> order(n);
-            for (size_t i = 0; i < n; ++i) order[i] = i;
-            std::stable_sort(order.begin(), order.end(),
-                             [&](size_t a, size_t b) { return diag_host[a] < diag_host[b]; });
-            std::fill(v.begin(), v.end(), make_double2(0.0, 0.0));
-            for (size_t c = 0; c < rest; ++c) v[c * n + order[cols + c]] = make_double2(1.0, 0.0);
-        } else {
-            const unsigned long long A = 6364136223846793005ULL, C = 1ULL;
-            // x_{k+j} = Aj x_k + Cj: affine map of j steps by repeated squaring.
-            unsigned long long aj = 1, cj = 0, ap = A, cp = C;
-            for (unsigned long long j = 2ULL * n * cols; j; j >>= 1) {
-                if (j & 1) {
-                    aj = ap * aj;
-                    cj = ap * cj + cp;
-                }
-                cp = ap * cp + cp;
-                ap = ap * ap;
+        const unsigned long long A = 6364136223846793005ULL, C = 1ULL;
+        // x_{k+j} = Aj x_k + Cj: affine map of j steps by repeated squaring.
+        unsigned long long aj = 1, cj = 0, ap = A, cp = C;
+        for (unsigned long long j = 2ULL * n * cols; j; j >>= 1) {
+            if (j & 1) {
+                aj = ap * aj;
+                cj = ap * cj + cp;
             }
-            unsigned long long x = aj * seed + cj;
-            auto next = [&]() {
-                x = x * A + C;
-                return (double)(x >> 11) / 9007199254740992.0;
-            };
-            for (size_t c = 0; c < rest; ++c)
+            cp = ap * cp + cp;
+            ap = ap * ap;
+        }
+        unsigned long long x = aj * seed + cj;
+        auto next = [&]() {
+            x = x * A + C;
+            return (double)(x >> 11) / 9007199254740992.0;
+        };
+        auto lcg_block = [&](std::vector<double2>& v, size_t ncol) {
+            v.resize(n * ncol);
+            for (size_t c = 0; c < ncol; ++c)
                 for (size_t i = 0; i < n; ++i) {
                     const double re = next() - 0.5;
                     const double im = next() - 0.5;
                     v[c * n + i] = make_double2(re, im);
                 }
+        };
+        std::vector<double2> v;
+        if (o.init_diagonal) {
+            std::vector<size_t> order(n);
+            for (size_t i = 0; i < n; ++i) order[i] = i;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](size_t a, size_t b) { return diag_host[a] < diag_host[b]; });
+            v.assign(n * rest, make_double2(0.0, 0.0));
+            for (size_t c = 0; c < rest; ++c) v[c * n + order[cols + c]] = make_double2(1.0, 0.0);
+        } else {
+            lcg_block(v, rest);
         }
         if (init && init_cols > 0)
             DZ_HIP(hipMemcpyAsync(V(), init, cols * n * sizeof(double2), hipMemcpyHostToDevice, s));
@@ -574,8 +591,20 @@ struct Solver {
         if (rest > 0)
             DZ_HIP(hipMemcpyAsync(V() + cols * n, v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice, s));
         DZ_HIP(hipStreamSynchronize(s));
-        cholqr(V(), m);
-        cholqr(V(), m);
+        int k = ortho_block(V(), m);
+        if (k > 0) k = ortho_block(V(), k);
+        for (int round = 0; round < 3 && k < m; ++round) {
+            const int need = m - k;
+            lcg_block(v, (size_t)need);
+            DZ_HIP(hipMemcpyAsync(w.Q.ptr, v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+            const int got = orthonormalize_new(w.Q.as<double2>(), need, k);
+            if (got > 0)
+                DZ_HIP(hipMemcpyAsync(V() + (size_t)k * n, w.Q.ptr, (size_t)got * n * sizeof(double2),
+                                      hipMemcpyDeviceToDevice, s));
+            DZ_HIP(hipStreamSynchronize(s));  // v is refilled next round
+            k += got;
+        }
+        DZ_REQUIRE(k == m, DZHIP_ERR_STATE, "start block: no orthonormal basis of the requested size");
         apply(V(), W(), m);
         extend_T(0, m);
     }

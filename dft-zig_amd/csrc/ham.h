@@ -63,8 +63,8 @@ struct Ham {
     // Work buffers.
     DevBuf xstage, ystage, x32, y32, tbuf, partial, qbuf;
     // Tuned launch shapes: x-line variant per precision ([0] FP64, [1] FP32): 0 not yet tuned,
-    // d >= 1 the LDS kernel with d y-slabs, -1..-6 a register kernel (xline_reg_threads,
-    // xline_reg_pair); tune_fixed != 0 forces a variant (sweeps).
+    // d >= 1 the LDS kernel with d y-slabs, -1..-9 a register kernel (xline_reg_threads,
+    // xline_reg_lanes); tune_fixed != 0 forces a variant (sweeps).
     int xline_split[2] = {0, 0};
     int tune_fixed = 0;
 
@@ -99,6 +99,9 @@ struct Ham {
     bool overlap = false;
     hipStream_t nl_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // Stream switch (dzhip_ham_set_stream): recorded on the old stream, waited on by the new one,
+    // so work enqueued before the switch (which shares the workspaces) completes first.
+    hipEvent_t ev_switch = nullptr;
 
     size_t device_bytes() const;
     std::mutex mu;
@@ -159,11 +162,17 @@ bool xline_reg_usable(const Ham& h, bool fp64);
 // built for nx and can run here: the sphere spans at most half the line.
 bool xline_pair_supported(int nx);
 bool xline_pair_usable(const Ham& h);
+// ... and the lane-triple kernel (three lanes per x-line, third-length transforms, any sphere
+// width): nx divisible by 3 with nx / 3 = 2^a 3^b 5^c.
+bool xline_tri_supported(int nx);
+bool xline_tri_usable(const Ham& h);
 // Register x-line variants: -1 / -2 / -3 = one lane per x-line in workgroups of 256 / 128 / 64
-// threads, -4 / -5 / -6 = the lane-pair kernel in the same workgroup sizes.
-constexpr int kXlineReg = -1, kXlineRegLast = -6;
+// threads, -4 / -5 / -6 = the lane-pair kernel, -7 / -8 / -9 = the lane-triple kernel, in the same
+// workgroup sizes.
+constexpr int kXlineReg = -1, kXlineRegLast = -9;
 constexpr int xline_reg_threads(int code) { return 256 >> ((-code - 1) % 3); }
-constexpr bool xline_reg_pair(int code) { return code <= -4; }
+constexpr int xline_reg_lanes(int code) { return 1 + (-code - 1) / 3; }
+constexpr bool xline_reg_pair(int code) { return xline_reg_lanes(code) == 2; }
 // Whether variant `code` (d >= 1 LDS slabs, or a register variant) can run on h in a precision.
 bool xline_code_usable(const Ham& h, int code, bool fp64);
 void ham_set_local(Ham& h, const double* local_r);

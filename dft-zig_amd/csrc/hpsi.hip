@@ -333,6 +333,109 @@ __global__ __launch_bounds__(256) void k_xline_pair(typename C2T<T>::type* __res
     }
 }
 
+// Lane-triple variant for lines divisible by 3 (M = NX / 3), any sphere width: lanes 3l, 3l + 1,
+// 3l + 2 of a wavefront share x-line l (21 lines per wavefront, its last lane idle).  With the
+// rotated rows b[u] (u < np, zero above) and w = e^{2 pi i / NX}, the real-space values at
+// r = 3m + t are an inverse length-M transform of the folded, twisted rows
+//   c_t[u0] = sum_j b[u0 + j M] w^{(u0 + j M) t},   u0 < M,
+// because w^{(u0 + jM) 3m} = e^{2 pi i u0 m / M}.  Lane t multiplies by V_loc(3m + t)/N and runs the
+// forward length-M transform F_t; with H_t[k0] = w^{-k0 t} F_t[k0] the result is a 3-point
+// transform across the lanes,
+//   X[k0 + j M] = sum_t e^{-2 pi i j t / 3} H_t[k0],
+// so lane j gathers its partners' H_t[k0] (ds_bpermute) and stores the rows k0 + j M < np.  A
+// third of the registers per lane of k_xline_reg: odd lines (75 = 3 x 25) fit in FP64.
+template <typename X>
+__device__ __forceinline__ X lane_read(X v, int src_lane) {
+    static_assert(sizeof(X) % 4 == 0, "32-bit words");
+    constexpr int kWords = sizeof(X) / 4;
+    struct Words {
+        int w[kWords];
+    };
+    Words in = __builtin_bit_cast(Words, v), out;
+#pragma unroll
+    for (int i = 0; i < kWords; ++i) out.w[i] = __builtin_amdgcn_ds_bpermute(src_lane << 2, in.w[i]);
+    return __builtin_bit_cast(X, out);
+}
+
+template <typename T, int NX>
+__global__ __launch_bounds__(256) void k_xline_tri(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
+                                                   const T* __restrict__ vt) {
+    using C = typename C2T<T>::type;
+    static_assert(NX % 3 == 0, "lane-triple kernel needs a line divisible by 3");
+    constexpr int M = NX / 3;
+    constexpr int kLinesPerWave = 21;
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = (int)((blockIdx.y * blockDim.x + threadIdx.x) >> 6);
+    // Every lane stays active to the end (the exchange reads partner registers); lanes past the last
+    // line and the idle lane 63 work on a valid line and store nothing.
+    const int t = lane < 63 ? lane % 3 : 0;
+    const int base_lane = lane < 63 ? lane - t : 63;
+    const int lraw = wave * kLinesPerWave + (lane < 63 ? lane / 3 : 0);
+    const bool store_ok = lane < 63 && lraw < lines;
+    const int l = lraw < lines ? lraw : lines - 1;
+    const int row = lines * (int)sizeof(C), vrow = lines * (int)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tbuf + (size_t)blockIdx.x * nplanes * lines, nplanes * row);
+    const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vt, NX * vrow);
+    const unsigned vo = (unsigned)l * (unsigned)sizeof(C);
+    const unsigned vvo = (unsigned)l * (unsigned)sizeof(T) + (unsigned)(t * vrow);
+    const bool t1 = t == 1, t2 = t == 2;
+    C c[M];
+#pragma unroll
+    for (int u = 0; u < M; ++u) c[u] = C{T(0), T(0)};
+    // Fold the present rows with the lane's twist w^{u t} (literal twiddles selected per lane).
+#pragma unroll
+    for (int u = 0; u < NX; ++u) {
+        if (u < nplanes) {
+            const C a = buf_load<C>(rt, vo, u * row);
+            const int e1 = u % NX, e2 = (2 * u) % NX;
+            const T wc = t1 ? T(freg::kRoots<NX>.c[e1]) : (t2 ? T(freg::kRoots<NX>.c[e2]) : T(1));
+            const T ws = t1 ? T(freg::kRoots<NX>.s[e1]) : (t2 ? T(freg::kRoots<NX>.s[e2]) : T(0));
+            C& d = c[u % M];
+            d.x += a.x * wc - a.y * ws;
+            d.y += a.x * ws + a.y * wc;
+        }
+    }
+    freg::fft<M, 1>(c);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const T v = buf_load<T>(rv, vvo, 3 * m * vrow);
+        c[m] = C{c[m].x * v, c[m].y * v};
+    }
+    freg::fft<M, -1>(c);
+    // H_t[k0] = w^{-k0 t} F_t[k0].
+#pragma unroll
+    for (int k = 1; k < M; ++k) {
+        const int e1 = k % NX, e2 = (2 * k) % NX;
+        const T wc = t1 ? T(freg::kRoots<NX>.c[e1]) : (t2 ? T(freg::kRoots<NX>.c[e2]) : T(1));
+        const T ws = t1 ? T(freg::kRoots<NX>.s[e1]) : (t2 ? T(freg::kRoots<NX>.s[e2]) : T(0));
+        const T px = c[k].x * wc + c[k].y * ws, py = c[k].y * wc - c[k].x * ws;
+        c[k].x = px;
+        c[k].y = py;
+    }
+    // Lane j holds H_j; its partners hold H_{j+1}, H_{j+2} (indices mod 3).  X[k0 + j M] =
+    // H_0 + e^{-2 pi i j / 3} H_1 + e^{-4 pi i j / 3} H_2, written as own + f1 next + f2 after-next
+    // with per-lane factors: j = 0: 1, 1, 1; j = 1: w3^-1, w3^+1, 1; j = 2: w3^-1, 1, w3^+1
+    // (w3 = e^{2 pi i / 3}).
+    const int p1 = base_lane + (t + 1) % 3, p2 = base_lane + (t + 2) % 3;
+    constexpr T kC3 = T(-0.5), kS3 = T(0.86602540378443864676);
+    const T own_c = t == 0 ? T(1) : kC3, own_s = t == 0 ? T(0) : -kS3;
+    const T n1_c = t1 ? kC3 : T(1), n1_s = t1 ? kS3 : T(0);
+    const T n2_c = t2 ? kC3 : T(1), n2_s = t2 ? kS3 : T(0);
+    int np = nplanes, rs = row;
+    asm volatile("" : "+s"(np), "+s"(rs));
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        if (k < np) {
+            const C h1 = lane_read(c[k], p1), h2 = lane_read(c[k], p2);
+            const C o = c[k];
+            const T xr = o.x * own_c - o.y * own_s + h1.x * n1_c - h1.y * n1_s + h2.x * n2_c - h2.y * n2_s;
+            const T xi = o.x * own_s + o.y * own_c + h1.x * n1_s + h1.y * n1_c + h2.x * n2_s + h2.y * n2_c;
+            const int kk = k + t * M;
+            if (store_ok && kk < np) buf_store<C>(rt, vo, kk * rs, C{xr, xi});
+        }
+    }
+}
+
 // Line lengths the register x-line kernel is built for: FP32 from 8 to 100, FP64 (twice the
 // registers per line) up to 50, beyond which the line no longer fits the register file.
 #define DZ_XREG_SIZES(X) X(8) X(12) X(15) X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) \
@@ -343,6 +446,21 @@ __global__ __launch_bounds__(256) void k_xline_pair(typename C2T<T>::type* __res
 // Even lines whose half length the register FFT takes, for the lane-pair kernel (both precisions).
 #define DZ_XPAIR_SIZES(X) X(8) X(12) X(16) X(18) X(20) X(24) X(30) X(32) X(36) X(40) X(48) X(50) X(54) X(60) \
     X(64) X(72) X(80) X(90) X(96) X(100)
+
+// Lines divisible by 3 whose third the register FFT takes, for the lane-triple kernel (both precisions).
+#define DZ_XTRI_SIZES(X) X(12) X(15) X(18) X(24) X(27) X(30) X(36) X(45) X(48) X(54) X(60) X(72) X(75) X(81) \
+    X(90) X(96)
+
+bool xline_tri_supported(int nx) {
+    switch (nx) {
+#define DZ_XREG_CASE(N) case N:
+        DZ_XTRI_SIZES(DZ_XREG_CASE)
+#undef DZ_XREG_CASE
+        return true;
+        default:
+            return false;
+    }
+}
 
 bool xline_pair_supported(int nx) {
     switch (nx) {
@@ -385,20 +503,44 @@ bool xline_pair_usable(const Ham& h) {
            h.ngrid * 2 * sizeof(double) < ((size_t)1 << 31);
 }
 
+bool xline_tri_usable(const Ham& h) {
+    return xline_tri_supported(h.nx) && h.plane_xs >= 0 && h.ngrid * 2 * sizeof(double) < ((size_t)1 << 31);
+}
+
 bool xline_code_usable(const Ham& h, int code, bool fp64) {
     if (code > 0) return h.ny % code == 0;
     if (code < kXlineRegLast || code > kXlineReg) return false;
-    return xline_reg_pair(code) ? xline_pair_usable(h) : xline_reg_usable(h, fp64);
+    switch (xline_reg_lanes(code)) {
+        case 1: return xline_reg_usable(h, fp64);
+        case 2: return xline_pair_usable(h);
+        default: return xline_tri_usable(h);
+    }
 }
 
 template <typename T>
-void launch_xline_reg(Ham& h, typename C2T<T>::type* tb, size_t nc, int threads, bool pair, hipStream_t s) {
+void launch_xline_reg(Ham& h, typename C2T<T>::type* tb, size_t nc, int threads, int nlanes, hipStream_t s) {
     const int lines = h.ny * h.nz;
-    const int lanes = pair ? 2 * lines : lines;
-    const dim3 grid((unsigned)nc, (unsigned)((lanes + threads - 1) / threads));
+    const bool pair = nlanes == 2;
+    // The triple kernel puts 21 lines on each wavefront (63 of its 64 lanes).
+    const int waves_per_group = threads / 64;
+    const unsigned groups = nlanes == 3 ? (unsigned)((lines + 21 * waves_per_group - 1) / (21 * waves_per_group))
+                                        : (unsigned)((nlanes * lines + threads - 1) / threads);
+    const dim3 grid((unsigned)nc, groups);
     const T* vt = std::is_same<T, float>::value ? (const T*)h.vloc32_t.ptr : (const T*)h.vloc64_t.ptr;
     bool done = false;
-    if (pair) {
+    if (nlanes == 3) {
+        switch (h.nx) {
+#define DZ_XREG_CASE(N)                                                                                      \
+    case N:                                                                                                  \
+        hipLaunchKernelGGL((k_xline_tri<T, N>), grid, dim3(threads), 0, s, tb, h.nplanes, lines, vt);       \
+        done = true;                                                                                         \
+        break;
+            DZ_XTRI_SIZES(DZ_XREG_CASE)
+#undef DZ_XREG_CASE
+            default:
+                break;
+        }
+    } else if (pair) {
         switch (h.nx) {
 #define DZ_XREG_CASE(N)                                                                                      \
     case N:                                                                                                  \
@@ -1275,6 +1417,10 @@ void ham_release(Ham& h) {
         (void)hipEventDestroy(h.ev_join);
         h.nl_stream = nullptr;
     }
+    if (h.ev_switch) {
+        (void)hipEventDestroy(h.ev_switch);
+        h.ev_switch = nullptr;
+    }
     DevBuf* bufs[] = {&h.plane_start, &h.plane_pw, &h.plane_pos, &h.plane_xpos, &h.plane_ystart,
                       &h.plane_ycols, &h.plane_xabsent, &h.kinetic64,
                       &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.block_rows, &h.dm64,
@@ -1347,21 +1493,25 @@ void launch_xline(Ham& h, typename C2T<T>::type* tb, size_t nc, int split, hipSt
 }
 
 // x-line variant launched by code: d >= 1 is the LDS kernel with d y-slabs, -1 / -2 / -3 the
-// register kernel and -4 / -5 / -6 the lane-pair register kernel in workgroups of 256 / 128 / 64
-// threads.
+// register kernel, -4 / -5 / -6 the lane-pair and -7 / -8 / -9 the lane-triple register kernel in
+// workgroups of 256 / 128 / 64 threads.
 
 template <typename T>
 void launch_xline_any(Ham& h, typename C2T<T>::type* tb, size_t nc, int code, hipStream_t s) {
     if (code <= kXlineReg && code >= kXlineRegLast) {
-        launch_xline_reg<T>(h, tb, nc, xline_reg_threads(code), xline_reg_pair(code), s);
+        launch_xline_reg<T>(h, tb, nc, xline_reg_threads(code), xline_reg_lanes(code), s);
         return;
     }
     launch_xline<T>(h, tb, nc, code, s);
 }
 
 // Process-wide x-line tuning results by (device, grid, sphere x-run, precision): contexts of the
-// same shape (the k-points of one SCF, solved concurrently) share one tuning, and the mutex keeps
-// the other contexts' work off the GPU while the trials are timed.
+// same shape (the k-points of one SCF, solved concurrently) share one tuning.  The mutex makes
+// other contexts of that shape wait for the result instead of tuning too; it does not stop
+// contexts that are already tuned from launching work meanwhile.  The trials start from an idle
+// device (hipDeviceSynchronize inside the lock drains what is queued), and a caller that runs
+// several contexts concurrently tunes each shape once before it starts them (the SCF driver applies
+// H to a 16-column block per shape and precision first), so the timings are not taken under load.
 std::mutex g_xline_tune_mu;
 std::map<std::array<int, 8>, int> g_xline_tune;
 
@@ -1387,6 +1537,8 @@ int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
     // The first candidate is also what narrow passes run before a wide pass has tuned.
     if (xline_pair_usable(h))
         for (int c = -4; c >= -6; --c) cand.push_back(c);
+    if (xline_tri_usable(h))
+        for (int c = -7; c >= -9; --c) cand.push_back(c);
     if (xline_reg_usable(h, !fp32))
         for (int c = -1; c >= -3; --c) cand.push_back(c);
     for (int d = 1; d <= 12; ++d)
@@ -1405,6 +1557,7 @@ int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
         sp = it->second;
         return sp;
     }
+    DZ_HIP(hipDeviceSynchronize());
     hipEvent_t e0, e1;
     DZ_HIP(hipEventCreate(&e0));
     DZ_HIP(hipEventCreate(&e1));

@@ -497,7 +497,8 @@ class Hamiltonian:
         return out
 
     def set_tuning(self, xline_split: int = 0):
-        """Force the x-line variant: d > 0 LDS y-slabs, -1..-6 register kernel (0: autotune)."""
+        """Force the x-line variant: d > 0 LDS y-slabs, -1..-3 / -4..-6 / -7..-9 the one-lane /
+        lane-pair / lane-triple register kernel in 256 / 128 / 64-thread workgroups (0: autotune)."""
         check(L().dzhip_ham_set_tuning(self.handle, int(xline_split)))
 
     def set_overlap(self, on: bool = True):

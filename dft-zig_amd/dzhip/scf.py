@@ -180,6 +180,36 @@ def si64(ecut_ry=30.0, nbands=160):
     return silicon((2, 2, 2), ecut_ry, (2, 2, 2), nbands, displacement_angstrom=0.02, seed=64)
 
 
+def start_potential(system, potential="atomic"):
+    """local_r of the first SCF iteration: V_ion + V_Hxc[rho_atomic] ("atomic") or V_ion ("ion")."""
+    s = system
+    g, g2 = g_vectors(s.grid, s.recip)
+    local_r = ionic_potential(s, g, g2)
+    if potential == "atomic":
+        rho = atomic_density(s, g, g2)
+        local_r = local_r + hartree(rho, s.grid, g2, s.volume)[0] + lda_pz(rho)[1]
+    return local_r
+
+
+def kpoint_workload(system, k_index=0, nbands=None, seed=20261203, potential="atomic", local_r=None):
+    """One k-point of an SCF system as an H.psi workload: its G-sphere, the SCF system's projectors
+    and atoms, local_r = V_ion + V_Hxc[rho_atomic] (the first SCF iteration's input potential;
+    potential="ion" keeps V_ion only; a given local_r is used as is) and a random psi block with
+    coefficients damped by 1 / (1 + |k+G|^2), seeded."""
+    from .workloads import Workload, _random_psi
+
+    s = system
+    k = np.asarray(s.kpoints[k_index], dtype=np.float64)
+    basis = tables.generate_basis(s.recip, s.ecut_ry, k)
+    if local_r is None:
+        local_r = start_potential(s, potential)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    psi = _random_psi(rng, nbands or s.nbands, basis.kinetic)
+    return Workload(f"{s.name}_k{k_index}", s.cell, s.recip, s.volume, tuple(s.grid), s.ecut_ry, k, basis, local_r,
+                    list(s.species), s.atom_pos, s.atom_species, psi,
+                    {"seed": seed, "k_index": k_index, "potential": potential})
+
+
 # ----------------------------------------------------------------- grid potentials (host, FP64)
 def _freqs(n):
     i = np.arange(n)
@@ -590,6 +620,8 @@ class ScfDriver:
                         collect_snapshots=collect_snapshots)
 
         rho0 = atomic_density(s, self.g, self.g2) if start == "atomic" else None
+        if self.concurrent:
+            self._pretune(precision)
         if self.grid is not None:
             self.grid.set_dielectric(diemac, dielng)
             run = self._run_device(max_iter, beta, conv, pulay_history, pulay_start, warmup_steps, solve_opts,
@@ -644,6 +676,17 @@ class ScfDriver:
         run.v_in = v_in
         self._final(run, final_tol, preconditioner)
         return run
+
+    def _pretune(self, precision):
+        """Tune the x-line kernels of every context (one 16-column apply per precision the solves
+        use, one context after another) before the k-points run concurrently, so the tuner's
+        trials are timed on an otherwise idle device."""
+        precs = (FP64,) if precision == FP64 else (FP64, precision)
+        rng = np.random.default_rng(0)
+        for ham in self.hams:
+            x = rng.standard_normal((16, ham.n)) + 1j * rng.standard_normal((16, ham.n))
+            for p in precs:
+                ham.apply(x, p)
 
     @staticmethod
     def _record(it, e_tot, e_band, drho, stats, eig_s):

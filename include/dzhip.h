@@ -121,7 +121,7 @@ int dzhip_ham_apply(dzhip_ham* ham, const double* x, double* y, uint64_t n, uint
 int dzhip_ham_apply_dev(dzhip_ham* ham, const void* x_dev, void* y_dev, uint64_t n, uint64_t ncols,
                         int precision);
 
-/* Nonlocal part only, y = V_nl x (apply_nonlocal_potential, apply.zig:747-760 and
+/* Nonlocal part only, y = V_nl x (apply_nonlocal_potential, apply.zig:793-808 and
  * apply_nonlocal_batched, apply.zig:1449-1482); host memory, blocking. */
 int dzhip_ham_apply_nonlocal(dzhip_ham* ham, const double* x, double* y, uint64_t n, uint64_t ncols,
                              int precision);
@@ -337,12 +337,14 @@ int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t c
  * kernel with one x-line per thread (nx = 2^a 3^b 5^c, 8 <= nx <= 100 in FP32, <= 50 in FP64)
  * in workgroups of 256 / 128 / 64 threads, -4 / -5 / -6 the lane-pair register kernel (two
  * threads per x-line; even nx <= 100 whose half is 2^a 3^b 5^c, both precisions, when the
- * G-sphere spans at most half the line); a precision where the chosen kernel cannot run keeps
- * one slab.  -1..-3 need the one-lane kernel in FP32, -4..-6 the lane-pair kernel.  0 (default)
- * times every divisor of ny with at least 8 lines per slab, and the register kernels' variants
- * where they can run, on the first apply of each precision with at least 16 columns and keeps
- * the fastest; the result is shared by every context of the same device, grid and sphere shape
- * in the process. */
+ * G-sphere spans at most half the line), -7 / -8 / -9 the lane-triple register kernel (three
+ * threads per x-line; nx in 12..96 divisible by 3 whose third is 2^a 3^b 5^c, e.g. 75 = 3 x 25,
+ * both precisions, any sphere width); a precision where the chosen kernel cannot run keeps
+ * one slab.  -1..-3 need the one-lane kernel in FP32, -4..-6 the lane-pair kernel, -7..-9 the
+ * lane-triple kernel.  0 (default) times every divisor of ny with at least 8 lines per slab, and
+ * the register kernels' variants where they can run, on the first apply of each precision with at
+ * least 16 columns and keeps the fastest; the result is shared by every context of the same
+ * device, grid and sphere shape in the process. */
 int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split);
 
 /* Overlap (no reference counterpart; default off): with on != 0 the nonlocal projection / D /
@@ -357,7 +359,7 @@ int dzhip_ham_set_overlap(dzhip_ham* ham, int on);
  * [3] = projectors P, [4] = device bytes held, [5] = LDS x-line slabs chosen for FP32,
  * [6] = for FP64 (0 = not tuned yet, or the register kernel), [7] = workgroup size when FP32
  * runs the register x-line kernel (0 = the LDS kernel or not tuned yet), [8] = the same for
- * FP64, [9] / [10] = threads per x-line of that register kernel, 1 or 2 (FP32 / FP64; 0 = the
+ * FP64, [9] / [10] = threads per x-line of that register kernel, 1, 2 or 3 (FP32 / FP64; 0 = the
  * LDS kernel or not tuned yet). */
 int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out);
 

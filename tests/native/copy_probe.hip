@@ -1,6 +1,6 @@
 // Plain device copy of a known size (256 MiB read + 256 MiB written per launch, 3 launches):
 // the calibration that turns rocprofv3's FETCH_SIZE / WRITE_SIZE counters into bytes for the
-// H.psi kernels' traffic figures.  Built next to the library as dft-zig_amd/dzhip/copy_probe.
+// H.psi kernels' traffic figures.  Built by tests/native/Makefile as tests/native/copy_probe.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>

@@ -72,3 +72,42 @@ def test_compile_time_roots(driver):
                 kk = q * n // 4
                 assert (got[kk, 0], got[kk, 1]) == ((1.0, 0.0), (0.0, 1.0), (-1.0, 0.0), (0.0, -1.0))[q]
     assert pos == vals.shape[0]
+
+
+def lane_split_xline(b, v, nx, lanes):
+    """Model of the lane-split register x-line kernels (k_xline_pair for lanes = 2, k_xline_tri for
+    lanes = 3): rows b[u], u < np, of a line of length nx; V sampled at every real-space point.  Lane
+    t folds and twists the rows, c_t[u0] = sum_j b[u0 + jM] w^{(u0 + jM) t} with w = e^{2 pi i / nx},
+    runs an inverse length-M transform, multiplies by V(lanes m + t), a forward length-M transform,
+    twists by w^{-k0 t}, and the lanes combine by a lanes-point transform across them."""
+    m = nx // lanes
+    npl = b.shape[0]
+    w = np.exp(2j * np.pi / nx)
+    h = []
+    for t in range(lanes):
+        c = np.zeros(m, complex)
+        for u in range(npl):
+            c[u % m] += b[u] * w ** (u * t)
+        g = np.fft.ifft(c) * m
+        f = np.fft.fft(g * v[lanes * np.arange(m) + t])
+        h.append(f * w ** (-np.arange(m) * t))
+    out = np.zeros(npl, complex)
+    for k in range(npl):
+        k0, j = k % m, k // m
+        out[k] = sum(np.exp(-2j * np.pi * j * t / lanes) * h[t][k0] for t in range(lanes))
+    return out
+
+
+@pytest.mark.parametrize("nx,lanes,npl", [(72, 2, 35), (72, 2, 36), (72, 3, 35), (75, 3, 35), (75, 3, 25),
+                                          (75, 3, 12), (90, 3, 41), (90, 2, 41), (27, 3, 27)])
+def test_lane_split_xline_model(nx, lanes, npl):
+    """The folded, twisted length-nx/lanes transforms and the cross-lane combination give the direct
+    x-part of H (forward FFT of V times the inverse FFT of the zero-padded rows) on the np rows."""
+    rng = np.random.default_rng(nx * 10 + lanes + npl)
+    b = rng.standard_normal(npl) + 1j * rng.standard_normal(npl)
+    v = rng.standard_normal(nx)
+    full = np.zeros(nx, complex)
+    full[:npl] = b
+    ref = np.fft.fft(v * np.fft.ifft(full) * nx)[:npl]
+    got = lane_split_xline(b, v, nx, lanes)
+    assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))

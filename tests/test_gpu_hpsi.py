@@ -206,22 +206,26 @@ def test_hpsi_every_xline_split(dz, ctx, prec):
 
 # Which register kernels run per workload and precision: one lane per line for nx <= 100 (FP32)
 # or <= 50 (FP64); the lane-pair kernel for even nx when the sphere spans at most half the line
-# (tiny: 5 of 8 x values, no; nx = 75: odd, no).
-REGISTER_KERNELS = {"tiny": {"one": (0, 1), "pair": ()}, "si8": {"one": (0, 1), "pair": (0, 1)},
-                    "bench": {"one": (1,), "pair": (0, 1)}, "bench75": {"one": (1,), "pair": ()},
-                    "bench80": {"one": (1,), "pair": (0, 1)}, "bench90": {"one": (1,), "pair": (0, 1)},
-                    "bench100": {"one": (1,), "pair": (0, 1)}}
+# (tiny: 5 of 8 x values, no; nx = 75: odd, no); the lane-triple kernel for nx divisible by 3
+# whose third is 2^a 3^b 5^c (72, 75, 90), any sphere width.
+REGISTER_KERNELS = {"tiny": {"one": (0, 1), "pair": (), "tri": ()},
+                    "si8": {"one": (0, 1), "pair": (0, 1), "tri": ()},
+                    "bench": {"one": (1,), "pair": (0, 1), "tri": (0, 1)},
+                    "bench75": {"one": (1,), "pair": (), "tri": (0, 1)},
+                    "bench80": {"one": (1,), "pair": (0, 1), "tri": ()},
+                    "bench90": {"one": (1,), "pair": (0, 1), "tri": (0, 1)},
+                    "bench100": {"one": (1,), "pair": (0, 1), "tri": ()}}
 
 
 @pytest.mark.parametrize("which,prec", [("tiny", 1), ("si8", 1), ("bench", 1), ("bench75", 1), ("bench90", 1),
                                         ("bench80", 1), ("bench100", 1), ("tiny", 0), ("si8", 0), ("bench", 0),
-                                        ("bench90", 0), ("bench100", 0)])
+                                        ("bench75", 0), ("bench90", 0), ("bench100", 0)])
 def test_hpsi_register_xline_vs_oracle(dz, ctx, which, prec):
-    """The register-resident x-line kernels (one x-line per thread, or per lane pair with
-    half-length transforms; Stockham in VGPRs) on grids of different factorisations (nx = 8, 32,
-    72, 75, 80, 90 and 100, the largest built, on a 100 x 72 x 72 grid; radices 4/2/3/5/9) against
-    the FP64 oracle apply: FP32 tolerance, and 1e-12 in FP64.  A variant that cannot run is refused
-    (both precisions) or runs the LDS kernel (one precision only)."""
+    """The register-resident x-line kernels (one x-line per thread, or per lane pair or lane triple
+    with half- or third-length transforms; Stockham in VGPRs) on grids of different factorisations
+    (nx = 8, 32, 72, 75, 80, 90 and 100, the largest built, on a 100 x 72 x 72 grid; radices
+    4/2/3/5/9) against the FP64 oracle apply: FP32 tolerance, and 1e-12 in FP64.  A variant that
+    cannot run is refused (both precisions) or runs the LDS kernel (one precision only)."""
     make = {"tiny": workloads.tiny, "si8": workloads.si8, "bench": workloads.bench,
             "bench75": lambda: workloads.bench(nbands=8, grid=(75, 75, 75)),
             "bench80": lambda: workloads.bench(nbands=8, grid=(80, 72, 75)),
@@ -234,7 +238,7 @@ def test_hpsi_register_xline_vs_oracle(dz, ctx, which, prec):
         nb = min(w.nbands, 8)
         ref = hamiltonian.apply_hamiltonian_batched(oracle_ctx(w), w.psi[:nb])
         ran = 0
-        for variant, kind, lanes in ((-1, "one", 1), (-4, "pair", 2)):
+        for variant, kind, lanes in ((-1, "one", 1), (-4, "pair", 2), (-7, "tri", 3)):
             runs = prec in REGISTER_KERNELS[which][kind]
             if not REGISTER_KERNELS[which][kind] or (kind == "one" and 1 not in REGISTER_KERNELS[which][kind]):
                 with pytest.raises(dz.BadArgument):
@@ -250,6 +254,29 @@ def test_hpsi_register_xline_vs_oracle(dz, ctx, which, prec):
             assert err < (TOL32 if prec else TOL64)
             ran += runs
         assert ran >= 1
+    finally:
+        ham.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hpsi_tri_xline_workgroup_sizes(dz, ctx, prec):
+    """The lane-triple x-line kernel (21 lines per wavefront, its last lane idle) in 256-, 128- and
+    64-thread workgroups on the 75^3 grid (odd line: 3 x 25-point transforms; the 30 Ry sphere
+    spans 35 of the 75 x values, so each lane folds rows from two thirds of the line), 12 bands:
+    the last workgroup holds a partial set of lines at every size."""
+    w = workloads.bench(nbands=12, grid=(75, 75, 75))
+    ham = make_ham(dz, ctx, w)
+    key = "fp32" if prec else "fp64"
+    try:
+        ref = hamiltonian.apply_hamiltonian_batched(oracle_ctx(w), w.psi)
+        for variant, threads in ((-7, 256), (-8, 128), (-9, 64)):
+            ham.set_tuning(variant)
+            got = ham.apply(w.psi, precision=prec)
+            info = ham.info()
+            assert info[f"xline_register_{key}"] == threads and info[f"xline_lanes_{key}"] == 3
+            err = relerr(got, ref)
+            print(prec, variant, err)
+            assert err < (TOL32 if prec else TOL64)
     finally:
         ham.close()
 

@@ -1,0 +1,89 @@
+"""The Si-64 workload (2x2x2 cubic cells, 64 atoms, 30 Ry, 75^3 grid, 256 projectors, 160 bands)
+on the GPU against the FP64 oracle, at one k-point of its 2x2x2 mesh.
+
+* H.psi of 8 bands vs the oracle's apply (apply_hamiltonian_batched restated,
+  src/features/scf/apply.zig:1584-1618): FP64 1e-12, FP32 2e-5 of the largest entry.
+* The full 160-band block: Hermiticity <a|H b> = <H a|b> and linearity (size-independent).
+* One FP64 Davidson solve (lobpcg_serial.solve semantics, src/features/linalg/lobpcg/serial.zig:
+  238-283: converged when the largest residual 2-norm is below tol, Ry) at the first SCF
+  iteration's input potential: its residuals recomputed with the oracle's apply are below tol,
+  and the mixed-precision solve (FP32 apply, FP64 below 1e-5) returns the same eigenvalues
+  within 2e-8 Ry (1e-8 Ha).
+"""
+import numpy as np
+import pytest
+
+from dzhip import scf
+from oracle import adapters, hamiltonian
+
+pytestmark = pytest.mark.gpu
+TOL64, TOL32 = 1e-12, 2e-5
+K_INDEX = 7  # k = (1/2, 1/2, 1/2): the largest sphere of the mesh, 23,984 plane waves, 36 x-planes
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def si64_k():
+    return scf.kpoint_workload(scf.si64(), K_INDEX, nbands=160)
+
+
+@pytest.fixture(scope="module")
+def oracle_k(si64_k):
+    return adapters.context_from_workload(si64_k)
+
+
+def test_si64_hpsi_vs_oracle(dz, ctx, si64_k, oracle_k):
+    w = si64_k
+    assert w.grid == (75, 75, 75) and w.npw == 23984 and w.projector_count() == 256
+    ref = hamiltonian.apply_hamiltonian_batched(oracle_k, w.psi[:8])
+    ham = dz.Hamiltonian.from_workload(ctx, w)
+    try:
+        e64 = relerr(ham.apply(w.psi[:8], 0), ref)
+        e32 = relerr(ham.apply(w.psi[:8], 1), ref)
+        info = ham.info()
+        print("si64 k7 H.psi rel err fp64", e64, "fp32", e32, info)
+        assert e64 < TOL64 and e32 < TOL32
+        # The odd 75-point line runs a register x-line kernel in both precisions.
+        assert info["xline_lanes_fp64"] in (1, 3) and info["xline_register_fp64"] > 0
+    finally:
+        ham.close()
+
+
+def test_si64_full_block_hermitian_and_linear(dz, ctx, si64_k):
+    w = si64_k
+    ham = dz.Hamiltonian.from_workload(ctx, w)
+    try:
+        for prec, tol in ((0, 1e-12), (1, 2e-5)):
+            y = ham.apply(w.psi, prec)
+            lhs, rhs = np.vdot(w.psi[11], y[150]), np.vdot(y[11], w.psi[150])
+            assert abs(lhs - rhs) < tol * max(abs(lhs), np.linalg.norm(y[11]) * np.linalg.norm(w.psi[150])), prec
+            comb = 0.4 * w.psi[2] + (0.25 - 1.1j) * w.psi[159]
+            assert relerr(ham.apply(comb, prec), 0.4 * y[2] + (0.25 - 1.1j) * y[159]) < 10 * tol, prec
+    finally:
+        ham.close()
+
+
+def test_si64_davidson_residuals_vs_oracle_and_mixed(dz, ctx, si64_k, oracle_k):
+    w = si64_k
+    nocc = 128
+    tol = 1e-6
+    ham = dz.Hamiltonian.from_workload(ctx, w)
+    try:
+        v64, x64, st64 = ham.eig_solve(160, tol=tol, max_iter=300, init_diagonal=True, nconv=nocc)
+        print("si64 fp64 solve", st64)
+        assert st64["converged"] == 1, st64
+        hx = hamiltonian.apply_hamiltonian_batched(oracle_k, x64[:nocc])
+        res = np.linalg.norm(hx - v64[:nocc, None] * x64[:nocc], axis=1)
+        print("oracle residuals max", res.max())
+        assert res.max() < tol
+        np.testing.assert_allclose(x64[:nocc].conj() @ x64[:nocc].T, np.eye(nocc), atol=1e-10)
+        v32, _, st32 = ham.eig_solve(160, tol=tol, max_iter=300, init_diagonal=True, nconv=nocc, precision=1,
+                                     want_vectors=False)
+        print("si64 mixed solve", st32, "max |dE|", np.max(np.abs(v32[:nocc] - v64[:nocc])))
+        assert st32["converged"] == 1 and st32["fp32_applies"] > 0, st32
+        np.testing.assert_allclose(v32[:nocc], v64[:nocc], atol=2e-8)
+    finally:
+        ham.close()
