@@ -46,8 +46,6 @@ def test_si64_hpsi_vs_oracle(dz, ctx, si64_k, oracle_k):
         info = ham.info()
         print("si64 k7 H.psi rel err fp64", e64, "fp32", e32, info)
         assert e64 < TOL64 and e32 < TOL32
-        # The odd 75-point line runs a register x-line kernel in both precisions.
-        assert info["xline_lanes_fp64"] in (1, 3) and info["xline_register_fp64"] > 0
     finally:
         ham.close()
 
@@ -58,6 +56,12 @@ def test_si64_full_block_hermitian_and_linear(dz, ctx, si64_k):
     try:
         for prec, tol in ((0, 1e-12), (1, 2e-5)):
             y = ham.apply(w.psi, prec)
+            # A 160-column pass tunes the x-line kernel: the odd 75-point line runs a register
+            # kernel (one lane per line in FP32, three lanes per line in FP64).
+            info = ham.info()
+            key = "fp32" if prec else "fp64"
+            print("si64 tuned", prec, info)
+            assert info[f"xline_lanes_{key}"] in ((1, 3) if prec else (3,)) and info[f"xline_register_{key}"] > 0
             lhs, rhs = np.vdot(w.psi[11], y[150]), np.vdot(y[11], w.psi[150])
             assert abs(lhs - rhs) < tol * max(abs(lhs), np.linalg.norm(y[11]) * np.linalg.norm(w.psi[150])), prec
             comb = 0.4 * w.psi[2] + (0.25 - 1.1j) * w.psi[159]
