@@ -278,6 +278,64 @@ def hpsi_block(dzhip, torch, ham, w, steps, warmup):
             "err": {"max_band_rel_l2": err, "max_abs": abserr}}
 
 
+def batched_al4(dzhip, torch, ctx, steps, warmup):
+    """Uneven multi-k batching (fcc Al, 4-atom cubic cell, 30 Ry, 30^3 grid, 8x8x8 mesh reduced by
+    symmetry to 35 k-points of 1,208-1,260 plane waves, 12 bands): one apply of all k-points x bands
+    as a per-k loop of device applies, as the set's launch sequence, and as its captured graph."""
+    from dzhip import scf as dscf
+
+    al = dscf.aluminium(symmetry=True, kmesh=(8, 8, 8))
+    local_r = dscf.start_potential(al)
+    ws = [dscf.kpoint_workload(al, k, nbands=al.nbands, local_r=local_r) for k in range(len(al.kpoints))]
+    hams = [dzhip.Hamiltonian.from_workload(ctx, w) for w in ws]
+    hs = dzhip.HamSet(hams)
+    stream = torch.cuda.Stream()
+    ngrid = int(np.prod(al.grid))
+    out = {"workload": "fcc Al 4-atom cubic cell, 30 Ry, 30^3 grid, 35 IBZ k-points (8x8x8, 48 ops + time "
+                       "reversal), N_pw 1208-1260, 12 bands per k",
+           "k_points": len(ws), "n_pw_total": hs.total}
+    try:
+        for h in hams:
+            h.set_stream(stream.cuda_stream)
+        hs.set_stream(stream.cuda_stream)
+        for prec, tag, cdt, rdt in ((dzhip.FP64, "fp64", np.complex128, np.float64),
+                                    (dzhip.FP32, "fp32", np.complex64, np.float32)):
+            xs = [torch.from_numpy(w.psi.astype(cdt).view(rdt).copy()).cuda() for w in ws]
+            ys = [torch.empty_like(x) for x in xs]
+            xpk = torch.from_numpy(hs.pack([w.psi.astype(cdt) for w in ws]).view(rdt).copy()).cuda()
+            ypk = torch.empty_like(xpk)
+            torch.cuda.synchronize()
+            runs = {"per_k_loop": lambda: [h.apply_dev(x.data_ptr(), y.data_ptr(), al.nbands, prec)
+                                           for h, x, y in zip(hams, xs, ys)],
+                    "set": lambda: hs.apply_dev(xpk.data_ptr(), ypk.data_ptr(), al.nbands, prec, False),
+                    "set_graph": lambda: hs.apply_dev(xpk.data_ptr(), ypk.data_ptr(), al.nbands, prec, True)}
+            nbytes = sum(hpsi_bytes(al.nbands, w.npw, ngrid, tag) for w in ws)
+            res = {}
+            for name, fn in runs.items():
+                for _ in range(max(warmup, 1)):
+                    fn()
+                stream.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(steps):
+                    fn()
+                e1.record(stream)
+                stream.synchronize()
+                ms = e0.elapsed_time(e1) / steps
+                res[name] = {"ms": round(ms, 4), "gbs": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+            res["speedup_graph_vs_loop"] = round(res["per_k_loop"]["ms"] / res["set_graph"]["ms"], 2)
+            out[tag] = res
+        out["graphs_captured"] = hs.info()["graphs"]
+    finally:
+        for h in hams:
+            h.set_stream(None)
+        hs.set_stream(None)
+        hs.close()
+        for h in hams:
+            h.close()
+    return out
+
+
 def roofline(w, info, phases, prec):
     """Roofline of the dominant kernel of an apply (per-phase HIP events on the launch stream)."""
     nb, npw = w.nbands, w.npw
@@ -302,7 +360,7 @@ def roofline(w, info, phases, prec):
 def tuning_info(ham):
     t = ham.info()
     return {k: t[k] for k in ("xline_split_fp32", "xline_split_fp64", "xline_register_fp32", "xline_register_fp64",
-                              "xline_lanes_fp32", "xline_lanes_fp64")}
+                              "xline_lanes_fp32", "xline_lanes_fp64", "xline_kind_fp32", "xline_kind_fp64")}
 
 
 SCF_ARMS = ("fp64", "fp64_concurrent", "mixed_batched", "mixed_batched_learned")
@@ -320,6 +378,8 @@ def main():
     ap.add_argument("--scf-arms", default=",".join(SCF_ARMS),
                     help="comma-separated SCF forms to run (fp64 gives E_inf)")
     ap.add_argument("--no-isolated", action="store_true", help="skip the isolated 72^3 H.psi bench")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 35-k Al-4 batched apply")
+    ap.add_argument("--no-gaas", action="store_true", help="skip the GaAs-64 H.psi block")
     ap.add_argument("--eig", action="store_true",
                     help="instead of the bench line: one cold and two warm FP64 Davidson solves of 128 bands "
                          "on the Si-64 k-point (12 iterations), with the solver's section times")
@@ -327,7 +387,8 @@ def main():
                     help="run the nonlocal kernels on a second stream beside the local ones (timed pass only)")
     ap.add_argument("--xline-split", type=int, default=0,
                     help="fix the x-line variant: d > 0 LDS y-slabs, -1..-3 one-lane, -4..-6 lane-pair, "
-                         "-7..-9 lane-triple register kernel (0 = autotune on the first apply)")
+                         "-7..-9 lane-triple, -10..-12 residue-sequential register kernel (0 = autotune on the "
+                         "first apply)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (k-point sharding is "
@@ -375,6 +436,25 @@ def main():
                     "tuning": tuning_info(hb)}
         hb.close()
 
+    batched = None if args.no_batched else batched_al4(dzhip, torch, ctx, args.steps, args.warmup)
+
+    gaas = None
+    if not args.no_gaas:
+        wg = dscf.kpoint_workload(dscf.gaas64(), 0, nbands=160)
+        hg = dzhip.Hamiltonian.from_workload(ctx, wg)
+        rg = hpsi_block(dzhip, torch, hg, wg, args.steps, args.warmup)
+        gaas = {"workload": "GaAs-64 zincblende 2x2x2 supercell (a = 11.306 A, N(0, 0.05 A) displacements), 35 Ry, "
+                            "90^3 grid, Gamma, N_pw = 34049, 160 bands, 256 projectors",
+                "fp64": {"value": round(rg["gbs64"], 2), "unit": "GB/s", "ms_per_step": round(rg["ms64"], 4),
+                         "frac_hbm": round(rg["gbs64"] / HBM_PEAK_GBS, 4)},
+                "fp32": {"value": round(rg["gbs32"], 2), "unit": "GB/s", "ms_per_step": round(rg["ms32"], 4),
+                         "frac_hbm": round(rg["gbs32"] / HBM_PEAK_GBS, 4)},
+                "fp32_vs_fp64_err": rg["err"],
+                "phases_ms_fp64": {k: round(v, 5) for k, v in rg["ph64"].items()},
+                "phases_ms_fp32": {k: round(v, 5) for k, v in rg["ph32"].items()},
+                "tuning": tuning_info(hg)}
+        hg.close()
+
     cpu = None if args.no_cpu else cpu_baseline(system)
     arms = [a for a in args.scf_arms.split(",") if a]
 
@@ -407,6 +487,8 @@ def main():
         "phases_ms_fp64": {k: round(v, 5) for k, v in r["ph64"].items()},
         "phases_ms_fp32": {k: round(v, 5) for k, v in r["ph32"].items()},
         "isolated_hpsi_bench": isolated,
+        "batched_multi_k_al4": batched,
+        "gaas64_hpsi": gaas,
         "scf_time_to_1e-6_Ha": None if args.scf == "none" else scf_leg(args.scf, args.scf_max_iter, arms),
     }
     print(json.dumps(out), flush=True)

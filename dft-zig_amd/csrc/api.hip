@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "ham.h"
+#include "heev.h"
 #include "potential.h"
 
 namespace dzhip {
@@ -49,6 +50,7 @@ using namespace dzhip;
 struct dzhip_ctx : Context {};
 struct dzhip_fft_plan : FftPlan {};
 struct dzhip_ham : Ham {};
+struct dzhip_hamset : HamSet {};
 struct dzhip_scf_grid : ScfGrid {};
 
 namespace dzhip {
@@ -102,9 +104,85 @@ void dzhip_ctx_destroy(dzhip_ctx* ctx) {
     if (!ctx) return;
     (void)guarded([&] {
         DeviceGuard g(ctx->device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->heev) {
+            ctx->heev->release();
+            delete ctx->heev;
+            ctx->heev = nullptr;
+        }
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     });
     delete ctx;
+}
+
+// ----------------------------------------------------------------- dense Hermitian eigensolver
+int dzhip_heev(dzhip_ctx* ctx, const double* a, uint64_t lda, uint64_t m, uint64_t nev, double* values,
+               double* vectors, uint64_t ldz) {
+    return guarded([&] {
+        DZ_REQUIRE(ctx, DZHIP_ERR_STATE, "NULL context");
+        DZ_REQUIRE(a && values && vectors, DZHIP_ERR_ARG, "NULL array");
+        DZ_REQUIRE(m >= 1 && m <= (uint64_t)kHeevMaxM && nev >= 1 && nev <= m && lda >= m && ldz >= m, DZHIP_ERR_ARG,
+                   "need 1 <= nev <= m <= 1024 and lda, ldz >= m");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        if (!ctx->heev) ctx->heev = new HeevWork();
+        hipStream_t s = ctx->stream;
+        double2 *da = nullptr, *dz = nullptr;
+        double* dw = nullptr;
+        auto cleanup = [&] {
+            if (da) (void)hipFree(da);
+            if (dz) (void)hipFree(dz);
+            if (dw) (void)hipFree(dw);
+        };
+        try {
+            DZ_HIP(hipMalloc(&da, (size_t)lda * m * sizeof(double2)));
+            DZ_HIP(hipMalloc(&dz, (size_t)m * nev * sizeof(double2)));
+            DZ_HIP(hipMalloc(&dw, (size_t)nev * sizeof(double)));
+            DZ_HIP(hipMemcpyAsync(da, a, (size_t)lda * m * sizeof(double2), hipMemcpyHostToDevice, s));
+            heev_lowest(*ctx->heev, da, (int)lda, (int)m, (int)nev, dw, dz, (int)m, s);
+            DZ_REQUIRE(heev_error_flag(*ctx->heev, s) == 0, DZHIP_ERR_HIP,
+                       "Hermitian eigensolver: tridiagonalisation barrier did not complete");
+            DZ_HIP(hipMemcpyAsync(values, dw, nev * sizeof(double), hipMemcpyDeviceToHost, s));
+            DZ_HIP(hipMemcpy2DAsync(vectors, ldz * sizeof(double2), dz, m * sizeof(double2), m * sizeof(double2), nev,
+                                    hipMemcpyDeviceToHost, s));
+            DZ_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+    });
+}
+
+int dzhip_heev_dev(dzhip_ctx* ctx, const void* a, uint64_t lda, uint64_t m, uint64_t nev, void* values,
+                   void* vectors, uint64_t ldz, void* stream) {
+    return guarded([&] {
+        DZ_REQUIRE(ctx, DZHIP_ERR_STATE, "NULL context");
+        DZ_REQUIRE(a && values && vectors, DZHIP_ERR_ARG, "NULL array");
+        DZ_REQUIRE(m >= 1 && m <= (uint64_t)kHeevMaxM && nev >= 1 && nev <= m && lda >= m && ldz >= m, DZHIP_ERR_ARG,
+                   "need 1 <= nev <= m <= 1024 and lda, ldz >= m");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        if (!ctx->heev) ctx->heev = new HeevWork();
+        hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->stream;
+        heev_lowest(*ctx->heev, static_cast<const double2*>(a), (int)lda, (int)m, (int)nev, static_cast<double*>(values),
+                    static_cast<double2*>(vectors), (int)ldz, s);
+    });
+}
+
+int dzhip_heev_check(dzhip_ctx* ctx, void* stream) {
+    return guarded([&] {
+        DZ_REQUIRE(ctx, DZHIP_ERR_STATE, "NULL context");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->stream;
+        if (!ctx->heev) {
+            DZ_HIP(hipStreamSynchronize(s));
+            return;
+        }
+        DZ_REQUIRE(heev_error_flag(*ctx->heev, s) == 0, DZHIP_ERR_HIP,
+                   "Hermitian eigensolver: tridiagonalisation barrier did not complete");
+    });
 }
 
 // ----------------------------------------------------------------- FFT
@@ -279,6 +357,136 @@ int dzhip_ham_apply_dev(dzhip_ham* ham, const void* x_dev, void* y_dev, uint64_t
             ham_apply_dev<float>(*ham, x_dev, y_dev, ncols, true, true);
         else
             throw Error{DZHIP_ERR_ARG, "bad precision"};
+    });
+}
+
+// ----------------------------------------------------------------- batched multi-k apply
+dzhip_hamset* dzhip_hamset_create(dzhip_ham* const* hams, uint64_t count) {
+    dzhip_hamset* out = nullptr;
+    int rc = guarded([&] {
+        DZ_REQUIRE(hams && count > 0, DZHIP_ERR_ARG, "no k-point contexts");
+        for (uint64_t k = 0; k < count; ++k) DZ_REQUIRE(hams[k], DZHIP_ERR_STATE, "NULL ham in the set");
+        auto* hs = new dzhip_hamset();
+        hs->ctx = hams[0]->ctx;
+        for (uint64_t k = 0; k < count; ++k) hs->ks.push_back(hams[k]);
+        try {
+            DeviceGuard g(hs->ctx->device);
+            // The members' tables are read while each member is locked in turn.
+            for (uint64_t k = 0; k < count; ++k) {
+                for (uint64_t j = 0; j < k; ++j)
+                    DZ_REQUIRE(hams[j] != hams[k], DZHIP_ERR_ARG, "a context appears twice in the set");
+            }
+            hamset_build(*hs);
+        } catch (...) {
+            hamset_release(*hs);
+            delete hs;
+            throw;
+        }
+        out = hs;
+    });
+    return rc == DZHIP_OK ? out : nullptr;
+}
+
+void dzhip_hamset_destroy(dzhip_hamset* set) {
+    if (!set) return;
+    (void)guarded([&] {
+        DeviceGuard g(set->ctx->device);
+        (void)hipStreamSynchronize(set->stream());
+        if (set->ev_switch) (void)hipEventDestroy(set->ev_switch);
+        hamset_release(*set);
+    });
+    delete set;
+}
+
+int dzhip_hamset_offsets(dzhip_hamset* set, uint64_t* off, uint64_t n_off) {
+    return guarded([&] {
+        DZ_REQUIRE(set, DZHIP_ERR_STATE, "NULL set");
+        DZ_REQUIRE(off, DZHIP_ERR_ARG, "NULL output");
+        for (uint64_t k = 0; k < n_off && k < set->npw_off.size(); ++k) off[k] = set->npw_off[k];
+    });
+}
+
+int dzhip_hamset_apply_dev(dzhip_hamset* set, const void* x_dev, void* y_dev, uint64_t ncols, int precision,
+                           int use_graph) {
+    return guarded([&] {
+        DZ_REQUIRE(set, DZHIP_ERR_STATE, "NULL set");
+        if (ncols == 0) return;
+        DZ_REQUIRE(x_dev && y_dev, DZHIP_ERR_ARG, "NULL x or y");
+        DZ_REQUIRE(x_dev != y_dev, DZHIP_ERR_ARG, "x and y must not alias");
+        DZ_REQUIRE(precision == DZHIP_FP64 || precision == DZHIP_FP32, DZHIP_ERR_ARG, "bad precision");
+        std::lock_guard<std::mutex> lk(set->mu);
+        std::lock_guard<std::mutex> l0(set->ks[0]->mu);
+        DeviceGuard g(set->ctx->device);
+        if (precision == DZHIP_FP64)
+            hamset_apply_dev<double>(*set, x_dev, y_dev, ncols, use_graph != 0);
+        else
+            hamset_apply_dev<float>(*set, x_dev, y_dev, ncols, use_graph != 0);
+    });
+}
+
+int dzhip_hamset_apply(dzhip_hamset* set, const double* const* x, double* const* y, uint64_t ncols, int precision) {
+    return guarded([&] {
+        DZ_REQUIRE(set, DZHIP_ERR_STATE, "NULL set");
+        if (ncols == 0) return;
+        DZ_REQUIRE(x && y, DZHIP_ERR_ARG, "NULL x or y");
+        DZ_REQUIRE(precision == DZHIP_FP64 || precision == DZHIP_FP32, DZHIP_ERR_ARG, "bad precision");
+        DZ_REQUIRE(ncols <= 256, DZHIP_ERR_ARG, "batched apply: at most 256 columns");
+        const size_t nk = set->ks.size();
+        for (size_t k = 0; k < nk; ++k) DZ_REQUIRE(x[k] && y[k], DZHIP_ERR_ARG, "NULL x or y block");
+        std::lock_guard<std::mutex> lk(set->mu);
+        std::lock_guard<std::mutex> l0(set->ks[0]->mu);
+        DeviceGuard g(set->ctx->device);
+        hipStream_t s = set->stream();
+        const size_t tot = set->npw_total;
+        const size_t esz = precision == DZHIP_FP64 ? sizeof(double2) : sizeof(float2);
+        DevBuf dx, dy, d64;
+        dx.ensure(tot * ncols * esz);
+        dy.ensure(tot * ncols * esz);
+        if (precision == DZHIP_FP32) d64.ensure(tot * ncols * sizeof(double2));
+        double2* stage = precision == DZHIP_FP64 ? dx.as<double2>() : d64.as<double2>();
+        for (size_t k = 0; k < nk; ++k) {
+            const size_t n = set->ks[k]->npw;
+            DZ_HIP(hipMemcpy2DAsync(stage + set->npw_off[k], tot * sizeof(double2), x[k], n * sizeof(double2),
+                                    n * sizeof(double2), ncols, hipMemcpyHostToDevice, s));
+        }
+        if (precision == DZHIP_FP64) {
+            hamset_apply_dev<double>(*set, dx.ptr, dy.ptr, ncols, false);
+        } else {
+            convert_to_float(d64.as<double2>(), dx.as<float2>(), tot * ncols, s);
+            hamset_apply_dev<float>(*set, dx.ptr, dy.ptr, ncols, false);
+            convert_to_double(dy.as<float2>(), d64.as<double2>(), tot * ncols, s);
+        }
+        const double2* res = precision == DZHIP_FP64 ? dy.as<double2>() : d64.as<double2>();
+        for (size_t k = 0; k < nk; ++k) {
+            const size_t n = set->ks[k]->npw;
+            DZ_HIP(hipMemcpy2DAsync(y[k], n * sizeof(double2), res + set->npw_off[k], tot * sizeof(double2),
+                                    n * sizeof(double2), ncols, hipMemcpyDeviceToHost, s));
+        }
+        DZ_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int dzhip_hamset_set_stream(dzhip_hamset* set, void* stream) {
+    return guarded([&] {
+        DZ_REQUIRE(set, DZHIP_ERR_STATE, "NULL set");
+        std::lock_guard<std::mutex> lk(set->mu);
+        DeviceGuard g(set->ctx->device);
+        const hipStream_t prev = set->stream();
+        set->user_stream = reinterpret_cast<hipStream_t>(stream);
+        const hipStream_t next = set->stream();
+        if (next == prev) return;
+        // The set's workspaces are shared by every apply: the new stream waits for the old one.
+        if (!set->ev_switch) DZ_HIP(hipEventCreateWithFlags(&set->ev_switch, hipEventDisableTiming));
+        DZ_HIP(hipEventRecord(set->ev_switch, prev));
+        DZ_HIP(hipStreamWaitEvent(next, set->ev_switch, 0));
+    });
+}
+
+int dzhip_hamset_info(dzhip_hamset* set, uint64_t* out, uint64_t n_out) {
+    return guarded([&] {
+        DZ_REQUIRE(set, DZHIP_ERR_STATE, "NULL set");
+        const uint64_t v[4] = {set->ks.size(), set->npw_total, (uint64_t)set->np_total, set->graph_builds};
+        for (uint64_t i = 0; i < n_out && i < 4; ++i) out[i] = v[i];
     });
 }
 
@@ -669,8 +877,8 @@ int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split) {
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
         DZ_REQUIRE(xline_split == 0 || xline_code_usable(*ham, xline_split, false), DZHIP_ERR_ARG,
-                   "xline_split must divide ny, or be -1..-3 (-4..-6, -7..-9) with a (lane-pair, lane-triple) "
-                   "register kernel for this context (0 = autotune)");
+                   "xline_split must divide ny, or be -1..-3 (-4..-6, -7..-9, -10..-12) with a (lane-pair, "
+                   "lane-triple, residue-sequential) register kernel for this context (0 = autotune)");
         std::lock_guard<std::mutex> lk(ham->mu);
         ham->tune_fixed = xline_split;
         ham->xline_split[0] = ham->xline_split[1] = 0;
@@ -691,11 +899,12 @@ int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out) {
         auto slabs = [](int code) { return code > 0 ? (uint64_t)code : 0; };
         auto reg = [](int code) { return code < 0 ? (uint64_t)xline_reg_threads(code) : 0u; };
         auto waves = [](int code) { return code < 0 ? (uint64_t)xline_reg_lanes(code) : 0u; };
-        const uint64_t v[11] = {ham->npw, ham->ngrid, (uint64_t)ham->nplanes, (uint64_t)ham->nproj,
+        auto kind = [](int code) { return code < 0 ? (uint64_t)(1 + xline_reg_kind(code)) : 0u; };
+        const uint64_t v[13] = {ham->npw, ham->ngrid, (uint64_t)ham->nplanes, (uint64_t)ham->nproj,
                                 ham->device_bytes(), slabs(ham->xline_split[1]), slabs(ham->xline_split[0]),
                                 reg(ham->xline_split[1]), reg(ham->xline_split[0]), waves(ham->xline_split[1]),
-                                waves(ham->xline_split[0])};
-        for (uint64_t i = 0; i < n_out && i < 11; ++i) out[i] = v[i];
+                                waves(ham->xline_split[0]), kind(ham->xline_split[1]), kind(ham->xline_split[0])};
+        for (uint64_t i = 0; i < n_out && i < 13; ++i) out[i] = v[i];
     });
 }
 

@@ -58,6 +58,23 @@ __device__ __forceinline__ C cscale(C a, T s) { return C{a.x * s, a.y * s}; }
 template <typename C>
 __device__ __forceinline__ C cmul_i(C a) { return C{-a.y, a.x}; }
 
+// ----------------------------------------------------------------- diagonal preconditioners
+// Kinetic shift 1 / (|k+G|^2 - lambda) with |denominator| clamped at 1e-8, sign kept
+// (src/features/linalg/lobpcg/common.zig:214-226).
+__device__ __forceinline__ double precond_kinetic_shift(double diag, double lam) {
+    double den = diag - lam;
+    if (fabs(den) < 1e-8) den = den >= 0.0 ? 1e-8 : -1e-8;
+    return 1.0 / den;
+}
+// Teter-Payne-Allan K(x) = (27 + 18x + 12x^2 + 8x^3) / (27 + 18x + 12x^2 + 8x^3 + 16x^4),
+// x = |k+G|^2 / <T>_b (PRB 40, 12255); <T>_b is floored at 1e-30 so that a band of zero kinetic
+// energy gives K(0) = 1 at G = 0 and K -> 0 elsewhere instead of 0 / 0.
+__device__ __forceinline__ double precond_tpa(double diag, double ekin) {
+    const double x = diag / fmax(ekin, 1e-30);
+    const double p = 27.0 + x * (18.0 + x * (12.0 + x * 8.0));
+    return p / (p + 16.0 * x * x * x * x);
+}
+
 // ----------------------------------------------------------------- FFT axis
 // One transform length n = prod radix[s] (radices 2/3/4/5/8/9), with device tables:
 //   twiddle[t] = exp(-2 pi i t / n), t in [0, n)   (inverse uses the conjugate)
@@ -96,10 +113,13 @@ struct AxisTables {
 };
 
 // ----------------------------------------------------------------- context
+struct HeevWork;
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
+    HeevWork* heev = nullptr;  // dense eigensolver workspace (dzhip_heev*), created on first use
 };
 
 __host__ __device__ inline size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }

@@ -10,7 +10,9 @@
 //   * new vectors are orthogonalised against the basis with two passes of block classical
 //     Gram-Schmidt (GEMMs) and orthonormalised with Cholesky QR (in place of one-vector MGS);
 //   * the projected matrix T = V^H H V is extended by the new columns only;
-//   * Rayleigh-Ritz diagonalises T with rocSOLVER zheevd on the device;
+//   * Rayleigh-Ritz takes the lowest nbands eigenpairs of T with the library's own device
+//     Hermitian eigensolver (heev.hip: one-launch tridiagonalisation, multisection, inverse
+//     iteration), rocSOLVER zheevd only for subspaces above 1024;
 //   * a full subspace restarts from [Ritz vectors, new block], reusing H x = W c (no H applies
 //     are discarded).
 // All subspace algebra is FP64.  In mixed mode the H apply runs in FP32 until the largest
@@ -26,6 +28,7 @@
 #include <vector>
 
 #include "ham.h"
+#include "heev.h"
 #include "herm3.h"
 
 namespace dzhip {
@@ -76,16 +79,7 @@ __global__ void k_precondition(const double2* __restrict__ r, const int* __restr
     if (i >= n) return;
     const int b = sel[j];
     const double2 v = r[(size_t)b * ld + i];
-    double f;
-    if (mode == 0) {
-        double den = diag[i] - lam[b];
-        if (fabs(den) < 1e-8) den = den >= 0.0 ? 1e-8 : -1e-8;
-        f = 1.0 / den;
-    } else {
-        const double x = diag[i] / ekin[b];
-        const double p = 27.0 + x * (18.0 + x * (12.0 + x * 8.0));
-        f = p / (p + 16.0 * x * x * x * x);
-    }
+    const double f = mode == 0 ? precond_kinetic_shift(diag[i], lam[b]) : precond_tpa(diag[i], ekin[b]);
     q[(size_t)j * ld + i] = make_double2(v.x * f, v.y * f);
 }
 
@@ -205,9 +199,7 @@ __global__ void k_precondition_one(const double2* __restrict__ r, const double* 
                                    double2* __restrict__ p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double den = diag[i] - lam;
-    if (fabs(den) < 1e-8) den = den >= 0.0 ? 1e-8 : -1e-8;
-    const double f = 1.0 / den;
+    const double f = precond_kinetic_shift(diag[i], lam);
     p[i] = make_double2(r[i].x * f, r[i].y * f);
 }
 
@@ -280,13 +272,6 @@ __global__ void k_mirror_new(double2* __restrict__ t, int ld, int m0, int a) {
     }
 }
 
-__global__ void k_copy_block(const double2* __restrict__ src, int lds, double2* __restrict__ dst, int ldd,
-                             int rows, int cols) {
-    const int c = blockIdx.y;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < rows && c < cols) dst[(size_t)c * ldd + r] = src[(size_t)c * lds + r];
-}
-
 // ----------------------------------------------------------------- solver
 struct EigWork {
     rocblas_handle blas = nullptr;
@@ -305,7 +290,9 @@ struct EigWork {
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_tag;
     size_t ev_used = 0;
+    HeevWork heev;  // dense Hermitian eigensolver workspace (subspace problems)
     ~EigWork() {
+        heev.release();
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         if (blas) rocblas_destroy_handle(blas);
         DevBuf* bufs[] = {&V, &W, &X, &HX, &R, &Q, &HQ, &T, &Tw, &S, &G, &evals, &ework, &info,
@@ -342,6 +329,32 @@ static void section_times(EigWork& w, double* out) {
         if (w.ev_tag[i] < SEC_COUNT) out[w.ev_tag[i]] += ms;
     }
     w.ev_used = 0;
+}
+
+// Lowest nev eigenpairs of the Hermitian m x m matrix A (lower triangle, leading dimension lda)
+// into Y (leading dimension ldy) and the host vector returned (ascending).  The library's device
+// eigensolver up to m = kHeevMaxM; rocSOLVER zheevd beyond (it overwrites a copy of A in Y,
+// which must then hold m columns).
+static std::vector<double> subspace_eigh(EigWork& w, const double2* A, int lda, int m, int nev, double2* Y, int ldy,
+                                         hipStream_t s) {
+    std::vector<double> ev(nev);
+    if (m <= kHeevMaxM) {
+        heev_lowest(w.heev, A, lda, m, nev, w.evals.as<double>(), Y, ldy, s);
+        DZ_HIP(hipMemcpyAsync(ev.data(), w.evals.ptr, nev * sizeof(double), hipMemcpyDeviceToHost, s));
+        DZ_REQUIRE(heev_error_flag(w.heev, s) == 0, DZHIP_ERR_HIP,
+                   "Hermitian eigensolver: tridiagonalisation barrier did not complete");
+        return ev;
+    }
+    DZ_HIP(hipMemcpy2DAsync(Y, (size_t)ldy * sizeof(double2), A, (size_t)lda * sizeof(double2),
+                            (size_t)m * sizeof(double2), m, hipMemcpyDeviceToDevice, s));
+    DZ_BLAS(rocsolver_zheevd(w.blas, rocblas_evect_original, rocblas_fill_lower, m, reinterpret_cast<zc*>(Y), ldy,
+                             w.evals.as<double>(), w.ework.as<double>(), w.info.as<rocblas_int>()));
+    rocblas_int info = 0;
+    DZ_HIP(hipMemcpyAsync(ev.data(), w.evals.ptr, nev * sizeof(double), hipMemcpyDeviceToHost, s));
+    DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
+    DZ_HIP(hipStreamSynchronize(s));
+    DZ_REQUIRE(info == 0, DZHIP_ERR_HIP, "zheevd did not converge");
+    return ev;
 }
 
 static void eig_work_release(Ham& h) {
@@ -485,20 +498,15 @@ struct Solver {
             return k;
         }
         st.ortho_fallbacks++;
-        DZ_BLAS(rocsolver_zheevd(w.blas, rocblas_evect_original, rocblas_fill_upper, k, w.S.as<zc>(), k,
-                                 w.evals.as<double>(), w.ework.as<double>(), w.info.as<rocblas_int>()));
-        std::vector<double> lam(k);
-        DZ_HIP(hipMemcpyAsync(lam.data(), w.evals.ptr, k * sizeof(double), hipMemcpyDeviceToHost, s));
-        DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
-        DZ_HIP(hipStreamSynchronize(s));
-        DZ_REQUIRE(info == 0, DZHIP_ERR_HIP, "zheevd of the Gram matrix did not converge");
+        // Eigen-decomposition of the Gram matrix (its copy in S, lower triangle) into G.
+        const std::vector<double> lam = subspace_eigh(w, w.S.as<double2>(), k, k, k, w.G.as<double2>(), k, s);
         const double thr = 1e-12 * std::max(lam[k - 1], 0.0);
         int first = 0;
         while (first < k && !(lam[first] > thr)) ++first;
         const int kk = k - first;
         if (kk == 0) return 0;
         gemm(rocblas_operation_none, rocblas_operation_none, ldn, kk, k, one, Q, ldn,
-             w.S.as<double2>() + (size_t)first * k, k, zero, w.HQ.as<double2>(), ldn);
+             w.G.as<double2>() + (size_t)first * k, k, zero, w.HQ.as<double2>(), ldn);
         std::vector<int> idx(kk);
         std::vector<double> scl(kk);
         for (int j = 0; j < kk; ++j) {
@@ -609,20 +617,10 @@ struct Solver {
         extend_T(0, m);
     }
 
-    // Eigen-decomposition of T[0:m, 0:m]; eigenvectors land in Tw, eigenvalues (host) returned.
+    // Lowest nb eigenpairs of T[0:m, 0:m] (m >= nb); eigenvectors land in Tw, eigenvalues (host)
+    // returned.
     std::vector<double> rayleigh_ritz(int m) {
-        hipLaunchKernelGGL(k_copy_block, dim3((unsigned)((m + 255) / 256), (unsigned)m), dim3(256), 0, s, T(),
-                           max_sub, w.Tw.as<double2>(), max_sub, m, m);
-        DZ_HIP(hipGetLastError());
-        DZ_BLAS(rocsolver_zheevd(w.blas, rocblas_evect_original, rocblas_fill_lower, m, w.Tw.as<zc>(), max_sub,
-                                 w.evals.as<double>(), w.ework.as<double>(), w.info.as<rocblas_int>()));
-        std::vector<double> ev(m);
-        rocblas_int info = 0;
-        DZ_HIP(hipMemcpyAsync(ev.data(), w.evals.ptr, m * sizeof(double), hipMemcpyDeviceToHost, s));
-        DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
-        DZ_HIP(hipStreamSynchronize(s));
-        DZ_REQUIRE(info == 0, DZHIP_ERR_HIP, "zheevd did not converge");
-        return ev;
+        return subspace_eigh(w, T(), max_sub, m, std::min(m, nb), w.Tw.as<double2>(), max_sub, s);
     }
 
     void ritz(int m) {
@@ -1145,16 +1143,13 @@ void cg_solve(Ham& h, const double* diag, uint64_t nbands, const dzhip_eig_opts&
     DZ_BLAS(rocblas_zgemm(w.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, nb, nb, (int)n, &one,
                           reinterpret_cast<const zc*>(V), (int)n, reinterpret_cast<const zc*>(HV), (int)n, &zero,
                           w.Tw.as<zc>(), nb));
-    DZ_BLAS(rocsolver_zheevd(w.blas, rocblas_evect_original, rocblas_fill_lower, nb, w.Tw.as<zc>(), nb,
-                             w.evals.as<double>(), w.ework.as<double>(), w.info.as<rocblas_int>()));
+    // Eigenvectors of T (in Tw) into S (nb x nb).
+    const std::vector<double> ev = subspace_eigh(w, w.Tw.as<double2>(), nb, nb, nb, w.S.as<double2>(), nb, s);
     DZ_BLAS(rocblas_zgemm(w.blas, rocblas_operation_none, rocblas_operation_none, (int)n, nb, nb, &one,
-                          reinterpret_cast<const zc*>(V), (int)n, w.Tw.as<zc>(), nb, &zero, w.X.as<zc>(), (int)n));
-    rocblas_int info = 0;
-    DZ_HIP(hipMemcpyAsync(values, w.evals.ptr, nb * sizeof(double), hipMemcpyDeviceToHost, s));
-    DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
+                          reinterpret_cast<const zc*>(V), (int)n, w.S.as<zc>(), nb, &zero, w.X.as<zc>(), (int)n));
+    std::copy(ev.begin(), ev.end(), values);
     if (vectors) DZ_HIP(hipMemcpyAsync(vectors, w.X.ptr, n * nb * sizeof(double2), hipMemcpyDeviceToHost, s));
     DZ_HIP(hipStreamSynchronize(s));
-    DZ_REQUIRE(info == 0, DZHIP_ERR_HIP, "zheevd did not converge");
     w.x_cols = nb;
     w.x_converged = all_conv;
     st.converged = all_conv ? 1 : 0;

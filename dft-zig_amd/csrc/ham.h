@@ -63,8 +63,8 @@ struct Ham {
     // Work buffers.
     DevBuf xstage, ystage, x32, y32, tbuf, partial, qbuf;
     // Tuned launch shapes: x-line variant per precision ([0] FP64, [1] FP32): 0 not yet tuned,
-    // d >= 1 the LDS kernel with d y-slabs, -1..-9 a register kernel (xline_reg_threads,
-    // xline_reg_lanes); tune_fixed != 0 forces a variant (sweeps).
+    // d >= 1 the LDS kernel with d y-slabs, -1..-12 a register kernel (xline_reg_threads,
+    // xline_reg_kind); tune_fixed != 0 forces a variant (sweeps).
     int xline_split[2] = {0, 0};
     int tune_fixed = 0;
 
@@ -169,15 +169,64 @@ bool xline_tri_usable(const Ham& h);
 // Register x-line variants: -1 / -2 / -3 = one lane per x-line in workgroups of 256 / 128 / 64
 // threads, -4 / -5 / -6 = the lane-pair kernel, -7 / -8 / -9 = the lane-triple kernel, in the same
 // workgroup sizes.
-constexpr int kXlineReg = -1, kXlineRegLast = -9;
+constexpr int kXlineReg = -1, kXlineRegLast = -12;
+// Register x-line kernel kinds (codes -1..-3, -4..-6, -7..-9, -10..-12 in that order).
+constexpr int kXlineOneLane = 0, kXlinePair = 1, kXlineTriple = 2, kXlineSeq3 = 3;
 constexpr int xline_reg_threads(int code) { return 256 >> ((-code - 1) % 3); }
-constexpr int xline_reg_lanes(int code) { return 1 + (-code - 1) / 3; }
-constexpr bool xline_reg_pair(int code) { return xline_reg_lanes(code) == 2; }
+constexpr int xline_reg_kind(int code) { return (-code - 1) / 3; }
+// Threads per x-line: one (one-lane and residue-sequential kernels), two or three.
+constexpr int xline_reg_lanes(int code) { return xline_reg_kind(code) == kXlineSeq3 ? 1 : 1 + xline_reg_kind(code); }
+constexpr bool xline_reg_pair(int code) { return xline_reg_kind(code) == kXlinePair; }
 // Whether variant `code` (d >= 1 LDS slabs, or a register variant) can run on h in a precision.
 bool xline_code_usable(const Ham& h, int code, bool fp64);
 void ham_set_local(Ham& h, const double* local_r);
 // local_r already on the device (read on h's stream).
 void ham_set_local_dev(Ham& h, const double* local_dev);
+
+// ----------------------------------------------------------------- batched multi-k apply
+// Several k-point contexts of one grid and projector set applied in one launch sequence
+// (optionally one graph launch).  Device layout of x and y: band-major over the concatenated
+// spheres, element (k, b, G) at b * npw_total + npw_off[k] + G.  The members' local potential is
+// taken from member 0 (all members hold the same one in an SCF iteration).
+struct HamSet {
+    Context* ctx = nullptr;
+    std::vector<Ham*> ks;
+    hipStream_t user_stream = nullptr;
+    hipStream_t stream() const { return user_stream ? user_stream : ctx->stream; }
+    int nx = 0, ny = 0, nz = 0, nproj = 0;
+    size_t npw_total = 0;
+    int np_total = 0, max_planes = 0;
+    std::vector<size_t> npw_off;
+    std::vector<int> np_off;
+    DevBuf plane_start, plane_pw, plane_pos, plane_ystart, plane_ycols, kpl, kinetic64, kinetic32, a64, a32;
+    // Nonlocal segment tables for seg_cols columns in precision seg_prec (element size).
+    size_t seg_cols = 0;
+    int seg_prec = 0, nks_total = 0, ntiles = 0;
+    DevBuf chunks, kseg, tiles;
+    DevBuf tbuf, partial, qbuf;
+    // Captured launch sequence and what it was captured for.
+    struct GraphKey {
+        const void* x;
+        void* y;
+        size_t nc;
+        int prec, code;
+        hipStream_t s;
+        void *tbuf, *partial, *qbuf;
+        bool operator==(const GraphKey& o) const {
+            return x == o.x && y == o.y && nc == o.nc && prec == o.prec && code == o.code && s == o.s &&
+                   tbuf == o.tbuf && partial == o.partial && qbuf == o.qbuf;
+        }
+    };
+    hipGraphExec_t graph_exec = nullptr;
+    GraphKey graph_key{};
+    uint64_t graph_builds = 0;
+    hipEvent_t ev_switch = nullptr;
+    std::mutex mu;
+};
+void hamset_build(HamSet& hs);
+template <typename T>
+void hamset_apply_dev(HamSet& hs, const void* x, void* y, size_t ncols, bool use_graph);
+void hamset_release(HamSet& hs);
 
 // ----------------------------------------------------------------- SCF grid (potential.hip)
 struct ScfGrid;

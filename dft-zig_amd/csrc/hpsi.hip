@@ -209,6 +209,13 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned vof
     else __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(BufU4, v), r, voff, soff, 0);
 }
 
+// Planes of the x-line pass of one workgroup: (first plane, plane count) of k-point blockIdx.z in a
+// batched apply over several k-points (kpl != nullptr, the planes of all k-points concatenated per
+// band), or (0, nplanes) for a single context.
+__device__ __forceinline__ int2 xl_planes(const int2* __restrict__ kpl, int nplanes) {
+    return kpl ? kpl[blockIdx.z] : make_int2(0, nplanes);
+}
+
 // x-line pass with one x-line per thread held in registers (nx fixed at compile time): thread
 // l = z ny + y of band blockIdx.x loads its values of the np sphere planes (consecutive threads
 // read consecutive addresses of each plane), runs the inverse x-transform, multiplies by V_loc/N
@@ -220,21 +227,22 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned vof
 // and the twiddles are compile-time constants.
 template <typename T, int NX>
 __global__ __launch_bounds__(256) void k_xline_reg(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
-                                                   const T* __restrict__ vt) {
+                                                   const T* __restrict__ vt, const int2* __restrict__ kpl) {
     using C = typename C2T<T>::type;
     // Bands run fastest over the grid (blockIdx.x), so the workgroups that follow each other
     // read the same V_loc lines while they are still in cache.
     const unsigned l = blockIdx.y * blockDim.x + threadIdx.x;
     if (l >= (unsigned)lines) return;
     const int row = lines * (int)sizeof(C), vrow = lines * (int)sizeof(T);
-    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tbuf + (size_t)blockIdx.x * nplanes * lines, nplanes * row);
+    const int2 kp = xl_planes(kpl, nplanes);
+    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tbuf + ((size_t)blockIdx.x * nplanes + kp.x) * lines, kp.y * row);
     const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vt, NX * vrow);
     const unsigned vo = l * (unsigned)sizeof(C), vvo = l * (unsigned)sizeof(T);
     C a[NX];
 #pragma unroll
     for (int u = 0; u < NX; ++u) {
         a[u] = C{T(0), T(0)};
-        if (u < nplanes) a[u] = buf_load<C>(rt, vo, u * row);
+        if (u < kp.y) a[u] = buf_load<C>(rt, vo, u * row);
     }
     freg::fft<NX, 1>(a);
 #pragma unroll
@@ -245,7 +253,7 @@ __global__ __launch_bounds__(256) void k_xline_reg(typename C2T<T>::type* __rest
     freg::fft<NX, -1>(a);
     // Fresh copies of the scalars keep the compiler from holding the load side's row offsets and
     // comparisons live across both transforms.
-    int np = nplanes, rs = row;
+    int np = kp.y, rs = row;
     asm volatile("" : "+s"(np), "+s"(rs));
 #pragma unroll
     for (int u = 0; u < NX; ++u)
@@ -279,7 +287,7 @@ __device__ __forceinline__ X pair_partner(X v) {
 
 template <typename T, int NX>
 __global__ __launch_bounds__(256) void k_xline_pair(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
-                                                    const T* __restrict__ vt) {
+                                                    const T* __restrict__ vt, const int2* __restrict__ kpl) {
     using C = typename C2T<T>::type;
     constexpr int M = NX / 2;
     static_assert(NX % 2 == 0, "pair kernel needs an even line");
@@ -289,7 +297,8 @@ __global__ __launch_bounds__(256) void k_xline_pair(typename C2T<T>::type* __res
     const bool odd = (g & 1u) != 0;
     if (l >= (unsigned)lines) return;
     const int row = lines * (int)sizeof(C), vrow = lines * (int)sizeof(T);
-    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tbuf + (size_t)blockIdx.x * nplanes * lines, nplanes * row);
+    const int2 kp = xl_planes(kpl, nplanes);
+    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tbuf + ((size_t)blockIdx.x * nplanes + kp.x) * lines, kp.y * row);
     const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vt, NX * vrow);
     const unsigned vo = l * (unsigned)sizeof(C);
     const unsigned vvo = l * (unsigned)sizeof(T) + (odd ? (unsigned)vrow : 0u);
@@ -297,7 +306,7 @@ __global__ __launch_bounds__(256) void k_xline_pair(typename C2T<T>::type* __res
 #pragma unroll
     for (int u = 0; u < M; ++u) {
         a[u] = C{T(0), T(0)};
-        if (u < nplanes) a[u] = buf_load<C>(rt, vo, u * row);
+        if (u < kp.y) a[u] = buf_load<C>(rt, vo, u * row);
     }
     // Lane 1: b[u] e^{+2 pi i u / NX} (the product with literal twiddles, then a per-lane select).
 #pragma unroll
@@ -322,7 +331,7 @@ __global__ __launch_bounds__(256) void k_xline_pair(typename C2T<T>::type* __res
         a[k].x = odd ? px : a[k].x;
         a[k].y = odd ? py : a[k].y;
     }
-    int np = nplanes, rs = row;
+    int np = kp.y, rs = row;
     asm volatile("" : "+s"(np), "+s"(rs));
 #pragma unroll
     for (int k = 0; k < M; ++k) {
@@ -359,7 +368,7 @@ __device__ __forceinline__ X lane_read(X v, int src_lane) {
 
 template <typename T, int NX>
 __global__ __launch_bounds__(256) void k_xline_tri(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
-                                                   const T* __restrict__ vt) {
+                                                   const T* __restrict__ vt, const int2* __restrict__ kpl) {
     using C = typename C2T<T>::type;
     static_assert(NX % 3 == 0, "lane-triple kernel needs a line divisible by 3");
     constexpr int M = NX / 3;
@@ -374,7 +383,8 @@ __global__ __launch_bounds__(256) void k_xline_tri(typename C2T<T>::type* __rest
     const bool store_ok = lane < 63 && lraw < lines;
     const int l = lraw < lines ? lraw : lines - 1;
     const int row = lines * (int)sizeof(C), vrow = lines * (int)sizeof(T);
-    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tbuf + (size_t)blockIdx.x * nplanes * lines, nplanes * row);
+    const int2 kp = xl_planes(kpl, nplanes);
+    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tbuf + ((size_t)blockIdx.x * nplanes + kp.x) * lines, kp.y * row);
     const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vt, NX * vrow);
     const unsigned vo = (unsigned)l * (unsigned)sizeof(C);
     const unsigned vvo = (unsigned)l * (unsigned)sizeof(T) + (unsigned)(t * vrow);
@@ -385,7 +395,7 @@ __global__ __launch_bounds__(256) void k_xline_tri(typename C2T<T>::type* __rest
     // Fold the present rows with the lane's twist w^{u t} (literal twiddles selected per lane).
 #pragma unroll
     for (int u = 0; u < NX; ++u) {
-        if (u < nplanes) {
+        if (u < kp.y) {
             const C a = buf_load<C>(rt, vo, u * row);
             const int e1 = u % NX, e2 = (2 * u) % NX;
             const T wc = t1 ? T(freg::kRoots<NX>.c[e1]) : (t2 ? T(freg::kRoots<NX>.c[e2]) : T(1));
@@ -421,7 +431,7 @@ __global__ __launch_bounds__(256) void k_xline_tri(typename C2T<T>::type* __rest
     const T own_c = t == 0 ? T(1) : kC3, own_s = t == 0 ? T(0) : -kS3;
     const T n1_c = t1 ? kC3 : T(1), n1_s = t1 ? kS3 : T(0);
     const T n2_c = t2 ? kC3 : T(1), n2_s = t2 ? kS3 : T(0);
-    int np = nplanes, rs = row;
+    int np = kp.y, rs = row;
     asm volatile("" : "+s"(np), "+s"(rs));
 #pragma unroll
     for (int k = 0; k < M; ++k) {
@@ -434,6 +444,104 @@ __global__ __launch_bounds__(256) void k_xline_tri(typename C2T<T>::type* __rest
             if (store_ok && kk < np) buf_store<C>(rt, vo, kk * rs, C{xr, xi});
         }
     }
+}
+
+// Residue-sequential variant for lines divisible by 3 (M = NX / 3) whose sphere spans at most two
+// thirds of the line (np <= 2M): one thread per x-line, like k_xline_reg, but the line is never
+// held whole.  With the rotated rows b[u], w = e^{2 pi i / NX} and x = 3m + t, the three residues t
+// are done one after another, so every twiddle is a compile-time constant:
+//   c_t[u0] = sum_j b[u0 + jM] w^{(u0 + jM) t}        (u0 < M; fold of the present rows)
+//   psi(3m + t) = inverse length-M transform of c_t;  times V_loc(3m + t)/N
+//   H_t[k0] = w^{-k0 t} F_t[k0],  F_t = forward length-M transform
+//   X[k0 + jM] = sum_t e^{-2 pi i j t / 3} H_t[k0]     (j = 0, 1: rows below np <= 2M)
+// X[k0] accumulates in registers, X[k0 + M] (k0 < np - M) in this thread's LDS column
+// acc[k0][threadIdx.x].  The rows are re-read for each residue (cache hits after the first).
+// Registers: two length-M arrays, so FP64 lines of 75 fit at two waves per SIMD.
+template <typename T, int NX>
+__global__ __launch_bounds__(256) void k_xline_seq3(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
+                                                    const T* __restrict__ vt, const int2* __restrict__ kpl) {
+    using C = typename C2T<T>::type;
+    static_assert(NX % 3 == 0, "residue-sequential kernel needs a line divisible by 3");
+    constexpr int M = NX / 3;
+    extern __shared__ unsigned char smem_raw[];
+    C* acc = reinterpret_cast<C*>(smem_raw);
+    const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
+    const unsigned l = blockIdx.y * blockDim.x + threadIdx.x;
+    if (l >= (unsigned)lines) return;  // no barrier below: each thread uses only its own LDS column
+    const int row = lines * (int)sizeof(C), vrow = lines * (int)sizeof(T);
+    const int2 kp = xl_planes(kpl, nplanes);
+    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(tbuf + ((size_t)blockIdx.x * nplanes + kp.x) * lines, kp.y * row);
+    const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vt, NX * vrow);
+    const unsigned vo = l * (unsigned)sizeof(C), vvo = l * (unsigned)sizeof(T);
+    const int n1 = kp.y - M;  // rows k0 + M present for k0 < n1 (n1 <= M)
+    constexpr T kC3 = T(-0.5), kS3 = T(0.86602540378443864676);
+    C x0[M];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        C c[M];
+#pragma unroll
+        for (int u = 0; u < M; ++u) c[u] = C{T(0), T(0)};
+        // A fresh copy of the offset per residue, tied to the previous residue's result: the rows
+        // are loaded again after that residue is done instead of being kept live in registers.
+        unsigned vot = vo;
+        if (t == 0) asm volatile("" : "+v"(vot));
+        else asm volatile("" : "+v"(vot) : "v"(x0[M - 1].x));
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {
+            if (u < kp.y) {
+                const C b = buf_load<C>(rt, vot, u * row);
+                C& d = c[u % M];
+                if (t == 0 || (u * t) % NX == 0) {
+                    d.x += b.x;
+                    d.y += b.y;
+                } else {
+                    const T wc = T(freg::kRoots<NX>.c[(u * t) % NX]), ws = T(freg::kRoots<NX>.s[(u * t) % NX]);
+                    d.x += b.x * wc - b.y * ws;
+                    d.y += b.x * ws + b.y * wc;
+                }
+            }
+        }
+        freg::fft<M, 1>(c);
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const T v = buf_load<T>(rv, vvo, (3 * m + t) * vrow);
+            c[m] = C{c[m].x * v, c[m].y * v};
+        }
+        freg::fft<M, -1>(c);
+        if (t > 0) {
+#pragma unroll
+            for (int k = 1; k < M; ++k) {
+                const T wc = T(freg::kRoots<NX>.c[(k * t) % NX]), ws = T(freg::kRoots<NX>.s[(k * t) % NX]);
+                const T px = c[k].x * wc + c[k].y * ws, py = c[k].y * wc - c[k].x * ws;
+                c[k].x = px;
+                c[k].y = py;
+            }
+        }
+        // j = 1 coefficient e^{-2 pi i t / 3}: 1, (-1/2, -sqrt3/2), (-1/2, +sqrt3/2).
+        const T fc = t == 0 ? T(1) : kC3, fs = t == 0 ? T(0) : (t == 1 ? -kS3 : kS3);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            if (t == 0) x0[k] = c[k];
+            else {
+                x0[k].x += c[k].x;
+                x0[k].y += c[k].y;
+            }
+            if (k < n1) {
+                C& a1 = acc[(size_t)k * nt + tid];
+                const C h = C{c[k].x * fc - c[k].y * fs, c[k].x * fs + c[k].y * fc};
+                if (t == 0) a1 = h;
+                else a1 = C{a1.x + h.x, a1.y + h.y};
+            }
+        }
+    }
+    int np = kp.y, rs = row;
+    asm volatile("" : "+s"(np), "+s"(rs));
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        if (k < np) buf_store<C>(rt, vo, k * rs, x0[k]);
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        if (k < n1) buf_store<C>(rt, vo, (k + M) * rs, acc[(size_t)k * nt + tid]);
 }
 
 // Line lengths the register x-line kernel is built for: FP32 from 8 to 100, FP64 (twice the
@@ -507,32 +615,58 @@ bool xline_tri_usable(const Ham& h) {
     return xline_tri_supported(h.nx) && h.plane_xs >= 0 && h.ngrid * 2 * sizeof(double) < ((size_t)1 << 31);
 }
 
+bool xline_seq3_usable(const Ham& h) {
+    return xline_tri_supported(h.nx) && h.plane_xs >= 0 && h.nplanes <= 2 * (h.nx / 3) &&
+           h.ngrid * 2 * sizeof(double) < ((size_t)1 << 31);
+}
+
 bool xline_code_usable(const Ham& h, int code, bool fp64) {
     if (code > 0) return h.ny % code == 0;
     if (code < kXlineRegLast || code > kXlineReg) return false;
-    switch (xline_reg_lanes(code)) {
-        case 1: return xline_reg_usable(h, fp64);
-        case 2: return xline_pair_usable(h);
-        default: return xline_tri_usable(h);
+    switch (xline_reg_kind(code)) {
+        case kXlineOneLane: return xline_reg_usable(h, fp64);
+        case kXlinePair: return xline_pair_usable(h);
+        case kXlineTriple: return xline_tri_usable(h);
+        default: return xline_seq3_usable(h);
     }
 }
 
 template <typename T>
-void launch_xline_reg(Ham& h, typename C2T<T>::type* tb, size_t nc, int threads, int nlanes, hipStream_t s) {
+// Register x-line kernel `kind` on h's grid.  A batched apply over nk k-points passes kpl (per
+// k-point: first plane and plane count within each band's planes), the planes per band and the
+// largest per-k plane count; a single context passes kpl = nullptr, nk = 1 and its own counts.
+void launch_xline_reg(Ham& h, typename C2T<T>::type* tb, size_t nc, int threads, int kind, hipStream_t s,
+                      const int2* kpl = nullptr, int nk = 1, int planes_per_band = -1, int max_planes = -1) {
     const int lines = h.ny * h.nz;
+    const int nplanes = planes_per_band >= 0 ? planes_per_band : h.nplanes;
+    const int np_max = max_planes >= 0 ? max_planes : h.nplanes;
+    const int nlanes = kind == kXlineSeq3 ? 1 : kind + 1;
     const bool pair = nlanes == 2;
     // The triple kernel puts 21 lines on each wavefront (63 of its 64 lanes).
     const int waves_per_group = threads / 64;
     const unsigned groups = nlanes == 3 ? (unsigned)((lines + 21 * waves_per_group - 1) / (21 * waves_per_group))
                                         : (unsigned)((nlanes * lines + threads - 1) / threads);
-    const dim3 grid((unsigned)nc, groups);
+    const dim3 grid((unsigned)nc, groups, (unsigned)nk);
     const T* vt = std::is_same<T, float>::value ? (const T*)h.vloc32_t.ptr : (const T*)h.vloc64_t.ptr;
     bool done = false;
-    if (nlanes == 3) {
+    if (kind == kXlineSeq3) {
+        const size_t lds = (size_t)std::max(0, np_max - h.nx / 3) * threads * sizeof(typename C2T<T>::type);
         switch (h.nx) {
 #define DZ_XREG_CASE(N)                                                                                      \
     case N:                                                                                                  \
-        hipLaunchKernelGGL((k_xline_tri<T, N>), grid, dim3(threads), 0, s, tb, h.nplanes, lines, vt);       \
+        hipLaunchKernelGGL((k_xline_seq3<T, N>), grid, dim3(threads), lds, s, tb, nplanes, lines, vt, kpl);    \
+        done = true;                                                                                         \
+        break;
+            DZ_XTRI_SIZES(DZ_XREG_CASE)
+#undef DZ_XREG_CASE
+            default:
+                break;
+        }
+    } else if (nlanes == 3) {
+        switch (h.nx) {
+#define DZ_XREG_CASE(N)                                                                                      \
+    case N:                                                                                                  \
+        hipLaunchKernelGGL((k_xline_tri<T, N>), grid, dim3(threads), 0, s, tb, nplanes, lines, vt, kpl);       \
         done = true;                                                                                         \
         break;
             DZ_XTRI_SIZES(DZ_XREG_CASE)
@@ -544,7 +678,7 @@ void launch_xline_reg(Ham& h, typename C2T<T>::type* tb, size_t nc, int threads,
         switch (h.nx) {
 #define DZ_XREG_CASE(N)                                                                                      \
     case N:                                                                                                  \
-        hipLaunchKernelGGL((k_xline_pair<T, N>), grid, dim3(threads), 0, s, tb, h.nplanes, lines, vt);      \
+        hipLaunchKernelGGL((k_xline_pair<T, N>), grid, dim3(threads), 0, s, tb, nplanes, lines, vt, kpl);      \
         done = true;                                                                                         \
         break;
             DZ_XPAIR_SIZES(DZ_XREG_CASE)
@@ -556,7 +690,7 @@ void launch_xline_reg(Ham& h, typename C2T<T>::type* tb, size_t nc, int threads,
         switch (h.nx) {
 #define DZ_XREG_CASE(N)                                                                                      \
     case N:                                                                                                  \
-        hipLaunchKernelGGL((k_xline_reg<T, N>), grid, dim3(threads), 0, s, tb, h.nplanes, lines, vt);       \
+        hipLaunchKernelGGL((k_xline_reg<T, N>), grid, dim3(threads), 0, s, tb, nplanes, lines, vt, kpl);       \
         done = true;                                                                                         \
         break;
             DZ_XREG_SIZES(DZ_XREG_CASE)
@@ -568,7 +702,7 @@ void launch_xline_reg(Ham& h, typename C2T<T>::type* tb, size_t nc, int threads,
         switch (h.nx) {
 #define DZ_XREG_CASE(N)                                                                                      \
     case N:                                                                                                  \
-        hipLaunchKernelGGL((k_xline_reg<T, N>), grid, dim3(threads), 0, s, tb, h.nplanes, lines, vt);       \
+        hipLaunchKernelGGL((k_xline_reg<T, N>), grid, dim3(threads), 0, s, tb, nplanes, lines, vt, kpl);       \
         done = true;                                                                                         \
         break;
             DZ_XREG64_SIZES(DZ_XREG_CASE)
@@ -825,7 +959,8 @@ template <typename T>
 __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::type* __restrict__ a,
                                                         const typename C2T<T>::type* __restrict__ x,
                                                         size_t ldx, typename C2T<T>::type* __restrict__ partial,
-                                                        int P, int npw, int ncols, int kchunk, int ptiles) {
+                                                        int P, int npw, int ncols, int kchunk, int ptiles,
+                                                        const int2* __restrict__ chunks, int lda) {
     using C = typename C2T<T>::type;
     using M = Mfma<T>;
     extern __shared__ unsigned char smem_raw[];
@@ -837,8 +972,14 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::t
     const int pt = lin % ptiles, ks = lin / ptiles;
     const int p0 = pt * kNlP;
     const int b0 = blockIdx.z * kNlBands;
-    const int k0 = ks * kchunk;
-    const int k1 = min(npw, k0 + kchunk);
+    // G range of K-chunk ks: a fixed-size split of one context's sphere, or (batched apply over
+    // several k-points, G of all k-points concatenated) the chunk table's range within one k-point.
+    int k0 = ks * kchunk, k1 = min(npw, ks * kchunk + kchunk);
+    if (chunks) {
+        const int2 ck = chunks[ks];
+        k0 = ck.x;
+        k1 = ck.y;
+    }
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int bg = wave % kNlBandGroups, half = wave / kNlBandGroups;
     const int i16 = lane & 15, k4 = lane >> 4;
@@ -851,7 +992,7 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::t
         const int e = tid + i * kNlThreads;                                                   \
         const int r = e >> 5, c = e & 31, g = (KT) + c, b = b0 + r;                           \
         const bool ok = r < kNlBands && b < ncols && g < k1;                                  \
-        const C v = x[ok ? b * (int)ldx + g : 0];                                             \
+        const C v = x[ok ? (size_t)b * ldx + g : 0];                                          \
         xre[i] = ok ? v.x : T(0);                                                             \
         xim[i] = ok ? v.y : T(0);                                                             \
     }                                                                                         \
@@ -859,7 +1000,7 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_proj(const typename C2T<T>::t
         const int e = tid + i * kNlThreads;                                                   \
         const int r = e >> 5, c = e & 31, g = (KT) + c, p = p0 + r;                           \
         const bool ok = r < kNlP && p < P && g < k1;                                          \
-        const C v = a[ok ? p * npw + g : 0];                                                  \
+        const C v = a[ok ? (size_t)p * lda + g : 0];                                          \
         are[i] = ok ? v.x : T(0);                                                             \
         aim[i] = ok ? v.y : T(0);                                                             \
     }
@@ -959,17 +1100,26 @@ __global__ __launch_bounds__(kDijThreads) void k_nl_dij(const typename C2T<T>::t
                                                         int nks, typename C2T<T>::type* __restrict__ q,
                                                         const ProjMeta* __restrict__ meta,
                                                         const T* __restrict__ dm, int P, int ncols, T inv_vol,
-                                                        int split) {
+                                                        int split, const int2* __restrict__ kseg) {
     using C = typename C2T<T>::type;
     extern __shared__ unsigned char smem_raw[];
     C* part = reinterpret_cast<C*>(smem_raw);                     // [split][P]
     C* sum = split == 1 ? part : part + (size_t)split * P;       // [P]
     const int b = blockIdx.x;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    // K-chunks to sum: all of them, or (batched apply) those of k-point blockIdx.y, whose q block
+    // follows the earlier k-points'.
+    int ks_lo = 0, ks_hi = nks;
+    if (kseg) {
+        const int2 r = kseg[blockIdx.y];
+        ks_lo = r.x;
+        ks_hi = r.y;
+        q += (size_t)blockIdx.y * ncols * P;
+    }
     for (int idx = tid; idx < split * P; idx += nthr) {
         const int j = idx / P, p = idx - j * P;
         C s = C{T(0), T(0)};
-        for (int ks = j; ks < nks; ks += split) {
+        for (int ks = ks_lo + j; ks < ks_hi; ks += split) {
             const C v = partial[((size_t)ks * ncols + b) * P + p];
             s.x += v.x;
             s.y += v.y;
@@ -1006,7 +1156,8 @@ template <typename T>
 __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::type* __restrict__ a,
                                                         const typename C2T<T>::type* __restrict__ q,
                                                         typename C2T<T>::type* __restrict__ y, size_t ldy,
-                                                        int P, int npw, int ncols) {
+                                                        int P, int npw, int ncols, const int4* __restrict__ tiles,
+                                                        int lda) {
     using C = typename C2T<T>::type;
     using M = Mfma<T>;
     extern __shared__ unsigned char smem_raw[];
@@ -1015,7 +1166,15 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::t
     constexpr int kNlGPitch = kNlG + 1;
     constexpr int GB = NlBackTile<T>::kBlocks / 2;  // G blocks per wavefront
     C* as = qs + kNlBands * kNlPitch;        // [kNlK][kNlGPitch]     (p, G)
-    const int g0 = blockIdx.x * kNlG;
+    // G tile: fixed-size tiles of one context's sphere, or (batched apply) the tile table's
+    // (first G, end of its k-point's G range, k-point) with that k-point's q block.
+    int g0 = blockIdx.x * kNlG, gend = npw;
+    if (tiles) {
+        const int4 tl = tiles[blockIdx.x];
+        g0 = tl.x;
+        gend = tl.y;
+        q += (size_t)tl.z * ncols * P;
+    }
     const int b0 = blockIdx.z * kNlBands;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int bg = wave % kNlBandGroups, half = wave / kNlBandGroups;
@@ -1036,8 +1195,8 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::t
     _Pragma("unroll") for (int i = 0; i < kAPer; ++i) {                                      \
         const int e = tid + i * kNlThreads;                                                   \
         const int r = e / kNlG, c = e % kNlG, p = (PT) + r, g = g0 + c;                       \
-        const bool ok = r < kNlK && p < P && g < npw;                                         \
-        const C v = a[ok ? p * npw + g : 0];                                                  \
+        const bool ok = r < kNlK && p < P && g < gend;                                        \
+        const C v = a[ok ? (size_t)p * lda + g : 0];                                          \
         are[i] = ok ? v.x : T(0);                                                             \
         aim[i] = ok ? v.y : T(0);                                                             \
     }
@@ -1107,7 +1266,7 @@ __global__ __launch_bounds__(kNlThreads) void k_nl_back(const typename C2T<T>::t
                 const int b = b0 + bg * 32 + bb * 16 + M::row(k4, r);
                 const int g = g0 + (half * GB + gb) * 16 + i16;
                 const T c1 = acc1[bb][gb][r], c2 = acc2[bb][gb][r];
-                if (b < ncols && g < npw) y[(size_t)b * ldy + g] = C{c1 - c2, acc3[bb][gb][r] - c1 - c2};
+                if (b < ncols && g < gend) y[(size_t)b * ldy + g] = C{c1 - c2, acc3[bb][gb][r] - c1 - c2};
             }
 }
 
@@ -1499,7 +1658,7 @@ void launch_xline(Ham& h, typename C2T<T>::type* tb, size_t nc, int split, hipSt
 template <typename T>
 void launch_xline_any(Ham& h, typename C2T<T>::type* tb, size_t nc, int code, hipStream_t s) {
     if (code <= kXlineReg && code >= kXlineRegLast) {
-        launch_xline_reg<T>(h, tb, nc, xline_reg_threads(code), xline_reg_lanes(code), s);
+        launch_xline_reg<T>(h, tb, nc, xline_reg_threads(code), xline_reg_kind(code), s);
         return;
     }
     launch_xline<T>(h, tb, nc, code, s);
@@ -1537,6 +1696,8 @@ int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
     // The first candidate is also what narrow passes run before a wide pass has tuned.
     if (xline_pair_usable(h))
         for (int c = -4; c >= -6; --c) cand.push_back(c);
+    if (xline_seq3_usable(h))
+        for (int c = -10; c >= -12; --c) cand.push_back(c);
     if (xline_tri_usable(h))
         for (int c = -7; c >= -9; --c) cand.push_back(c);
     if (xline_reg_usable(h, !fp32))
@@ -1640,13 +1801,13 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
             set_max_dynamic_lds((const void*)k_nl_proj<T>, lds);
             hipLaunchKernelGGL(k_nl_proj<T>, dim3(ptiles * nks, 1, btiles), dim3(kNlThreads), lds, snl,
                                Sel<T>::a(h), xc, ld, h.partial.as<C>(), h.nproj, (int)h.npw, (int)nc,
-                               kchunk, ptiles);
+                               kchunk, ptiles, (const int2*)nullptr, (int)h.npw);
             DZ_HIP(hipGetLastError());
             const size_t lds_d = dij_lds_bytes(h.nproj, sizeof(C));
             set_max_dynamic_lds((const void*)k_nl_dij<T>, lds_d);
             hipLaunchKernelGGL(k_nl_dij<T>, dim3((unsigned)nc), dim3(kDijThreads), lds_d, snl, h.partial.as<C>(),
                                nks, h.qbuf.as<C>(), h.meta.as<ProjMeta>(), Sel<T>::dm(h), h.nproj, (int)nc,
-                               (T)h.inv_volume, dij_split(h.nproj, sizeof(C)));
+                               (T)h.inv_volume, dij_split(h.nproj, sizeof(C)), (const int2*)nullptr);
             DZ_HIP(hipGetLastError());
         }
         mark(h, evs, 1);
@@ -1657,7 +1818,7 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
             hipLaunchKernelGGL(k_nl_back<T>, dim3((unsigned)((h.npw + kNlG - 1) / kNlG), 1,
                                                   (unsigned)((nc + kNlBands - 1) / kNlBands)),
                                dim3(kNlThreads), lds, snl, Sel<T>::a(h), h.qbuf.as<C>(), yc, ld, h.nproj,
-                               (int)h.npw, (int)nc);
+                               (int)h.npw, (int)nc, (const int4*)nullptr, (int)h.npw);
             DZ_HIP(hipGetLastError());
         }
         if (overlap) DZ_HIP(hipEventRecord(h.ev_join, snl));
@@ -1735,5 +1896,280 @@ void ham_density_dev(Ham& h, const double2* psi, size_t nbands, const double* wt
 
 template void ham_apply_dev<double>(Ham&, const void*, void*, size_t, bool, bool);
 template void ham_apply_dev<float>(Ham&, const void*, void*, size_t, bool, bool);
+
+// ----------------------------------------------------------------- batched multi-k apply
+// All k-points of a set as one sphere: per band, the plane waves of every k-point follow each
+// other (k-point k at offset npw_off[k], band stride npw_total) and so do their x-planes, so the
+// yz-plane kernels run unchanged over the concatenated plane tables; the x-line kernels take each
+// k-point's plane range (blockIdx.z), and the nonlocal kernels each k-point's K-chunks, D block and
+// G tiles from small tables.  Every per-k quantity keeps the per-context split (K-chunk boundaries,
+// G tiles, x-line kernel), so the set's result equals the per-context applies bit for bit.
+
+template <typename V>
+static std::vector<V> download(const DevBuf& b, size_t n) {
+    std::vector<V> v(n);
+    if (n) DZ_HIP(hipMemcpy(v.data(), b.ptr, n * sizeof(V), hipMemcpyDeviceToHost));
+    return v;
+}
+
+void hamset_build(HamSet& hs) {
+    DZ_REQUIRE(!hs.ks.empty(), DZHIP_ERR_ARG, "empty k-point set");
+    const Ham& h0 = *hs.ks[0];
+    hs.nx = h0.nx;
+    hs.ny = h0.ny;
+    hs.nz = h0.nz;
+    hs.nproj = h0.nproj;
+    const int nk = (int)hs.ks.size();
+    std::vector<int> start{0}, pw, pos, ystart{0}, kpl;
+    std::vector<short> ycols;
+    hs.npw_off.assign(nk + 1, 0);
+    hs.np_off.assign(nk + 1, 0);
+    hs.max_planes = 0;
+    for (int k = 0; k < nk; ++k) {
+        const Ham& h = *hs.ks[k];
+        DZ_REQUIRE(h.ctx->device == h0.ctx->device, DZHIP_ERR_ARG, "k-point contexts on different devices");
+        DZ_REQUIRE(h.nx == h0.nx && h.ny == h0.ny && h.nz == h0.nz, DZHIP_ERR_SIZE, "k-points on different grids");
+        DZ_REQUIRE(h.nproj == h0.nproj && h.nblocks == h0.nblocks, DZHIP_ERR_ARG,
+                   "k-points with different projector sets");
+        DZ_REQUIRE(h.has_local, DZHIP_ERR_STATE, "local potential not set on every k-point");
+        const size_t off = hs.npw_off[k];
+        const std::vector<int> st = download<int>(h.plane_start, h.nplanes + 1);
+        const std::vector<int> pk = download<int>(h.plane_pw, h.npw);
+        const std::vector<int> qk = download<int>(h.plane_pos, h.npw);
+        const std::vector<int> ys = download<int>(h.plane_ystart, h.nplanes + 1);
+        const std::vector<short> yc = download<short>(h.plane_ycols, (size_t)ys[h.nplanes]);
+        const int e0 = start.back(), c0 = ystart.back();
+        for (int p = 1; p <= h.nplanes; ++p) {
+            start.push_back(e0 + st[p]);
+            ystart.push_back(c0 + ys[p]);
+        }
+        for (size_t j = 0; j < h.npw; ++j) {
+            pw.push_back((int)(off + pk[j]));
+            pos.push_back(qk[j]);
+        }
+        ycols.insert(ycols.end(), yc.begin(), yc.end());
+        kpl.push_back(hs.np_off[k]);
+        kpl.push_back(h.nplanes);
+        hs.npw_off[k + 1] = off + h.npw;
+        hs.np_off[k + 1] = hs.np_off[k] + h.nplanes;
+        hs.max_planes = std::max(hs.max_planes, h.nplanes);
+    }
+    hs.npw_total = hs.npw_off[nk];
+    hs.np_total = hs.np_off[nk];
+    DZ_REQUIRE(hs.npw_total < ((size_t)1 << 31) && hs.npw_total * kMaxColsPerPass < ((size_t)1 << 31) * 8,
+               DZHIP_ERR_SIZE, "k-point set too large");
+    upload(hs.plane_start, start);
+    upload(hs.plane_pw, pw);
+    upload(hs.plane_pos, pos);
+    upload(hs.plane_ystart, ystart);
+    upload(hs.plane_ycols, ycols);
+    upload(hs.kpl, kpl);
+    hs.kinetic64.ensure(hs.npw_total * sizeof(double));
+    hs.kinetic32.ensure(hs.npw_total * sizeof(float));
+    if (hs.nproj > 0) {
+        hs.a64.ensure((size_t)hs.nproj * hs.npw_total * sizeof(double2));
+        hs.a32.ensure((size_t)hs.nproj * hs.npw_total * sizeof(float2));
+    }
+    for (int k = 0; k < nk; ++k) {
+        const Ham& h = *hs.ks[k];
+        const size_t off = hs.npw_off[k];
+        DZ_HIP(hipMemcpy(hs.kinetic64.as<double>() + off, h.kinetic64.ptr, h.npw * sizeof(double),
+                         hipMemcpyDeviceToDevice));
+        DZ_HIP(hipMemcpy(hs.kinetic32.as<float>() + off, h.kinetic32.ptr, h.npw * sizeof(float),
+                         hipMemcpyDeviceToDevice));
+        if (hs.nproj > 0) {
+            DZ_HIP(hipMemcpy2D(hs.a64.as<double2>() + off, hs.npw_total * sizeof(double2), h.a64.ptr,
+                               h.npw * sizeof(double2), h.npw * sizeof(double2), hs.nproj, hipMemcpyDeviceToDevice));
+            DZ_HIP(hipMemcpy2D(hs.a32.as<float2>() + off, hs.npw_total * sizeof(float2), h.a32.ptr,
+                               h.npw * sizeof(float2), h.npw * sizeof(float2), hs.nproj, hipMemcpyDeviceToDevice));
+        }
+    }
+    hs.seg_cols = 0;
+}
+
+// Nonlocal segment tables for ncols bands: per k-point the K-chunks a single context's apply of
+// that many columns would use, their ranges in the concatenated G axis, and the G tiles.
+template <typename T>
+static void hamset_segments(HamSet& hs, size_t nc) {
+    if (hs.seg_cols == nc && hs.seg_prec == (int)sizeof(T)) return;
+    constexpr int kNlP = nl_p_tile<T>();
+    constexpr int kNlG = nl_g_tile<T>();
+    const int ptiles = (hs.nproj + kNlP - 1) / kNlP;
+    const int btiles = (int)((nc + kNlBands - 1) / kNlBands);
+    std::vector<int> chunks, kseg;
+    std::vector<int> tiles;
+    for (size_t k = 0; k < hs.ks.size(); ++k) {
+        const int npw = (int)hs.ks[k]->npw;
+        const int off = (int)hs.npw_off[k];
+        // The same split as ham_apply_dev for this k-point alone.
+        const int ktiles = (npw + kNlK - 1) / kNlK;
+        int nks = std::max(1, std::min(ktiles, (256 + ptiles * btiles - 1) / std::max(1, ptiles * btiles)));
+        const int kchunk = (int)round_up((npw + nks - 1) / nks, kNlK);
+        nks = (npw + kchunk - 1) / kchunk;
+        kseg.push_back((int)chunks.size() / 2);
+        for (int c = 0; c < nks; ++c) {
+            chunks.push_back(off + c * kchunk);
+            chunks.push_back(off + std::min(npw, (c + 1) * kchunk));
+        }
+        kseg.push_back((int)chunks.size() / 2);
+        for (int g = 0; g < npw; g += kNlG) {
+            tiles.push_back(off + g);
+            tiles.push_back(off + npw);
+            tiles.push_back((int)k);
+            tiles.push_back(0);
+        }
+    }
+    upload(hs.chunks, chunks);
+    upload(hs.kseg, kseg);
+    upload(hs.tiles, tiles);
+    hs.nks_total = (int)chunks.size() / 2;
+    hs.ntiles = (int)tiles.size() / 4;
+    hs.seg_cols = nc;
+    hs.seg_prec = (int)sizeof(T);
+}
+
+// x-line kernel of the set: the register variant member 0 runs for this precision (tuned on its
+// first wide apply), which must be usable on every member (same grid; their plane runs).
+template <typename T>
+static int hamset_xline_code(HamSet& hs, size_t nc, hipStream_t s) {
+    using C = typename C2T<T>::type;
+    constexpr bool fp32 = std::is_same<T, float>::value;
+    Ham& h0 = *hs.ks[0];
+    int code = h0.tune_fixed;
+    if (code == 0) {
+        code = h0.xline_split[fp32 ? 1 : 0];
+        if (code == 0) {
+            h0.tbuf.ensure(nc * h0.nplanes * (size_t)h0.ny * h0.nz * sizeof(C));
+            code = xline_variant<T>(h0, h0.tbuf.as<C>(), nc, s);
+        }
+    }
+    auto usable_all = [&](int c) {
+        if (c > 0) return false;
+        for (Ham* h : hs.ks)
+            if (!xline_code_usable(*h, c, !fp32)) return false;
+        return true;
+    };
+    if (!usable_all(code)) {
+        code = 0;
+        for (int c = kXlineReg; c >= kXlineRegLast && code == 0; --c)
+            if (usable_all(c)) code = c;
+    }
+    DZ_REQUIRE(code != 0, DZHIP_ERR_ARG, "no register x-line kernel runs on every k-point of the set");
+    return code;
+}
+
+// The apply's launch sequence for the whole set (no host synchronisation, no allocation: it is
+// what the graph captures).
+template <typename T>
+static void hamset_launch(HamSet& hs, const void* xv, void* yv, size_t nc, int code, hipStream_t s) {
+    using C = typename C2T<T>::type;
+    const C* x = reinterpret_cast<const C*>(xv);
+    C* y = reinterpret_cast<C*>(yv);
+    Ham& h0 = *hs.ks[0];
+    const size_t ld = hs.npw_total;
+    const int nk = (int)hs.ks.size();
+    const bool do_nl = hs.nproj > 0;
+    const C* tw = Sel<T>::tw(h0);
+    C* tb = hs.tbuf.as<C>();
+    const C* a = std::is_same<T, float>::value ? (const C*)hs.a32.ptr : (const C*)hs.a64.ptr;
+    const T* kin = std::is_same<T, float>::value ? (const T*)hs.kinetic32.ptr : (const T*)hs.kinetic64.ptr;
+    if (do_nl) {
+        constexpr int kNlP = nl_p_tile<T>();
+        const int ptiles = (hs.nproj + kNlP - 1) / kNlP;
+        const int btiles = (int)((nc + kNlBands - 1) / kNlBands);
+        hipLaunchKernelGGL(k_nl_proj<T>, dim3(ptiles * hs.nks_total, 1, btiles), dim3(kNlThreads),
+                           nl_proj_lds_bytes<T>(), s, a, x, ld, hs.partial.as<C>(), hs.nproj, (int)ld, (int)nc, 0,
+                           ptiles, hs.chunks.as<int2>(), (int)ld);
+        DZ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_nl_dij<T>, dim3((unsigned)nc, (unsigned)nk), dim3(kDijThreads),
+                           dij_lds_bytes(hs.nproj, sizeof(C)), s, hs.partial.as<C>(), hs.nks_total, hs.qbuf.as<C>(),
+                           h0.meta.as<ProjMeta>(), Sel<T>::dm(h0), hs.nproj, (int)nc, (T)h0.inv_volume,
+                           dij_split(hs.nproj, sizeof(C)), hs.kseg.as<int2>());
+        DZ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_nl_back<T>, dim3((unsigned)hs.ntiles, 1, (unsigned)((nc + kNlBands - 1) / kNlBands)),
+                           dim3(kNlThreads), nl_back_lds_bytes<T>(), s, a, hs.qbuf.as<C>(), y, ld, hs.nproj, (int)ld,
+                           (int)nc, hs.tiles.as<int4>(), (int)ld);
+        DZ_HIP(hipGetLastError());
+    }
+    const size_t lds_p = plane_lds_bytes<C>(hs.ny, hs.nz);
+    constexpr int nt = TileThreads<T>::value;
+    hipLaunchKernelGGL(k_plane_inv<T>, dim3(hs.np_total, (unsigned)nc), dim3(nt), lds_p, s, x, ld, tb,
+                       hs.plane_start.as<int>(), hs.plane_pw.as<int>(), hs.plane_pos.as<int>(),
+                       hs.plane_ystart.as<int>(), hs.plane_ycols.as<short>(), hs.ny, hs.nz, hs.np_total, h0.tab.ax[1],
+                       h0.tab.ax[2], tw);
+    DZ_HIP(hipGetLastError());
+    launch_xline_reg<T>(h0, tb, nc, xline_reg_threads(code), xline_reg_kind(code), s, hs.kpl.as<int2>(), nk,
+                        hs.np_total, hs.max_planes);
+    hipLaunchKernelGGL(k_plane_fwd<T>, dim3(hs.np_total, (unsigned)nc), dim3(nt), lds_p, s, tb, x, ld, y, ld, kin,
+                       hs.plane_start.as<int>(), hs.plane_pw.as<int>(), hs.plane_pos.as<int>(),
+                       hs.plane_ystart.as<int>(), hs.plane_ycols.as<short>(), hs.ny, hs.nz, hs.np_total, h0.tab.ax[1],
+                       h0.tab.ax[2], tw, do_nl ? 1 : 0);
+    DZ_HIP(hipGetLastError());
+}
+
+template <typename T>
+void hamset_apply_dev(HamSet& hs, const void* x, void* y, size_t nc, bool use_graph) {
+    using C = typename C2T<T>::type;
+    if (nc == 0) return;
+    DZ_REQUIRE(nc <= kMaxColsPerPass, DZHIP_ERR_ARG, "batched apply: at most 256 columns");
+    hipStream_t s = hs.stream();
+    Ham& h0 = *hs.ks[0];
+    // Everything that allocates, synchronises or sets kernel attributes happens here, before
+    // the launch sequence (which a graph may capture).
+    const int code = hamset_xline_code<T>(hs, nc, s);
+    hamset_segments<T>(hs, nc);
+    hs.tbuf.ensure(nc * (size_t)hs.np_total * hs.ny * hs.nz * sizeof(C));
+    if (hs.nproj > 0) {
+        hs.partial.ensure((size_t)hs.nks_total * nc * hs.nproj * sizeof(C));
+        hs.qbuf.ensure(hs.ks.size() * nc * hs.nproj * sizeof(C));
+        set_max_dynamic_lds((const void*)k_nl_proj<T>, nl_proj_lds_bytes<T>());
+        set_max_dynamic_lds((const void*)k_nl_dij<T>, dij_lds_bytes(hs.nproj, sizeof(C)));
+        set_max_dynamic_lds((const void*)k_nl_back<T>, nl_back_lds_bytes<T>());
+    }
+    set_max_dynamic_lds((const void*)k_plane_inv<T>, plane_lds_bytes<C>(hs.ny, hs.nz));
+    set_max_dynamic_lds((const void*)k_plane_fwd<T>, plane_lds_bytes<C>(hs.ny, hs.nz));
+    (void)h0;
+    if (!use_graph) {
+        hamset_launch<T>(hs, x, y, nc, code, s);
+        return;
+    }
+    const HamSet::GraphKey key{x, y, nc, (int)sizeof(T), code, s, hs.tbuf.ptr, hs.partial.ptr, hs.qbuf.ptr};
+    if (!(hs.graph_exec && hs.graph_key == key)) {
+        if (hs.graph_exec) {
+            DZ_HIP(hipGraphExecDestroy(hs.graph_exec));
+            hs.graph_exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        DZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            hamset_launch<T>(hs, x, y, nc, code, s);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        DZ_HIP(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&hs.graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        DZ_HIP(e);
+        hs.graph_key = key;
+        hs.graph_builds++;
+    }
+    DZ_HIP(hipGraphLaunch(hs.graph_exec, s));
+}
+
+void hamset_release(HamSet& hs) {
+    if (hs.graph_exec) {
+        (void)hipGraphExecDestroy(hs.graph_exec);
+        hs.graph_exec = nullptr;
+    }
+    DevBuf* bufs[] = {&hs.plane_start, &hs.plane_pw, &hs.plane_pos, &hs.plane_ystart, &hs.plane_ycols, &hs.kpl,
+                      &hs.kinetic64, &hs.kinetic32, &hs.a64, &hs.a32, &hs.chunks, &hs.kseg, &hs.tiles, &hs.tbuf,
+                      &hs.partial, &hs.qbuf};
+    for (DevBuf* b : bufs) b->release();
+}
+
+template void hamset_apply_dev<double>(HamSet&, const void*, void*, size_t, bool);
+template void hamset_apply_dev<float>(HamSet&, const void*, void*, size_t, bool);
 
 }  // namespace dzhip

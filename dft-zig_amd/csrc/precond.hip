@@ -34,18 +34,9 @@ __device__ __forceinline__ float dsilu(float u) {
     const float sg = 1.0f / (1.0f + __expf(-u));
     return sg * (1.0f + u * (1.0f - sg));
 }
-__device__ __forceinline__ double tpa(double x) {
-    const double p = 27.0 + x * (18.0 + x * (12.0 + x * 8.0));
-    return p / (p + 16.0 * x * x * x * x);
-}
 // Base factor B(G, b) for diag = |k+G|^2: kinetic shift (base DZHIP_PRECOND_KINETIC) or TPA.
 __device__ __forceinline__ double base_factor(int base, double diag, double lam, double ekin) {
-    if (base == DZHIP_PRECOND_KINETIC) {
-        double den = diag - lam;
-        if (fabs(den) < 1e-8) den = den >= 0.0 ? 1e-8 : -1e-8;
-        return 1.0 / den;
-    }
-    return tpa(diag / ekin);
+    return base == DZHIP_PRECOND_KINETIC ? precond_kinetic_shift(diag, lam) : precond_tpa(diag, ekin);
 }
 
 constexpr int kInferThreads = 256;

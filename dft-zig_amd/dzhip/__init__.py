@@ -39,7 +39,9 @@ EXPORTS = (
     "dzhip_ham_set_local_grid", "dzhip_scf_grid_add_solution_density", "dzhip_scf_grid_hxc",
     "dzhip_scf_grid_mix", "dzhip_scf_grid_read", "dzhip_ham_set_tuning", "dzhip_scf_grid_set_dielectric",
     "dzhip_scf_grid_set_xc", "dzhip_ham_set_overlap",
-    "dzhip_scf_grid_init_rho",
+    "dzhip_scf_grid_init_rho", "dzhip_heev", "dzhip_heev_dev", "dzhip_heev_check",
+    "dzhip_hamset_create", "dzhip_hamset_destroy", "dzhip_hamset_offsets", "dzhip_hamset_apply_dev",
+    "dzhip_hamset_apply", "dzhip_hamset_set_stream", "dzhip_hamset_info",
 )
 
 
@@ -169,6 +171,15 @@ def _load():
         "dzhip_scf_grid_set_xc": ([vp, i32], i32),
         "dzhip_ham_set_overlap": ([vp, i32], i32),
         "dzhip_scf_grid_init_rho": ([vp, dp, u64], i32),
+        "dzhip_heev": ([vp, dp, u64, u64, u64, dp, dp, u64], i32),
+        "dzhip_heev_dev": ([vp, vp, u64, u64, u64, vp, vp, u64, vp], i32),
+        "dzhip_heev_check": ([vp, vp], i32),
+        "dzhip_hamset_create": ([ct.POINTER(vp), u64], vp), "dzhip_hamset_destroy": ([vp], None),
+        "dzhip_hamset_offsets": ([vp, ct.POINTER(ct.c_uint64), u64], i32),
+        "dzhip_hamset_apply_dev": ([vp, vp, vp, u64, i32, i32], i32),
+        "dzhip_hamset_apply": ([vp, ct.POINTER(vp), ct.POINTER(vp), u64, i32], i32),
+        "dzhip_hamset_set_stream": ([vp, vp], i32),
+        "dzhip_hamset_info": ([vp, ct.POINTER(ct.c_uint64), u64], i32),
     }
     missing = set(EXPORTS) - set(sig)
     if missing:
@@ -241,6 +252,27 @@ class Context:
         hist = np.zeros(max(1, epochs))
         check(L().dzhip_mlp_train(self.handle, _fp(smp), smp.shape[1], _fp(p), epochs, batch, lr, seed, _dp(hist)))
         return p, hist[:epochs]
+
+    def heev(self, a, nev=None):
+        """Lowest nev eigenpairs of the Hermitian matrix a (full storage) on the device:
+        (values ascending, vectors as columns of an m x nev array)."""
+        a = np.asarray(a, dtype=np.complex128)
+        m = a.shape[0]
+        if a.shape != (m, m):
+            raise InvalidMatrixSize("heev needs a square matrix")
+        nev = m if nev is None else int(nev)
+        at = np.ascontiguousarray(a.T)  # column-major a
+        w = np.zeros(nev)
+        zt = np.zeros((nev, m), dtype=np.complex128)  # column-major m x nev
+        check(L().dzhip_heev(self.handle, _dp(at.view(np.float64)), m, m, nev, _dp(w), _dp(zt.view(np.float64)), m))
+        return w, zt.T
+
+    def heev_dev(self, a_ptr, lda, m, nev, w_ptr, z_ptr, ldz, stream=None):
+        """Device-pointer form (enqueue only); heev_check() synchronises and reports a failure."""
+        check(L().dzhip_heev_dev(self.handle, a_ptr, lda, m, nev, w_ptr, z_ptr, ldz, stream))
+
+    def heev_check(self, stream=None):
+        check(L().dzhip_heev_check(self.handle, stream))
 
     def close(self):
         if self.handle:
@@ -506,11 +538,11 @@ class Hamiltonian:
         check(L().dzhip_ham_set_overlap(self.handle, 1 if on else 0))
 
     def info(self):
-        out = (ct.c_uint64 * 11)()
-        check(L().dzhip_ham_info(self.handle, out, 11))
+        out = (ct.c_uint64 * 13)()
+        check(L().dzhip_ham_info(self.handle, out, 13))
         keys = ("n_pw", "grid_points", "planes", "projectors", "device_bytes", "xline_split_fp32",
                 "xline_split_fp64", "xline_register_fp32", "xline_register_fp64", "xline_lanes_fp32",
-                "xline_lanes_fp64")
+                "xline_lanes_fp64", "xline_kind_fp32", "xline_kind_fp64")
         return dict(zip(keys, [int(v) for v in out]))
 
     def close(self):
@@ -526,6 +558,66 @@ class Hamiltonian:
 
 
 GRID_RHO, GRID_VIN, GRID_VOUT, GRID_LOCAL, GRID_VION = range(5)
+
+
+class HamSet:
+    """Several k-point Hamiltonians of one grid applied in one launch sequence (dzhip_hamset_*).
+    Device layout: band-major over the concatenated spheres, (k, b, G) at b * total + offsets[k] + G."""
+
+    def __init__(self, hams):
+        self.hams = list(hams)
+        arr = (ct.c_void_p * len(self.hams))(*[h.handle for h in self.hams])
+        self.handle = L().dzhip_hamset_create(arr, len(self.hams))
+        if not self.handle:
+            raise _ERRS.get(-5, DzhipError)(f"dzhip_hamset_create failed: {last_error()}")
+        off = (ct.c_uint64 * (len(self.hams) + 1))()
+        check(L().dzhip_hamset_offsets(self.handle, off, len(self.hams) + 1))
+        self.offsets = [int(o) for o in off]
+        self.total = self.offsets[-1]
+
+    def apply(self, blocks, precision: int = FP64):
+        """Per-k host blocks (ncols, n_pw(k)) -> per-k H blocks, all k-points in one launch sequence."""
+        xs = [_as_c128(b) for b in blocks]
+        ncols = xs[0].shape[0]
+        ys = [np.empty_like(x) for x in xs]
+        xa = (ct.c_void_p * len(xs))(*[x.ctypes.data for x in xs])
+        ya = (ct.c_void_p * len(ys))(*[y.ctypes.data for y in ys])
+        check(L().dzhip_hamset_apply(self.handle, xa, ya, ncols, precision))
+        return ys
+
+    def apply_dev(self, x_ptr: int, y_ptr: int, ncols: int, precision: int = FP64, use_graph: bool = False):
+        check(L().dzhip_hamset_apply_dev(self.handle, ct.c_void_p(x_ptr), ct.c_void_p(y_ptr), ncols, precision,
+                                         1 if use_graph else 0))
+
+    def set_stream(self, stream_handle):
+        check(L().dzhip_hamset_set_stream(self.handle, ct.c_void_p(stream_handle or 0)))
+
+    def info(self):
+        out = (ct.c_uint64 * 4)()
+        check(L().dzhip_hamset_info(self.handle, out, 4))
+        return dict(zip(("k_points", "n_pw_total", "planes_total", "graphs"), [int(v) for v in out]))
+
+    def pack(self, blocks):
+        """Per-k (ncols, n_pw(k)) blocks -> the set's band-major (ncols, total) layout."""
+        ncols = blocks[0].shape[0]
+        out = np.zeros((ncols, self.total), dtype=np.result_type(*[b.dtype for b in blocks]))
+        for k, b in enumerate(blocks):
+            out[:, self.offsets[k]:self.offsets[k + 1]] = b
+        return out
+
+    def unpack(self, arr):
+        return [arr[:, self.offsets[k]:self.offsets[k + 1]] for k in range(len(self.hams))]
+
+    def close(self):
+        if self.handle:
+            L().dzhip_hamset_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ScfGrid:

@@ -16,7 +16,7 @@ PKG_DIR = Path(__file__).resolve().parent
 CSRC = PKG_DIR.parent / "csrc"
 OBJ_DIR = PKG_DIR.parent / "_build"
 LIB_PATH = PKG_DIR / "libdzhip.so"
-SOURCES = ["api.hip", "fft.hip", "hpsi.hip", "eig.hip", "precond.hip", "potential.hip"]
+SOURCES = ["api.hip", "fft.hip", "hpsi.hip", "eig.hip", "precond.hip", "potential.hip", "heev.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

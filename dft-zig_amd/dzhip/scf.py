@@ -58,6 +58,15 @@ class ScfSystem:
     smearing: float = 0.0  # Fermi-Dirac width (Ry); 0 = fixed occupations
     atom_rho_width: float = 1.2  # bohr: Gaussian valence charge per atom of the atomic start density
     xc: str = "lda_pz"  # "lda_pz" or "pbe" (PBE runs on the device grid)
+    # Per-species GTH local parameters (Z, rloc, C1) for structures of several species; None = the
+    # single set (z_ion, rloc, c1) for every atom.
+    species_local: list | None = None
+
+    def local_params(self):
+        """(Z, rloc, C1) of every species."""
+        if self.species_local is not None:
+            return list(self.species_local)
+        return [(self.z_ion, self.rloc, self.c1)] * len(self.species)
 
 
 # GTH-form parameters of a four-electron Si-like pseudo-atom (Hartree units for C1 and h): the
@@ -126,15 +135,91 @@ def mp_mesh_time_reversal(recip, mesh):
     return pts, [w / tot for w in wts]
 
 
+def lattice_symmetry_ops(cell, atom_frac, atom_species, tol=1e-6):
+    """Integer matrices W (fractional coordinates, rows x -> x W + t) of the space-group operations
+    of the structure: lattice automorphisms (W g W^T = g, g = cell cell^T, entries in {-1, 0, 1})
+    that map the atoms onto atoms of the same species with some translation t (the role of
+    symmetry.get_symmetry_ops, src/features/symmetry/symmetry.zig)."""
+    import itertools
+
+    cell = np.asarray(cell, dtype=np.float64)
+    g = cell @ cell.T
+    frac = np.asarray(atom_frac, dtype=np.float64) % 1.0
+    spc = np.asarray(atom_species)
+    ops = []
+    for entries in itertools.product((-1, 0, 1), repeat=9):
+        w = np.array(entries, dtype=np.int64).reshape(3, 3)
+        if abs(round(np.linalg.det(w))) != 1 or not np.allclose(w @ g @ w.T, g, atol=tol * np.max(np.abs(g))):
+            continue
+        mapped = frac @ w
+        found = False
+        for b in np.nonzero(spc == spc[0])[0]:
+            t = frac[b] - mapped[0]
+            img = (mapped + t) % 1.0
+            ok = True
+            for a in range(frac.shape[0]):
+                d = (img[a] - frac) % 1.0
+                d = np.minimum(d, 1.0 - d)
+                hit = np.nonzero((np.max(d, axis=1) < tol) & (spc == spc[a]))[0]
+                if hit.size == 0:
+                    ok = False
+                    break
+            if ok:
+                found = True
+                break
+        if found:
+            ops.append(w)
+    return ops
+
+
+def ibz_kmesh(cell, mesh, atom_frac, atom_species, time_reversal=True):
+    """Unshifted Monkhorst-Pack mesh reduced to its irreducible part, as reduction.reduce_kmesh
+    (src/features/kpoints/reduction.zig:55-120): mesh points in flat order (x index fastest), each
+    not yet covered starts an orbit under the k-space images of the symmetry operations (k W^T on
+    the integer indices, modulo the mesh) and time reversal; weight = orbit size / mesh size; the
+    representative is the seed, k_frac = i / n wrapped to [-1/2, 1/2) (mesh.zig:39-47).
+    Returns (Cartesian points, weights, fractional points)."""
+    recip = tables.reciprocal(np.asarray(cell, dtype=np.float64))
+    n = np.array(mesh, dtype=np.int64)
+    ops = lattice_symmetry_ops(cell, atom_frac, atom_species)
+    total = int(np.prod(n))
+    used = np.zeros(total, dtype=bool)
+
+    def flat(idx):
+        idx = idx % n
+        return int(idx[0] + n[0] * (idx[1] + n[1] * idx[2]))
+
+    pts, wts, fr = [], [], []
+    for f in range(total):
+        if used[f]:
+            continue
+        seed = np.array([f % n[0], (f // n[0]) % n[1], f // (n[0] * n[1])], dtype=np.int64)
+        group = {f}
+        for w in ops:
+            img = seed @ w.T
+            group.add(flat(img))
+            if time_reversal:
+                group.add(flat(-img))
+        for q in group:
+            used[q] = True
+        kf = seed / n
+        kf = kf - np.floor(kf + 0.5)
+        pts.append(kf @ recip)
+        wts.append(len(group) / total)
+        fr.append(kf)
+    return pts, wts, fr
+
+
 AL_GTH = {"z": 3.0, "rloc": 0.45, "c1": -8.491351, "r_s": 0.460104, "h_s": 5.088340, "r_p": 0.536744,
           "h_p": 2.193438}
 
 
 def aluminium(ecut_ry=30.0, kmesh=(4, 4, 4), nbands=12, smearing_ry=0.01, grid=None, displace=None,
-              xc="lda_pz") -> ScfSystem:
+              xc="lda_pz", symmetry=False) -> ScfSystem:
     """fcc Al in the 4-atom cubic cell (a = 4.05 A), GTH-form local part and s/p Gaussian
     projectors (Goedecker-Teter-Hutter Al LDA parameters), Fermi-Dirac smearing; k-points from an
-    unshifted mesh reduced by time reversal (4x4x4 -> 36 points with uneven plane-wave counts).
+    unshifted mesh reduced by time reversal (4x4x4 -> 36 points with uneven plane-wave counts), or
+    with symmetry=True by the structure's space group and time reversal (8x8x8 -> 35 points).
 
     `displace` (bohr, 3-vector) moves the second atom off its site.  In the perfect crystal,
     partially occupied levels are symmetry-degenerate, and the density then depends at the 1e-9
@@ -149,7 +234,10 @@ def aluminium(ecut_ry=30.0, kmesh=(4, 4, 4), nbands=12, smearing_ry=0.01, grid=N
         pos[1] += np.asarray(displace, dtype=np.float64)
     g = AL_GTH
     sp = SpeciesDesc([0, 1], [g["r_s"], g["r_p"]], np.diag([2.0 * g["h_s"], 2.0 * g["h_p"]]))
-    kpts, kw = mp_mesh_time_reversal(recip, kmesh)
+    if symmetry:
+        kpts, kw, _ = ibz_kmesh(cell, kmesh, pos @ np.linalg.inv(cell), np.zeros(4, np.int32))
+    else:
+        kpts, kw = mp_mesh_time_reversal(recip, kmesh)
     grid = tuple(grid) if grid is not None else tuple(tables.auto_grid(ecut_ry, recip))
     return ScfSystem("al4", cell, recip, abs(float(np.linalg.det(cell))), grid, ecut_ry, pos, np.zeros(4, np.int32),
                      [sp], g["z"], g["rloc"], g["c1"], 12.0, nbands, kpts, kw, smearing_ry, xc=xc)
@@ -178,6 +266,58 @@ def si64(ecut_ry=30.0, nbands=160):
     """2x2x2 supercell of the cubic cell, atoms displaced by N(0, 0.02 A), unshifted 2x2x2 mesh
     (8 k-points: the displacements leave only the identity), 160 bands."""
     return silicon((2, 2, 2), ecut_ry, (2, 2, 2), nbands, displacement_angstrom=0.02, seed=64)
+
+
+# GTH-form parameters of Ga (3 valence electrons) and As (5) for the zincblende cell (Hartree units
+# for C1 and h): local radius of the Hartwigsen-Goedecker-Hutter LDA sets with the leading s and p
+# projector of each (radius and strength); the deeper projectors and the d channel of those sets
+# are left out, which keeps the structure a gapped semiconductor with 4 bonds per atom.
+GA_GTH = {"z": 3.0, "rloc": 0.56, "c1": 0.0, "r_s": 0.610791, "h_s": 2.369325, "r_p": 0.704596, "h_p": 0.746305}
+AS_GTH = {"z": 5.0, "rloc": 0.52, "c1": 0.0, "r_s": 0.456400, "h_s": 4.560761, "r_p": 0.550562, "h_p": 1.812247}
+GAAS_A_ANGSTROM = 5.653
+
+
+def gaas(cells=(2, 2, 2), ecut_ry=35.0, kpoints=None, nbands=160, displacement_angstrom=0.05, seed=5,
+         grid=None) -> ScfSystem:
+    """Zincblende GaAs in n1 x n2 x n3 conventional cubic cells (Ga on the fcc sites, As at
+    (1/4, 1/4, 1/4) + fcc; a = 5.653 A, so the 2x2x2 supercell has a = 11.306 A and 64 atoms), atoms
+    displaced by N(0, displacement) (numpy PCG64(seed)), Gamma only unless k-points are given."""
+    a = GAAS_A_ANGSTROM * tables.ANGSTROM_TO_BOHR
+    n1, n2, n3 = cells
+    cell = np.diag([a * n1, a * n2, a * n3])
+    recip = tables.reciprocal(cell)
+    fcc = DIAMOND_FRAC[:4]
+    pos, spc = [], []
+    for i in range(n1):
+        for j in range(n2):
+            for k in range(n3):
+                for f in fcc:
+                    pos.append((f + np.array([i, j, k])) * a)
+                    spc.append(0)
+                for f in fcc:
+                    pos.append((f + 0.25 + np.array([i, j, k])) * a)
+                    spc.append(1)
+    pos = np.array(pos)
+    if displacement_angstrom > 0:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        pos = pos + rng.normal(0.0, displacement_angstrom * tables.ANGSTROM_TO_BOHR, size=pos.shape)
+    species, local = [], []
+    for g in (GA_GTH, AS_GTH):
+        species.append(SpeciesDesc([0, 1], [g["r_s"], g["r_p"]], np.diag([2.0 * g["h_s"], 2.0 * g["h_p"]])))
+        local.append((g["z"], g["rloc"], g["c1"]))
+    spc = np.array(spc, dtype=np.int32)
+    nelec = float(sum(local[s][0] for s in spc))
+    kpts, kw = kpoints if kpoints is not None else ([np.zeros(3)], [1.0])
+    grid = tuple(grid) if grid is not None else tuple(tables.auto_grid(ecut_ry, recip))
+    return ScfSystem(f"gaas{pos.shape[0]}", cell, recip, abs(float(np.linalg.det(cell))), grid, ecut_ry, pos, spc,
+                     species, 0.0, 0.0, 0.0, nelec, nbands, [np.asarray(k, dtype=np.float64) for k in kpts], list(kw),
+                     species_local=local)
+
+
+def gaas64(ecut_ry=35.0, nbands=160):
+    """The 64-atom GaAs supercell (2x2x2 cubic cells, a = 11.306 A, displacements N(0, 0.05 A)),
+    35 Ry (90^3 grid), Gamma point, 160 bands."""
+    return gaas((2, 2, 2), ecut_ry, nbands=nbands)
 
 
 def start_potential(system, potential="atomic"):
@@ -272,28 +412,32 @@ def hartree(rho, grid, g2, volume):
 
 
 def ionic_potential(system, g, g2):
-    """V_ion(r) = sum_G e^{iG.r} (1/Omega) sum_a e^{-iG.tau_a} v(|G|), v of the GTH local form."""
+    """V_ion(r) = sum_G e^{iG.r} (1/Omega) sum_s v_s(|G|) sum_{a in s} e^{-iG.tau_a}, v_s of the GTH
+    local form of species s."""
     s = system
-    sf = np.zeros(g.shape[0], dtype=np.complex128)
-    for tau in s.atom_pos:
-        sf += np.exp(-1j * (g @ tau))
-    vq = np.zeros(g.shape[0])
+    total = np.zeros(g.shape[0], dtype=np.complex128)
     nz = g2 > 1e-20
     q2 = g2[nz]
-    vq[nz] = 2.0 * (-4.0 * math.pi * s.z_ion / q2 + s.c1 * (2.0 * math.pi) ** 1.5 * s.rloc ** 3) \
-        * np.exp(-0.5 * q2 * s.rloc ** 2)
-    return to_r(sf * vq * (g.shape[0] / s.volume), s.grid).real
+    for sp, (z, rloc, c1) in enumerate(s.local_params()):
+        sf = np.zeros(g.shape[0], dtype=np.complex128)
+        for tau in s.atom_pos[np.asarray(s.atom_species) == sp]:
+            sf += np.exp(-1j * (g @ tau))
+        vq = np.zeros(g.shape[0])
+        vq[nz] = 2.0 * (-4.0 * math.pi * z / q2 + c1 * (2.0 * math.pi) ** 1.5 * rloc ** 3) * np.exp(-0.5 * q2 * rloc ** 2)
+        total += sf * vq
+    return to_r(total * (g.shape[0] / s.volume), s.grid).real
 
 
 def atomic_density(system, g, g2):
     """Superposition of atomic valence charges (core_density.zig:128-178): rho(G) =
-    (1/Omega) sum_a Z e^{-G^2 a^2 / 2} e^{-iG.tau_a}, a Gaussian of width a per atom (the
+    (1/Omega) sum_a Z_a e^{-G^2 a^2 / 2} e^{-iG.tau_a}, a Gaussian of width a per atom (the
     synthetic species carry no PP_RHOATOM), so int rho = nelec."""
     s = system
+    zs = [p[0] for p in s.local_params()]
     sf = np.zeros(g.shape[0], dtype=np.complex128)
-    for tau in s.atom_pos:
-        sf += np.exp(-1j * (g @ tau))
-    rho_g = s.z_ion * np.exp(-0.5 * g2 * s.atom_rho_width ** 2) * sf / s.volume
+    for tau, sp in zip(s.atom_pos, s.atom_species):
+        sf += zs[int(sp)] * np.exp(-1j * (g @ tau))
+    rho_g = np.exp(-0.5 * g2 * s.atom_rho_width ** 2) * sf / s.volume
     return to_r(rho_g * g.shape[0], s.grid).real
 
 

@@ -67,6 +67,24 @@ void dzhip_fft_plan_destroy(dzhip_fft_plan* plan);
 int dzhip_fft_forward(dzhip_fft_plan* plan, double* data, uint64_t count);
 int dzhip_fft_inverse(dzhip_fft_plan* plan, double* data, uint64_t count);
 
+/* ------------------------------------------------------------------ dense Hermitian eigensolver
+ * The lowest nev eigenpairs of a dense Hermitian m x m matrix: the subspace eigenproblem of the
+ * eigensolvers.  Replaces hermitian_eigen_decomp_small and linalg.hermitian_eigen_decomp
+ * (src/features/linalg/lobpcg/common.zig:440-537), which call LAPACK zheev there; the device
+ * eigensolvers call it internally for their Rayleigh-Ritz step.  a: complex interleaved,
+ * column-major, leading dimension lda >= m; only its lower triangle is read, as zheev with
+ * uplo = 'L' (not modified).  values[0:nev]
+ * ascending; vectors: nev columns of length m, leading dimension ldz >= m, orthonormal.
+ * 1 <= nev <= m <= 1024, else DZHIP_ERR_ARG.  Blocking (host memory). */
+int dzhip_heev(dzhip_ctx* ctx, const double* a, uint64_t lda, uint64_t m, uint64_t nev, double* values,
+               double* vectors, uint64_t ldz);
+/* The same on device memory (a, values, vectors device pointers), enqueued on `stream` (NULL = the
+ * context's stream); returns once enqueued.  dzhip_heev_check(ctx, stream) synchronises the stream
+ * and reports DZHIP_ERR_HIP if the last call's tridiagonalisation did not complete. */
+int dzhip_heev_dev(dzhip_ctx* ctx, const void* a, uint64_t lda, uint64_t m, uint64_t nev, void* values,
+                   void* vectors, uint64_t ldz, void* stream);
+int dzhip_heev_check(dzhip_ctx* ctx, void* stream);
+
 /* ------------------------------------------------------------------ H·psi
  * Per-k Hamiltonian context: the data ApplyContext + KpointApplyCache carry
  * (src/features/scf/apply.zig:45-574, nonlocal_context.zig:9-246).
@@ -120,6 +138,33 @@ int dzhip_ham_apply(dzhip_ham* ham, const double* x, double* y, uint64_t n, uint
  * context's stream (see dzhip_ham_set_stream); returns without waiting. */
 int dzhip_ham_apply_dev(dzhip_ham* ham, const void* x_dev, void* y_dev, uint64_t n, uint64_t ncols,
                         int precision);
+
+/* ------------------------------------------------------------------ batched multi-k apply
+ * All bands of several k-points in one launch sequence (and, with use_graph, one captured
+ * hipGraph launch): replaces the per-k-point loop of apply_hamiltonian_batched
+ * (src/features/scf/apply.zig:1584-1618) over the k-point work queue
+ * (src/features/scf/kpoint/workers.zig:133-180).  The members are existing contexts of one grid
+ * and projector set (one per k-point, uneven n_pw allowed) with a register x-line kernel that runs
+ * on all of them; they must outlive the set and hold the same local potential (the set applies
+ * member 0's).  Each k-point's result equals its own dzhip_ham_apply_dev bit for bit when the
+ * members use the same x-line variant (dzhip_ham_set_tuning). */
+typedef struct dzhip_hamset dzhip_hamset;
+dzhip_hamset* dzhip_hamset_create(dzhip_ham* const* hams, uint64_t count);
+void dzhip_hamset_destroy(dzhip_hamset* set);
+/* Offsets of the k-points in the concatenated layout: off[k] = sum_{k' < k} n_pw(k'), k <= count
+ * (off[count] = the total). */
+int dzhip_hamset_offsets(dzhip_hamset* set, uint64_t* off, uint64_t n_off);
+/* Device apply.  x_dev, y_dev: band-major over the concatenated spheres, element (k, b, G) at
+ * b * total + off[k] + G (double2 or float2 by precision); ncols <= 256.  Enqueued on the set's
+ * stream (dzhip_hamset_set_stream; default the context of member 0).  use_graph != 0 captures the
+ * launch sequence once per (x, y, ncols, precision, stream) and replays it. */
+int dzhip_hamset_apply_dev(dzhip_hamset* set, const void* x_dev, void* y_dev, uint64_t ncols, int precision,
+                           int use_graph);
+/* Host blocks per k-point, column-major [n_pw(k) x ncols] as the reference's apply_batch; blocking. */
+int dzhip_hamset_apply(dzhip_hamset* set, const double* const* x, double* const* y, uint64_t ncols, int precision);
+int dzhip_hamset_set_stream(dzhip_hamset* set, void* stream);
+/* out[0] = k-points, [1] = total n_pw, [2] = total x-planes, [3] = graphs captured so far. */
+int dzhip_hamset_info(dzhip_hamset* set, uint64_t* out, uint64_t n_out);
 
 /* Nonlocal part only, y = V_nl x (apply_nonlocal_potential, apply.zig:793-808 and
  * apply_nonlocal_batched, apply.zig:1449-1482); host memory, blocking. */
@@ -339,9 +384,11 @@ int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t c
  * threads per x-line; even nx <= 100 whose half is 2^a 3^b 5^c, both precisions, when the
  * G-sphere spans at most half the line), -7 / -8 / -9 the lane-triple register kernel (three
  * threads per x-line; nx in 12..96 divisible by 3 whose third is 2^a 3^b 5^c, e.g. 75 = 3 x 25,
- * both precisions, any sphere width); a precision where the chosen kernel cannot run keeps
- * one slab.  -1..-3 need the one-lane kernel in FP32, -4..-6 the lane-pair kernel, -7..-9 the
- * lane-triple kernel.  0 (default) times every divisor of ny with at least 8 lines per slab, and
+ * both precisions, any sphere width), -10 / -11 / -12 the residue-sequential register kernel (one
+ * thread per x-line doing the three residues x mod 3 in turn; the same line lengths, both
+ * precisions, when the G-sphere spans at most two thirds of the line); a precision where the
+ * chosen kernel cannot run keeps one slab.  -1..-3 need the one-lane kernel in FP32, -4..-6 the
+ * lane-pair kernel, -7..-9 the lane-triple kernel, -10..-12 the residue-sequential kernel.  0 (default) times every divisor of ny with at least 8 lines per slab, and
  * the register kernels' variants where they can run, on the first apply of each precision with at
  * least 16 columns and keeps the fastest; the result is shared by every context of the same
  * device, grid and sphere shape in the process. */
@@ -360,7 +407,8 @@ int dzhip_ham_set_overlap(dzhip_ham* ham, int on);
  * [6] = for FP64 (0 = not tuned yet, or the register kernel), [7] = workgroup size when FP32
  * runs the register x-line kernel (0 = the LDS kernel or not tuned yet), [8] = the same for
  * FP64, [9] / [10] = threads per x-line of that register kernel, 1, 2 or 3 (FP32 / FP64; 0 = the
- * LDS kernel or not tuned yet). */
+ * LDS kernel or not tuned yet), [11] / [12] = its kind (FP32 / FP64): 1 one-lane, 2 lane-pair,
+ * 3 lane-triple, 4 residue-sequential (0 = the LDS kernel or not tuned yet). */
 int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out);
 
 #ifdef __cplusplus

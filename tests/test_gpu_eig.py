@@ -114,3 +114,23 @@ def test_eig_nconv_converges_lowest_bands(dz, ctx, si8_exact):
     assert st8["converged"] == 1
     np.testing.assert_allclose(v8[:8], exact[:8], atol=EV_TOL)
     assert st8["apply_calls"] <= st_all["apply_calls"]
+
+
+def test_density_solution_guard(dz, ctx):
+    """Regression: the density of the last solution reads that solution's columns on the device;
+    asking for it before any solve, or for more bands than the solve returned, is DZHIP_ERR_STATE
+    (BadState), not an access past the solution buffer."""
+    w = workloads.si8(nbands=4)
+    ham = dz.Hamiltonian.from_workload(ctx, w)
+    try:
+        with pytest.raises(dz.BadState):
+            ham.density_solution(np.ones(2))
+        ham.eig_solve(4, tol=1e-6, max_iter=60, init_diagonal=True, want_vectors=False)
+        with pytest.raises(dz.BadState):
+            ham.density_solution(np.ones(6))
+        rho = ham.density_solution(np.full(4, 0.5))
+        assert rho.shape == (int(np.prod(w.grid)),) and np.all(np.isfinite(rho))
+        # Integrated density = sum of weights (each solution vector has unit norm).
+        assert abs(rho.sum() * (1.0 / w.inv_volume) / rho.size - 2.0) < 1e-10
+    finally:
+        ham.close()

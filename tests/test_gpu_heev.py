@@ -1,0 +1,82 @@
+"""The device Hermitian eigensolver (dzhip_heev: Householder tridiagonalisation in one cooperative
+launch, multisection, inverse iteration, back-transformation) against numpy.linalg.eigh, which is
+LAPACK zheevd: the reference's subspace eigenproblems call LAPACK zheev
+(src/features/linalg/lobpcg/common.zig:440-537).
+
+Bars: eigenvalues within 1e-12 of the spectral radius; Z^H Z = I within 1e-12; residuals
+|A z - lambda z| within 1e-11 of the spectral radius.  Sizes 1, 2, 20, 164 and 648 (the block
+Davidson's first and largest subspaces at 160 bands), exact and near degeneracies, and a
+Rayleigh-Ritz matrix of the Si-8 Hamiltonian."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def random_hermitian(m, rng, spectrum=None):
+    q, _ = np.linalg.qr(rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m)))
+    lam = np.sort(rng.uniform(-2.0, 30.0, m)) if spectrum is None else np.asarray(spectrum, dtype=float)
+    a = (q * lam) @ q.conj().T
+    return 0.5 * (a + a.conj().T), lam
+
+
+def check(ctx, a, nev, tol_val=1e-12, tol_res=1e-11):
+    w, z = ctx.heev(a, nev)
+    ref = np.linalg.eigvalsh(a)[:nev]
+    scale = max(1.0, float(np.max(np.abs(ref))), float(np.linalg.norm(a, 2)))
+    dval = float(np.max(np.abs(w - ref)))
+    orth = float(np.max(np.abs(z.conj().T @ z - np.eye(nev))))
+    res = float(np.max(np.linalg.norm(a @ z - z * w, axis=0)))
+    print(f"m={a.shape[0]} nev={nev} |dw|={dval:.2e} orth={orth:.2e} res={res:.2e} scale={scale:.1f}")
+    assert dval <= tol_val * scale
+    assert orth <= 1e-12
+    assert res <= tol_res * scale
+    return w, z
+
+
+@pytest.mark.parametrize("m,nev", [(1, 1), (2, 2), (20, 20), (20, 7), (164, 160), (648, 160), (648, 648)])
+def test_heev_random(ctx, m, nev):
+    rng = np.random.default_rng(1000 + m + nev)
+    a, _ = random_hermitian(m, rng)
+    check(ctx, a, nev)
+
+
+def test_heev_degenerate_and_clustered(ctx):
+    """Exactly degenerate triplets and pairs (symmetry-degenerate bands), a cluster split by
+    1e-13, and a zero off-diagonal block (a reducible tridiagonal)."""
+    rng = np.random.default_rng(7)
+    lam = np.sort(np.concatenate([np.repeat(rng.uniform(-1, 1, 20), 3), np.repeat(rng.uniform(1, 5, 30), 2),
+                                  1.5 + 1e-13 * np.arange(6), rng.uniform(5, 20, 100)]))
+    a, _ = random_hermitian(lam.size, rng, lam)
+    check(ctx, a, 160)
+    # Block diagonal: the reduction meets zero columns (tau = 0 reflectors).
+    b1, _ = random_hermitian(40, rng)
+    b = np.zeros((80, 80), dtype=complex)
+    b[:40, :40] = b1
+    b[40:, 40:] = b1
+    check(ctx, b, 80)
+    # Real diagonal matrix: every reflector is the identity.
+    check(ctx, np.diag(np.linspace(-3, 3, 50)).astype(complex), 50)
+
+
+def test_heev_errors(dz, ctx):
+    with pytest.raises(dz.BadArgument):
+        ctx.heev(np.eye(4, dtype=complex), 5)
+    with pytest.raises(dz.BadArgument):
+        ctx.heev(np.eye(1025, dtype=complex), 4)
+
+
+def test_heev_rayleigh_ritz_matrix(dz, ctx):
+    """T = V^H H V of the Si-8 Hamiltonian over an orthonormalised random block (the subspace
+    matrix the Davidson solver diagonalises)."""
+    from dzhip import workloads
+    from oracle import adapters, hamiltonian
+
+    w = workloads.si8(nbands=8)
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal((w.npw, 200)) + 1j * rng.standard_normal((w.npw, 200))
+    v, _ = np.linalg.qr(v)
+    hv = hamiltonian.apply_hamiltonian_batched(adapters.context_from_workload(w), v.T).T
+    t = v.conj().T @ hv
+    t = 0.5 * (t + t.conj().T)
+    check(ctx, t, 160)

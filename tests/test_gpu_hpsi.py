@@ -178,6 +178,48 @@ def test_apply_dev_with_torch_buffers(dz, ctx):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
+def test_phase_timers(dz, ctx, prec):
+    """The ScfProfile-shaped phase timers (src/features/scf/logging.zig:114-181): a profiled apply
+    gives bit for bit the unprofiled result, every phase is timed on every call, and the phase
+    sum of the 160-band bench block is within 10 % of the block's own time (HIP events around the
+    unprofiled applies on the same stream)."""
+    import torch
+
+    w = workloads.bench(nbands=160)
+    ham = make_ham(dz, ctx, w)
+    dt, cdt = (np.float64, np.complex128) if prec == 0 else (np.float32, np.complex64)
+    x = torch.from_numpy(w.psi.astype(cdt).view(dt).copy()).cuda()
+    y0, y1 = torch.empty_like(x), torch.empty_like(x)
+    stream = torch.cuda.Stream()
+    ham.set_stream(stream.cuda_stream)
+    try:
+        for _ in range(3):
+            ham.apply_dev(x.data_ptr(), y0.data_ptr(), 160, prec)
+        steps = 10
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            ham.apply_dev(x.data_ptr(), y0.data_ptr(), 160, prec)
+        e1.record(stream)
+        stream.synchronize()
+        block_ms = e0.elapsed_time(e1) / steps
+        ham.profile(True)
+        for _ in range(steps):
+            ham.apply_dev(x.data_ptr(), y1.data_ptr(), 160, prec)
+        stream.synchronize()
+        phases, calls = ham.phase_times()
+        ham.profile(False)
+        assert torch.equal(y0, y1)
+        assert calls == steps and all(v > 0.0 for v in phases.values()), (calls, phases)
+        total = sum(phases.values()) / calls
+        print(prec, block_ms, total, phases)
+        assert abs(total - block_ms) <= 0.1 * block_ms
+    finally:
+        ham.set_stream(None)
+        ham.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
 def test_hpsi_every_xline_split(dz, ctx, prec):
     """Every y-slab count the x-line tuner may pick gives the golden H.psi (Si-8, ny = 32)."""
     w = workloads.si8()
@@ -207,14 +249,16 @@ def test_hpsi_every_xline_split(dz, ctx, prec):
 # Which register kernels run per workload and precision: one lane per line for nx <= 100 (FP32)
 # or <= 50 (FP64); the lane-pair kernel for even nx when the sphere spans at most half the line
 # (tiny: 5 of 8 x values, no; nx = 75: odd, no); the lane-triple kernel for nx divisible by 3
-# whose third is 2^a 3^b 5^c (72, 75, 90), any sphere width.
-REGISTER_KERNELS = {"tiny": {"one": (0, 1), "pair": (), "tri": ()},
-                    "si8": {"one": (0, 1), "pair": (0, 1), "tri": ()},
-                    "bench": {"one": (1,), "pair": (0, 1), "tri": (0, 1)},
-                    "bench75": {"one": (1,), "pair": (), "tri": (0, 1)},
-                    "bench80": {"one": (1,), "pair": (0, 1), "tri": ()},
-                    "bench90": {"one": (1,), "pair": (0, 1), "tri": (0, 1)},
-                    "bench100": {"one": (1,), "pair": (0, 1), "tri": ()}}
+# whose third is 2^a 3^b 5^c (72, 75, 90), any sphere width; the residue-sequential kernel for the
+# same lines when the sphere spans at most two thirds of the line (35 planes at 72, 75 and 90).
+REGISTER_KERNELS = {"tiny": {"one": (0, 1), "pair": (), "tri": (), "seq": ()},
+                    "si8": {"one": (0, 1), "pair": (0, 1), "tri": (), "seq": ()},
+                    "bench": {"one": (1,), "pair": (0, 1), "tri": (0, 1), "seq": (0, 1)},
+                    "bench75": {"one": (1,), "pair": (), "tri": (0, 1), "seq": (0, 1)},
+                    "bench80": {"one": (1,), "pair": (0, 1), "tri": (), "seq": ()},
+                    "bench90": {"one": (1,), "pair": (0, 1), "tri": (0, 1), "seq": (0, 1)},
+                    "bench100": {"one": (1,), "pair": (0, 1), "tri": (), "seq": ()}}
+KIND = {"one": 1, "pair": 2, "tri": 3, "seq": 4}
 
 
 @pytest.mark.parametrize("which,prec", [("tiny", 1), ("si8", 1), ("bench", 1), ("bench75", 1), ("bench90", 1),
@@ -238,7 +282,7 @@ def test_hpsi_register_xline_vs_oracle(dz, ctx, which, prec):
         nb = min(w.nbands, 8)
         ref = hamiltonian.apply_hamiltonian_batched(oracle_ctx(w), w.psi[:nb])
         ran = 0
-        for variant, kind, lanes in ((-1, "one", 1), (-4, "pair", 2), (-7, "tri", 3)):
+        for variant, kind, lanes in ((-1, "one", 1), (-4, "pair", 2), (-7, "tri", 3), (-10, "seq", 1)):
             runs = prec in REGISTER_KERNELS[which][kind]
             if not REGISTER_KERNELS[which][kind] or (kind == "one" and 1 not in REGISTER_KERNELS[which][kind]):
                 with pytest.raises(dz.BadArgument):
@@ -249,6 +293,7 @@ def test_hpsi_register_xline_vs_oracle(dz, ctx, which, prec):
             info = ham.info()
             assert info[f"xline_register_{key}"] == (256 if runs else 0)
             assert info[f"xline_lanes_{key}"] == (lanes if runs else 0)
+            assert info[f"xline_kind_{key}"] == (KIND[kind] if runs else 0)
             err = relerr(got, ref)
             print(which, prec, w.grid, variant, runs, err)
             assert err < (TOL32 if prec else TOL64)
@@ -279,6 +324,37 @@ def test_hpsi_tri_xline_workgroup_sizes(dz, ctx, prec):
             assert err < (TOL32 if prec else TOL64)
     finally:
         ham.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hpsi_seq3_xline_workgroup_sizes_and_wide_sphere(dz, ctx, prec):
+    """The residue-sequential x-line kernel (one thread per line, the three residues x mod 3 one
+    after another, rows k + 25 accumulated in LDS) in 256-, 128- and 64-thread workgroups on the
+    75^3 grid, 12 bands; and a sphere wider than two thirds of the line (nx = 48: the 15 Ha
+    sphere spans 35 > 32 x values) where it is refused."""
+    w = workloads.bench(nbands=12, grid=(75, 75, 75))
+    ham = make_ham(dz, ctx, w)
+    key = "fp32" if prec else "fp64"
+    try:
+        ref = hamiltonian.apply_hamiltonian_batched(oracle_ctx(w), w.psi)
+        for variant, threads in ((-10, 256), (-11, 128), (-12, 64)):
+            ham.set_tuning(variant)
+            got = ham.apply(w.psi, precision=prec)
+            info = ham.info()
+            assert info[f"xline_register_{key}"] == threads and info[f"xline_kind_{key}"] == 4
+            err = relerr(got, ref)
+            print(prec, variant, err)
+            assert err < (TOL32 if prec else TOL64)
+    finally:
+        ham.close()
+    wide = workloads.bench(nbands=4, grid=(48, 72, 72))
+    hw = make_ham(dz, ctx, wide)
+    try:
+        assert hw.info()["planes"] > 32
+        with pytest.raises(dz.BadArgument):
+            hw.set_tuning(-10)
+    finally:
+        hw.close()
 
 
 @pytest.mark.parametrize("prec", [0, 1])

@@ -229,3 +229,34 @@ def test_oracle_pbe_scf_golden():
 def test_pbe_needs_the_device_grid():
     with pytest.raises(ValueError):
         dscf.ScfDriver(dscf.al4_small(xc="pbe"), device_grid=False)
+
+
+def test_ibz_kmesh_reference_counts():
+    """Symmetry reduction of unshifted meshes (reduction.reduce_kmesh,
+    src/features/kpoints/reduction.zig:55-120): the reference's own test, fcc Al primitive cell
+    6x6x6 -> 16 points with weights summing to 1 (src/features/kpoints/kpoints.zig:94-125); the
+    cubic 4-atom Al cell 8x8x8 -> 35 points whose plane-wave counts at 30 Ry span 1,208-1,260;
+    2x2x2 -> 4 (Gamma, X, M, R with weights 1, 3, 3, 1 over 8); every mesh point lies in exactly one
+    orbit."""
+    from dzhip import scf as dscf
+    from dzhip import tables as dtab
+
+    prim = np.array([[0, 2.025, 2.025], [2.025, 0, 2.025], [2.025, 2.025, 0]]) * dtab.ANGSTROM_TO_BOHR
+    pts, wts, _ = dscf.ibz_kmesh(prim, (6, 6, 6), np.zeros((1, 3)), np.zeros(1, int))
+    assert len(pts) == 16 and abs(sum(wts) - 1.0) < 1e-12
+    a = 4.05 * dtab.ANGSTROM_TO_BOHR
+    cell = np.diag([a, a, a])
+    frac = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]])
+    assert len(dscf.lattice_symmetry_ops(cell, frac, np.zeros(4, int))) == 48
+    pts, wts, fr = dscf.ibz_kmesh(cell, (2, 2, 2), frac, np.zeros(4, int))
+    assert sorted(round(w * 8) for w in wts) == [1, 1, 3, 3]
+    al = dscf.aluminium(symmetry=True, kmesh=(8, 8, 8))
+    assert len(al.kpoints) == 35 and abs(sum(al.kweights) - 1.0) < 1e-12
+    npw = [dtab.generate_basis(al.recip, al.ecut_ry, k).kinetic.shape[0] for k in al.kpoints]
+    assert min(npw) == 1208 and max(npw) == 1260 and al.grid == (30, 30, 30)
+    # Orbits partition the mesh: the weights count 512 distinct points.
+    assert sum(round(w * 512) for w in al.kweights) == 512
+    # A structure with only the identity (one displaced atom) keeps k and -k together only.
+    pts, wts, _ = dscf.ibz_kmesh(cell, (4, 4, 4), frac + np.array([[0, 0, 0], [0.013, 0.007, 0.002], [0, 0, 0],
+                                                                      [0, 0, 0]]), np.zeros(4, int))
+    assert len(pts) == 36
