@@ -172,29 +172,49 @@ def scf_leg(name, max_iter, arms, log=sys.stderr):
     from dzhip import precond
     from dzhip import scf as dscf
 
-    system = {"si64": dscf.si64, "si8": dscf.si8_small}[name]()
-    nocc = int(np.ceil(system.nelec / 2))
+    system = {"si64": dscf.si64, "si8": dscf.si8_small, "gaas64": dscf.gaas64,
+              "al4": lambda: dscf.aluminium(symmetry=True, kmesh=(8, 8, 8), xc="pbe")}[name]()
+    nocc = int(np.ceil(system.nelec / 2)) if system.smearing == 0.0 else 0
+    settings = dict(SCF_COMMON)
+    if name == "al4":
+        # The perfect fcc cell: LCG start vectors (diagonal ones can miss symmetry-forbidden states),
+        # every band converged (Fermi-Dirac occupations), the metal's own dielectric constant.
+        settings.update(init_diagonal=False, diemac=1.0)
     out = {"system": system.name, "grid": list(system.grid), "k_points": len(system.kpoints),
-           "bands": system.nbands, "settings": dict(SCF_COMMON), "max_iter": max_iter}
-    params = None
-    if "mixed_batched_learned" in arms:
+           "bands": system.nbands, "settings": settings, "max_iter": max_iter}
+    params, learned_base = None, dzhip.PRECOND_KINETIC
+    if any(a.endswith("learned") for a in arms):
         t0 = time.perf_counter()
         tctx = dzhip.Context(0)
-        # Training system at the cutoff of the run, so the features cover its |k+G|^2 range.
-        train_systems = [dscf.silicon(ecut_ry=system.ecut_ry, nbands=24,
-                                      kpoints=([np.zeros(3), np.array([0.11, 0.23, 0.05])], [0.5, 0.5]))]
-        params, hist, nsamp = precond.train_on_scf(tctx, train_systems, base=dzhip.PRECOND_KINETIC, steps=12,
+        if name == "gaas64":
+            # Residual/correction pairs of Si-8 and Al-4 runs at the cutoff of the run (so the
+            # features cover its |k+G|^2 range), on the TPA base the GaAs comparison measures against.
+            learned_base = dzhip.PRECOND_TPA
+            train_systems = [dscf.silicon(ecut_ry=system.ecut_ry, nbands=24,
+                                          kpoints=([np.zeros(3), np.array([0.11, 0.23, 0.05])], [0.5, 0.5])),
+                             dscf.aluminium(ecut_ry=30.0, kmesh=(2, 2, 2), nbands=10,
+                                            displace=dscf.AL_SMALL_DISPLACEMENT)]
+        else:
+            # Training system at the cutoff of the run, so the features cover its |k+G|^2 range.
+            train_systems = [dscf.silicon(ecut_ry=system.ecut_ry, nbands=24,
+                                          kpoints=([np.zeros(3), np.array([0.11, 0.23, 0.05])], [0.5, 0.5]))]
+        params, hist, nsamp = precond.train_on_scf(tctx, train_systems, base=learned_base, steps=12,
                                                    diemac=12.0, start="atomic", init_diagonal=True)
         tctx.close()
         train_s = time.perf_counter() - t0
         print(f"learned preconditioner: {nsamp} samples, loss {hist[0]:.4f} -> {hist[-1]:.4f}, {train_s:.1f} s",
               file=log, flush=True)
-        out["learned_training"] = {"samples": nsamp, "base": "kinetic shift", "loss_start": float(hist[0]),
+        out["learned_training"] = {"samples": nsamp,
+                                   "base": "TPA" if learned_base == dzhip.PRECOND_TPA else "kinetic shift",
+                                   "systems": [t.name for t in train_systems], "loss_start": float(hist[0]),
                                    "loss_end": float(hist[-1]), "seconds": round(train_s, 2)}
     forms = {"fp64": (dzhip.FP64, False, dzhip.PRECOND_KINETIC),
              "fp64_concurrent": (dzhip.FP64, True, dzhip.PRECOND_KINETIC),
              "mixed_batched": (dzhip.FP32, True, dzhip.PRECOND_KINETIC),
-             "mixed_batched_learned": (dzhip.FP32, True, dzhip.PRECOND_MLP)}
+             "mixed_batched_learned": (dzhip.FP32, True, dzhip.PRECOND_MLP),
+             # GaAs-64 (Gamma only): the TPA preconditioner against the network on the TPA base.
+             "tpa": (dzhip.FP32, False, dzhip.PRECOND_TPA),
+             "tpa_learned": (dzhip.FP32, False, dzhip.PRECOND_MLP)}
     runs, setup = {}, {}
     for tag in arms:
         prec, conc, pc = forms[tag]
@@ -202,7 +222,7 @@ def scf_leg(name, max_iter, arms, log=sys.stderr):
         setup[tag] = drv.setup_s
         fallbacks = []
         if pc == dzhip.PRECOND_MLP:
-            drv.set_precond_model(params, precond.ENERGY_SCALE, dzhip.PRECOND_KINETIC)
+            drv.set_precond_model(params, precond.ENERGY_SCALE, learned_base)
             drv.solve_hook = lambda k, ham, st: fallbacks.append(st["precond_fallback"])
 
         def show(r, tag=tag):
@@ -212,17 +232,18 @@ def scf_leg(name, max_iter, arms, log=sys.stderr):
                   f"gpu_ms[{sec}]", file=log, flush=True)
         try:
             runs[tag] = drv.run(max_iter=max_iter, precision=prec, preconditioner=pc, nconv=nocc, progress=show,
-                                **SCF_COMMON)
+                                **settings)
         finally:
             drv.close()
         if pc == dzhip.PRECOND_MLP:
             out["learned_training"]["scf_solves_fallback"] = {"solves": len(fallbacks),
                                                               "fell_back": sum(1 for f in fallbacks if f > 0)}
-    ref = runs.get("fp64")
+    base_arm = "fp64" if "fp64" in runs else arms[0]
+    ref = runs.get(base_arm)
     e_inf = ref.energies[-1] if ref is not None and ref.converged else None
     out["e_inf_ry"] = e_inf
-    out["e_inf_source"] = ("plain FP64 GPU run, converged (RMS density change below conv)" if e_inf is not None
-                           else "none: the plain FP64 run did not converge, so no time-to-1e-6 Ha is reported")
+    out["e_inf_source"] = (f"{base_arm} GPU run, converged (RMS density change below conv)" if e_inf is not None
+                           else f"none: the {base_arm} run did not converge, so no time-to-1e-6 Ha is reported")
     for tag, run in runs.items():
         t = run.time_to(e_inf) if e_inf is not None else None
         out[tag] = {"t_1e-6_Ha_s": None if t is None else round(t, 3), "iterations": len(run.records),
@@ -236,9 +257,9 @@ def scf_leg(name, max_iter, arms, log=sys.stderr):
                     "solver_gpu_ms": {k[2:-3]: round(sum(r.sections_ms.get(k, 0.0) for r in run.records), 1)
                                       for k in dscf.SECTIONS},
                     "fp32_applies": sum(r.fp32_applies for r in run.records)}
-    a = out.get("fp64", {}).get("t_1e-6_Ha_s")
+    a = out.get(base_arm, {}).get("t_1e-6_Ha_s")
     for tag in arms:
-        if tag == "fp64":
+        if tag == base_arm:
             continue
         b = out[tag]["t_1e-6_Ha_s"]
         out[f"speedup_{tag}"] = None if not a or not b else round(a / b, 3)
@@ -372,7 +393,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--scf", default="si64", choices=("none", "si8", "si64"),
+    ap.add_argument("--scf", default="si64", choices=("none", "si8", "si64", "gaas64", "al4"),
                     help="also time the SCF to 1e-6 Ha (plain FP64 vs the concurrent and mixed forms)")
     ap.add_argument("--scf-max-iter", type=int, default=60)
     ap.add_argument("--scf-arms", default=",".join(SCF_ARMS),
@@ -457,6 +478,10 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(system)
     arms = [a for a in args.scf_arms.split(",") if a]
+    if args.scf == "gaas64" and args.scf_arms == ",".join(SCF_ARMS):
+        arms = ["tpa", "tpa_learned"]
+    if args.scf == "al4" and args.scf_arms == ",".join(SCF_ARMS):
+        arms = ["fp64", "fp64_concurrent", "mixed_batched"]
 
     out = {
         "metric": METRIC,
