@@ -128,13 +128,16 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_cn(const double2* __restr
             Bs[c * kGramPitch + kk] = make_double2(br[q], bi[q]);
         }
     };
-    v4d_t cr[2][2], ci[2][2];
+    // conj(a) b = (ar br + ai bi) + i (ar bi - ai br) by three real MFMAs (Gauss):
+    // p1 = sum ar br, p2 = sum ai bi, p3 = sum (ar - ai)(br + bi); Re = p1 + p2, Im = p3 - p1 + p2.
+    v4d_t p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) {
-            cr[x][y] = v4d_t{0, 0, 0, 0};
-            ci[x][y] = v4d_t{0, 0, 0, 0};
+            p1[x][y] = v4d_t{0, 0, 0, 0};
+            p2[x][y] = v4d_t{0, 0, 0, 0};
+            p3[x][y] = v4d_t{0, 0, 0, 0};
         }
     if (k0 < k1) {
         fetch(k0);
@@ -155,11 +158,10 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_cn(const double2* __restr
             for (int x = 0; x < 2; ++x)
 #pragma unroll
                 for (int y = 0; y < 2; ++y) {
-                    // conj(a) b = (ar br + ai bi) + i (ar bi - ai br)
-                    cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x].x, bv[y].x, cr[x][y], 0, 0, 0);
-                    cr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x].y, bv[y].y, cr[x][y], 0, 0, 0);
-                    ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x].x, bv[y].y, ci[x][y], 0, 0, 0);
-                    ci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[x].y, bv[y].x, ci[x][y], 0, 0, 0);
+                    p1[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x].x, bv[y].x, p1[x][y], 0, 0, 0);
+                    p2[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x].y, bv[y].y, p2[x][y], 0, 0, 0);
+                    p3[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x].x - av[x].y, bv[y].x + bv[y].y, p3[x][y],
+                                                                   0, 0, 0);
                 }
         }
         __syncthreads();
@@ -175,7 +177,110 @@ __global__ __launch_bounds__(kGramThreads) void k_gram_cn(const double2* __restr
             for (int r = 0; r < 4; ++r) {
                 const int i = i_base + wi * 32 + x * 16 + l4 + 4 * r;
                 const int j = j_base + wj * 32 + y * 16 + l16;
-                if (i < M && j < N) partial[((size_t)ks * N + j) * M + i] = make_double2(cr[x][y][r], ci[x][y][r]);
+                const double a1 = p1[x][y][r], a2 = p2[x][y][r];
+                if (i < M && j < N) partial[((size_t)ks * N + j) * M + i] = make_double2(a1 + a2, p3[x][y][r] - a1 + a2);
+            }
+}
+
+// C[:, 0:N] = alpha A[:, 0:K] B[0:K, 0:N] + beta C, alpha and beta real: A is n x K (leading
+// dimension lda), B is K x N (ldb), C is n x N (ldc), all column-major.  For the solver's products
+// of the tall basis with small coefficient blocks (n = n_pw ~ 24k rows; K, N up to a few hundred):
+// one workgroup per 64 x 64 tile of C, K in stages of 16 through LDS with a register prefetch of
+// the next stage, FP64 MFMA 16x16x4 with each complex product as three real MFMAs.  The MFMA
+// computes C^T = B^T A^T so that a wavefront's accumulator slots hold 16 consecutive rows of a
+// column of C (coalesced loads of the old C and stores).
+__global__ __launch_bounds__(kGramThreads) void k_gemm_nn(const double2* __restrict__ A, int lda,
+                                                          const double2* __restrict__ B, int ldb,
+                                                          double2* __restrict__ C, int ldc, int n, int K, int N,
+                                                          double alpha, double beta) {
+    __shared__ double2 As[kGramTile * kGramPitch];  // [row][k]
+    __shared__ double2 Bs[kGramTile * kGramPitch];  // [column][k]
+    const int r_base = blockIdx.x * kGramTile, c_base = blockIdx.y * kGramTile;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wr = wave & 1, wc = wave >> 1;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    double2 av[kGramPer], bv[kGramPer];
+    auto fetch = [&](int kt) {
+#pragma unroll
+        for (int q = 0; q < kGramPer; ++q) {
+            const int e = tid + q * kGramThreads;
+            // A: consecutive threads read consecutive rows of one column (coalesced).
+            const int ra = e % kGramTile, ka = e / kGramTile;
+            const int row = r_base + ra, k = kt + ka;
+            av[q] = (row < n && k < K) ? A[(size_t)k * lda + row] : make_double2(0.0, 0.0);
+            // B: consecutive threads read consecutive k of one column.
+            const int kb = e % kGramK, cb = e / kGramK;
+            const int col = c_base + cb, k2 = kt + kb;
+            bv[q] = (col < N && k2 < K) ? B[(size_t)col * ldb + k2] : make_double2(0.0, 0.0);
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < kGramPer; ++q) {
+            const int e = tid + q * kGramThreads;
+            As[(e % kGramTile) * kGramPitch + e / kGramTile] = av[q];
+            Bs[(e / kGramK) * kGramPitch + e % kGramK] = bv[q];
+        }
+    };
+    // b a = (br ar - bi ai) + i (br ai + bi ar): p1 = sum br ar, p2 = sum bi ai,
+    // p3 = sum (br + bi)(ar + ai); Re = p1 - p2, Im = p3 - p1 - p2.
+    v4d_t p1[2][2], p2[2][2], p3[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            p1[x][y] = v4d_t{0, 0, 0, 0};
+            p2[x][y] = v4d_t{0, 0, 0, 0};
+            p3[x][y] = v4d_t{0, 0, 0, 0};
+        }
+    fetch(0);
+    stash();
+    __syncthreads();
+    for (int kt = 0; kt < K; kt += kGramK) {
+        const bool more = kt + kGramK < K;
+        if (more) fetch(kt + kGramK);
+#pragma unroll
+        for (int kk = 0; kk < kGramK; kk += 4) {
+            double2 bo[2], ao[2];
+            // Operand 0 (MFMA rows): columns of C; operand 1 (MFMA columns): rows of C.
+#pragma unroll
+            for (int x = 0; x < 2; ++x) bo[x] = Bs[(wc * 32 + x * 16 + l16) * kGramPitch + kk + l4];
+#pragma unroll
+            for (int y = 0; y < 2; ++y) ao[y] = As[(wr * 32 + y * 16 + l16) * kGramPitch + kk + l4];
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    p1[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bo[x].x, ao[y].x, p1[x][y], 0, 0, 0);
+                    p2[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bo[x].y, ao[y].y, p2[x][y], 0, 0, 0);
+                    p3[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bo[x].x + bo[x].y, ao[y].x + ao[y].y, p3[x][y],
+                                                                   0, 0, 0);
+                }
+        }
+        __syncthreads();
+        if (more) stash();
+        __syncthreads();
+    }
+    // Slot r of a 16x16 block: MFMA row (l/16) + 4 r = column of C, MFMA column l % 16 = row of C.
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = c_base + wc * 32 + x * 16 + l4 + 4 * r;
+                const int row = r_base + wr * 32 + y * 16 + l16;
+                if (col < N && row < n) {
+                    const double a1 = p1[x][y][r], a2 = p2[x][y][r];
+                    double2 v = make_double2(alpha * (a1 - a2), alpha * (p3[x][y][r] - a1 - a2));
+                    double2* cp = C + (size_t)col * ldc + row;
+                    if (beta != 0.0) {
+                        const double2 o = *cp;
+                        v.x += beta * o.x;
+                        v.y += beta * o.y;
+                    }
+                    *cp = v;
+                }
             }
 }
 
@@ -418,6 +523,15 @@ struct Solver {
 
     void gemm(rocblas_operation ta, rocblas_operation tb, int m, int nn, int k, zc alpha, const double2* A,
               int lda, const double2* B, int ldb, zc beta, double2* C, int ldc) {
+        if (m <= 0 || nn <= 0) return;
+        if (ta == rocblas_operation_none && tb == rocblas_operation_none && alpha.imag() == 0.0 && beta.imag() == 0.0) {
+            // Basis times coefficients: the library's own MFMA kernel.
+            hipLaunchKernelGGL(k_gemm_nn, dim3((unsigned)((m + kGramTile - 1) / kGramTile),
+                                               (unsigned)((nn + kGramTile - 1) / kGramTile)),
+                               dim3(kGramThreads), 0, s, A, lda, B, ldb, C, ldc, m, k, nn, alpha.real(), beta.real());
+            DZ_HIP(hipGetLastError());
+            return;
+        }
         DZ_BLAS(rocblas_zgemm(w.blas, ta, tb, m, nn, k, &alpha, reinterpret_cast<const zc*>(A), lda,
                               reinterpret_cast<const zc*>(B), ldb, &beta, reinterpret_cast<zc*>(C), ldc));
     }

@@ -19,8 +19,8 @@ bool factor_fft_len(int n, std::vector<int>& radices) {
     radices.clear();
     if (n < 1 || n > kMaxFftLen) return false;
     int m = n;
-    // Largest register radices first: fewer LDS round trips per line.
-    for (int r : {9, 8, 4, 2, 3, 5}) {
+    // Largest register radices first: fewer LDS round trips per line (75 = 15 x 5, 90 = 9 x 10).
+    for (int r : {9, 8, 15, 10, 6, 4, 2, 3, 5}) {
         while (m % r == 0) { radices.push_back(r); m /= r; }
     }
     if (m != 1) return false;

@@ -1,7 +1,7 @@
 // In-LDS batched mixed-radix complex FFT on the lines of a tile.
 //
 // Algorithm (own design): in-place Cooley-Tukey with register butterflies of radix
-// 2, 3, 4, 5, 8 or 9.  The decimation-in-time form takes its input at digit-reversed
+// 2, 3, 4, 5, 6, 8, 9, 10 or 15 (the composite ones as two-level register DFTs).  The decimation-in-time form takes its input at digit-reversed
 // positions (AxisTables::perm) and leaves natural order; the decimation-in-frequency
 // form takes natural order and leaves frequency f at position perm[f].  Each butterfly
 // reads and writes the same R slots, so any assignment of butterflies to threads is
@@ -17,6 +17,7 @@
 #pragma once
 
 #include "dzhip_internal.h"
+#include "fft_reg.h"
 
 #if defined(__HIP_DEVICE_COMPILE__)
 #define DZ_BARRIER() __syncthreads()
@@ -161,13 +162,54 @@ __host__ __device__ __forceinline__ void dft9(C* v, T sign) {
 }
 
 template <typename T, typename C, int R>
+__host__ __device__ __forceinline__ void dft_r(C* v, T sign);
+
+// Composite radix N = N1 N2 in registers (Cooley-Tukey): inputs j = N2 j1 + j2, outputs
+// k = k1 + N1 k2; N1-point DFTs over j1, twiddles w_N^(j2 k1) (compile-time constants), N2-point
+// DFTs over j2.  Every index is a compile-time constant, so the final reordering is a renaming.
+template <typename T, typename C, int N1, int N2>
+__host__ __device__ __forceinline__ void dft_ct(C* v, T sign) {
+    constexpr int N = N1 * N2;
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) {
+        C t[N1];
+#pragma unroll
+        for (int j1 = 0; j1 < N1; ++j1) t[j1] = v[N2 * j1 + j2];
+        dft_r<T, C, N1>(t, sign);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) v[N2 * k1 + j2] = t[k1];
+    }
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1)
+#pragma unroll
+        for (int j2 = 1; j2 < N2; ++j2) {
+            const freg::CosSin w = freg::unit_root((long)j2 * k1, N);
+            const T c = T(w.c), sn = sign * T(w.s);
+            C& a = v[N2 * k1 + j2];
+            a = C{a.x * c - a.y * sn, a.x * sn + a.y * c};
+        }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) dft_r<T, C, N2>(v + N2 * k1, sign);
+    C out[N];
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) out[k1 + N1 * k2] = v[N2 * k1 + k2];
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = out[k];
+}
+
+template <typename T, typename C, int R>
 __host__ __device__ __forceinline__ void dft_r(C* v, T sign) {
     if constexpr (R == 2) dft2<T>(v[0], v[1]);
     if constexpr (R == 3) dft3<T>(v[0], v[1], v[2], sign);
     if constexpr (R == 4) dft4<T>(v[0], v[1], v[2], v[3], sign);
     if constexpr (R == 5) dft5<T>(v, sign);
+    if constexpr (R == 6) dft_ct<T, C, 2, 3>(v, sign);
     if constexpr (R == 8) dft8<T>(v, sign);
     if constexpr (R == 9) dft9<T>(v, sign);
+    if constexpr (R == 10) dft_ct<T, C, 2, 5>(v, sign);
+    if constexpr (R == 15) dft_ct<T, C, 3, 5>(v, sign);
 }
 
 // w^t for the transform sign S (+1 inverse, -1 forward): the table holds exp(-2 pi i t / n).
@@ -240,7 +282,10 @@ __host__ __device__ __forceinline__ void fft_stage_dispatch(int R, C* buf, int n
         case 3: fft_stage<T, C, 3, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
         case 4: fft_stage<T, C, 4, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
         case 5: fft_stage<T, C, 5, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
+        case 6: fft_stage<T, C, 6, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
         case 8: fft_stage<T, C, 8, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
+        case 10: fft_stage<T, C, 10, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
+        case 15: fft_stage<T, C, 15, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
         default: fft_stage<T, C, 9, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
     }
 }

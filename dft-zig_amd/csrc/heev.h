@@ -21,10 +21,7 @@ constexpr int kHeevMaxM = 1024;
 
 struct HeevWork {
     DevBuf vh, tau, d, e, pbuf, nbuf, sync, z, zt, gram, fac;
-    int coop_groups = 0;     // co-resident workgroups of the tridiagonalisation (queried once)
-    int device = -1;
-    int err_offset = 0;      // int index of the error flag in sync
-    int cols_per_group = 0;  // trailing columns per workgroup (0 = default)
+    int err_offset = 0;  // int index of the error flag in sync
     void release();
 };
 

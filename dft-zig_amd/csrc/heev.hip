@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 namespace dzhip {
 
@@ -24,18 +25,22 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Sum of n values over the workgroup (NT threads); every thread gets the n sums in out[].
 // red holds (NT / 64) * n doubles.
-template <int NT>
-__device__ __forceinline__ void block_sums(double* vals, int n, double* red, double* out) {
+template <int NT, int N>
+__device__ __forceinline__ void block_sums(double (&vals)[N], double* red, double (&out)[N]) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int k = 0; k < n; ++k) vals[k] = wave_sum(vals[k]);
+#pragma unroll
+    for (int k = 0; k < N; ++k) vals[k] = wave_sum(vals[k]);
     __syncthreads();
-    if (lane == 0)
-        for (int k = 0; k < n; ++k) red[wv * n + k] = vals[k];
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) red[wv * N + k] = vals[k];
+    }
     __syncthreads();
-    for (int k = 0; k < n; ++k) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
         double s = 0.0;
 #pragma unroll
-        for (int i = 0; i < NT / 64; ++i) s += red[i * n + k];
+        for (int i = 0; i < NT / 64; ++i) s += red[i * N + k];
         out[k] = s;
     }
 }
@@ -109,17 +114,23 @@ struct TrdArgs {
 //   E. column j+1 = published column - v_j conj(w_j[j+1]) - w_j conj(v_j[j+1]).
 // The owner never updates column j+1 in place at step j, so the published copy stays valid while
 // the others read it; pbuf and nbuf alternate by step parity.
+// NCL (columns per workgroup) is a template parameter so the per-column partial sums stay in
+// registers.
+template <int NCL>
 __global__ __launch_bounds__(kTrdThreads) void k_hetrd(TrdArgs a) {
     extern __shared__ double2 lds[];
     __shared__ double red[(kTrdThreads / 64) * 2 * kTrdMaxCols];
     __shared__ int s_ok;
-    const int m = a.m, ncl = a.ncl, G = (int)gridDim.x, g = (int)blockIdx.x, t = (int)threadIdx.x;
+    constexpr int ncl = NCL;
+    const int m = a.m, G = (int)gridDim.x, g = (int)blockIdx.x, t = (int)threadIdx.x;
     constexpr int NT = kTrdThreads;
     double2* cols = lds;
-    double2* bufs[4] = {cols + (size_t)ncl * m, cols + (size_t)(ncl + 1) * m, cols + (size_t)(ncl + 2) * m,
-                        cols + (size_t)(ncl + 3) * m};
-    // Roles: vc = current column (becomes v_j in place), wn = w_j, vp/wp = v_{j-1}, w_{j-1}.
-    int ivc = 0, iwn = 1, ivp = 2, iwp = 3;
+    // Roles, rotated every step: vc = current column (becomes v_j in place), wn = w_j,
+    // vp / wp = v_{j-1}, w_{j-1}.
+    double2* vc_b = cols + (size_t)ncl * m;
+    double2* wn_b = cols + (size_t)(ncl + 1) * m;
+    double2* vp_b = cols + (size_t)(ncl + 2) * m;
+    double2* wp_b = cols + (size_t)(ncl + 3) * m;
     // Only the lower triangle of A is read (the upper one is its mirror; the diagonal is real), as
     // LAPACK's zheev with uplo = 'L'.
     for (int c = 0; c < ncl; ++c) {
@@ -136,20 +147,16 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd(TrdArgs a) {
             cols[(size_t)c * m + r] = x;
         }
     }
-    {
-        double2* vc = bufs[ivc];
-        for (int r = t; r < m; r += NT) vc[r] = r == 0 ? make_double2(a.A[0].x, 0.0) : a.A[r];
-    }
+    for (int r = t; r < m; r += NT) vc_b[r] = r == 0 ? make_double2(a.A[0].x, 0.0) : a.A[r];
     __syncthreads();
-    double2 tau_prev = make_double2(0.0, 0.0);
     for (int j = 0; j + 1 < m; ++j) {
-        double2* vc = bufs[ivc];
+        double2* vc = vc_b;
         // A. reflector of column j (rows j+1..m-1).
-        double xs[2];
-        xs[0] = 0.0;
+        double xs[1] = {0.0};
         for (int r = j + 2 + t; r < m; r += NT) xs[0] += vc[r].x * vc[r].x + vc[r].y * vc[r].y;
-        double xn2;
-        block_sums<NT>(xs, 1, red, &xn2);
+        double xo[1];
+        block_sums<NT>(xs, red, xo);
+        const double xn2 = xo[0];
         const double2 alpha = vc[j + 1];
         double beta;
         double2 tau, scale;
@@ -177,11 +184,13 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd(TrdArgs a) {
         if (g == 0)
             for (int r = j + 1 + t; r < m; r += NT) a.vh[(size_t)j * m + r] = vc[r];
         // B. fused update (step j-1) and product (step j) over the own columns i >= j+1.
-        const double2* vp = bufs[ivp];
-        const double2* wp = bufs[iwp];
-        double part[2 * kTrdMaxCols];
+        const double2* vp = vp_b;
+        const double2* wp = wp_b;
+        double part[2 * ncl];
+#pragma unroll
         for (int c = 0; c < 2 * ncl; ++c) part[c] = 0.0;
         double2* nb_out = a.nbuf + (size_t)((j + 1) & 1) * m;
+#pragma unroll
         for (int c = 0; c < ncl; ++c) {
             const int i = g + c * G;
             if (i <= j || i >= m) continue;
@@ -207,17 +216,18 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd(TrdArgs a) {
             part[2 * c] = acc.x;
             part[2 * c + 1] = acc.y;
         }
-        double sums[2 * kTrdMaxCols];
-        block_sums<NT>(part, 2 * ncl, red, sums);
+        double sums[2 * ncl];
+        block_sums<NT>(part, red, sums);
         double2* pb = a.pbuf + (size_t)(j & 1) * m;
-        if (t < ncl) {
-            const int i = g + t * G;
-            if (i > j && i < m) pb[i] = cm(tau, make_double2(sums[2 * t], sums[2 * t + 1]));
+#pragma unroll
+        for (int c = 0; c < ncl; ++c) {
+            const int i = g + c * G;
+            if (t == c && i > j && i < m) pb[i] = cm(tau, make_double2(sums[2 * c], sums[2 * c + 1]));
         }
         // C.
         if (!grid_sync(a.flags, (unsigned)(j + 1), a.err, &s_ok)) return;
         // D. w_j = p_j - (tau_j / 2) (p_j^H v_j) v_j.
-        double2* wn = bufs[iwn];
+        double2* wn = wn_b;
         double ps[2] = {0.0, 0.0};
         for (int r = j + 1 + t; r < m; r += NT) {
             const double2 p = pb[r];
@@ -227,7 +237,7 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd(TrdArgs a) {
             ps[1] += pr.y;
         }
         double s2[2];
-        block_sums<NT>(ps, 2, red, s2);
+        block_sums<NT>(ps, red, s2);
         const double2 al = cm(make_double2(-0.5 * tau.x, -0.5 * tau.y), make_double2(s2[0], s2[1]));
         for (int r = j + 1 + t; r < m; r += NT) {
             const double2 u = cm(al, vc[r]);
@@ -236,7 +246,7 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd(TrdArgs a) {
         }
         __syncthreads();
         // E. column j+1 after the step-j update, into the buffer of v_{j-1} (no longer needed).
-        double2* vnext = bufs[ivp];
+        double2* vnext = vp_b;
         const double2* nb_in = a.nbuf + (size_t)((j + 1) & 1) * m;
         const double2 wj1 = wn[j + 1], vj1 = vc[j + 1];
         for (int r = j + 1 + t; r < m; r += NT) {
@@ -248,14 +258,13 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd(TrdArgs a) {
         __syncthreads();
         if (j + 2 == m && g == 0 && t == 0) a.d[m - 1] = vnext[m - 1].x;
         // Rotate: v_{j-1} <- v_j, w_{j-1} <- w_j, current column <- column j+1.
-        const int old_vp = ivp, old_wp = iwp;
-        ivp = ivc;
-        iwp = iwn;
-        ivc = old_vp;
-        iwn = old_wp;
-        tau_prev = tau;
+        double2* const old_vp = vp_b;
+        double2* const old_wp = wp_b;
+        vp_b = vc_b;
+        wp_b = wn_b;
+        vc_b = old_vp;
+        wn_b = old_wp;
     }
-    (void)tau_prev;
     if (m == 1 && g == 0 && t == 0) a.d[0] = a.A[0].x;
 }
 
@@ -283,7 +292,8 @@ __device__ __forceinline__ int sturm_count(const double* d, const double* e2, in
 // at 64 interior points of [lo, hi] and keep the sub-interval that holds eigenvalue idx.
 // Also writes the Gershgorin scale of T to scal[0] (block 0).
 __global__ __launch_bounds__(256) void k_stebz(const double* __restrict__ d_in, const double* __restrict__ e_in, int m,
-                                               int nev, double* __restrict__ w, double* __restrict__ scal) {
+                                               int nev, double* __restrict__ w, double* __restrict__ scal,
+                                               const int* __restrict__ err) {
     extern __shared__ double sh[];
     double* d = sh;
     double* e2 = sh + m;
@@ -329,6 +339,12 @@ __global__ __launch_bounds__(256) void k_stebz(const double* __restrict__ d_in, 
     const int lane = t & 63;
     const int idx = (int)blockIdx.x * 4 + (t >> 6);
     if (idx >= nev) return;
+    // A tridiagonalisation whose grid barrier did not complete leaves NaN eigenvalues (the signal
+    // a caller without a synchronising check sees).
+    if (*err != 0) {
+        if (lane == 0) w[idx] = __builtin_nan("");
+        return;
+    }
     double lo = gl, hi = gu;
     for (int round = 0; round < 14; ++round) {
         const double width = hi - lo;
@@ -529,25 +545,51 @@ __global__ __launch_bounds__(256) void k_unmtr(const double2* __restrict__ vh, c
 
 size_t trd_lds_bytes(int m, int ncl) { return (size_t)(ncl + 4) * (size_t)m * sizeof(double2); }
 
-// Launch shape of the tridiagonalisation: G workgroups holding ncl columns each.
-void trd_shape(HeevWork& w, int m, int* G_out, int* ncl_out) {
-    int dev = 0;
-    DZ_HIP(hipGetDevice(&dev));
-    if (w.device != dev || w.coop_groups == 0) {
+// The tridiagonalisation kernel for ncl columns per workgroup (1..kTrdMaxCols).
+const void* trd_kernel(int ncl) {
+    static const void* const table[kTrdMaxCols] = {
+        (const void*)k_hetrd<1>,  (const void*)k_hetrd<2>,  (const void*)k_hetrd<3>,  (const void*)k_hetrd<4>,
+        (const void*)k_hetrd<5>,  (const void*)k_hetrd<6>,  (const void*)k_hetrd<7>,  (const void*)k_hetrd<8>,
+        (const void*)k_hetrd<9>,  (const void*)k_hetrd<10>, (const void*)k_hetrd<11>, (const void*)k_hetrd<12>,
+        (const void*)k_hetrd<13>, (const void*)k_hetrd<14>, (const void*)k_hetrd<15>, (const void*)k_hetrd<16>};
+    return table[std::max(1, std::min(ncl, kTrdMaxCols)) - 1];
+}
+
+// Co-resident workgroups per device for the tridiagonalisation (one per CU at its LDS size),
+// queried once per device.
+struct CoopInfo {
+    int groups = 0;
+    int cols_per_group = 0;
+};
+std::mutex g_coop_mu;
+CoopInfo g_coop[64];
+
+CoopInfo coop_info(int dev) {
+    DZ_REQUIRE(dev >= 0 && dev < 64, DZHIP_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_coop_mu);
+    CoopInfo& c = g_coop[dev];
+    if (c.groups == 0) {
         int cus = 0, coop = 0;
         DZ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         DZ_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
         DZ_REQUIRE(coop != 0, DZHIP_ERR_HIP, "device does not support cooperative launches");
-        set_max_dynamic_lds((const void*)k_hetrd, kMaxLdsBytes - 2048);
-        w.coop_groups = cus;
-        w.device = dev;
+        for (int c = 1; c <= kTrdMaxCols; ++c) set_max_dynamic_lds(trd_kernel(c), kMaxLdsBytes - 2048);
         // Launch-shape override for sweeps: trailing columns per workgroup.
-        if (const char* env = std::getenv("DZHIP_HEEV_COLS")) w.cols_per_group = std::max(0, std::atoi(env));
+        if (const char* env = std::getenv("DZHIP_HEEV_COLS")) c.cols_per_group = std::max(0, std::atoi(env));
+        c.groups = cus;
     }
-    const int per = w.cols_per_group > 0 ? w.cols_per_group : kTrdColsPerGroup;
-    int G = std::max(1, std::min((m + per - 1) / per, w.coop_groups));
+    return c;
+}
+
+// Launch shape of the tridiagonalisation: G workgroups holding ncl columns each.
+void trd_shape(int m, int* G_out, int* ncl_out) {
+    int dev = 0;
+    DZ_HIP(hipGetDevice(&dev));
+    const CoopInfo ci = coop_info(dev);
+    const int per = ci.cols_per_group > 0 ? ci.cols_per_group : kTrdColsPerGroup;
+    int G = std::max(1, std::min((m + per - 1) / per, ci.groups));
     int ncl = (m + G - 1) / G;
-    while (trd_lds_bytes(m, ncl) > kMaxLdsBytes - 2048 && G < w.coop_groups) {
+    while (trd_lds_bytes(m, ncl) > kMaxLdsBytes - 2048 && G < ci.groups) {
         ++G;
         ncl = (m + G - 1) / G;
     }
@@ -555,35 +597,76 @@ void trd_shape(HeevWork& w, int m, int* G_out, int* ncl_out) {
                "Hermitian eigensolver: matrix too large for the tridiagonalisation's LDS");
     // Every workgroup must be co-resident (one per CU at this LDS size).
     int per_cu = 0;
-    DZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_hetrd, kTrdThreads,
+    DZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trd_kernel(ncl), kTrdThreads,
                                                         trd_lds_bytes(m, ncl)));
-    DZ_REQUIRE(per_cu >= 1 && G <= per_cu * w.coop_groups, DZHIP_ERR_HIP,
+    DZ_REQUIRE(per_cu >= 1 && G <= per_cu * ci.groups, DZHIP_ERR_HIP,
                "Hermitian eigensolver: grid does not fit on the device");
     *G_out = G;
     *ncl_out = ncl;
 }
 
-void run_tridiag(HeevWork& w, const double2* A, int lda, int m, hipStream_t s) {
+// Device buffers of one eigensolver call.
+struct HeevBufs {
+    double2 *vh, *tau, *pbuf, *nbuf;
+    double *d, *e, *z, *zt, *gram, *fac;
+    unsigned* sync;  // (G + 1) * 16 words: barrier flags, then the error flag
+    int err_offset;
+};
+
+void run_tridiag(const HeevBufs& b, int G, int ncl, const double2* A, int lda, int m, hipStream_t s) {
+    // Barrier flags (one 64-byte line per workgroup) and the error flag after them, cleared by a
+    // kernel (not a memset node) on the same stream.
+    const int nsync = (G + 1) * 16;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nsync + 255) / 256)), dim3(256), 0, s, b.sync, nsync);
+    DZ_HIP(hipGetLastError());
+    TrdArgs args{A, lda, m, ncl, b.vh, b.tau, b.d, b.e, b.pbuf, b.nbuf, b.sync,
+                 reinterpret_cast<int*>(b.sync) + b.err_offset};
+    void* kargs[] = {&args};
+    DZ_HIP(hipLaunchCooperativeKernel(trd_kernel(ncl), dim3((unsigned)G), dim3(kTrdThreads), kargs,
+                                      (unsigned)trd_lds_bytes(m, ncl), s));
+}
+
+void run_eig(const HeevBufs& b, int m, int nev, double* w_dev, double2* Y, int ldy, hipStream_t s) {
+    double* scal = b.gram + (size_t)nev * nev;
+    hipLaunchKernelGGL(k_stebz, dim3((unsigned)((nev + 3) / 4)), dim3(256), 2 * m * sizeof(double), s, b.d, b.e, m,
+                       nev, w_dev, scal, reinterpret_cast<const int*>(b.sync) + b.err_offset);
+    DZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_stein, dim3((unsigned)((nev + 63) / 64)), dim3(64), 0, s, b.d, b.e, m, nev, w_dev, scal, b.z,
+                       m, b.fac);
+    DZ_HIP(hipGetLastError());
+    double* zc = b.z;
+    double* zn = b.zt;
+    for (int it = 0; it < 2; ++it) {
+        hipLaunchKernelGGL(k_gram_real, dim3((unsigned)((nev + 15) / 16), (unsigned)((nev + 15) / 16)), dim3(256), 0, s,
+                           zc, m, m, nev, b.gram);
+        DZ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_ns_update, dim3((unsigned)((m + 255) / 256), (unsigned)nev), dim3(256), 0, s, zc, m, m,
+                           nev, b.gram, zn);
+        DZ_HIP(hipGetLastError());
+        std::swap(zc, zn);
+    }
+    hipLaunchKernelGGL(k_unmtr, dim3((unsigned)((nev + 3) / 4)), dim3(256), 0, s, b.vh, b.tau, m, nev, zc, m, Y, ldy);
+    DZ_HIP(hipGetLastError());
+}
+
+HeevBufs work_bufs(HeevWork& w, int m, int nev, int G) {
     w.vh.ensure((size_t)m * m * sizeof(double2));
     w.tau.ensure((size_t)m * sizeof(double2));
     w.d.ensure((size_t)m * sizeof(double));
     w.e.ensure((size_t)m * sizeof(double));
     w.pbuf.ensure((size_t)2 * m * sizeof(double2));
     w.nbuf.ensure((size_t)2 * m * sizeof(double2));
-    int G = 1, ncl = 1;
-    trd_shape(w, m, &G, &ncl);
-    // Barrier flags (one 64-byte line per workgroup) and the error flag after them, cleared by a
-    // kernel (not a memset node) on the same stream.
-    const int nsync = (G + 1) * 16;
-    w.sync.ensure((size_t)nsync * sizeof(unsigned));
-    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nsync + 255) / 256)), dim3(256), 0, s, w.sync.as<unsigned>(), nsync);
-    DZ_HIP(hipGetLastError());
+    w.sync.ensure((size_t)(G + 1) * 16 * sizeof(unsigned));
+    if (nev > 0) {
+        w.z.ensure((size_t)m * nev * sizeof(double));
+        w.zt.ensure((size_t)m * nev * sizeof(double));
+        w.gram.ensure((size_t)nev * nev * sizeof(double) + 64);
+        w.fac.ensure((size_t)nev * 5 * m * sizeof(double));
+    }
     w.err_offset = G * 16;
-    TrdArgs args{A, lda, m, ncl, w.vh.as<double2>(), w.tau.as<double2>(), w.d.as<double>(), w.e.as<double>(),
-                 w.pbuf.as<double2>(), w.nbuf.as<double2>(), w.sync.as<unsigned>(), w.sync.as<int>() + w.err_offset};
-    void* kargs[] = {&args};
-    DZ_HIP(hipLaunchCooperativeKernel((const void*)k_hetrd, dim3((unsigned)G), dim3(kTrdThreads), kargs,
-                                      (unsigned)trd_lds_bytes(m, ncl), s));
+    return HeevBufs{w.vh.as<double2>(), w.tau.as<double2>(), w.pbuf.as<double2>(), w.nbuf.as<double2>(),
+                    w.d.as<double>(),   w.e.as<double>(),     w.z.as<double>(),     w.zt.as<double>(),
+                    w.gram.as<double>(), w.fac.as<double>(),  w.sync.as<unsigned>(), G * 16};
 }
 
 }  // namespace
@@ -591,45 +674,24 @@ void run_tridiag(HeevWork& w, const double2* A, int lda, int m, hipStream_t s) {
 void HeevWork::release() {
     DevBuf* bufs[] = {&vh, &tau, &d, &e, &pbuf, &nbuf, &sync, &z, &zt, &gram, &fac};
     for (DevBuf* b : bufs) b->release();
-    coop_groups = 0;
-    device = -1;
 }
 
 void heev_tridiag_only(HeevWork& w, const double2* A, int lda, int m, hipStream_t s) {
     DZ_REQUIRE(m >= 1 && m <= kHeevMaxM && lda >= m, DZHIP_ERR_ARG, "Hermitian eigensolver: bad size");
-    run_tridiag(w, A, lda, m, s);
+    int G = 1, ncl = 1;
+    trd_shape(m, &G, &ncl);
+    run_tridiag(work_bufs(w, m, 0, G), G, ncl, A, lda, m, s);
 }
 
 void heev_lowest(HeevWork& w, const double2* A, int lda, int m, int nev, double* w_dev, double2* Y, int ldy,
                  hipStream_t s) {
     DZ_REQUIRE(m >= 1 && m <= kHeevMaxM && nev >= 1 && nev <= m && lda >= m && ldy >= m, DZHIP_ERR_ARG,
                "Hermitian eigensolver: need 1 <= nev <= m <= 1024 and leading dimensions >= m");
-    run_tridiag(w, A, lda, m, s);
-    w.z.ensure((size_t)m * nev * sizeof(double));
-    w.zt.ensure((size_t)m * nev * sizeof(double));
-    w.gram.ensure((size_t)nev * nev * sizeof(double) + 64);
-    w.fac.ensure((size_t)nev * 5 * m * sizeof(double));
-    double* scal = w.gram.as<double>() + (size_t)nev * nev;
-    hipLaunchKernelGGL(k_stebz, dim3((unsigned)((nev + 3) / 4)), dim3(256), 2 * m * sizeof(double), s,
-                       w.d.as<double>(), w.e.as<double>(), m, nev, w_dev, scal);
-    DZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_stein, dim3((unsigned)((nev + 63) / 64)), dim3(64), 0, s, w.d.as<double>(), w.e.as<double>(),
-                       m, nev, w_dev, scal, w.z.as<double>(), m, w.fac.as<double>());
-    DZ_HIP(hipGetLastError());
-    double* zc = w.z.as<double>();
-    double* zn = w.zt.as<double>();
-    for (int it = 0; it < 2; ++it) {
-        hipLaunchKernelGGL(k_gram_real, dim3((unsigned)((nev + 15) / 16), (unsigned)((nev + 15) / 16)), dim3(256), 0, s,
-                           zc, m, m, nev, w.gram.as<double>());
-        DZ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_ns_update, dim3((unsigned)((m + 255) / 256), (unsigned)nev), dim3(256), 0, s, zc, m, m,
-                           nev, w.gram.as<double>(), zn);
-        DZ_HIP(hipGetLastError());
-        std::swap(zc, zn);
-    }
-    hipLaunchKernelGGL(k_unmtr, dim3((unsigned)((nev + 3) / 4)), dim3(256), 0, s, w.vh.as<double2>(),
-                       w.tau.as<double2>(), m, nev, zc, m, Y, ldy);
-    DZ_HIP(hipGetLastError());
+    int G = 1, ncl = 1;
+    trd_shape(m, &G, &ncl);
+    const HeevBufs b = work_bufs(w, m, nev, G);
+    run_tridiag(b, G, ncl, A, lda, m, s);
+    run_eig(b, m, nev, w_dev, Y, ldy, s);
 }
 
 int heev_error_flag(HeevWork& w, hipStream_t s) {
