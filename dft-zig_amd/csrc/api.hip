@@ -88,6 +88,8 @@ dzhip_ctx* dzhip_ctx_create(int device) {
         DZ_REQUIRE(device >= 0 && device < n, DZHIP_ERR_ARG,
                    "device " + std::to_string(device) + " not present (" + std::to_string(n) + " visible)");
         DeviceGuard g(device);
+        // Set-up of this device for the allocation-free eigensolver (its scratch array's address).
+        heev_scratch_init(device);
         auto* c = new dzhip_ctx();
         c->device = device;
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -167,6 +169,20 @@ int dzhip_heev_dev(dzhip_ctx* ctx, const void* a, uint64_t lda, uint64_t m, uint
         hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->stream;
         heev_lowest(*ctx->heev, static_cast<const double2*>(a), (int)lda, (int)m, (int)nev, static_cast<double*>(values),
                     static_cast<double2*>(vectors), (int)ldz, s);
+    });
+}
+
+int dzhip_heev_dev_noalloc(dzhip_ctx* ctx, const void* a, uint64_t lda, uint64_t m, uint64_t nev, void* values,
+                           void* vectors, uint64_t ldz, void* stream) {
+    return guarded([&] {
+        DZ_REQUIRE(ctx, DZHIP_ERR_STATE, "NULL context");
+        DZ_REQUIRE(a && values && vectors, DZHIP_ERR_ARG, "NULL array");
+        DZ_REQUIRE(m >= 1 && m <= (uint64_t)kHeevNoAllocMaxM && nev >= 1 && nev <= m && lda >= m && ldz >= m,
+                   DZHIP_ERR_ARG, "allocation-free eigensolver: need 1 <= nev <= m <= 256 and lda, ldz >= m");
+        DeviceGuard g(ctx->device);
+        hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->stream;
+        heev_lowest_noalloc(ctx->device, static_cast<const double2*>(a), (int)lda, (int)m, (int)nev,
+                            static_cast<double*>(values), static_cast<double2*>(vectors), (int)ldz, s);
     });
 }
 

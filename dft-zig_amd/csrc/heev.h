@@ -25,6 +25,9 @@ struct HeevWork {
     void release();
 };
 
+// Largest matrix of the allocation-free variant (its buffers live in a 6 MiB device array).
+constexpr int kHeevNoAllocMaxM = 256;
+
 // Lowest nev eigenpairs of the Hermitian matrix A (column-major, leading dimension lda; only the
 // lower triangle is read, the imaginary part of the diagonal is ignored; A is not modified).  Eigenvalues (ascending) -> w_dev[0:nev] (device); eigenvectors ->
 // Y[:, 0:nev] (complex, leading dimension ldy).  Enqueued on stream s.  Throws on a size outside
@@ -37,5 +40,12 @@ void heev_tridiag_only(HeevWork& w, const double2* A, int lda, int m, hipStream_
 
 // Error flag of the last tridiagonalisation (a grid barrier that did not complete); synchronises s.
 int heev_error_flag(HeevWork& w, hipStream_t s);
+
+// Allocation-free variant for m <= kHeevNoAllocMaxM: every buffer in the library's static device
+// scratch array of `device` (heev_scratch_init must have run for it, with it current).  A failed
+// tridiagonalisation leaves NaN eigenvalues.  Calls that overlap on one device race.
+void heev_scratch_init(int device);
+void heev_lowest_noalloc(int device, const double2* A, int lda, int m, int nev, double* w_dev, double2* Y, int ldy,
+                         hipStream_t s);
 
 }  // namespace dzhip

@@ -605,7 +605,7 @@ void trd_shape(int m, int* G_out, int* ncl_out) {
     *ncl_out = ncl;
 }
 
-// Device buffers of one eigensolver call.
+// Device buffers of one eigensolver call (from a HeevWork, or carved from the static scratch array).
 struct HeevBufs {
     double2 *vh, *tau, *pbuf, *nbuf;
     double *d, *e, *z, *zt, *gram, *fac;
@@ -669,7 +669,77 @@ HeevBufs work_bufs(HeevWork& w, int m, int nev, int G) {
                     w.gram.as<double>(), w.fac.as<double>(),  w.sync.as<unsigned>(), G * 16};
 }
 
+// ----------------------------------------------------------------- allocation-free variant
+// A file-scope device array holds every buffer of an eigensolver call of m <= kHeevNoAllocMaxM, for
+// callers that cannot allocate.  Its device address is taken once per device at set-up
+// (heev_scratch_init, from dzhip_ctx_create with the device current) and only that address is used
+// on the host.  Calls that overlap on one device share the array and race.
+constexpr size_t kHeevNoAllocBytes = 6u << 20;
+static __device__ __align__(256) unsigned char scratch[kHeevNoAllocBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+size_t carve(size_t& off, size_t bytes) {
+    const size_t at = off;
+    off = round_up(off + bytes, 256);
+    return at;
+}
+
+// Offsets of the buffers in the scratch for (m, nev, G); returns the bytes needed.
+size_t scratch_layout(int m, int nev, int G, size_t out[11]) {
+    size_t off = 0;
+    out[0] = carve(off, (size_t)m * m * sizeof(double2));       // vh
+    out[1] = carve(off, (size_t)m * sizeof(double2));           // tau
+    out[2] = carve(off, (size_t)2 * m * sizeof(double2));       // pbuf
+    out[3] = carve(off, (size_t)2 * m * sizeof(double2));       // nbuf
+    out[4] = carve(off, (size_t)m * sizeof(double));            // d
+    out[5] = carve(off, (size_t)m * sizeof(double));            // e
+    out[6] = carve(off, (size_t)m * nev * sizeof(double));      // z
+    out[7] = carve(off, (size_t)m * nev * sizeof(double));      // zt
+    out[8] = carve(off, (size_t)nev * nev * sizeof(double) + 64);  // gram + scale
+    out[9] = carve(off, (size_t)nev * 5 * m * sizeof(double));  // fac
+    out[10] = carve(off, (size_t)(G + 1) * 16 * sizeof(unsigned));  // sync
+    return off;
+}
+
 }  // namespace
+
+void heev_scratch_init(int device) {
+    DZ_REQUIRE(device >= 0 && device < 64, DZHIP_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    DZ_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    DZ_REQUIRE(ptr != nullptr, DZHIP_ERR_HIP, "no device address for the eigensolver scratch");
+    g_scratch_ptr[device] = ptr;
+}
+
+void heev_lowest_noalloc(int device, const double2* A, int lda, int m, int nev, double* w_dev, double2* Y, int ldy,
+                         hipStream_t s) {
+    DZ_REQUIRE(m >= 1 && m <= kHeevNoAllocMaxM && nev >= 1 && nev <= m && lda >= m && ldy >= m, DZHIP_ERR_ARG,
+               "allocation-free Hermitian eigensolver: need 1 <= nev <= m <= 256 (its scratch is 6 MiB) and "
+               "leading dimensions >= m");
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = device >= 0 && device < 64 ? g_scratch_ptr[device] : nullptr;
+    }
+    DZ_REQUIRE(base, DZHIP_ERR_STATE, "eigensolver scratch not set up on this device (dzhip_ctx_create)");
+    int G = 1, ncl = 1;
+    trd_shape(m, &G, &ncl);
+    size_t off[11];
+    const size_t need = scratch_layout(m, nev, G, off);
+    DZ_REQUIRE(need <= kHeevNoAllocBytes, DZHIP_ERR_ARG, "allocation-free Hermitian eigensolver: scratch too small");
+    unsigned char* p = static_cast<unsigned char*>(base);
+    const HeevBufs b{reinterpret_cast<double2*>(p + off[0]), reinterpret_cast<double2*>(p + off[1]),
+                     reinterpret_cast<double2*>(p + off[2]), reinterpret_cast<double2*>(p + off[3]),
+                     reinterpret_cast<double*>(p + off[4]),  reinterpret_cast<double*>(p + off[5]),
+                     reinterpret_cast<double*>(p + off[6]),  reinterpret_cast<double*>(p + off[7]),
+                     reinterpret_cast<double*>(p + off[8]),  reinterpret_cast<double*>(p + off[9]),
+                     reinterpret_cast<unsigned*>(p + off[10]), G * 16};
+    run_tridiag(b, G, ncl, A, lda, m, s);
+    run_eig(b, m, nev, w_dev, Y, ldy, s);
+}
 
 void HeevWork::release() {
     DevBuf* bufs[] = {&vh, &tau, &d, &e, &pbuf, &nbuf, &sync, &z, &zt, &gram, &fac};

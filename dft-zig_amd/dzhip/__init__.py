@@ -41,7 +41,7 @@ EXPORTS = (
     "dzhip_scf_grid_set_xc", "dzhip_ham_set_overlap",
     "dzhip_scf_grid_init_rho", "dzhip_heev", "dzhip_heev_dev", "dzhip_heev_check",
     "dzhip_hamset_create", "dzhip_hamset_destroy", "dzhip_hamset_offsets", "dzhip_hamset_apply_dev",
-    "dzhip_hamset_apply", "dzhip_hamset_set_stream", "dzhip_hamset_info",
+    "dzhip_hamset_apply", "dzhip_hamset_set_stream", "dzhip_hamset_info", "dzhip_heev_dev_noalloc",
 )
 
 
@@ -174,6 +174,7 @@ def _load():
         "dzhip_heev": ([vp, dp, u64, u64, u64, dp, dp, u64], i32),
         "dzhip_heev_dev": ([vp, vp, u64, u64, u64, vp, vp, u64, vp], i32),
         "dzhip_heev_check": ([vp, vp], i32),
+        "dzhip_heev_dev_noalloc": ([vp, vp, u64, u64, u64, vp, vp, u64, vp], i32),
         "dzhip_hamset_create": ([ct.POINTER(vp), u64], vp), "dzhip_hamset_destroy": ([vp], None),
         "dzhip_hamset_offsets": ([vp, ct.POINTER(ct.c_uint64), u64], i32),
         "dzhip_hamset_apply_dev": ([vp, vp, vp, u64, i32, i32], i32),
@@ -270,6 +271,10 @@ class Context:
     def heev_dev(self, a_ptr, lda, m, nev, w_ptr, z_ptr, ldz, stream=None):
         """Device-pointer form (enqueue only); heev_check() synchronises and reports a failure."""
         check(L().dzhip_heev_dev(self.handle, a_ptr, lda, m, nev, w_ptr, z_ptr, ldz, stream))
+
+    def heev_dev_noalloc(self, a_ptr, lda, m, nev, w_ptr, z_ptr, ldz, stream=None):
+        """The allocation-free form (m <= 256, the library's static scratch; enqueue only)."""
+        check(L().dzhip_heev_dev_noalloc(self.handle, a_ptr, lda, m, nev, w_ptr, z_ptr, ldz, stream))
 
     def heev_check(self, stream=None):
         check(L().dzhip_heev_check(self.handle, stream))

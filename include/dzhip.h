@@ -84,6 +84,13 @@ int dzhip_heev(dzhip_ctx* ctx, const double* a, uint64_t lda, uint64_t m, uint64
 int dzhip_heev_dev(dzhip_ctx* ctx, const void* a, uint64_t lda, uint64_t m, uint64_t nev, void* values,
                    void* vectors, uint64_t ldz, void* stream);
 int dzhip_heev_check(dzhip_ctx* ctx, void* stream);
+/* dzhip_heev_dev without any allocation, for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid it); it has no counterpart in
+ * the reference.  m <= 256: every buffer lives in the library's 6 MiB static device array, one per
+ * device, whose address dzhip_ctx_create takes.  Enqueue only; a tridiagonalisation that did not
+ * complete leaves NaN eigenvalues.  Calls that overlap on one device share the array and race. */
+int dzhip_heev_dev_noalloc(dzhip_ctx* ctx, const void* a, uint64_t lda, uint64_t m, uint64_t nev, void* values,
+                           void* vectors, uint64_t ldz, void* stream);
 
 /* ------------------------------------------------------------------ H·psi
  * Per-k Hamiltonian context: the data ApplyContext + KpointApplyCache carry
