@@ -7,8 +7,9 @@
 //      columns and each reflector costs one grid barrier.
 //   2. The lowest nev eigenvalues of T by 64-way multisection with Sturm counts (one wavefront
 //      per eigenvalue).
-//   3. Eigenvectors of T by inverse iteration (tridiagonal LU with partial pivoting); eigenvalues
-//      closer than a cluster threshold are iterated together with Gram-Schmidt, then two
+//   3. Eigenvectors of T by inverse iteration (tridiagonal LU with partial pivoting in LDS, one
+//      wavefront per eigenvalue); the members of a cluster of eigenvalues closer than a threshold
+//      are then orthonormalised together (two classical Gram-Schmidt passes), and two
 //      Newton-Schulz steps Z <- Z (3 I - Z^T Z) / 2 orthonormalise all nev vectors.
 //   4. Back-transformation Y = Q Z (one wavefront per eigenvector).
 #pragma once
@@ -20,7 +21,7 @@ namespace dzhip {
 constexpr int kHeevMaxM = 1024;
 
 struct HeevWork {
-    DevBuf vh, tau, d, e, pbuf, nbuf, sync, z, zt, gram, fac;
+    DevBuf vh, tau, d, e, pbuf, nbuf, sync, z, zt, gram;
     int err_offset = 0;  // int index of the error flag in sync
     void release();
 };

@@ -15,6 +15,7 @@ namespace {
 constexpr int kTrdThreads = 256;
 constexpr int kTrdColsPerGroup = 6;   // trailing-matrix columns held in each workgroup's LDS
 constexpr int kTrdMaxCols = 16;
+constexpr int kTrdRowsPerThread = (1024 + 255) / 256;  // rows per thread at m <= kHeevMaxM
 constexpr unsigned kSpinLimit = 1u << 22;  // barrier polls (each after s_sleep) before giving up
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -45,21 +46,41 @@ __device__ __forceinline__ void block_sums(double (&vals)[N], double* red, doubl
     }
 }
 
+// Data that one workgroup writes and others read inside the tridiagonalisation (the published
+// p_j and column j+1) goes through relaxed agent-scope atomics: such stores write through the
+// XCD's L2 and such loads do not hit a stale L2 copy, so the barrier below needs no L2 writeback
+// or invalidate (an agent-scope fence costs several microseconds per workgroup), only that every
+// store has completed before the flag goes out.
+__device__ __forceinline__ void xstore(double2* p, double2 v) {
+    double* q = reinterpret_cast<double*>(p);
+    __hip_atomic_store(q, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double2 xload(const double2* p) {
+    const double* q = reinterpret_cast<const double*>(p);
+    return make_double2(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                        __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
 // Barrier over the whole (co-resident) grid.  Each workgroup owns a flag on its own 64-byte line
 // and stores the barrier's epoch there (no read-modify-write on a shared counter, whose atomics
 // serialise: about 70 ns per arriving workgroup); wavefront 0 of every workgroup then polls all
-// flags (64 per load instruction) until each has reached the epoch.  The global writes before
-// the barrier are released at device scope and acquired after it.  A wait longer than
-// kSpinLimit polls (or an error raised elsewhere) sets *err; every thread returns false then and
-// the kernel winds down.
-__device__ bool grid_sync(unsigned* flags, unsigned epoch, int* err, int* s_ok) {
-    __threadfence();
+// flags (64 per load instruction) until each has reached the epoch.  Every thread first waits for
+// its own memory operations to complete (s_waitcnt 0), so the exchanged data written with xstore
+// is in place before the flag is.  With part != nullptr, wavefront 0 then also sums part[0:G]
+// (one complex value per workgroup, written with xstore before the barrier) into sum[0:2].
+// A wait longer than kSpinLimit polls (or an error raised elsewhere) sets *err; every thread
+// returns false then and the kernel winds down.
+__device__ bool grid_sync(unsigned* flags, unsigned epoch, int* err, int* s_ok, const double2* part = nullptr,
+                          double* sum = nullptr) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
     if (threadIdx.x < 64) {
         const int lane = (int)threadIdx.x;
         const int G = (int)gridDim.x;
         if (lane == 0)
-            __hip_atomic_store(flags + (size_t)blockIdx.x * 16, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(flags + (size_t)blockIdx.x * 16, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned spins = 0;
         int ok = 1;
         while (true) {
@@ -75,11 +96,24 @@ __device__ bool grid_sync(unsigned* flags, unsigned epoch, int* err, int* s_ok) 
             }
             __builtin_amdgcn_s_sleep(1);
         }
+        if (part && ok) {
+            double sx = 0.0, sy = 0.0;
+            for (int q = lane; q < G; q += 64) {
+                const double2 v = xload(part + q);
+                sx += v.x;
+                sy += v.y;
+            }
+            sx = wave_sum(sx);
+            sy = wave_sum(sy);
+            if (lane == 0) {
+                sum[0] = sx;
+                sum[1] = sy;
+            }
+        }
         if (lane == 0) *s_ok = ok;
-        __threadfence();
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     __syncthreads();
-    __threadfence();
     return *s_ok != 0;
 }
 
@@ -96,7 +130,7 @@ struct TrdArgs {
     double2* tau;  // m
     double* d;     // m
     double* e;     // m - 1
-    double2* pbuf; // 2 x m: tau_j A v_j, double-buffered by step parity
+    double2* pbuf; // 2 x m: tau_j A v_j, then 2 x G: per-workgroup parts of p_j^H v_j; by step parity
     double2* nbuf; // 2 x m: the next column after the previous rank-2 update
     unsigned* flags;  // one per workgroup, 16 apart
     int* err;
@@ -104,26 +138,31 @@ struct TrdArgs {
 
 // Householder tridiagonalisation A = Q T Q^H, Q = H_0 ... H_{m-2}, H_j = I - tau_j v_j v_j^H, with
 // zhetd2's reflectors (zlarfg: H_j^H x = beta e_1, beta real).  Workgroup g holds the trailing
-// columns i = g + c G (c < ncl) in LDS for the whole reduction.  Step j:
-//   A. every workgroup forms v_j, tau_j from the current column j (in LDS);
+// columns i = g + c G (c < NCL) in LDS for the whole reduction, and every workgroup keeps its own
+// copy of the current column.  Step j (thread t works on rows r = j+1+t (mod 256) throughout, so
+// that its phases need no workgroup barrier between them):
+//   A. every workgroup forms v_j, tau_j from the current column j (norm of rows j+2.. summed at
+//      the end of the previous step);
 //   B. fused pass over its own columns i > j: the rank-2 update of step j-1
 //      a_i -= v_{j-1} conj(w_{j-1}[i]) + w_{j-1} conj(v_{j-1}[i]), then p_j[i] = tau_j a_i^H v_j
-//      (A is Hermitian, so (A v)_i = a_i^H v); the owner of column j+1 publishes it (updated);
-//   C. grid barrier;
+//      (A is Hermitian, so (A v)_i = a_i^H v); the owner of column j+1 publishes it (updated), and
+//      every workgroup publishes its part of p_j^H v_j;
+//   C. grid barrier, which also sums those parts;
 //   D. w_j = p_j - (tau_j / 2) (p_j^H v_j) v_j, from the published p_j;
-//   E. column j+1 = published column - v_j conj(w_j[j+1]) - w_j conj(v_j[j+1]).
+//   E. column j+1 = published column - v_j conj(w_j[j+1]) - w_j conj(v_j[j+1]), and the norm of
+//      its rows j+3.. for the next step.
 // The owner never updates column j+1 in place at step j, so the published copy stays valid while
-// the others read it; pbuf and nbuf alternate by step parity.
-// NCL (columns per workgroup) is a template parameter so the per-column partial sums stay in
-// registers.
+// the others read it; the published buffers alternate by step parity.
 template <int NCL>
 __global__ __launch_bounds__(kTrdThreads) void k_hetrd(TrdArgs a) {
     extern __shared__ double2 lds[];
     __shared__ double red[(kTrdThreads / 64) * 2 * kTrdMaxCols];
+    __shared__ double s_dot[2];
     __shared__ int s_ok;
     constexpr int ncl = NCL;
-    const int m = a.m, G = (int)gridDim.x, g = (int)blockIdx.x, t = (int)threadIdx.x;
     constexpr int NT = kTrdThreads;
+    const int m = a.m, G = (int)gridDim.x, g = (int)blockIdx.x, t = (int)threadIdx.x;
+    const int wv = t >> 6, lane = t & 63;
     double2* cols = lds;
     // Roles, rotated every step: vc = current column (becomes v_j in place), wn = w_j,
     // vp / wp = v_{j-1}, w_{j-1}.
@@ -149,15 +188,22 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd(TrdArgs a) {
     }
     for (int r = t; r < m; r += NT) vc_b[r] = r == 0 ? make_double2(a.A[0].x, 0.0) : a.A[r];
     __syncthreads();
+    double xs = 0.0;  // this thread's part of |column j, rows j+2..|^2
+    for (int r = 1 + t; r < m; r += NT)
+        if (r >= 2) xs += vc_b[r].x * vc_b[r].x + vc_b[r].y * vc_b[r].y;
     for (int j = 0; j + 1 < m; ++j) {
         double2* vc = vc_b;
-        // A. reflector of column j (rows j+1..m-1).
-        double xs[1] = {0.0};
-        for (int r = j + 2 + t; r < m; r += NT) xs[0] += vc[r].x * vc[r].x + vc[r].y * vc[r].y;
-        double xo[1];
-        block_sums<NT>(xs, red, xo);
-        const double xn2 = xo[0];
+        // A. reflector of column j (rows j+1..m-1).  Every thread reads alpha between the two
+        // barriers; thread 0 overwrites its row only after the second.
+        xs = wave_sum(xs);
+        __syncthreads();
         const double2 alpha = vc[j + 1];
+        const double dj = vc[j].x;
+        if (lane == 0) red[wv] = xs;
+        __syncthreads();
+        double xn2 = 0.0;
+#pragma unroll
+        for (int q = 0; q < NT / 64; ++q) xn2 += red[q];
         double beta;
         double2 tau, scale;
         if (xn2 == 0.0 && alpha.y == 0.0) {
@@ -173,89 +219,127 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd(TrdArgs a) {
             scale = make_double2(den.x / dd, -den.y / dd);
         }
         if (g == 0 && t == 0) {
-            a.d[j] = vc[j].x;
+            a.d[j] = dj;
             a.e[j] = beta;
             a.tau[j] = tau;
         }
-        __syncthreads();  // every thread has read alpha before vc[j + 1] becomes 1
-        for (int r = j + 2 + t; r < m; r += NT) vc[r] = cm(vc[r], scale);
-        if (t == 0) vc[j + 1] = make_double2(1.0, 0.0);
-        __syncthreads();
-        if (g == 0)
-            for (int r = j + 1 + t; r < m; r += NT) a.vh[(size_t)j * m + r] = vc[r];
+        for (int r = j + 1 + t; r < m; r += NT) {
+            const double2 v = r == j + 1 ? make_double2(1.0, 0.0) : cm(vc[r], scale);
+            vc[r] = v;
+            if (g == 0) a.vh[(size_t)j * m + r] = v;
+        }
         // B. fused update (step j-1) and product (step j) over the own columns i >= j+1.
         const double2* vp = vp_b;
         const double2* wp = wp_b;
         double part[2 * ncl];
-#pragma unroll
-        for (int c = 0; c < 2 * ncl; ++c) part[c] = 0.0;
-        double2* nb_out = a.nbuf + (size_t)((j + 1) & 1) * m;
+        double2 wi[ncl], vi[ncl];
 #pragma unroll
         for (int c = 0; c < ncl; ++c) {
             const int i = g + c * G;
-            if (i <= j || i >= m) continue;
-            double2* col = cols + (size_t)c * m;
-            const double2 wi = j > 0 ? wp[i] : make_double2(0.0, 0.0);
-            const double2 vi = j > 0 ? vp[i] : make_double2(0.0, 0.0);
-            double2 acc = make_double2(0.0, 0.0);
-            for (int r = j + 1 + t; r < m; r += NT) {
+            const bool live = j > 0 && i > j && i < m;
+            wi[c] = live ? wp[i] : make_double2(0.0, 0.0);
+            vi[c] = live ? vp[i] : make_double2(0.0, 0.0);
+            part[2 * c] = 0.0;
+            part[2 * c + 1] = 0.0;
+        }
+        double2* nb_out = a.nbuf + (size_t)((j + 1) & 1) * m;
+        for (int r = j + 1 + t; r < m; r += NT) {
+            const double2 vr = vc[r];
+            const double2 vpr = j > 0 ? vp[r] : make_double2(0.0, 0.0);
+            const double2 wpr = j > 0 ? wp[r] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int c = 0; c < ncl; ++c) {
+                const int i = g + c * G;
+                if (i <= j || i >= m) continue;
+                double2* col = cols + (size_t)c * m;
                 double2 x = col[r];
                 if (j > 0) {
                     // x -= vp[r] conj(wi) + wp[r] conj(vi)
-                    const double2 u1 = cjm(wi, vp[r]);
-                    const double2 u2 = cjm(vi, wp[r]);
+                    const double2 u1 = cjm(wi[c], vpr);
+                    const double2 u2 = cjm(vi[c], wpr);
                     x.x -= u1.x + u2.x;
                     x.y -= u1.y + u2.y;
                     col[r] = x;
                 }
-                const double2 pr = cjm(x, vc[r]);
-                acc.x += pr.x;
-                acc.y += pr.y;
-                if (i == j + 1) nb_out[r] = x;
+                const double2 pr = cjm(x, vr);
+                part[2 * c] += pr.x;
+                part[2 * c + 1] += pr.y;
+                if (i == j + 1) xstore(nb_out + r, x);
             }
-            part[2 * c] = acc.x;
-            part[2 * c + 1] = acc.y;
         }
         double sums[2 * ncl];
         block_sums<NT>(part, red, sums);
         double2* pb = a.pbuf + (size_t)(j & 1) * m;
+        double2* dpart = a.pbuf + (size_t)2 * m + (size_t)(j & 1) * G;
+        double2 pdot = make_double2(0.0, 0.0);
 #pragma unroll
         for (int c = 0; c < ncl; ++c) {
             const int i = g + c * G;
-            if (t == c && i > j && i < m) pb[i] = cm(tau, make_double2(sums[2 * c], sums[2 * c + 1]));
+            if (i > j && i < m) {
+                const double2 p = cm(tau, make_double2(sums[2 * c], sums[2 * c + 1]));
+                if (t == c) xstore(pb + i, p);
+                const double2 u = cjm(p, vc[i]);
+                pdot.x += u.x;
+                pdot.y += u.y;
+            }
         }
+        if (t == 0) xstore(dpart + g, pdot);
         // C.
         if (!grid_sync(a.flags, (unsigned)(j + 1), a.err, &s_ok)) return;
-        // D. w_j = p_j - (tau_j / 2) (p_j^H v_j) v_j.
-        double2* wn = wn_b;
-        double ps[2] = {0.0, 0.0};
-        for (int r = j + 1 + t; r < m; r += NT) {
-            const double2 p = pb[r];
-            wn[r] = p;
-            const double2 pr = cjm(p, vc[r]);
-            ps[0] += pr.x;
-            ps[1] += pr.y;
+        // Every published value this thread needs, loaded together: p_j and column j+1 on its own
+        // rows, p_j[j+1], and (wavefront 0) the parts of p_j^H v_j.
+        const double2* nb_in = a.nbuf + (size_t)((j + 1) & 1) * m;
+        double2 pr_in[kTrdRowsPerThread], nr_in[kTrdRowsPerThread];
+#pragma unroll
+        for (int q = 0; q < kTrdRowsPerThread; ++q) {
+            const int r = j + 1 + t + q * NT;
+            if (r < m) {
+                pr_in[q] = xload(pb + r);
+                nr_in[q] = xload(nb_in + r);
+            }
         }
-        double s2[2];
-        block_sums<NT>(ps, red, s2);
-        const double2 al = cm(make_double2(-0.5 * tau.x, -0.5 * tau.y), make_double2(s2[0], s2[1]));
-        for (int r = j + 1 + t; r < m; r += NT) {
-            const double2 u = cm(al, vc[r]);
-            wn[r].x += u.x;
-            wn[r].y += u.y;
+        const double2 pj1 = xload(pb + j + 1);
+        if (wv == 0) {
+            double sx = 0.0, sy = 0.0;
+            for (int q = lane; q < G; q += 64) {
+                const double2 v = xload(dpart + q);
+                sx += v.x;
+                sy += v.y;
+            }
+            sx = wave_sum(sx);
+            sy = wave_sum(sy);
+            if (lane == 0) {
+                s_dot[0] = sx;
+                s_dot[1] = sy;
+            }
         }
         __syncthreads();
+        // D. w_j = p_j - (tau_j / 2) (p_j^H v_j) v_j on the own rows; row j+1 by every thread.
+        double2* wn = wn_b;
+        const double2 al = cm(make_double2(-0.5 * tau.x, -0.5 * tau.y), make_double2(s_dot[0], s_dot[1]));
+#pragma unroll
+        for (int q = 0; q < kTrdRowsPerThread; ++q) {
+            const int r = j + 1 + t + q * NT;
+            if (r < m) {
+                const double2 u = cm(al, vc[r]);
+                wn[r] = make_double2(pr_in[q].x + u.x, pr_in[q].y + u.y);
+            }
+        }
+        const double2 wj1 = make_double2(pj1.x + al.x, pj1.y + al.y);  // v_j[j+1] = 1
         // E. column j+1 after the step-j update, into the buffer of v_{j-1} (no longer needed).
         double2* vnext = vp_b;
-        const double2* nb_in = a.nbuf + (size_t)((j + 1) & 1) * m;
-        const double2 wj1 = wn[j + 1], vj1 = vc[j + 1];
-        for (int r = j + 1 + t; r < m; r += NT) {
-            const double2 x = nb_in[r];
+        xs = 0.0;
+#pragma unroll
+        for (int q = 0; q < kTrdRowsPerThread; ++q) {
+            const int r = j + 1 + t + q * NT;
+            if (r >= m) continue;
+            const double2 x = nr_in[q];
             const double2 u1 = cjm(wj1, vc[r]);
-            const double2 u2 = cjm(vj1, wn[r]);
-            vnext[r] = make_double2(x.x - u1.x - u2.x, x.y - u1.y - u2.y);
+            const double2 w = wn[r];  // conj(v_j[j+1]) w_j[r] with v_j[j+1] = 1
+            const double2 y = make_double2(x.x - u1.x - w.x, x.y - u1.y - w.y);
+            vnext[r] = y;
+            if (r >= j + 3) xs += y.x * y.x + y.y * y.y;
         }
-        __syncthreads();
         if (j + 2 == m && g == 0 && t == 0) a.d[m - 1] = vnext[m - 1].x;
         // Rotate: v_{j-1} <- v_j, w_{j-1} <- w_j, current column <- column j+1.
         double2* const old_vp = vp_b;
@@ -273,24 +357,32 @@ __global__ void k_zero_u32(unsigned* p, int n) {
     if (i < n) p[i] = 0u;
 }
 
-// Sturm count: number of eigenvalues of T(d, e) below x (e2 = e^2), with tiny pivots replaced
-// by -pivmin (LAPACK dlaebz's rule).
-__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int m, double x, double pivmin) {
-    double q = d[0] - x;
-    int cnt = 0;
-    if (fabs(q) < pivmin) q = -pivmin;
-    if (q < 0.0) ++cnt;
-    for (int k = 1; k < m; ++k) {
-        q = d[k] - x - e2[k - 1] / q;
-        if (fabs(q) < pivmin) q = -pivmin;
-        if (q < 0.0) ++cnt;
+// Sturm counts: numbers of eigenvalues of T(d, e) below x[0..NP) (e2 = e^2), with tiny pivots
+// replaced by -pivmin (LAPACK dlaebz's rule).  The NP recurrences are independent and interleaved.
+template <int NP>
+__device__ __forceinline__ void sturm_counts(const double* d, const double* e2, int m, const double (&x)[NP],
+                                             double pivmin, int (&cnt)[NP]) {
+    double q[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        q[p] = d[0] - x[p];
+        if (fabs(q[p]) < pivmin) q[p] = -pivmin;
+        cnt[p] = q[p] < 0.0 ? 1 : 0;
     }
-    return cnt;
+    for (int k = 1; k < m; ++k) {
+        const double dk = d[k], ek = e2[k - 1];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            q[p] = dk - x[p] - ek / q[p];
+            if (fabs(q[p]) < pivmin) q[p] = -pivmin;
+            cnt[p] += q[p] < 0.0 ? 1 : 0;
+        }
+    }
 }
 
 // Eigenvalue idx (ascending) of T by multisection: each round the 64 lanes of a wavefront count
-// at 64 interior points of [lo, hi] and keep the sub-interval that holds eigenvalue idx.
-// Also writes the Gershgorin scale of T to scal[0] (block 0).
+// at NP points each (64 NP interior points of [lo, hi]) and keep the sub-interval that holds
+// eigenvalue idx.  Also writes the Gershgorin scale of T to scal[0] (block 0).
 __global__ __launch_bounds__(256) void k_stebz(const double* __restrict__ d_in, const double* __restrict__ e_in, int m,
                                                int nev, double* __restrict__ w, double* __restrict__ scal,
                                                const int* __restrict__ err) {
@@ -345,21 +437,30 @@ __global__ __launch_bounds__(256) void k_stebz(const double* __restrict__ d_in, 
         if (lane == 0) w[idx] = __builtin_nan("");
         return;
     }
+    constexpr int NP = 1;  // more points per lane lose: the FP64 divisions bound the issue rate
+    constexpr double kParts = 64.0 * NP + 1.0;
     double lo = gl, hi = gu;
     for (int round = 0; round < 14; ++round) {
         const double width = hi - lo;
         if (width <= 2.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)) + 2.0 * pivmin) break;
-        const double x = lo + width * (double)(lane + 1) / 65.0;
-        const int cnt = sturm_count(d, e2, m, x, pivmin);
-        const unsigned long long below = __ballot(cnt <= idx);
-        // Highest lane whose point still has eigenvalue idx at or above it.
+        double x[NP];
+        int cnt[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) x[p] = lo + width * (double)(lane * NP + p + 1) / kParts;
+        sturm_counts<NP>(d, e2, m, x, pivmin, cnt);
+        // Highest point (lane * NP + p) that still has eigenvalue idx at or above it; -1 if none.
+        int L = -1;
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+            if (cnt[p] <= idx) L = lane * NP + p;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) L = max(L, __shfl_xor(L, o, 64));
         double nlo = lo, nhi = hi;
-        if (below == 0ull) {
-            nhi = lo + width * 1.0 / 65.0;
+        if (L < 0) {
+            nhi = lo + width * 1.0 / kParts;
         } else {
-            const int L = 63 - __clzll(below);
-            nlo = lo + width * (double)(L + 1) / 65.0;
-            if (L < 63) nhi = lo + width * (double)(L + 2) / 65.0;
+            nlo = lo + width * (double)(L + 1) / kParts;
+            if (L < 64 * NP - 1) nhi = lo + width * (double)(L + 2) / kParts;
         }
         lo = nlo;
         hi = nhi;
@@ -367,93 +468,167 @@ __global__ __launch_bounds__(256) void k_stebz(const double* __restrict__ d_in, 
     if (lane == 0) w[idx] = 0.5 * (lo + hi);
 }
 
-// Eigenvectors of T by inverse iteration, one thread per cluster of eigenvalues closer than
-// tau_c (consecutive gaps).  Members are iterated in order, each (T - lambda I) solve followed by
-// Gram-Schmidt against the cluster's earlier members; three iterations from a seeded start.
-// fac: 5 m doubles per eigenvalue (LU of T - lambda I with partial pivoting, as dlagtf).
-__global__ __launch_bounds__(64) void k_stein(const double* __restrict__ d, const double* __restrict__ e, int m,
+// Eigenvalues closer than this (consecutive gaps, relative to the Gershgorin scale) form a cluster
+// whose inverse-iteration vectors are orthonormalised together.
+__device__ __forceinline__ double cluster_tol(double bnorm) { return 1e-12 * fmax(bnorm, 1.0); }
+
+// Eigenvector of T for eigenvalue k by inverse iteration, one wavefront per eigenvalue (all
+// independent).  T - lambda I = P L U with partial pivoting (dlagtf's factorisation, U with two
+// superdiagonals; pivots below eps |T| are raised to that size), then three solves from a seeded
+// start, each followed by scaling and normalisation.  T, the factors and the iterate live in LDS;
+// the recurrences run on lane 0, the rest on all lanes.  LDS: 7 m doubles + m bytes.
+__global__ __launch_bounds__(64) void k_stein(const double* __restrict__ d_in, const double* __restrict__ e_in, int m,
                                               int nev, const double* __restrict__ w, const double* __restrict__ scal,
-                                              double* __restrict__ Z, int ldz, double* __restrict__ fac) {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= nev) return;
-    const double bnorm = fmax(scal[0], 1e-300);
-    const double tau_c = 1e-12 * fmax(bnorm, 1.0);
-    if (i > 0 && w[i] - w[i - 1] <= tau_c) return;
-    int last = i;
-    while (last + 1 < nev && w[last + 1] - w[last] <= tau_c) ++last;
-    double* u0 = fac + (size_t)i * 5 * m;
-    double* u1 = u0 + m;
+                                              double* __restrict__ Z, int ldz) {
+    extern __shared__ double sh[];
+    const int k = (int)blockIdx.x, lane = (int)threadIdx.x;
+    if (k >= nev) return;
+    double* d = sh;
+    double* e = d + m;
+    double* u0i = e + m;  // 1 / u0
+    double* u1 = u0i + m;
     double* u2 = u1 + m;
     double* lm = u2 + m;
-    double* pv = lm + m;
+    double* x = lm + m;
+    unsigned char* pv = reinterpret_cast<unsigned char*>(x + m);
+    for (int r = lane; r < m; r += 64) {
+        d[r] = d_in[r];
+        e[r] = r + 1 < m ? e_in[r] : 0.0;
+        // Seeded start (splitmix64 of eigenvalue and row).
+        unsigned long long h = 0x9E3779B97F4A7C15ull * (unsigned long long)(k + 1) + (unsigned long long)r;
+        h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+        h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+        h ^= h >> 31;
+        x[r] = (double)(h >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    __syncthreads();
+    const double bnorm = fmax(scal[0], 1e-300);
     const double pert = 2.220446049250313e-16 * bnorm + 1e-300;
-    for (int k = i; k <= last; ++k) {
-        const double lam = w[k];
-        // Factor T - lam I = P L U (dlagtf): U has two superdiagonals.
+    const double lam = w[k];
+    if (lane == 0) {
         double ca = d[0] - lam, cb = m > 1 ? e[0] : 0.0;
         for (int r = 0; r + 1 < m; ++r) {
             const double c = e[r];
             const double na = d[r + 1] - lam;
-            const double nb = r + 2 < m ? e[r + 1] : 0.0;
+            const double nb = e[r + 1];
+            double u0;
             if (fabs(ca) >= fabs(c)) {
                 const double l = ca != 0.0 ? c / ca : 0.0;
-                u0[r] = ca;
+                u0 = ca;
                 u1[r] = cb;
                 u2[r] = 0.0;
                 lm[r] = l;
-                pv[r] = 0.0;
+                pv[r] = 0;
                 ca = na - l * cb;
                 cb = nb;
             } else {
                 const double l = ca / c;
-                u0[r] = c;
+                u0 = c;
                 u1[r] = na;
                 u2[r] = nb;
                 lm[r] = l;
-                pv[r] = 1.0;
+                pv[r] = 1;
                 ca = cb - l * na;
                 cb = -l * nb;
             }
+            if (fabs(u0) < pert) u0 = u0 < 0.0 ? -pert : pert;
+            u0i[r] = 1.0 / u0;
         }
-        u0[m - 1] = ca;
-        for (int r = 0; r < m; ++r)
-            if (fabs(u0[r]) < pert) u0[r] = u0[r] < 0.0 ? -pert : pert;
-        double* x = Z + (size_t)k * ldz;
-        unsigned long long s = 0x9E3779B97F4A7C15ull * (unsigned long long)(k + 1) + 12345ull;
-        for (int r = 0; r < m; ++r) {
-            s = s * 6364136223846793005ull + 1442695040888963407ull;
-            x[r] = (double)(s >> 11) * (1.0 / 9007199254740992.0) - 0.5;
-        }
-        for (int it = 0; it < 3; ++it) {
-            // Solve P L U y = x in place.
+        if (fabs(ca) < pert) ca = ca < 0.0 ? -pert : pert;
+        u0i[m - 1] = 1.0 / ca;
+    }
+    __syncthreads();
+    for (int it = 0; it < 3; ++it) {
+        if (lane == 0) {
+            // P L y = x, then U z = y, in place.
+            double xr = x[0];
             for (int r = 0; r + 1 < m; ++r) {
-                if (pv[r] != 0.0) {
-                    const double tmp = x[r];
-                    x[r] = x[r + 1];
-                    x[r + 1] = tmp;
+                double xn = x[r + 1];
+                if (pv[r]) {
+                    const double tmp = xr;
+                    xr = xn;
+                    xn = tmp;
                 }
-                x[r + 1] -= lm[r] * x[r];
+                x[r] = xr;
+                xr = xn - lm[r] * xr;
             }
-            x[m - 1] /= u0[m - 1];
-            if (m > 1) x[m - 2] = (x[m - 2] - u1[m - 2] * x[m - 1]) / u0[m - 2];
-            for (int r = m - 3; r >= 0; --r) x[r] = (x[r] - u1[r] * x[r + 1] - u2[r] * x[r + 2]) / u0[r];
-            // Scale down before Gram-Schmidt (the solve amplifies by up to 1 / pert).
-            double mx = 0.0;
-            for (int r = 0; r < m; ++r) mx = fmax(mx, fabs(x[r]));
-            const double inv_mx = mx > 0.0 ? 1.0 / mx : 1.0;
-            for (int r = 0; r < m; ++r) x[r] *= inv_mx;
-            for (int p = i; p < k; ++p) {
-                const double* zp = Z + (size_t)p * ldz;
-                double dot = 0.0;
-                for (int r = 0; r < m; ++r) dot += zp[r] * x[r];
-                for (int r = 0; r < m; ++r) x[r] -= dot * zp[r];
+            x[m - 1] = xr;
+            double z1 = xr * u0i[m - 1];
+            x[m - 1] = z1;
+            if (m > 1) {
+                double z0 = (x[m - 2] - u1[m - 2] * z1) * u0i[m - 2];
+                x[m - 2] = z0;
+                for (int r = m - 3; r >= 0; --r) {
+                    const double zr = (x[r] - u1[r] * z0 - u2[r] * z1) * u0i[r];
+                    x[r] = zr;
+                    z1 = z0;
+                    z0 = zr;
+                }
             }
-            double nrm = 0.0;
-            for (int r = 0; r < m; ++r) nrm += x[r] * x[r];
-            nrm = sqrt(nrm);
-            const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
-            for (int r = 0; r < m; ++r) x[r] *= inv;
         }
+        __syncthreads();
+        // Scale by the largest entry (the solve amplifies by up to 1 / pert), then normalise.
+        double mx = 0.0;
+        for (int r = lane; r < m; r += 64) mx = fmax(mx, fabs(x[r]));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+        const double inv_mx = mx > 0.0 ? 1.0 / mx : 1.0;
+        double s2 = 0.0;
+        for (int r = lane; r < m; r += 64) {
+            const double v = x[r] * inv_mx;
+            s2 += v * v;
+        }
+        s2 = wave_sum(s2);
+        const double sc = inv_mx / sqrt(fmax(s2, 1e-300));
+        for (int r = lane; r < m; r += 64) x[r] *= sc;
+        __syncthreads();
+    }
+    for (int r = lane; r < m; r += 64) Z[(size_t)k * ldz + r] = x[r];
+}
+
+// Members of each cluster (consecutive eigenvalues closer than cluster_tol), in order, made
+// orthonormal against the cluster's earlier members by two passes of classical Gram-Schmidt (one
+// workgroup per cluster; the first member leads).  Inverse iteration gives each member a vector in
+// the cluster's invariant subspace, so the orthonormal result spans the same subspace.
+__global__ __launch_bounds__(256) void k_stein_cluster(const double* __restrict__ w, const double* __restrict__ scal,
+                                                       int nev, int m, double* __restrict__ Z, int ldz) {
+    __shared__ double dots[kHeevMaxM];
+    __shared__ double red[4];
+    const int i = (int)blockIdx.x;
+    if (i >= nev) return;
+    const double tau_c = cluster_tol(scal[0]);
+    if (i > 0 && w[i] - w[i - 1] <= tau_c) return;
+    int last = i;
+    while (last + 1 < nev && w[last + 1] - w[last] <= tau_c) ++last;
+    if (last == i) return;
+    const int t = (int)threadIdx.x, wv = t >> 6, lane = t & 63;
+    for (int k = i + 1; k <= last; ++k) {
+        double* x = Z + (size_t)k * ldz;
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int p = i + wv; p < k; p += 4) {
+                const double* zp = Z + (size_t)p * ldz;
+                double sdot = 0.0;
+                for (int r = lane; r < m; r += 64) sdot += zp[r] * x[r];
+                sdot = wave_sum(sdot);
+                if (lane == 0) dots[p - i] = sdot;
+            }
+            __syncthreads();
+            for (int r = t; r < m; r += 256) {
+                double acc = x[r];
+                for (int p = i; p < k; ++p) acc -= dots[p - i] * Z[(size_t)p * ldz + r];
+                x[r] = acc;
+            }
+            __syncthreads();
+        }
+        double s2 = 0.0;
+        for (int r = t; r < m; r += 256) s2 += x[r] * x[r];
+        s2 = wave_sum(s2);
+        if (lane == 0) red[wv] = s2;
+        __syncthreads();
+        const double nrm = sqrt(red[0] + red[1] + red[2] + red[3]);
+        const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+        for (int r = t; r < m; r += 256) x[r] *= inv;
+        __syncthreads();
     }
 }
 
@@ -492,58 +667,128 @@ __global__ __launch_bounds__(256) void k_ns_update(const double* __restrict__ Z,
     out[(size_t)j * ldz + r] = acc;
 }
 
-// Y[:, col] = Q Z[:, col] = H_0 (H_1 (... H_{m-2} z)), one wavefront per column, the column held in
-// registers (lane l holds rows l + 64 q).
-constexpr int kUnmtrRegs = kHeevMaxM / 64;
+// Y[:, cols] = Q Z[:, cols] = H_0 (H_1 (... H_{m-2} Z)).  One workgroup per kUnmtrWgCols columns,
+// each wavefront holding kUnmtrWaveCols of them in registers (lane l: rows l + 64 q).  The
+// reflectors are staged through LDS in blocks of unmtr_block(NQ) (rows <= k zero, row k+1 one), and the
+// next block's global loads are issued into registers before the current block is applied.
+// NQ: rows per lane (m <= 64 NQ).
+constexpr int kUnmtrWaveCols = 2;
+constexpr int kUnmtrWgCols = 4 * kUnmtrWaveCols;
+// Reflectors per LDS stage (fewer for the larger register sets).
+constexpr int unmtr_block(int nq) { return nq <= 8 ? 8 : 4; }
 
+template <int NQ>
 __global__ __launch_bounds__(256) void k_unmtr(const double2* __restrict__ vh, const double2* __restrict__ tau, int m,
                                                int nev, const double* __restrict__ Z, int ldz, double2* __restrict__ Y,
                                                int ldy) {
-    const int col = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int lane = (int)(threadIdx.x & 63);
-    if (col >= nev) return;
-    double2 y[kUnmtrRegs];
+    extern __shared__ double2 vs[];  // kUnmtrBlk x m
+    __shared__ double2 ts[8];
+    const int t = (int)threadIdx.x, wv = t >> 6, lane = t & 63;
+    constexpr int kUnmtrRegs = NQ;
+    constexpr int kUnmtrBlk = unmtr_block(NQ);
+    constexpr int kUnmtrPre = kUnmtrBlk * 64 * NQ / 256;  // staged values per thread
+    const int col0 = (int)blockIdx.x * kUnmtrWgCols + wv * kUnmtrWaveCols;
+    double2 y[kUnmtrWaveCols][kUnmtrRegs];
 #pragma unroll
-    for (int q = 0; q < kUnmtrRegs; ++q) {
-        const int r = lane + 64 * q;
-        y[q] = make_double2(r < m ? Z[(size_t)col * ldz + r] : 0.0, 0.0);
-    }
-    const int nq = (m + 63) / 64;
-    for (int k = m - 2; k >= 0; --k) {
-        const double2 tk = tau[k];
-        if (tk.x == 0.0 && tk.y == 0.0) continue;
-        const double2* v = vh + (size_t)k * m;
-        double sr = 0.0, si = 0.0;
-        double2 vr[kUnmtrRegs];
+    for (int c = 0; c < kUnmtrWaveCols; ++c)
 #pragma unroll
         for (int q = 0; q < kUnmtrRegs; ++q) {
             const int r = lane + 64 * q;
-            vr[q] = make_double2(0.0, 0.0);
-            if (q < nq && r > k && r < m) {
-                vr[q] = r == k + 1 ? make_double2(1.0, 0.0) : v[r];
-                const double2 p = cjm(vr[q], y[q]);
-                sr += p.x;
-                si += p.y;
+            y[c][q] = make_double2(r < m && col0 + c < nev ? Z[(size_t)(col0 + c) * ldz + r] : 0.0, 0.0);
+        }
+    // Block b holds reflectors klo(b) .. khi(b), applied from khi down; blocks from the bottom up.
+    double2 pre[kUnmtrPre];
+    auto fetch = [&](int khi) {
+        const int klo = khi - kUnmtrBlk + 1 < 0 ? 0 : khi - kUnmtrBlk + 1;
+        const int cnt = (khi - klo + 1) * m;
+#pragma unroll
+        for (int q = 0; q < kUnmtrPre; ++q) {
+            const int idx = t + 256 * q;
+            double2 v = make_double2(0.0, 0.0);
+            if (idx < cnt) {
+                const int sft = idx / m, r = idx - sft * m, k = klo + sft;
+                if (r == k + 1) v = make_double2(1.0, 0.0);
+                else if (r > k + 1) v = vh[(size_t)k * m + r];
+            }
+            pre[q] = v;
+        }
+    };
+    if (m >= 2) fetch(m - 2);
+    for (int khi = m - 2; khi >= 0; khi -= kUnmtrBlk) {
+        const int klo = khi - kUnmtrBlk + 1 < 0 ? 0 : khi - kUnmtrBlk + 1;
+        const int nk = khi - klo + 1;
+        __syncthreads();  // the previous block is consumed
+#pragma unroll
+        for (int q = 0; q < kUnmtrPre; ++q) {
+            const int idx = t + 256 * q;
+            if (idx < nk * m) vs[idx] = pre[q];
+        }
+        if (t < nk) ts[t] = tau[klo + t];
+        __syncthreads();
+        if (klo > 0) fetch(klo - 1);
+        for (int sft = nk - 1; sft >= 0; --sft) {
+            const int k = klo + sft;
+            const double2 tk = ts[sft];
+            if (tk.x == 0.0 && tk.y == 0.0) continue;
+            const double2* v = vs + (size_t)sft * m;
+            double acc[2 * kUnmtrWaveCols];
+#pragma unroll
+            for (int c = 0; c < 2 * kUnmtrWaveCols; ++c) acc[c] = 0.0;
+#pragma unroll
+            for (int q = 0; q < kUnmtrRegs; ++q) {
+                const int r = lane + 64 * q;
+                if (r < m && r > k) {
+                    const double2 vr = v[r];
+#pragma unroll
+                    for (int c = 0; c < kUnmtrWaveCols; ++c) {
+                        const double2 p = cjm(vr, y[c][q]);
+                        acc[2 * c] += p.x;
+                        acc[2 * c + 1] += p.y;
+                    }
+                }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+                for (int c = 0; c < 2 * kUnmtrWaveCols; ++c) acc[c] += __shfl_xor(acc[c], o, 64);
+            double2 tsc[kUnmtrWaveCols];
+#pragma unroll
+            for (int c = 0; c < kUnmtrWaveCols; ++c) tsc[c] = cm(tk, make_double2(acc[2 * c], acc[2 * c + 1]));
+#pragma unroll
+            for (int q = 0; q < kUnmtrRegs; ++q) {
+                const int r = lane + 64 * q;
+                if (r < m && r > k) {
+                    const double2 vr = v[r];
+#pragma unroll
+                    for (int c = 0; c < kUnmtrWaveCols; ++c) {
+                        const double2 u = cm(vr, tsc[c]);
+                        y[c][q].x -= u.x;
+                        y[c][q].y -= u.y;
+                    }
+                }
             }
         }
-        sr = wave_sum(sr);
-        si = wave_sum(si);
-        const double2 ts = cm(tk, make_double2(sr, si));
+    }
+#pragma unroll
+    for (int c = 0; c < kUnmtrWaveCols; ++c)
 #pragma unroll
         for (int q = 0; q < kUnmtrRegs; ++q) {
-            const double2 u = cm(vr[q], ts);
-            y[q].x -= u.x;
-            y[q].y -= u.y;
+            const int r = lane + 64 * q;
+            if (r < m && col0 + c < nev) Y[(size_t)(col0 + c) * ldy + r] = y[c][q];
         }
-    }
-#pragma unroll
-    for (int q = 0; q < kUnmtrRegs; ++q) {
-        const int r = lane + 64 * q;
-        if (r < m) Y[(size_t)col * ldy + r] = y[q];
-    }
+}
+
+size_t unmtr_lds_bytes(int m) { return (size_t)unmtr_block((m + 63) / 64) * m * sizeof(double2); }
+
+const void* unmtr_kernel(int m) {
+    if (m <= 256) return (const void*)k_unmtr<4>;
+    if (m <= 512) return (const void*)k_unmtr<8>;
+    if (m <= 768) return (const void*)k_unmtr<12>;
+    return (const void*)k_unmtr<16>;
 }
 
 size_t trd_lds_bytes(int m, int ncl) { return (size_t)(ncl + 4) * (size_t)m * sizeof(double2); }
+size_t stein_lds_bytes(int m) { return (size_t)7 * m * sizeof(double) + (size_t)m; }
 
 // The tridiagonalisation kernel for ncl columns per workgroup (1..kTrdMaxCols).
 const void* trd_kernel(int ncl) {
@@ -574,6 +819,8 @@ CoopInfo coop_info(int dev) {
         DZ_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
         DZ_REQUIRE(coop != 0, DZHIP_ERR_HIP, "device does not support cooperative launches");
         for (int c = 1; c <= kTrdMaxCols; ++c) set_max_dynamic_lds(trd_kernel(c), kMaxLdsBytes - 2048);
+        set_max_dynamic_lds((const void*)k_stein, stein_lds_bytes(kHeevMaxM));
+        for (int mm : {256, 512, 768, kHeevMaxM}) set_max_dynamic_lds(unmtr_kernel(mm), unmtr_lds_bytes(kHeevMaxM));
         // Launch-shape override for sweeps: trailing columns per workgroup.
         if (const char* env = std::getenv("DZHIP_HEEV_COLS")) c.cols_per_group = std::max(0, std::atoi(env));
         c.groups = cus;
@@ -608,7 +855,7 @@ void trd_shape(int m, int* G_out, int* ncl_out) {
 // Device buffers of one eigensolver call (from a HeevWork, or carved from the static scratch array).
 struct HeevBufs {
     double2 *vh, *tau, *pbuf, *nbuf;
-    double *d, *e, *z, *zt, *gram, *fac;
+    double *d, *e, *z, *zt, *gram;
     unsigned* sync;  // (G + 1) * 16 words: barrier flags, then the error flag
     int err_offset;
 };
@@ -631,8 +878,10 @@ void run_eig(const HeevBufs& b, int m, int nev, double* w_dev, double2* Y, int l
     hipLaunchKernelGGL(k_stebz, dim3((unsigned)((nev + 3) / 4)), dim3(256), 2 * m * sizeof(double), s, b.d, b.e, m,
                        nev, w_dev, scal, reinterpret_cast<const int*>(b.sync) + b.err_offset);
     DZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_stein, dim3((unsigned)((nev + 63) / 64)), dim3(64), 0, s, b.d, b.e, m, nev, w_dev, scal, b.z,
-                       m, b.fac);
+    hipLaunchKernelGGL(k_stein, dim3((unsigned)nev), dim3(64), stein_lds_bytes(m), s, b.d, b.e, m, nev, w_dev, scal,
+                       b.z, m);
+    DZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_stein_cluster, dim3((unsigned)nev), dim3(256), 0, s, w_dev, scal, nev, m, b.z, m);
     DZ_HIP(hipGetLastError());
     double* zc = b.z;
     double* zn = b.zt;
@@ -645,7 +894,15 @@ void run_eig(const HeevBufs& b, int m, int nev, double* w_dev, double2* Y, int l
         DZ_HIP(hipGetLastError());
         std::swap(zc, zn);
     }
-    hipLaunchKernelGGL(k_unmtr, dim3((unsigned)((nev + 3) / 4)), dim3(256), 0, s, b.vh, b.tau, m, nev, zc, m, Y, ldy);
+    {
+        const double2* vh = b.vh;
+        const double2* tau = b.tau;
+        const double* zcc = zc;
+        int mm = m, nn = nev, ldz = m, ldyy = ldy;
+        void* kargs[] = {&vh, &tau, &mm, &nn, &zcc, &ldz, &Y, &ldyy};
+        DZ_HIP(hipLaunchKernel(unmtr_kernel(m), dim3((unsigned)((nev + kUnmtrWgCols - 1) / kUnmtrWgCols)), dim3(256),
+                               kargs, unmtr_lds_bytes(m), s));
+    }
     DZ_HIP(hipGetLastError());
 }
 
@@ -654,19 +911,18 @@ HeevBufs work_bufs(HeevWork& w, int m, int nev, int G) {
     w.tau.ensure((size_t)m * sizeof(double2));
     w.d.ensure((size_t)m * sizeof(double));
     w.e.ensure((size_t)m * sizeof(double));
-    w.pbuf.ensure((size_t)2 * m * sizeof(double2));
+    w.pbuf.ensure((size_t)2 * (m + G) * sizeof(double2));
     w.nbuf.ensure((size_t)2 * m * sizeof(double2));
     w.sync.ensure((size_t)(G + 1) * 16 * sizeof(unsigned));
     if (nev > 0) {
         w.z.ensure((size_t)m * nev * sizeof(double));
         w.zt.ensure((size_t)m * nev * sizeof(double));
         w.gram.ensure((size_t)nev * nev * sizeof(double) + 64);
-        w.fac.ensure((size_t)nev * 5 * m * sizeof(double));
     }
     w.err_offset = G * 16;
     return HeevBufs{w.vh.as<double2>(), w.tau.as<double2>(), w.pbuf.as<double2>(), w.nbuf.as<double2>(),
                     w.d.as<double>(),   w.e.as<double>(),     w.z.as<double>(),     w.zt.as<double>(),
-                    w.gram.as<double>(), w.fac.as<double>(),  w.sync.as<unsigned>(), G * 16};
+                    w.gram.as<double>(), w.sync.as<unsigned>(), G * 16};
 }
 
 // ----------------------------------------------------------------- allocation-free variant
@@ -686,19 +942,18 @@ size_t carve(size_t& off, size_t bytes) {
 }
 
 // Offsets of the buffers in the scratch for (m, nev, G); returns the bytes needed.
-size_t scratch_layout(int m, int nev, int G, size_t out[11]) {
+size_t scratch_layout(int m, int nev, int G, size_t out[10]) {
     size_t off = 0;
     out[0] = carve(off, (size_t)m * m * sizeof(double2));       // vh
     out[1] = carve(off, (size_t)m * sizeof(double2));           // tau
-    out[2] = carve(off, (size_t)2 * m * sizeof(double2));       // pbuf
+    out[2] = carve(off, (size_t)2 * (m + G) * sizeof(double2)); // pbuf
     out[3] = carve(off, (size_t)2 * m * sizeof(double2));       // nbuf
     out[4] = carve(off, (size_t)m * sizeof(double));            // d
     out[5] = carve(off, (size_t)m * sizeof(double));            // e
     out[6] = carve(off, (size_t)m * nev * sizeof(double));      // z
     out[7] = carve(off, (size_t)m * nev * sizeof(double));      // zt
     out[8] = carve(off, (size_t)nev * nev * sizeof(double) + 64);  // gram + scale
-    out[9] = carve(off, (size_t)nev * 5 * m * sizeof(double));  // fac
-    out[10] = carve(off, (size_t)(G + 1) * 16 * sizeof(unsigned));  // sync
+    out[9] = carve(off, (size_t)(G + 1) * 16 * sizeof(unsigned));  // sync
     return off;
 }
 
@@ -727,7 +982,7 @@ void heev_lowest_noalloc(int device, const double2* A, int lda, int m, int nev, 
     DZ_REQUIRE(base, DZHIP_ERR_STATE, "eigensolver scratch not set up on this device (dzhip_ctx_create)");
     int G = 1, ncl = 1;
     trd_shape(m, &G, &ncl);
-    size_t off[11];
+    size_t off[10];
     const size_t need = scratch_layout(m, nev, G, off);
     DZ_REQUIRE(need <= kHeevNoAllocBytes, DZHIP_ERR_ARG, "allocation-free Hermitian eigensolver: scratch too small");
     unsigned char* p = static_cast<unsigned char*>(base);
@@ -735,14 +990,13 @@ void heev_lowest_noalloc(int device, const double2* A, int lda, int m, int nev, 
                      reinterpret_cast<double2*>(p + off[2]), reinterpret_cast<double2*>(p + off[3]),
                      reinterpret_cast<double*>(p + off[4]),  reinterpret_cast<double*>(p + off[5]),
                      reinterpret_cast<double*>(p + off[6]),  reinterpret_cast<double*>(p + off[7]),
-                     reinterpret_cast<double*>(p + off[8]),  reinterpret_cast<double*>(p + off[9]),
-                     reinterpret_cast<unsigned*>(p + off[10]), G * 16};
+                     reinterpret_cast<double*>(p + off[8]),  reinterpret_cast<unsigned*>(p + off[9]), G * 16};
     run_tridiag(b, G, ncl, A, lda, m, s);
     run_eig(b, m, nev, w_dev, Y, ldy, s);
 }
 
 void HeevWork::release() {
-    DevBuf* bufs[] = {&vh, &tau, &d, &e, &pbuf, &nbuf, &sync, &z, &zt, &gram, &fac};
+    DevBuf* bufs[] = {&vh, &tau, &d, &e, &pbuf, &nbuf, &sync, &z, &zt, &gram};
     for (DevBuf* b : bufs) b->release();
 }
 
