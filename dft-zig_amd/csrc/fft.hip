@@ -19,8 +19,10 @@ bool factor_fft_len(int n, std::vector<int>& radices) {
     radices.clear();
     if (n < 1 || n > kMaxFftLen) return false;
     int m = n;
-    // Largest register radices first: fewer LDS round trips per line (75 = 15 x 5, 90 = 9 x 10).
-    for (int r : {9, 8, 15, 10, 6, 4, 2, 3, 5}) {
+    // Largest register radices first: fewer LDS round trips per line (90 = 9 x 10).  No radix 15:
+    // its 15 complex values do not fit beside the FP64 tile kernels' other live state at their
+    // 128-VGPR budget (the forward plane kernel then spills).
+    for (int r : {9, 8, 10, 6, 4, 2, 3, 5}) {
         while (m % r == 0) { radices.push_back(r); m /= r; }
     }
     if (m != 1) return false;

@@ -1,8 +1,8 @@
 // In-LDS batched mixed-radix complex FFT on the lines of a tile.
 //
 // Algorithm (own design): in-place Cooley-Tukey with register butterflies of radix
-// 2, 3, 4, 5, 6, 8, 9, 10 or 15 (the composite ones as two-level register DFTs).  The decimation-in-time form takes its input at digit-reversed
-// positions (AxisTables::perm) and leaves natural order; the decimation-in-frequency
+// 2, 3, 4, 5, 6, 8, 9 or 10 (6 and 10 as two-level register DFTs).  The decimation-in-time form
+// takes its input at digit-reversed positions (AxisTables::perm) and leaves natural order; the decimation-in-frequency
 // form takes natural order and leaves frequency f at position perm[f].  Each butterfly
 // reads and writes the same R slots, so any assignment of butterflies to threads is
 // race-free within a stage and one LDS tile suffices (an FP64 72x73 tile is 84 KB,
@@ -209,7 +209,6 @@ __host__ __device__ __forceinline__ void dft_r(C* v, T sign) {
     if constexpr (R == 8) dft8<T>(v, sign);
     if constexpr (R == 9) dft9<T>(v, sign);
     if constexpr (R == 10) dft_ct<T, C, 2, 5>(v, sign);
-    if constexpr (R == 15) dft_ct<T, C, 3, 5>(v, sign);
 }
 
 // w^t for the transform sign S (+1 inverse, -1 forward): the table holds exp(-2 pi i t / n).
@@ -285,7 +284,6 @@ __host__ __device__ __forceinline__ void fft_stage_dispatch(int R, C* buf, int n
         case 6: fft_stage<T, C, 6, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
         case 8: fft_stage<T, C, 8, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
         case 10: fft_stage<T, C, 10, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
-        case 15: fft_stage<T, C, 15, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
         default: fft_stage<T, C, 9, DIT, S>(buf, n, span, ls, tw, m_sub, tid, nthr); break;
     }
 }

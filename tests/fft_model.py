@@ -5,9 +5,9 @@ import numpy as np
 
 
 def factor(n):
-    """factor_fft_len: register radices 9, 8, 15, 10, 6, 4, 2, 3, 5 in that order."""
+    """factor_fft_len: register radices 9, 8, 10, 6, 4, 2, 3, 5 in that order."""
     out, m = [], n
-    for r in (9, 8, 15, 10, 6, 4, 2, 3, 5):
+    for r in (9, 8, 10, 6, 4, 2, 3, 5):
         while m % r == 0:
             out.append(r)
             m //= r
