@@ -440,6 +440,20 @@ static void section_times(EigWork& w, double* out) {
 // into Y (leading dimension ldy) and the host vector returned (ascending).  The library's device
 // eigensolver up to m = kHeevMaxM; rocSOLVER zheevd beyond (it overwrites a copy of A in Y,
 // which must then hold m columns).
+// ok = 1 when the Cholesky factorisation of the Gram matrix succeeded (info == 0) and every pivot
+// keeps at least 1e-7 of its column's norm (R_jj > 1e-7 sqrt(G_jj); G's diagonal from its copy
+// gcopy, R in r, both k x k with leading dimension k).  One workgroup.
+__global__ __launch_bounds__(256) void k_chol_check(const double2* __restrict__ gcopy, const double2* __restrict__ r,
+                                                    int k, const rocblas_int* __restrict__ info, int* __restrict__ ok) {
+    int bad = 0;
+    for (int j = (int)threadIdx.x; j < k; j += 256) {
+        const double g = gcopy[(size_t)j * (k + 1)].x, rj = r[(size_t)j * (k + 1)].x;
+        if (!(rj > 1e-7 * sqrt(fmax(g, 0.0)))) bad = 1;
+    }
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) *ok = (*info == 0 && !bad) ? 1 : 0;
+}
+
 static std::vector<double> subspace_eigh(EigWork& w, const double2* A, int lda, int m, int nev, double2* Y, int ldy,
                                          hipStream_t s) {
     std::vector<double> ev(nev);
@@ -593,20 +607,18 @@ struct Solver {
         const int ldn = (int)n;
         gram(Q, k, Q, k, w.G.as<double2>(), k);
         DZ_HIP(hipMemcpyAsync(w.S.ptr, w.G.ptr, (size_t)k * k * sizeof(double2), hipMemcpyDeviceToDevice, s));
-        std::vector<double> gdiag(k), rdiag(k);
-        // Real parts of the diagonals: one double every k + 1 complex elements.
-        DZ_HIP(hipMemcpy2DAsync(gdiag.data(), sizeof(double), w.G.ptr, (size_t)(k + 1) * sizeof(double2),
-                                sizeof(double), k, hipMemcpyDeviceToHost, s));
         DZ_BLAS(rocsolver_zpotrf(w.blas, rocblas_fill_upper, k, w.G.as<zc>(), k, w.info.as<rocblas_int>()));
-        rocblas_int info = 0;
-        DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
-        DZ_HIP(hipMemcpy2DAsync(rdiag.data(), sizeof(double), w.G.ptr, (size_t)(k + 1) * sizeof(double2),
-                                sizeof(double), k, hipMemcpyDeviceToHost, s));
+        // The factorisation's verdict on the device (info and every pivot against its column's norm,
+        // the latter from the copy in S), one int back to the host.
+        int* ok_dev = w.info.as<int>() + 1;
+        hipLaunchKernelGGL(k_chol_check, dim3(1), dim3(256), 0, s, w.S.as<double2>(), w.G.as<double2>(), k,
+                           w.info.as<rocblas_int>(), ok_dev);
+        DZ_HIP(hipGetLastError());
+        int ok = 0;
+        DZ_HIP(hipMemcpyAsync(&ok, ok_dev, sizeof(int), hipMemcpyDeviceToHost, s));
         DZ_HIP(hipStreamSynchronize(s));
         const zc one(1.0, 0.0), zero(0.0, 0.0);
-        bool pivots_ok = info == 0;
-        for (int j = 0; j < k && pivots_ok; ++j) pivots_ok = rdiag[j] > 1e-7 * std::sqrt(std::max(gdiag[j], 0.0));
-        if (pivots_ok) {
+        if (ok) {
             DZ_BLAS(rocblas_ztrsm(w.blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                                   rocblas_diagonal_non_unit, ldn, k, &one, w.G.as<zc>(), k, reinterpret_cast<zc*>(Q), ldn));
             return k;
