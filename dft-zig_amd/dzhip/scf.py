@@ -278,10 +278,15 @@ GAAS_A_ANGSTROM = 5.653
 
 
 def gaas(cells=(2, 2, 2), ecut_ry=35.0, kpoints=None, nbands=160, displacement_angstrom=0.05, seed=5,
-         grid=None) -> ScfSystem:
+         grid=None, smearing_ry=0.01) -> ScfSystem:
     """Zincblende GaAs in n1 x n2 x n3 conventional cubic cells (Ga on the fcc sites, As at
     (1/4, 1/4, 1/4) + fcc; a = 5.653 A, so the 2x2x2 supercell has a = 11.306 A and 64 atoms), atoms
-    displaced by N(0, displacement) (numpy PCG64(seed)), Gamma only unless k-points are given."""
+    displaced by N(0, displacement) (numpy PCG64(seed)), Gamma only unless k-points are given.
+
+    Fermi-Dirac smearing (smearing_ry; 0 = fixed occupations): with these GTH-form parameters the
+    64-atom cell at Gamma has no gap at the Fermi level (levels 126-128 lie within 1.5e-3 Ry and
+    levels 128 and 129 within 2e-4 Ry), so with fixed occupations the highest occupied level
+    changes between SCF iterations and the density never settles."""
     a = GAAS_A_ANGSTROM * tables.ANGSTROM_TO_BOHR
     n1, n2, n3 = cells
     cell = np.diag([a * n1, a * n2, a * n3])
@@ -311,13 +316,13 @@ def gaas(cells=(2, 2, 2), ecut_ry=35.0, kpoints=None, nbands=160, displacement_a
     grid = tuple(grid) if grid is not None else tuple(tables.auto_grid(ecut_ry, recip))
     return ScfSystem(f"gaas{pos.shape[0]}", cell, recip, abs(float(np.linalg.det(cell))), grid, ecut_ry, pos, spc,
                      species, 0.0, 0.0, 0.0, nelec, nbands, [np.asarray(k, dtype=np.float64) for k in kpts], list(kw),
-                     species_local=local)
+                     smearing_ry, species_local=local)
 
 
-def gaas64(ecut_ry=35.0, nbands=160):
+def gaas64(ecut_ry=35.0, nbands=160, smearing_ry=0.01):
     """The 64-atom GaAs supercell (2x2x2 cubic cells, a = 11.306 A, displacements N(0, 0.05 A)),
-    35 Ry (90^3 grid), Gamma point, 160 bands."""
-    return gaas((2, 2, 2), ecut_ry, nbands=nbands)
+    35 Ry (90^3 grid), Gamma point, 160 bands, Fermi-Dirac smearing 0.01 Ry."""
+    return gaas((2, 2, 2), ecut_ry, nbands=nbands, smearing_ry=smearing_ry)
 
 
 def start_potential(system, potential="atomic"):
