@@ -270,6 +270,49 @@ def scf_leg(name, max_iter, arms, log=sys.stderr):
     return out
 
 
+def heev_profile(dzhip, torch, ctx, sizes=(164, 328, 492, 648), nev=160, calls=10):
+    """The library's dense eigensolver (dzhip_heev_dev) against torch.linalg.eigh (rocSOLVER) on
+    random Hermitian matrices with a prescribed spectrum (eigenvalues uniform in [-2, 30], random
+    unitary eigenvectors, seeded) at the Rayleigh-Ritz sizes of a 160-band Davidson solve; one line
+    per size with both times (HIP events, 3 warm-up calls) and the eigensolver's largest eigenvalue
+    difference to numpy's eigvalsh and largest |Z^H Z - I|."""
+    st = torch.cuda.Stream()
+    rng = np.random.default_rng(0)
+    for m in sizes:
+        q, _ = np.linalg.qr(rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m)))
+        lam = np.sort(rng.uniform(-2.0, 30.0, m))
+        a = (q * lam) @ q.conj().T
+        a = 0.5 * (a + a.conj().T)
+        at = torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
+        w = torch.zeros(nev, dtype=torch.float64, device="cuda")
+        z = torch.zeros((nev, m), dtype=torch.complex128, device="cuda")
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for _ in range(3):
+            ctx.heev_dev(at.data_ptr(), m, m, nev, w.data_ptr(), z.data_ptr(), m, st.cuda_stream)
+        ctx.heev_check(st.cuda_stream)
+        e0.record(st)
+        for _ in range(calls):
+            ctx.heev_dev(at.data_ptr(), m, m, nev, w.data_ptr(), z.data_ptr(), m, st.cuda_stream)
+        e1.record(st)
+        ctx.heev_check(st.cuda_stream)
+        ms = e0.elapsed_time(e1) / calls
+        err = float(np.max(np.abs(w.cpu().numpy() - np.linalg.eigvalsh(a)[:nev])))
+        zz = z.cpu().numpy().T
+        orth = float(np.max(np.abs(zz.conj().T @ zz - np.eye(nev))))
+        with torch.cuda.stream(st):
+            for _ in range(2):
+                torch.linalg.eigh(at)
+            e0.record(st)
+            for _ in range(calls):
+                torch.linalg.eigh(at)
+            e1.record(st)
+        torch.cuda.synchronize()
+        ms_t = e0.elapsed_time(e1) / calls
+        print(json.dumps({"heev": {"m": m, "nev": nev, "ms": round(ms, 3), "torch_eigh_ms": round(ms_t, 3),
+                                   "max_dlambda": err, "max_orth": orth}}), flush=True)
+
+
 def eig_profile(ham):
     """Section times of one cold and two warm FP64 Davidson solves (128 bands, 12 iterations,
     diagonal start) on the Si-64 k-point context, one JSON line per solve."""
@@ -408,6 +451,9 @@ def main():
     ap.add_argument("--eig", action="store_true",
                     help="instead of the bench line: one cold and two warm FP64 Davidson solves of 128 bands "
                          "on the Si-64 k-point (12 iterations), with the solver's section times")
+    ap.add_argument("--heev", action="store_true",
+                    help="instead of the bench line: the dense eigensolver against torch.linalg.eigh at the "
+                         "Rayleigh-Ritz sizes m = 164..648 (nev = 160)")
     ap.add_argument("--overlap", action="store_true",
                     help="run the nonlocal kernels on a second stream beside the local ones (timed pass only)")
     ap.add_argument("--xline-split", type=int, default=0,
@@ -428,6 +474,11 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.heev:
+        hctx = dzhip.Context(0)
+        heev_profile(dzhip, torch, hctx)
+        hctx.close()
+        return
     system = dscf.si64()
     w = dscf.kpoint_workload(system, SI64_K, nbands=system.nbands)
     ctx = dzhip.Context(0)

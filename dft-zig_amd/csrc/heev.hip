@@ -18,10 +18,29 @@ constexpr int kTrdMaxCols = 16;
 constexpr int kTrdRowsPerThread = (1024 + 255) / 256;  // rows per thread at m <= kHeevMaxM
 constexpr unsigned kSpinLimit = 1u << 22;  // barrier polls (each after s_sleep) before giving up
 
+// One DPP row rotation of a double (both halves moved by the same row_ror:N control).
+template <int CTRL>
+__device__ __forceinline__ double dpp_rot(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over the 64 lanes of a wavefront (all lanes active), the same value in every lane: rotations
+// by 8, 4, 2, 1 within each 16-lane row leave every lane of the row holding the row's sum (the same
+// pairs added in every lane, so bitwise equal), then the four row sums are read from lanes 0, 16,
+// 32 and 48.  No LDS traffic (a __shfl_xor butterfly is six ds_bpermute round trips).
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_rot<0x128>(v);  // row_ror:8
+    v += dpp_rot<0x124>(v);  // row_ror:4
+    v += dpp_rot<0x122>(v);  // row_ror:2
+    v += dpp_rot<0x121>(v);  // row_ror:1
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 // Sum of n values over the workgroup (NT threads); every thread gets the n sums in out[].
@@ -748,9 +767,7 @@ __global__ __launch_bounds__(256) void k_unmtr(const double2* __restrict__ vh, c
                 }
             }
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1)
-#pragma unroll
-                for (int c = 0; c < 2 * kUnmtrWaveCols; ++c) acc[c] += __shfl_xor(acc[c], o, 64);
+            for (int c = 0; c < 2 * kUnmtrWaveCols; ++c) acc[c] = wave_sum(acc[c]);
             double2 tsc[kUnmtrWaveCols];
 #pragma unroll
             for (int c = 0; c < kUnmtrWaveCols; ++c) tsc[c] = cm(tk, make_double2(acc[2 * c], acc[2 * c + 1]));
