@@ -894,7 +894,8 @@ int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split) {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
         DZ_REQUIRE(xline_split == 0 || xline_code_usable(*ham, xline_split, false), DZHIP_ERR_ARG,
                    "xline_split must divide ny, or be -1..-3 (-4..-6, -7..-9, -10..-12) with a (lane-pair, "
-                   "lane-triple, residue-sequential) register kernel for this context (0 = autotune)");
+                   "lane-triple, residue-sequential) register kernel for this context, or -13 where the "
+                   "convolution kernel runs (0 = autotune)");
         std::lock_guard<std::mutex> lk(ham->mu);
         ham->tune_fixed = xline_split;
         ham->xline_split[0] = ham->xline_split[1] = 0;

@@ -53,6 +53,9 @@ struct Ham {
     // one cyclic run (plane_xs = -1 otherwise), and V_loc/N in x-major order [x][z][y] (FP32, FP64).
     int plane_xs = -1;
     DevBuf vloc32_t, vloc64_t;
+    // Convolution x-line kernel: the forward x-transform of every line of V_loc/N, [z][y][kx]
+    // complex (FP64, FP32).
+    DevBuf chat64, chat32;
     bool has_local = false;
 
     // Nonlocal: complex projector matrix A[p][G] = phi(G) e^{iG.tau_a} (P x n_pw), D tables.
@@ -64,7 +67,7 @@ struct Ham {
     DevBuf xstage, ystage, x32, y32, tbuf, partial, qbuf;
     // Tuned launch shapes: x-line variant per precision ([0] FP64, [1] FP32): 0 not yet tuned,
     // d >= 1 the LDS kernel with d y-slabs, -1..-12 a register kernel (xline_reg_threads,
-    // xline_reg_kind); tune_fixed != 0 forces a variant (sweeps).
+    // xline_reg_kind), kXlineConv the convolution kernel; tune_fixed != 0 forces a variant (sweeps).
     int xline_split[2] = {0, 0};
     int tune_fixed = 0;
 
@@ -170,14 +173,22 @@ bool xline_tri_usable(const Ham& h);
 // threads, -4 / -5 / -6 = the lane-pair kernel, -7 / -8 / -9 = the lane-triple kernel, in the same
 // workgroup sizes.
 constexpr int kXlineReg = -1, kXlineRegLast = -12;
+// The convolution x-line kernel on the matrix cores (one variant; kind 4 in xline_reg_kind's
+// numbering).
+constexpr int kXlineConv = -13;
 // Register x-line kernel kinds (codes -1..-3, -4..-6, -7..-9, -10..-12 in that order).
 constexpr int kXlineOneLane = 0, kXlinePair = 1, kXlineTriple = 2, kXlineSeq3 = 3;
 constexpr int xline_reg_threads(int code) { return 256 >> ((-code - 1) % 3); }
 constexpr int xline_reg_kind(int code) { return (-code - 1) / 3; }
 // Threads per x-line: one (one-lane and residue-sequential kernels), two or three.
-constexpr int xline_reg_lanes(int code) { return xline_reg_kind(code) == kXlineSeq3 ? 1 : 1 + xline_reg_kind(code); }
+constexpr int xline_reg_lanes(int code) {
+    return code == kXlineConv ? 0 : xline_reg_kind(code) == kXlineSeq3 ? 1 : 1 + xline_reg_kind(code);
+}
 constexpr bool xline_reg_pair(int code) { return xline_reg_kind(code) == kXlinePair; }
-// Whether variant `code` (d >= 1 LDS slabs, or a register variant) can run on h in a precision.
+// ... and the convolution kernel: the sphere's x-planes form one cyclic run of at most 64 planes.
+bool xline_conv_usable(const Ham& h);
+// Whether variant `code` (d >= 1 LDS slabs, a register variant or the convolution kernel) can run
+// on h in a precision.
 bool xline_code_usable(const Ham& h, int code, bool fp64);
 void ham_set_local(Ham& h, const double* local_r);
 // local_r already on the device (read on h's stream).

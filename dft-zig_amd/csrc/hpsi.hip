@@ -622,6 +622,7 @@ bool xline_seq3_usable(const Ham& h) {
 
 bool xline_code_usable(const Ham& h, int code, bool fp64) {
     if (code > 0) return h.ny % code == 0;
+    if (code == kXlineConv) return xline_conv_usable(h);
     if (code < kXlineRegLast || code > kXlineReg) return false;
     switch (xline_reg_kind(code)) {
         case kXlineOneLane: return xline_reg_usable(h, fp64);
@@ -913,6 +914,168 @@ template <> struct Mfma<double> {
     }
     static __device__ __forceinline__ int row(int k4, int r) { return k4 + 4 * r; }
 };
+
+// x-line pass as a convolution over the sphere's x-frequencies, on the matrix cores.  The x-part of
+// H is diagonal in x, so on the present frequencies kx = x_s + p (p < np, the sphere's cyclic
+// x-run) of a line it is
+//   Y[q] = sum_{p < np} c[(q - p) mod nx] X[p],   c[d] = sum_x (V_loc(x)/N) e^{-2 pi i d x / nx},
+// a Toeplitz matrix that does not depend on the run's start x_s.  Over all bands of a line it is
+// an np x np by np x bands complex product, which replaces the two x-transforms.  One workgroup
+// per ConvLines<T> consecutive lines (one 128-byte segment of every row) and 16 bands; each
+// wavefront owns (line, 16-row block) jobs and keeps their accumulators in registers.  The rows
+// stream through LDS in chunks of kConvChunk (every load of a chunk issued before its LDS
+// stores); per chunk the jobs take their k-steps of 4 with four real MFMAs per complex product
+// into two accumulators (A from the workgroup's c values in LDS, B from the chunk).  The results go back in place
+// through the same chunk buffer, kConvChunk rows at a time, as coalesced stores.
+template <typename T> struct ConvLines { static constexpr int value = 16; };
+template <> struct ConvLines<double> { static constexpr int value = 8; };
+constexpr int kConvBands = 16;
+constexpr int kConvMaxPlanes = 48;  // three 16-row blocks: a wavefront's accumulators stay in VGPRs
+constexpr int kConvChunk = 12;      // rows per LDS chunk (three k-steps)
+constexpr int kConvThreads = 512;   // eight wavefronts share a tile; each holds a few jobs
+
+// LDS index of (row p, band b, line li) in a chunk [p][b][li]; the line index is XORed with the
+// band so that the 16 bands of one (p, li), read by 16 lanes, fall in different banks.
+template <int LT>
+__device__ __forceinline__ int conv_idx(int p, int b, int li) {
+    return (p * kConvBands + b) * LT + (li ^ (b & (LT - 1)));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kConvThreads, 2) void k_xline_conv(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
+                                                    int nc, const typename C2T<T>::type* __restrict__ chat, int nx,
+                                                    const int2* __restrict__ kpl) {
+    using C = typename C2T<T>::type;
+    using M = Mfma<T>;
+    constexpr int LT = ConvLines<T>::value;
+    constexpr int NB = kConvBands;
+    constexpr int kWaves = kConvThreads / 64;
+    constexpr int kMaxJobs = LT * (kConvMaxPlanes / 16) / kWaves;  // per wavefront
+    constexpr int kChunkElems = kConvChunk * NB * LT;
+    constexpr int kPer = kChunkElems / kConvThreads;           // loads per thread per chunk
+    static_assert(kChunkElems % kConvThreads == 0 && (LT * (kConvMaxPlanes / 16)) % kWaves == 0,
+                  "chunk and jobs must split evenly over the workgroup");
+    extern __shared__ unsigned char smem_raw[];
+    C* xs = reinterpret_cast<C*>(smem_raw);  // one chunk [kConvChunk][NB][LT]
+    C* cs = xs + kChunkElems;                // c of the workgroup's lines [LT][nx]
+    const int tid = (int)threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int2 kp = xl_planes(kpl, nplanes);
+    const int np = kp.y;
+    const int b0 = (int)blockIdx.x * NB, l0 = (int)blockIdx.y * LT;
+    C* base = tbuf + ((size_t)b0 * nplanes + kp.x) * lines + l0;
+    const int nlt = min(LT, lines - l0);
+    for (int i = tid; i < LT * nx; i += kConvThreads)
+        cs[i] = i < nlt * nx ? chat[(size_t)l0 * nx + i] : C{T(0), T(0)};
+    const int nmt = (np + 15) >> 4;
+    const int njobs = LT * nmt;
+    typename M::acc re[kMaxJobs], im[kMaxJobs];
+#pragma unroll
+    for (int jj = 0; jj < kMaxJobs; ++jj) {
+        re[jj] = typename M::acc{0, 0, 0, 0};
+        im[jj] = typename M::acc{0, 0, 0, 0};
+    }
+    // Chunk rows k0 .. k0 + kConvChunk - 1 (rows >= np zero), loaded into registers one chunk ahead:
+    // the next chunk's loads are in flight while the current one is multiplied.
+    C v[kPer];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = tid + u * kConvThreads;
+            const int li = e % LT, r = e / LT, b = r % NB, p = k0 + r / NB;
+            v[u] = (p < np && b0 + b < nc && li < nlt) ? base[((size_t)b * nplanes + p) * lines + li] : C{T(0), T(0)};
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < np; k0 += kConvChunk) {
+        __syncthreads();  // the previous chunk is consumed
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = tid + u * kConvThreads;
+            const int li = e % LT, r = e / LT, b = r % NB;
+            xs[conv_idx<LT>(r / NB, b, li)] = v[u];
+        }
+        __syncthreads();
+        if (k0 + kConvChunk < np) fetch(k0 + kConvChunk);
+#pragma unroll
+        for (int jj = 0; jj < kMaxJobs; ++jj) {
+            const int job = wave + kWaves * jj;
+            if (job >= njobs) break;
+            const int li = job % LT, mt = job / LT;
+            const int q = mt * 16 + l16;  // output row of this lane's A operand
+            const C* cl = cs + li * nx;
+#pragma unroll
+            for (int ks = 0; ks < kConvChunk; ks += 4) {
+                const int p = k0 + ks + l4;
+                C a = C{T(0), T(0)};
+                if (q < np && p < np) {
+                    int d = q - p;
+                    d += d < 0 ? nx : 0;
+                    a = cl[d];
+                }
+                const C bv = xs[conv_idx<LT>(ks + l4, l16, li)];
+                // a b = (ar br - ai bi) + i (ar bi + ai br): four real products, two accumulators
+                re[jj] = M::mma(a.x, bv.x, re[jj]);
+                re[jj] = M::mma(-a.y, bv.y, re[jj]);
+                im[jj] = M::mma(a.x, bv.y, im[jj]);
+                im[jj] = M::mma(a.y, bv.x, im[jj]);
+            }
+        }
+    }
+    // Results back in place through the chunk buffer, kConvChunk rows at a time.
+    for (int r0 = 0; r0 < np; r0 += kConvChunk) {
+        __syncthreads();  // the buffer is free
+#pragma unroll
+        for (int jj = 0; jj < kMaxJobs; ++jj) {
+            const int job = wave + kWaves * jj;
+            if (job >= njobs) break;
+            const int li = job % LT, mt = job / LT;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int q = mt * 16 + M::row(l4, r);
+                if (q >= r0 && q < r0 + kConvChunk && q < np) xs[conv_idx<LT>(q - r0, l16, li)] = C{re[jj][r], im[jj][r]};
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = tid + u * kConvThreads;
+            const int li = e % LT, r = e / LT, b = r % NB, q = r0 + r / NB;
+            if (q < np && b0 + b < nc && li < nlt)
+                base[((size_t)b * nplanes + q) * lines + li] = xs[conv_idx<LT>(r / NB, b, li)];
+        }
+    }
+}
+
+template <typename T>
+size_t xline_conv_lds_bytes(int nx) {
+    return ((size_t)kConvChunk * kConvBands * ConvLines<T>::value + (size_t)ConvLines<T>::value * nx) *
+           sizeof(typename C2T<T>::type);
+}
+
+bool xline_conv_usable(const Ham& h) {
+    return h.plane_xs >= 0 && h.nplanes >= 1 && h.nplanes <= kConvMaxPlanes &&
+           xline_conv_lds_bytes<double>(h.nx) <= kMaxLdsBytes;
+}
+
+// Convolution x-line kernel on h's grid (kpl / nk / planes_per_band / max_planes as for
+// launch_xline_reg); reads h's per-line transform of V_loc/N.
+template <typename T>
+void launch_xline_conv(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s, const int2* kpl = nullptr,
+                       int nk = 1, int planes_per_band = -1, int max_planes = -1) {
+    using C = typename C2T<T>::type;
+    const int lines = h.ny * h.nz;
+    const int nplanes = planes_per_band >= 0 ? planes_per_band : h.nplanes;
+    const int np_max = max_planes >= 0 ? max_planes : h.nplanes;
+    DZ_REQUIRE(np_max >= 1 && np_max <= kConvMaxPlanes, DZHIP_ERR_STATE, "convolution x-line kernel: too many planes");
+    const size_t lds = xline_conv_lds_bytes<T>(h.nx);
+    set_max_dynamic_lds((const void*)k_xline_conv<T>, lds);
+    const C* chat = std::is_same<T, float>::value ? (const C*)h.chat32.ptr : (const C*)h.chat64.ptr;
+    const dim3 grid((unsigned)((nc + kConvBands - 1) / kConvBands),
+                    (unsigned)((lines + ConvLines<T>::value - 1) / ConvLines<T>::value), (unsigned)nk);
+    hipLaunchKernelGGL(k_xline_conv<T>, grid, dim3(kConvThreads), lds, s, tb, nplanes, lines, (int)nc, chat, h.nx, kpl);
+    DZ_HIP(hipGetLastError());
+}
 
 // Workgroup tiles of the nonlocal GEMMs: 10 wavefronts = 5 band groups of 32 bands x 2 halves
 // of the projector (projection) or G (back-projection) tile, so one workgroup covers a 160-band
@@ -1288,7 +1451,7 @@ size_t Ham::device_bytes() const {
                             &plane_ycols, &plane_xabsent, &kinetic64,
                             &kinetic32, &vloc64, &vloc32, &a64, &a32, &meta, &block_rows, &dm64, &dm32,
                             &xstage, &ystage, &x32, &y32, &tbuf, &partial, &qbuf,
-                            &vloc32_t, &vloc64_t};
+                            &vloc32_t, &vloc64_t, &chat64, &chat32};
     size_t s = tab.bytes;
     for (const DevBuf* b : bufs) s += b->bytes;
     return s;
@@ -1518,12 +1681,41 @@ __global__ void k_local_xmajor(const float* __restrict__ v, float* __restrict__ 
     vt64[o] = v64[i];
 }
 
+// chat[l][d] = sum_x vt[x][l] w^(d x), w = e^{-2 pi i / nx} (the x axis' twiddle table): the forward
+// x-transform of each line l = y + ny z of V_loc/N, the convolution x-line kernel's operand, in FP64
+// and FP32.
+__global__ void k_line_dft(const double* __restrict__ vt, const double2* __restrict__ tw, int nx, int lines,
+                           double2* __restrict__ c64, float2* __restrict__ c32) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)lines * nx) return;
+    const int d = (int)(i % nx);
+    const size_t l = i / nx;
+    double sr = 0.0, si = 0.0;
+    int t = 0;
+    for (int x = 0; x < nx; ++x) {
+        const double v = vt[(size_t)x * lines + l];
+        const double2 w = tw[t];
+        sr += v * w.x;
+        si += v * w.y;
+        t += d;
+        if (t >= nx) t -= nx;
+    }
+    c64[i] = make_double2(sr, si);
+    c32[i] = make_float2((float)sr, (float)si);
+}
+
 static void local_xmajor(Ham& h) {
     h.vloc32_t.ensure(h.ngrid * sizeof(float));
     h.vloc64_t.ensure(h.ngrid * sizeof(double));
     hipLaunchKernelGGL(k_local_xmajor, dim3((unsigned)((h.ngrid + 255) / 256)), dim3(256), 0, h.stream(),
                        h.vloc32.as<float>(), h.vloc32_t.as<float>(), h.vloc64.as<double>(), h.vloc64_t.as<double>(),
                        h.nx, h.ny, h.nz);
+    DZ_HIP(hipGetLastError());
+    h.chat64.ensure(h.ngrid * sizeof(double2));
+    h.chat32.ensure(h.ngrid * sizeof(float2));
+    hipLaunchKernelGGL(k_line_dft, dim3((unsigned)((h.ngrid + 255) / 256)), dim3(256), 0, h.stream(),
+                       h.vloc64_t.as<double>(), h.tab.tw64 + h.tab.ax[0].tw_off, h.nx, h.ny * h.nz,
+                       h.chat64.as<double2>(), h.chat32.as<float2>());
     DZ_HIP(hipGetLastError());
 }
 
@@ -1657,6 +1849,10 @@ void launch_xline(Ham& h, typename C2T<T>::type* tb, size_t nc, int split, hipSt
 
 template <typename T>
 void launch_xline_any(Ham& h, typename C2T<T>::type* tb, size_t nc, int code, hipStream_t s) {
+    if (code == kXlineConv) {
+        launch_xline_conv<T>(h, tb, nc, s);
+        return;
+    }
     if (code <= kXlineReg && code >= kXlineRegLast) {
         launch_xline_reg<T>(h, tb, nc, xline_reg_threads(code), xline_reg_kind(code), s);
         return;
@@ -1702,6 +1898,7 @@ int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
         for (int c = -7; c >= -9; --c) cand.push_back(c);
     if (xline_reg_usable(h, !fp32))
         for (int c = -1; c >= -3; --c) cand.push_back(c);
+    if (xline_conv_usable(h)) cand.push_back(kXlineConv);
     for (int d = 1; d <= 12; ++d)
         if (h.ny % d == 0 && h.ny / d >= 8) cand.push_back(d);
     if (cand.empty()) cand.push_back(1);
@@ -2098,8 +2295,11 @@ static void hamset_launch(HamSet& hs, const void* xv, void* yv, size_t nc, int c
                        hs.plane_ystart.as<int>(), hs.plane_ycols.as<short>(), hs.ny, hs.nz, hs.np_total, h0.tab.ax[1],
                        h0.tab.ax[2], tw);
     DZ_HIP(hipGetLastError());
-    launch_xline_reg<T>(h0, tb, nc, xline_reg_threads(code), xline_reg_kind(code), s, hs.kpl.as<int2>(), nk,
-                        hs.np_total, hs.max_planes);
+    if (code == kXlineConv)
+        launch_xline_conv<T>(h0, tb, nc, s, hs.kpl.as<int2>(), nk, hs.np_total, hs.max_planes);
+    else
+        launch_xline_reg<T>(h0, tb, nc, xline_reg_threads(code), xline_reg_kind(code), s, hs.kpl.as<int2>(), nk,
+                            hs.np_total, hs.max_planes);
     hipLaunchKernelGGL(k_plane_fwd<T>, dim3(hs.np_total, (unsigned)nc), dim3(nt), lds_p, s, tb, x, ld, y, ld, kin,
                        hs.plane_start.as<int>(), hs.plane_pw.as<int>(), hs.plane_pos.as<int>(),
                        hs.plane_ystart.as<int>(), hs.plane_ycols.as<short>(), hs.ny, hs.nz, hs.np_total, h0.tab.ax[1],

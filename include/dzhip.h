@@ -395,8 +395,12 @@ int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t c
  * thread per x-line doing the three residues x mod 3 in turn; the same line lengths, both
  * precisions, when the G-sphere spans at most two thirds of the line); a precision where the
  * chosen kernel cannot run keeps one slab.  -1..-3 need the one-lane kernel in FP32, -4..-6 the
- * lane-pair kernel, -7..-9 the lane-triple kernel, -10..-12 the residue-sequential kernel.  0 (default) times every divisor of ny with at least 8 lines per slab, and
- * the register kernels' variants where they can run, on the first apply of each precision with at
+ * lane-pair kernel, -7..-9 the lane-triple kernel, -10..-12 the residue-sequential kernel.
+ * -13 selects the convolution x-line kernel (the x-part of H on a line's present x-frequencies as
+ * a Toeplitz matrix product over all bands on the matrix cores; the sphere's x-planes one cyclic
+ * run of at most 64 planes, both precisions).  0 (default) times every divisor of ny with at
+ * least 8 lines per slab, and
+ * the register and convolution kernels' variants where they can run, on the first apply of each precision with at
  * least 16 columns and keeps the fastest; the result is shared by every context of the same
  * device, grid and sphere shape in the process. */
 int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split);
@@ -414,8 +418,9 @@ int dzhip_ham_set_overlap(dzhip_ham* ham, int on);
  * [6] = for FP64 (0 = not tuned yet, or the register kernel), [7] = workgroup size when FP32
  * runs the register x-line kernel (0 = the LDS kernel or not tuned yet), [8] = the same for
  * FP64, [9] / [10] = threads per x-line of that register kernel, 1, 2 or 3 (FP32 / FP64; 0 = the
- * LDS kernel or not tuned yet), [11] / [12] = its kind (FP32 / FP64): 1 one-lane, 2 lane-pair,
- * 3 lane-triple, 4 residue-sequential (0 = the LDS kernel or not tuned yet). */
+ * LDS or convolution kernel, or not tuned yet), [11] / [12] = its kind (FP32 / FP64): 1 one-lane,
+ * 2 lane-pair, 3 lane-triple, 4 residue-sequential, 5 convolution (0 = the LDS kernel or not
+ * tuned yet). */
 int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out);
 
 #ifdef __cplusplus

@@ -26,7 +26,7 @@ def al35(dz, ctx):
         h.close()
 
 
-@pytest.mark.parametrize("code", [-4, -10])
+@pytest.mark.parametrize("code", [-4, -10, -13])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_hamset_equals_per_k_applies(dz, ctx, al35, prec, code):
     import torch

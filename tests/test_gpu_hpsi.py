@@ -258,7 +258,7 @@ REGISTER_KERNELS = {"tiny": {"one": (0, 1), "pair": (), "tri": (), "seq": ()},
                     "bench80": {"one": (1,), "pair": (0, 1), "tri": (), "seq": ()},
                     "bench90": {"one": (1,), "pair": (0, 1), "tri": (0, 1), "seq": (0, 1)},
                     "bench100": {"one": (1,), "pair": (0, 1), "tri": (), "seq": ()}}
-KIND = {"one": 1, "pair": 2, "tri": 3, "seq": 4}
+KIND = {"one": 1, "pair": 2, "tri": 3, "seq": 4, "conv": 5}
 
 
 @pytest.mark.parametrize("which,prec", [("tiny", 1), ("si8", 1), ("bench", 1), ("bench75", 1), ("bench90", 1),
@@ -299,6 +299,37 @@ def test_hpsi_register_xline_vs_oracle(dz, ctx, which, prec):
             assert err < (TOL32 if prec else TOL64)
             ran += runs
         assert ran >= 1
+    finally:
+        ham.close()
+
+
+@pytest.mark.parametrize("which,nbands", [("tiny", 4), ("si8", 8), ("bench", 20), ("bench75", 20), ("bench90", 8),
+                                          ("bench100", 8)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hpsi_conv_xline_vs_oracle(dz, ctx, which, nbands, prec):
+    """The convolution x-line kernel (tuning code -13: the x-part of H on each line's present
+    x-frequencies as a Toeplitz matrix over all bands, MFMA products, 16-band x 8- or 16-line
+    workgroup tiles) against the FP64 oracle apply on grids with x-lines of 8 to 100 points, band
+    counts that leave a partial 16-band tile (20) or a single partial one (4, 8), and line counts
+    that leave a partial line tile (75 x 75, 90 x 90)."""
+    make = {"tiny": lambda: workloads.tiny(), "si8": lambda: workloads.si8(nbands=nbands),
+            "bench": lambda: workloads.bench(nbands=nbands),
+            "bench75": lambda: workloads.bench(nbands=nbands, grid=(75, 75, 75)),
+            "bench90": lambda: workloads.bench(nbands=nbands, grid=(90, 90, 90)),
+            "bench100": lambda: workloads.bench(nbands=nbands, grid=(100, 72, 72))}
+    w = make[which]()
+    ham = make_ham(dz, ctx, w)
+    key = "fp32" if prec else "fp64"
+    try:
+        nb = min(w.nbands, nbands)
+        ref = hamiltonian.apply_hamiltonian_batched(oracle_ctx(w), w.psi[:nb])
+        ham.set_tuning(-13)
+        got = ham.apply(w.psi[:nb], precision=prec)
+        info = ham.info()
+        assert info[f"xline_kind_{key}"] == 5 and info[f"xline_lanes_{key}"] == 0
+        err = relerr(got, ref)
+        print(which, prec, w.grid, nb, err)
+        assert err < (TOL32 if prec else TOL64)
     finally:
         ham.close()
 

@@ -57,11 +57,12 @@ def test_si64_full_block_hermitian_and_linear(dz, ctx, si64_k):
         for prec, tol in ((0, 1e-12), (1, 2e-5)):
             y = ham.apply(w.psi, prec)
             # A 160-column pass tunes the x-line kernel: the odd 75-point line runs a register
-            # kernel (one lane per line in FP32, three lanes per line in FP64).
+            # kernel (one lane per line in FP32: kind 1, three lanes: kind 3) or the convolution
+            # kernel on the matrix cores (kind 5, the FP64 choice); never the LDS kernel.
             info = ham.info()
             key = "fp32" if prec else "fp64"
             print("si64 tuned", prec, info)
-            assert info[f"xline_lanes_{key}"] in ((1, 3) if prec else (3,)) and info[f"xline_register_{key}"] > 0
+            assert info[f"xline_kind_{key}"] in ((1, 3, 5) if prec else (3, 5)) and info[f"xline_register_{key}"] > 0
             lhs, rhs = np.vdot(w.psi[11], y[150]), np.vdot(y[11], w.psi[150])
             assert abs(lhs - rhs) < tol * max(abs(lhs), np.linalg.norm(y[11]) * np.linalg.norm(w.psi[150])), prec
             comb = 0.4 * w.psi[2] + (0.25 - 1.1j) * w.psi[159]

@@ -1,0 +1,99 @@
+"""CPU model of the convolution x-line kernel (dft-zig_amd/csrc/hpsi.hip, k_xline_conv): the same
+12-row LDS chunks with their XOR-swizzled index, the same (line, 16-row block) jobs over eight
+wavefronts, the MFMA 16x16x4 lane layouts (A[i = l % 16][k = l / 16], B[k = l / 16][j = l % 16],
+accumulator r of lane l at row l / 16 + 4 r in FP64 and 4 (l / 16) + r in FP32), the four-product
+complex form and the chunked write-back, checked against the x-line step written with FFTs:
+inverse transform of the present frequencies x_s + p, times V/N, forward transform (numpy)."""
+import numpy as np
+import pytest
+
+KCONV_BANDS = 16
+
+
+def conv_idx(p, b, li, lt):
+    return (p * KCONV_BANDS + b) * lt + (li ^ (b & (lt - 1)))
+
+
+CHUNK = 12
+
+
+def model_kernel(tile, chat, np_, nx, lt, fp64):
+    """tile: [np][bands=16][lt] complex rows of one workgroup; chat: [lt][nx]."""
+    nmt = (np_ + 15) >> 4
+    jobs = {}
+    for wave in range(8):
+        for jj in range(lt * 3 // 8):
+            job = wave + 8 * jj
+            if job < lt * nmt:
+                jobs[job] = [np.zeros((16, 16)), np.zeros((16, 16))]
+    for k0 in range(0, np_, CHUNK):
+        xs = np.zeros(CHUNK * KCONV_BANDS * lt, complex)
+        for r in range(CHUNK):
+            for b in range(KCONV_BANDS):
+                for li in range(lt):
+                    p = k0 + r
+                    xs[conv_idx(r, b, li, lt)] = tile[p, b, li] if p < np_ else 0.0
+        for job, (re, im) in jobs.items():
+            li, mt = job % lt, job // lt
+            for ks in range(0, CHUNK, 4):
+                a = np.zeros((16, 4), complex)
+                bm = np.zeros((4, 16), complex)
+                for lane in range(64):
+                    l16, l4 = lane & 15, lane >> 4
+                    q, p = mt * 16 + l16, k0 + ks + l4
+                    if q < np_ and p < np_:
+                        a[l16, l4] = chat[li, (q - p) % nx]
+                    bm[l4, l16] = xs[conv_idx(ks + l4, l16, li, lt)]
+                re += a.real @ bm.real - a.imag @ bm.imag
+                im += a.real @ bm.imag + a.imag @ bm.real
+    out = np.zeros((np_, KCONV_BANDS, lt), complex)
+    for r0 in range(0, np_, CHUNK):
+        xs = np.zeros(CHUNK * KCONV_BANDS * lt, complex)
+        for job, (re, im) in jobs.items():
+            li, mt = job % lt, job // lt
+            # Read the accumulators out through the lane layout, as the kernel does.
+            for lane in range(64):
+                l16, l4 = lane & 15, lane >> 4
+                for r in range(4):
+                    row = l4 + 4 * r if fp64 else 4 * l4 + r
+                    q = mt * 16 + row
+                    if r0 <= q < r0 + CHUNK and q < np_:
+                        xs[conv_idx(q - r0, l16, li, lt)] = re[row, l16] + 1j * im[row, l16]
+        for r in range(CHUNK):
+            for b in range(KCONV_BANDS):
+                for li in range(lt):
+                    if r0 + r < np_:
+                        out[r0 + r, b, li] = xs[conv_idx(r, b, li, lt)]
+    return out
+
+
+@pytest.mark.parametrize("nx,np_,xs,fp64", [(75, 36, 57, True), (75, 36, 57, False), (30, 13, 24, True),
+                                             (90, 45, 68, False), (32, 17, 0, True), (48, 48, 0, True)])
+def test_conv_kernel_model_equals_fft_form(nx, np_, xs, fp64):
+    lt = 8 if fp64 else 16
+    rng = np.random.default_rng(nx + np_)
+    v = rng.standard_normal((lt, nx)) / nx  # V/N on the workgroup's lines
+    tile = rng.standard_normal((np_, KCONV_BANDS, lt)) + 1j * rng.standard_normal((np_, KCONV_BANDS, lt))
+    # chat[l][d] = sum_x v(x) e^{-2 pi i d x / nx} (k_line_dft)
+    x = np.arange(nx)
+    chat = np.array([[np.sum(v[l] * np.exp(-2j * np.pi * d * x / nx)) for d in range(nx)] for l in range(lt)])
+    got = model_kernel(tile, chat, np_, nx, lt, fp64)
+    freqs = (xs + np.arange(np_)) % nx
+    for li in range(lt):
+        for b in range(KCONV_BANDS):
+            full = np.zeros(nx, complex)
+            full[freqs] = tile[:, b, li]
+            psi = np.fft.ifft(full) * nx  # inverse transform without 1/N (V holds it)
+            ref = np.fft.fft(psi * v[li])[freqs]
+            np.testing.assert_allclose(got[:, b, li], ref, rtol=0, atol=1e-11 * max(1.0, np.abs(ref).max()))
+
+
+def test_conv_swizzle_is_a_permutation():
+    for lt in (8, 16):
+        idx = {conv_idx(p, b, li, lt) for p in range(CHUNK) for b in range(KCONV_BANDS) for li in range(lt)}
+        assert idx == set(range(CHUNK * KCONV_BANDS * lt))
+        # The 16 bands of one (p, li) land in distinct 128-byte bank rows' slots.
+        slot = 16 if lt == 8 else 8  # bytes per element (FP64 / FP32)
+        for li in range(lt):
+            pos = [(conv_idx(1, b, li, lt) * slot) % 128 // slot for b in range(KCONV_BANDS)]
+            assert len(set(pos)) == min(KCONV_BANDS, 128 // slot)
