@@ -428,7 +428,8 @@ def roofline(w, info, phases, prec):
 def tuning_info(ham):
     t = ham.info()
     return {k: t[k] for k in ("xline_split_fp32", "xline_split_fp64", "xline_register_fp32", "xline_register_fp64",
-                              "xline_lanes_fp32", "xline_lanes_fp64", "xline_kind_fp32", "xline_kind_fp64")}
+                              "xline_lanes_fp32", "xline_lanes_fp64", "xline_kind_fp32", "xline_kind_fp64",
+                              "plane_kind_fp32", "plane_kind_fp64")}
 
 
 SCF_ARMS = ("fp64", "fp64_concurrent", "mixed_batched", "mixed_batched_learned")
@@ -456,6 +457,8 @@ def main():
                          "Rayleigh-Ritz sizes m = 164..648 (nev = 160)")
     ap.add_argument("--overlap", action="store_true",
                     help="run the nonlocal kernels on a second stream beside the local ones (timed pass only)")
+    ap.add_argument("--plane-kernel", type=int, default=0,
+                    help="force the plane kernels of the headline block: 1 full-tile, 2 z-register (0: autotune)")
     ap.add_argument("--xline-split", type=int, default=0,
                     help="fix the x-line variant: d > 0 LDS y-slabs, -1..-3 one-lane, -4..-6 lane-pair, "
                          "-7..-9 lane-triple, -10..-12 residue-sequential register kernel (0 = autotune on the "
@@ -485,6 +488,8 @@ def main():
     ham = dzhip.Hamiltonian.from_workload(ctx, w)
     if args.xline_split:
         ham.set_tuning(args.xline_split)
+    if args.plane_kernel:
+        ham.set_plane_kernel(args.plane_kernel)
     if args.overlap:
         ham.set_overlap(True)
     if args.eig:

@@ -902,6 +902,19 @@ int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split) {
     });
 }
 
+int dzhip_ham_set_plane_kernel(dzhip_ham* ham, int kind) {
+    return guarded([&] {
+        DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
+        DZ_REQUIRE(kind == 0 || kind == kPlaneTile || (kind == kPlaneZreg && plane_zreg_usable(*ham)),
+                   DZHIP_ERR_ARG,
+                   "plane kernel must be 0 (autotune), 1 (full-tile LDS kernels) or 2 (z-register kernels, "
+                   "nz in 12..96 divisible by 3 with nz / 3 = 2^a 3^b 5^c and ny <= 126)");
+        std::lock_guard<std::mutex> lk(ham->mu);
+        ham->plane_fixed = kind;
+        ham->plane_kind[0] = ham->plane_kind[1] = 0;
+    });
+}
+
 int dzhip_ham_set_overlap(dzhip_ham* ham, int on) {
     return guarded([&] {
         DZ_REQUIRE(ham, DZHIP_ERR_STATE, "NULL ham");
@@ -917,11 +930,12 @@ int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out) {
         auto reg = [](int code) { return code < 0 ? (uint64_t)xline_reg_threads(code) : 0u; };
         auto waves = [](int code) { return code < 0 ? (uint64_t)xline_reg_lanes(code) : 0u; };
         auto kind = [](int code) { return code < 0 ? (uint64_t)(1 + xline_reg_kind(code)) : 0u; };
-        const uint64_t v[13] = {ham->npw, ham->ngrid, (uint64_t)ham->nplanes, (uint64_t)ham->nproj,
+        const uint64_t v[15] = {ham->npw, ham->ngrid, (uint64_t)ham->nplanes, (uint64_t)ham->nproj,
                                 ham->device_bytes(), slabs(ham->xline_split[1]), slabs(ham->xline_split[0]),
                                 reg(ham->xline_split[1]), reg(ham->xline_split[0]), waves(ham->xline_split[1]),
-                                waves(ham->xline_split[0]), kind(ham->xline_split[1]), kind(ham->xline_split[0])};
-        for (uint64_t i = 0; i < n_out && i < 13; ++i) out[i] = v[i];
+                                waves(ham->xline_split[0]), kind(ham->xline_split[1]), kind(ham->xline_split[0]),
+                                (uint64_t)ham->plane_kind[1], (uint64_t)ham->plane_kind[0]};
+        for (uint64_t i = 0; i < n_out && i < 15; ++i) out[i] = v[i];
     });
 }
 

@@ -46,6 +46,13 @@ struct Ham {
     //   plane_xabsent: LDS rows px[x] of the x values with no plane wave (zeroed in the x-line tile)
     int nplanes = 0, nxabsent = 0;
     DevBuf plane_start, plane_pw, plane_pos, plane_xpos, plane_ystart, plane_ycols, plane_xabsent;
+    // Active z-rows of the z-register plane kernels:
+    //   plane_zstart[p]..plane_zstart[p+1]: plane_zrows entries = the z-frequencies (natural
+    //                  index) present in plane p, ascending; row b of the plane is plane_zrows[b]
+    //   plane_posr[j]: LDS position py[y] + (ny+1)*b of entry j in the plane's active rows
+    //   zrow_max:      most active rows of any plane (sizes the LDS)
+    int zrow_max = 0;
+    DevBuf plane_zstart, plane_zrows, plane_posr;
 
     DevBuf kinetic64, kinetic32;
     DevBuf vloc64, vloc32;  // V_loc(r)/N in Ry
@@ -70,6 +77,10 @@ struct Ham {
     // xline_reg_kind), kXlineConv the convolution kernel; tune_fixed != 0 forces a variant (sweeps).
     int xline_split[2] = {0, 0};
     int tune_fixed = 0;
+    // Plane kernels per precision ([0] FP64, [1] FP32): 0 not yet tuned, kPlaneTile the full-tile
+    // LDS kernels, kPlaneZreg the z-register kernels; plane_fixed != 0 forces one.
+    int plane_kind[2] = {0, 0};
+    int plane_fixed = 0;
 
     // Profiling: one set of phase-boundary events per column pass, recorded on the stream
     // without waiting; read (and recycled) by collect_profile().
@@ -190,6 +201,10 @@ bool xline_conv_usable(const Ham& h);
 // Whether variant `code` (d >= 1 LDS slabs, a register variant or the convolution kernel) can run
 // on h in a precision.
 bool xline_code_usable(const Ham& h, int code, bool fp64);
+// Plane kernels: the full-tile LDS kernels (any grid), or the z-register kernels (nz in 12..96
+// divisible by 3 with nz / 3 = 2^a 3^b 5^c, ny <= 126).
+constexpr int kPlaneTile = 1, kPlaneZreg = 2;
+bool plane_zreg_usable(const Ham& h);
 void ham_set_local(Ham& h, const double* local_r);
 // local_r already on the device (read on h's stream).
 void ham_set_local_dev(Ham& h, const double* local_dev);

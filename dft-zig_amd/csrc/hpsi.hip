@@ -805,6 +805,218 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_fwd(
     }
 }
 
+// Plane kernels with the z-transform in registers.  The LDS holds only the rows of z-frequencies
+// that hold plane waves in the plane (its "active rows": about half of nz for a sphere in a grid of
+// twice its diameter), each a y-line of pitch ny + 1, so an FP64 75 x 75 plane needs 45 KB of LDS
+// instead of the 91 KB full tile and three workgroups share a CU.  The z-transform over all ny
+// y-lines runs in registers with three lanes per line (21 lines per wavefront; NZ = 3M with
+// M = 2^a 3^b 5^c): lane t holds the residue z = 3u + t, u < M, read from or written to the [z][y]
+// plane in global memory (consecutive lanes, consecutive y).  With w = e^{2 pi i / NZ}:
+//   inverse:  c_t[k0] = w^{t k0} sum_j Y[k0 + jM] e^{+2 pi i j t / 3}  (the three lanes of a line
+//             read the Y they need from the active rows), psi(3u + t) = inverse length-M
+//             transform of c_t;
+//   forward:  F_t = forward length-M transform of psi(3u + t), H_t[k0] = w^{-t k0} F_t[k0],
+//             X[k0 + jM] = sum_t e^{-2 pi i j t / 3} H_t[k0] (lane j gathers H_{j+1}, H_{j+2} from
+//             its partners and writes X to its active row).
+// The y-transforms run on the active rows only, in LDS, with the plane waves at the digit-reversed
+// y positions of the full-tile kernels.
+constexpr int kZrLinesPerWave = 21;
+constexpr int kZrMaxThreads = 384;  // ny <= 126
+
+__host__ __device__ inline int plane_zr_threads(int ny) {
+    return 64 * ((ny + kZrLinesPerWave - 1) / kZrLinesPerWave);
+}
+template <typename C>
+__host__ __device__ inline size_t plane_zr_lds_bytes(int ny, int nz, int nrow_max) {
+    return ((size_t)nrow_max * (ny + 1) + ny) * sizeof(C) + round_up((size_t)nz * sizeof(short), 16);
+}
+
+template <typename T, int NZ>
+__global__ __launch_bounds__(kZrMaxThreads) void k_plane_inv_zr(
+    const typename C2T<T>::type* __restrict__ x, size_t ldx, typename C2T<T>::type* __restrict__ tbuf,
+    const int* __restrict__ pstart, const int* __restrict__ ppw, const int* __restrict__ pposr,
+    const int* __restrict__ zstart, const short* __restrict__ zrows, int ny, int nplanes, FftAxis ay,
+    const typename C2T<T>::type* __restrict__ tw) {
+    using C = typename C2T<T>::type;
+    static_assert(NZ % 3 == 0, "z-register plane kernel needs nz divisible by 3");
+    constexpr int M = NZ / 3;
+    extern __shared__ unsigned char smem_raw[];
+    const int pitch = ny + 1;
+    const int p = blockIdx.x;
+    const size_t col = blockIdx.y;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int r0 = zstart[p], nrow = zstart[p + 1] - r0;
+    C* buf = reinterpret_cast<C*>(smem_raw);
+    C* twy = buf + nrow * pitch;
+    short* zb = reinterpret_cast<short*>(twy + ny);  // active row of z-frequency k, or -1
+    const C* xc = x + col * ldx;
+    const int j0 = pstart[p], j1 = pstart[p + 1];
+    C pv[SpherePrefetch<T>::value];
+    int pp[SpherePrefetch<T>::value];
+#pragma unroll
+    for (int u = 0; u < SpherePrefetch<T>::value; ++u) {
+        const int j = j0 + tid + u * nthr;
+        pp[u] = -1;
+        if (j < j1) {
+            pp[u] = pposr[j];
+            pv[u] = xc[ppw[j]];
+        }
+    }
+    for (int i = tid; i < nrow * pitch; i += nthr) buf[i] = C{T(0), T(0)};
+    load_twiddles(twy, tw + ay.tw_off, ny, tid, nthr);
+    for (int k = tid; k < NZ; k += nthr) zb[k] = -1;
+    __syncthreads();
+    for (int i = tid; i < nrow; i += nthr) zb[zrows[r0 + i]] = (short)i;
+#pragma unroll
+    for (int u = 0; u < SpherePrefetch<T>::value; ++u)
+        if (pp[u] >= 0) buf[pp[u]] = pv[u];
+    for (int j = j0 + tid + SpherePrefetch<T>::value * nthr; j < j1; j += nthr) buf[pposr[j]] = xc[ppw[j]];
+    __syncthreads();
+    // Inverse along y on the active rows (ends with a barrier).
+    lds_fft_lines<T, C, true, 1>(buf, ay, twy, LineSet{nrow, 1, pitch, nullptr}, tid, nthr);
+    const int lane = tid & 63, wave = tid >> 6;
+    const int yl = wave * kZrLinesPerWave + lane / 3;
+    if (lane == 63 || yl >= ny) return;  // no barrier below
+    const int t = lane % 3;
+    const bool t1 = t == 1, t2 = t == 2;
+    constexpr T kC3 = T(-0.5), kS3 = T(0.86602540378443864676);
+    // e^{+2 pi i t / 3} = (fc, fs) multiplies Y[k0 + M]; its conjugate multiplies Y[k0 + 2M].
+    const T fc = t == 0 ? T(1) : kC3, fs = t1 ? kS3 : (t2 ? -kS3 : T(0));
+    C c[M];
+#pragma unroll
+    for (int k0 = 0; k0 < M; ++k0) {
+        C sum = C{T(0), T(0)};
+        const int b0 = zb[k0], b1 = zb[k0 + M], b2 = zb[k0 + 2 * M];
+        if (b0 >= 0) sum = buf[b0 * pitch + yl];
+        if (b1 >= 0) {
+            const C v = buf[b1 * pitch + yl];
+            sum.x += v.x * fc - v.y * fs;
+            sum.y += v.x * fs + v.y * fc;
+        }
+        if (b2 >= 0) {
+            const C v = buf[b2 * pitch + yl];
+            sum.x += v.x * fc + v.y * fs;
+            sum.y += v.y * fc - v.x * fs;
+        }
+        const int e1 = k0 % NZ, e2 = (2 * k0) % NZ;
+        const T wc = t1 ? T(freg::kRoots<NZ>.c[e1]) : (t2 ? T(freg::kRoots<NZ>.c[e2]) : T(1));
+        const T ws = t1 ? T(freg::kRoots<NZ>.s[e1]) : (t2 ? T(freg::kRoots<NZ>.s[e2]) : T(0));
+        c[k0] = C{sum.x * wc - sum.y * ws, sum.x * ws + sum.y * wc};
+    }
+    freg::fft<M, 1>(c);
+    C* out = tbuf + (col * nplanes + p) * (size_t)ny * NZ;
+#pragma unroll
+    for (int u = 0; u < M; ++u) out[(3 * u + t) * ny + yl] = c[u];
+}
+
+template <typename T, int NZ>
+__global__ __launch_bounds__(kZrMaxThreads) void k_plane_fwd_zr(
+    const typename C2T<T>::type* __restrict__ tbuf, const typename C2T<T>::type* __restrict__ x, size_t ldx,
+    typename C2T<T>::type* __restrict__ y, size_t ldy, const T* __restrict__ kinetic,
+    const int* __restrict__ pstart, const int* __restrict__ ppw, const int* __restrict__ pposr,
+    const int* __restrict__ zstart, const short* __restrict__ zrows, int ny, int nplanes, FftAxis ay,
+    const typename C2T<T>::type* __restrict__ tw, int accumulate) {
+    using C = typename C2T<T>::type;
+    static_assert(NZ % 3 == 0, "z-register plane kernel needs nz divisible by 3");
+    constexpr int M = NZ / 3;
+    extern __shared__ unsigned char smem_raw[];
+    const int pitch = ny + 1;
+    const int p = blockIdx.x;
+    const size_t col = blockIdx.y;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int r0 = zstart[p], nrow = zstart[p + 1] - r0;
+    C* buf = reinterpret_cast<C*>(smem_raw);
+    C* twy = buf + nrow * pitch;
+    short* zb = reinterpret_cast<short*>(twy + ny);
+    const C* xc = x + col * ldx;
+    C* yc = y + col * ldy;
+    const int j0 = pstart[p], j1 = pstart[p + 1];
+    // Register lines: every lane takes part in the exchange; lanes past the last line and the idle
+    // lane 63 work on a valid line and write nothing.
+    const int lane = tid & 63, wave = tid >> 6;
+    const int t = lane < 63 ? lane % 3 : 0;
+    const int base_lane = lane < 63 ? lane - t : 63;
+    const int yraw = wave * kZrLinesPerWave + (lane < 63 ? lane / 3 : 0);
+    const bool own = lane < 63 && yraw < ny;
+    const int yl = yraw < ny ? yraw : ny - 1;
+    const C* in = tbuf + (col * nplanes + p) * (size_t)ny * NZ;
+    C a[M];
+#pragma unroll
+    for (int u = 0; u < M; ++u) a[u] = in[(3 * u + t) * ny + yl];
+    load_twiddles(twy, tw + ay.tw_off, ny, tid, nthr);
+    for (int k = tid; k < NZ; k += nthr) zb[k] = -1;
+    __syncthreads();
+    for (int i = tid; i < nrow; i += nthr) zb[zrows[r0 + i]] = (short)i;
+    __syncthreads();
+    freg::fft<M, -1>(a);
+    const bool t1 = t == 1, t2 = t == 2;
+#pragma unroll
+    for (int k = 1; k < M; ++k) {
+        const int e1 = k % NZ, e2 = (2 * k) % NZ;
+        const T wc = t1 ? T(freg::kRoots<NZ>.c[e1]) : (t2 ? T(freg::kRoots<NZ>.c[e2]) : T(1));
+        const T ws = t1 ? T(freg::kRoots<NZ>.s[e1]) : (t2 ? T(freg::kRoots<NZ>.s[e2]) : T(0));
+        const T px = a[k].x * wc + a[k].y * ws, py = a[k].y * wc - a[k].x * ws;
+        a[k].x = px;
+        a[k].y = py;
+    }
+    // Lane j holds H_j, its partners H_{j+1} and H_{j+2} (indices mod 3):
+    // X[k0 + jM] = own f0 + next f1 + after-next f2 with j = 0: 1, 1, 1; j = 1: w3^-1, w3, 1;
+    // j = 2: w3^-1, 1, w3 (w3 = e^{2 pi i / 3}).
+    const int p1 = base_lane + (t + 1) % 3, p2 = base_lane + (t + 2) % 3;
+    constexpr T kC3 = T(-0.5), kS3 = T(0.86602540378443864676);
+    const T own_c = t == 0 ? T(1) : kC3, own_s = t == 0 ? T(0) : -kS3;
+    const T n1_c = t1 ? kC3 : T(1), n1_s = t1 ? kS3 : T(0);
+    const T n2_c = t2 ? kC3 : T(1), n2_s = t2 ? kS3 : T(0);
+#pragma unroll
+    for (int k0 = 0; k0 < M; ++k0) {
+        const int b0 = zb[k0], b1 = zb[k0 + M], b2 = zb[k0 + 2 * M];
+        if ((b0 & b1 & b2) < 0) continue;  // none of the three frequencies is active (uniform)
+        const C h1 = lane_read(a[k0], p1), h2 = lane_read(a[k0], p2);
+        const C o = a[k0];
+        const T xr = o.x * own_c - o.y * own_s + h1.x * n1_c - h1.y * n1_s + h2.x * n2_c - h2.y * n2_s;
+        const T xi = o.x * own_s + o.y * own_c + h1.x * n1_s + h1.y * n1_c + h2.x * n2_s + h2.y * n2_c;
+        const int bm = t == 0 ? b0 : (t1 ? b1 : b2);
+        if (own && bm >= 0) buf[bm * pitch + yl] = C{xr, xi};
+    }
+    int gg[SpherePrefetch<T>::value], gp[SpherePrefetch<T>::value];
+    T gk[SpherePrefetch<T>::value];
+    C gx[SpherePrefetch<T>::value], gy[SpherePrefetch<T>::value];
+#pragma unroll
+    for (int u = 0; u < SpherePrefetch<T>::value; ++u) {
+        const int j = j0 + tid + u * nthr;
+        gg[u] = -1;
+        if (j < j1) {
+            const int g = ppw[j];
+            gg[u] = g;
+            gp[u] = pposr[j];
+            gk[u] = kinetic[g];
+            gx[u] = xc[g];
+            gy[u] = accumulate ? yc[g] : C{T(0), T(0)};
+        }
+    }
+    __syncthreads();
+    // Forward along y on the active rows; frequency y lands at its digit-reversed position.
+    lds_fft_lines<T, C, false, -1>(buf, ay, twy, LineSet{nrow, 1, pitch, nullptr}, tid, nthr);
+#pragma unroll
+    for (int u = 0; u < SpherePrefetch<T>::value; ++u)
+        if (gg[u] >= 0) {
+            const C v = buf[gp[u]];
+            yc[gg[u]] = C{v.x + gk[u] * gx[u].x + gy[u].x, v.y + gk[u] * gx[u].y + gy[u].y};
+        }
+    for (int j = j0 + tid + SpherePrefetch<T>::value * nthr; j < j1; j += nthr) {
+        const int g = ppw[j];
+        const C v = buf[pposr[j]];
+        const T kin = kinetic[g];
+        const C xv = xc[g];
+        C o = C{v.x + kin * xv.x, v.y + kin * xv.y};
+        if (accumulate) {
+            const C prev = yc[g];
+            o = C{o.x + prev.x, o.y + prev.y};
+        }
+        yc[g] = o;
+    }
+}
+
 // Density: one workgroup per (z, band group).  For each band of the group, the x-lines of the
 // z-plane are gathered from T (sphere planes after the z/y inverse transforms), inverse
 // transformed along x, and w_b |psi_b(r)|^2 is accumulated in FP64 registers; each thread owns
@@ -1449,6 +1661,7 @@ __global__ void k_d2f(const double* __restrict__ in, float* __restrict__ out, si
 size_t Ham::device_bytes() const {
     const DevBuf* bufs[] = {&plane_start, &plane_pw, &plane_pos, &plane_xpos, &plane_ystart,
                             &plane_ycols, &plane_xabsent, &kinetic64,
+                            &plane_zstart, &plane_zrows, &plane_posr,
                             &kinetic32, &vloc64, &vloc32, &a64, &a32, &meta, &block_rows, &dm64, &dm32,
                             &xstage, &ystage, &x32, &y32, &tbuf, &partial, &qbuf,
                             &vloc32_t, &vloc64_t, &chat64, &chat32};
@@ -1536,13 +1749,39 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
     std::vector<int> pw(h.npw), pos(h.npw);
     const int pitch = h.ny + 1;  // LDS row pitch of the [z][y] plane tile
     std::vector<std::vector<char>> yused(h.nplanes, std::vector<char>(h.ny, 0));
+    std::vector<std::vector<int>> zrow(h.nplanes, std::vector<int>(h.nz, -1));
+    std::vector<int> pidx(h.npw);
     for (size_t i = 0; i < h.npw; ++i) {
         const int p = plane_of_x[xs[i]];
         const int j = fill[p]++;
         pw[j] = (int)i;
+        pidx[j] = p;
         pos[j] = py[ys[i]] + pitch * pz[zs[i]];
         yused[p][ys[i]] = 1;
+        zrow[p][zs[i]] = 0;
     }
+    // Active z-rows per plane (ascending z-frequency) and the entries' positions in them.
+    std::vector<int> zstart(h.nplanes + 1, 0);
+    std::vector<short> zrows;
+    h.zrow_max = 0;
+    for (int p = 0; p < h.nplanes; ++p) {
+        int b = 0;
+        for (int zz = 0; zz < h.nz; ++zz)
+            if (zrow[p][zz] >= 0) {
+                zrow[p][zz] = b++;
+                zrows.push_back((short)zz);
+            }
+        zstart[p + 1] = (int)zrows.size();
+        h.zrow_max = std::max(h.zrow_max, b);
+    }
+    std::vector<int> posr(h.npw);
+    for (size_t j = 0; j < h.npw; ++j) {
+        const size_t i = (size_t)pw[j];
+        posr[j] = py[ys[i]] + pitch * zrow[pidx[j]][zs[i]];
+    }
+    upload(h.plane_zstart, zstart);
+    upload(h.plane_zrows, zrows);
+    upload(h.plane_posr, posr);
     // Per plane, the LDS columns (digit-reversed y) that hold plane waves: only these need
     // the z-transform.
     std::vector<int> ystart(h.nplanes + 1, 0);
@@ -1776,7 +2015,8 @@ void ham_release(Ham& h) {
                       &h.plane_ycols, &h.plane_xabsent, &h.kinetic64,
                       &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.block_rows, &h.dm64,
                       &h.dm32, &h.xstage, &h.ystage, &h.x32, &h.y32, &h.tbuf, &h.partial, &h.qbuf,
-                      &h.dens_rho, &h.dens_w, &h.vloc32_t, &h.vloc64_t};
+                      &h.dens_rho, &h.dens_w, &h.vloc32_t, &h.vloc64_t, &h.chat64, &h.chat32,
+                      &h.plane_zstart, &h.plane_zrows, &h.plane_posr};
     for (DevBuf* b : bufs) b->release();
     h.tab.release();
     for (auto& set : h.ev_sets)
@@ -1941,6 +2181,147 @@ int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
     return sp;
 }
 
+bool plane_zreg_usable(const Ham& h) {
+    return xline_tri_supported(h.nz) && h.ny <= kZrLinesPerWave * (kZrMaxThreads / 64) && h.zrow_max > 0 &&
+           plane_zr_lds_bytes<double2>(h.ny, h.nz, h.zrow_max) <= kMaxLdsBytes;
+}
+
+// Sphere -> [z][y] planes of every band (kind: kPlaneTile or kPlaneZreg).
+template <typename T>
+void launch_plane_inv(Ham& h, int kind, const typename C2T<T>::type* x, size_t ld, typename C2T<T>::type* tb,
+                      size_t nc, hipStream_t s) {
+    using C = typename C2T<T>::type;
+    const C* tw = Sel<T>::tw(h);
+    const dim3 grid((unsigned)h.nplanes, (unsigned)nc);
+    if (kind == kPlaneZreg) {
+        const size_t lds = plane_zr_lds_bytes<C>(h.ny, h.nz, h.zrow_max);
+        const dim3 block((unsigned)plane_zr_threads(h.ny));
+        bool done = false;
+        switch (h.nz) {
+#define DZ_ZR_CASE(N)                                                                                        \
+    case N:                                                                                                  \
+        set_max_dynamic_lds((const void*)k_plane_inv_zr<T, N>, lds);                                         \
+        hipLaunchKernelGGL((k_plane_inv_zr<T, N>), grid, block, lds, s, x, ld, tb, h.plane_start.as<int>(),  \
+                           h.plane_pw.as<int>(), h.plane_posr.as<int>(), h.plane_zstart.as<int>(),           \
+                           h.plane_zrows.as<short>(), h.ny, h.nplanes, h.tab.ax[1], tw);                     \
+        done = true;                                                                                         \
+        break;
+            DZ_XTRI_SIZES(DZ_ZR_CASE)
+#undef DZ_ZR_CASE
+            default:
+                break;
+        }
+        DZ_REQUIRE(done, DZHIP_ERR_STATE, "no z-register plane kernel for this nz");
+    } else {
+        const size_t lds = plane_lds_bytes<C>(h.ny, h.nz);
+        set_max_dynamic_lds((const void*)k_plane_inv<T>, lds);
+        hipLaunchKernelGGL(k_plane_inv<T>, grid, dim3(TileThreads<T>::value), lds, s, x, ld, tb,
+                           h.plane_start.as<int>(), h.plane_pw.as<int>(), h.plane_pos.as<int>(),
+                           h.plane_ystart.as<int>(), h.plane_ycols.as<short>(), h.ny, h.nz, h.nplanes,
+                           h.tab.ax[1], h.tab.ax[2], tw);
+    }
+    DZ_HIP(hipGetLastError());
+}
+
+// [z][y] planes -> y (+)= forward transform at the sphere + kinetic * x.
+template <typename T>
+void launch_plane_fwd(Ham& h, int kind, const typename C2T<T>::type* tb, const typename C2T<T>::type* x,
+                      size_t ld, typename C2T<T>::type* y, int accumulate, size_t nc, hipStream_t s) {
+    using C = typename C2T<T>::type;
+    const C* tw = Sel<T>::tw(h);
+    const dim3 grid((unsigned)h.nplanes, (unsigned)nc);
+    if (kind == kPlaneZreg) {
+        const size_t lds = plane_zr_lds_bytes<C>(h.ny, h.nz, h.zrow_max);
+        const dim3 block((unsigned)plane_zr_threads(h.ny));
+        bool done = false;
+        switch (h.nz) {
+#define DZ_ZR_CASE(N)                                                                                        \
+    case N:                                                                                                  \
+        set_max_dynamic_lds((const void*)k_plane_fwd_zr<T, N>, lds);                                         \
+        hipLaunchKernelGGL((k_plane_fwd_zr<T, N>), grid, block, lds, s, tb, x, ld, y, ld, Sel<T>::kin(h),    \
+                           h.plane_start.as<int>(), h.plane_pw.as<int>(), h.plane_posr.as<int>(),            \
+                           h.plane_zstart.as<int>(), h.plane_zrows.as<short>(), h.ny, h.nplanes,             \
+                           h.tab.ax[1], tw, accumulate);                                                     \
+        done = true;                                                                                         \
+        break;
+            DZ_XTRI_SIZES(DZ_ZR_CASE)
+#undef DZ_ZR_CASE
+            default:
+                break;
+        }
+        DZ_REQUIRE(done, DZHIP_ERR_STATE, "no z-register plane kernel for this nz");
+    } else {
+        const size_t lds = plane_lds_bytes<C>(h.ny, h.nz);
+        set_max_dynamic_lds((const void*)k_plane_fwd<T>, lds);
+        hipLaunchKernelGGL(k_plane_fwd<T>, grid, dim3(TileThreads<T>::value), lds, s, tb, x, ld, y, ld,
+                           Sel<T>::kin(h), h.plane_start.as<int>(), h.plane_pw.as<int>(), h.plane_pos.as<int>(),
+                           h.plane_ystart.as<int>(), h.plane_ycols.as<short>(), h.ny, h.nz, h.nplanes,
+                           h.tab.ax[1], h.tab.ax[2], tw, accumulate);
+    }
+    DZ_HIP(hipGetLastError());
+}
+
+// Process-wide plane-kernel choices by (device, grid, sphere shape, precision), shared like the
+// x-line tuning.
+std::mutex g_plane_tune_mu;
+std::map<std::array<int, 8>, int> g_plane_tune;
+
+// Plane kernels, chosen once per shape and precision by timing plane_inv + plane_fwd of each kind
+// that can run on the pass's real shape.  The trials write tbuf and y (not accumulating), so this
+// runs before plane_inv and before the nonlocal chain writes y.
+template <typename T>
+int plane_variant(Ham& h, const typename C2T<T>::type* x, typename C2T<T>::type* y, typename C2T<T>::type* tb,
+                  size_t nc, hipStream_t s) {
+    constexpr bool fp32 = std::is_same<T, float>::value;
+    int& kind = h.plane_kind[fp32 ? 1 : 0];
+    if (kind != 0) return kind;
+    const bool zr = plane_zreg_usable(h);
+    if (h.plane_fixed != 0) {
+        kind = h.plane_fixed == kPlaneZreg && !zr ? kPlaneTile : h.plane_fixed;
+        return kind;
+    }
+    if (!zr) {
+        kind = kPlaneTile;
+        return kind;
+    }
+    if (nc < kXlineTuneMinCols) return kPlaneZreg;
+    const std::array<int, 8> key = {h.ctx->device, h.nx, h.ny, h.nz, h.nplanes, h.zrow_max, (int)h.npw, fp32 ? 1 : 0};
+    std::lock_guard<std::mutex> lk(g_plane_tune_mu);
+    const auto it = g_plane_tune.find(key);
+    if (it != g_plane_tune.end()) {
+        kind = it->second;
+        return kind;
+    }
+    DZ_HIP(hipDeviceSynchronize());
+    hipEvent_t e0, e1;
+    DZ_HIP(hipEventCreate(&e0));
+    DZ_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_k = kPlaneZreg;
+    for (int k : {kPlaneZreg, kPlaneTile}) {
+        launch_plane_inv<T>(h, k, x, h.npw, tb, nc, s);
+        launch_plane_fwd<T>(h, k, tb, x, h.npw, y, 0, nc, s);
+        DZ_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r) {
+            launch_plane_inv<T>(h, k, x, h.npw, tb, nc, s);
+            launch_plane_fwd<T>(h, k, tb, x, h.npw, y, 0, nc, s);
+        }
+        DZ_HIP(hipEventRecord(e1, s));
+        DZ_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        DZ_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_k = k;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    kind = best_k;
+    g_plane_tune[key] = best_k;
+    return kind;
+}
+
 template <typename T>
 void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, bool nonlocal) {
     using C = typename C2T<T>::type;
@@ -1949,7 +2330,6 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
     if (ncols == 0) return;
     DZ_REQUIRE(!local || h.has_local, DZHIP_ERR_STATE, "local potential not set");
     hipStream_t s = h.stream();
-    const C* tw = Sel<T>::tw(h);
     const size_t ld = h.npw;
     const bool do_nl = nonlocal && h.nproj > 0;
 
@@ -1971,13 +2351,14 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
             h.qbuf.ensure(nc * h.nproj * sizeof(C));
         }
         C* tb = nullptr;
-        int split = 1;
+        int split = 1, pkind = kPlaneTile;
         if (local) {
             h.tbuf.ensure(nc * h.nplanes * (size_t)h.ny * h.nz * sizeof(C));
             tb = h.tbuf.as<C>();
             // Tuned before plane_inv fills tbuf (the trial launches transform tbuf in place) and
             // before the nonlocal kernels start, so the trials run alone.
             split = xline_variant<T>(h, tb, nc, s);
+            pkind = plane_variant<T>(h, xc, yc, tb, nc, s);
         }
         // The nonlocal chain (matrix cores) and plane_inv + x-line (memory and latency bound) are
         // independent: they run on two streams and join before plane_fwd, which adds the local
@@ -2021,25 +2402,12 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
         if (overlap) DZ_HIP(hipEventRecord(h.ev_join, snl));
         mark(h, evs, 2);
         if (local) {
-            const size_t lds_p = plane_lds_bytes<C>(h.ny, h.nz);
-            constexpr int nt = TileThreads<T>::value;
-            set_max_dynamic_lds((const void*)k_plane_inv<T>, lds_p);
-            set_max_dynamic_lds((const void*)k_plane_fwd<T>, lds_p);
-            hipLaunchKernelGGL(k_plane_inv<T>, dim3(h.nplanes, (unsigned)nc), dim3(nt), lds_p, s,
-                               xc, ld, tb, h.plane_start.as<int>(), h.plane_pw.as<int>(),
-                               h.plane_pos.as<int>(), h.plane_ystart.as<int>(), h.plane_ycols.as<short>(),
-                               h.ny, h.nz, h.nplanes, h.tab.ax[1], h.tab.ax[2], tw);
-            DZ_HIP(hipGetLastError());
+            launch_plane_inv<T>(h, pkind, xc, ld, tb, nc, s);
             mark(h, evs, 3);
             launch_xline_any<T>(h, tb, nc, split, s);
             mark(h, evs, 4);
             if (overlap) DZ_HIP(hipStreamWaitEvent(s, h.ev_join, 0));
-            hipLaunchKernelGGL(k_plane_fwd<T>, dim3(h.nplanes, (unsigned)nc), dim3(nt), lds_p, s,
-                               tb, xc, ld, yc, ld, Sel<T>::kin(h), h.plane_start.as<int>(),
-                               h.plane_pw.as<int>(), h.plane_pos.as<int>(), h.plane_ystart.as<int>(),
-                               h.plane_ycols.as<short>(), h.ny, h.nz, h.nplanes, h.tab.ax[1],
-                               h.tab.ax[2], tw, do_nl ? 1 : 0);
-            DZ_HIP(hipGetLastError());
+            launch_plane_fwd<T>(h, pkind, tb, xc, ld, yc, do_nl ? 1 : 0, nc, s);
         } else {
             if (!do_nl) {
                 const size_t n = nc * ld;

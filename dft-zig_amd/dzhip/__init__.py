@@ -37,7 +37,7 @@ EXPORTS = (
     "dzhip_mlp_loss_grad", "dzhip_mlp_train", "dzhip_cg_solve",
     "dzhip_scf_grid_create", "dzhip_scf_grid_destroy", "dzhip_scf_grid_set_ion", "dzhip_scf_grid_init",
     "dzhip_ham_set_local_grid", "dzhip_scf_grid_add_solution_density", "dzhip_scf_grid_hxc",
-    "dzhip_scf_grid_mix", "dzhip_scf_grid_read", "dzhip_ham_set_tuning", "dzhip_scf_grid_set_dielectric",
+    "dzhip_scf_grid_mix", "dzhip_scf_grid_read", "dzhip_ham_set_tuning", "dzhip_ham_set_plane_kernel", "dzhip_scf_grid_set_dielectric",
     "dzhip_scf_grid_set_xc", "dzhip_ham_set_overlap",
     "dzhip_scf_grid_init_rho", "dzhip_heev", "dzhip_heev_dev", "dzhip_heev_check",
     "dzhip_hamset_create", "dzhip_hamset_destroy", "dzhip_hamset_offsets", "dzhip_hamset_apply_dev",
@@ -167,6 +167,7 @@ def _load():
         "dzhip_scf_grid_mix": ([vp, ct.c_double, i32], i32),
         "dzhip_scf_grid_read": ([vp, i32, dp, u64], i32),
         "dzhip_ham_set_tuning": ([vp, i32], i32),
+        "dzhip_ham_set_plane_kernel": ([vp, i32], i32),
         "dzhip_scf_grid_set_dielectric": ([vp, ct.c_double, ct.c_double], i32),
         "dzhip_scf_grid_set_xc": ([vp, i32], i32),
         "dzhip_ham_set_overlap": ([vp, i32], i32),
@@ -538,16 +539,21 @@ class Hamiltonian:
         lane-pair / lane-triple register kernel in 256 / 128 / 64-thread workgroups (0: autotune)."""
         check(L().dzhip_ham_set_tuning(self.handle, int(xline_split)))
 
+    def set_plane_kernel(self, kind: int = 0):
+        """Force the plane kernels: 1 full-tile LDS, 2 z-register (0: autotune)."""
+        check(L().dzhip_ham_set_plane_kernel(self.handle, int(kind)))
+
     def set_overlap(self, on: bool = True):
         """Run the nonlocal kernels on a second stream, concurrently with the local ones."""
         check(L().dzhip_ham_set_overlap(self.handle, 1 if on else 0))
 
     def info(self):
-        out = (ct.c_uint64 * 13)()
-        check(L().dzhip_ham_info(self.handle, out, 13))
+        out = (ct.c_uint64 * 15)()
+        check(L().dzhip_ham_info(self.handle, out, 15))
         keys = ("n_pw", "grid_points", "planes", "projectors", "device_bytes", "xline_split_fp32",
                 "xline_split_fp64", "xline_register_fp32", "xline_register_fp64", "xline_lanes_fp32",
-                "xline_lanes_fp64", "xline_kind_fp32", "xline_kind_fp64")
+                "xline_lanes_fp64", "xline_kind_fp32", "xline_kind_fp64", "plane_kind_fp32",
+                "plane_kind_fp64")
         return dict(zip(keys, [int(v) for v in out]))
 
     def close(self):

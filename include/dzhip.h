@@ -405,6 +405,15 @@ int dzhip_scf_grid_read(dzhip_scf_grid* grid, int which, double* out, uint64_t c
  * device, grid and sphere shape in the process. */
 int dzhip_ham_set_tuning(dzhip_ham* ham, int xline_split);
 
+/* Plane-kernel choice (no reference counterpart): the sphere <-> [z][y] plane transforms run either
+ * as full-tile LDS kernels (kind 1: the whole ny x nz plane of one band in LDS, z- and y-transforms
+ * as LDS stages; any grid) or as z-register kernels (kind 2: only the plane's active z-frequency rows
+ * in LDS for the y-transforms, the z-transform of every y-line in registers, three lanes per line;
+ * nz in 12..96 divisible by 3 with nz / 3 = 2^a 3^b 5^c and ny <= 126).  0 (default) times both on
+ * the first apply of each precision with at least 16 columns and keeps the faster; the result is
+ * shared like the x-line tuning.  Results agree to rounding either way. */
+int dzhip_ham_set_plane_kernel(dzhip_ham* ham, int kind);
+
 /* Overlap (no reference counterpart; default off): with on != 0 the nonlocal projection / D /
  * back-projection kernels run on a second stream of the context, concurrently with the local
  * plane_inv and x-line kernels, and join before plane_fwd.  Results are identical either way;
@@ -420,6 +429,7 @@ int dzhip_ham_set_overlap(dzhip_ham* ham, int on);
  * FP64, [9] / [10] = threads per x-line of that register kernel, 1, 2 or 3 (FP32 / FP64; 0 = the
  * LDS or convolution kernel, or not tuned yet), [11] / [12] = its kind (FP32 / FP64): 1 one-lane,
  * 2 lane-pair, 3 lane-triple, 4 residue-sequential, 5 convolution (0 = the LDS kernel or not
+ * tuned yet), [13] / [14] = the plane kernels (FP32 / FP64): 1 full-tile, 2 z-register (0 = not
  * tuned yet). */
 int dzhip_ham_info(dzhip_ham* ham, uint64_t* out, uint64_t n_out);
 

@@ -334,6 +334,67 @@ def test_hpsi_conv_xline_vs_oracle(dz, ctx, which, nbands, prec):
         ham.close()
 
 
+@pytest.mark.parametrize("grid,nbands", [((72, 72, 72), 12), ((75, 75, 75), 12), ((90, 90, 90), 6),
+                                         ((75, 64, 90), 6), ((100, 72, 45), 6), ((40, 120, 48), 4)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hpsi_zreg_plane_vs_oracle(dz, ctx, grid, nbands, prec):
+    """The z-register plane kernels (plane kernel 2: the y-transforms on the plane's active
+    z-frequency rows in LDS, the z-transform of every y-line in registers on three lanes per line)
+    against the FP64 oracle apply, and against the full-tile kernels (plane kernel 1) on the same
+    context: z-lines of 45 to 90 points (thirds of 15 to 30), ny from 64 to 120 (four to six
+    wavefronts of 21 lines, the last partly idle), non-cubic grids, and the auto choice."""
+    w = workloads.bench(nbands=nbands, grid=grid)
+    ham = make_ham(dz, ctx, w)
+    key = "fp32" if prec else "fp64"
+    tol = TOL32 if prec else TOL64
+    try:
+        ref = hamiltonian.apply_hamiltonian_batched(oracle_ctx(w), w.psi)
+        ham.set_plane_kernel(2)
+        got = ham.apply(w.psi, precision=prec)
+        assert ham.info()[f"plane_kind_{key}"] == 2
+        ham.set_plane_kernel(1)
+        tile = ham.apply(w.psi, precision=prec)
+        assert ham.info()[f"plane_kind_{key}"] == 1
+        ham.set_plane_kernel(0)
+        auto = ham.apply(w.psi, precision=prec)
+        errs = (relerr(got, ref), relerr(tile, ref), relerr(auto, ref), relerr(got, tile))
+        print(grid, prec, errs, ham.info()[f"plane_kind_{key}"])
+        assert max(errs) < tol, errs
+    finally:
+        ham.close()
+    # No projectors: plane_fwd writes y instead of adding to the nonlocal result.
+    ham2 = dz.Hamiltonian(ctx, w.grid, tables.fft_indices(w.basis, w.grid), w.basis.kinetic,
+                          w.basis.cart, None, w.inv_volume)
+    try:
+        ham2.set_local(w.local_r)
+        ham2.set_plane_kernel(2)
+        ctx_o = oracle_ctx(w)
+        ctx_o.species = []
+        ref_l = hamiltonian.apply_hamiltonian_batched(ctx_o, w.psi[:2])
+        assert relerr(ham2.apply(w.psi[:2], precision=prec), ref_l) < tol
+    finally:
+        ham2.close()
+
+
+def test_plane_kernel_choice_errors(dz, ctx):
+    """Plane kernel 2 needs nz divisible by 3 with a register-FFT third (8^3: no); 1 and 0 always
+    run; other codes are rejected."""
+    w = workloads.tiny()
+    ham = make_ham(dz, ctx, w)
+    try:
+        with pytest.raises(dz.BadArgument):
+            ham.set_plane_kernel(2)
+        with pytest.raises(dz.BadArgument):
+            ham.set_plane_kernel(3)
+        ham.set_plane_kernel(0)
+        g = ham.apply(w.psi)
+        assert ham.info()["plane_kind_fp64"] == 1
+        ref = hamiltonian.apply_hamiltonian_batched(oracle_ctx(w), w.psi)
+        assert relerr(g, ref) < TOL64
+    finally:
+        ham.close()
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_hpsi_tri_xline_workgroup_sizes(dz, ctx, prec):
     """The lane-triple x-line kernel (21 lines per wavefront, its last lane idle) in 256-, 128- and
