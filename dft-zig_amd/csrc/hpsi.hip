@@ -820,6 +820,8 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_fwd(
 //             its partners and writes X to its active row).
 // The y-transforms run on the active rows only, in LDS, with the plane waves at the digit-reversed
 // y positions of the full-tile kernels.
+// Three wavefronts per SIMD (at most 168 VGPRs): with the LDS of a 75^3 or 90^3 plane's active
+// rows, two or three workgroups of four or five wavefronts then fit a CU.
 constexpr int kZrLinesPerWave = 21;
 constexpr int kZrMaxThreads = 384;  // ny <= 126
 
@@ -832,7 +834,7 @@ __host__ __device__ inline size_t plane_zr_lds_bytes(int ny, int nz, int nrow_ma
 }
 
 template <typename T, int NZ>
-__global__ __launch_bounds__(kZrMaxThreads) void k_plane_inv_zr(
+__global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_inv_zr(
     const typename C2T<T>::type* __restrict__ x, size_t ldx, typename C2T<T>::type* __restrict__ tbuf,
     const int* __restrict__ pstart, const int* __restrict__ ppw, const int* __restrict__ pposr,
     const int* __restrict__ zstart, const short* __restrict__ zrows, int ny, int nplanes, FftAxis ay,
@@ -910,7 +912,7 @@ __global__ __launch_bounds__(kZrMaxThreads) void k_plane_inv_zr(
 }
 
 template <typename T, int NZ>
-__global__ __launch_bounds__(kZrMaxThreads) void k_plane_fwd_zr(
+__global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_fwd_zr(
     const typename C2T<T>::type* __restrict__ tbuf, const typename C2T<T>::type* __restrict__ x, size_t ldx,
     typename C2T<T>::type* __restrict__ y, size_t ldy, const T* __restrict__ kinetic,
     const int* __restrict__ pstart, const int* __restrict__ ppw, const int* __restrict__ pposr,
@@ -959,6 +961,10 @@ __global__ __launch_bounds__(kZrMaxThreads) void k_plane_fwd_zr(
         a[k].x = px;
         a[k].y = py;
     }
+    // The transform is finished here: otherwise the compiler sinks its last stage into the
+    // conditional blocks below and keeps the earlier stages' values live across all of them.
+#pragma unroll
+    for (int k = 0; k < M; ++k) asm volatile("" : "+v"(a[k].x), "+v"(a[k].y));
     // Lane j holds H_j, its partners H_{j+1} and H_{j+2} (indices mod 3):
     // X[k0 + jM] = own f0 + next f1 + after-next f2 with j = 0: 1, 1, 1; j = 1: w3^-1, w3, 1;
     // j = 2: w3^-1, 1, w3 (w3 = e^{2 pi i / 3}).
