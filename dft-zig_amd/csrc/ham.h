@@ -49,10 +49,12 @@ struct Ham {
     // Active z-rows of the z-register plane kernels:
     //   plane_zstart[p]..plane_zstart[p+1]: plane_zrows entries = the z-frequencies (natural
     //                  index) present in plane p, ascending; row b of the plane is plane_zrows[b]
-    //   plane_posr[j]: LDS position py[y] + (ny+1)*b of entry j in the plane's active rows
+    //   plane_posr[j]: LDS position py[y] + (ny|1)*b of entry j in the plane's active rows
     //   zrow_max:      most active rows of any plane (sizes the LDS)
+    //   plane_pw0[p]:  first plane-wave index of plane p when its entries are one run of the
+    //                  basis (plane_pw[j] = plane_pw0[p] + j - plane_start[p]), else -1
     int zrow_max = 0;
-    DevBuf plane_zstart, plane_zrows, plane_posr;
+    DevBuf plane_zstart, plane_zrows, plane_posr, plane_pw0;
 
     DevBuf kinetic64, kinetic32;
     DevBuf vloc64, vloc32;  // V_loc(r)/N in Ry

@@ -807,7 +807,7 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_fwd(
 
 // Plane kernels with the z-transform in registers.  The LDS holds only the rows of z-frequencies
 // that hold plane waves in the plane (its "active rows": about half of nz for a sphere in a grid of
-// twice its diameter), each a y-line of pitch ny + 1, so an FP64 75 x 75 plane needs 45 KB of LDS
+// twice its diameter), each a y-line of odd pitch ny | 1, so an FP64 75 x 75 plane needs 43 KB of LDS
 // instead of the 91 KB full tile and three workgroups share a CU.  The z-transform over all ny
 // y-lines runs in registers with three lanes per line (21 lines per wavefront; NZ = 3M with
 // M = 2^a 3^b 5^c): lane t holds the residue z = 3u + t, u < M, read from or written to the [z][y]
@@ -828,22 +828,25 @@ constexpr int kZrMaxThreads = 384;  // ny <= 126
 __host__ __device__ inline int plane_zr_threads(int ny) {
     return 64 * ((ny + kZrLinesPerWave - 1) / kZrLinesPerWave);
 }
+// Row pitch of the active rows, odd in elements: the y-stages' threads take consecutive rows of
+// one butterfly, and an odd pitch spreads their 16-byte (FP64) accesses over distinct banks.
+__host__ __device__ inline int zr_pitch(int ny) { return ny | 1; }
 template <typename C>
 __host__ __device__ inline size_t plane_zr_lds_bytes(int ny, int nz, int nrow_max) {
-    return ((size_t)nrow_max * (ny + 1) + ny) * sizeof(C) + round_up((size_t)nz * sizeof(short), 16);
+    return ((size_t)nrow_max * zr_pitch(ny) + ny) * sizeof(C) + round_up((size_t)nz * sizeof(short), 16);
 }
 
 template <typename T, int NZ>
 __global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_inv_zr(
     const typename C2T<T>::type* __restrict__ x, size_t ldx, typename C2T<T>::type* __restrict__ tbuf,
-    const int* __restrict__ pstart, const int* __restrict__ ppw, const int* __restrict__ pposr,
-    const int* __restrict__ zstart, const short* __restrict__ zrows, int ny, int nplanes, FftAxis ay,
-    const typename C2T<T>::type* __restrict__ tw) {
+    const int* __restrict__ pstart, const int* __restrict__ ppw, const int* __restrict__ ppw0,
+    const int* __restrict__ pposr, const int* __restrict__ zstart, const short* __restrict__ zrows, int ny,
+    int nplanes, FftAxis ay, const typename C2T<T>::type* __restrict__ tw) {
     using C = typename C2T<T>::type;
     static_assert(NZ % 3 == 0, "z-register plane kernel needs nz divisible by 3");
     constexpr int M = NZ / 3;
     extern __shared__ unsigned char smem_raw[];
-    const int pitch = ny + 1;
+    const int pitch = zr_pitch(ny);
     const int p = blockIdx.x;
     const size_t col = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -853,6 +856,8 @@ __global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_inv_zr(
     short* zb = reinterpret_cast<short*>(twy + ny);  // active row of z-frequency k, or -1
     const C* xc = x + col * ldx;
     const int j0 = pstart[p], j1 = pstart[p + 1];
+    // A plane whose entries are one run of the basis (h-major order) is read without the index.
+    const int pw0 = ppw0[p];
     C pv[SpherePrefetch<T>::value];
     int pp[SpherePrefetch<T>::value];
 #pragma unroll
@@ -861,7 +866,7 @@ __global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_inv_zr(
         pp[u] = -1;
         if (j < j1) {
             pp[u] = pposr[j];
-            pv[u] = xc[ppw[j]];
+            pv[u] = xc[pw0 >= 0 ? pw0 + (j - j0) : ppw[j]];
         }
     }
     for (int i = tid; i < nrow * pitch; i += nthr) buf[i] = C{T(0), T(0)};
@@ -872,7 +877,8 @@ __global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_inv_zr(
 #pragma unroll
     for (int u = 0; u < SpherePrefetch<T>::value; ++u)
         if (pp[u] >= 0) buf[pp[u]] = pv[u];
-    for (int j = j0 + tid + SpherePrefetch<T>::value * nthr; j < j1; j += nthr) buf[pposr[j]] = xc[ppw[j]];
+    for (int j = j0 + tid + SpherePrefetch<T>::value * nthr; j < j1; j += nthr)
+        buf[pposr[j]] = xc[pw0 >= 0 ? pw0 + (j - j0) : ppw[j]];
     __syncthreads();
     // Inverse along y on the active rows (ends with a barrier).
     lds_fft_lines<T, C, true, 1>(buf, ay, twy, LineSet{nrow, 1, pitch, nullptr}, tid, nthr);
@@ -915,14 +921,14 @@ template <typename T, int NZ>
 __global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_fwd_zr(
     const typename C2T<T>::type* __restrict__ tbuf, const typename C2T<T>::type* __restrict__ x, size_t ldx,
     typename C2T<T>::type* __restrict__ y, size_t ldy, const T* __restrict__ kinetic,
-    const int* __restrict__ pstart, const int* __restrict__ ppw, const int* __restrict__ pposr,
-    const int* __restrict__ zstart, const short* __restrict__ zrows, int ny, int nplanes, FftAxis ay,
-    const typename C2T<T>::type* __restrict__ tw, int accumulate) {
+    const int* __restrict__ pstart, const int* __restrict__ ppw, const int* __restrict__ ppw0,
+    const int* __restrict__ pposr, const int* __restrict__ zstart, const short* __restrict__ zrows, int ny,
+    int nplanes, FftAxis ay, const typename C2T<T>::type* __restrict__ tw, int accumulate) {
     using C = typename C2T<T>::type;
     static_assert(NZ % 3 == 0, "z-register plane kernel needs nz divisible by 3");
     constexpr int M = NZ / 3;
     extern __shared__ unsigned char smem_raw[];
-    const int pitch = ny + 1;
+    const int pitch = zr_pitch(ny);
     const int p = blockIdx.x;
     const size_t col = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -984,6 +990,7 @@ __global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_fwd_zr(
         const int bm = t == 0 ? b0 : (t1 ? b1 : b2);
         if (own && bm >= 0) buf[bm * pitch + yl] = C{xr, xi};
     }
+    const int pw0 = ppw0[p];
     int gg[SpherePrefetch<T>::value], gp[SpherePrefetch<T>::value];
     T gk[SpherePrefetch<T>::value];
     C gx[SpherePrefetch<T>::value], gy[SpherePrefetch<T>::value];
@@ -992,7 +999,7 @@ __global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_fwd_zr(
         const int j = j0 + tid + u * nthr;
         gg[u] = -1;
         if (j < j1) {
-            const int g = ppw[j];
+            const int g = pw0 >= 0 ? pw0 + (j - j0) : ppw[j];
             gg[u] = g;
             gp[u] = pposr[j];
             gk[u] = kinetic[g];
@@ -1010,7 +1017,7 @@ __global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_fwd_zr(
             yc[gg[u]] = C{v.x + gk[u] * gx[u].x + gy[u].x, v.y + gk[u] * gx[u].y + gy[u].y};
         }
     for (int j = j0 + tid + SpherePrefetch<T>::value * nthr; j < j1; j += nthr) {
-        const int g = ppw[j];
+        const int g = pw0 >= 0 ? pw0 + (j - j0) : ppw[j];
         const C v = buf[pposr[j]];
         const T kin = kinetic[g];
         const C xv = xc[g];
@@ -1667,7 +1674,7 @@ __global__ void k_d2f(const double* __restrict__ in, float* __restrict__ out, si
 size_t Ham::device_bytes() const {
     const DevBuf* bufs[] = {&plane_start, &plane_pw, &plane_pos, &plane_xpos, &plane_ystart,
                             &plane_ycols, &plane_xabsent, &kinetic64,
-                            &plane_zstart, &plane_zrows, &plane_posr,
+                            &plane_zstart, &plane_zrows, &plane_posr, &plane_pw0,
                             &kinetic32, &vloc64, &vloc32, &a64, &a32, &meta, &block_rows, &dm64, &dm32,
                             &xstage, &ystage, &x32, &y32, &tbuf, &partial, &qbuf,
                             &vloc32_t, &vloc64_t, &chat64, &chat32};
@@ -1783,8 +1790,16 @@ void ham_build(Ham& h, const dzhip_ham_desc& d) {
     std::vector<int> posr(h.npw);
     for (size_t j = 0; j < h.npw; ++j) {
         const size_t i = (size_t)pw[j];
-        posr[j] = py[ys[i]] + pitch * zrow[pidx[j]][zs[i]];
+        posr[j] = py[ys[i]] + zr_pitch(h.ny) * zrow[pidx[j]][zs[i]];
     }
+    // First plane-wave index of each plane whose entries are one run of the basis, else -1.
+    std::vector<int> pw0(h.nplanes, -1);
+    for (int p = 0; p < h.nplanes; ++p) {
+        bool run = start[p + 1] > start[p];
+        for (int j = start[p] + 1; run && j < start[p + 1]; ++j) run = pw[j] == pw[j - 1] + 1;
+        if (run) pw0[p] = pw[start[p]];
+    }
+    upload(h.plane_pw0, pw0);
     upload(h.plane_zstart, zstart);
     upload(h.plane_zrows, zrows);
     upload(h.plane_posr, posr);
@@ -2022,7 +2037,7 @@ void ham_release(Ham& h) {
                       &h.kinetic32, &h.vloc64, &h.vloc32, &h.a64, &h.a32, &h.meta, &h.block_rows, &h.dm64,
                       &h.dm32, &h.xstage, &h.ystage, &h.x32, &h.y32, &h.tbuf, &h.partial, &h.qbuf,
                       &h.dens_rho, &h.dens_w, &h.vloc32_t, &h.vloc64_t, &h.chat64, &h.chat32,
-                      &h.plane_zstart, &h.plane_zrows, &h.plane_posr};
+                      &h.plane_zstart, &h.plane_zrows, &h.plane_posr, &h.plane_pw0};
     for (DevBuf* b : bufs) b->release();
     h.tab.release();
     for (auto& set : h.ev_sets)
@@ -2208,7 +2223,8 @@ void launch_plane_inv(Ham& h, int kind, const typename C2T<T>::type* x, size_t l
     case N:                                                                                                  \
         set_max_dynamic_lds((const void*)k_plane_inv_zr<T, N>, lds);                                         \
         hipLaunchKernelGGL((k_plane_inv_zr<T, N>), grid, block, lds, s, x, ld, tb, h.plane_start.as<int>(),  \
-                           h.plane_pw.as<int>(), h.plane_posr.as<int>(), h.plane_zstart.as<int>(),           \
+                           h.plane_pw.as<int>(), h.plane_pw0.as<int>(), h.plane_posr.as<int>(),              \
+                           h.plane_zstart.as<int>(),                                                         \
                            h.plane_zrows.as<short>(), h.ny, h.nplanes, h.tab.ax[1], tw);                     \
         done = true;                                                                                         \
         break;
@@ -2245,8 +2261,9 @@ void launch_plane_fwd(Ham& h, int kind, const typename C2T<T>::type* tb, const t
     case N:                                                                                                  \
         set_max_dynamic_lds((const void*)k_plane_fwd_zr<T, N>, lds);                                         \
         hipLaunchKernelGGL((k_plane_fwd_zr<T, N>), grid, block, lds, s, tb, x, ld, y, ld, Sel<T>::kin(h),    \
-                           h.plane_start.as<int>(), h.plane_pw.as<int>(), h.plane_posr.as<int>(),            \
-                           h.plane_zstart.as<int>(), h.plane_zrows.as<short>(), h.ny, h.nplanes,             \
+                           h.plane_start.as<int>(), h.plane_pw.as<int>(), h.plane_pw0.as<int>(),             \
+                           h.plane_posr.as<int>(), h.plane_zstart.as<int>(), h.plane_zrows.as<short>(), h.ny, \
+                           h.nplanes,                                                                        \
                            h.tab.ax[1], tw, accumulate);                                                     \
         done = true;                                                                                         \
         break;

@@ -376,6 +376,31 @@ def test_hpsi_zreg_plane_vs_oracle(dz, ctx, grid, nbands, prec):
         ham2.close()
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hpsi_zreg_plane_scattered_basis(dz, ctx, prec):
+    """A basis whose x-planes are not runs of the basis order (a seeded permutation of the 75^3
+    bench sphere) takes the indexed gathers of the z-register plane kernels; the h-major order
+    takes the direct reads.  Both against the oracle on the same permuted problem."""
+    import dataclasses
+
+    w = workloads.bench(nbands=6, grid=(75, 75, 75))
+    perm = np.random.default_rng(5).permutation(w.npw)
+    b = w.basis
+    wp = dataclasses.replace(w, basis=tables.Basis(b.h[perm], b.k[perm], b.l[perm], b.cart[perm],
+                                                   b.kpg[perm], b.kinetic[perm]),
+                             psi=np.ascontiguousarray(w.psi[:, perm]))
+    tol = TOL32 if prec else TOL64
+    for wk in (w, wp):
+        ham = make_ham(dz, ctx, wk)
+        try:
+            ham.set_plane_kernel(2)
+            got = ham.apply(wk.psi, precision=prec)
+            ref = hamiltonian.apply_hamiltonian_batched(oracle_ctx(wk), wk.psi)
+            assert relerr(got, ref) < tol
+        finally:
+            ham.close()
+
+
 def test_plane_kernel_choice_errors(dz, ctx):
     """Plane kernel 2 needs nz divisible by 3 with a register-FFT third (8^3: no); 1 and 0 always
     run; other codes are rejected."""
