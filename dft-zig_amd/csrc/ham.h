@@ -227,6 +227,9 @@ struct HamSet {
     std::vector<size_t> npw_off;
     std::vector<int> np_off;
     DevBuf plane_start, plane_pw, plane_pos, plane_ystart, plane_ycols, kpl, kinetic64, kinetic32, a64, a32;
+    // Active z-row tables of the z-register plane kernels over the concatenated planes.
+    int zrow_max = 0;
+    DevBuf plane_pw0, plane_posr, plane_zstart, plane_zrows;
     // Nonlocal segment tables for seg_cols columns in precision seg_prec (element size).
     size_t seg_cols = 0;
     int seg_prec = 0, nks_total = 0, ntiles = 0;
@@ -237,12 +240,12 @@ struct HamSet {
         const void* x;
         void* y;
         size_t nc;
-        int prec, code;
+        int prec, code, pkind;
         hipStream_t s;
         void *tbuf, *partial, *qbuf;
         bool operator==(const GraphKey& o) const {
-            return x == o.x && y == o.y && nc == o.nc && prec == o.prec && code == o.code && s == o.s &&
-                   tbuf == o.tbuf && partial == o.partial && qbuf == o.qbuf;
+            return x == o.x && y == o.y && nc == o.nc && prec == o.prec && code == o.code && pkind == o.pkind &&
+                   s == o.s && tbuf == o.tbuf && partial == o.partial && qbuf == o.qbuf;
         }
     };
     hipGraphExec_t graph_exec = nullptr;

@@ -2202,31 +2202,69 @@ int xline_variant(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t s) {
     return sp;
 }
 
-bool plane_zreg_usable(const Ham& h) {
-    return xline_tri_supported(h.nz) && h.ny <= kZrLinesPerWave * (kZrMaxThreads / 64) && h.zrow_max > 0 &&
-           plane_zr_lds_bytes<double2>(h.ny, h.nz, h.zrow_max) <= kMaxLdsBytes;
+// Plane tables of one context or of a k-point set's concatenated planes, for the plane launchers.
+struct PlaneArgs {
+    int ny, nz, nplanes, zrow_max;
+    const int *start, *pw, *pos, *ystart;
+    const short* ycols;
+    const int *pw0, *posr, *zstart;
+    const short* zrows;
+    FftAxis ay, az;
+};
+
+static PlaneArgs plane_args(const Ham& h) {
+    return PlaneArgs{h.ny, h.nz, h.nplanes, h.zrow_max, h.plane_start.as<int>(), h.plane_pw.as<int>(),
+                     h.plane_pos.as<int>(), h.plane_ystart.as<int>(), h.plane_ycols.as<short>(),
+                     h.plane_pw0.as<int>(), h.plane_posr.as<int>(), h.plane_zstart.as<int>(),
+                     h.plane_zrows.as<short>(), h.tab.ax[1], h.tab.ax[2]};
+}
+
+static bool plane_zreg_fits(int ny, int nz, int zrow_max) {
+    return xline_tri_supported(nz) && ny <= kZrLinesPerWave * (kZrMaxThreads / 64) && zrow_max > 0 &&
+           plane_zr_lds_bytes<double2>(ny, nz, zrow_max) <= kMaxLdsBytes;
+}
+
+bool plane_zreg_usable(const Ham& h) { return plane_zreg_fits(h.ny, h.nz, h.zrow_max); }
+
+// Kernel attributes of the plane kernels of `kind` (before any launch a graph may capture).
+template <typename T>
+void plane_prepare(const PlaneArgs& pa, int kind) {
+    using C = typename C2T<T>::type;
+    if (kind == kPlaneZreg) {
+        const size_t lds = plane_zr_lds_bytes<C>(pa.ny, pa.nz, pa.zrow_max);
+        switch (pa.nz) {
+#define DZ_ZR_CASE(N)                                                \
+    case N:                                                          \
+        set_max_dynamic_lds((const void*)k_plane_inv_zr<T, N>, lds); \
+        set_max_dynamic_lds((const void*)k_plane_fwd_zr<T, N>, lds); \
+        break;
+            DZ_XTRI_SIZES(DZ_ZR_CASE)
+#undef DZ_ZR_CASE
+            default:
+                break;
+        }
+    } else {
+        set_max_dynamic_lds((const void*)k_plane_inv<T>, plane_lds_bytes<C>(pa.ny, pa.nz));
+        set_max_dynamic_lds((const void*)k_plane_fwd<T>, plane_lds_bytes<C>(pa.ny, pa.nz));
+    }
 }
 
 // Sphere -> [z][y] planes of every band (kind: kPlaneTile or kPlaneZreg).
 template <typename T>
-void launch_plane_inv(Ham& h, int kind, const typename C2T<T>::type* x, size_t ld, typename C2T<T>::type* tb,
-                      size_t nc, hipStream_t s) {
+void launch_plane_inv(const PlaneArgs& pa, int kind, const typename C2T<T>::type* x, size_t ld,
+                      typename C2T<T>::type* tb, size_t nc, const typename C2T<T>::type* tw, hipStream_t s) {
     using C = typename C2T<T>::type;
-    const C* tw = Sel<T>::tw(h);
-    const dim3 grid((unsigned)h.nplanes, (unsigned)nc);
+    const dim3 grid((unsigned)pa.nplanes, (unsigned)nc);
     if (kind == kPlaneZreg) {
-        const size_t lds = plane_zr_lds_bytes<C>(h.ny, h.nz, h.zrow_max);
-        const dim3 block((unsigned)plane_zr_threads(h.ny));
+        const size_t lds = plane_zr_lds_bytes<C>(pa.ny, pa.nz, pa.zrow_max);
+        const dim3 block((unsigned)plane_zr_threads(pa.ny));
         bool done = false;
-        switch (h.nz) {
-#define DZ_ZR_CASE(N)                                                                                        \
-    case N:                                                                                                  \
-        set_max_dynamic_lds((const void*)k_plane_inv_zr<T, N>, lds);                                         \
-        hipLaunchKernelGGL((k_plane_inv_zr<T, N>), grid, block, lds, s, x, ld, tb, h.plane_start.as<int>(),  \
-                           h.plane_pw.as<int>(), h.plane_pw0.as<int>(), h.plane_posr.as<int>(),              \
-                           h.plane_zstart.as<int>(),                                                         \
-                           h.plane_zrows.as<short>(), h.ny, h.nplanes, h.tab.ax[1], tw);                     \
-        done = true;                                                                                         \
+        switch (pa.nz) {
+#define DZ_ZR_CASE(N)                                                                                         \
+    case N:                                                                                                   \
+        hipLaunchKernelGGL((k_plane_inv_zr<T, N>), grid, block, lds, s, x, ld, tb, pa.start, pa.pw, pa.pw0,   \
+                           pa.posr, pa.zstart, pa.zrows, pa.ny, pa.nplanes, pa.ay, tw);                       \
+        done = true;                                                                                          \
         break;
             DZ_XTRI_SIZES(DZ_ZR_CASE)
 #undef DZ_ZR_CASE
@@ -2235,37 +2273,30 @@ void launch_plane_inv(Ham& h, int kind, const typename C2T<T>::type* x, size_t l
         }
         DZ_REQUIRE(done, DZHIP_ERR_STATE, "no z-register plane kernel for this nz");
     } else {
-        const size_t lds = plane_lds_bytes<C>(h.ny, h.nz);
-        set_max_dynamic_lds((const void*)k_plane_inv<T>, lds);
-        hipLaunchKernelGGL(k_plane_inv<T>, grid, dim3(TileThreads<T>::value), lds, s, x, ld, tb,
-                           h.plane_start.as<int>(), h.plane_pw.as<int>(), h.plane_pos.as<int>(),
-                           h.plane_ystart.as<int>(), h.plane_ycols.as<short>(), h.ny, h.nz, h.nplanes,
-                           h.tab.ax[1], h.tab.ax[2], tw);
+        hipLaunchKernelGGL(k_plane_inv<T>, grid, dim3(TileThreads<T>::value), plane_lds_bytes<C>(pa.ny, pa.nz), s, x,
+                           ld, tb, pa.start, pa.pw, pa.pos, pa.ystart, pa.ycols, pa.ny, pa.nz, pa.nplanes, pa.ay,
+                           pa.az, tw);
     }
     DZ_HIP(hipGetLastError());
 }
 
 // [z][y] planes -> y (+)= forward transform at the sphere + kinetic * x.
 template <typename T>
-void launch_plane_fwd(Ham& h, int kind, const typename C2T<T>::type* tb, const typename C2T<T>::type* x,
-                      size_t ld, typename C2T<T>::type* y, int accumulate, size_t nc, hipStream_t s) {
+void launch_plane_fwd(const PlaneArgs& pa, int kind, const typename C2T<T>::type* tb,
+                      const typename C2T<T>::type* x, size_t ld, typename C2T<T>::type* y, const T* kin,
+                      int accumulate, size_t nc, const typename C2T<T>::type* tw, hipStream_t s) {
     using C = typename C2T<T>::type;
-    const C* tw = Sel<T>::tw(h);
-    const dim3 grid((unsigned)h.nplanes, (unsigned)nc);
+    const dim3 grid((unsigned)pa.nplanes, (unsigned)nc);
     if (kind == kPlaneZreg) {
-        const size_t lds = plane_zr_lds_bytes<C>(h.ny, h.nz, h.zrow_max);
-        const dim3 block((unsigned)plane_zr_threads(h.ny));
+        const size_t lds = plane_zr_lds_bytes<C>(pa.ny, pa.nz, pa.zrow_max);
+        const dim3 block((unsigned)plane_zr_threads(pa.ny));
         bool done = false;
-        switch (h.nz) {
-#define DZ_ZR_CASE(N)                                                                                        \
-    case N:                                                                                                  \
-        set_max_dynamic_lds((const void*)k_plane_fwd_zr<T, N>, lds);                                         \
-        hipLaunchKernelGGL((k_plane_fwd_zr<T, N>), grid, block, lds, s, tb, x, ld, y, ld, Sel<T>::kin(h),    \
-                           h.plane_start.as<int>(), h.plane_pw.as<int>(), h.plane_pw0.as<int>(),             \
-                           h.plane_posr.as<int>(), h.plane_zstart.as<int>(), h.plane_zrows.as<short>(), h.ny, \
-                           h.nplanes,                                                                        \
-                           h.tab.ax[1], tw, accumulate);                                                     \
-        done = true;                                                                                         \
+        switch (pa.nz) {
+#define DZ_ZR_CASE(N)                                                                                          \
+    case N:                                                                                                    \
+        hipLaunchKernelGGL((k_plane_fwd_zr<T, N>), grid, block, lds, s, tb, x, ld, y, ld, kin, pa.start, pa.pw, \
+                           pa.pw0, pa.posr, pa.zstart, pa.zrows, pa.ny, pa.nplanes, pa.ay, tw, accumulate);     \
+        done = true;                                                                                           \
         break;
             DZ_XTRI_SIZES(DZ_ZR_CASE)
 #undef DZ_ZR_CASE
@@ -2274,12 +2305,9 @@ void launch_plane_fwd(Ham& h, int kind, const typename C2T<T>::type* tb, const t
         }
         DZ_REQUIRE(done, DZHIP_ERR_STATE, "no z-register plane kernel for this nz");
     } else {
-        const size_t lds = plane_lds_bytes<C>(h.ny, h.nz);
-        set_max_dynamic_lds((const void*)k_plane_fwd<T>, lds);
-        hipLaunchKernelGGL(k_plane_fwd<T>, grid, dim3(TileThreads<T>::value), lds, s, tb, x, ld, y, ld,
-                           Sel<T>::kin(h), h.plane_start.as<int>(), h.plane_pw.as<int>(), h.plane_pos.as<int>(),
-                           h.plane_ystart.as<int>(), h.plane_ycols.as<short>(), h.ny, h.nz, h.nplanes,
-                           h.tab.ax[1], h.tab.ax[2], tw, accumulate);
+        hipLaunchKernelGGL(k_plane_fwd<T>, grid, dim3(TileThreads<T>::value), plane_lds_bytes<C>(pa.ny, pa.nz), s, tb,
+                           x, ld, y, ld, kin, pa.start, pa.pw, pa.pos, pa.ystart, pa.ycols, pa.ny, pa.nz, pa.nplanes,
+                           pa.ay, pa.az, tw, accumulate);
     }
     DZ_HIP(hipGetLastError());
 }
@@ -2321,13 +2349,16 @@ int plane_variant(Ham& h, const typename C2T<T>::type* x, typename C2T<T>::type*
     DZ_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     int best_k = kPlaneZreg;
+    const PlaneArgs pa = plane_args(h);
+    const typename C2T<T>::type* tw = Sel<T>::tw(h);
     for (int k : {kPlaneZreg, kPlaneTile}) {
-        launch_plane_inv<T>(h, k, x, h.npw, tb, nc, s);
-        launch_plane_fwd<T>(h, k, tb, x, h.npw, y, 0, nc, s);
+        plane_prepare<T>(pa, k);
+        launch_plane_inv<T>(pa, k, x, h.npw, tb, nc, tw, s);
+        launch_plane_fwd<T>(pa, k, tb, x, h.npw, y, Sel<T>::kin(h), 0, nc, tw, s);
         DZ_HIP(hipEventRecord(e0, s));
         for (int r = 0; r < 3; ++r) {
-            launch_plane_inv<T>(h, k, x, h.npw, tb, nc, s);
-            launch_plane_fwd<T>(h, k, tb, x, h.npw, y, 0, nc, s);
+            launch_plane_inv<T>(pa, k, x, h.npw, tb, nc, tw, s);
+            launch_plane_fwd<T>(pa, k, tb, x, h.npw, y, Sel<T>::kin(h), 0, nc, tw, s);
         }
         DZ_HIP(hipEventRecord(e1, s));
         DZ_HIP(hipEventSynchronize(e1));
@@ -2425,12 +2456,14 @@ void ham_apply_dev(Ham& h, const void* xv, void* yv, size_t ncols, bool local, b
         if (overlap) DZ_HIP(hipEventRecord(h.ev_join, snl));
         mark(h, evs, 2);
         if (local) {
-            launch_plane_inv<T>(h, pkind, xc, ld, tb, nc, s);
+            const PlaneArgs pa = plane_args(h);
+            plane_prepare<T>(pa, pkind);
+            launch_plane_inv<T>(pa, pkind, xc, ld, tb, nc, Sel<T>::tw(h), s);
             mark(h, evs, 3);
             launch_xline_any<T>(h, tb, nc, split, s);
             mark(h, evs, 4);
             if (overlap) DZ_HIP(hipStreamWaitEvent(s, h.ev_join, 0));
-            launch_plane_fwd<T>(h, pkind, tb, xc, ld, yc, do_nl ? 1 : 0, nc, s);
+            launch_plane_fwd<T>(pa, pkind, tb, xc, ld, yc, Sel<T>::kin(h), do_nl ? 1 : 0, nc, Sel<T>::tw(h), s);
         } else {
             if (!do_nl) {
                 const size_t n = nc * ld;
@@ -2508,8 +2541,9 @@ void hamset_build(HamSet& hs) {
     hs.nz = h0.nz;
     hs.nproj = h0.nproj;
     const int nk = (int)hs.ks.size();
-    std::vector<int> start{0}, pw, pos, ystart{0}, kpl;
-    std::vector<short> ycols;
+    std::vector<int> start{0}, pw, pos, ystart{0}, kpl, pw0, posr, zstart{0};
+    std::vector<short> ycols, zrows;
+    hs.zrow_max = 0;
     hs.npw_off.assign(nk + 1, 0);
     hs.np_off.assign(nk + 1, 0);
     hs.max_planes = 0;
@@ -2526,16 +2560,25 @@ void hamset_build(HamSet& hs) {
         const std::vector<int> qk = download<int>(h.plane_pos, h.npw);
         const std::vector<int> ys = download<int>(h.plane_ystart, h.nplanes + 1);
         const std::vector<short> yc = download<short>(h.plane_ycols, (size_t)ys[h.nplanes]);
-        const int e0 = start.back(), c0 = ystart.back();
+        const std::vector<int> w0 = download<int>(h.plane_pw0, h.nplanes);
+        const std::vector<int> rk = download<int>(h.plane_posr, h.npw);
+        const std::vector<int> zs = download<int>(h.plane_zstart, h.nplanes + 1);
+        const std::vector<short> zr = download<short>(h.plane_zrows, (size_t)zs[h.nplanes]);
+        const int e0 = start.back(), c0 = ystart.back(), r0 = zstart.back();
         for (int p = 1; p <= h.nplanes; ++p) {
             start.push_back(e0 + st[p]);
             ystart.push_back(c0 + ys[p]);
+            zstart.push_back(r0 + zs[p]);
         }
+        for (int p = 0; p < h.nplanes; ++p) pw0.push_back(w0[p] >= 0 ? (int)(off + w0[p]) : -1);
         for (size_t j = 0; j < h.npw; ++j) {
             pw.push_back((int)(off + pk[j]));
             pos.push_back(qk[j]);
+            posr.push_back(rk[j]);
         }
         ycols.insert(ycols.end(), yc.begin(), yc.end());
+        zrows.insert(zrows.end(), zr.begin(), zr.end());
+        hs.zrow_max = std::max(hs.zrow_max, h.zrow_max);
         kpl.push_back(hs.np_off[k]);
         kpl.push_back(h.nplanes);
         hs.npw_off[k + 1] = off + h.npw;
@@ -2551,6 +2594,10 @@ void hamset_build(HamSet& hs) {
     upload(hs.plane_pos, pos);
     upload(hs.plane_ystart, ystart);
     upload(hs.plane_ycols, ycols);
+    upload(hs.plane_pw0, pw0);
+    upload(hs.plane_posr, posr);
+    upload(hs.plane_zstart, zstart);
+    upload(hs.plane_zrows, zrows);
     upload(hs.kpl, kpl);
     hs.kinetic64.ensure(hs.npw_total * sizeof(double));
     hs.kinetic32.ensure(hs.npw_total * sizeof(float));
@@ -2646,10 +2693,30 @@ static int hamset_xline_code(HamSet& hs, size_t nc, hipStream_t s) {
     return code;
 }
 
+static PlaneArgs plane_args(const HamSet& hs) {
+    const Ham& h0 = *hs.ks[0];
+    return PlaneArgs{hs.ny, hs.nz, hs.np_total, hs.zrow_max, hs.plane_start.as<int>(), hs.plane_pw.as<int>(),
+                     hs.plane_pos.as<int>(), hs.plane_ystart.as<int>(), hs.plane_ycols.as<short>(),
+                     hs.plane_pw0.as<int>(), hs.plane_posr.as<int>(), hs.plane_zstart.as<int>(),
+                     hs.plane_zrows.as<short>(), h0.tab.ax[1], h0.tab.ax[2]};
+}
+
+// Plane kernels of the set: member 0's forced or tuned choice for this precision; untuned, the
+// z-register kernels where they run.
+template <typename T>
+static int hamset_plane_kind(const HamSet& hs) {
+    constexpr bool fp32 = std::is_same<T, float>::value;
+    const Ham& h0 = *hs.ks[0];
+    int kind = h0.plane_fixed != 0 ? h0.plane_fixed : h0.plane_kind[fp32 ? 1 : 0];
+    const bool zr = plane_zreg_fits(hs.ny, hs.nz, hs.zrow_max);
+    if (kind == 0) kind = zr ? kPlaneZreg : kPlaneTile;
+    return kind == kPlaneZreg && !zr ? kPlaneTile : kind;
+}
+
 // The apply's launch sequence for the whole set (no host synchronisation, no allocation: it is
 // what the graph captures).
 template <typename T>
-static void hamset_launch(HamSet& hs, const void* xv, void* yv, size_t nc, int code, hipStream_t s) {
+static void hamset_launch(HamSet& hs, const void* xv, void* yv, size_t nc, int code, int pkind, hipStream_t s) {
     using C = typename C2T<T>::type;
     const C* x = reinterpret_cast<const C*>(xv);
     C* y = reinterpret_cast<C*>(yv);
@@ -2679,23 +2746,14 @@ static void hamset_launch(HamSet& hs, const void* xv, void* yv, size_t nc, int c
                            (int)nc, hs.tiles.as<int4>(), (int)ld);
         DZ_HIP(hipGetLastError());
     }
-    const size_t lds_p = plane_lds_bytes<C>(hs.ny, hs.nz);
-    constexpr int nt = TileThreads<T>::value;
-    hipLaunchKernelGGL(k_plane_inv<T>, dim3(hs.np_total, (unsigned)nc), dim3(nt), lds_p, s, x, ld, tb,
-                       hs.plane_start.as<int>(), hs.plane_pw.as<int>(), hs.plane_pos.as<int>(),
-                       hs.plane_ystart.as<int>(), hs.plane_ycols.as<short>(), hs.ny, hs.nz, hs.np_total, h0.tab.ax[1],
-                       h0.tab.ax[2], tw);
-    DZ_HIP(hipGetLastError());
+    const PlaneArgs pa = plane_args(hs);
+    launch_plane_inv<T>(pa, pkind, x, ld, tb, nc, tw, s);
     if (code == kXlineConv)
         launch_xline_conv<T>(h0, tb, nc, s, hs.kpl.as<int2>(), nk, hs.np_total, hs.max_planes);
     else
         launch_xline_reg<T>(h0, tb, nc, xline_reg_threads(code), xline_reg_kind(code), s, hs.kpl.as<int2>(), nk,
                             hs.np_total, hs.max_planes);
-    hipLaunchKernelGGL(k_plane_fwd<T>, dim3(hs.np_total, (unsigned)nc), dim3(nt), lds_p, s, tb, x, ld, y, ld, kin,
-                       hs.plane_start.as<int>(), hs.plane_pw.as<int>(), hs.plane_pos.as<int>(),
-                       hs.plane_ystart.as<int>(), hs.plane_ycols.as<short>(), hs.ny, hs.nz, hs.np_total, h0.tab.ax[1],
-                       h0.tab.ax[2], tw, do_nl ? 1 : 0);
-    DZ_HIP(hipGetLastError());
+    launch_plane_fwd<T>(pa, pkind, tb, x, ld, y, kin, do_nl ? 1 : 0, nc, tw, s);
 }
 
 template <typename T>
@@ -2717,14 +2775,14 @@ void hamset_apply_dev(HamSet& hs, const void* x, void* y, size_t nc, bool use_gr
         set_max_dynamic_lds((const void*)k_nl_dij<T>, dij_lds_bytes(hs.nproj, sizeof(C)));
         set_max_dynamic_lds((const void*)k_nl_back<T>, nl_back_lds_bytes<T>());
     }
-    set_max_dynamic_lds((const void*)k_plane_inv<T>, plane_lds_bytes<C>(hs.ny, hs.nz));
-    set_max_dynamic_lds((const void*)k_plane_fwd<T>, plane_lds_bytes<C>(hs.ny, hs.nz));
+    const int pkind = hamset_plane_kind<T>(hs);
+    plane_prepare<T>(plane_args(hs), pkind);
     (void)h0;
     if (!use_graph) {
-        hamset_launch<T>(hs, x, y, nc, code, s);
+        hamset_launch<T>(hs, x, y, nc, code, pkind, s);
         return;
     }
-    const HamSet::GraphKey key{x, y, nc, (int)sizeof(T), code, s, hs.tbuf.ptr, hs.partial.ptr, hs.qbuf.ptr};
+    const HamSet::GraphKey key{x, y, nc, (int)sizeof(T), code, pkind, s, hs.tbuf.ptr, hs.partial.ptr, hs.qbuf.ptr};
     if (!(hs.graph_exec && hs.graph_key == key)) {
         if (hs.graph_exec) {
             DZ_HIP(hipGraphExecDestroy(hs.graph_exec));
@@ -2733,7 +2791,7 @@ void hamset_apply_dev(HamSet& hs, const void* x, void* y, size_t nc, bool use_gr
         hipGraph_t g = nullptr;
         DZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
-            hamset_launch<T>(hs, x, y, nc, code, s);
+            hamset_launch<T>(hs, x, y, nc, code, pkind, s);
         } catch (...) {
             (void)hipStreamEndCapture(s, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -2755,6 +2813,7 @@ void hamset_release(HamSet& hs) {
         hs.graph_exec = nullptr;
     }
     DevBuf* bufs[] = {&hs.plane_start, &hs.plane_pw, &hs.plane_pos, &hs.plane_ystart, &hs.plane_ycols, &hs.kpl,
+                      &hs.plane_pw0, &hs.plane_posr, &hs.plane_zstart, &hs.plane_zrows,
                       &hs.kinetic64, &hs.kinetic32, &hs.a64, &hs.a32, &hs.chunks, &hs.kseg, &hs.tiles, &hs.tbuf,
                       &hs.partial, &hs.qbuf};
     for (DevBuf* b : bufs) b->release();

@@ -154,7 +154,9 @@ int dzhip_ham_apply_dev(dzhip_ham* ham, const void* x_dev, void* y_dev, uint64_t
  * and projector set (one per k-point, uneven n_pw allowed) with a register x-line kernel that runs
  * on all of them; they must outlive the set and hold the same local potential (the set applies
  * member 0's).  Each k-point's result equals its own dzhip_ham_apply_dev bit for bit when the
- * members use the same x-line variant (dzhip_ham_set_tuning). */
+ * members use the same x-line variant and plane kernels (dzhip_ham_set_tuning,
+ * dzhip_ham_set_plane_kernel); the set runs member 0's plane kernels (untuned: the z-register
+ * kernels where they run). */
 typedef struct dzhip_hamset dzhip_hamset;
 dzhip_hamset* dzhip_hamset_create(dzhip_ham* const* hams, uint64_t count);
 void dzhip_hamset_destroy(dzhip_hamset* set);

@@ -3,8 +3,9 @@
 1,260 plane waves, 12 bands) in one launch sequence, against 35 per-k applies
 (apply_hamiltonian_batched per k-point, src/features/scf/apply.zig:1584-1618, over the k-point
 queue of src/features/scf/kpoint/workers.zig:133-180).  With every member on the same x-line
-kernel the set's result equals the per-k results bit for bit, in both precisions, eagerly and as a
-captured graph replayed between other work; the per-k loop and the set are timed."""
+kernel and plane kernels (full-tile or z-register) the set's result equals the per-k results bit for
+bit, in both precisions, eagerly and as a captured graph replayed between other work; the per-k loop
+and the set are timed."""
 import time
 
 import numpy as np
@@ -26,14 +27,16 @@ def al35(dz, ctx):
         h.close()
 
 
+@pytest.mark.parametrize("plane", [1, 2])
 @pytest.mark.parametrize("code", [-4, -10, -13])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_hamset_equals_per_k_applies(dz, ctx, al35, prec, code):
+def test_hamset_equals_per_k_applies(dz, ctx, al35, prec, code, plane):
     import torch
 
     al, ws, hams = al35
     for h in hams:
         h.set_tuning(code)
+        h.set_plane_kernel(plane)
     hs = dz.HamSet(hams)
     try:
         info = hs.info()
@@ -80,6 +83,7 @@ def test_hamset_timing_vs_per_k_loop(dz, ctx, al35):
     al, ws, hams = al35
     for h in hams:
         h.set_tuning(0)
+        h.set_plane_kernel(0)
     hs = dz.HamSet(hams)
     stream = torch.cuda.Stream()
     try:
