@@ -1187,7 +1187,14 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_xline_conv(typename C2T<T>:
     const int l16 = lane & 15, l4 = lane >> 4;
     const int2 kp = xl_planes(kpl, nplanes);
     const int np = kp.y;
-    const int b0 = (int)blockIdx.x * NB, l0 = (int)blockIdx.y * LT;
+    // Tiles in (band tile fastest, line tile) order, each XCD taking one contiguous run of them:
+    // workgroups are dealt to the 8 XCDs round-robin, and neighbouring line tiles share the cache
+    // lines at their boundary (a plane of ny nz elements is not a whole number of lines), which
+    // then meet in one L2 instead of being fetched and written back by two.
+    const unsigned ntile = gridDim.x * gridDim.y, lin = blockIdx.x + gridDim.x * blockIdx.y;
+    const unsigned xcd = lin & 7u, idx = lin >> 3, per = ntile >> 3, rem = ntile & 7u;
+    const unsigned tile = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    const int b0 = (int)(tile % gridDim.x) * NB, l0 = (int)(tile / gridDim.x) * LT;
     C* base = tbuf + ((size_t)b0 * nplanes + kp.x) * lines + l0;
     const int nlt = min(LT, lines - l0);
     for (int i = tid; i < LT * nx; i += kConvThreads)
