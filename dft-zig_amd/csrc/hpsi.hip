@@ -824,6 +824,10 @@ __global__ __launch_bounds__(TileThreads<T>::value) void k_plane_fwd(
 // rows, two or three workgroups of four or five wavefronts then fit a CU.
 constexpr int kZrLinesPerWave = 21;
 constexpr int kZrMaxThreads = 384;  // ny <= 126
+// Sphere entries per thread whose loads are issued together, ahead of the transforms (a 75^3
+// plane holds about 670 entries for 256 threads, so one round of loads instead of one latency
+// per entry).
+template <typename T> struct ZrPrefetch { static constexpr int value = 3; };
 
 __host__ __device__ inline int plane_zr_threads(int ny) {
     return 64 * ((ny + kZrLinesPerWave - 1) / kZrLinesPerWave);
@@ -858,16 +862,18 @@ __global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_inv_zr(
     const int j0 = pstart[p], j1 = pstart[p + 1];
     // A plane whose entries are one run of the basis (h-major order) is read without the index.
     const int pw0 = ppw0[p];
-    C pv[SpherePrefetch<T>::value];
-    int pp[SpherePrefetch<T>::value];
+    // Every plane holds at least one entry, so j0 is a valid index to read in place of a missing one.
+    C pv[ZrPrefetch<T>::value];
+    int pp[ZrPrefetch<T>::value];
 #pragma unroll
-    for (int u = 0; u < SpherePrefetch<T>::value; ++u) {
+    for (int u = 0; u < ZrPrefetch<T>::value; ++u) {
         const int j = j0 + tid + u * nthr;
-        pp[u] = -1;
-        if (j < j1) {
-            pp[u] = pposr[j];
-            pv[u] = xc[pw0 >= 0 ? pw0 + (j - j0) : ppw[j]];
-        }
+        const bool in = j < j1;
+        const int jc = in ? j : j0;
+        const int q = pposr[jc];
+        const C v = xc[pw0 >= 0 ? pw0 + (jc - j0) : ppw[jc]];
+        pp[u] = in ? q : -1;
+        pv[u] = v;
     }
     for (int i = tid; i < nrow * pitch; i += nthr) buf[i] = C{T(0), T(0)};
     load_twiddles(twy, tw + ay.tw_off, ny, tid, nthr);
@@ -875,9 +881,9 @@ __global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_inv_zr(
     __syncthreads();
     for (int i = tid; i < nrow; i += nthr) zb[zrows[r0 + i]] = (short)i;
 #pragma unroll
-    for (int u = 0; u < SpherePrefetch<T>::value; ++u)
+    for (int u = 0; u < ZrPrefetch<T>::value; ++u)
         if (pp[u] >= 0) buf[pp[u]] = pv[u];
-    for (int j = j0 + tid + SpherePrefetch<T>::value * nthr; j < j1; j += nthr)
+    for (int j = j0 + tid + ZrPrefetch<T>::value * nthr; j < j1; j += nthr)
         buf[pposr[j]] = xc[pw0 >= 0 ? pw0 + (j - j0) : ppw[j]];
     __syncthreads();
     // Inverse along y on the active rows (ends with a barrier).
@@ -991,11 +997,11 @@ __global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_fwd_zr(
         if (own && bm >= 0) buf[bm * pitch + yl] = C{xr, xi};
     }
     const int pw0 = ppw0[p];
-    int gg[SpherePrefetch<T>::value], gp[SpherePrefetch<T>::value];
-    T gk[SpherePrefetch<T>::value];
-    C gx[SpherePrefetch<T>::value], gy[SpherePrefetch<T>::value];
+    int gg[ZrPrefetch<T>::value], gp[ZrPrefetch<T>::value];
+    T gk[ZrPrefetch<T>::value];
+    C gx[ZrPrefetch<T>::value], gy[ZrPrefetch<T>::value];
 #pragma unroll
-    for (int u = 0; u < SpherePrefetch<T>::value; ++u) {
+    for (int u = 0; u < ZrPrefetch<T>::value; ++u) {
         const int j = j0 + tid + u * nthr;
         gg[u] = -1;
         if (j < j1) {
@@ -1011,12 +1017,12 @@ __global__ __launch_bounds__(kZrMaxThreads, 3) void k_plane_fwd_zr(
     // Forward along y on the active rows; frequency y lands at its digit-reversed position.
     lds_fft_lines<T, C, false, -1>(buf, ay, twy, LineSet{nrow, 1, pitch, nullptr}, tid, nthr);
 #pragma unroll
-    for (int u = 0; u < SpherePrefetch<T>::value; ++u)
+    for (int u = 0; u < ZrPrefetch<T>::value; ++u)
         if (gg[u] >= 0) {
             const C v = buf[gp[u]];
             yc[gg[u]] = C{v.x + gk[u] * gx[u].x + gy[u].x, v.y + gk[u] * gx[u].y + gy[u].y};
         }
-    for (int j = j0 + tid + SpherePrefetch<T>::value * nthr; j < j1; j += nthr) {
+    for (int j = j0 + tid + ZrPrefetch<T>::value * nthr; j < j1; j += nthr) {
         const int g = pw0 >= 0 ? pw0 + (j - j0) : ppw[j];
         const C v = buf[pposr[j]];
         const T kin = kinetic[g];
