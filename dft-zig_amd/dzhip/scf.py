@@ -30,7 +30,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import FP64, PRECOND_KINETIC, PRECOND_MLP, PRECOND_TPA, Context, Hamiltonian, ScfGrid, tables
+from . import FP64, PRECOND_KINETIC, PRECOND_MLP, PRECOND_TPA, Context, Hamiltonian, ScfGrid, tables, upf
 from .workloads import SpeciesDesc
 
 HA_PER_RY = 0.5
@@ -61,6 +61,9 @@ class ScfSystem:
     # Per-species GTH local parameters (Z, rloc, C1) for structures of several species; None = the
     # single set (z_ion, rloc, c1) for every atom.
     species_local: list | None = None
+    # Per-species UPF data (dzhip.upf.Upf) when the species were read from UPF files: V_loc(q) is
+    # then the file's short-range local form factor (Z from species_local).
+    species_upf: list | None = None
 
     def local_params(self):
         """(Z, rloc, C1) of every species."""
@@ -262,6 +265,18 @@ def al4_small(ecut_ry=8.0, nbands=10, xc="lda_pz"):
     return aluminium(ecut_ry=ecut_ry, kmesh=(2, 2, 2), nbands=nbands, displace=AL_SMALL_DISPLACEMENT, xc=xc)
 
 
+def with_upf(system, upfs):
+    """The system with every species read from a UPF file (dzhip.upf.Upf, one per species): the
+    projector tables are the files' radial tables over [0, sqrt(2 ecut) 1.5] (common.zig:124-147,
+    RadialTable 2048 points, nonlocal.zig:250-300), D from PP_DIJ, V_loc(q) the files' short-range
+    local form factor (form_factor.zig:225-256), Z their z_valence."""
+    import dataclasses
+
+    g_max = math.sqrt(2.0 * system.ecut_ry) * 1.5
+    return dataclasses.replace(system, species=[upf.UpfSpecies(u, g_max) for u in upfs],
+                               species_local=[(u.z_valence, 0.0, 0.0) for u in upfs], species_upf=list(upfs))
+
+
 def si64(ecut_ry=30.0, nbands=160):
     """2x2x2 supercell of the cubic cell, atoms displaced by N(0, 0.02 A), unshifted 2x2x2 mesh
     (8 k-points: the displacements leave only the identity), 160 bands."""
@@ -428,7 +443,12 @@ def ionic_potential(system, g, g2):
         for tau in s.atom_pos[np.asarray(s.atom_species) == sp]:
             sf += np.exp(-1j * (g @ tau))
         vq = np.zeros(g.shape[0])
-        vq[nz] = 2.0 * (-4.0 * math.pi * z / q2 + c1 * (2.0 * math.pi) ** 1.5 * rloc ** 3) * np.exp(-0.5 * q2 * rloc ** 2)
+        if s.species_upf is not None:
+            # The file's local form factor at each distinct |G| (hamiltonian.zig:449-473).
+            qs, inv = np.unique(np.sqrt(q2), return_inverse=True)
+            vq[nz] = upf.local_form_factor(s.species_upf[sp], qs)[inv]
+        else:
+            vq[nz] = 2.0 * (-4.0 * math.pi * z / q2 + c1 * (2.0 * math.pi) ** 1.5 * rloc ** 3) * np.exp(-0.5 * q2 * rloc ** 2)
         total += sf * vq
     return to_r(total * (g.shape[0] / s.volume), s.grid).real
 

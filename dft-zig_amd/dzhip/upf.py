@@ -231,6 +231,10 @@ class UpfSpecies:
     def dij(self):
         return self.upf.dij
 
+    @property
+    def m_total(self):
+        return sum(2 * b.l + 1 for b in self.upf.betas)
+
     def radial_fns(self):
         return [RadialTable(b.values, self.upf.r, self.upf.rab, b.l, self.g_max) for b in self.upf.betas]
 

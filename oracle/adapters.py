@@ -33,11 +33,35 @@ def gth_local_q(z_ion: float, rloc: float, c1: float):
     return fn
 
 
+# Radial tables of UPF species, built once per (file data, g_max): each is 2048 radial integrals.
+_UPF_TABLES = {}
+
+
+def upf_species(upf, g_max) -> projectors.Species:
+    """Oracle species from a UPF file's raw arrays: one RadialTable per beta over [0, g_max]
+    (nonlocal.zig:250-300, as fill_nonlocal_phi reads them), D from PP_DIJ."""
+    key = (id(upf), float(g_max))
+    if key not in _UPF_TABLES:
+        tabs = [projectors.RadialTable.build(b.values, upf.r, upf.rab, b.l, g_max) for b in upf.betas]
+        _UPF_TABLES[key] = (upf, tabs)
+    tabs = _UPF_TABLES[key][1]
+    return projectors.Species([b.l for b in upf.betas], [t.eval for t in tabs], upf.dij)
+
+
+def upf_local_q(upf):
+    """v(q) (Ry, per atom, without 1/Omega) of a UPF file's local part by the short-range split."""
+    return lambda q: projectors.local_vq_short_range(upf.v_local, upf.r, upf.rab, upf.z_valence, q)
+
+
 def context_for_k(system, k_cart, local_r) -> hamiltonian.HamContext:
-    """Oracle context of one k-point of an SCF system (raw parameters only)."""
+    """Oracle context of one k-point of an SCF system (raw parameters only; species read from
+    a UPF file carry `upf` and `g_max`)."""
     basis = pw.generate_basis(system.recip, system.ecut_ry, k_cart)
     sps = []
     for sp in system.species:
+        if hasattr(sp, "upf"):
+            sps.append(upf_species(sp.upf, sp.g_max))
+            continue
         rad = [projectors.gaussian_beta_radial(l, rc, projectors.gaussian_beta_norm(l, rc))
                for l, rc in zip(sp.l_list, sp.rc)]
         sps.append(projectors.Species(list(sp.l_list), rad, sp.dij))

@@ -7,6 +7,7 @@ Restates (paths relative to the dft-zig source tree):
   real_spherical_harmonic   nonlocal.zig:152-241 (l <= 4, m ordered -l..l)
   RadialTable               nonlocal.zig:250-300 (2048 points, linear interpolation, 0 beyond g_max)
   ctrap_weight              src/features/math/radial_integration.zig:7-13
+  local_vq_short_range      src/features/pseudopotential/form_factor.zig:225-256
   fill_nonlocal_radial_terms / fill_nonlocal_phi
                             src/features/scf/nonlocal_context.zig:437-477, 524-549
 """
@@ -114,6 +115,22 @@ def radial_projector(beta: np.ndarray, r: np.ndarray, rab: np.ndarray, l: int, g
     for i in range(n):
         s += r[i] * beta[i] * spherical_bessel(l, g * r[i]) * rab[i] * w[i]
     return s
+
+
+def local_vq_short_range(v_local: np.ndarray, r: np.ndarray, rab: np.ndarray, z_valence: float, q: float) -> float:
+    """V_loc(q) in Ry by the short-range split (form_factor.zig:225-256): 4 pi sum_i r_i^2
+    (V_loc(r_i) + 2Z/r_i) sinc(q r_i) rab_i w_i (the Coulomb term skipped at r < 1e-10), minus
+    8 pi Z / q^2 for q >= 1e-10; at q < 1e-10 the short-range part alone."""
+    n = r.shape[0]
+    w = ctrap_weights(n)
+    s = 0.0
+    for i in range(n):
+        vc = 0.0 if r[i] < 1e-10 else 2.0 * z_valence / r[i]
+        x = q * r[i]
+        sinc = 1.0 if abs(x) < 1e-12 else math.sin(x) / x
+        s += r[i] * r[i] * (v_local[i] + vc) * sinc * rab[i] * w[i]
+    v = 4.0 * math.pi * s
+    return v if q < 1e-10 else v - 8.0 * math.pi * z_valence / (q * q)
 
 
 @dataclass

@@ -211,7 +211,9 @@ def ionic_potential_r(grid, recip, volume, positions, vq_fn):
     sf = np.zeros(n, dtype=np.complex128)
     for tau in positions:
         sf += np.exp(-1j * (g @ tau))
-    vg[nz] = sf[nz] * np.array([vq_fn(x) for x in q[nz]]) / volume
+    # v(|G|) once per distinct |G| (bitwise the same values as one call per G).
+    qs, inv = np.unique(q[nz], return_inverse=True)
+    vg[nz] = sf[nz] * np.array([vq_fn(x) for x in qs])[inv] / volume
     return (fftconv.inverse3d(vg * n, grid)).real
 
 
@@ -364,8 +366,14 @@ def run_scf(system, n_iter=40, beta=0.3, conv=1e-8, pulay_history=0, pulay_start
     xc = getattr(system, "xc", "lda_pz")
     grid, recip, vol = system.grid, system.recip, system.volume
     n = int(np.prod(grid))
-    v_ion = ionic_potential_r(grid, recip, vol, system.atom_pos,
-                              adapters.gth_local_q(system.z_ion, system.rloc, system.c1))
+    upfs = getattr(system, "species_upf", None)
+    if upfs:
+        if len(upfs) != 1:
+            raise ValueError("the oracle SCF takes one species")
+        vq = adapters.upf_local_q(upfs[0])
+    else:
+        vq = adapters.gth_local_q(system.z_ion, system.rloc, system.c1)
+    v_ion = ionic_potential_r(grid, recip, vol, system.atom_pos, vq)
     rho = np.full(n, system.nelec / vol)
     vh, _ = hartree(rho, grid, recip, vol)
     _, vxc = xc_fields(rho, grid, recip, xc)

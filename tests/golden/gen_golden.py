@@ -96,8 +96,32 @@ def scf_al_pbe_file():
     print("scf al pbe", len(res.energies), res.energies[-1])
 
 
+def scf_upf_file():
+    """Oracle SCF of the small Si case with its species read from a UPF file (the GTH-form Si set
+    written as a UPF v2 file on a 1200-point log mesh, si_gth.upf): radial tables and the
+    short-range local form factor from the file's arrays."""
+    import hashlib
+
+    from dzhip import scf as dscf
+    from dzhip import upf
+    from oracle import scf as oscf
+
+    g = dscf.SI_GTH
+    path = HERE / "si_gth.upf"
+    upf.write_upf(path, "Si", g["z"], g["rloc"], g["c1"], [0, 1], [g["r_s"], g["r_p"]],
+                  np.diag([2 * g["h_s"], 2 * g["h_p"]]))
+    u = upf.read_upf(path)
+    system = dscf.with_upf(dscf.si8_small(), [u])
+    res = oscf.run_scf(system, n_iter=40, beta=0.3, conv=1e-10, pulay_history=8, pulay_start=4)
+    assert res.converged
+    sha = hashlib.sha256(path.read_bytes()).hexdigest()
+    np.savez_compressed(HERE / "scf_si8_upf.npz", energies=np.array(res.energies), drho=np.array(res.drho),
+                        eigenvalues=np.array(res.eigenvalues), rho=res.rho, upf_sha256=np.array(sha))
+    print("scf upf", len(res.energies), res.energies[-1])
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["hpsi", "fft", "scf", "scf_al", "scf_al_pbe"]
+    which = sys.argv[1:] or ["hpsi", "fft", "scf", "scf_al", "scf_al_pbe", "scf_upf"]
     if "hpsi" in which:
         hpsi_file("tiny")
         hpsi_file("si8")
@@ -109,3 +133,5 @@ if __name__ == "__main__":
         scf_al_file()
     if "scf_al_pbe" in which:
         scf_al_pbe_file()
+    if "scf_upf" in which:
+        scf_upf_file()
