@@ -690,8 +690,10 @@ __global__ __launch_bounds__(256) void k_ns_update(const double* __restrict__ Z,
 // each wavefront holding kUnmtrWaveCols of them in registers (lane l: rows l + 64 q).  The
 // reflectors are staged through LDS in blocks of unmtr_block(NQ) (rows <= k zero, row k+1 one), and the
 // next block's global loads are issued into registers before the current block is applied.
-// NQ: rows per lane (m <= 64 NQ).
-constexpr int kUnmtrWaveCols = 2;
+// NQ: rows per lane (m <= 64 NQ).  One eigenvector per wavefront (40 workgroups for 160 of them):
+// the reflector sequence is latency-bound, and halving each wavefront's dot products, sums and
+// updates per reflector beat two per wavefront by 3 % at m = 164..1000.
+constexpr int kUnmtrWaveCols = 1;
 constexpr int kUnmtrWgCols = 4 * kUnmtrWaveCols;
 // Reflectors per LDS stage (fewer for the larger register sets).
 constexpr int unmtr_block(int nq) { return nq <= 8 ? 8 : 4; }
@@ -702,6 +704,7 @@ __global__ __launch_bounds__(256) void k_unmtr(const double2* __restrict__ vh, c
                                                int ldy) {
     extern __shared__ double2 vs[];  // kUnmtrBlk x m
     __shared__ double2 ts[8];
+    static_assert(unmtr_block(NQ) <= 8, "ts holds one block's tau values");
     const int t = (int)threadIdx.x, wv = t >> 6, lane = t & 63;
     constexpr int kUnmtrRegs = NQ;
     constexpr int kUnmtrBlk = unmtr_block(NQ);
