@@ -246,6 +246,17 @@ def aluminium(ecut_ry=30.0, kmesh=(4, 4, 4), nbands=12, smearing_ry=0.01, grid=N
                      [sp], g["z"], g["rloc"], g["c1"], 12.0, nbands, kpts, kw, smearing_ry, xc=xc)
 
 
+def si8_config(ecut_ry=20.0, nbands=20):
+    """The 8-atom Si cell at its stated configuration: LDA, 20 Ry, auto grid (32^3), the unshifted
+    2x2x2 mesh of the cubic cell reduced by the structure's operations and time reversal to Gamma,
+    X, M and R with weights 1, 3, 3, 1 (/8)."""
+    s = silicon((1, 1, 1), ecut_ry, nbands=nbands)
+    kpts, kw, _ = ibz_kmesh(s.cell, (2, 2, 2), s.atom_pos @ np.linalg.inv(s.cell), s.atom_species)
+    import dataclasses
+
+    return dataclasses.replace(s, kpoints=[np.asarray(k, dtype=np.float64) for k in kpts], kweights=list(kw))
+
+
 def si8_small(ecut_ry=8.0, nbands=20):
     """Small SCF parity case: the 8-atom cubic cell at 8 Ry (24^3 grid, about 420 plane waves),
     k = Gamma and (1/4, 1/4, 1/4) in reciprocal-lattice units, weights 1/2 each."""

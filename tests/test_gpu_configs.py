@@ -8,6 +8,10 @@
   equal the oracle's dense
   eigh of the same Hamiltonian (dense_hamiltonian, the restatement of build_hamiltonian,
   src/features/hamiltonian/hamiltonian.zig:174-323) within 2e-8 Ry.
+* Si-8 at its stated configuration (LDA, 20 Ry, 32^3, the 2x2x2 mesh reduced to Gamma, X, M, R with
+  weights 1, 3, 3, 1): the device SCF (four k-points solved concurrently) converges, and at its
+  final input potential the 20 lowest eigenvalues at every k-point equal the oracle's dense eigh
+  of that Hamiltonian (V_ion rebuilt by the oracle from the GTH parameters) within 2e-8 Ry.
 * GaAs-64 (zincblende 2x2x2 supercell, a = 11.306 A, displacements N(0, 0.05 A), 35 Ry, 90^3
   grid, Gamma, 256 projectors): H.psi of 8 bands against the oracle's apply
   (apply_hamiltonian_batched, src/features/scf/apply.zig:1584-1618), FP64 1e-12 and FP32 2e-5 of
@@ -64,6 +68,35 @@ def test_al4_ibz_35k_eigenvalues_vs_dense(dz, ctx):
 @pytest.fixture(scope="module")
 def gaas64_w():
     return scf.kpoint_workload(scf.gaas64(), 0, nbands=160)
+
+
+def test_si8_config_scf_fixed_point_vs_dense(dz, ctx):
+    from oracle import scf as oscf
+
+    si = scf.si8_config()
+    assert si.grid == (32, 32, 32) and len(si.kpoints) == 4
+    assert sorted(round(w * 8) for w in si.kweights) == [1, 1, 3, 3]
+    drv = scf.ScfDriver(si, concurrent_k=True)
+    try:
+        # The Si-64 SCF settings: Pulay from the first iteration with the model dielectric, atomic start.
+        run = drv.run(max_iter=60, conv=1e-9, eig_tol=1e-10, eig_max_iter=400, final_tol=1e-10, pulay_start=1,
+                      diemac=12.0, start="atomic")
+    finally:
+        drv.close()
+    assert run.converged
+    v_ion = oscf.ionic_potential_r(si.grid, si.recip, si.volume, si.atom_pos,
+                                   adapters.gth_local_q(si.z_ion, si.rloc, si.c1))
+    local_r = v_ion + run.v_in
+
+    def dense(k):
+        h = hamiltonian.dense_hamiltonian(adapters.context_for_k(si, si.kpoints[k], local_r))
+        return np.linalg.eigvalsh(0.5 * (h + h.conj().T))[:si.nbands]
+
+    with ThreadPoolExecutor(4) as pool:
+        ref = list(pool.map(dense, range(4)))
+    err = max(float(np.max(np.abs(np.asarray(run.eigenvalues[k]) - ref[k]))) for k in range(4))
+    print("Si-8 config: iterations", len(run.records), "E", run.energies[-1], "max eigenvalue error", err)
+    assert err < 2e-8
 
 
 def test_gaas64_hpsi_vs_oracle(dz, ctx, gaas64_w):
