@@ -12,6 +12,9 @@
   weights 1, 3, 3, 1): the device SCF (four k-points solved concurrently) converges, and at its
   final input potential the 20 lowest eigenvalues at every k-point equal the oracle's dense eigh
   of that Hamiltonian (V_ion rebuilt by the oracle from the GTH parameters) within 2e-8 Ry.
+* Al-4 at its stated configuration (PBE, Fermi-Dirac 0.01 Ry, the 35 k-points above): the device
+  SCF (device grid, concurrent k-points) converges, and at its final input potential the 12
+  lowest eigenvalues at every k-point equal the oracle's dense eigh within 2e-8 Ry.
 * GaAs-64 (zincblende 2x2x2 supercell, a = 11.306 A, displacements N(0, 0.05 A), 35 Ry, 90^3
   grid, Gamma, 256 projectors): H.psi of 8 bands against the oracle's apply
   (apply_hamiltonian_batched, src/features/scf/apply.zig:1584-1618), FP64 1e-12 and FP32 2e-5 of
@@ -96,6 +99,32 @@ def test_si8_config_scf_fixed_point_vs_dense(dz, ctx):
         ref = list(pool.map(dense, range(4)))
     err = max(float(np.max(np.abs(np.asarray(run.eigenvalues[k]) - ref[k]))) for k in range(4))
     print("Si-8 config: iterations", len(run.records), "E", run.energies[-1], "max eigenvalue error", err)
+    assert err < 2e-8
+
+
+def test_al4_config_pbe_scf_fixed_point_vs_dense(dz, ctx):
+    from oracle import scf as oscf
+
+    al = scf.aluminium(symmetry=True, kmesh=(8, 8, 8), xc="pbe")
+    drv = scf.ScfDriver(al, concurrent_k=True, device_grid=True)
+    try:
+        run = drv.run(max_iter=60, conv=1e-8, eig_tol=1e-10, eig_max_iter=400, final_tol=1e-10, pulay_start=1,
+                      start="atomic")
+    finally:
+        drv.close()
+    assert run.converged
+    v_ion = oscf.ionic_potential_r(al.grid, al.recip, al.volume, al.atom_pos,
+                                   adapters.gth_local_q(al.z_ion, al.rloc, al.c1))
+    local_r = v_ion + run.v_in
+
+    def dense(k):
+        h = hamiltonian.dense_hamiltonian(adapters.context_for_k(al, al.kpoints[k], local_r))
+        return np.linalg.eigvalsh(0.5 * (h + h.conj().T))[:al.nbands]
+
+    with ThreadPoolExecutor(8) as pool:
+        ref = list(pool.map(dense, range(len(al.kpoints))))
+    err = max(float(np.max(np.abs(np.asarray(run.eigenvalues[k]) - ref[k]))) for k in range(len(al.kpoints)))
+    print("Al-4 config: iterations", len(run.records), "E", run.energies[-1], "max eigenvalue error", err)
     assert err < 2e-8
 
 
