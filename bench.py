@@ -37,6 +37,11 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "dft-zig_amd"))
 
+# Hardware queues of the HIP runtime (read once, when HIP initialises).  The concurrent-k SCF runs
+# eight k-point streams; the runtime's default of four queues puts several streams on one queue,
+# whose packets execute in order, so one stream's eigensolver holds back another's H applies.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 METRIC = "SCF time-to-1e-6 Ha & H·ψ GB/s, Si-64: mixed+batched+learned vs plain FP64"
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}

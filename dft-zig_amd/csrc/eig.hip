@@ -458,7 +458,8 @@ static std::vector<double> subspace_eigh(EigWork& w, const double2* A, int lda, 
                                          hipStream_t s) {
     std::vector<double> ev(nev);
     if (m <= kHeevMaxM) {
-        heev_lowest(w.heev, A, lda, m, nev, w.evals.as<double>(), Y, ldy, s);
+        TrdAdmission adm;  // returned after the synchronisation below
+        heev_lowest(w.heev, A, lda, m, nev, w.evals.as<double>(), Y, ldy, s, &adm);
         DZ_HIP(hipMemcpyAsync(ev.data(), w.evals.ptr, nev * sizeof(double), hipMemcpyDeviceToHost, s));
         DZ_REQUIRE(heev_error_flag(w.heev, s) == 0, DZHIP_ERR_HIP,
                    "Hermitian eigensolver: tridiagonalisation barrier did not complete");

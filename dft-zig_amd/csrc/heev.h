@@ -3,8 +3,11 @@
 // all on the device in a handful of launches.
 //
 //   1. Householder tridiagonalisation (zhetd2's reflectors, A = Q T Q^H with T real tridiagonal)
-//      in one cooperative launch: the trailing matrix is distributed over the workgroups' LDS by
-//      columns and each reflector costs one grid barrier.
+//      in one grid-synchronised launch: the trailing matrix is distributed over the workgroups'
+//      LDS by columns and each reflector costs one grid barrier.  The enqueue-only entry points
+//      launch it cooperatively; the solvers' Rayleigh-Ritz step launches it as an ordinary
+//      dispatch admitted by the device's tridiagonalisation gate (TrdAdmission), so that the
+//      grids of concurrent solves on different streams run side by side.
 //   2. The lowest nev eigenvalues of T by 64-way multisection with Sturm counts (one wavefront
 //      per eigenvalue).
 //   3. Eigenvectors of T by inverse iteration (tridiagonal LU with partial pivoting in LDS, one
@@ -29,12 +32,34 @@ struct HeevWork {
 // Largest matrix of the allocation-free variant (its buffers live in a 6 MiB device array).
 constexpr int kHeevNoAllocMaxM = 256;
 
+// Share of the device's tridiagonalisation gate held by one eigensolver call.  A grid barrier
+// needs all of its workgroups resident at once.  Workgroups are dealt to the XCDs round robin, so
+// a grid of G groups needs at most ceil(G / XCDs) CUs on any XCD (one workgroup per CU at worst).
+// The gate admits grids only while their sum stays within the CUs of one XCD; every other kernel
+// the library launches runs to completion without waiting on another workgroup,
+// so each admitted grid's workgroups are all placed once those retire, and no barrier waits on a
+// workgroup that cannot be placed.  The holder synchronises its stream (the grid has finished)
+// and returns its share when it goes out of scope.  Cooperative launches (the enqueue-only entry
+// points) are not counted: one that runs while concurrent solves hold the gate may find the
+// device short of CUs, and its barrier's spin limit then reports an error instead of waiting.
+struct TrdAdmission {
+    int device = -1, units = 0;
+    hipStream_t stream = nullptr;
+    TrdAdmission() = default;
+    TrdAdmission(const TrdAdmission&) = delete;
+    TrdAdmission& operator=(const TrdAdmission&) = delete;
+    ~TrdAdmission();
+};
+
 // Lowest nev eigenpairs of the Hermitian matrix A (column-major, leading dimension lda; only the
 // lower triangle is read, the imaginary part of the diagonal is ignored; A is not modified).  Eigenvalues (ascending) -> w_dev[0:nev] (device); eigenvectors ->
 // Y[:, 0:nev] (complex, leading dimension ldy).  Enqueued on stream s.  Throws on a size outside
 // 1 <= nev <= m <= kHeevMaxM.  After the call the host may read w_dev once s is synchronised.
+// With adm == nullptr the tridiagonalisation is a cooperative launch; otherwise it waits for a
+// share of the gate, launches as an ordinary dispatch and leaves the share in *adm (which must
+// be empty and must outlive the call's use of s).
 void heev_lowest(HeevWork& w, const double2* A, int lda, int m, int nev, double* w_dev, double2* Y, int ldy,
-                 hipStream_t s);
+                 hipStream_t s, TrdAdmission* adm = nullptr);
 
 // The tridiagonal stage only (for tests): d[0:m], e[0:m-1] of T on the device.
 void heev_tridiag_only(HeevWork& w, const double2* A, int lda, int m, hipStream_t s);

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdlib>
 #include <mutex>
 
@@ -880,7 +881,52 @@ struct HeevBufs {
     int err_offset;
 };
 
-void run_tridiag(const HeevBufs& b, int G, int ncl, const double2* A, int lda, int m, hipStream_t s) {
+// The tridiagonalisation gate of each device (see TrdAdmission): CUs per XCD taken by admitted
+// grids, out of cap.
+constexpr int kXcds = 8;
+struct TrdGate {
+    std::mutex mu;
+    std::condition_variable cv;
+    int used = 0, cap = 0;
+};
+TrdGate g_trd_gate[64];
+
+int trd_units(int G) { return (G + kXcds - 1) / kXcds; }
+
+void trd_gate_acquire(int dev, int units) {
+    TrdGate& gt = g_trd_gate[dev];
+    std::unique_lock<std::mutex> lk(gt.mu);
+    if (gt.cap == 0) {
+        int cus = 0;
+        DZ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        gt.cap = std::max(1, cus / kXcds);
+    }
+    DZ_REQUIRE(units >= 1 && units <= gt.cap, DZHIP_ERR_HIP, "Hermitian eigensolver: grid does not fit on the device");
+    gt.cv.wait(lk, [&] { return gt.used + units <= gt.cap; });
+    gt.used += units;
+}
+
+void trd_gate_release(int dev, int units) {
+    TrdGate& gt = g_trd_gate[dev];
+    {
+        std::lock_guard<std::mutex> lk(gt.mu);
+        gt.used -= units;
+    }
+    gt.cv.notify_all();
+}
+
+// Ordinary rather than cooperative launches of the gated tridiagonalisation; DZHIP_HEEV_COOP=1
+// forces the cooperative launch everywhere (for comparisons).
+bool trd_gated_enabled() {
+    static const bool on = [] {
+        const char* env = std::getenv("DZHIP_HEEV_COOP");
+        return !(env && std::atoi(env) != 0);
+    }();
+    return on;
+}
+
+void run_tridiag(const HeevBufs& b, int G, int ncl, const double2* A, int lda, int m, hipStream_t s,
+                 bool cooperative = true) {
     // Barrier flags (one 64-byte line per workgroup) and the error flag after them, cleared by a
     // kernel (not a memset node) on the same stream.
     const int nsync = (G + 1) * 16;
@@ -889,8 +935,13 @@ void run_tridiag(const HeevBufs& b, int G, int ncl, const double2* A, int lda, i
     TrdArgs args{A, lda, m, ncl, b.vh, b.tau, b.d, b.e, b.pbuf, b.nbuf, b.sync,
                  reinterpret_cast<int*>(b.sync) + b.err_offset};
     void* kargs[] = {&args};
-    DZ_HIP(hipLaunchCooperativeKernel(trd_kernel(ncl), dim3((unsigned)G), dim3(kTrdThreads), kargs,
-                                      (unsigned)trd_lds_bytes(m, ncl), s));
+    if (cooperative) {
+        DZ_HIP(hipLaunchCooperativeKernel(trd_kernel(ncl), dim3((unsigned)G), dim3(kTrdThreads), kargs,
+                                          (unsigned)trd_lds_bytes(m, ncl), s));
+    } else {
+        DZ_HIP(hipLaunchKernel(trd_kernel(ncl), dim3((unsigned)G), dim3(kTrdThreads), kargs,
+                               (unsigned)trd_lds_bytes(m, ncl), s));
+    }
 }
 
 void run_eig(const HeevBufs& b, int m, int nev, double* w_dev, double2* Y, int ldy, hipStream_t s) {
@@ -1027,14 +1078,33 @@ void heev_tridiag_only(HeevWork& w, const double2* A, int lda, int m, hipStream_
     run_tridiag(work_bufs(w, m, 0, G), G, ncl, A, lda, m, s);
 }
 
+TrdAdmission::~TrdAdmission() {
+    if (units <= 0) return;
+    (void)hipStreamSynchronize(stream);
+    trd_gate_release(device, units);
+}
+
 void heev_lowest(HeevWork& w, const double2* A, int lda, int m, int nev, double* w_dev, double2* Y, int ldy,
-                 hipStream_t s) {
+                 hipStream_t s, TrdAdmission* adm) {
     DZ_REQUIRE(m >= 1 && m <= kHeevMaxM && nev >= 1 && nev <= m && lda >= m && ldy >= m, DZHIP_ERR_ARG,
                "Hermitian eigensolver: need 1 <= nev <= m <= 1024 and leading dimensions >= m");
     int G = 1, ncl = 1;
     trd_shape(m, &G, &ncl);
     const HeevBufs b = work_bufs(w, m, nev, G);
-    run_tridiag(b, G, ncl, A, lda, m, s);
+    if (adm && trd_gated_enabled()) {
+        DZ_REQUIRE(adm->units == 0, DZHIP_ERR_STATE, "Hermitian eigensolver: admission already held");
+        int dev = 0;
+        DZ_HIP(hipGetDevice(&dev));
+        DZ_REQUIRE(dev >= 0 && dev < 64, DZHIP_ERR_ARG, "device index out of range");
+        const int units = trd_units(G);
+        trd_gate_acquire(dev, units);
+        adm->device = dev;
+        adm->units = units;
+        adm->stream = s;
+        run_tridiag(b, G, ncl, A, lda, m, s, false);
+    } else {
+        run_tridiag(b, G, ncl, A, lda, m, s);
+    }
     run_eig(b, m, nev, w_dev, Y, ldy, s);
 }
 

@@ -80,3 +80,41 @@ def test_heev_rayleigh_ritz_matrix(dz, ctx):
     t = v.conj().T @ hv
     t = 0.5 * (t + t.conj().T)
     check(ctx, t, 160)
+
+
+def test_heev_concurrent_threads(dz, ctx):
+    """Six host threads, each with its own context (own stream), solving at the same time.  The
+    blocking eigensolver launches its tridiagonalisation as an ordinary dispatch admitted by the
+    device's gate: at m = 648 a grid takes 14 CUs of each XCD, so at most two of these grids run
+    at once and the rest wait for their share.  Every result must equal the one-thread result."""
+    import threading
+
+    rng = np.random.default_rng(11)
+    sizes = (648, 492, 328, 648, 164, 492)
+    mats = [random_hermitian(m, rng)[0] for m in sizes]
+    ctxs = [dz.Context(0) for _ in sizes]
+    alone = [ctxs[0].heev(a, 160) for a in mats]
+    out, errs = [None] * len(sizes), []
+
+    def work(i):
+        try:
+            res = []
+            for _ in range(3):
+                res.append(ctxs[i].heev(mats[i], 160))
+            out[i] = res
+        except Exception as e:  # reported on the main thread
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(len(sizes))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for c in ctxs:
+        c.close()
+    assert not errs, errs
+    for i, a in enumerate(mats):
+        ref = np.linalg.eigvalsh(a)[:160]
+        for w, z in out[i]:
+            assert np.array_equal(w, alone[i][0]) and np.array_equal(z, alone[i][1])
+        assert np.max(np.abs(alone[i][0] - ref)) <= 1e-12 * max(1.0, float(np.max(np.abs(ref))))
