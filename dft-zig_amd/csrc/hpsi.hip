@@ -1156,8 +1156,12 @@ template <> struct Mfma<double> {
 // wavefront owns (line, 16-row block) jobs and keeps their accumulators in registers.  The rows
 // stream through LDS in chunks of kConvChunk (every load of a chunk issued before its LDS
 // stores); per chunk the jobs take their k-steps of 4 with four real MFMAs per complex product
-// into two accumulators (A from the workgroup's c values in LDS, B from the chunk).  The results go back in place
-// through the same chunk buffer, kConvChunk rows at a time, as coalesced stores.
+// into two accumulators (A from the workgroup's c values in LDS, B from the chunk).  In FP64 the
+// last n_p mod 16 <= 8 rows are not padded to a 16-row block: (line, 4-row group) jobs take them
+// with the 4x4x4 MFMA (four blocks = four groups of four bands, a third of the 16x16x4's time),
+// whose B operand sits in the same lanes as the 16x16x4's (band = lane % 16, k = lane / 16);
+// A[i][k] is lane i + 4 q + 16 k of block q and D[i][j] lane 16 i + 4 q + j.  The results go back
+// in place through the same chunk buffer, kConvChunk rows at a time, as coalesced stores.
 template <typename T> struct ConvLines { static constexpr int value = 16; };
 template <> struct ConvLines<double> { static constexpr int value = 8; };
 constexpr int kConvBands = 16;
@@ -1173,7 +1177,7 @@ __device__ __forceinline__ int conv_idx(int p, int b, int li) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(kConvThreads, 2) void k_xline_conv(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
+__global__ __launch_bounds__(kConvThreads, 4) void k_xline_conv(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
                                                     int nc, const typename C2T<T>::type* __restrict__ chat, int nx,
                                                     const int2* __restrict__ kpl) {
     using C = typename C2T<T>::type;
@@ -1182,6 +1186,8 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_xline_conv(typename C2T<T>:
     constexpr int NB = kConvBands;
     constexpr int kWaves = kConvThreads / 64;
     constexpr int kMaxJobs = LT * (kConvMaxPlanes / 16) / kWaves;  // per wavefront
+    constexpr bool kTail = std::is_same<T, double>::value;          // 4-row tail groups (FP64)
+    constexpr int kMaxTail = kTail ? LT * 2 / kWaves : 1;            // tail jobs per wavefront
     constexpr int kChunkElems = kConvChunk * NB * LT;
     constexpr int kPer = kChunkElems / kConvThreads;           // loads per thread per chunk
     static_assert(kChunkElems % kConvThreads == 0 && (LT * (kConvMaxPlanes / 16)) % kWaves == 0,
@@ -1205,14 +1211,21 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_xline_conv(typename C2T<T>:
     const int nlt = min(LT, lines - l0);
     for (int i = tid; i < LT * nx; i += kConvThreads)
         cs[i] = i < nlt * nx ? chat[(size_t)l0 * nx + i] : C{T(0), T(0)};
-    const int nmt = (np + 15) >> 4;
+    // 16-row blocks (the last one padded unless a tail of <= 8 rows goes to 4-row groups).
+    const int nrem = np & 15;
+    const bool tail = kTail && nrem > 0 && nrem <= 8;
+    const int nmt = tail ? np >> 4 : (np + 15) >> 4;
     const int njobs = LT * nmt;
+    const int ntjobs = tail ? LT * ((nrem + 3) >> 2) : 0;
     typename M::acc re[kMaxJobs], im[kMaxJobs];
 #pragma unroll
     for (int jj = 0; jj < kMaxJobs; ++jj) {
         re[jj] = typename M::acc{0, 0, 0, 0};
         im[jj] = typename M::acc{0, 0, 0, 0};
     }
+    double tre[kMaxTail], tim[kMaxTail];
+#pragma unroll
+    for (int tt = 0; tt < kMaxTail; ++tt) tre[tt] = tim[tt] = 0.0;
     // Chunk rows k0 .. k0 + kConvChunk - 1 (rows >= np zero), loaded into registers one chunk ahead:
     // the next chunk's loads are in flight while the current one is multiplied.
     C v[kPer];
@@ -1259,6 +1272,31 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_xline_conv(typename C2T<T>:
                 im[jj] = M::mma(a.y, bv.x, im[jj]);
             }
         }
+        if constexpr (kTail) {
+#pragma unroll
+            for (int tt = 0; tt < kMaxTail; ++tt) {
+                const int job = wave + kWaves * tt;
+                if (job >= ntjobs) break;
+                const int li = job % LT;
+                const int q = 16 * nmt + 4 * (job / LT) + (lane & 3);  // A row of this lane
+                const C* cl = cs + li * nx;
+#pragma unroll
+                for (int ks = 0; ks < kConvChunk; ks += 4) {
+                    const int p = k0 + ks + l4;
+                    C a = C{T(0), T(0)};
+                    if (q < np && p < np) {
+                        int d = q - p;
+                        d += d < 0 ? nx : 0;
+                        a = cl[d];
+                    }
+                    const C bv = xs[conv_idx<LT>(ks + l4, l16, li)];
+                    tre[tt] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.x, bv.x, tre[tt], 0, 0, 0);
+                    tre[tt] = __builtin_amdgcn_mfma_f64_4x4x4f64(-a.y, bv.y, tre[tt], 0, 0, 0);
+                    tim[tt] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.x, bv.y, tim[tt], 0, 0, 0);
+                    tim[tt] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.y, bv.x, tim[tt], 0, 0, 0);
+                }
+            }
+        }
     }
     // Results back in place through the chunk buffer, kConvChunk rows at a time.
     for (int r0 = 0; r0 < np; r0 += kConvChunk) {
@@ -1272,6 +1310,16 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_xline_conv(typename C2T<T>:
             for (int r = 0; r < 4; ++r) {
                 const int q = mt * 16 + M::row(l4, r);
                 if (q >= r0 && q < r0 + kConvChunk && q < np) xs[conv_idx<LT>(q - r0, l16, li)] = C{re[jj][r], im[jj][r]};
+            }
+        }
+        if constexpr (kTail) {
+#pragma unroll
+            for (int tt = 0; tt < kMaxTail; ++tt) {
+                const int job = wave + kWaves * tt;
+                if (job >= ntjobs) break;
+                const int li = job % LT;
+                const int q = 16 * nmt + 4 * (job / LT) + l4;  // D row of this lane
+                if (q >= r0 && q < r0 + kConvChunk && q < np) xs[conv_idx<LT>(q - r0, l16, li)] = C{T(tre[tt]), T(tim[tt])};
             }
         }
         __syncthreads();

@@ -1,5 +1,5 @@
-"""The device Hermitian eigensolver (dzhip_heev: Householder tridiagonalisation in one cooperative
-launch, multisection, inverse iteration, back-transformation) against numpy.linalg.eigh, which is
+"""The device Hermitian eigensolver (dzhip_heev: Householder tridiagonalisation in one
+grid-synchronised launch, multisection, inverse iteration, back-transformation) against numpy.linalg.eigh, which is
 LAPACK zheevd: the reference's subspace eigenproblems call LAPACK zheev
 (src/features/linalg/lobpcg/common.zig:440-537).
 

@@ -1,9 +1,12 @@
 """CPU model of the convolution x-line kernel (dft-zig_amd/csrc/hpsi.hip, k_xline_conv): the same
 12-row LDS chunks with their XOR-swizzled index, the same (line, 16-row block) jobs over eight
 wavefronts, the MFMA 16x16x4 lane layouts (A[i = l % 16][k = l / 16], B[k = l / 16][j = l % 16],
-accumulator r of lane l at row l / 16 + 4 r in FP64 and 4 (l / 16) + r in FP32), the four-product
-complex form and the chunked write-back, checked against the x-line step written with FFTs:
-inverse transform of the present frequencies x_s + p, times V/N, forward transform (numpy)."""
+accumulator r of lane l at row l / 16 + 4 r in FP64 and 4 (l / 16) + r in FP32), the FP64 tail
+of n_p mod 16 <= 8 rows as (line, 4-row group) jobs on the 4x4x4 MFMA (four blocks q = band
+groups; A[i][k] in lane i + 4 q + 16 k, B[k][j] in lane j + 4 q + 16 k, D[i][j] in lane
+16 i + 4 q + j, as measured on the device), the four-product complex form and the chunked
+write-back, checked against the x-line step written with FFTs: inverse transform of the present
+frequencies x_s + p, times V/N, forward transform (numpy)."""
 import numpy as np
 import pytest
 
@@ -17,15 +20,32 @@ def conv_idx(p, b, li, lt):
 CHUNK = 12
 
 
+def mfma_4x4x4(a_lanes, b_lanes):
+    """One 4x4x4 MFMA (four blocks) from per-lane operands: D lane 16 i + 4 q + j."""
+    d = np.zeros(64)
+    for q in range(4):
+        for i in range(4):
+            for j in range(4):
+                d[16 * i + 4 * q + j] = sum(a_lanes[i + 4 * q + 16 * k] * b_lanes[j + 4 * q + 16 * k] for k in range(4))
+    return d
+
+
 def model_kernel(tile, chat, np_, nx, lt, fp64):
     """tile: [np][bands=16][lt] complex rows of one workgroup; chat: [lt][nx]."""
-    nmt = (np_ + 15) >> 4
-    jobs = {}
+    nrem = np_ & 15
+    tail = fp64 and 0 < nrem <= 8
+    nmt = np_ >> 4 if tail else (np_ + 15) >> 4
+    ntjobs = lt * ((nrem + 3) >> 2) if tail else 0
+    jobs, tjobs = {}, {}
     for wave in range(8):
         for jj in range(lt * 3 // 8):
             job = wave + 8 * jj
             if job < lt * nmt:
                 jobs[job] = [np.zeros((16, 16)), np.zeros((16, 16))]
+        for tt in range(lt * 2 // 8):
+            job = wave + 8 * tt
+            if job < ntjobs:
+                tjobs[job] = [np.zeros(64), np.zeros(64)]  # per-lane accumulators
     for k0 in range(0, np_, CHUNK):
         xs = np.zeros(CHUNK * KCONV_BANDS * lt, complex)
         for r in range(CHUNK):
@@ -46,6 +66,18 @@ def model_kernel(tile, chat, np_, nx, lt, fp64):
                     bm[l4, l16] = xs[conv_idx(ks + l4, l16, li, lt)]
                 re += a.real @ bm.real - a.imag @ bm.imag
                 im += a.real @ bm.imag + a.imag @ bm.real
+        for job, (tre, tim) in tjobs.items():
+            li, g = job % lt, job // lt
+            for ks in range(0, CHUNK, 4):
+                a = np.zeros(64, complex)
+                bl = np.zeros(64, complex)
+                for lane in range(64):
+                    q, p = 16 * nmt + 4 * g + (lane & 3), k0 + ks + (lane >> 4)
+                    if q < np_ and p < np_:
+                        a[lane] = chat[li, (q - p) % nx]
+                    bl[lane] = xs[conv_idx(ks + (lane >> 4), lane & 15, li, lt)]
+                tre += mfma_4x4x4(a.real, bl.real) - mfma_4x4x4(a.imag, bl.imag)
+                tim += mfma_4x4x4(a.real, bl.imag) + mfma_4x4x4(a.imag, bl.real)
     out = np.zeros((np_, KCONV_BANDS, lt), complex)
     for r0 in range(0, np_, CHUNK):
         xs = np.zeros(CHUNK * KCONV_BANDS * lt, complex)
@@ -59,6 +91,12 @@ def model_kernel(tile, chat, np_, nx, lt, fp64):
                     q = mt * 16 + row
                     if r0 <= q < r0 + CHUNK and q < np_:
                         xs[conv_idx(q - r0, l16, li, lt)] = re[row, l16] + 1j * im[row, l16]
+        for job, (tre, tim) in tjobs.items():
+            li, g = job % lt, job // lt
+            for lane in range(64):
+                q = 16 * nmt + 4 * g + (lane >> 4)
+                if r0 <= q < r0 + CHUNK and q < np_:
+                    xs[conv_idx(q - r0, lane & 15, li, lt)] = tre[lane] + 1j * tim[lane]
         for r in range(CHUNK):
             for b in range(KCONV_BANDS):
                 for li in range(lt):
@@ -68,7 +106,9 @@ def model_kernel(tile, chat, np_, nx, lt, fp64):
 
 
 @pytest.mark.parametrize("nx,np_,xs,fp64", [(75, 36, 57, True), (75, 36, 57, False), (30, 13, 24, True),
-                                             (90, 45, 68, False), (32, 17, 0, True), (48, 48, 0, True)])
+                                             (90, 45, 68, False), (32, 17, 0, True), (48, 48, 0, True),
+                                             (80, 40, 60, True), (64, 38, 50, True), (90, 45, 68, True),
+                                             (40, 6, 37, True)])
 def test_conv_kernel_model_equals_fft_form(nx, np_, xs, fp64):
     lt = 8 if fp64 else 16
     rng = np.random.default_rng(nx + np_)
