@@ -1176,7 +1176,16 @@ __device__ __forceinline__ int conv_idx(int p, int b, int li) {
     return (p * kConvBands + b) * LT + (li ^ (b & (LT - 1)));
 }
 
-template <typename T>
+// 16-row blocks of n_p rows: FP64 sends a remainder of at most 8 rows to 4-row groups.
+__host__ __device__ constexpr int conv_blocks(int np, bool fp64) {
+    return fp64 && (np & 15) > 0 && (np & 15) <= 8 ? np >> 4 : (np + 15) >> 4;
+}
+
+// NMT: the most 16-row blocks of any line's n_p in the launch (1..3); it sizes the accumulators.
+// With at most two blocks the FP64 kernel has the registers for three real products per complex
+// one in its 16-row blocks (t1 = ar br, t2 = ai bi, t3 = (ar + ai)(br + bi); Re = t1 - t2,
+// Im = t3 - t1 - t2); the 4-row tail groups keep four.
+template <typename T, int NMT>
 __global__ __launch_bounds__(kConvThreads, 4) void k_xline_conv(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
                                                     int nc, const typename C2T<T>::type* __restrict__ chat, int nx,
                                                     const int2* __restrict__ kpl) {
@@ -1185,12 +1194,13 @@ __global__ __launch_bounds__(kConvThreads, 4) void k_xline_conv(typename C2T<T>:
     constexpr int LT = ConvLines<T>::value;
     constexpr int NB = kConvBands;
     constexpr int kWaves = kConvThreads / 64;
-    constexpr int kMaxJobs = LT * (kConvMaxPlanes / 16) / kWaves;  // per wavefront
+    constexpr int kMaxJobs = LT * NMT / kWaves;                      // per wavefront
     constexpr bool kTail = std::is_same<T, double>::value;          // 4-row tail groups (FP64)
     constexpr int kMaxTail = kTail ? LT * 2 / kWaves : 1;            // tail jobs per wavefront
+    constexpr bool k3 = kTail && NMT <= 2;                            // three products per complex one
     constexpr int kChunkElems = kConvChunk * NB * LT;
     constexpr int kPer = kChunkElems / kConvThreads;           // loads per thread per chunk
-    static_assert(kChunkElems % kConvThreads == 0 && (LT * (kConvMaxPlanes / 16)) % kWaves == 0,
+    static_assert(kChunkElems % kConvThreads == 0 && (LT * NMT) % kWaves == 0 && NMT >= 1 && NMT * 16 <= kConvMaxPlanes,
                   "chunk and jobs must split evenly over the workgroup");
     extern __shared__ unsigned char smem_raw[];
     C* xs = reinterpret_cast<C*>(smem_raw);  // one chunk [kConvChunk][NB][LT]
@@ -1214,14 +1224,16 @@ __global__ __launch_bounds__(kConvThreads, 4) void k_xline_conv(typename C2T<T>:
     // 16-row blocks (the last one padded unless a tail of <= 8 rows goes to 4-row groups).
     const int nrem = np & 15;
     const bool tail = kTail && nrem > 0 && nrem <= 8;
-    const int nmt = tail ? np >> 4 : (np + 15) >> 4;
+    const int nmt = conv_blocks(np, kTail);
     const int njobs = LT * nmt;
     const int ntjobs = tail ? LT * ((nrem + 3) >> 2) : 0;
-    typename M::acc re[kMaxJobs], im[kMaxJobs];
+    // Accumulators: (re, im) with four products, (t1, t2, t3) in (re, im, t3) with three.
+    typename M::acc re[kMaxJobs], im[kMaxJobs], t3[k3 ? kMaxJobs : 1];
 #pragma unroll
     for (int jj = 0; jj < kMaxJobs; ++jj) {
         re[jj] = typename M::acc{0, 0, 0, 0};
         im[jj] = typename M::acc{0, 0, 0, 0};
+        if constexpr (k3) t3[jj] = typename M::acc{0, 0, 0, 0};
     }
     double tre[kMaxTail], tim[kMaxTail];
 #pragma unroll
@@ -1265,11 +1277,17 @@ __global__ __launch_bounds__(kConvThreads, 4) void k_xline_conv(typename C2T<T>:
                     a = cl[d];
                 }
                 const C bv = xs[conv_idx<LT>(ks + l4, l16, li)];
-                // a b = (ar br - ai bi) + i (ar bi + ai br): four real products, two accumulators
-                re[jj] = M::mma(a.x, bv.x, re[jj]);
-                re[jj] = M::mma(-a.y, bv.y, re[jj]);
-                im[jj] = M::mma(a.x, bv.y, im[jj]);
-                im[jj] = M::mma(a.y, bv.x, im[jj]);
+                if constexpr (k3) {
+                    re[jj] = M::mma(a.x, bv.x, re[jj]);  // t1
+                    im[jj] = M::mma(a.y, bv.y, im[jj]);  // t2
+                    t3[jj] = M::mma(a.x + a.y, bv.x + bv.y, t3[jj]);
+                } else {
+                    // a b = (ar br - ai bi) + i (ar bi + ai br): four real products, two accumulators
+                    re[jj] = M::mma(a.x, bv.x, re[jj]);
+                    re[jj] = M::mma(-a.y, bv.y, re[jj]);
+                    im[jj] = M::mma(a.x, bv.y, im[jj]);
+                    im[jj] = M::mma(a.y, bv.x, im[jj]);
+                }
             }
         }
         if constexpr (kTail) {
@@ -1309,7 +1327,12 @@ __global__ __launch_bounds__(kConvThreads, 4) void k_xline_conv(typename C2T<T>:
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int q = mt * 16 + M::row(l4, r);
-                if (q >= r0 && q < r0 + kConvChunk && q < np) xs[conv_idx<LT>(q - r0, l16, li)] = C{re[jj][r], im[jj][r]};
+                if (q >= r0 && q < r0 + kConvChunk && q < np) {
+                    if constexpr (k3)
+                        xs[conv_idx<LT>(q - r0, l16, li)] = C{re[jj][r] - im[jj][r], t3[jj][r] - re[jj][r] - im[jj][r]};
+                    else
+                        xs[conv_idx<LT>(q - r0, l16, li)] = C{re[jj][r], im[jj][r]};
+                }
             }
         }
         if constexpr (kTail) {
@@ -1355,11 +1378,21 @@ void launch_xline_conv(Ham& h, typename C2T<T>::type* tb, size_t nc, hipStream_t
     const int np_max = max_planes >= 0 ? max_planes : h.nplanes;
     DZ_REQUIRE(np_max >= 1 && np_max <= kConvMaxPlanes, DZHIP_ERR_STATE, "convolution x-line kernel: too many planes");
     const size_t lds = xline_conv_lds_bytes<T>(h.nx);
-    set_max_dynamic_lds((const void*)k_xline_conv<T>, lds);
+    // n_p of every line of the launch is <= np_max, and conv_blocks is non-decreasing in n_p.
+    const int nmt = conv_blocks(np_max, std::is_same<T, double>::value);
     const C* chat = std::is_same<T, float>::value ? (const C*)h.chat32.ptr : (const C*)h.chat64.ptr;
     const dim3 grid((unsigned)((nc + kConvBands - 1) / kConvBands),
                     (unsigned)((lines + ConvLines<T>::value - 1) / ConvLines<T>::value), (unsigned)nk);
-    hipLaunchKernelGGL(k_xline_conv<T>, grid, dim3(kConvThreads), lds, s, tb, nplanes, lines, (int)nc, chat, h.nx, kpl);
+    auto launch = [&](auto kern) {
+        set_max_dynamic_lds((const void*)kern, lds);
+        hipLaunchKernelGGL(kern, grid, dim3(kConvThreads), lds, s, tb, nplanes, lines, (int)nc, chat, h.nx, kpl);
+    };
+    if (nmt <= 1)
+        launch(k_xline_conv<T, 1>);
+    else if (nmt == 2)
+        launch(k_xline_conv<T, 2>);
+    else
+        launch(k_xline_conv<T, 3>);
     DZ_HIP(hipGetLastError());
 }
 

@@ -4,8 +4,9 @@ wavefronts, the MFMA 16x16x4 lane layouts (A[i = l % 16][k = l / 16], B[k = l / 
 accumulator r of lane l at row l / 16 + 4 r in FP64 and 4 (l / 16) + r in FP32), the FP64 tail
 of n_p mod 16 <= 8 rows as (line, 4-row group) jobs on the 4x4x4 MFMA (four blocks q = band
 groups; A[i][k] in lane i + 4 q + 16 k, B[k][j] in lane j + 4 q + 16 k, D[i][j] in lane
-16 i + 4 q + j, as measured on the device), the four-product complex form and the chunked
-write-back, checked against the x-line step written with FFTs: inverse transform of the present
+16 i + 4 q + j, as measured on the device), the four-product complex form (three products,
+Re = t1 - t2 and Im = t3 - t1 - t2, in the FP64 16-row blocks when there are at most two) and the
+chunked write-back, checked against the x-line step written with FFTs: inverse transform of the present
 frequencies x_s + p, times V/N, forward transform (numpy)."""
 import numpy as np
 import pytest
@@ -36,12 +37,14 @@ def model_kernel(tile, chat, np_, nx, lt, fp64):
     tail = fp64 and 0 < nrem <= 8
     nmt = np_ >> 4 if tail else (np_ + 15) >> 4
     ntjobs = lt * ((nrem + 3) >> 2) if tail else 0
-    jobs, tjobs = {}, {}
+    three = fp64 and nmt <= 2
+    jobs, tjobs, t3s = {}, {}, {}
     for wave in range(8):
         for jj in range(lt * 3 // 8):
             job = wave + 8 * jj
             if job < lt * nmt:
                 jobs[job] = [np.zeros((16, 16)), np.zeros((16, 16))]
+                t3s[job] = np.zeros((16, 16))
         for tt in range(lt * 2 // 8):
             job = wave + 8 * tt
             if job < ntjobs:
@@ -64,8 +67,13 @@ def model_kernel(tile, chat, np_, nx, lt, fp64):
                     if q < np_ and p < np_:
                         a[l16, l4] = chat[li, (q - p) % nx]
                     bm[l4, l16] = xs[conv_idx(ks + l4, l16, li, lt)]
-                re += a.real @ bm.real - a.imag @ bm.imag
-                im += a.real @ bm.imag + a.imag @ bm.real
+                if three:  # re holds t1, im t2, t3s[job] t3
+                    re += a.real @ bm.real
+                    im += a.imag @ bm.imag
+                    t3s[job] += (a.real + a.imag) @ (bm.real + bm.imag)
+                else:
+                    re += a.real @ bm.real - a.imag @ bm.imag
+                    im += a.real @ bm.imag + a.imag @ bm.real
         for job, (tre, tim) in tjobs.items():
             li, g = job % lt, job // lt
             for ks in range(0, CHUNK, 4):
@@ -90,7 +98,11 @@ def model_kernel(tile, chat, np_, nx, lt, fp64):
                     row = l4 + 4 * r if fp64 else 4 * l4 + r
                     q = mt * 16 + row
                     if r0 <= q < r0 + CHUNK and q < np_:
-                        xs[conv_idx(q - r0, l16, li, lt)] = re[row, l16] + 1j * im[row, l16]
+                        if three:
+                            t1, t2, t3 = re[row, l16], im[row, l16], t3s[job][row, l16]
+                            xs[conv_idx(q - r0, l16, li, lt)] = (t1 - t2) + 1j * (t3 - t1 - t2)
+                        else:
+                            xs[conv_idx(q - r0, l16, li, lt)] = re[row, l16] + 1j * im[row, l16]
         for job, (tre, tim) in tjobs.items():
             li, g = job % lt, job // lt
             for lane in range(64):
