@@ -1156,12 +1156,13 @@ template <> struct Mfma<double> {
 // wavefront owns (line, 16-row block) jobs and keeps their accumulators in registers.  The rows
 // stream through LDS in chunks of kConvChunk (every load of a chunk issued before its LDS
 // stores); per chunk the jobs take their k-steps of 4 with four real MFMAs per complex product
-// into two accumulators (A from the workgroup's c values in LDS, B from the chunk).  In FP64 the
-// last n_p mod 16 <= 8 rows are not padded to a 16-row block: (line, 4-row group) jobs take them
-// with the 4x4x4 MFMA (four blocks = four groups of four bands, a third of the 16x16x4's time),
-// whose B operand sits in the same lanes as the 16x16x4's (band = lane % 16, k = lane / 16);
-// A[i][k] is lane i + 4 q + 16 k of block q and D[i][j] lane 16 i + 4 q + j.  The results go back
-// in place through the same chunk buffer, kConvChunk rows at a time, as coalesced stores.
+// into two accumulators (A from the workgroup's c values in LDS, B from the chunk).  In FP64
+// launches of at most two 16-row blocks the last n_p mod 16 <= 8 rows are not padded to a block:
+// (line, 4-row group) jobs take them with the 4x4x4 MFMA (four blocks = four groups of four
+// bands, a third of the 16x16x4's time), whose B operand sits in the same lanes as the
+// 16x16x4's (band = lane % 16, k = lane / 16); A[i][k] is lane i + 4 q + 16 k of block q and
+// D[i][j] lane 16 i + 4 q + j.  The results go back in place through the same chunk buffer,
+// kConvChunk rows at a time, as coalesced stores.
 template <typename T> struct ConvLines { static constexpr int value = 16; };
 template <> struct ConvLines<double> { static constexpr int value = 8; };
 constexpr int kConvBands = 16;
@@ -1182,11 +1183,11 @@ __host__ __device__ constexpr int conv_blocks(int np, bool fp64) {
 }
 
 // NMT: the most 16-row blocks of any line's n_p in the launch (1..3); it sizes the accumulators.
-// With at most two blocks the FP64 kernel has the registers for three real products per complex
-// one in its 16-row blocks (t1 = ar br, t2 = ai bi, t3 = (ar + ai)(br + bi); Re = t1 - t2,
-// Im = t3 - t1 - t2); the 4-row tail groups keep four.
+// With at most two blocks the FP64 kernel has the registers for the 4-row tail groups and for
+// three real products per complex one in its 16-row blocks (t1 = ar br, t2 = ai bi,
+// t3 = (ar + ai)(br + bi); Re = t1 - t2, Im = t3 - t1 - t2); the tail groups keep four.
 template <typename T, int NMT>
-__global__ __launch_bounds__(kConvThreads, 4) void k_xline_conv(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
+__global__ __launch_bounds__(kConvThreads, NMT <= 2 ? 4 : 2) void k_xline_conv(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
                                                     int nc, const typename C2T<T>::type* __restrict__ chat, int nx,
                                                     const int2* __restrict__ kpl) {
     using C = typename C2T<T>::type;
@@ -1195,9 +1196,11 @@ __global__ __launch_bounds__(kConvThreads, 4) void k_xline_conv(typename C2T<T>:
     constexpr int NB = kConvBands;
     constexpr int kWaves = kConvThreads / 64;
     constexpr int kMaxJobs = LT * NMT / kWaves;                      // per wavefront
-    constexpr bool kTail = std::is_same<T, double>::value;          // 4-row tail groups (FP64)
-    constexpr int kMaxTail = kTail ? LT * 2 / kWaves : 1;            // tail jobs per wavefront
-    constexpr bool k3 = kTail && NMT <= 2;                            // three products per complex one
+    // 4-row tail groups and three products per complex one: FP64 with at most two 16-row blocks
+    // (a launch whose largest n_p needs three blocks pads every line's last block, as in FP32).
+    constexpr bool kTail = std::is_same<T, double>::value && NMT <= 2;
+    constexpr int kMaxTail = kTail ? LT * 2 / kWaves : 1;  // tail jobs per wavefront
+    constexpr bool k3 = kTail;
     constexpr int kChunkElems = kConvChunk * NB * LT;
     constexpr int kPer = kChunkElems / kConvThreads;           // loads per thread per chunk
     static_assert(kChunkElems % kConvThreads == 0 && (LT * NMT) % kWaves == 0 && NMT >= 1 && NMT * 16 <= kConvMaxPlanes,
@@ -1224,7 +1227,7 @@ __global__ __launch_bounds__(kConvThreads, 4) void k_xline_conv(typename C2T<T>:
     // 16-row blocks (the last one padded unless a tail of <= 8 rows goes to 4-row groups).
     const int nrem = np & 15;
     const bool tail = kTail && nrem > 0 && nrem <= 8;
-    const int nmt = conv_blocks(np, kTail);
+    const int nmt = conv_blocks(np, kTail);  // <= NMT: n_p <= the launch's largest
     const int njobs = LT * nmt;
     const int ntjobs = tail ? LT * ((nrem + 3) >> 2) : 0;
     // Accumulators: (re, im) with four products, (t1, t2, t3) in (re, im, t3) with three.
