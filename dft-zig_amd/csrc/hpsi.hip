@@ -1186,8 +1186,13 @@ __host__ __device__ constexpr int conv_blocks(int np, bool fp64) {
 // With at most two blocks the FP64 kernel has the registers for the 4-row tail groups and for
 // three real products per complex one in its 16-row blocks (t1 = ar br, t2 = ai bi,
 // t3 = (ar + ai)(br + bi); Re = t1 - t2, Im = t3 - t1 - t2); the tail groups keep four.
+// Four wavefronts per SIMD (128 VGPRs) except FP64 with three blocks, which fits in 123 on its own
+// and runs faster scheduled without the bound.
 template <typename T, int NMT>
-__global__ __launch_bounds__(kConvThreads, NMT <= 2 ? 4 : 2) void k_xline_conv(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
+constexpr int conv_waves_per_eu() { return NMT <= 2 || !std::is_same<T, double>::value ? 4 : 2; }
+
+template <typename T, int NMT>
+__global__ __launch_bounds__(kConvThreads, (conv_waves_per_eu<T, NMT>())) void k_xline_conv(typename C2T<T>::type* __restrict__ tbuf, int nplanes, int lines,
                                                     int nc, const typename C2T<T>::type* __restrict__ chat, int nx,
                                                     const int2* __restrict__ kpl) {
     using C = typename C2T<T>::type;
