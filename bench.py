@@ -40,7 +40,12 @@ sys.path.insert(0, str(ROOT / "dft-zig_amd"))
 # Hardware queues of the HIP runtime (read once, when HIP initialises).  The concurrent-k SCF runs
 # eight k-point streams; the runtime's default of four queues puts several streams on one queue,
 # whose packets execute in order, so one stream's eigensolver holds back another's H applies.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+try:
+    _queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
+except ValueError:
+    _queues = 0
+if _queues < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 METRIC = "SCF time-to-1e-6 Ha & H·ψ GB/s, Si-64: mixed+batched+learned vs plain FP64"
 HBM_PEAK_GBS = 8000.0
@@ -190,7 +195,8 @@ def scf_leg(name, max_iter, arms, log=sys.stderr):
         # every band converged (Fermi-Dirac occupations), the metal's own dielectric constant.
         settings.update(init_diagonal=False, diemac=1.0)
     out = {"system": system.name, "grid": list(system.grid), "k_points": len(system.kpoints),
-           "bands": system.nbands, "settings": settings, "max_iter": max_iter}
+           "bands": system.nbands, "settings": settings, "max_iter": max_iter,
+           "gpu_max_hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))}
     params, learned_base = None, dzhip.PRECOND_KINETIC
     if any(a.endswith("learned") for a in arms):
         t0 = time.perf_counter()
