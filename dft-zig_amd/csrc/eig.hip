@@ -440,6 +440,146 @@ static void section_times(EigWork& w, double* out) {
 // into Y (leading dimension ldy) and the host vector returned (ascending).  The library's device
 // eigensolver up to m = kHeevMaxM; rocSOLVER zheevd beyond (it overwrites a copy of A in Y,
 // which must then hold m columns).
+// Upper Cholesky factorisation A = R^H R of a Hermitian k x k matrix (column-major, leading
+// dimension lda; the upper triangle is read and overwritten with R, the lower one is left alone),
+// k <= kPotrfMax, one workgroup of 256 threads, right-looking in blocks of kPotrfNb rows:
+//   1. the diagonal block by wavefront 0, lane l holding column l of it in registers; the row
+//      R[j, :] of each step reaches the other lanes by lane shuffles;
+//   2. the block's rows of R right of it, R_d^{-H} A[block, rest] by forward substitution, one
+//      thread per column (the rows also go to LDS);
+//   3. the trailing upper triangle minus the Gram products of those rows, 4 x 4 tiles per thread.
+// *info = 0, or j + 1 for the first pivot j that is not positive (the factorisation stops there),
+// as LAPACK's zpotrf reports it.
+constexpr int kPotrfMax = 256;
+constexpr int kPotrfNb = 16;
+__global__ __launch_bounds__(256) void k_potrf_upper(double2* __restrict__ A, int lda, int k, int* __restrict__ info) {
+    __shared__ double2 dblk[kPotrfNb][kPotrfNb];              // R of the diagonal block [row][column]
+    __shared__ double2 rows[kPotrfNb][kPotrfMax - kPotrfNb];  // its rows right of it [row][column - c0]
+    __shared__ int s_fail;
+    const int tid = (int)threadIdx.x;
+    if (tid == 0) {
+        s_fail = 0;
+        *info = 0;
+    }
+    __syncthreads();
+    for (int jb = 0; jb < k; jb += kPotrfNb) {
+        const int nb = min(kPotrfNb, k - jb);
+        if (tid < 64) {
+            const int l = tid;
+            double2 col[kPotrfNb];  // A[jb + i][jb + l], i <= l
+#pragma unroll
+            for (int i = 0; i < kPotrfNb; ++i)
+                col[i] = (l < nb && i <= l) ? A[(size_t)(jb + l) * lda + jb + i] : double2{0.0, 0.0};
+            int fail = 0;
+#pragma unroll
+            for (int j = 0; j < kPotrfNb; ++j) {
+                if (j < nb && fail == 0) {  // uniform over the wavefront
+                    const double d = __shfl(col[j].x, j);
+                    if (!(d > 0.0)) {
+                        fail = jb + j + 1;
+                    } else {
+                        const double r = sqrt(d), inv = 1.0 / r;
+                        if (l == j) col[j] = double2{r, 0.0};
+                        if (l > j) col[j] = double2{col[j].x * inv, col[j].y * inv};  // R[j][l]
+                        // A[i][l] -= conj(R[j][i]) R[j][l] for j < i <= l
+#pragma unroll
+                        for (int i = j + 1; i < kPotrfNb; ++i) {
+                            const double rx = __shfl(col[j].x, i), ry = __shfl(col[j].y, i);  // R[j][i]
+                            if (l >= i && l < nb) {
+                                col[i].x -= rx * col[j].x + ry * col[j].y;
+                                col[i].y -= rx * col[j].y - ry * col[j].x;
+                            }
+                        }
+                    }
+                }
+            }
+            if (l < nb) {
+#pragma unroll
+                for (int i = 0; i < kPotrfNb; ++i) {
+                    if (i <= l) {
+                        A[(size_t)(jb + l) * lda + jb + i] = col[i];
+                        dblk[i][l] = col[i];
+                    }
+                }
+            }
+            if (l == 0 && fail) {
+                s_fail = fail;
+                *info = fail;
+            }
+        }
+        __syncthreads();
+        if (s_fail) return;
+        const int c0 = jb + nb, m = k - c0;
+        if (m <= 0) break;
+        // Rows of R right of the block: solve R_d^H y = A[jb:jb+nb, c] for each column c.
+        if (tid < m) {
+            const int c = c0 + tid;
+            double2 y[kPotrfNb];
+#pragma unroll
+            for (int i = 0; i < kPotrfNb; ++i) {
+                if (i < nb) {
+                    double2 v = A[(size_t)c * lda + jb + i];
+#pragma unroll
+                    for (int t = 0; t < i; ++t) {  // v -= conj(R_d[t][i]) y[t]
+                        const double2 r = dblk[t][i];
+                        v.x -= r.x * y[t].x + r.y * y[t].y;
+                        v.y -= r.x * y[t].y - r.y * y[t].x;
+                    }
+                    const double inv = 1.0 / dblk[i][i].x;
+                    y[i] = double2{v.x * inv, v.y * inv};
+                    A[(size_t)c * lda + jb + i] = y[i];
+                    rows[i][tid] = y[i];
+                }
+            }
+        }
+        __syncthreads();
+        // Trailing upper triangle: A[i][c] -= sum_t conj(R[t][i]) R[t][c], c0 <= i <= c < k, in
+        // 4 x 4 tiles (ti <= tc) per thread: eight LDS reads per t feed 16 complex products.
+        const int nt = (m + 3) >> 2, ntiles = nt * (nt + 1) / 2;
+        for (int e = tid; e < ntiles; e += 256) {
+            int tc = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);  // tile column: tc (tc + 1) / 2 <= e
+            while (tc * (tc + 1) / 2 > e) --tc;
+            while ((tc + 1) * (tc + 2) / 2 <= e) ++tc;
+            const int ti = e - tc * (tc + 1) / 2;
+            double2 acc[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = double2{0.0, 0.0};
+            for (int t = 0; t < nb; ++t) {
+                double2 ri[4], rc[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int i = 4 * ti + a, c = 4 * tc + a;
+                    ri[a] = i < m ? rows[t][i] : double2{0.0, 0.0};
+                    rc[a] = c < m ? rows[t][c] : double2{0.0, 0.0};
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        acc[a][b].x += ri[a].x * rc[b].x + ri[a].y * rc[b].y;
+                        acc[a][b].y += ri[a].x * rc[b].y - ri[a].y * rc[b].x;
+                    }
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int c = 4 * tc + b;
+                if (c >= m) continue;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int i = 4 * ti + a;
+                    if (i > c) continue;
+                    double2* p = A + (size_t)(c0 + c) * lda + c0 + i;
+                    const double2 v = *p;
+                    *p = double2{v.x - acc[a][b].x, v.y - acc[a][b].y};
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ok = 1 when the Cholesky factorisation of the Gram matrix succeeded (info == 0) and every pivot
 // keeps at least 1e-7 of its column's norm (R_jj > 1e-7 sqrt(G_jj); G's diagonal from its copy
 // gcopy, R in r, both k x k with leading dimension k).  One workgroup.
@@ -598,6 +738,19 @@ struct Solver {
         return kk;
     }
 
+    // G = R^H R in place (upper triangle, leading dimension k), the verdict in w.info[0]: the
+    // library's one-workgroup factorisation up to kPotrfMax, rocSOLVER's zpotrf beyond.
+    void potrf_upper(double2* G, int k) {
+        static_assert(sizeof(rocblas_int) == sizeof(int), "info is read as int");
+        if (k <= kPotrfMax) {
+            hipLaunchKernelGGL(k_potrf_upper, dim3(1), dim3(256), 0, s, G, k, k, w.info.as<int>());
+            DZ_HIP(hipGetLastError());
+        } else {
+            DZ_BLAS(rocsolver_zpotrf(w.blas, rocblas_fill_upper, k, reinterpret_cast<zc*>(G), k,
+                                     w.info.as<rocblas_int>()));
+        }
+    }
+
     // Orthonormalise Q[:, 0:k] in place: Cholesky QR, or, when the Gram matrix is not
     // numerically positive definite, its eigen-decomposition G = U L U^H with the directions of
     // L < 1e-12 max(L) dropped and Q <- Q U L^{-1/2} (compacted to the front).  Returns the
@@ -608,7 +761,7 @@ struct Solver {
         const int ldn = (int)n;
         gram(Q, k, Q, k, w.G.as<double2>(), k);
         DZ_HIP(hipMemcpyAsync(w.S.ptr, w.G.ptr, (size_t)k * k * sizeof(double2), hipMemcpyDeviceToDevice, s));
-        DZ_BLAS(rocsolver_zpotrf(w.blas, rocblas_fill_upper, k, w.G.as<zc>(), k, w.info.as<rocblas_int>()));
+        potrf_upper(w.G.as<double2>(), k);
         // The factorisation's verdict on the device (info and every pivot against its column's norm,
         // the latter from the copy in S), one int back to the host.
         int* ok_dev = w.info.as<int>() + 1;
@@ -653,7 +806,7 @@ struct Solver {
     void cholqr(double2* Q, int k) {
         const int ldn = (int)n;
         gram(Q, k, Q, k, w.G.as<double2>(), k);
-        DZ_BLAS(rocsolver_zpotrf(w.blas, rocblas_fill_upper, k, w.G.as<zc>(), k, w.info.as<rocblas_int>()));
+        potrf_upper(w.G.as<double2>(), k);
         rocblas_int info = 0;
         DZ_HIP(hipMemcpyAsync(&info, w.info.ptr, sizeof(info), hipMemcpyDeviceToHost, s));
         DZ_HIP(hipStreamSynchronize(s));
