@@ -141,10 +141,7 @@ int dzhip_heev(dzhip_ctx* ctx, const double* a, uint64_t lda, uint64_t m, uint64
             DZ_HIP(hipMalloc(&dz, (size_t)m * nev * sizeof(double2)));
             DZ_HIP(hipMalloc(&dw, (size_t)nev * sizeof(double)));
             DZ_HIP(hipMemcpyAsync(da, a, (size_t)lda * m * sizeof(double2), hipMemcpyHostToDevice, s));
-            TrdAdmission adm;  // returned after the synchronisation below
-            heev_lowest(*ctx->heev, da, (int)lda, (int)m, (int)nev, dw, dz, (int)m, s, &adm);
-            DZ_REQUIRE(heev_error_flag(*ctx->heev, s) == 0, DZHIP_ERR_HIP,
-                       "Hermitian eigensolver: tridiagonalisation barrier did not complete");
+            heev_lowest_shared(*ctx->heev, da, (int)lda, (int)m, (int)nev, dw, dz, (int)m, s);
             DZ_HIP(hipMemcpyAsync(values, dw, nev * sizeof(double), hipMemcpyDeviceToHost, s));
             DZ_HIP(hipMemcpy2DAsync(vectors, ldz * sizeof(double2), dz, m * sizeof(double2), m * sizeof(double2), nev,
                                     hipMemcpyDeviceToHost, s));

@@ -598,11 +598,9 @@ static std::vector<double> subspace_eigh(EigWork& w, const double2* A, int lda, 
                                          hipStream_t s) {
     std::vector<double> ev(nev);
     if (m <= kHeevMaxM) {
-        TrdAdmission adm;  // returned after the synchronisation below
-        heev_lowest(w.heev, A, lda, m, nev, w.evals.as<double>(), Y, ldy, s, &adm);
+        heev_lowest_shared(w.heev, A, lda, m, nev, w.evals.as<double>(), Y, ldy, s);
         DZ_HIP(hipMemcpyAsync(ev.data(), w.evals.ptr, nev * sizeof(double), hipMemcpyDeviceToHost, s));
-        DZ_REQUIRE(heev_error_flag(w.heev, s) == 0, DZHIP_ERR_HIP,
-                   "Hermitian eigensolver: tridiagonalisation barrier did not complete");
+        DZ_HIP(hipStreamSynchronize(s));
         return ev;
     }
     DZ_HIP(hipMemcpy2DAsync(Y, (size_t)ldy * sizeof(double2), A, (size_t)lda * sizeof(double2),

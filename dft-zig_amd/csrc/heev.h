@@ -36,10 +36,13 @@ constexpr int kHeevNoAllocMaxM = 256;
 // needs all of its workgroups resident at once.  Workgroups are dealt to the XCDs round robin, so
 // a grid of G groups needs at most ceil(G / XCDs) CUs on any XCD (one workgroup per CU at worst).
 // The gate admits grids only while their sum stays within the CUs of one XCD; every other kernel
-// the library launches runs to completion without waiting on another workgroup,
-// so each admitted grid's workgroups are all placed once those retire, and no barrier waits on a
-// workgroup that cannot be placed.  The holder synchronises its stream (the grid has finished)
-// and returns its share when it goes out of scope.  Cooperative launches (the enqueue-only entry
+// the library launches runs to completion without waiting on another workgroup, so an admitted
+// grid's missing workgroups are placed as those retire.  That can take long: with many streams
+// issuing kernels, smaller workgroups keep refilling the CUs a tridiagonalisation workgroup (80-100
+// KB of LDS) waits for (one barrier ran out of its spin limit in a concurrent-k SCF of some
+// thousands of calls).  An admitted grid's barrier therefore gives up after 2^19 polls (an eighth
+// of the cooperative launch's limit) and heev_lowest_shared repeats the call cooperatively.  The holder synchronises its stream
+// (the grid has finished) and returns its share when it goes out of scope.  Cooperative launches (the enqueue-only entry
 // points) are not counted: one that runs while concurrent solves hold the gate may find the
 // device short of CUs, and its barrier's spin limit then reports an error instead of waiting.
 struct TrdAdmission {
@@ -66,6 +69,14 @@ void heev_tridiag_only(HeevWork& w, const double2* A, int lda, int m, hipStream_
 
 // Error flag of the last tridiagonalisation (a grid barrier that did not complete); synchronises s.
 int heev_error_flag(HeevWork& w, hipStream_t s);
+
+// heev_lowest through the gate, synchronised: the tridiagonalisation runs as an admitted ordinary
+// dispatch whose barrier gives up after 2^19 polls without all its workgroups (other streams'
+// kernels can keep the CUs it waits for), and is then repeated as a cooperative launch.  Throws if
+// that fails too.  heev_shared_retries counts the repeats in the process.
+void heev_lowest_shared(HeevWork& w, const double2* A, int lda, int m, int nev, double* w_dev, double2* Y, int ldy,
+                        hipStream_t s);
+unsigned long long heev_shared_retries();
 
 // Allocation-free variant for m <= kHeevNoAllocMaxM: every buffer in the library's static device
 // scratch array of `device` (heev_scratch_init must have run for it, with it current).  A failed

@@ -4,6 +4,7 @@
 #include "heev.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <condition_variable>
 #include <cstdlib>
@@ -18,6 +19,11 @@ constexpr int kTrdColsPerGroup = 6;   // trailing-matrix columns held in each wo
 constexpr int kTrdMaxCols = 16;
 constexpr int kTrdRowsPerThread = (1024 + 255) / 256;  // rows per thread at m <= kHeevMaxM
 constexpr unsigned kSpinLimit = 1u << 22;  // barrier polls (each after s_sleep) before giving up
+// Barrier polls of a tridiagonalisation admitted by the gate: its grid may find the device short
+// of CUs for a long time while other streams' kernels keep taking them, and it is then repeated as
+// a cooperative launch (heev_lowest_shared).  Si-64 concurrent-k SCF, mixed arm to 1e-6 Ha: 9.16 s
+// at 2^16 polls (frequent repeats), 8.60 s at 2^19, 10.21 s with cooperative launches only.
+constexpr unsigned kSpinLimitShared = 1u << 19;
 
 // One DPP row rotation of a double (both halves moved by the same row_ror:N control).
 template <int CTRL>
@@ -89,10 +95,10 @@ __device__ __forceinline__ double2 xload(const double2* p) {
 // its own memory operations to complete (s_waitcnt 0), so the exchanged data written with xstore
 // is in place before the flag is.  With part != nullptr, wavefront 0 then also sums part[0:G]
 // (one complex value per workgroup, written with xstore before the barrier) into sum[0:2].
-// A wait longer than kSpinLimit polls (or an error raised elsewhere) sets *err; every thread
+// A wait longer than spin_limit polls (or an error raised elsewhere) sets *err; every thread
 // returns false then and the kernel winds down.
-__device__ bool grid_sync(unsigned* flags, unsigned epoch, int* err, int* s_ok, const double2* part = nullptr,
-                          double* sum = nullptr) {
+__device__ bool grid_sync(unsigned* flags, unsigned epoch, int* err, int* s_ok, unsigned spin_limit,
+                          const double2* part = nullptr, double* sum = nullptr) {
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
@@ -109,7 +115,7 @@ __device__ bool grid_sync(unsigned* flags, unsigned epoch, int* err, int* s_ok, 
                 arrived &= __hip_atomic_load(flags + (size_t)q * 16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= epoch;
             if (__all(arrived)) break;
             const int bad = lane == 0 ? __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-            if (__any(bad) || ++spins > kSpinLimit) {
+            if (__any(bad) || ++spins > spin_limit) {
                 if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
@@ -154,6 +160,7 @@ struct TrdArgs {
     double2* nbuf; // 2 x m: the next column after the previous rank-2 update
     unsigned* flags;  // one per workgroup, 16 apart
     int* err;
+    unsigned spin_limit;  // barrier polls before giving up
 };
 
 // Householder tridiagonalisation A = Q T Q^H, Q = H_0 ... H_{m-2}, H_j = I - tau_j v_j v_j^H, with
@@ -305,7 +312,7 @@ __global__ __launch_bounds__(kTrdThreads) void k_hetrd(TrdArgs a) {
         }
         if (t == 0) xstore(dpart + g, pdot);
         // C.
-        if (!grid_sync(a.flags, (unsigned)(j + 1), a.err, &s_ok)) return;
+        if (!grid_sync(a.flags, (unsigned)(j + 1), a.err, &s_ok, a.spin_limit)) return;
         // Every published value this thread needs, loaded together: p_j and column j+1 on its own
         // rows, p_j[j+1], and (wavefront 0) the parts of p_j^H v_j.
         const double2* nb_in = a.nbuf + (size_t)((j + 1) & 1) * m;
@@ -925,6 +932,16 @@ bool trd_gated_enabled() {
     return on;
 }
 
+// Spin limit of admitted launches; DZHIP_HEEV_SHARED_SPIN overrides it (a test sets 1 so that the
+// barrier gives up at once and the cooperative repeat runs).
+unsigned shared_spin_limit() {
+    static const unsigned v = [] {
+        const char* env = std::getenv("DZHIP_HEEV_SHARED_SPIN");
+        return env ? (unsigned)std::max(1, std::atoi(env)) : kSpinLimitShared;
+    }();
+    return v;
+}
+
 void run_tridiag(const HeevBufs& b, int G, int ncl, const double2* A, int lda, int m, hipStream_t s,
                  bool cooperative = true) {
     // Barrier flags (one 64-byte line per workgroup) and the error flag after them, cleared by a
@@ -933,7 +950,7 @@ void run_tridiag(const HeevBufs& b, int G, int ncl, const double2* A, int lda, i
     hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nsync + 255) / 256)), dim3(256), 0, s, b.sync, nsync);
     DZ_HIP(hipGetLastError());
     TrdArgs args{A, lda, m, ncl, b.vh, b.tau, b.d, b.e, b.pbuf, b.nbuf, b.sync,
-                 reinterpret_cast<int*>(b.sync) + b.err_offset};
+                 reinterpret_cast<int*>(b.sync) + b.err_offset, cooperative ? kSpinLimit : shared_spin_limit()};
     void* kargs[] = {&args};
     if (cooperative) {
         DZ_HIP(hipLaunchCooperativeKernel(trd_kernel(ncl), dim3((unsigned)G), dim3(kTrdThreads), kargs,
@@ -1107,6 +1124,24 @@ void heev_lowest(HeevWork& w, const double2* A, int lda, int m, int nev, double*
     }
     run_eig(b, m, nev, w_dev, Y, ldy, s);
 }
+
+std::atomic<unsigned long long> g_trd_retries{0};
+
+void heev_lowest_shared(HeevWork& w, const double2* A, int lda, int m, int nev, double* w_dev, double2* Y, int ldy,
+                        hipStream_t s) {
+    {
+        TrdAdmission adm;
+        heev_lowest(w, A, lda, m, nev, w_dev, Y, ldy, s, &adm);
+        if (heev_error_flag(w, s) == 0) return;  // synchronises s
+    }
+    // The admitted grid gave up waiting for its workgroups (A is unchanged): cooperative launch.
+    g_trd_retries.fetch_add(1, std::memory_order_relaxed);
+    heev_lowest(w, A, lda, m, nev, w_dev, Y, ldy, s);
+    DZ_REQUIRE(heev_error_flag(w, s) == 0, DZHIP_ERR_HIP,
+               "Hermitian eigensolver: tridiagonalisation barrier did not complete");
+}
+
+unsigned long long heev_shared_retries() { return g_trd_retries.load(std::memory_order_relaxed); }
 
 int heev_error_flag(HeevWork& w, hipStream_t s) {
     int flag = 0;

@@ -118,3 +118,30 @@ def test_heev_concurrent_threads(dz, ctx):
         for w, z in out[i]:
             assert np.array_equal(w, alone[i][0]) and np.array_equal(z, alone[i][1])
         assert np.max(np.abs(alone[i][0] - ref)) <= 1e-12 * max(1.0, float(np.max(np.abs(ref))))
+
+
+def test_heev_shared_launch_falls_back_to_cooperative():
+    """The blocking eigensolver's admitted launch gives up when its barrier waits longer than its
+    spin limit and the call is repeated cooperatively.  Forced from the host: a child process with
+    the admitted launch's spin limit at one poll (its barrier gives up at once; every workgroup
+    exits normally) must still return the exact spectrum."""
+    import os
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    script = (
+        "import sys, numpy as np; sys.path.insert(0, %r); import dzhip\n"
+        "rng = np.random.default_rng(5); m = 648\n"
+        "q, _ = np.linalg.qr(rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m)))\n"
+        "lam = np.sort(rng.uniform(-2.0, 30.0, m)); a = (q * lam) @ q.conj().T; a = 0.5 * (a + a.conj().T)\n"
+        "ctx = dzhip.Context(0); w, z = ctx.heev(a, 160)\n"
+        "print(float(np.max(np.abs(w - lam[:160]))), float(np.max(np.abs(z.conj().T @ z - np.eye(160)))))\n"
+    ) % str(root / "dft-zig_amd")
+    env = dict(os.environ, DZHIP_HEEV_SHARED_SPIN="1")
+    print("child: heev m=648 nev=160 with DZHIP_HEEV_SHARED_SPIN=1", flush=True)
+    out = subprocess.run([sys.executable, "-c", script], env=env, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr[-2000:]
+    dval, orth = map(float, out.stdout.split()[-2:])
+    assert dval < 1e-12 * 30.0 and orth < 1e-12

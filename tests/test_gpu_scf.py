@@ -132,7 +132,11 @@ def test_scf_device_grid_matches_host(dz, ctx, gold, system, precision, concurre
     and the same fixed point as the host path."""
     drv = dscf.ScfDriver(system, concurrent_k=concurrent, device_grid=True)
     try:
-        run = drv.run(max_iter=60, conv=1e-11, precision=precision, final_tol=1e-10, eig_tol=1e-10 if precision == FP64 else 1e-4,
+        # FP32 applies: the x-line kernel the autotuner times fastest may differ from run to run,
+        # and with it the rounding of the early iterations; the way down to conv = 1e-11 then
+        # takes a varying number of steps (up to about 60 seen), hence the headroom.
+        run = drv.run(max_iter=60 if precision == FP64 else 90, conv=1e-11, precision=precision, final_tol=1e-10,
+                      eig_tol=1e-10 if precision == FP64 else 1e-4,
                       eig_max_iter=400, warmup_steps=0 if precision == FP64 else 2, eig_tol_ratio=None if precision == FP64 else 0.3,
                       eig_tol_min=1e-10)
         v_dev = drv.grid.read(dz.GRID_VIN)
