@@ -230,7 +230,7 @@ def scf_leg(name, max_iter, arms, log=sys.stderr):
              # GaAs-64 (Gamma only): the TPA preconditioner against the network on the TPA base.
              "tpa": (dzhip.FP32, False, dzhip.PRECOND_TPA),
              "tpa_learned": (dzhip.FP32, False, dzhip.PRECOND_MLP)}
-    runs, setup = {}, {}
+    runs, setup, retries = {}, {}, {}
     for tag in arms:
         prec, conc, pc = forms[tag]
         drv = dscf.ScfDriver(system, concurrent_k=conc, device_grid=True)
@@ -245,11 +245,13 @@ def scf_leg(name, max_iter, arms, log=sys.stderr):
             print(f"scf {name} {tag} it {r.iteration} E {r.energy_ry:.10f} drho {r.drho:.3e} t {r.wall_s:.2f}s "
                   f"eig {r.eig_s:.2f}s eig_it {r.eig_iterations} applies {r.applies} fp32 {r.fp32_applies} "
                   f"gpu_ms[{sec}]", file=log, flush=True)
+        retries0 = dzhip.heev_retries()
         try:
             runs[tag] = drv.run(max_iter=max_iter, precision=prec, preconditioner=pc, nconv=nocc, progress=show,
                                 **settings)
         finally:
             drv.close()
+        retries[tag] = dzhip.heev_retries() - retries0
         if pc == dzhip.PRECOND_MLP:
             out["learned_training"]["scf_solves_fallback"] = {"solves": len(fallbacks),
                                                               "fell_back": sum(1 for f in fallbacks if f > 0)}
@@ -271,7 +273,10 @@ def scf_leg(name, max_iter, arms, log=sys.stderr):
                     "applies": sum(r.applies for r in run.records),
                     "solver_gpu_ms": {k[2:-3]: round(sum(r.sections_ms.get(k, 0.0) for r in run.records), 1)
                                       for k in dscf.SECTIONS},
-                    "fp32_applies": sum(r.fp32_applies for r in run.records)}
+                    "fp32_applies": sum(r.fp32_applies for r in run.records),
+                    # Rayleigh-Ritz tridiagonalisations whose ordinary launch gave up waiting for its
+                    # workgroups beside other k-points' work and was repeated cooperatively.
+                    "heev_cooperative_repeats": retries[tag]}
     a = out.get(base_arm, {}).get("t_1e-6_Ha_s")
     for tag in arms:
         if tag == base_arm:

@@ -184,6 +184,8 @@ int dzhip_heev_dev_noalloc(dzhip_ctx* ctx, const void* a, uint64_t lda, uint64_t
     });
 }
 
+uint64_t dzhip_heev_retries(void) { return (uint64_t)heev_shared_retries(); }
+
 int dzhip_heev_check(dzhip_ctx* ctx, void* stream) {
     return guarded([&] {
         DZ_REQUIRE(ctx, DZHIP_ERR_STATE, "NULL context");

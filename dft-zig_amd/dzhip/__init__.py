@@ -39,7 +39,7 @@ EXPORTS = (
     "dzhip_ham_set_local_grid", "dzhip_scf_grid_add_solution_density", "dzhip_scf_grid_hxc",
     "dzhip_scf_grid_mix", "dzhip_scf_grid_read", "dzhip_ham_set_tuning", "dzhip_ham_set_plane_kernel", "dzhip_scf_grid_set_dielectric",
     "dzhip_scf_grid_set_xc", "dzhip_ham_set_overlap",
-    "dzhip_scf_grid_init_rho", "dzhip_heev", "dzhip_heev_dev", "dzhip_heev_check",
+    "dzhip_scf_grid_init_rho", "dzhip_heev", "dzhip_heev_dev", "dzhip_heev_check", "dzhip_heev_retries",
     "dzhip_hamset_create", "dzhip_hamset_destroy", "dzhip_hamset_offsets", "dzhip_hamset_apply_dev",
     "dzhip_hamset_apply", "dzhip_hamset_set_stream", "dzhip_hamset_info", "dzhip_heev_dev_noalloc",
 )
@@ -175,6 +175,7 @@ def _load():
         "dzhip_heev": ([vp, dp, u64, u64, u64, dp, dp, u64], i32),
         "dzhip_heev_dev": ([vp, vp, u64, u64, u64, vp, vp, u64, vp], i32),
         "dzhip_heev_check": ([vp, vp], i32),
+        "dzhip_heev_retries": ([], u64),
         "dzhip_heev_dev_noalloc": ([vp, vp, u64, u64, u64, vp, vp, u64, vp], i32),
         "dzhip_hamset_create": ([ct.POINTER(vp), u64], vp), "dzhip_hamset_destroy": ([vp], None),
         "dzhip_hamset_offsets": ([vp, ct.POINTER(ct.c_uint64), u64], i32),
@@ -206,6 +207,12 @@ def L():
 
 def last_error() -> str:
     return L().dzhip_last_error().decode(errors="replace")
+
+
+def heev_retries() -> int:
+    """Rayleigh-Ritz tridiagonalisations repeated cooperatively in this process (their ordinary
+    launch beside other k-points' work gave up waiting for its workgroups)."""
+    return int(L().dzhip_heev_retries())
 
 
 def check(rc: int) -> None:

@@ -84,6 +84,10 @@ int dzhip_heev(dzhip_ctx* ctx, const double* a, uint64_t lda, uint64_t m, uint64
 int dzhip_heev_dev(dzhip_ctx* ctx, const void* a, uint64_t lda, uint64_t m, uint64_t nev, void* values,
                    void* vectors, uint64_t ldz, void* stream);
 int dzhip_heev_check(dzhip_ctx* ctx, void* stream);
+/* Diagnostic, no counterpart in the reference: how many times in this process a solver's
+ * Rayleigh-Ritz tridiagonalisation, launched as an ordinary dispatch beside other k-points' work,
+ * gave up waiting for its workgroups and was repeated as a cooperative launch. */
+uint64_t dzhip_heev_retries(void);
 /* dzhip_heev_dev without any allocation, for callers that cannot allocate device memory (code
  * under graph capture, tight serving loops, harnesses that forbid it); it has no counterpart in
  * the reference.  m <= 256: every buffer lives in the library's 6 MiB static device array, one per
